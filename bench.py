@@ -1,0 +1,179 @@
+"""Benchmark: closest-hit throughput of the MI355X traversal on the 1M-triangle
+scene (1920x1080 jittered primary rays), plus the path tracer's samples/s.
+
+A step is one closest-hit launch over the 2,073,600 primary rays of the
+config-2 scene (1,164,290 triangles), rays and BVH resident in HBM.  Prints one
+JSON line.  Only one GPU is used: --gpus above 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mrays/s closest-hit; sec to RMSE<=0.01 vs converged ref, cached vs plain PT"
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(scene, nodes, rays):
+    """The oracle's closest-hit (reference order and the kernel's pruned order)
+    on the host cores, over the same rays and BVH."""
+    from oracle import Oracle
+    from oracle.pyoracle import MODE_PRUNED, MODE_REF
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    o = Oracle.from_scene(scene, nodes)
+    res = {}
+    for name, mode in (("reference_order", MODE_REF), ("pruned", MODE_PRUNED)):
+        o.trace(rays[:4096], mode, threads)  # warm-up
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            _, st = o.trace(rays, mode, threads)
+            times.append(time.perf_counter() - t0)
+        t = float(np.median(times))
+        res[name] = {"mrays_per_s": rays.size / t / 1e6, "inner_per_ray": st.inner / st.rays,
+                     "leaf_per_ray": st.leaf / st.rays, "seconds": t}
+    return threads, res
+
+
+def load_traffic(kernel: str):
+    f = ROOT / "profiles" / f"{kernel}_traffic.json"
+    if f.exists():
+        try:
+            return json.loads(f.read_text()).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            return None
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pt-frames", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
+        return 2
+
+    import strelka_amd as sa
+
+    scene = sa.SceneData.generate(2)
+    ntri = int(scene.inst["indexCount"].sum()) // 3
+    rays = scene.jittered_rays(0x5354524B00000002)
+    n = rays.size
+    ctx = sa.Context(0)
+    t0 = time.perf_counter()
+    ctx.set_scene(scene)
+    ctx.build_bvh()
+    build_s = time.perf_counter() - t0
+    nodes = ctx.get_bvh()
+    d_rays = ctx.alloc(n * 32)
+    d_hits = ctx.alloc(n * 20)
+    ctx.upload(d_rays, rays)
+
+    for _ in range(args.warmup):
+        ctx.trace_closest_device(d_rays, n, d_hits)
+    ctx.sync()
+    ctx.record(0)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.trace_closest_device(d_rays, n, d_hits)
+    ctx.record(1)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    kernel_ms = ctx.elapsed_ms(0, 1) / args.steps
+    mrays = n * args.steps / wall / 1e6
+
+    # reference-order (unpruned skip-link) GPU kernel on the same rays
+    ctx.record(2)
+    for _ in range(max(1, args.steps // 10)):
+        ctx.trace_closest_device(d_rays, n, d_hits, reference_order=True)
+    ctx.record(3)
+    ref_kernel_ms = ctx.elapsed_ms(2, 3) / max(1, args.steps // 10)
+
+    # path tracer: 1 spp per frame at 1080p, 8 bounces, RR after 3
+    p = sa.render_params(scene.camera, spp=1, max_depth=8, frame=0, sampling=1, sky=scene.sky)
+    d_out = ctx.alloc(n * 16)
+    ctx.render_device(p, d_out)
+    ctx.sync()
+    ctx.record(4)
+    for f in range(args.pt_frames):
+        p.pt.frameNumber = f + 1
+        ctx.render_device(p, d_out)
+    ctx.record(5)
+    pt_ms = ctx.elapsed_ms(4, 5) / max(1, args.pt_frames)
+    ctx.free(d_out)
+    ctx.free(d_rays)
+    ctx.free(d_hits)
+
+    cpu = None
+    roof = None
+    if not args.no_cpu_baseline:
+        cores, res = cpu_baseline(scene, nodes, rays)
+        pr = res["pruned"]
+        bytes_per_ray = 32 + 20 + 64 * pr["inner_per_ray"] + 48 * pr["leaf_per_ray"]
+        achieved = n * bytes_per_ray / (kernel_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("trace_closest_config2"),
+                "kernel": "k_trace_closest", "kernel_ms": round(kernel_ms, 4),
+                "bytes_per_ray": round(bytes_per_ray, 1),
+                "node_fetches_per_ray": round(pr["inner_per_ray"], 2),
+                "triangle_tests_per_ray": round(pr["leaf_per_ray"], 2)}
+        cpu = {"value": round(res["reference_order"]["mrays_per_s"], 4), "unit": "Mrays/s", "cores": cores,
+               "kind": "port",
+               "sample": f"the full {n}-ray 1080p primary frame, oracle closestHit in reference order "
+                         f"(skip-link, unpruned), median of 3, {cpu_model()}",
+               "pruned_mrays_per_s": round(pr["mrays_per_s"], 4)}
+
+    out = {
+        "metric": METRIC,
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded PCG scene and rays)",
+        "config": {"workload": "config2: 1920x1080 jittered primary rays, closest hit, 1M-triangle scene",
+                   "rays": n, "triangles": ntri, "bvh_nodes": int(nodes.size), "bvh_build_s": round(build_s, 3)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "extra": {"reference_order_gpu_mrays_per_s": round(n / (ref_kernel_ms * 1e-3) / 1e6, 2),
+                  "pt_1080p_1spp_frame_ms": round(pt_ms, 3),
+                  "pt_samples_per_s": round(n / (pt_ms * 1e-3), 1),
+                  "pt_max_depth": 8},
+    }
+    print(json.dumps(out))
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

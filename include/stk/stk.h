@@ -1,0 +1,167 @@
+/*
+ * stk.h - C ABI of the MI355X path-tracing library (libstrelka_amd.so).
+ *
+ * This is the drop-in boundary that replaces the Vulkan render passes of the
+ * reference renderer (oka::PtRender, include/render/ptrender.h:40-283).  Plain
+ * pointers and sizes only; nothing throws across it.  One context owns one HIP
+ * stream on one device; calls on a context must be serialised by the caller,
+ * different contexts are independent.
+ *
+ * Reference entry points replaced (file:line in ikryukov/Strelka):
+ *   stk_create / stk_destroy     PtRender::init / cleanup / setSharedContext
+ *                                (include/render/ptrender.h:47-82,
+ *                                 src/render/ptrender.cpp:40-195)
+ *   stk_set_geometry             Scene::getVertices/getIndices/getInstances read
+ *                                by createVertexBuffer/IndexBuffer/InstanceBuffer
+ *                                (src/render/ptrender.cpp:400-601, read once at
+ *                                frame 0, :656-667)
+ *   stk_set_lights               createLightsBuffer (src/render/ptrender.cpp:436-458)
+ *   stk_set_materials            createMdlBuffers (src/render/ptrender.cpp:603-640);
+ *                                Lambert/GGX records instead of MDL arg blocks
+ *   stk_build_bvh / stk_set_bvh  createBvhBuffer + BvhBuilder::build
+ *                                (src/render/ptrender.cpp:460-532,
+ *                                 src/render/bvh.cpp:237-254)
+ *   stk_trace_closest[_device]   closestHit (shaders/raytracing.h:198-247)
+ *   stk_trace_any[_device]       anyHit (shaders/raytracing.h:249-286)
+ *   stk_render                   PtRender::drawFrame passes: path tracer,
+ *                                reduction, accumulation, tonemap
+ *                                (src/render/ptrender.cpp:641-913)
+ *   stk_reset_accumulation       mPtIteration = 0 (src/render/ptrender.cpp:714-726)
+ */
+#ifndef STK_H
+#define STK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "stk_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(_WIN32)
+#define STK_API __declspec(dllexport)
+#else
+#define STK_API __attribute__((visibility("default")))
+#endif
+
+/* Result codes; the reference's Result{eOk,eFail,eOutOfMemory}
+ * (include/render/common.h:53-58) widened with device/state/timeout cases. */
+typedef enum stk_status {
+    STK_OK = 0,
+    STK_E_INVALID = 1, /* bad argument, size or layout */
+    STK_E_OOM = 2,     /* device or host allocation failed */
+    STK_E_DEVICE = 3,  /* a HIP call failed or no device is present */
+    STK_E_STATE = 4,   /* call order violated (e.g. trace before geometry) */
+    STK_E_TIMEOUT = 5  /* a bounded device wait expired */
+} stk_status;
+
+typedef struct stk_ctx stk_ctx;
+
+/* ---- BVH build options (BvhBuilder::build, src/render/bvh.cpp:107-167) ---- */
+typedef struct StkBvhOptions {
+    uint32_t maxDepth;   /* root at depth 0; leaves at depth <= maxDepth (<= 32) */
+    uint32_t numBins;    /* SAH bins per axis (0 = 32) */
+    float traversalCost; /* SAH costs, reference uses 1 and 1 */
+    float intersectionCost;
+} StkBvhOptions;
+
+/* Header of the packed BVH: root box and root reference. */
+typedef struct StkBvhHeader {
+    float lo[3];
+    int32_t root; /* >= 0 packed node, < 0 leaf (~triangle), INT32_MIN empty */
+    float hi[3];
+    uint32_t numTriangles;
+} StkBvhHeader;
+
+/* trace flags */
+#define STK_TRACE_REFERENCE_ORDER 0x1u /* stackless skip-link walk, unpruned */
+
+/* ---- library / context ---------------------------------------------------- */
+STK_API const char* stk_version(void);
+STK_API stk_status stk_create(int hipDevice, stk_ctx** out);
+STK_API void stk_destroy(stk_ctx* ctx);
+STK_API const char* stk_last_error(const stk_ctx* ctx);
+STK_API stk_status stk_sync(stk_ctx* ctx);
+/* The hipStream_t every call of this context is enqueued on. */
+STK_API void* stk_stream(stk_ctx* ctx);
+
+/* ---- scene input (deep-copied into device memory owned by the context) ---- */
+STK_API stk_status stk_set_geometry(stk_ctx* ctx, const StkVertex* vb, uint32_t numVertices, const uint32_t* ib,
+                                    uint32_t numIndices, const StkInstance* inst, uint32_t numInstances);
+STK_API stk_status stk_set_lights(stk_ctx* ctx, const StkLight* lights, uint32_t numLights);
+STK_API stk_status stk_set_materials(stk_ctx* ctx, const StkMaterial* mats, uint32_t numMaterials);
+/* Builds the BVH on the host (binned SAH, one triangle per leaf, preorder plus
+ * skip links) and uploads it.  opts may be NULL for defaults. */
+STK_API stk_status stk_build_bvh(stk_ctx* ctx, const StkBvhOptions* opts);
+/* Uses a caller-built skip-link BVH (e.g. the reference builder's output). */
+STK_API stk_status stk_set_bvh(stk_ctx* ctx, const StkBvhNode* nodes, uint32_t numNodes);
+/* Copies the skip-link BVH of the context out (numNodes receives the count;
+ * nodes may be NULL to query the count). */
+STK_API stk_status stk_get_bvh(stk_ctx* ctx, StkBvhNode* nodes, uint32_t capacity, uint32_t* numNodes);
+
+/* ---- traversal ------------------------------------------------------------ */
+/* Host buffers; synchronous. */
+STK_API stk_status stk_trace_closest(stk_ctx* ctx, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags);
+STK_API stk_status stk_trace_any(stk_ctx* ctx, const StkRay* rays, uint32_t n, StkHit* hits);
+/* Device buffers; enqueued on the context stream, asynchronous. */
+STK_API stk_status stk_trace_closest_device(stk_ctx* ctx, const StkRay* dRays, uint32_t n, StkHit* dHits,
+                                            uint32_t flags);
+STK_API stk_status stk_trace_any_device(stk_ctx* ctx, const StkRay* dRays, uint32_t n, StkHit* dHits);
+/* Camera rays of generateCameraRay (shaders/pathtracerMdl.hlsl:165-202) for
+ * every pixel, sample `sampleNum` of frame p->frameNumber, written in pixel
+ * order into dRays (device, W*H rays).  Asynchronous. */
+STK_API stk_status stk_generate_camera_rays_device(stk_ctx* ctx, const StkPathTracerParam* p, uint32_t sampleNum,
+                                                   StkRay* dRays);
+
+/* ---- rendering ------------------------------------------------------------ */
+/* Progressive: traces p->pt.spp samples per pixel of frame p->pt.frameNumber,
+ * reduces them, accumulates into the history (iteration counter kept by the
+ * context, reset by stk_reset_accumulation or a dimension change) and
+ * tonemaps.  outRgba receives W*H RGBA32F (tonemapped if enableTonemap,
+ * otherwise the accumulated radiance); it may be a host or a device pointer
+ * (outIsDevice).  Synchronous for host output, asynchronous otherwise. */
+STK_API stk_status stk_render(stk_ctx* ctx, const StkRenderParams* p, float* outRgba, int outIsDevice);
+STK_API stk_status stk_reset_accumulation(stk_ctx* ctx);
+/* Per-sample radiance of the last stk_render call, sampleBuffer layout
+ * [(pixel*spp + s)*3 + c] (shaders/pathtracerMdl.hlsl:420-422). */
+STK_API stk_status stk_get_samples(stk_ctx* ctx, float* out, size_t count);
+/* The accumulated (not tonemapped) RGBA32F image. */
+STK_API stk_status stk_get_accumulation(stk_ctx* ctx, float* out, size_t count);
+
+/* ---- device memory and timing helpers for callers without a HIP binding --- */
+STK_API stk_status stk_device_alloc(stk_ctx* ctx, size_t bytes, void** out);
+STK_API stk_status stk_device_free(stk_ctx* ctx, void* ptr);
+STK_API stk_status stk_copy_to_device(stk_ctx* ctx, void* dst, const void* src, size_t bytes);
+STK_API stk_status stk_copy_to_host(stk_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Records a HIP event on the context stream (slot 0..15). */
+STK_API stk_status stk_event_record(stk_ctx* ctx, int slot);
+/* Milliseconds between two recorded slots (waits for the second). */
+STK_API stk_status stk_event_elapsed(stk_ctx* ctx, int slotBegin, int slotEnd, float* ms);
+
+/* ---- host-only BVH tools (no device needed) ------------------------------ */
+typedef struct stk_bvh stk_bvh;
+/* Flattens instances to world-space triangles (createBvhBuffer,
+ * src/render/ptrender.cpp:460-512) and builds the BVH. */
+STK_API stk_status stk_bvh_build_host(const StkVertex* vb, uint32_t numVertices, const uint32_t* ib,
+                                      uint32_t numIndices, const StkInstance* inst, uint32_t numInstances,
+                                      const StkBvhOptions* opts, stk_bvh** out);
+/* Derives the packed form from a skip-link node array. */
+STK_API stk_status stk_bvh_from_nodes(const StkVertex* vb, uint32_t numVertices, const uint32_t* ib,
+                                      uint32_t numIndices, const StkInstance* inst, uint32_t numInstances,
+                                      const StkBvhNode* nodes, uint32_t numNodes, stk_bvh** out);
+STK_API void stk_bvh_free(stk_bvh* b);
+STK_API uint32_t stk_bvh_num_nodes(const stk_bvh* b);
+STK_API const StkBvhNode* stk_bvh_nodes(const stk_bvh* b);
+STK_API uint32_t stk_bvh_num_packed(const stk_bvh* b);
+STK_API const StkPackedNode* stk_bvh_packed(const stk_bvh* b);
+STK_API uint32_t stk_bvh_num_triangles(const stk_bvh* b);
+STK_API const StkTriangle* stk_bvh_triangles(const stk_bvh* b);
+STK_API StkBvhHeader stk_bvh_header(const stk_bvh* b);
+STK_API uint32_t stk_bvh_depth(const stk_bvh* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

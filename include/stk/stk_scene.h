@@ -1,0 +1,65 @@
+/*
+ * stk_scene.h - seeded synthetic workloads (part of libstrelka_amd.so).
+ *
+ * The reference renderer loads USD stages; it has no synthetic scenes and no
+ * benchmark workloads.  These generators build the measured configurations
+ * from a PCG32 stream (O'Neill 2014, XSH-RR, multiplier 6364136223846793005)
+ * seeded with 0x5354524B00000000 + config, one stream id per purpose
+ * (1 geometry, 2 materials, 3 ray jitter).  The meshes go through the same
+ * Scene container as user content (createMesh / createInstance / createLight
+ * of src/scene/scene.cpp).
+ *
+ *   config 1: icosphere (3 subdivisions, r = 1, 1,280 triangles) inside an
+ *             inward-facing box [-2,2]^3 (12 triangles); camera at (0,0,1.9)
+ *             looking down -z, vfov 60, 256x256.
+ *   config 2: 512x512-quad fBm heightfield over [-10,10]^2 (524,288
+ *             triangles) + 2,000 icospheres of 320 triangles, one 2x2 quad
+ *             light at y = 8 facing -y (intensity 100), Lambert/GGX
+ *             materials, 1920x1080 camera; 1,164,290 triangles.
+ *   config 3: one of the 32 uneven scenes of the batched launch (variant
+ *             0..31), built as config 2 with ~N_i triangles, 960x540.
+ *   config 6: a tiny scene (heightfield 16x16, 8 spheres) for smoke tests.
+ */
+#ifndef STK_SCENE_H
+#define STK_SCENE_H
+
+#include "stk.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct stk_scene stk_scene;
+
+typedef struct StkSceneArrays {
+    const StkVertex* vb;
+    uint32_t numVertices;
+    const uint32_t* ib;
+    uint32_t numIndices;
+    const StkInstance* inst;
+    uint32_t numInstances;
+    const StkLight* lights;
+    uint32_t numLights;
+    const StkMaterial* mats;
+    uint32_t numMaterials;
+    uint32_t numTriangles;
+    uint32_t pad;
+    StkPathTracerParam camera; /* matrices, dimension; spp = 1, maxDepth = 8 */
+    float sky[4];              /* constant sky radiance of the config */
+} StkSceneArrays;
+
+/* flags: bit 0 = reproduce the reference icosahedron faces (10,2,1),(10,11,3) */
+#define STK_SCENE_REF_ICOSPHERE 0x1u
+
+STK_API stk_status stk_scene_generate(uint32_t config, uint32_t variant, uint32_t flags, stk_scene** out);
+STK_API void stk_scene_free(stk_scene* s);
+STK_API stk_status stk_scene_get(const stk_scene* s, StkSceneArrays* out);
+/* Config-1 style jittered primary rays of the scene camera for every pixel:
+ * pixel position = pixel + U[0,1)^2, origin = camera + U[-1e-3,1e-3]^3, PCG
+ * stream 3 of `seed`.  rays holds W*H entries, pixel order. */
+STK_API stk_status stk_scene_jittered_rays(const stk_scene* s, uint64_t seed, StkRay* rays, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,196 @@
+/*
+ * stk_types.h - byte-exact data contracts shared by the HIP library, the C-ABI
+ * callers and the CPU oracle.
+ *
+ * Every struct mirrors a GPU-visible buffer record of the reference renderer;
+ * the offsets are pinned with static asserts so that a buffer produced for the
+ * reference can be handed to this library unchanged.
+ *
+ * Matrices are 16 floats in column-major order (element (row r, col c) lives at
+ * m[c*4 + r]), i.e. the memory order of glm::mat4, which is what the reference
+ * uploads into its constant and structured buffers.
+ */
+#ifndef STK_TYPES_H
+#define STK_TYPES_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STK_INVALID_INDEX 0xFFFFFFFFu
+
+/* Vertex record, 32 B.  Mirrors `struct Vertex` (shaders/raytracing.h:17-25)
+ * and oka::Scene::Vertex (include/scene/scene.h:54-63). */
+typedef struct StkVertex {
+    float position[3];
+    uint32_t tangent; /* 10:10:10 packed, decoded like a normal */
+    uint32_t normal;  /* 10:10:10 packed over [-1,1] (shaders/pack.h:4-13) */
+    uint32_t uv;      /* 16:16 packed over [-10,10] (shaders/pack.h:15-23) */
+    float pad0;
+    float pad1;
+} StkVertex;
+
+/* BVH node in the reference's flat skip-link layout, 32 B.  Mirrors
+ * `struct BVHNode` (shaders/raytracing.h:27-33, include/render/bvh.h:17-24).
+ * Inner node: bounds, instId = 0xFFFFFFFF, nodeOffset = skip (miss) link.
+ * Leaf: minBounds[0] holds the primitive id bit-cast to float, instId is the
+ * instance id, maxBounds = 0, nodeOffset = skip link.  Nodes are stored in DFS
+ * preorder; the hit (descend) link of node i is i+1. */
+typedef struct StkBvhNode {
+    float minBounds[3];
+    int32_t instId;
+    float maxBounds[3];
+    int32_t nodeOffset;
+} StkBvhNode;
+
+/* Per-instance constants, 208 B.  Mirrors `struct InstanceConstants`
+ * (shaders/instanceconstants.h:19-28), filled as in
+ * src/render/ptrender.cpp:574-587. */
+typedef struct StkInstance {
+    float objectToWorld[16];
+    float worldToObject[16];
+    float normalMatrix[16];
+    int32_t materialId;
+    int32_t indexOffset;
+    int32_t indexCount;
+    int32_t lightId; /* -1 for non-light instances */
+} StkInstance;
+
+/* Light record, 112 B.  Mirrors `struct UniformLight` (shaders/lights.h:10-19)
+ * and oka::Scene::Light (include/scene/scene.h:120-129).  type 0 = rect (four
+ * world-space corners), 1 = disc, 2 = sphere (points[0].x = radius,
+ * points[1] = centre). */
+typedef struct StkLight {
+    float points[4][4];
+    float color[4];
+    float normal[4];
+    int32_t type;
+    float pad0;
+    float pad2;
+    float pad3;
+} StkLight;
+
+/* Ray, 32 B.  Mirrors `struct Ray` (shaders/ray.h:12-16): o.xyz origin,
+ * o.w = tmax, d.xyz direction, d.w unused. */
+typedef struct StkRay {
+    float o[4];
+    float d[4];
+} StkRay;
+
+/* Hit record, 20 B.  Mirrors `struct Hit` (shaders/raytracing.h:90-96).
+ * A miss is reported as instId = primId = 0xFFFFFFFF, t = 1e9. */
+typedef struct StkHit {
+    float bary[2];
+    float t;
+    uint32_t instId;
+    uint32_t primId;
+} StkHit;
+
+/* Per-frame path tracer constants, 336 B.  Mirrors `struct PathTracerParam`
+ * (shaders/pathtracerparam.h:20-41), filled as in
+ * src/render/ptrender.cpp:755-774. */
+typedef struct StkPathTracerParam {
+    float viewToWorld[16];
+    float clipToView[16];
+    float viewToClip[16];
+    float worldToView[16];
+    float camPos[4];
+    int32_t dimension[2];
+    float invDimension[2];
+    uint32_t frameNumber;
+    uint32_t maxDepth;
+    uint32_t debug;
+    uint32_t numLights;
+    int32_t len;
+    int32_t spp;
+    int32_t iteration;
+    uint32_t stratifiedSamplingType; /* 0 centre, 1 random, 2 stratified, 3 CMJ */
+    float pad0[4];
+} StkPathTracerParam;
+
+/* Material record of this library, 32 B.  The reference binds MDL argument
+ * blocks here (shaders/pathtracerMdl.hlsl:276); this library evaluates the
+ * non-MDL Lambert and GGX models instead. */
+enum { STK_MAT_LAMBERT = 0, STK_MAT_GGX = 1 };
+typedef struct StkMaterial {
+    uint32_t type;
+    float roughness; /* GGX alpha = roughness^2 */
+    float pad0;
+    float pad1;
+    float albedo[4];
+} StkMaterial;
+
+/* Pre-transformed world-space triangle, 48 B: {v0, e0, e1} exactly as
+ * getTriangle (shaders/raytracing.h:134-153) computes them, plus the ids the
+ * hit record needs and the preorder index of the leaf that references it
+ * (used for the order-independent tie-break). */
+typedef struct StkTriangle {
+    float v0[3];
+    uint32_t leaf; /* preorder index of the leaf node in the skip-link layout */
+    float e0[3];
+    uint32_t instId;
+    float e1[3];
+    uint32_t primId;
+} StkTriangle;
+
+/* Packed BVH2 node of the MI355X traversal, 64 B (one aligned line).
+ * Children boxes are interleaved per axis so one node is four 16-B loads:
+ *   x = {c0.lo.x, c0.hi.x, c1.lo.x, c1.hi.x}, y = ..., z = ...
+ * child[i] >= 0 : index of an inner packed node
+ * child[i] <  0 : leaf, triangle index = ~child[i] (its box is not tested: the
+ *                 reference never tests leaf boxes, shaders/raytracing.h:211-231)
+ */
+typedef struct StkPackedNode {
+    float x[4];
+    float y[4];
+    float z[4];
+    int32_t child[2];
+    uint32_t pad[2];
+} StkPackedNode;
+
+/* Parameters of one render call (progressive; one call adds `spp` samples). */
+typedef struct StkRenderParams {
+    StkPathTracerParam pt;   /* camera matrices, dimension, spp, maxDepth ... */
+    uint32_t rrStartDepth;   /* RR applies when depth > rrStartDepth (reference: 3) */
+    uint32_t refCompat;      /* 1 = reproduce the reference estimator exactly */
+    uint32_t tonemapperType; /* 0 Reinhard, 1 ACES fitted, 2 ACES film */
+    uint32_t enableAccumulation;
+    float skyColor[4];       /* constant miss radiance (reference adds 0) */
+    uint32_t enableTonemap;
+    uint32_t keepSamples;    /* keep per-sample radiance for stk_get_samples */
+    uint32_t pad[2];
+} StkRenderParams;
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#ifdef __cplusplus
+#define STK_STATIC_ASSERT static_assert
+#else
+#define STK_STATIC_ASSERT _Static_assert
+#endif
+
+STK_STATIC_ASSERT(sizeof(StkVertex) == 32, "Vertex must be 32 B");
+STK_STATIC_ASSERT(sizeof(StkBvhNode) == 32, "BVHNode must be 32 B");
+STK_STATIC_ASSERT(offsetof(StkBvhNode, instId) == 12, "BVHNode.instId offset");
+STK_STATIC_ASSERT(offsetof(StkBvhNode, nodeOffset) == 28, "BVHNode.nodeOffset offset");
+STK_STATIC_ASSERT(sizeof(StkInstance) == 208, "InstanceConstants must be 208 B");
+STK_STATIC_ASSERT(offsetof(StkInstance, materialId) == 192, "InstanceConstants.materialId offset");
+STK_STATIC_ASSERT(offsetof(StkInstance, lightId) == 204, "InstanceConstants.lightId offset");
+STK_STATIC_ASSERT(sizeof(StkLight) == 112, "UniformLight must be 112 B");
+STK_STATIC_ASSERT(offsetof(StkLight, type) == 96, "UniformLight.type offset");
+STK_STATIC_ASSERT(sizeof(StkRay) == 32, "Ray must be 32 B");
+STK_STATIC_ASSERT(sizeof(StkHit) == 20, "Hit must be 20 B");
+STK_STATIC_ASSERT(sizeof(StkPathTracerParam) == 336, "PathTracerParam must be 336 B");
+STK_STATIC_ASSERT(offsetof(StkPathTracerParam, dimension) == 272, "PathTracerParam.dimension offset");
+STK_STATIC_ASSERT(offsetof(StkPathTracerParam, frameNumber) == 288, "PathTracerParam.frameNumber offset");
+STK_STATIC_ASSERT(offsetof(StkPathTracerParam, len) == 304, "PathTracerParam.len offset");
+STK_STATIC_ASSERT(sizeof(StkMaterial) == 32, "Material must be 32 B");
+STK_STATIC_ASSERT(sizeof(StkTriangle) == 48, "Triangle must be 48 B");
+STK_STATIC_ASSERT(sizeof(StkPackedNode) == 64, "PackedNode must be 64 B");
+
+#endif /* STK_TYPES_H */

@@ -1,0 +1,149 @@
+"""ctypes binding of oracle/build/liboracle.so (TEST INFRASTRUCTURE ONLY)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "build" / "liboracle.so"
+
+# record layouts of include/stk/stk_types.h (kept here so the oracle does not
+# import the product package)
+VERTEX = np.dtype([("position", "<f4", 3), ("tangent", "<u4"), ("normal", "<u4"), ("uv", "<u4"),
+                   ("pad0", "<f4"), ("pad1", "<f4")])
+BVH_NODE = np.dtype([("minBounds", "<f4", 3), ("instId", "<i4"), ("maxBounds", "<f4", 3), ("nodeOffset", "<i4")])
+RAY = np.dtype([("o", "<f4", 4), ("d", "<f4", 4)])
+HIT = np.dtype([("bary", "<f4", 2), ("t", "<f4"), ("instId", "<u4"), ("primId", "<u4")])
+
+MODE_REF, MODE_PRUNED, MODE_BRUTE, MODE_ANY = 0, 1, 2, 3
+
+
+class OrcScene(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("nnodes", C.c_uint32), ("inst", C.c_void_p), ("ninst", C.c_uint32),
+                ("vb", C.c_void_p), ("nv", C.c_uint32), ("ib", C.c_void_p), ("ni", C.c_uint32),
+                ("lights", C.c_void_p), ("nlights", C.c_uint32), ("mats", C.c_void_p), ("nmats", C.c_uint32)]
+
+
+class OracleStats(C.Structure):
+    _fields_ = [("inner", C.c_uint64), ("leaf", C.c_uint64), ("rays", C.c_uint64), ("hits", C.c_uint64)]
+
+
+def build_if_missing() -> Path:
+    if not LIB_PATH.exists():
+        from . import build as _b
+        _b.build()
+    return LIB_PATH
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        build_if_missing()
+        lib = C.CDLL(str(LIB_PATH))
+        P, U32, F = C.c_void_p, C.c_uint32, C.c_float
+        FP = C.POINTER(C.c_float)
+        sigs = {
+            "orc_intersect_ray_box": (C.c_int, [P, FP, FP, FP, FP]),
+            "orc_ray_triangle": (C.c_int, [FP, FP, FP, FP, FP, FP, FP]),
+            "orc_transform_point": (None, [FP, FP, FP]),
+            "orc_get_triangle": (None, [P, U32, U32, FP, FP, FP]),
+            "orc_closest_hit": (C.c_int, [P, P, P, P]),
+            "orc_closest_hit_pruned": (C.c_int, [P, P, P, P]),
+            "orc_closest_hit_brute": (C.c_int, [P, P, P]),
+            "orc_any_hit": (C.c_int, [P, P, P, P]),
+            "orc_trace_batch": (None, [P, P, U32, P, C.c_int, C.c_int, P]),
+            "orc_jenkins": (U32, [U32]),
+            "orc_init_rng": (U32, [U32, U32, U32, U32]),
+            "orc_xorshift": (U32, [C.POINTER(U32)]),
+            "orc_rand": (F, [C.POINTER(U32)]),
+            "orc_permute": (U32, [U32, U32, U32]),
+            "orc_randfloat": (F, [U32, U32]),
+            "orc_camera_ray": (None, [P, U32, U32, C.POINTER(U32), U32, P]),
+            "orc_offset_ray": (None, [FP, FP, FP]),
+            "orc_sample_rect_light": (None, [P, FP, FP, FP, FP, FP, FP, FP, FP, FP]),
+            "orc_bsdf_sample": (None, [P, FP, FP, FP, FP, P]),
+            "orc_bsdf_eval": (None, [P, FP, FP, FP, FP, FP, FP, FP]),
+            "orc_pt_sample": (None, [P, P, U32, U32, U32, FP]),
+            "orc_pt_frame": (None, [P, P, U32, U32, U32, U32, FP, C.c_int]),
+            "orc_reduce": (None, [FP, U32, U32, FP]),
+            "orc_accumulate": (None, [FP, FP, U32, U32, FP]),
+            "orc_tonemap": (None, [FP, U32, U32, FP]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def vptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def default_threads() -> int:
+    n = os.cpu_count() or 1
+    return max(1, min(n, 64))
+
+
+class Oracle:
+    """Holds the scene arrays and the skip-link BVH the oracle walks."""
+
+    def __init__(self, vb, ib, inst, nodes, lights=None, mats=None):
+        self.lib = load()
+        self.vb = np.ascontiguousarray(vb)
+        self.ib = np.ascontiguousarray(ib, dtype=np.uint32)
+        self.inst = np.ascontiguousarray(inst)
+        self.nodes = np.ascontiguousarray(nodes)
+        self.lights = None if lights is None else np.ascontiguousarray(lights)
+        self.mats = None if mats is None else np.ascontiguousarray(mats)
+        s = OrcScene()
+        s.nodes, s.nnodes = self.nodes.ctypes.data, self.nodes.size
+        s.inst, s.ninst = self.inst.ctypes.data, self.inst.size
+        s.vb, s.nv = self.vb.ctypes.data, self.vb.size
+        s.ib, s.ni = self.ib.ctypes.data, self.ib.size
+        if self.lights is not None and self.lights.size:
+            s.lights, s.nlights = self.lights.ctypes.data, self.lights.size
+        if self.mats is not None and self.mats.size:
+            s.mats, s.nmats = self.mats.ctypes.data, self.mats.size
+        self.scene = s
+
+    @classmethod
+    def from_scene(cls, scene, nodes):
+        return cls(scene.vb, scene.ib, scene.inst, nodes, scene.lights, scene.mats)
+
+    def trace(self, rays: np.ndarray, mode: int = MODE_REF, threads: int | None = None):
+        rays = np.ascontiguousarray(rays, dtype=RAY)
+        hits = np.zeros(rays.size, dtype=HIT)
+        st = OracleStats()
+        self.lib.orc_trace_batch(C.byref(self.scene), vptr(rays), rays.size, vptr(hits), mode,
+                                 threads or default_threads(), C.byref(st))
+        return hits, st
+
+    def get_triangle(self, inst: int, prim: int):
+        v0, e0, e1 = (np.zeros(3, np.float32) for _ in range(3))
+        self.lib.orc_get_triangle(C.byref(self.scene), inst, prim, fptr(v0), fptr(e0), fptr(e1))
+        return v0, e0, e1
+
+    def pt_frame(self, params, x0: int, y0: int, w: int, h: int, threads: int | None = None) -> np.ndarray:
+        spp = params.pt.spp
+        out = np.zeros(w * h * spp * 3, dtype=np.float32)
+        self.lib.orc_pt_frame(C.byref(self.scene), C.byref(params), x0, y0, w, h, fptr(out),
+                              threads or default_threads())
+        return out
+
+    def pt_sample(self, params, px: int, py: int, s: int) -> np.ndarray:
+        out = np.zeros(3, dtype=np.float32)
+        self.lib.orc_pt_sample(C.byref(self.scene), C.byref(params), px, py, s, fptr(out))
+        return out

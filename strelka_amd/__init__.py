@@ -1,0 +1,282 @@
+"""strelka_amd - MI355X (gfx950) path-tracing hot path behind a C ABI.
+
+Python mirror of the library's C entry points (include/stk/stk.h).  All work
+runs in libstrelka_amd.so: the BVH builder and scene container in host C++, the
+traversal / path-tracing / post-process kernels in HIP.  Importing this package
+fails if the library has not been built; nothing here falls back to Python or
+CPU code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from ._abi import (BVH_NODE, HIT, INSTANCE, INVALID, LIGHT, MATERIAL, PACKED_NODE, RAY, TRIANGLE, VERTEX,
+                   BvhHeader, BvhOptions, PathTracerParam, RenderParams, SceneArrays)
+
+__all__ = ["lib", "StkError", "SceneData", "HostBvh", "Context", "RenderParams", "PathTracerParam", "VERTEX",
+           "BVH_NODE", "INSTANCE", "LIGHT", "RAY", "HIT", "MATERIAL", "TRIANGLE", "PACKED_NODE", "INVALID",
+           "render_params"]
+
+_LIB = _abi.load()
+
+
+def lib() -> C.CDLL:
+    return _LIB
+
+
+class StkError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{_abi.STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _copy(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    buf = (C.c_char * (n * dtype.itemsize)).from_address(ptr)
+    return np.frombuffer(bytes(buf), dtype=dtype).copy()
+
+
+@dataclass
+class SceneData:
+    """Flat scene buffers (numpy copies) in the reference's GPU record layouts."""
+
+    vb: np.ndarray
+    ib: np.ndarray
+    inst: np.ndarray
+    lights: np.ndarray
+    mats: np.ndarray
+    camera: PathTracerParam
+    sky: tuple
+    _handle: C.c_void_p | None = None
+
+    @property
+    def num_triangles(self) -> int:
+        return int(self.ib.size // 3)
+
+    @classmethod
+    def generate(cls, config: int, variant: int = 0, ref_icosphere: bool = False) -> "SceneData":
+        h = C.c_void_p()
+        st = _LIB.stk_scene_generate(config, variant, 1 if ref_icosphere else 0, C.byref(h))
+        if st:
+            raise StkError(st, f"stk_scene_generate(config={config}, variant={variant})")
+        a = SceneArrays()
+        _LIB.stk_scene_get(h, C.byref(a))
+        cam = PathTracerParam()
+        C.memmove(C.byref(cam), C.byref(a.camera), C.sizeof(PathTracerParam))
+        sd = cls(vb=_copy(a.vb, a.numVertices, VERTEX), ib=_copy(a.ib, a.numIndices, np.dtype("<u4")),
+                 inst=_copy(a.inst, a.numInstances, INSTANCE), lights=_copy(a.lights, a.numLights, LIGHT),
+                 mats=_copy(a.mats, a.numMaterials, MATERIAL), camera=cam, sky=tuple(a.sky), _handle=h)
+        return sd
+
+    def jittered_rays(self, seed: int) -> np.ndarray:
+        w, h = self.camera.dimension[0], self.camera.dimension[1]
+        rays = np.zeros(w * h, dtype=RAY)
+        st = _LIB.stk_scene_jittered_rays(self._handle, C.c_uint64(seed), _ptr(rays), rays.size)
+        if st:
+            raise StkError(st, "stk_scene_jittered_rays")
+        return rays
+
+    def __del__(self):
+        if self._handle:
+            _LIB.stk_scene_free(self._handle)
+            self._handle = None
+
+
+class HostBvh:
+    """BVH built or packed on the host (no device needed)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+        n = _LIB.stk_bvh_num_nodes(handle)
+        self.nodes = _copy(_LIB.stk_bvh_nodes(handle), n, BVH_NODE)
+        self.packed = _copy(_LIB.stk_bvh_packed(handle), _LIB.stk_bvh_num_packed(handle), PACKED_NODE)
+        self.tris = _copy(_LIB.stk_bvh_triangles(handle), _LIB.stk_bvh_num_triangles(handle), TRIANGLE)
+        self.header = _LIB.stk_bvh_header(handle)
+        self.depth = int(_LIB.stk_bvh_depth(handle))
+
+    @classmethod
+    def build(cls, scene: SceneData, max_depth: int = 32, bins: int = 32) -> "HostBvh":
+        opts = BvhOptions(max_depth, bins, 1.0, 1.0)
+        h = C.c_void_p()
+        st = _LIB.stk_bvh_build_host(_ptr(scene.vb), scene.vb.size, _ptr(scene.ib), scene.ib.size, _ptr(scene.inst),
+                                     scene.inst.size, C.byref(opts), C.byref(h))
+        if st:
+            raise StkError(st, "stk_bvh_build_host")
+        return cls(h)
+
+    @classmethod
+    def from_nodes(cls, scene: SceneData, nodes: np.ndarray) -> "HostBvh":
+        nodes = np.ascontiguousarray(nodes, dtype=BVH_NODE)
+        h = C.c_void_p()
+        st = _LIB.stk_bvh_from_nodes(_ptr(scene.vb), scene.vb.size, _ptr(scene.ib), scene.ib.size, _ptr(scene.inst),
+                                     scene.inst.size, _ptr(nodes), nodes.size, C.byref(h))
+        if st:
+            raise StkError(st, "stk_bvh_from_nodes")
+        return cls(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _LIB.stk_bvh_free(self._h)
+            self._h = None
+
+
+def render_params(camera: PathTracerParam, spp: int = 1, max_depth: int = 8, frame: int = 0,
+                  sampling: int = 1, sky=(0.0, 0.0, 0.0), tonemap: int = 1, accumulate: bool = True,
+                  keep_samples: bool = False, rr_start: int = 3, enable_tonemap: bool = True) -> RenderParams:
+    p = RenderParams()
+    C.memmove(C.byref(p.pt), C.byref(camera), C.sizeof(PathTracerParam))
+    p.pt.spp = spp
+    p.pt.maxDepth = max_depth
+    p.pt.frameNumber = frame
+    p.pt.stratifiedSamplingType = sampling
+    p.rrStartDepth = rr_start
+    p.refCompat = 1
+    p.tonemapperType = tonemap
+    p.enableAccumulation = 1 if accumulate else 0
+    p.enableTonemap = 1 if enable_tonemap else 0
+    p.keepSamples = 1 if keep_samples else 0
+    for i in range(3):
+        p.skyColor[i] = float(sky[i])
+    p.skyColor[3] = 0.0
+    return p
+
+
+class Context:
+    """One device context (stk_ctx): owns device buffers and one HIP stream."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        st = _LIB.stk_create(device, C.byref(h))
+        if st:
+            raise StkError(st, f"stk_create(device={device})")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _check(self, st: int, what: str):
+        if st:
+            raise StkError(st, f"{what}: {_LIB.stk_last_error(self._h).decode()}")
+
+    def close(self):
+        if self._h:
+            _LIB.stk_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+    # -- scene -----------------------------------------------------------------
+    def set_scene(self, s: SceneData):
+        self._check(_LIB.stk_set_geometry(self._h, _ptr(s.vb), s.vb.size, _ptr(s.ib), s.ib.size, _ptr(s.inst),
+                                          s.inst.size), "stk_set_geometry")
+        self._check(_LIB.stk_set_lights(self._h, _ptr(s.lights), s.lights.size), "stk_set_lights")
+        self._check(_LIB.stk_set_materials(self._h, _ptr(s.mats), s.mats.size), "stk_set_materials")
+
+    def build_bvh(self, max_depth: int = 32, bins: int = 32):
+        opts = BvhOptions(max_depth, bins, 1.0, 1.0)
+        self._check(_LIB.stk_build_bvh(self._h, C.byref(opts)), "stk_build_bvh")
+
+    def set_bvh(self, nodes: np.ndarray):
+        nodes = np.ascontiguousarray(nodes, dtype=BVH_NODE)
+        self._check(_LIB.stk_set_bvh(self._h, _ptr(nodes), nodes.size), "stk_set_bvh")
+
+    def get_bvh(self) -> np.ndarray:
+        n = C.c_uint32()
+        self._check(_LIB.stk_get_bvh(self._h, None, 0, C.byref(n)), "stk_get_bvh")
+        out = np.zeros(n.value, dtype=BVH_NODE)
+        self._check(_LIB.stk_get_bvh(self._h, _ptr(out), out.size, C.byref(n)), "stk_get_bvh")
+        return out
+
+    # -- traversal (host buffers) ----------------------------------------------
+    def trace_closest(self, rays: np.ndarray, reference_order: bool = False) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=RAY)
+        hits = np.zeros(rays.size, dtype=HIT)
+        self._check(_LIB.stk_trace_closest(self._h, _ptr(rays), rays.size, _ptr(hits),
+                                           1 if reference_order else 0), "stk_trace_closest")
+        return hits
+
+    def trace_any(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=RAY)
+        hits = np.zeros(rays.size, dtype=HIT)
+        self._check(_LIB.stk_trace_any(self._h, _ptr(rays), rays.size, _ptr(hits)), "stk_trace_any")
+        return hits
+
+    # -- device buffers, events ------------------------------------------------
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self._check(_LIB.stk_device_alloc(self._h, nbytes, C.byref(p)), "stk_device_alloc")
+        return p.value
+
+    def free(self, ptr: int):
+        self._check(_LIB.stk_device_free(self._h, C.c_void_p(ptr)), "stk_device_free")
+
+    def upload(self, dst: int, a: np.ndarray):
+        self._check(_LIB.stk_copy_to_device(self._h, C.c_void_p(dst), _ptr(a), a.nbytes), "stk_copy_to_device")
+
+    def download(self, src: int, out: np.ndarray):
+        self._check(_LIB.stk_copy_to_host(self._h, _ptr(out), C.c_void_p(src), out.nbytes), "stk_copy_to_host")
+        return out
+
+    def trace_closest_device(self, d_rays: int, n: int, d_hits: int, reference_order: bool = False):
+        self._check(_LIB.stk_trace_closest_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits),
+                                                  1 if reference_order else 0), "stk_trace_closest_device")
+
+    def trace_any_device(self, d_rays: int, n: int, d_hits: int):
+        self._check(_LIB.stk_trace_any_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits)),
+                    "stk_trace_any_device")
+
+    def camera_rays_device(self, camera: PathTracerParam, sample: int, d_rays: int):
+        self._check(_LIB.stk_generate_camera_rays_device(self._h, C.byref(camera), sample, C.c_void_p(d_rays)),
+                    "stk_generate_camera_rays_device")
+
+    def record(self, slot: int):
+        self._check(_LIB.stk_event_record(self._h, slot), "stk_event_record")
+
+    def elapsed_ms(self, a: int, b: int) -> float:
+        ms = C.c_float()
+        self._check(_LIB.stk_event_elapsed(self._h, a, b, C.byref(ms)), "stk_event_elapsed")
+        return float(ms.value)
+
+    def sync(self):
+        self._check(_LIB.stk_sync(self._h), "stk_sync")
+
+    # -- rendering ---------------------------------------------------------------
+    def render(self, p: RenderParams, out: np.ndarray | None = None) -> np.ndarray:
+        w, h = p.pt.dimension[0], p.pt.dimension[1]
+        if out is None:
+            out = np.zeros((h, w, 4), dtype=np.float32)
+        self._check(_LIB.stk_render(self._h, C.byref(p), _ptr(out), 0), "stk_render")
+        return out
+
+    def render_device(self, p: RenderParams, d_out: int):
+        self._check(_LIB.stk_render(self._h, C.byref(p), C.c_void_p(d_out), 1), "stk_render")
+
+    def reset_accumulation(self):
+        self._check(_LIB.stk_reset_accumulation(self._h), "stk_reset_accumulation")
+
+    def samples(self, n: int) -> np.ndarray:
+        out = np.zeros(n, dtype=np.float32)
+        self._check(_LIB.stk_get_samples(self._h, _ptr(out), n), "stk_get_samples")
+        return out
+
+    def accumulation(self, w: int, h: int) -> np.ndarray:
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        self._check(_LIB.stk_get_accumulation(self._h, _ptr(out), out.size), "stk_get_accumulation")
+        return out

@@ -1,0 +1,135 @@
+"""ctypes binding of libstrelka_amd.so (include/stk/stk.h, include/stk/stk_scene.h).
+
+The structs below mirror include/stk/stk_types.h byte for byte; their sizes are
+checked on import.  Loading fails loudly when the library has not been built:
+there is no Python fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libstrelka_amd.so"
+
+# ---- numpy record types (array buffers) ------------------------------------
+VERTEX = np.dtype([("position", "<f4", 3), ("tangent", "<u4"), ("normal", "<u4"), ("uv", "<u4"),
+                   ("pad0", "<f4"), ("pad1", "<f4")])
+BVH_NODE = np.dtype([("minBounds", "<f4", 3), ("instId", "<i4"), ("maxBounds", "<f4", 3), ("nodeOffset", "<i4")])
+INSTANCE = np.dtype([("objectToWorld", "<f4", 16), ("worldToObject", "<f4", 16), ("normalMatrix", "<f4", 16),
+                     ("materialId", "<i4"), ("indexOffset", "<i4"), ("indexCount", "<i4"), ("lightId", "<i4")])
+LIGHT = np.dtype([("points", "<f4", (4, 4)), ("color", "<f4", 4), ("normal", "<f4", 4), ("type", "<i4"),
+                  ("pad", "<f4", 3)])
+RAY = np.dtype([("o", "<f4", 4), ("d", "<f4", 4)])
+HIT = np.dtype([("bary", "<f4", 2), ("t", "<f4"), ("instId", "<u4"), ("primId", "<u4")])
+MATERIAL = np.dtype([("type", "<u4"), ("roughness", "<f4"), ("pad", "<f4", 2), ("albedo", "<f4", 4)])
+TRIANGLE = np.dtype([("v0", "<f4", 3), ("leaf", "<u4"), ("e0", "<f4", 3), ("instId", "<u4"), ("e1", "<f4", 3),
+                     ("primId", "<u4")])
+PACKED_NODE = np.dtype([("x", "<f4", 4), ("y", "<f4", 4), ("z", "<f4", 4), ("child", "<i4", 2), ("pad", "<u4", 2)])
+
+for _dt, _sz in ((VERTEX, 32), (BVH_NODE, 32), (INSTANCE, 208), (LIGHT, 112), (RAY, 32), (HIT, 20),
+                 (MATERIAL, 32), (TRIANGLE, 48), (PACKED_NODE, 64)):
+    assert _dt.itemsize == _sz, (_dt, _sz)
+
+INVALID = 0xFFFFFFFF
+
+
+# ---- ctypes structs passed by pointer -------------------------------------
+class PathTracerParam(C.Structure):
+    _fields_ = [("viewToWorld", C.c_float * 16), ("clipToView", C.c_float * 16), ("viewToClip", C.c_float * 16),
+                ("worldToView", C.c_float * 16), ("camPos", C.c_float * 4), ("dimension", C.c_int32 * 2),
+                ("invDimension", C.c_float * 2), ("frameNumber", C.c_uint32), ("maxDepth", C.c_uint32),
+                ("debug", C.c_uint32), ("numLights", C.c_uint32), ("len", C.c_int32), ("spp", C.c_int32),
+                ("iteration", C.c_int32), ("stratifiedSamplingType", C.c_uint32), ("pad0", C.c_float * 4)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("pt", PathTracerParam), ("rrStartDepth", C.c_uint32), ("refCompat", C.c_uint32),
+                ("tonemapperType", C.c_uint32), ("enableAccumulation", C.c_uint32), ("skyColor", C.c_float * 4),
+                ("enableTonemap", C.c_uint32), ("keepSamples", C.c_uint32), ("pad", C.c_uint32 * 2)]
+
+
+class BvhOptions(C.Structure):
+    _fields_ = [("maxDepth", C.c_uint32), ("numBins", C.c_uint32), ("traversalCost", C.c_float),
+                ("intersectionCost", C.c_float)]
+
+
+class BvhHeader(C.Structure):
+    _fields_ = [("lo", C.c_float * 3), ("root", C.c_int32), ("hi", C.c_float * 3), ("numTriangles", C.c_uint32)]
+
+
+class SceneArrays(C.Structure):
+    _fields_ = [("vb", C.c_void_p), ("numVertices", C.c_uint32), ("ib", C.c_void_p), ("numIndices", C.c_uint32),
+                ("inst", C.c_void_p), ("numInstances", C.c_uint32), ("lights", C.c_void_p),
+                ("numLights", C.c_uint32), ("mats", C.c_void_p), ("numMaterials", C.c_uint32),
+                ("numTriangles", C.c_uint32), ("pad", C.c_uint32), ("camera", PathTracerParam),
+                ("sky", C.c_float * 4)]
+
+
+assert C.sizeof(PathTracerParam) == 336
+assert C.sizeof(RenderParams) == 336 + 48
+assert C.sizeof(BvhHeader) == 32
+
+STATUS = {0: "STK_OK", 1: "STK_E_INVALID", 2: "STK_E_OOM", 3: "STK_E_DEVICE", 4: "STK_E_STATE", 5: "STK_E_TIMEOUT"}
+
+# (name, restype, argtypes) for every entry point declared in include/stk/*.h
+P = C.c_void_p
+U32 = C.c_uint32
+PROTOTYPES = [
+    ("stk_version", C.c_char_p, []),
+    ("stk_create", C.c_int, [C.c_int, C.POINTER(P)]),
+    ("stk_destroy", None, [P]),
+    ("stk_last_error", C.c_char_p, [P]),
+    ("stk_sync", C.c_int, [P]),
+    ("stk_stream", P, [P]),
+    ("stk_set_geometry", C.c_int, [P, P, U32, P, U32, P, U32]),
+    ("stk_set_lights", C.c_int, [P, P, U32]),
+    ("stk_set_materials", C.c_int, [P, P, U32]),
+    ("stk_build_bvh", C.c_int, [P, C.POINTER(BvhOptions)]),
+    ("stk_set_bvh", C.c_int, [P, P, U32]),
+    ("stk_get_bvh", C.c_int, [P, P, U32, C.POINTER(U32)]),
+    ("stk_trace_closest", C.c_int, [P, P, U32, P, U32]),
+    ("stk_trace_any", C.c_int, [P, P, U32, P]),
+    ("stk_trace_closest_device", C.c_int, [P, P, U32, P, U32]),
+    ("stk_trace_any_device", C.c_int, [P, P, U32, P]),
+    ("stk_generate_camera_rays_device", C.c_int, [P, C.POINTER(PathTracerParam), U32, P]),
+    ("stk_render", C.c_int, [P, C.POINTER(RenderParams), P, C.c_int]),
+    ("stk_reset_accumulation", C.c_int, [P]),
+    ("stk_get_samples", C.c_int, [P, P, C.c_size_t]),
+    ("stk_get_accumulation", C.c_int, [P, P, C.c_size_t]),
+    ("stk_device_alloc", C.c_int, [P, C.c_size_t, C.POINTER(P)]),
+    ("stk_device_free", C.c_int, [P, P]),
+    ("stk_copy_to_device", C.c_int, [P, P, P, C.c_size_t]),
+    ("stk_copy_to_host", C.c_int, [P, P, P, C.c_size_t]),
+    ("stk_event_record", C.c_int, [P, C.c_int]),
+    ("stk_event_elapsed", C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+    ("stk_bvh_build_host", C.c_int, [P, U32, P, U32, P, U32, C.POINTER(BvhOptions), C.POINTER(P)]),
+    ("stk_bvh_from_nodes", C.c_int, [P, U32, P, U32, P, U32, P, U32, C.POINTER(P)]),
+    ("stk_bvh_free", None, [P]),
+    ("stk_bvh_num_nodes", U32, [P]),
+    ("stk_bvh_nodes", P, [P]),
+    ("stk_bvh_num_packed", U32, [P]),
+    ("stk_bvh_packed", P, [P]),
+    ("stk_bvh_num_triangles", U32, [P]),
+    ("stk_bvh_triangles", P, [P]),
+    ("stk_bvh_header", BvhHeader, [P]),
+    ("stk_bvh_depth", U32, [P]),
+    ("stk_scene_generate", C.c_int, [U32, U32, U32, C.POINTER(P)]),
+    ("stk_scene_free", None, [P]),
+    ("stk_scene_get", C.c_int, [P, C.POINTER(SceneArrays)]),
+    ("stk_scene_jittered_rays", C.c_int, [P, C.c_uint64, P, U32]),
+]
+
+
+def load(path: Path | None = None) -> C.CDLL:
+    path = Path(path or LIB_PATH)
+    if not path.exists():
+        raise ImportError(f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                          "(there is no CPU fallback for the HIP library)")
+    lib = C.CDLL(str(path))
+    for name, res, args in PROTOTYPES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib

@@ -1,0 +1,443 @@
+// bvh.cpp - host BVH builder (replaces the reference's Embree build) and the
+// packer that turns the reference's skip-link layout into the GPU layout.
+//
+// Reference: BvhBuilder::buildEmbree / repackEmbree (src/render/bvh.cpp:107-235),
+// AABB padding (include/render/bvh.h:138-151).  The reference asks Embree for a
+// binary SAH tree with one primitive per leaf and unit traversal and
+// intersection costs; Embree is not available, so this is our own binned SAH
+// builder emitting the same node layout: DFS preorder, the hit link of node i
+// is i+1, nodeOffset is the miss (skip) link, a left child's skip link is its
+// right sibling and a right child's skip link is its parent's.
+#include "bvh.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <thread>
+
+namespace stk {
+
+StkBvhOptions defaultBvhOptions()
+{
+    StkBvhOptions o;
+    o.maxDepth = 32;
+    o.numBins = 32;
+    o.traversalCost = 1.0f;
+    o.intersectionCost = 1.0f;
+    return o;
+}
+
+bool flattenInstances(const StkVertex* vb, uint32_t nv, const uint32_t* ib, uint32_t ni, const StkInstance* inst,
+                      uint32_t ninst, std::vector<BvhInputTri>& out, std::string& err)
+{
+    out.clear();
+    size_t total = 0;
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const StkInstance& ic = inst[i];
+        if (ic.indexOffset < 0 || ic.indexCount < 0 || ic.indexCount % 3 != 0 ||
+            (uint64_t)ic.indexOffset + (uint64_t)ic.indexCount > ni) {
+            err = "instance " + std::to_string(i) + " has an index range outside the index buffer";
+            return false;
+        }
+        total += (size_t)ic.indexCount / 3;
+    }
+    out.reserve(total);
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const StkInstance& ic = inst[i];
+        const uint32_t ntri = (uint32_t)ic.indexCount / 3;
+        for (uint32_t p = 0; p < ntri; ++p) {
+            BvhInputTri t;
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t vi = ib[(uint32_t)ic.indexOffset + p * 3 + k];
+                if (vi >= nv) {
+                    err = "index buffer entry outside the vertex buffer";
+                    return false;
+                }
+                t.v[k] = xform4(ic.objectToWorld, ld3(vb[vi].position), 1.0f);
+            }
+            t.instId = i;
+            t.primId = p;
+            out.push_back(t);
+        }
+    }
+    return true;
+}
+
+namespace {
+
+struct PrimRef {
+    float lo[3], hi[3], c[3];
+    uint32_t id;
+};
+
+struct Box {
+    float lo[3] = {1e30f, 1e30f, 1e30f};
+    float hi[3] = {-1e30f, -1e30f, -1e30f};
+    void grow(const float* l, const float* h)
+    {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], l[a]);
+            hi[a] = std::max(hi[a], h[a]);
+        }
+    }
+    // AABB::halfArea (include/render/bvh.h:64-67)
+    float halfArea() const
+    {
+        if (hi[0] < lo[0]) return 0.0f;
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        return dx * (dy + dz) + dy * dz;
+    }
+};
+
+inline uint32_t ceilLog2(uint32_t n)
+{
+    uint32_t l = 0;
+    while ((1u << l) < n) ++l;
+    return l;
+}
+
+class Builder {
+public:
+    Builder(const std::vector<BvhInputTri>& tris, const StkBvhOptions& o, std::vector<StkBvhNode>& nodes)
+        : tris_(tris), opts_(o), nodes_(nodes)
+    {
+    }
+
+    void run()
+    {
+        const uint32_t n = (uint32_t)tris_.size();
+        refs_.resize(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            const BvhInputTri& t = tris_[i];
+            PrimRef& r = refs_[i];
+            // boundingBox (include/render/bvh.h:138-151): max padded by 1e-6
+            const float* v0 = &t.v[0].x;
+            const float* v1 = &t.v[1].x;
+            const float* v2 = &t.v[2].x;
+            for (int a = 0; a < 3; ++a) {
+                r.lo[a] = std::min(v0[a], std::min(v1[a], v2[a]));
+                r.hi[a] = std::max(v0[a], std::max(v1[a], v2[a])) + 1e-6f;
+                r.c[a] = (r.lo[a] + r.hi[a]) * 0.5f;
+            }
+            r.id = i;
+        }
+        nodes_.assign(2 * (size_t)n - 1, StkBvhNode{});
+        maxDepthSeen_ = 0;
+        emit(0, n, 0, STK_INVALID_INDEX, 0);
+    }
+
+    uint32_t depth() const { return maxDepthSeen_.load(); }
+
+private:
+    void noteDepth(uint32_t d)
+    {
+        uint32_t cur = maxDepthSeen_.load();
+        while (d > cur && !maxDepthSeen_.compare_exchange_weak(cur, d)) {
+        }
+    }
+
+    void emit(uint32_t begin, uint32_t end, uint32_t idx, uint32_t skip, uint32_t depth)
+    {
+        const uint32_t n = end - begin;
+        StkBvhNode& node = nodes_[idx];
+        node.nodeOffset = (int32_t)skip;
+        if (n == 1) {
+            const BvhInputTri& t = tris_[refs_[begin].id];
+            node.minBounds[0] = bits_to_float(t.primId);
+            node.minBounds[1] = 0.0f;
+            node.minBounds[2] = 0.0f;
+            node.maxBounds[0] = node.maxBounds[1] = node.maxBounds[2] = 0.0f;
+            node.instId = (int32_t)t.instId;
+            noteDepth(depth);
+            return;
+        }
+        Box b;
+        for (uint32_t i = begin; i < end; ++i) b.grow(refs_[i].lo, refs_[i].hi);
+        for (int a = 0; a < 3; ++a) {
+            node.minBounds[a] = b.lo[a];
+            node.maxBounds[a] = b.hi[a];
+        }
+        node.instId = (int32_t)STK_INVALID_INDEX;
+        const uint32_t mid = split(begin, end, depth, b);
+        const uint32_t nl = mid - begin;
+        const uint32_t left = idx + 1, right = idx + 2 * nl; // left subtree holds 2*nl-1 nodes
+        if (n > 65536) {
+            std::thread th([&] { emit(begin, mid, left, right, depth + 1); });
+            emit(mid, end, right, skip, depth + 1);
+            th.join();
+        } else {
+            emit(begin, mid, left, right, depth + 1);
+            emit(mid, end, right, skip, depth + 1);
+        }
+    }
+
+    uint32_t medianSplit(uint32_t begin, uint32_t end, int axis)
+    {
+        const uint32_t mid = begin + (end - begin) / 2;
+        std::nth_element(refs_.begin() + begin, refs_.begin() + mid, refs_.begin() + end,
+                         [axis](const PrimRef& a, const PrimRef& b) {
+                             return a.c[axis] < b.c[axis] || (a.c[axis] == b.c[axis] && a.id < b.id);
+                         });
+        return mid;
+    }
+
+    uint32_t split(uint32_t begin, uint32_t end, uint32_t depth, const Box& nodeBox)
+    {
+        const uint32_t n = end - begin;
+        Box cb;
+        for (uint32_t i = begin; i < end; ++i) cb.grow(refs_[i].c, refs_[i].c);
+        float ext[3];
+        int axis = 0;
+        for (int a = 0; a < 3; ++a) {
+            ext[a] = cb.hi[a] - cb.lo[a];
+            if (ext[a] > ext[axis]) axis = a;
+        }
+        if (!(ext[axis] > 0.0f)) return begin + n / 2; // all centroids coincide
+        if (n <= 2) return medianSplit(begin, end, axis);
+
+        const uint32_t maxD = opts_.maxDepth;
+        const uint32_t nbins = std::min<uint32_t>(opts_.numBins ? opts_.numBins : 32, 64);
+        struct Bin {
+            Box b;
+            uint32_t n = 0;
+        };
+        float bestCost = INFINITY;
+        int bestAxis = -1;
+        uint32_t bestBin = 0;
+        const float parentArea = nodeBox.halfArea();
+        for (int a = 0; a < 3; ++a) {
+            if (!(ext[a] > 0.0f)) continue;
+            Bin bins[64];
+            const float scale = (float)nbins / ext[a] * (1.0f - 1e-6f);
+            for (uint32_t i = begin; i < end; ++i) {
+                const PrimRef& r = refs_[i];
+                uint32_t bi = (uint32_t)((r.c[a] - cb.lo[a]) * scale);
+                if (bi >= nbins) bi = nbins - 1;
+                bins[bi].b.grow(r.lo, r.hi);
+                bins[bi].n++;
+            }
+            float rightArea[64];
+            uint32_t rightN[64];
+            Box acc;
+            uint32_t cnt = 0;
+            for (uint32_t i = nbins - 1; i > 0; --i) {
+                acc.grow(bins[i].b.lo, bins[i].b.hi);
+                cnt += bins[i].n;
+                rightArea[i] = acc.halfArea();
+                rightN[i] = cnt;
+            }
+            Box lacc;
+            uint32_t lcnt = 0;
+            for (uint32_t i = 0; i + 1 < nbins; ++i) {
+                lacc.grow(bins[i].b.lo, bins[i].b.hi);
+                lcnt += bins[i].n;
+                const uint32_t rcnt = rightN[i + 1];
+                if (lcnt == 0 || rcnt == 0) continue;
+                const uint32_t bigger = std::max(lcnt, rcnt);
+                if (depth + 1 + ceilLog2(bigger) > maxD) continue; // keep the depth bound
+                const float cost = opts_.traversalCost + opts_.intersectionCost *
+                                                             (lacc.halfArea() * (float)lcnt +
+                                                              rightArea[i + 1] * (float)rcnt) /
+                                                             (parentArea > 0.0f ? parentArea : 1.0f);
+                if (cost < bestCost) {
+                    bestCost = cost;
+                    bestAxis = a;
+                    bestBin = i;
+                }
+            }
+        }
+        if (bestAxis < 0) return medianSplit(begin, end, axis);
+        const int a = bestAxis;
+        const float scale = (float)nbins / ext[a] * (1.0f - 1e-6f);
+        auto it = std::partition(refs_.begin() + begin, refs_.begin() + end, [&](const PrimRef& r) {
+            uint32_t bi = (uint32_t)((r.c[a] - cb.lo[a]) * scale);
+            if (bi >= nbins) bi = nbins - 1;
+            return bi <= bestBin;
+        });
+        const uint32_t mid = (uint32_t)(it - refs_.begin());
+        if (mid == begin || mid == end) return medianSplit(begin, end, axis);
+        return mid;
+    }
+
+    const std::vector<BvhInputTri>& tris_;
+    StkBvhOptions opts_;
+    std::vector<StkBvhNode>& nodes_;
+    std::vector<PrimRef> refs_;
+    std::atomic<uint32_t> maxDepthSeen_{0};
+};
+
+} // namespace
+
+bool buildBvh(const std::vector<BvhInputTri>& tris, const StkBvhOptions& optsIn, Bvh& out, std::string& err)
+{
+    out = Bvh{};
+    StkBvhOptions opts = optsIn;
+    if (opts.maxDepth == 0) opts.maxDepth = 32;
+    if (opts.maxDepth > 32) {
+        err = "maxDepth above 32 is not supported by the GPU traversal stack";
+        return false;
+    }
+    if (tris.empty()) {
+        out.header.root = INT32_MIN;
+        return true;
+    }
+    if (ceilLog2((uint32_t)tris.size()) > opts.maxDepth) {
+        err = "too many triangles for the requested maxDepth";
+        return false;
+    }
+    Builder b(tris, opts, out.nodes);
+    b.run();
+    out.depth = b.depth();
+    return true;
+}
+
+bool packBvh(const StkVertex* vb, uint32_t nv, const uint32_t* ib, uint32_t ni, const StkInstance* inst,
+             uint32_t ninst, Bvh& bvh, std::string& err)
+{
+    const uint32_t n = (uint32_t)bvh.nodes.size();
+    bvh.packed.clear();
+    bvh.tris.clear();
+    bvh.leafToTri.assign(n, STK_INVALID_INDEX);
+    bvh.header = StkBvhHeader{};
+    if (n == 0) {
+        bvh.header.root = INT32_MIN;
+        bvh.depth = 0;
+        return true;
+    }
+    // Validate the skip-link structure by walking it as a binary tree.
+    std::vector<uint32_t> packedIndex(n, STK_INVALID_INDEX);
+    std::vector<uint8_t> seen(n, 0);
+    struct Item {
+        uint32_t idx, skip, depth;
+    };
+    std::vector<Item> stack;
+    stack.push_back({0, STK_INVALID_INDEX, 0});
+    uint32_t nleaf = 0, ninner = 0, maxDepth = 0;
+    std::vector<uint32_t> order; // preorder as walked
+    order.reserve(n);
+    while (!stack.empty()) {
+        Item it = stack.back();
+        stack.pop_back();
+        if (it.idx >= n || seen[it.idx]) {
+            err = "BVH node index out of range or visited twice";
+            return false;
+        }
+        seen[it.idx] = 1;
+        order.push_back(it.idx);
+        const StkBvhNode& nd = bvh.nodes[it.idx];
+        if ((uint32_t)nd.nodeOffset != it.skip) {
+            err = "BVH skip link of node " + std::to_string(it.idx) + " is inconsistent";
+            return false;
+        }
+        maxDepth = std::max(maxDepth, it.depth);
+        if ((uint32_t)nd.instId != STK_INVALID_INDEX) {
+            const uint32_t prim = float_to_bits(nd.minBounds[0]);
+            if ((uint32_t)nd.instId >= ninst || prim >= (uint32_t)inst[nd.instId].indexCount / 3) {
+                err = "BVH leaf references a missing instance or primitive";
+                return false;
+            }
+            ++nleaf;
+            continue;
+        }
+        ++ninner;
+        const uint32_t left = it.idx + 1;
+        if (left >= n) {
+            err = "BVH inner node without children";
+            return false;
+        }
+        const uint32_t right = (uint32_t)bvh.nodes[left].nodeOffset;
+        if (right >= n || right <= left) {
+            err = "BVH right child link is invalid";
+            return false;
+        }
+        stack.push_back({right, it.skip, it.depth + 1});
+        stack.push_back({left, right, it.depth + 1});
+    }
+    if (nleaf + ninner != n || ninner + 1 != nleaf) {
+        err = "BVH is not a full binary tree over its nodes";
+        return false;
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (order[i] != i) {
+            err = "BVH nodes are not stored in preorder";
+            return false;
+        }
+    bvh.depth = maxDepth;
+    // Number inner nodes and leaves in preorder.
+    uint32_t pi = 0, ti = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if ((uint32_t)bvh.nodes[i].instId == STK_INVALID_INDEX)
+            packedIndex[i] = pi++;
+        else
+            bvh.leafToTri[i] = ti++;
+    }
+    bvh.tris.resize(ti);
+    for (uint32_t i = 0; i < n; ++i) {
+        const StkBvhNode& nd = bvh.nodes[i];
+        if ((uint32_t)nd.instId == STK_INVALID_INDEX) continue;
+        const StkInstance& ic = inst[nd.instId];
+        const uint32_t prim = float_to_bits(nd.minBounds[0]);
+        uint32_t vi[3];
+        for (int k = 0; k < 3; ++k) {
+            const uint64_t at = (uint64_t)(uint32_t)ic.indexOffset + (uint64_t)prim * 3 + (uint64_t)k;
+            if (at >= ni || ib[at] >= nv) {
+                err = "BVH leaf triangle indexes outside the geometry buffers";
+                return false;
+            }
+            vi[k] = ib[at];
+        }
+        // getTriangle (shaders/raytracing.h:134-153)
+        const F3 v0 = xform4(ic.objectToWorld, ld3(vb[vi[0]].position), 1.0f);
+        const F3 e0 = xform4(ic.objectToWorld, ld3(vb[vi[1]].position), 1.0f) - v0;
+        const F3 e1 = xform4(ic.objectToWorld, ld3(vb[vi[2]].position), 1.0f) - v0;
+        StkTriangle& t = bvh.tris[bvh.leafToTri[i]];
+        t.v0[0] = v0.x;
+        t.v0[1] = v0.y;
+        t.v0[2] = v0.z;
+        t.e0[0] = e0.x;
+        t.e0[1] = e0.y;
+        t.e0[2] = e0.z;
+        t.e1[0] = e1.x;
+        t.e1[1] = e1.y;
+        t.e1[2] = e1.z;
+        t.leaf = i;
+        t.instId = (uint32_t)nd.instId;
+        t.primId = prim;
+    }
+    bvh.packed.resize(pi);
+    for (uint32_t i = 0; i < n; ++i) {
+        const StkBvhNode& nd = bvh.nodes[i];
+        if ((uint32_t)nd.instId != STK_INVALID_INDEX) continue;
+        StkPackedNode& p = bvh.packed[packedIndex[i]];
+        memset(&p, 0, sizeof(p));
+        const uint32_t c[2] = {i + 1, (uint32_t)bvh.nodes[i + 1].nodeOffset};
+        for (int k = 0; k < 2; ++k) {
+            const StkBvhNode& cn = bvh.nodes[c[k]];
+            if ((uint32_t)cn.instId != STK_INVALID_INDEX) {
+                p.child[k] = ~(int32_t)bvh.leafToTri[c[k]];
+                continue;
+            }
+            p.child[k] = (int32_t)packedIndex[c[k]];
+            p.x[2 * k + 0] = cn.minBounds[0];
+            p.x[2 * k + 1] = cn.maxBounds[0];
+            p.y[2 * k + 0] = cn.minBounds[1];
+            p.y[2 * k + 1] = cn.maxBounds[1];
+            p.z[2 * k + 0] = cn.minBounds[2];
+            p.z[2 * k + 1] = cn.maxBounds[2];
+        }
+    }
+    StkBvhHeader& h = bvh.header;
+    h.numTriangles = ti;
+    if ((uint32_t)bvh.nodes[0].instId != STK_INVALID_INDEX) {
+        h.root = ~(int32_t)0;
+    } else {
+        h.root = 0;
+        for (int a = 0; a < 3; ++a) {
+            h.lo[a] = bvh.nodes[0].minBounds[a];
+            h.hi[a] = bvh.nodes[0].maxBounds[a];
+        }
+    }
+    return true;
+}
+
+} // namespace stk

@@ -1,0 +1,76 @@
+// bvh_api.cpp - host-only BVH entry points of the C ABI (no device needed).
+#include <new>
+
+#include "../../include/stk/stk.h"
+#include "bvh.hpp"
+
+struct stk_bvh {
+    stk::Bvh bvh;
+};
+
+extern "C" {
+
+STK_API stk_status stk_bvh_build_host(const StkVertex* vb, uint32_t nv, const uint32_t* ib, uint32_t ni,
+                                      const StkInstance* inst, uint32_t ninst, const StkBvhOptions* opts,
+                                      stk_bvh** out)
+{
+    if (!out || (nv && !vb) || (ni && !ib) || (ninst && !inst)) return STK_E_INVALID;
+    *out = nullptr;
+    stk_bvh* b = new (std::nothrow) stk_bvh();
+    if (!b) return STK_E_OOM;
+    try {
+        std::vector<stk::BvhInputTri> tris;
+        std::string err;
+        const StkBvhOptions o = opts ? *opts : stk::defaultBvhOptions();
+        if (!stk::flattenInstances(vb, nv, ib, ni, inst, ninst, tris, err) || !stk::buildBvh(tris, o, b->bvh, err) ||
+            !stk::packBvh(vb, nv, ib, ni, inst, ninst, b->bvh, err)) {
+            delete b;
+            return STK_E_INVALID;
+        }
+    } catch (const std::bad_alloc&) {
+        delete b;
+        return STK_E_OOM;
+    }
+    *out = b;
+    return STK_OK;
+}
+
+STK_API stk_status stk_bvh_from_nodes(const StkVertex* vb, uint32_t nv, const uint32_t* ib, uint32_t ni,
+                                      const StkInstance* inst, uint32_t ninst, const StkBvhNode* nodes,
+                                      uint32_t numNodes, stk_bvh** out)
+{
+    if (!out || (numNodes && !nodes)) return STK_E_INVALID;
+    *out = nullptr;
+    stk_bvh* b = new (std::nothrow) stk_bvh();
+    if (!b) return STK_E_OOM;
+    try {
+        std::string err;
+        b->bvh.nodes.assign(nodes, nodes + numNodes);
+        if (!stk::packBvh(vb, nv, ib, ni, inst, ninst, b->bvh, err)) {
+            delete b;
+            return STK_E_INVALID;
+        }
+    } catch (const std::bad_alloc&) {
+        delete b;
+        return STK_E_OOM;
+    }
+    *out = b;
+    return STK_OK;
+}
+
+STK_API void stk_bvh_free(stk_bvh* b) { delete b; }
+STK_API uint32_t stk_bvh_num_nodes(const stk_bvh* b) { return b ? (uint32_t)b->bvh.nodes.size() : 0; }
+STK_API const StkBvhNode* stk_bvh_nodes(const stk_bvh* b) { return b ? b->bvh.nodes.data() : nullptr; }
+STK_API uint32_t stk_bvh_num_packed(const stk_bvh* b) { return b ? (uint32_t)b->bvh.packed.size() : 0; }
+STK_API const StkPackedNode* stk_bvh_packed(const stk_bvh* b) { return b ? b->bvh.packed.data() : nullptr; }
+STK_API uint32_t stk_bvh_num_triangles(const stk_bvh* b) { return b ? (uint32_t)b->bvh.tris.size() : 0; }
+STK_API const StkTriangle* stk_bvh_triangles(const stk_bvh* b) { return b ? b->bvh.tris.data() : nullptr; }
+STK_API StkBvhHeader stk_bvh_header(const stk_bvh* b)
+{
+    StkBvhHeader h{};
+    h.root = INT32_MIN;
+    return b ? b->bvh.header : h;
+}
+STK_API uint32_t stk_bvh_depth(const stk_bvh* b) { return b ? b->bvh.depth : 0; }
+
+} // extern "C"

@@ -1,0 +1,326 @@
+// scenegen.cpp - seeded synthetic workloads (see include/stk/stk_scene.h).
+#include <cmath>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "../../include/stk/stk_scene.h"
+#include "scene.hpp"
+
+namespace stk {
+
+// PCG32 XSH-RR (O'Neill 2014).
+struct Pcg32 {
+    uint64_t state = 0, inc = 1;
+    Pcg32(uint64_t seed, uint64_t seq)
+    {
+        inc = (seq << 1u) | 1u;
+        next();
+        state += seed;
+        next();
+    }
+    uint32_t next()
+    {
+        const uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((-rot) & 31u));
+    }
+    float uniform() { return (float)(next() >> 8) * (1.0f / 16777216.0f); } // [0,1)
+    float uniform(float a, float b) { return a + (b - a) * uniform(); }
+};
+
+static constexpr uint64_t kSeedBase = 0x5354524B00000000ULL; // "STRK"
+
+namespace {
+
+// Value noise on a 256x256 wrapped lattice, smoothstep interpolation.
+struct ValueNoise {
+    std::vector<float> lat;
+    explicit ValueNoise(Pcg32& rng) : lat(256 * 256)
+    {
+        for (float& v : lat) v = rng.uniform(-1.0f, 1.0f);
+    }
+    float at(int x, int y) const { return lat[(size_t)(y & 255) * 256 + (size_t)(x & 255)]; }
+    float eval(float x, float y) const
+    {
+        const float fx = std::floor(x), fy = std::floor(y);
+        const int ix = (int)fx, iy = (int)fy;
+        float tx = x - fx, ty = y - fy;
+        tx = tx * tx * (3.0f - 2.0f * tx);
+        ty = ty * ty * (3.0f - 2.0f * ty);
+        const float a = at(ix, iy), b = at(ix + 1, iy), c = at(ix, iy + 1), d = at(ix + 1, iy + 1);
+        return (a + (b - a) * tx) + ((c + (d - c) * tx) - (a + (b - a) * tx)) * ty;
+    }
+    // fBm: 6 octaves, gain 0.5, lacunarity 2, normalised to [-1, 1]
+    float fbm(float x, float y) const
+    {
+        float sum = 0.0f, amp = 1.0f, freq = 1.0f, norm = 0.0f;
+        for (int o = 0; o < 6; ++o) {
+            sum += amp * eval(x * freq + 17.0f * (float)o, y * freq + 31.0f * (float)o);
+            norm += amp;
+            amp *= 0.5f;
+            freq *= 2.0f;
+        }
+        return sum / norm;
+    }
+};
+
+void addHeightfield(Scene& sc, Pcg32& rng, uint32_t res, float extent, float amplitude, int32_t material)
+{
+    ValueNoise noise(rng);
+    const float baseFreq = 0.2f;
+    auto height = [&](float x, float z) { return amplitude * noise.fbm(x * baseFreq + 64.0f, z * baseFreq + 64.0f); };
+    const uint32_t nv = res + 1;
+    std::vector<StkVertex> vb((size_t)nv * nv);
+    const float step = 2.0f * extent / (float)res;
+    const float h = step * 0.5f;
+    for (uint32_t j = 0; j < nv; ++j)
+        for (uint32_t i = 0; i < nv; ++i) {
+            const float x = -extent + step * (float)i, z = -extent + step * (float)j;
+            StkVertex& v = vb[(size_t)j * nv + i];
+            v = StkVertex{};
+            v.position[0] = x;
+            v.position[1] = height(x, z);
+            v.position[2] = z;
+            const float dx = (height(x + h, z) - height(x - h, z)) / (2.0f * h);
+            const float dz = (height(x, z + h) - height(x, z - h)) / (2.0f * h);
+            F3 n = mk3(-dx, 1.0f, -dz);
+            const float l = std::sqrt(dot(n, n));
+            n = mk3(n.x / l, n.y / l, n.z / l);
+            v.normal = packNormal(n);
+            v.tangent = packNormal(mk3(1.0f, 0.0f, 0.0f));
+            v.uv = packUV((float)i / (float)res, (float)j / (float)res);
+        }
+    std::vector<uint32_t> ib;
+    ib.reserve((size_t)res * res * 6);
+    for (uint32_t j = 0; j < res; ++j)
+        for (uint32_t i = 0; i < res; ++i) {
+            const uint32_t a = j * nv + i, b = a + 1, c = a + nv, d = c + 1;
+            const uint32_t t[6] = {a, c, b, b, c, d}; // geometric normals point up (+y)
+            ib.insert(ib.end(), t, t + 6);
+        }
+    const uint32_t mesh = sc.createMesh(vb, ib);
+    sc.createInstance(mesh, material, Mat4::identity());
+}
+
+StkMaterial randomMaterial(Pcg32& rng)
+{
+    StkMaterial m{};
+    m.type = rng.uniform() < 0.5f ? STK_MAT_LAMBERT : STK_MAT_GGX;
+    m.albedo[0] = rng.uniform(0.1f, 0.9f);
+    m.albedo[1] = rng.uniform(0.1f, 0.9f);
+    m.albedo[2] = rng.uniform(0.1f, 0.9f);
+    m.albedo[3] = 1.0f;
+    m.roughness = rng.uniform(0.05f, 1.0f);
+    return m;
+}
+
+void addSpheres(Scene& sc, Pcg32& geo, Pcg32& matRng, uint32_t count, float extent, int subdiv)
+{
+    if (count == 0) return;
+    const uint32_t mesh = sc.createIcosphereMesh(subdiv, false);
+    for (uint32_t i = 0; i < count; ++i) {
+        const float cx = geo.uniform(-extent, extent), cy = geo.uniform(0.5f, 4.0f), cz = geo.uniform(-extent, extent);
+        const float r = geo.uniform(0.05f, 0.5f);
+        const int32_t mat = (int32_t)sc.addMaterial(randomMaterial(matRng));
+        sc.createInstance(mesh, mat, Mat4::translate(cx, cy, cz) * Mat4::scale(r, r, r));
+    }
+}
+
+void addQuadLight(Scene& sc, float y, float size, float intensity)
+{
+    LightDesc d;
+    d.type = 0;
+    d.xform = Mat4::translate(0.0f, y, 0.0f) * Mat4::rotateX(-(float)M_PI / 2.0f); // emits towards -y
+    d.width = size;
+    d.height = size;
+    d.intensity = intensity;
+    sc.createLight(d);
+}
+
+void addInwardBox(Scene& sc, float half, int32_t material)
+{
+    // six faces, each two triangles, per-face vertices with inward normals
+    std::vector<StkVertex> vb;
+    std::vector<uint32_t> ib;
+    for (int axis = 0; axis < 3; ++axis)
+        for (int side = -1; side <= 1; side += 2) {
+            const int u = (axis + 1) % 3, v = (axis + 2) % 3;
+            float corners[4][3];
+            const float su[4] = {-1, 1, 1, -1}, sv[4] = {-1, -1, 1, 1};
+            for (int k = 0; k < 4; ++k) {
+                corners[k][axis] = half * (float)side;
+                corners[k][u] = half * su[k];
+                corners[k][v] = half * sv[k];
+            }
+            F3 n = mk3(0, 0, 0);
+            (&n.x)[axis] = -(float)side; // inward
+            const uint32_t base = (uint32_t)vb.size();
+            for (int k = 0; k < 4; ++k) {
+                StkVertex vx{};
+                vx.position[0] = corners[k][0];
+                vx.position[1] = corners[k][1];
+                vx.position[2] = corners[k][2];
+                vx.normal = packNormal(n);
+                vx.tangent = packNormal(mk3(1.0f, 0.0f, 0.0f));
+                vb.push_back(vx);
+            }
+            // orient the winding so the geometric normal points inwards
+            const F3 p0 = ld3(corners[0]), p1 = ld3(corners[1]), p2 = ld3(corners[2]);
+            const bool flip = dot(cross(p1 - p0, p2 - p0), n) < 0.0f;
+            const uint32_t t[6] = {0, 1, 2, 0, 2, 3};
+            for (int k = 0; k < 6; k += 3) {
+                ib.push_back(base + t[k]);
+                ib.push_back(base + (flip ? t[k + 2] : t[k + 1]));
+                ib.push_back(base + (flip ? t[k + 1] : t[k + 2]));
+            }
+        }
+    const uint32_t mesh = sc.createMesh(vb, ib);
+    sc.createInstance(mesh, material, Mat4::identity());
+}
+
+} // namespace
+} // namespace stk
+
+struct stk_scene {
+    stk::Scene scene;
+    std::vector<StkInstance> inst;
+    StkPathTracerParam camera{};
+    float sky[4] = {0, 0, 0, 0};
+    uint32_t config = 0;
+};
+
+using namespace stk;
+
+extern "C" STK_API stk_status stk_scene_generate(uint32_t config, uint32_t variant, uint32_t flags, stk_scene** out)
+{
+    if (!out) return STK_E_INVALID;
+    *out = nullptr;
+    std::unique_ptr<stk_scene> s(new (std::nothrow) stk_scene());
+    if (!s) return STK_E_OOM;
+    s->config = config;
+    Scene& sc = s->scene;
+    const bool refIco = (flags & STK_SCENE_REF_ICOSPHERE) != 0;
+    try {
+        if (config == 1) {
+            StkMaterial sphere{};
+            sphere.type = STK_MAT_LAMBERT;
+            sphere.albedo[0] = sphere.albedo[1] = sphere.albedo[2] = 0.8f;
+            StkMaterial wall = sphere;
+            wall.albedo[0] = wall.albedo[1] = wall.albedo[2] = 0.5f;
+            const int32_t ms = (int32_t)sc.addMaterial(sphere), mw = (int32_t)sc.addMaterial(wall);
+            const uint32_t mesh = sc.createIcosphereMesh(3, refIco);
+            sc.createInstance(mesh, ms, Mat4::identity());
+            addInwardBox(sc, 2.0f, mw);
+            cameraMatrices(mk3(0, 0, 1.9f), mk3(0, 0, 0), mk3(0, 1, 0), 60.0f, 256, 256, 0.01f, 100.0f, s->camera);
+        } else if (config == 2 || config == 3 || config == 6) {
+            uint32_t hfRes = 512, spheres = 2000, width = 1920, height = 1080;
+            float extent = 10.0f, lightY = 8.0f, lightSize = 2.0f, intensity = 100.0f;
+            uint64_t stream = 0;
+            if (config == 3) {
+                if (variant >= 32) return STK_E_INVALID;
+                Pcg32 pick(kSeedBase + 3, 100 + variant);
+                const double lnN = std::log(1e4) + (std::log(2e6) - std::log(1e4)) * (double)pick.uniform();
+                const double N = std::round(std::exp(lnN));
+                hfRes = (uint32_t)std::max(2.0, std::round(std::sqrt(0.45 * N / 2.0)));
+                spheres = (uint32_t)std::max(1.0, std::round(0.55 * N / 320.0));
+                width = 960;
+                height = 540;
+                stream = 16 * (uint64_t)(variant + 1);
+            } else if (config == 6) {
+                hfRes = 16;
+                spheres = 8;
+                width = 64;
+                height = 48;
+                extent = 4.0f;
+                lightY = 3.0f;
+                lightSize = 1.0f;
+                intensity = 10.0f;
+            }
+            Pcg32 geo(kSeedBase + config, 1 + stream), mat(kSeedBase + config, 2 + stream);
+            StkMaterial ground{};
+            ground.type = STK_MAT_LAMBERT;
+            ground.albedo[0] = 0.55f;
+            ground.albedo[1] = 0.5f;
+            ground.albedo[2] = 0.45f;
+            ground.roughness = 1.0f;
+            const int32_t mg = (int32_t)sc.addMaterial(ground);
+            addHeightfield(sc, geo, hfRes, extent, 1.5f, mg);
+            addSpheres(sc, geo, mat, spheres, extent, 2);
+            addQuadLight(sc, lightY, lightSize, intensity);
+            if (config == 6)
+                cameraMatrices(mk3(0, 3.0f, 6.0f), mk3(0, 0.3f, 0), mk3(0, 1, 0), 55.0f, width, height, 0.01f, 100.0f,
+                               s->camera);
+            else
+                // 60 degrees below the horizon; the whole frame falls on the heightfield
+                cameraMatrices(mk3(0, 17.0f, 9.5f), mk3(0, 0.0f, -0.3f), mk3(0, 1, 0), 24.0f, width, height, 0.01f,
+                               1000.0f, s->camera);
+            s->sky[0] = 0.5f;
+            s->sky[1] = 0.6f;
+            s->sky[2] = 0.8f;
+        } else {
+            return STK_E_INVALID;
+        }
+        s->inst = sc.getInstanceConstants();
+    } catch (const std::bad_alloc&) {
+        return STK_E_OOM;
+    }
+    *out = s.release();
+    return STK_OK;
+}
+
+extern "C" STK_API void stk_scene_free(stk_scene* s) { delete s; }
+
+extern "C" STK_API stk_status stk_scene_get(const stk_scene* s, StkSceneArrays* out)
+{
+    if (!s || !out) return STK_E_INVALID;
+    const Scene& sc = s->scene;
+    out->vb = sc.getVertices().data();
+    out->numVertices = (uint32_t)sc.getVertices().size();
+    out->ib = sc.getIndices().data();
+    out->numIndices = (uint32_t)sc.getIndices().size();
+    out->inst = s->inst.data();
+    out->numInstances = (uint32_t)s->inst.size();
+    out->lights = sc.getLights().data();
+    out->numLights = (uint32_t)sc.getLights().size();
+    out->mats = sc.getMaterials().data();
+    out->numMaterials = (uint32_t)sc.getMaterials().size();
+    out->numTriangles = out->numIndices / 3;
+    out->pad = 0;
+    out->camera = s->camera;
+    for (int i = 0; i < 4; ++i) out->sky[i] = s->sky[i];
+    return STK_OK;
+}
+
+extern "C" STK_API stk_status stk_scene_jittered_rays(const stk_scene* s, uint64_t seed, StkRay* rays, uint32_t n)
+{
+    if (!s || !rays) return STK_E_INVALID;
+    const StkPathTracerParam& p = s->camera;
+    const uint32_t W = (uint32_t)p.dimension[0], H = (uint32_t)p.dimension[1];
+    if ((uint64_t)W * H != n) return STK_E_INVALID;
+    Pcg32 rng(seed, 3);
+    const F3 eye = xform4(p.viewToWorld, mk3(0, 0, 0), 1.0f);
+    for (uint32_t py = 0; py < H; ++py)
+        for (uint32_t px = 0; px < W; ++px) {
+            const float jx = rng.uniform(), jy = rng.uniform();
+            const float ndcx = ((float)px + jx) / (float)W * 2.0f - 1.0f;
+            const float ndcy = ((float)py + jy) / (float)H * 2.0f - 1.0f;
+            const float* C = p.clipToView;
+            const F3 v = mk3(C[0] * ndcx + C[4] * ndcy + C[8] + C[12], C[1] * ndcx + C[5] * ndcy + C[9] + C[13],
+                             C[2] * ndcx + C[6] * ndcy + C[10] + C[14]);
+            const F3 w = xform4(p.viewToWorld, v, 0.0f);
+            const double l = std::sqrt((double)w.x * w.x + (double)w.y * w.y + (double)w.z * w.z);
+            StkRay& r = rays[(size_t)py * W + px];
+            r.o[0] = eye.x + rng.uniform(-1e-3f, 1e-3f);
+            r.o[1] = eye.y + rng.uniform(-1e-3f, 1e-3f);
+            r.o[2] = eye.z + rng.uniform(-1e-3f, 1e-3f);
+            r.o[3] = 1e9f;
+            r.d[0] = (float)(w.x / l);
+            r.d[1] = (float)(w.y / l);
+            r.d[2] = (float)(w.z / l);
+            r.d[3] = 0.0f;
+        }
+    return STK_OK;
+}

@@ -1,0 +1,161 @@
+"""GPU traversal parity through the C ABI: hit IDs, t and barycentrics must be
+bit-identical to the CPU oracle (reference order, shaders/raytracing.h:198-286)
+on the same BVH and rays."""
+import numpy as np
+import pytest
+
+import strelka_amd as sa
+from oracle import Oracle
+from oracle.pyoracle import MODE_ANY, MODE_PRUNED, MODE_REF
+
+pytestmark = pytest.mark.gpu
+
+INVALID = 0xFFFFFFFF
+
+
+def same(a, b):
+    return a.view(np.uint8).tobytes() == b.view(np.uint8).tobytes()
+
+
+def first_diff(a, b):
+    bad = np.flatnonzero(a.view(np.uint8).reshape(a.size, -1).any(axis=1) ^ False)
+    return bad
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = sa.Context(0)
+    yield c
+    c.close()
+
+
+def setup(ctx, config):
+    s = sa.SceneData.generate(config)
+    ctx.set_scene(s)
+    ctx.build_bvh()
+    nodes = ctx.get_bvh()
+    return s, Oracle.from_scene(s, nodes)
+
+
+def test_config1_all_rays_bit_exact(ctx):
+    print("case: config 1, 65,536 jittered rays", flush=True)
+    s, o = setup(ctx, 1)
+    rays = s.jittered_rays(0x5354524B00000001)
+    ref, _ = o.trace(rays, MODE_REF)
+    gpu = ctx.trace_closest(rays)
+    gpu_ref = ctx.trace_closest(rays, reference_order=True)
+    mism = np.flatnonzero(gpu.view(np.uint8).reshape(-1, 20) != ref.view(np.uint8).reshape(-1, 20))
+    assert same(gpu, ref), f"{np.unique(mism // 20).size} rays differ"
+    assert same(gpu_ref, ref)
+    anyh_o, _ = o.trace(rays, MODE_ANY)
+    anyh = ctx.trace_any(rays)
+    assert np.array_equal(anyh["instId"], anyh_o["instId"]) and np.array_equal(anyh["primId"], anyh_o["primId"])
+    assert np.array_equal(anyh["t"], anyh_o["t"])
+
+
+def test_config2_subsample_bit_exact_and_full_frame_orders_agree(ctx):
+    print("case: config 2, 1080p jittered rays", flush=True)
+    s, o = setup(ctx, 2)
+    rays = s.jittered_rays(0x5354524B00000002)
+    sub = rays[::37].copy()
+    ref, _ = o.trace(sub, MODE_PRUNED)
+    gpu = ctx.trace_closest(sub)
+    assert same(gpu, ref)
+    gpu_ref = ctx.trace_closest(sub[:8192], reference_order=True)
+    assert same(gpu_ref, ref[:8192])
+    # full 2,073,600 rays: the pruned near-first kernel and the skip-link
+    # kernel must agree bit for bit (order independence of the result rule)
+    full = ctx.trace_closest(rays)
+    full_ref = ctx.trace_closest(rays, reference_order=True)
+    assert same(full, full_ref)
+    assert np.all(full["instId"] != INVALID)
+
+
+def test_device_entry_points_and_camera_rays(ctx):
+    print("case: device buffers + camera ray kernel, config 1", flush=True)
+    s, o = setup(ctx, 1)
+    cam = s.camera
+    cam.stratifiedSamplingType = 1
+    cam.frameNumber = 3
+    cam.spp = 2
+    n = cam.dimension[0] * cam.dimension[1]
+    d_rays = ctx.alloc(n * 32)
+    d_hits = ctx.alloc(n * 20)
+    try:
+        ctx.camera_rays_device(cam, 1, d_rays)
+        rays = ctx.download(d_rays, np.zeros(n, dtype=sa.RAY))
+        # the oracle's generateCameraRay with the same RNG seed
+        import ctypes as C
+        from oracle import load
+        lib = load()
+        exp = np.zeros(1, dtype=sa.RAY)
+        for i in list(range(0, n, 997)) + [n - 1]:
+            px, py = i % cam.dimension[0], i // cam.dimension[0]
+            frame = cam.frameNumber * cam.spp + 1
+            st = C.c_uint32(lib.orc_init_rng(px, py, cam.dimension[0], frame))
+            lib.orc_camera_ray(C.byref(cam), px, py, C.byref(st), frame, C.c_void_p(exp.ctypes.data))
+            assert exp.tobytes() == rays[i:i + 1].tobytes(), f"pixel {px},{py}"
+        ctx.trace_closest_device(d_rays, n, d_hits)
+        ctx.sync()
+        hits = ctx.download(d_hits, np.zeros(n, dtype=sa.HIT))
+        ref, _ = o.trace(rays, MODE_REF)
+        assert same(hits, ref)
+    finally:
+        ctx.free(d_rays)
+        ctx.free(d_hits)
+
+
+def test_edge_cases(ctx):
+    print("case: edge cases (empty ray batch, tmax, axis-parallel, NaN rays, single triangle)", flush=True)
+    s, o = setup(ctx, 1)
+    assert ctx.trace_closest(np.zeros(0, dtype=sa.RAY)).size == 0
+    rays = s.jittered_rays(5)[:1000].copy()
+    rays["o"][:, 3] = 1e-3
+    h = ctx.trace_closest(rays)
+    assert np.all(h["instId"] == INVALID) and np.all(h["t"] == np.float32(1e9))
+    ax = np.zeros(8, dtype=sa.RAY)
+    for i, d in enumerate([(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (0, 0, 0),
+                           (np.nan, 0, 1)]):
+        ax[i]["o"] = (0.0, 0.0, 1.5, 1e9)
+        ax[i]["d"] = (*d, 0.0)
+    ref, _ = o.trace(ax, MODE_REF)
+    assert same(ctx.trace_closest(ax), ref)
+    assert same(ctx.trace_closest(ax, reference_order=True), ref)
+    # one-triangle scene: the root is a leaf
+    one = sa.SceneData(vb=s.vb, ib=s.ib, inst=s.inst[:1].copy(), lights=s.lights[:0], mats=s.mats, camera=s.camera,
+                       sky=s.sky)
+    one.inst["indexCount"][0] = 3
+    ctx.set_scene(one)
+    ctx.build_bvh()
+    o1 = Oracle.from_scene(one, ctx.get_bvh())
+    r = s.jittered_rays(9)
+    ref, _ = o1.trace(r, MODE_REF)
+    assert same(ctx.trace_closest(r), ref)
+    # empty scene: every ray misses
+    empty = sa.SceneData(vb=s.vb, ib=s.ib, inst=s.inst[:0].copy(), lights=s.lights[:0], mats=s.mats,
+                         camera=s.camera, sky=s.sky)
+    ctx.set_scene(empty)
+    ctx.build_bvh()
+    h = ctx.trace_closest(r[:4096])
+    assert np.all(h["instId"] == INVALID)
+    h = ctx.trace_any(r[:4096])
+    assert np.all(h["instId"] == INVALID)
+
+
+def test_errors_are_reported_not_thrown(ctx):
+    print("case: error paths", flush=True)
+    c = sa.Context(0)
+    with pytest.raises(sa.StkError) as e:
+        c.trace_closest(np.zeros(4, dtype=sa.RAY))
+    assert e.value.status == 4  # STK_E_STATE: no BVH yet
+    with pytest.raises(sa.StkError) as e:
+        c.build_bvh()
+    assert e.value.status == 4
+    s = sa.SceneData.generate(1)
+    bad = s.inst.copy()
+    bad["indexCount"][0] = 10 ** 8
+    with pytest.raises(sa.StkError) as e:
+        c.set_scene(sa.SceneData(vb=s.vb, ib=s.ib, inst=bad, lights=s.lights, mats=s.mats, camera=s.camera,
+                                 sky=s.sky))
+    assert e.value.status == 1
+    c.close()
