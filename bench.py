@@ -73,12 +73,32 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pt-frames", type=int, default=4)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run the plain copy kernel over this many MiB (calibrates memory counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         return 2
 
     import strelka_amd as sa
+
+    if args.copy_mib:
+        ctx = sa.Context(0)
+        nbytes = args.copy_mib << 20
+        src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+        for _ in range(args.warmup):
+            ctx.copy_device(dst, src, nbytes)
+        ctx.record(0)
+        for _ in range(args.steps):
+            ctx.copy_device(dst, src, nbytes)
+        ctx.record(1)
+        ms = ctx.elapsed_ms(0, 1) / args.steps
+        print(json.dumps({"copy_bytes": nbytes, "kernel_ms": round(ms, 4),
+                          "read_plus_write_GBs": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}))
+        ctx.free(src)
+        ctx.free(dst)
+        ctx.close()
+        return 0
 
     scene = sa.SceneData.generate(2)
     ntri = int(scene.inst["indexCount"].sum()) // 3

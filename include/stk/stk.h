@@ -76,7 +76,11 @@ typedef struct StkBvhHeader {
 } StkBvhHeader;
 
 /* trace flags */
-#define STK_TRACE_REFERENCE_ORDER 0x1u /* stackless skip-link walk, unpruned */
+/* Stackless skip-link walk in the reference's visit order, unpruned (closest)
+ * or stopping at the first leaf in preorder (any).  Without the flag the
+ * packed near-first (closest) or packed left-first (any) walks run; they
+ * return the same hit records. */
+#define STK_TRACE_REFERENCE_ORDER 0x1u
 
 /* ---- library / context ---------------------------------------------------- */
 STK_API const char* stk_version(void);
@@ -104,11 +108,12 @@ STK_API stk_status stk_get_bvh(stk_ctx* ctx, StkBvhNode* nodes, uint32_t capacit
 /* ---- traversal ------------------------------------------------------------ */
 /* Host buffers; synchronous. */
 STK_API stk_status stk_trace_closest(stk_ctx* ctx, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags);
-STK_API stk_status stk_trace_any(stk_ctx* ctx, const StkRay* rays, uint32_t n, StkHit* hits);
+STK_API stk_status stk_trace_any(stk_ctx* ctx, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags);
 /* Device buffers; enqueued on the context stream, asynchronous. */
 STK_API stk_status stk_trace_closest_device(stk_ctx* ctx, const StkRay* dRays, uint32_t n, StkHit* dHits,
                                             uint32_t flags);
-STK_API stk_status stk_trace_any_device(stk_ctx* ctx, const StkRay* dRays, uint32_t n, StkHit* dHits);
+STK_API stk_status stk_trace_any_device(stk_ctx* ctx, const StkRay* dRays, uint32_t n, StkHit* dHits,
+                                        uint32_t flags);
 /* Camera rays of generateCameraRay (shaders/pathtracerMdl.hlsl:165-202) for
  * every pixel, sample `sampleNum` of frame p->frameNumber, written in pixel
  * order into dRays (device, W*H rays).  Asynchronous. */
@@ -135,6 +140,9 @@ STK_API stk_status stk_device_alloc(stk_ctx* ctx, size_t bytes, void** out);
 STK_API stk_status stk_device_free(stk_ctx* ctx, void* ptr);
 STK_API stk_status stk_copy_to_device(stk_ctx* ctx, void* dst, const void* src, size_t bytes);
 STK_API stk_status stk_copy_to_host(stk_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy through a plain streaming kernel (bytes a multiple of
+ * 16); its known traffic calibrates the memory counters of a profiler. */
+STK_API stk_status stk_device_copy(stk_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* Records a HIP event on the context stream (slot 0..15). */
 STK_API stk_status stk_event_record(stk_ctx* ctx, int slot);
 /* Milliseconds between two recorded slots (waits for the second). */

@@ -212,10 +212,11 @@ class Context:
                                            1 if reference_order else 0), "stk_trace_closest")
         return hits
 
-    def trace_any(self, rays: np.ndarray) -> np.ndarray:
+    def trace_any(self, rays: np.ndarray, reference_order: bool = False) -> np.ndarray:
         rays = np.ascontiguousarray(rays, dtype=RAY)
         hits = np.zeros(rays.size, dtype=HIT)
-        self._check(_LIB.stk_trace_any(self._h, _ptr(rays), rays.size, _ptr(hits)), "stk_trace_any")
+        self._check(_LIB.stk_trace_any(self._h, _ptr(rays), rays.size, _ptr(hits), 1 if reference_order else 0),
+                    "stk_trace_any")
         return hits
 
     # -- device buffers, events ------------------------------------------------
@@ -234,13 +235,16 @@ class Context:
         self._check(_LIB.stk_copy_to_host(self._h, _ptr(out), C.c_void_p(src), out.nbytes), "stk_copy_to_host")
         return out
 
+    def copy_device(self, dst: int, src: int, nbytes: int):
+        self._check(_LIB.stk_device_copy(self._h, C.c_void_p(dst), C.c_void_p(src), nbytes), "stk_device_copy")
+
     def trace_closest_device(self, d_rays: int, n: int, d_hits: int, reference_order: bool = False):
         self._check(_LIB.stk_trace_closest_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits),
                                                   1 if reference_order else 0), "stk_trace_closest_device")
 
-    def trace_any_device(self, d_rays: int, n: int, d_hits: int):
-        self._check(_LIB.stk_trace_any_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits)),
-                    "stk_trace_any_device")
+    def trace_any_device(self, d_rays: int, n: int, d_hits: int, reference_order: bool = False):
+        self._check(_LIB.stk_trace_any_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits),
+                                              1 if reference_order else 0), "stk_trace_any_device")
 
     def camera_rays_device(self, camera: PathTracerParam, sample: int, d_rays: int):
         self._check(_LIB.stk_generate_camera_rays_device(self._h, C.byref(camera), sample, C.c_void_p(d_rays)),

@@ -90,9 +90,9 @@ PROTOTYPES = [
     ("stk_set_bvh", C.c_int, [P, P, U32]),
     ("stk_get_bvh", C.c_int, [P, P, U32, C.POINTER(U32)]),
     ("stk_trace_closest", C.c_int, [P, P, U32, P, U32]),
-    ("stk_trace_any", C.c_int, [P, P, U32, P]),
+    ("stk_trace_any", C.c_int, [P, P, U32, P, U32]),
     ("stk_trace_closest_device", C.c_int, [P, P, U32, P, U32]),
-    ("stk_trace_any_device", C.c_int, [P, P, U32, P]),
+    ("stk_trace_any_device", C.c_int, [P, P, U32, P, U32]),
     ("stk_generate_camera_rays_device", C.c_int, [P, C.POINTER(PathTracerParam), U32, P]),
     ("stk_render", C.c_int, [P, C.POINTER(RenderParams), P, C.c_int]),
     ("stk_reset_accumulation", C.c_int, [P]),
@@ -102,6 +102,7 @@ PROTOTYPES = [
     ("stk_device_free", C.c_int, [P, P]),
     ("stk_copy_to_device", C.c_int, [P, P, P, C.c_size_t]),
     ("stk_copy_to_host", C.c_int, [P, P, P, C.c_size_t]),
+    ("stk_device_copy", C.c_int, [P, P, P, C.c_size_t]),
     ("stk_event_record", C.c_int, [P, C.c_int]),
     ("stk_event_elapsed", C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     ("stk_bvh_build_host", C.c_int, [P, U32, P, U32, P, U32, C.POINTER(BvhOptions), C.POINTER(P)]),

@@ -319,13 +319,13 @@ STK_API stk_status stk_trace_closest_device(stk_ctx* c, const StkRay* rays, uint
     return STK_OK;
 }
 
-STK_API stk_status stk_trace_any_device(stk_ctx* c, const StkRay* rays, uint32_t n, StkHit* hits)
+STK_API stk_status stk_trace_any_device(stk_ctx* c, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags)
 {
     if (!validCtx(c)) return STK_E_INVALID;
     if (!c->haveBvh) return fail(c, STK_E_STATE, "stk_trace_any: no BVH");
     if (n && (!rays || !hits)) return fail(c, STK_E_INVALID, "stk_trace_any: null buffer");
     (void)hipSetDevice(c->device);
-    STK_HIP(c, stk::launchTraceAny(bvhView(c), rays, n, hits, c->stream));
+    STK_HIP(c, stk::launchTraceAny(bvhView(c), rays, n, hits, (flags & STK_TRACE_REFERENCE_ORDER) != 0, c->stream));
     return STK_OK;
 }
 
@@ -343,7 +343,7 @@ static stk_status traceHost(stk_ctx* c, const StkRay* rays, uint32_t n, StkHit* 
     StkRay* dr = static_cast<StkRay*>(c->rayScratch.ptr);
     StkHit* dh = static_cast<StkHit*>(c->hitScratch.ptr);
     if (any)
-        STK_HIP(c, stk::launchTraceAny(bvhView(c), dr, n, dh, c->stream));
+        STK_HIP(c, stk::launchTraceAny(bvhView(c), dr, n, dh, (flags & STK_TRACE_REFERENCE_ORDER) != 0, c->stream));
     else
         STK_HIP(c, stk::launchTraceClosest(bvhView(c), dr, n, dh, (flags & STK_TRACE_REFERENCE_ORDER) != 0,
                                            c->stream));
@@ -357,9 +357,9 @@ STK_API stk_status stk_trace_closest(stk_ctx* c, const StkRay* rays, uint32_t n,
     return traceHost(c, rays, n, hits, 0, flags);
 }
 
-STK_API stk_status stk_trace_any(stk_ctx* c, const StkRay* rays, uint32_t n, StkHit* hits)
+STK_API stk_status stk_trace_any(stk_ctx* c, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags)
 {
-    return traceHost(c, rays, n, hits, 1, 0);
+    return traceHost(c, rays, n, hits, 1, flags);
 }
 
 STK_API stk_status stk_generate_camera_rays_device(stk_ctx* c, const StkPathTracerParam* p, uint32_t sampleNum,
@@ -518,6 +518,14 @@ STK_API stk_status stk_copy_to_host(stk_ctx* c, void* dst, const void* src, size
     if (!validCtx(c) || (bytes && (!dst || !src))) return STK_E_INVALID;
     STK_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     STK_HIP(c, hipStreamSynchronize(c->stream));
+    return STK_OK;
+}
+
+STK_API stk_status stk_device_copy(stk_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    if (!validCtx(c) || (bytes && (!dst || !src)) || bytes % 16) return STK_E_INVALID;
+    (void)hipSetDevice(c->device);
+    STK_HIP(c, stk::launchCopy(dst, src, bytes, c->stream));
     return STK_OK;
 }
 

@@ -49,13 +49,16 @@ struct PtLaunch {
 
 hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, bool refOrder,
                               hipStream_t s);
-hipError_t launchTraceAny(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, hipStream_t s);
+hipError_t launchTraceAny(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, bool refOrder,
+                          hipStream_t s);
 hipError_t launchCameraRays(const StkPathTracerParam& p, uint32_t sampleNum, StkRay* rays, hipStream_t s);
 
 // One sample pass of the wavefront path tracer; accumulates invSpp * radiance
 // into accum (float4 per pixel) and optionally stores per-sample radiance.
 hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, PathState& ps, float4* accum,
                                float* samples, hipStream_t s);
+// Plain 16-B-per-lane copy kernel (bytes must be a multiple of 16).
+hipError_t launchCopy(void* dst, const void* src, size_t bytes, hipStream_t s);
 // Zeroes n float4 (a kernel, so the call can be captured in a graph).
 hipError_t launchZero(float4* p, size_t n, hipStream_t s);
 // Running-mean accumulation (shaders/accumulation.hlsl:86-97) + tonemap

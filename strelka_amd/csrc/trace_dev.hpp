@@ -249,6 +249,58 @@ __device__ __forceinline__ bool any_ref(const BvhView& bvh, const RayF& r, HitF&
     return false;
 }
 
+// anyHit with the reference's result - the first leaf in DFS preorder whose
+// triangle is hit with t < tmax - over the packed nodes: children are visited
+// left first and a leaf right child waits on the stack while its inner left
+// sibling's subtree is walked, so leaves are tested in preorder and the walk
+// stops at the first hit, two boxes per 64-B fetch.
+__device__ __forceinline__ bool any_packed(const BvhView& bvh, const RayF& r, uint32_t* stack, int stride, HitF& h)
+{
+    miss(h);
+    const StkBvhHeader& hd = bvh.hdr;
+    if (hd.root == INT32_MIN) return false;
+    int next = hd.root;
+    if (hd.root >= 0) {
+        float t0;
+        if (!(slab(r, hd.lo[0], hd.hi[0], hd.lo[1], hd.hi[1], hd.lo[2], hd.hi[2], t0) && t0 <= r.tmax)) return false;
+    }
+    int sp = 0;
+    for (;;) {
+        // resolve leaves in DFS order until an inner node is next
+        while (next < 0) {
+            const uint32_t tri = (uint32_t)~next;
+            const float4* p = bvh.tris + 3 * (size_t)tri;
+            const float4 a = p[0], b = p[1], c = p[2];
+            float t, u, v;
+            if (tri_test(r, a, b, c, t, u, v) && t < r.tmax) {
+                h.t = t;
+                h.leaf = float_to_bits(a.w);
+                h.inst = float_to_bits(b.w);
+                h.prim = float_to_bits(c.w);
+                return true;
+            }
+            if (sp == 0) return false;
+            next = (int)stack[--sp * stride];
+        }
+        const float4* p = bvh.packed + 4 * (size_t)next;
+        const float4 bx = p[0], by = p[1], bz = p[2];
+        const int4 ch = *reinterpret_cast<const int4*>(p + 3);
+        const int c0 = ch.x, c1 = ch.y;
+        float ta, tb;
+        const bool ok0 = c0 < 0 || (slab(r, bx.x, bx.y, by.x, by.y, bz.x, bz.y, ta) && ta <= r.tmax);
+        const bool ok1 = c1 < 0 || (slab(r, bx.z, bx.w, by.z, by.w, bz.z, bz.w, tb) && tb <= r.tmax);
+        if (ok0) {
+            if (ok1) stack[sp++ * stride] = (uint32_t)c1;
+            next = c0;
+        } else if (ok1) {
+            next = c1;
+        } else {
+            if (sp == 0) return false;
+            next = (int)stack[--sp * stride];
+        }
+    }
+}
+
 // XCD-aware block order: hardware deals workgroups round-robin over the 8
 // XCDs; remap so XCD k owns a contiguous range of logical blocks (and so a
 // contiguous range of rays, which share BVH nodes in that XCD's L2).

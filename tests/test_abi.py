@@ -60,6 +60,7 @@ def test_null_arguments_are_rejected_without_a_device():
     assert lib.stk_sync(None) == 1
     assert lib.stk_set_geometry(None, None, 0, None, 0, None, 0) == 1
     assert lib.stk_trace_closest(None, None, 0, None, 0) == 1
+    assert lib.stk_trace_any(None, None, 0, None, 0) == 1
     assert lib.stk_last_error(None) == b"null context"
     assert lib.stk_scene_generate(99, 0, 0, C.byref(C.c_void_p())) == 1
     assert lib.stk_bvh_build_host(None, 0, None, 0, None, 0, None, None) == 1
