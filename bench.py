@@ -66,6 +66,80 @@ def load_traffic(kernel: str):
     return None
 
 
+def cache_study(args) -> dict:
+    """Time to reach an RMSE target against a converged plain render, for the
+    plain path tracer and the cached one (training included in its time).
+    Config-2 scene at the study resolution, 1 spp per frame, 8 bounces,
+    RR after 3, cache queried after 2 bounces, ACES-tonemapped RMSE."""
+    import strelka_amd as sa
+
+    scene = sa.SceneData.generate(2)
+    W, H = args.study_res
+    cam = scene.camera
+    cam.dimension[0], cam.dimension[1] = W, H
+    cam.invDimension[0], cam.invDimension[1] = 1.0 / W, 1.0 / H
+    ctx = sa.Context(0)
+    ctx.set_scene(scene)
+    ctx.build_bvh()
+    npix = W * H
+    train_paths = max(1024, int(65536 * npix / (1920 * 1080)))
+    d_out = ctx.alloc(npix * 16)
+    out = np.zeros((H, W, 4), np.float32)
+
+    def params(frame, mode):
+        return sa.render_params(cam, spp=1, max_depth=8, frame=frame, sampling=1, sky=scene.sky, tonemap=1,
+                                cache_mode=mode, cache_query_depth=2, cache_train_paths=train_paths)
+
+    ctx.reset_accumulation()
+    t0 = time.perf_counter()
+    for f in range(args.study_ref_spp):
+        ctx.render_device(params(1_000_000 + f, 0), d_out)
+        if f % 64 == 63:
+            ctx.sync()
+            print(f"reference {f + 1}/{args.study_ref_spp} spp", file=sys.stderr, flush=True)
+    ctx.sync()
+    ref_s = time.perf_counter() - t0
+    ref_tm = ctx.download(d_out, out.copy())[..., :3]
+    ref_lin = ctx.accumulation(W, H)[..., :3]
+
+    checkpoints = sorted({1, 2, 4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512} | {args.study_frames})
+
+    def run(mode):
+        ctx.reset_accumulation()
+        curve, t_ms = [], 0.0
+        for f in range(args.study_frames):
+            ctx.record(0)
+            ctx.render_device(params(f, mode), d_out)
+            ctx.record(1)
+            t_ms += ctx.elapsed_ms(0, 1)
+            if f + 1 in checkpoints:
+                tm = ctx.download(d_out, out.copy())[..., :3]
+                lin = ctx.accumulation(W, H)[..., :3]
+                curve.append({"frames": f + 1, "ms": round(t_ms, 2),
+                              "rmse": round(float(np.sqrt(np.mean((tm - ref_tm) ** 2))), 5),
+                              "rmse_linear": round(float(np.sqrt(np.mean((lin - ref_lin) ** 2))), 5)})
+                print(f"mode {mode} frame {f + 1}: {curve[-1]}", file=sys.stderr, flush=True)
+        return curve
+
+    plain = run(0)
+    ctx.cache_create(log2_table=19, base_res=16, max_res=2048, lr=args.study_lr, ema=args.study_ema)
+    cached = run(1)
+    ctx.free(d_out)
+    ctx.close()
+
+    def time_to(curve, target):
+        for p in curve:
+            if p["rmse"] <= target:
+                return p["ms"] / 1e3
+        return None
+
+    tp, tc = time_to(plain, args.study_target), time_to(cached, args.study_target)
+    return {"study": "cache", "resolution": [W, H], "target_rmse_tonemapped": args.study_target,
+            "reference_spp": args.study_ref_spp, "reference_s": round(ref_s, 2), "train_paths_per_frame": train_paths,
+            "time_to_target_s": {"plain": tp, "cached": tc},
+            "speedup": (tp / tc) if (tp and tc) else None, "plain": plain, "cached": cached}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,12 +149,24 @@ def main() -> int:
     ap.add_argument("--pt-frames", type=int, default=4)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (calibrates memory counters)")
+    ap.add_argument("--cache-study", action="store_true",
+                    help="only run the time-to-RMSE study of the cached vs plain path tracer")
+    ap.add_argument("--study-res", type=int, nargs=2, default=(960, 540))
+    ap.add_argument("--study-frames", type=int, default=256)
+    ap.add_argument("--study-ref-spp", type=int, default=2048)
+    ap.add_argument("--study-target", type=float, default=0.01)
+    ap.add_argument("--study-lr", type=float, default=1e-2)
+    ap.add_argument("--study-ema", type=float, default=0.99)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         return 2
 
     import strelka_amd as sa
+
+    if args.cache_study:
+        print(json.dumps(cache_study(args)))
+        return 0
 
     if args.copy_mib:
         ctx = sa.Context(0)

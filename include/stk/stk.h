@@ -134,6 +134,59 @@ STK_API stk_status stk_reset_accumulation(stk_ctx* ctx);
 STK_API stk_status stk_get_samples(stk_ctx* ctx, float* out, size_t count);
 /* The accumulated (not tonemapped) RGBA32F image. */
 STK_API stk_status stk_get_accumulation(stk_ctx* ctx, float* out, size_t count);
+/* Queue counters of the last wavefront pass (host copy of 8 words):
+ * [0],[1] path queues of the last two bounces, [2] shadow rays of the last
+ * bounce, [3] cache queries, [4] cache training records produced. */
+STK_API stk_status stk_get_counters(stk_ctx* ctx, uint32_t out[8]);
+
+/* ---- radiance cache -------------------------------------------------------
+ * No counterpart in the reference renderer: an online-trained cache of
+ * outgoing radiance (multi-resolution hash grid of 16 levels x 2 FP16
+ * features feeding a 64-wide MLP with 4 ReLU hidden layers on MFMA, FP16
+ * weights/activations, FP32 accumulation, Adam).  The plain path tracer
+ * (cacheMode 0) stays available as the fallback and the judge. */
+typedef struct StkCacheConfig {
+    uint32_t log2TableSize;  /* hash-table entries per level: 2^19 */
+    uint32_t baseResolution; /* coarsest grid resolution: 16 */
+    uint32_t maxResolution;  /* finest grid resolution: 2048 */
+    uint32_t seed;           /* weight initialisation */
+    float learningRate;      /* Adam: 1e-2 */
+    float beta1, beta2, epsilon;
+    uint32_t maxBatch;       /* capacity of one training step, in records */
+    float emaDecay;          /* queries read an exponential moving average of the
+                                weights with this decay (0 = the raw weights) */
+    uint32_t pad[2];
+} StkCacheConfig;
+
+typedef struct StkCacheInfo {
+    float lo[3];
+    float scale;              /* normalised position = (p - lo) * scale */
+    uint32_t resolution[16];
+    uint32_t dense[16];       /* 1 if the level is indexed densely */
+    uint32_t log2TableSize;
+    uint32_t gridParams;      /* 16 * 2^log2TableSize * 2 */
+    uint32_t mlpParams;       /* 4 * 64 * 64 + 64 * 16 */
+    uint32_t step;            /* Adam steps taken */
+} StkCacheInfo;
+
+/* Creates (or re-creates) the cache of a context; the scene bounds come from
+ * the BVH, so a BVH must be set.  cfg may be NULL for the defaults above. */
+STK_API stk_status stk_cache_create(stk_ctx* ctx, const StkCacheConfig* cfg);
+STK_API stk_status stk_cache_destroy(stk_ctx* ctx);
+STK_API stk_status stk_cache_info(stk_ctx* ctx, StkCacheInfo* out);
+/* Predicted radiance (float4 rgb + 0 per record), device buffers, async. */
+STK_API stk_status stk_cache_infer(stk_ctx* ctx, const StkCacheRecord* dRecs, uint32_t n, float* dOut);
+/* One training step (backward + Adam) on n device records; lossOut (host,
+ * may be NULL) receives the mean relative-L2 loss.  Synchronous. */
+STK_API stk_status stk_cache_train_step(stk_ctx* ctx, const StkCacheRecord* dRecs, uint32_t n, float* lossOut);
+/* Gradients of the summed relative-L2 loss for n device records, copied to
+ * host arrays (grid: gridParams floats, mlp: mlpParams floats, W[l][in][out]);
+ * parameters are left unchanged.  Synchronous. */
+STK_API stk_status stk_cache_gradients(stk_ctx* ctx, const StkCacheRecord* dRecs, uint32_t n, float* gridGrad,
+                                       float* mlpGrad, float* lossSum);
+/* FP32 master parameters in/out (host arrays of gridParams / mlpParams). */
+STK_API stk_status stk_cache_get_params(stk_ctx* ctx, float* grid, float* mlp);
+STK_API stk_status stk_cache_set_params(stk_ctx* ctx, const float* grid, const float* mlp);
 
 /* ---- device memory and timing helpers for callers without a HIP binding --- */
 STK_API stk_status stk_device_alloc(stk_ctx* ctx, size_t bytes, void** out);

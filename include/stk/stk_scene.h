@@ -19,6 +19,8 @@
  *   config 3: one of the 32 uneven scenes of the batched launch (variant
  *             0..31), built as config 2 with ~N_i triangles, 960x540.
  *   config 6: a tiny scene (heightfield 16x16, 8 spheres) for smoke tests.
+ *   config 7: the config-1 room closed and lit by a 1x1 ceiling light
+ *             (walls albedo 0.7), 128x128: long multi-bounce paths.
  */
 #ifndef STK_SCENE_H
 #define STK_SCENE_H

@@ -151,6 +151,20 @@ typedef struct StkPackedNode {
     uint32_t pad[2];
 } StkPackedNode;
 
+/* One radiance-cache sample, 64 B: the shading point, its shading normal, the
+ * outgoing direction, the roughness and (for training) the radiance target.
+ * payload is free for the caller (the renderer stores the path slot there). */
+typedef struct StkCacheRecord {
+    float position[3];
+    float roughness;
+    float normal[3];
+    uint32_t payload;
+    float direction[3];
+    float pad0;
+    float target[3];
+    float pad1;
+} StkCacheRecord;
+
 /* Parameters of one render call (progressive; one call adds `spp` samples). */
 typedef struct StkRenderParams {
     StkPathTracerParam pt;   /* camera matrices, dimension, spp, maxDepth ... */
@@ -161,7 +175,12 @@ typedef struct StkRenderParams {
     float skyColor[4];       /* constant miss radiance (reference adds 0) */
     uint32_t enableTonemap;
     uint32_t keepSamples;    /* keep per-sample radiance for stk_get_samples */
-    uint32_t pad[2];
+    /* radiance cache (needs stk_cache_create): 0 off (plain path tracer),
+     * 1 query and train, 2 query only (frozen cache), 3 train only */
+    uint32_t cacheMode;
+    uint32_t cacheQueryDepth; /* vertex depth at which paths query the cache */
+    uint32_t cacheTrainPaths; /* training paths per frame (extended to maxDepth) */
+    uint32_t pad[3];
 } StkRenderParams;
 
 #ifdef __cplusplus
@@ -192,5 +211,7 @@ STK_STATIC_ASSERT(offsetof(StkPathTracerParam, len) == 304, "PathTracerParam.len
 STK_STATIC_ASSERT(sizeof(StkMaterial) == 32, "Material must be 32 B");
 STK_STATIC_ASSERT(sizeof(StkTriangle) == 48, "Triangle must be 48 B");
 STK_STATIC_ASSERT(sizeof(StkPackedNode) == 64, "PackedNode must be 64 B");
+STK_STATIC_ASSERT(sizeof(StkCacheRecord) == 64, "CacheRecord must be 64 B");
+STK_STATIC_ASSERT(sizeof(StkRenderParams) == 336 + 64, "RenderParams must be 400 B");
 
 #endif /* STK_TYPES_H */

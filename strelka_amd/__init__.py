@@ -14,12 +14,12 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _abi
-from ._abi import (BVH_NODE, HIT, INSTANCE, INVALID, LIGHT, MATERIAL, PACKED_NODE, RAY, TRIANGLE, VERTEX,
-                   BvhHeader, BvhOptions, PathTracerParam, RenderParams, SceneArrays)
+from ._abi import (BVH_NODE, CACHE_RECORD, HIT, INSTANCE, INVALID, LIGHT, MATERIAL, PACKED_NODE, RAY, TRIANGLE,
+                   VERTEX, BvhHeader, BvhOptions, CacheConfig, CacheInfo, PathTracerParam, RenderParams, SceneArrays)
 
 __all__ = ["lib", "StkError", "SceneData", "HostBvh", "Context", "RenderParams", "PathTracerParam", "VERTEX",
            "BVH_NODE", "INSTANCE", "LIGHT", "RAY", "HIT", "MATERIAL", "TRIANGLE", "PACKED_NODE", "INVALID",
-           "render_params"]
+           "render_params", "CACHE_RECORD"]
 
 _LIB = _abi.load()
 
@@ -131,7 +131,8 @@ class HostBvh:
 
 def render_params(camera: PathTracerParam, spp: int = 1, max_depth: int = 8, frame: int = 0,
                   sampling: int = 1, sky=(0.0, 0.0, 0.0), tonemap: int = 1, accumulate: bool = True,
-                  keep_samples: bool = False, rr_start: int = 3, enable_tonemap: bool = True) -> RenderParams:
+                  keep_samples: bool = False, rr_start: int = 3, enable_tonemap: bool = True,
+                  cache_mode: int = 0, cache_query_depth: int = 2, cache_train_paths: int = 65536) -> RenderParams:
     p = RenderParams()
     C.memmove(C.byref(p.pt), C.byref(camera), C.sizeof(PathTracerParam))
     p.pt.spp = spp
@@ -144,6 +145,9 @@ def render_params(camera: PathTracerParam, spp: int = 1, max_depth: int = 8, fra
     p.enableAccumulation = 1 if accumulate else 0
     p.enableTonemap = 1 if enable_tonemap else 0
     p.keepSamples = 1 if keep_samples else 0
+    p.cacheMode = cache_mode
+    p.cacheQueryDepth = cache_query_depth
+    p.cacheTrainPaths = cache_train_paths
     for i in range(3):
         p.skyColor[i] = float(sky[i])
     p.skyColor[3] = 0.0
@@ -284,3 +288,72 @@ class Context:
         out = np.zeros((h, w, 4), dtype=np.float32)
         self._check(_LIB.stk_get_accumulation(self._h, _ptr(out), out.size), "stk_get_accumulation")
         return out
+
+    def counters(self) -> np.ndarray:
+        out = np.zeros(8, np.uint32)
+        self._check(_LIB.stk_get_counters(self._h, _ptr(out)), "stk_get_counters")
+        return out
+
+    # -- radiance cache ----------------------------------------------------------
+    def cache_create(self, log2_table: int = 19, base_res: int = 16, max_res: int = 2048, seed: int = 1,
+                     lr: float = 1e-2, beta1: float = 0.9, beta2: float = 0.99, eps: float = 1e-15,
+                     max_batch: int = 1 << 18, ema: float = 0.99):
+        cfg = CacheConfig(log2_table, base_res, max_res, seed, lr, beta1, beta2, eps, max_batch, ema)
+        self._check(_LIB.stk_cache_create(self._h, C.byref(cfg)), "stk_cache_create")
+
+    def cache_info(self) -> CacheInfo:
+        info = CacheInfo()
+        self._check(_LIB.stk_cache_info(self._h, C.byref(info)), "stk_cache_info")
+        return info
+
+    def _upload_records(self, recs: np.ndarray) -> int:
+        recs = np.ascontiguousarray(recs, dtype=CACHE_RECORD)
+        d = self.alloc(max(1, recs.nbytes))
+        self.upload(d, recs)
+        return d
+
+    def cache_infer(self, recs: np.ndarray) -> np.ndarray:
+        d = self._upload_records(recs)
+        o = self.alloc(max(1, recs.size * 16))
+        try:
+            self._check(_LIB.stk_cache_infer(self._h, C.c_void_p(d), recs.size, C.c_void_p(o)), "stk_cache_infer")
+            self.sync()
+            return self.download(o, np.zeros((recs.size, 4), np.float32))
+        finally:
+            self.free(d)
+            self.free(o)
+
+    def cache_train_step(self, recs: np.ndarray) -> float:
+        d = self._upload_records(recs)
+        loss = C.c_float()
+        try:
+            self._check(_LIB.stk_cache_train_step(self._h, C.c_void_p(d), recs.size, C.byref(loss)),
+                        "stk_cache_train_step")
+        finally:
+            self.free(d)
+        return float(loss.value)
+
+    def cache_gradients(self, recs: np.ndarray):
+        info = self.cache_info()
+        g = np.zeros(info.gridParams, np.float32)
+        m = np.zeros(info.mlpParams, np.float32)
+        loss = C.c_float()
+        d = self._upload_records(recs)
+        try:
+            self._check(_LIB.stk_cache_gradients(self._h, C.c_void_p(d), recs.size, _ptr(g), _ptr(m), C.byref(loss)),
+                        "stk_cache_gradients")
+        finally:
+            self.free(d)
+        return g, m, float(loss.value)
+
+    def cache_get_params(self):
+        info = self.cache_info()
+        g = np.zeros(info.gridParams, np.float32)
+        m = np.zeros(info.mlpParams, np.float32)
+        self._check(_LIB.stk_cache_get_params(self._h, _ptr(g), _ptr(m)), "stk_cache_get_params")
+        return g, m
+
+    def cache_set_params(self, grid: np.ndarray | None, mlp: np.ndarray | None):
+        grid = None if grid is None else np.ascontiguousarray(grid, np.float32)
+        mlp = None if mlp is None else np.ascontiguousarray(mlp, np.float32)
+        self._check(_LIB.stk_cache_set_params(self._h, _ptr(grid), _ptr(mlp)), "stk_cache_set_params")

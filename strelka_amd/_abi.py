@@ -47,7 +47,25 @@ class PathTracerParam(C.Structure):
 class RenderParams(C.Structure):
     _fields_ = [("pt", PathTracerParam), ("rrStartDepth", C.c_uint32), ("refCompat", C.c_uint32),
                 ("tonemapperType", C.c_uint32), ("enableAccumulation", C.c_uint32), ("skyColor", C.c_float * 4),
-                ("enableTonemap", C.c_uint32), ("keepSamples", C.c_uint32), ("pad", C.c_uint32 * 2)]
+                ("enableTonemap", C.c_uint32), ("keepSamples", C.c_uint32), ("cacheMode", C.c_uint32),
+                ("cacheQueryDepth", C.c_uint32), ("cacheTrainPaths", C.c_uint32), ("pad", C.c_uint32 * 3)]
+
+
+class CacheConfig(C.Structure):
+    _fields_ = [("log2TableSize", C.c_uint32), ("baseResolution", C.c_uint32), ("maxResolution", C.c_uint32),
+                ("seed", C.c_uint32), ("learningRate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("epsilon", C.c_float), ("maxBatch", C.c_uint32), ("emaDecay", C.c_float), ("pad", C.c_uint32 * 2)]
+
+
+class CacheInfo(C.Structure):
+    _fields_ = [("lo", C.c_float * 3), ("scale", C.c_float), ("resolution", C.c_uint32 * 16),
+                ("dense", C.c_uint32 * 16), ("log2TableSize", C.c_uint32), ("gridParams", C.c_uint32),
+                ("mlpParams", C.c_uint32), ("step", C.c_uint32)]
+
+
+CACHE_RECORD = np.dtype([("position", "<f4", 3), ("roughness", "<f4"), ("normal", "<f4", 3), ("payload", "<u4"),
+                         ("direction", "<f4", 3), ("pad0", "<f4"), ("target", "<f4", 3), ("pad1", "<f4")])
+assert CACHE_RECORD.itemsize == 64
 
 
 class BvhOptions(C.Structure):
@@ -68,7 +86,8 @@ class SceneArrays(C.Structure):
 
 
 assert C.sizeof(PathTracerParam) == 336
-assert C.sizeof(RenderParams) == 336 + 48
+assert C.sizeof(RenderParams) == 336 + 64
+assert C.sizeof(CacheConfig) == 48
 assert C.sizeof(BvhHeader) == 32
 
 STATUS = {0: "STK_OK", 1: "STK_E_INVALID", 2: "STK_E_OOM", 3: "STK_E_DEVICE", 4: "STK_E_STATE", 5: "STK_E_TIMEOUT"}
@@ -98,6 +117,15 @@ PROTOTYPES = [
     ("stk_reset_accumulation", C.c_int, [P]),
     ("stk_get_samples", C.c_int, [P, P, C.c_size_t]),
     ("stk_get_accumulation", C.c_int, [P, P, C.c_size_t]),
+    ("stk_get_counters", C.c_int, [P, P]),
+    ("stk_cache_create", C.c_int, [P, C.POINTER(CacheConfig)]),
+    ("stk_cache_destroy", C.c_int, [P]),
+    ("stk_cache_info", C.c_int, [P, C.POINTER(CacheInfo)]),
+    ("stk_cache_infer", C.c_int, [P, P, U32, P]),
+    ("stk_cache_train_step", C.c_int, [P, P, U32, C.POINTER(C.c_float)]),
+    ("stk_cache_gradients", C.c_int, [P, P, U32, P, P, C.POINTER(C.c_float)]),
+    ("stk_cache_get_params", C.c_int, [P, P, P]),
+    ("stk_cache_set_params", C.c_int, [P, P, P]),
     ("stk_device_alloc", C.c_int, [P, C.c_size_t, C.POINTER(P)]),
     ("stk_device_free", C.c_int, [P, P]),
     ("stk_copy_to_device", C.c_int, [P, P, P, C.c_size_t]),

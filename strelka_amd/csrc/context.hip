@@ -10,8 +10,11 @@
 #include <string>
 #include <vector>
 
+#include <cmath>
+
 #include "../../include/stk/stk.h"
 #include "bvh.hpp"
+#include "cache.hpp"
 #include "kernels.hpp"
 
 namespace {
@@ -55,10 +58,20 @@ struct stk_ctx {
     uint32_t width = 0, height = 0, iteration = 0;
     size_t samplesCount = 0;
 
+    // radiance cache
+    stk::CacheBuffers cache;
+    bool haveCache = false;
+    StkCacheConfig cacheCfg{};
+    DevBuf cGridH, cGridP, cGridM, cGridV, cGridG, cMlpP, cMlpM, cMlpV, cMlpG, cFwd, cBwd, cXT, cDyT, cLoss;
+    DevBuf cGridE, cGridHE, cMlpE, cFwdE, cBwdE;
+    DevBuf queryRecs, queryOut, trainVerts, trainCount, trainBatch;
+
     std::vector<DevBuf*> owned()
     {
         return {&vb, &ib, &inst, &lights, &mats, &packed, &tris, &refNodes, &rayScratch, &hitScratch,
-                &accum, &history, &out, &samples};
+                &accum, &history, &out, &samples, &cGridH, &cGridP, &cGridM, &cGridV, &cGridG, &cMlpP,
+                &cMlpM, &cMlpV, &cMlpG, &cFwd, &cBwd, &cXT, &cDyT, &cLoss, &cGridE, &cGridHE, &cMlpE,
+                &cFwdE, &cBwdE, &queryRecs, &queryOut, &trainVerts, &trainCount, &trainBatch};
     }
 };
 
@@ -440,12 +453,44 @@ STK_API stk_status stk_render(stk_ctx* c, const StkRenderParams* p, float* outRg
     lp.rrStartDepth = p->rrStartDepth;
     lp.refCompat = p->refCompat;
     lp.invSpp = 1.0f / (float)p->pt.spp;
+    stk::PtCache& pc = lp.cache;
+    pc = stk::PtCache{};
+    if (p->cacheMode != 0) {
+        if (!c->haveCache) return fail(c, STK_E_STATE, "stk_render: cacheMode set but no cache (stk_cache_create)");
+        if (p->cacheMode > 3) return fail(c, STK_E_INVALID, "stk_render: cacheMode must be 0..3");
+        pc.mode = p->cacheMode;
+        pc.queryDepth = p->cacheQueryDepth;
+        const uint32_t paths = p->cacheTrainPaths ? p->cacheTrainPaths : 65536u;
+        pc.trainStride = std::max<uint32_t>(1u, (uint32_t)(npix / paths));
+        const size_t maxTrain = (npix + pc.trainStride - 1) / pc.trainStride;
+        pc.trainCap = (uint32_t)std::min<size_t>(c->cache.nCap, maxTrain * p->pt.maxDepth);
+        if ((st = devReserve(c, c->queryRecs, npix * sizeof(stk::CacheRecord))) != STK_OK) return st;
+        if ((st = devReserve(c, c->queryOut, npix * sizeof(float4))) != STK_OK) return st;
+        if ((st = devReserve(c, c->trainVerts, maxTrain * p->pt.maxDepth * 5 * sizeof(float4))) != STK_OK) return st;
+        if ((st = devReserve(c, c->trainCount, maxTrain * sizeof(uint32_t))) != STK_OK) return st;
+        if ((st = devReserve(c, c->trainBatch, (size_t)pc.trainCap * sizeof(stk::CacheRecord))) != STK_OK) return st;
+        pc.queryRecs = static_cast<stk::CacheRecord*>(c->queryRecs.ptr);
+        pc.queryOut = static_cast<float4*>(c->queryOut.ptr);
+        pc.trainVerts = static_cast<float4*>(c->trainVerts.ptr);
+        pc.trainCount = static_cast<uint32_t*>(c->trainCount.ptr);
+        pc.trainBatch = static_cast<stk::CacheRecord*>(c->trainBatch.ptr);
+    }
     float4* accum = static_cast<float4*>(c->accum.ptr);
     STK_HIP(c, stk::launchZero(accum, npix, c->stream));
     float* samples = nsamples ? static_cast<float*>(c->samples.ptr) : nullptr;
     for (int s = 0; s < p->pt.spp; ++s) {
         lp.sampleNum = (uint32_t)s;
-        STK_HIP(c, stk::launchPathTracePass(sv, lp, c->ps, accum, samples, c->stream));
+        if (pc.mode != 0) {
+            // a different subset of pixels trains each pass
+            uint32_t h = p->pt.frameNumber * (uint32_t)p->pt.spp + (uint32_t)s;
+            h ^= h >> 16;
+            h *= 0x7feb352du;
+            h ^= h >> 15;
+            pc.trainOffset = h % pc.trainStride;
+            pc.numTrainPaths = (uint32_t)((npix - pc.trainOffset + pc.trainStride - 1) / pc.trainStride);
+        }
+        STK_HIP(c, stk::launchPathTracePass(sv, lp, c->ps, accum, samples, c->haveCache ? &c->cache : nullptr,
+                                            c->stream));
     }
     const int acc = p->enableAccumulation != 0;
     STK_HIP(c, stk::launchAccumulateTonemap(accum, static_cast<float4*>(c->history.ptr),
@@ -484,6 +529,252 @@ STK_API stk_status stk_get_accumulation(stk_ctx* c, float* out, size_t count)
     if (count > (size_t)c->width * c->height * 4 || !c->history.ptr)
         return fail(c, STK_E_STATE, "stk_get_accumulation: nothing rendered or count too large");
     STK_HIP(c, hipMemcpyAsync(out, c->history.ptr, count * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    STK_HIP(c, hipStreamSynchronize(c->stream));
+    return STK_OK;
+}
+
+// ---- radiance cache -----------------------------------------------------------
+
+static StkCacheConfig defaultCacheConfig()
+{
+    StkCacheConfig c{};
+    c.log2TableSize = 19;
+    c.baseResolution = 16;
+    c.maxResolution = 2048;
+    c.seed = 1;
+    c.learningRate = 1e-2f;
+    c.beta1 = 0.9f;
+    c.beta2 = 0.99f;
+    c.epsilon = 1e-15f;
+    c.maxBatch = 1u << 18;
+    c.emaDecay = 0.99f;
+    return c;
+}
+
+STK_API stk_status stk_cache_destroy(stk_ctx* c)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    DevBuf* bufs[] = {&c->cGridH, &c->cGridP, &c->cGridM, &c->cGridV, &c->cGridG, &c->cMlpP, &c->cMlpM,
+                      &c->cMlpV, &c->cMlpG, &c->cFwd, &c->cBwd, &c->cXT, &c->cDyT, &c->cLoss,
+                      &c->cGridE, &c->cGridHE, &c->cMlpE, &c->cFwdE, &c->cBwdE};
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (DevBuf* b : bufs) devRelease(c, *b);
+    c->cache = stk::CacheBuffers{};
+    c->haveCache = false;
+    return STK_OK;
+}
+
+STK_API stk_status stk_cache_create(stk_ctx* c, const StkCacheConfig* cfgIn)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    if (!c->haveBvh) return fail(c, STK_E_STATE, "stk_cache_create: the scene bounds come from the BVH; set one first");
+    StkCacheConfig cfg = cfgIn ? *cfgIn : defaultCacheConfig();
+    if (cfg.log2TableSize < 8 || cfg.log2TableSize > 24 || cfg.baseResolution < 2 ||
+        cfg.maxResolution < cfg.baseResolution || cfg.maxResolution > (1u << 20) || cfg.maxBatch == 0 ||
+        cfg.maxBatch > (1u << 24) || !(cfg.learningRate > 0.0f) || !(cfg.emaDecay >= 0.0f && cfg.emaDecay < 1.0f))
+        return fail(c, STK_E_INVALID, "stk_cache_create: configuration out of range");
+    stk_cache_destroy(c);
+    stk::CacheBuffers& cb = c->cache;
+    stk::GridDesc& gd = cb.grid;
+    const StkBvhHeader& h = c->bvh.header;
+    float ext = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        gd.lo[a] = (h.root >= 0) ? h.lo[a] : 0.0f;
+        if (h.root >= 0) ext = std::max(ext, h.hi[a] - h.lo[a]);
+    }
+    gd.scale = ext > 0.0f ? 1.0f / ext : 1.0f;
+    const uint64_t T = 1ull << cfg.log2TableSize;
+    gd.tableMask = (uint32_t)(T - 1);
+    gd.log2Table = cfg.log2TableSize;
+    const double growth = std::exp((std::log((double)cfg.maxResolution) - std::log((double)cfg.baseResolution)) /
+                                   (double)(stk::kCacheLevels - 1));
+    for (int l = 0; l < stk::kCacheLevels; ++l) {
+        const uint32_t res = (uint32_t)std::floor((double)cfg.baseResolution * std::pow(growth, (double)l) + 1e-9);
+        gd.res[l] = res;
+        const uint64_t s = (uint64_t)res + 1;
+        gd.dense[l] = (s * s * s <= T) ? 1u : 0u;
+    }
+    cb.gridParams = (size_t)stk::kCacheLevels * T * stk::kCacheFeatures;
+    cb.nCap = (cfg.maxBatch + 1023u) & ~1023u;
+    cb.lr = cfg.learningRate;
+    cb.beta1 = cfg.beta1;
+    cb.beta2 = cfg.beta2;
+    cb.eps = cfg.epsilon;
+    cb.step = 0;
+    const size_t gp = cb.gridParams, mp = stk::kCacheMlpParams;
+    struct {
+        DevBuf* b;
+        size_t bytes;
+    } allocs[] = {{&c->cGridH, gp * 2},  {&c->cGridP, gp * 4}, {&c->cGridM, gp * 4},   {&c->cGridV, gp * 4},
+                  {&c->cGridG, gp * 4},  {&c->cMlpP, mp * 4},  {&c->cMlpM, mp * 4},    {&c->cMlpV, mp * 4},
+                  {&c->cMlpG, mp * 4},   {&c->cFwd, 68 * 256 * 2}, {&c->cBwd, 68 * 256 * 2},
+                  {&c->cXT, (size_t)5 * 64 * cb.nCap * 2}, {&c->cDyT, (size_t)272 * cb.nCap * 2},
+                  {&c->cLoss, 64}};
+    (void)hipSetDevice(c->device);
+    for (auto& a : allocs) {
+        stk_status st = devReserve(c, *a.b, a.bytes);
+        if (st != STK_OK) {
+            stk_cache_destroy(c);
+            return st;
+        }
+    }
+    cb.emaDecay = cfg.emaDecay;
+    if (cb.emaDecay > 0.0f) {
+        struct {
+            DevBuf* b;
+            size_t bytes;
+        } ema[] = {{&c->cGridE, gp * 4}, {&c->cGridHE, gp * 2}, {&c->cMlpE, mp * 4}, {&c->cFwdE, 68 * 256 * 2},
+                   {&c->cBwdE, 68 * 256 * 2}};
+        for (auto& a : ema) {
+            stk_status st = devReserve(c, *a.b, a.bytes);
+            if (st != STK_OK) {
+                stk_cache_destroy(c);
+                return st;
+            }
+        }
+        cb.gridE = static_cast<float*>(c->cGridE.ptr);
+        cb.gridHE = static_cast<__half*>(c->cGridHE.ptr);
+        cb.mlpE = static_cast<float*>(c->cMlpE.ptr);
+        cb.wFwdE = static_cast<__half*>(c->cFwdE.ptr);
+        cb.wBwdE = static_cast<__half*>(c->cBwdE.ptr);
+    }
+    cb.gridH = static_cast<__half*>(c->cGridH.ptr);
+    cb.gridP = static_cast<float*>(c->cGridP.ptr);
+    cb.gridM = static_cast<float*>(c->cGridM.ptr);
+    cb.gridV = static_cast<float*>(c->cGridV.ptr);
+    cb.gridG = static_cast<float*>(c->cGridG.ptr);
+    cb.mlpP = static_cast<float*>(c->cMlpP.ptr);
+    cb.mlpM = static_cast<float*>(c->cMlpM.ptr);
+    cb.mlpV = static_cast<float*>(c->cMlpV.ptr);
+    cb.mlpG = static_cast<float*>(c->cMlpG.ptr);
+    cb.wFwd = static_cast<__half*>(c->cFwd.ptr);
+    cb.wBwd = static_cast<__half*>(c->cBwd.ptr);
+    cb.xT = static_cast<__half*>(c->cXT.ptr);
+    cb.dyT = static_cast<__half*>(c->cDyT.ptr);
+    cb.loss = static_cast<float*>(c->cLoss.ptr);
+    // initialisation: grid U(-1e-4, 1e-4); weights He-uniform U(-sqrt(6/64), sqrt(6/64))
+    std::vector<float> grid(gp), mlp(mp);
+    uint64_t s = 0x9E3779B97F4A7C15ull ^ ((uint64_t)cfg.seed << 1);
+    auto uni = [&s]() {
+        s = s * 6364136223846793005ull + 1442695040888963407ull;
+        return (float)((s >> 40) & 0xFFFFFF) * (1.0f / 16777216.0f);
+    };
+    for (float& x : grid) x = (uni() * 2.0f - 1.0f) * 1e-4f;
+    const float a = std::sqrt(6.0f / 64.0f);
+    for (float& x : mlp) x = (uni() * 2.0f - 1.0f) * a;
+    for (size_t in = 0; in < 64; ++in)
+        for (size_t o = 3; o < 16; ++o) mlp[16384 + in * 16 + o] = 0.0f; // unused output columns
+    STK_HIP(c, hipMemcpyAsync(cb.gridP, grid.data(), gp * 4, hipMemcpyHostToDevice, c->stream));
+    STK_HIP(c, hipMemcpyAsync(cb.mlpP, mlp.data(), mp * 4, hipMemcpyHostToDevice, c->stream));
+    STK_HIP(c, hipMemsetAsync(cb.gridM, 0, gp * 4, c->stream));
+    STK_HIP(c, hipMemsetAsync(cb.gridV, 0, gp * 4, c->stream));
+    STK_HIP(c, hipMemsetAsync(cb.gridG, 0, gp * 4, c->stream));
+    STK_HIP(c, hipMemsetAsync(cb.mlpM, 0, mp * 4, c->stream));
+    STK_HIP(c, hipMemsetAsync(cb.mlpV, 0, mp * 4, c->stream));
+    STK_HIP(c, hipMemsetAsync(cb.mlpG, 0, mp * 4, c->stream));
+    STK_HIP(c, stk::cachePack(cb, c->stream));
+    STK_HIP(c, hipStreamSynchronize(c->stream));
+    c->cacheCfg = cfg;
+    c->haveCache = true;
+    return STK_OK;
+}
+
+STK_API stk_status stk_cache_info(stk_ctx* c, StkCacheInfo* out)
+{
+    if (!validCtx(c) || !out) return STK_E_INVALID;
+    if (!c->haveCache) return fail(c, STK_E_STATE, "stk_cache_info: no cache");
+    const stk::GridDesc& gd = c->cache.grid;
+    for (int a = 0; a < 3; ++a) out->lo[a] = gd.lo[a];
+    out->scale = gd.scale;
+    for (int l = 0; l < 16; ++l) {
+        out->resolution[l] = gd.res[l];
+        out->dense[l] = gd.dense[l];
+    }
+    out->log2TableSize = gd.log2Table;
+    out->gridParams = (uint32_t)c->cache.gridParams;
+    out->mlpParams = stk::kCacheMlpParams;
+    out->step = c->cache.step;
+    return STK_OK;
+}
+
+STK_API stk_status stk_cache_infer(stk_ctx* c, const StkCacheRecord* recs, uint32_t n, float* out)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    if (!c->haveCache) return fail(c, STK_E_STATE, "stk_cache_infer: no cache");
+    if (n && (!recs || !out)) return fail(c, STK_E_INVALID, "stk_cache_infer: null buffer");
+    (void)hipSetDevice(c->device);
+    STK_HIP(c, stk::cacheInfer(c->cache, reinterpret_cast<const stk::CacheRecord*>(recs), n, nullptr,
+                               reinterpret_cast<float4*>(out), c->stream));
+    return STK_OK;
+}
+
+STK_API stk_status stk_cache_train_step(stk_ctx* c, const StkCacheRecord* recs, uint32_t n, float* lossOut)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    if (!c->haveCache) return fail(c, STK_E_STATE, "stk_cache_train_step: no cache");
+    if (n == 0 || !recs) return fail(c, STK_E_INVALID, "stk_cache_train_step: empty batch");
+    if (n > c->cache.nCap) return fail(c, STK_E_INVALID, "stk_cache_train_step: batch above maxBatch");
+    (void)hipSetDevice(c->device);
+    STK_HIP(c, stk::cacheBackward(c->cache, reinterpret_cast<const stk::CacheRecord*>(recs), n, nullptr, c->stream));
+    float loss = 0.0f;
+    STK_HIP(c, hipMemcpyAsync(&loss, c->cache.loss, 4, hipMemcpyDeviceToHost, c->stream));
+    STK_HIP(c, stk::cacheAdam(c->cache, 1.0f / (float)n, nullptr, n, c->stream));
+    STK_HIP(c, hipStreamSynchronize(c->stream));
+    if (lossOut) *lossOut = loss / (float)n;
+    return STK_OK;
+}
+
+STK_API stk_status stk_cache_gradients(stk_ctx* c, const StkCacheRecord* recs, uint32_t n, float* gridGrad,
+                                       float* mlpGrad, float* lossSum)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    if (!c->haveCache) return fail(c, STK_E_STATE, "stk_cache_gradients: no cache");
+    if (n == 0 || !recs || n > c->cache.nCap) return fail(c, STK_E_INVALID, "stk_cache_gradients: bad batch");
+    (void)hipSetDevice(c->device);
+    const stk::CacheBuffers& cb = c->cache;
+    STK_HIP(c, stk::cacheZeroGrads(cb, c->stream));
+    STK_HIP(c, stk::cacheBackward(cb, reinterpret_cast<const stk::CacheRecord*>(recs), n, nullptr, c->stream));
+    if (gridGrad)
+        STK_HIP(c, hipMemcpyAsync(gridGrad, cb.gridG, cb.gridParams * 4, hipMemcpyDeviceToHost, c->stream));
+    if (mlpGrad)
+        STK_HIP(c, hipMemcpyAsync(mlpGrad, cb.mlpG, stk::kCacheMlpParams * 4, hipMemcpyDeviceToHost, c->stream));
+    if (lossSum) STK_HIP(c, hipMemcpyAsync(lossSum, cb.loss, 4, hipMemcpyDeviceToHost, c->stream));
+    STK_HIP(c, stk::cacheZeroGrads(cb, c->stream));
+    STK_HIP(c, hipStreamSynchronize(c->stream));
+    return STK_OK;
+}
+
+STK_API stk_status stk_cache_get_params(stk_ctx* c, float* grid, float* mlp)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    if (!c->haveCache) return fail(c, STK_E_STATE, "stk_cache_get_params: no cache");
+    if (grid)
+        STK_HIP(c, hipMemcpyAsync(grid, c->cache.gridP, c->cache.gridParams * 4, hipMemcpyDeviceToHost, c->stream));
+    if (mlp)
+        STK_HIP(c, hipMemcpyAsync(mlp, c->cache.mlpP, stk::kCacheMlpParams * 4, hipMemcpyDeviceToHost, c->stream));
+    STK_HIP(c, hipStreamSynchronize(c->stream));
+    return STK_OK;
+}
+
+STK_API stk_status stk_cache_set_params(stk_ctx* c, const float* grid, const float* mlp)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    if (!c->haveCache) return fail(c, STK_E_STATE, "stk_cache_set_params: no cache");
+    if (grid)
+        STK_HIP(c, hipMemcpyAsync(c->cache.gridP, grid, c->cache.gridParams * 4, hipMemcpyHostToDevice, c->stream));
+    if (mlp)
+        STK_HIP(c, hipMemcpyAsync(c->cache.mlpP, mlp, stk::kCacheMlpParams * 4, hipMemcpyHostToDevice, c->stream));
+    STK_HIP(c, stk::cachePack(c->cache, c->stream));
+    STK_HIP(c, hipStreamSynchronize(c->stream));
+    return STK_OK;
+}
+
+STK_API stk_status stk_get_counters(stk_ctx* c, uint32_t out[8])
+{
+    if (!validCtx(c) || !out) return STK_E_INVALID;
+    if (!c->ps.counters) return fail(c, STK_E_STATE, "stk_get_counters: nothing rendered");
+    STK_HIP(c, hipMemcpyAsync(out, c->ps.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     STK_HIP(c, hipStreamSynchronize(c->stream));
     return STK_OK;
 }

@@ -137,6 +137,9 @@ __global__ __launch_bounds__(kBlock) void k_pt_raygen(PtLaunch lp, PathState ps)
     ps.col[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     ps.rng[slot] = rng;
     ps.queue[0][slot] = slot;
+    const PtCache& pc = lp.cache;
+    if (pc.mode != 0 && pc.mode != 2 && slot % pc.trainStride == pc.trainOffset)
+        pc.trainCount[slot / pc.trainStride] = 0;
 }
 
 __global__ void k_pt_setup(PathState ps, int depth)
@@ -170,11 +173,13 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, 
     const int cur = depth & 1;
     const uint32_t count = ps.counters[cur];
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    bool cont = false, shadow = false;
+    bool cont = false, shadow = false, cacheQuery = false;
     uint32_t slot = 0;
     F3 nO = mk3(0, 0, 0), nD = mk3(0, 0, 0);
     F3 sO = mk3(0, 0, 0), sD = mk3(0, 0, 0), cd = mk3(0, 0, 0), cg = mk3(0, 0, 0);
     float sT = 0.0f;
+    CacheRecord qrec;
+    const PtCache& pc = lp.cache;
     if (q < count) {
         slot = ps.queue[cur][q];
         const float4 h = ps.hit[q];
@@ -218,10 +223,28 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, 
                     mat.albedo[3] = 1.0f;
                 }
                 const F3 k1 = -f3(ps.rayD[slot]);
+                const float rough = mat.type == STK_MAT_GGX ? mat.roughness : 1.0f;
+                const bool isTrain = pc.mode != 0 && pc.mode != 2 && slot % pc.trainStride == pc.trainOffset;
+                if ((pc.mode == 1 || pc.mode == 2) && !isTrain && depth == (int)pc.queryDepth) {
+                    // the path ends here: its remaining radiance comes from the cache
+                    cacheQuery = true;
+                    qrec.posRough = make_float4(P.x, P.y, P.z, rough);
+                    qrec.normal = make_float4(N.x, N.y, N.z, bits_to_float(slot));
+                    qrec.dir = make_float4(k1.x, k1.y, k1.z, 0.0f);
+                    qrec.target = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                } else if (isTrain) {
+                    float4* v = pc.trainVerts + ((size_t)(slot / pc.trainStride) * lp.pt.maxDepth + depth) * 5;
+                    v[0] = make_float4(P.x, P.y, P.z, rough);
+                    v[1] = make_float4(N.x, N.y, N.z, 0.0f);
+                    v[2] = make_float4(k1.x, k1.y, k1.z, 0.0f);
+                    v[3] = f4(col);
+                    v[4] = f4(thr);
+                    pc.trainCount[slot / pc.trainStride] = (uint32_t)depth + 1;
+                }
                 // sampleLights / estimateDirectLighting (shaders/pathtracerMdl.hlsl:31-92)
                 F3 toLight = mk3(0, 0, 0), radiance = mk3(0, 0, 0);
                 float lightPdf = 0.0f, dist = 0.0f;
-                if (sc.numLights > 0) {
+                if (sc.numLights > 0 && !cacheQuery) {
                     const uint32_t lid = (uint32_t)((float)sc.numLights * rnd(rng));
                     const float selPdf = 1.0f / (float)sc.numLights;
                     const StkLight& l = sc.lights[lid < sc.numLights ? lid : sc.numLights - 1];
@@ -257,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, 
                         sT = dist;
                     }
                 }
-                if (alive) {
+                if (alive && !cacheQuery) {
                     const float x0 = rnd(rng);
                     const float x1 = rnd(rng);
                     (void)rnd(rng);
@@ -299,6 +322,56 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, 
         ps.shCd[si] = f4(cd);
         ps.shCg[si] = f4(cg);
     }
+    if (pc.mode == 1 || pc.mode == 2) {
+        const uint32_t ci = wave_push(&ps.counters[3], cacheQuery);
+        if (cacheQuery) pc.queryRecs[ci] = qrec;
+    }
+}
+
+// Adds the cache's answer to the paths that ended at a query:
+// radiance += throughput * max(prediction, 0).
+__global__ __launch_bounds__(kBlock) void k_cache_apply(PtLaunch lp, PathState ps)
+{
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= ps.counters[3]) return;
+    const uint32_t slot = float_to_bits(lp.cache.queryRecs[q].normal.w);
+    const float4 o = lp.cache.queryOut[q];
+    const F3 thr = f3(ps.thr[slot]);
+    const F3 col = f3(ps.col[slot]) + thr * mk3(fmaxf(o.x, 0.0f), fmaxf(o.y, 0.0f), fmaxf(o.z, 0.0f));
+    ps.col[slot] = f4(col);
+}
+
+// Training targets: the radiance a training path gathered from vertex d on,
+// divided by the throughput at d - a one-sample estimate of the outgoing
+// radiance at d under the same estimator as the plain path tracer.
+__global__ __launch_bounds__(kBlock) void k_cache_targets(PtLaunch lp, PathState ps)
+{
+    const PtCache& pc = lp.cache;
+    const uint32_t maxDepth = lp.pt.maxDepth;
+    const uint32_t id = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t tidx = id / maxDepth, d = id % maxDepth;
+    bool emit = false;
+    CacheRecord rec;
+    if (tidx < pc.numTrainPaths && d < pc.trainCount[tidx]) {
+        const uint32_t slot = tidx * pc.trainStride + pc.trainOffset;
+        const float4* v = pc.trainVerts + ((size_t)tidx * maxDepth + d) * 5;
+        const float4 before = v[3], thr = v[4], fin = ps.col[slot];
+        if (thr.x > 1e-8f && thr.y > 1e-8f && thr.z > 1e-8f) {
+            rec.posRough = v[0];
+            rec.normal = v[1];
+            rec.dir = v[2];
+            // one-sample targets are clamped to [0, kTargetMax]: Russian-roulette
+            // survivors and near-light NEE samples can be arbitrarily large, and
+            // a few such fireflies would dominate the loss
+            const float kTargetMax = 100.0f;
+            rec.target = make_float4(fminf(fmaxf((fin.x - before.x) / thr.x, 0.0f), kTargetMax),
+                                     fminf(fmaxf((fin.y - before.y) / thr.y, 0.0f), kTargetMax),
+                                     fminf(fmaxf((fin.z - before.z) / thr.z, 0.0f), kTargetMax), 0.0f);
+            emit = isfinite(fin.x) && isfinite(fin.y) && isfinite(fin.z);
+        }
+    }
+    const uint32_t i = wave_push(&ps.counters[4], emit);
+    if (emit && i < pc.trainCap) pc.trainBatch[i] = rec;
 }
 
 // Shadow rays use the stackless skip-link walk: for these long incoherent
@@ -439,11 +512,15 @@ hipError_t launchCameraRays(const StkPathTracerParam& p, uint32_t sampleNum, Stk
 }
 
 hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, PathState& ps, float4* accum,
-                               float* samples, hipStream_t s)
+                               float* samples, CacheBuffers* cb, hipStream_t s)
 {
     const uint32_t npix = (uint32_t)lp.pt.dimension[0] * (uint32_t)lp.pt.dimension[1];
     if (npix == 0) return hipSuccess;
     const dim3 g(blocks(npix)), b(kBlock);
+    const PtCache& pc = lp.cache;
+    const bool query = pc.mode == 1 || pc.mode == 2, train = pc.mode == 1 || pc.mode == 3;
+    // with a cache query at depth q, non-training paths end at q; training
+    // paths still run to maxDepth
     hipLaunchKernelGGL(k_pt_raygen, g, b, 0, s, lp, ps);
     for (int depth = 0; depth < (int)lp.pt.maxDepth; ++depth) {
         hipLaunchKernelGGL(k_pt_setup, dim3(1), dim3(64), 0, s, ps, depth);
@@ -451,7 +528,20 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, PathStat
         hipLaunchKernelGGL(k_pt_shade, g, b, 0, s, sc, lp, ps, depth);
         hipLaunchKernelGGL(k_pt_connect, g, b, 0, s, sc, ps);
     }
+    if (query && cb) {
+        hipError_t e = cacheInfer(*cb, pc.queryRecs, npix, ps.counters + 3, pc.queryOut, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_cache_apply, g, b, 0, s, lp, ps);
+    }
     hipLaunchKernelGGL(k_pt_finalize, g, b, 0, s, lp, ps, accum, samples, npix);
+    if (train && cb && pc.numTrainPaths > 0) {
+        hipLaunchKernelGGL(k_cache_targets, dim3(blocks((size_t)pc.numTrainPaths * lp.pt.maxDepth)), b, 0, s, lp,
+                           ps);
+        hipError_t e = cacheBackward(*cb, pc.trainBatch, pc.trainCap, ps.counters + 4, s);
+        if (e != hipSuccess) return e;
+        e = cacheAdam(*cb, 1.0f, ps.counters + 4, pc.trainCap, s);
+        if (e != hipSuccess) return e;
+    }
     return hipGetLastError();
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/stk/stk.h"
+#include "cache.hpp"
 #include "trace_dev.hpp"
 
 namespace stk {
@@ -38,6 +39,25 @@ struct SceneView {
     uint32_t numMats;
 };
 
+// Radiance-cache hooks of one pass.  Training paths are the slots with
+// slot % trainStride == trainOffset: they run to maxDepth and record every
+// surface vertex (5 float4: position+roughness, normal, outgoing direction,
+// radiance before the vertex, throughput at the vertex).  Other paths query
+// the cache at vertex depth queryDepth and end there.
+struct PtCache {
+    uint32_t mode = 0; // 0 off, 1 query + train, 2 query only, 3 train only
+    uint32_t queryDepth = 2;
+    uint32_t trainStride = 1;
+    uint32_t trainOffset = 0;
+    uint32_t trainCap = 0;
+    uint32_t numTrainPaths = 0;
+    CacheRecord* queryRecs = nullptr; // [npix]
+    float4* queryOut = nullptr;       // [npix]
+    float4* trainVerts = nullptr;     // [numTrainPaths * maxDepth * 5]
+    uint32_t* trainCount = nullptr;   // [numTrainPaths]
+    CacheRecord* trainBatch = nullptr; // [trainCap]
+};
+
 struct PtLaunch {
     StkPathTracerParam pt;
     float sky[4];
@@ -45,6 +65,7 @@ struct PtLaunch {
     uint32_t refCompat;
     uint32_t sampleNum; // 0 .. spp-1
     float invSpp;
+    PtCache cache;
 };
 
 hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, bool refOrder,
@@ -55,8 +76,10 @@ hipError_t launchCameraRays(const StkPathTracerParam& p, uint32_t sampleNum, Stk
 
 // One sample pass of the wavefront path tracer; accumulates invSpp * radiance
 // into accum (float4 per pixel) and optionally stores per-sample radiance.
+// With lp.cache.mode != 0, cb must be the context's cache: queries are answered
+// after the bounce loop and one training step runs on the pass's records.
 hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, PathState& ps, float4* accum,
-                               float* samples, hipStream_t s);
+                               float* samples, CacheBuffers* cb, hipStream_t s);
 // Plain 16-B-per-lane copy kernel (bytes must be a multiple of 16).
 hipError_t launchCopy(void* dst, const void* src, size_t bytes, hipStream_t s);
 // Zeroes n float4 (a kernel, so the call can be captured in a graph).

@@ -204,17 +204,20 @@ extern "C" STK_API stk_status stk_scene_generate(uint32_t config, uint32_t varia
     Scene& sc = s->scene;
     const bool refIco = (flags & STK_SCENE_REF_ICOSPHERE) != 0;
     try {
-        if (config == 1) {
+        if (config == 1 || config == 7) {
             StkMaterial sphere{};
             sphere.type = STK_MAT_LAMBERT;
             sphere.albedo[0] = sphere.albedo[1] = sphere.albedo[2] = 0.8f;
             StkMaterial wall = sphere;
-            wall.albedo[0] = wall.albedo[1] = wall.albedo[2] = 0.5f;
+            const float wa = config == 7 ? 0.7f : 0.5f;
+            wall.albedo[0] = wall.albedo[1] = wall.albedo[2] = wa;
             const int32_t ms = (int32_t)sc.addMaterial(sphere), mw = (int32_t)sc.addMaterial(wall);
             const uint32_t mesh = sc.createIcosphereMesh(3, refIco);
             sc.createInstance(mesh, ms, Mat4::identity());
             addInwardBox(sc, 2.0f, mw);
-            cameraMatrices(mk3(0, 0, 1.9f), mk3(0, 0, 0), mk3(0, 1, 0), 60.0f, 256, 256, 0.01f, 100.0f, s->camera);
+            const uint32_t res = config == 7 ? 128 : 256;
+            if (config == 7) addQuadLight(sc, 1.9f, 1.0f, 20.0f); // closed room lit from the ceiling
+            cameraMatrices(mk3(0, 0, 1.9f), mk3(0, 0, 0), mk3(0, 1, 0), 60.0f, res, res, 0.01f, 100.0f, s->camera);
         } else if (config == 2 || config == 3 || config == 6) {
             uint32_t hfRes = 512, spheres = 2000, width = 1920, height = 1080;
             float extent = 10.0f, lightY = 8.0f, lightSize = 2.0f, intensity = 100.0f;

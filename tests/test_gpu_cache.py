@@ -1,0 +1,169 @@
+"""Radiance cache on the GPU through the C ABI: the hand-written encode /
+forward / backward / Adam kernels against the FP64 CPU oracle
+(oracle/cache_ref.py, FP16 rounding emulated where the kernels round), and
+what the trained cache gains over the plain path tracer."""
+import numpy as np
+import pytest
+
+import strelka_amd as sa
+from oracle.cache_ref import CacheOracle, adam
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = sa.Context(0)
+    s = sa.SceneData.generate(6)
+    c.set_scene(s)
+    c.build_bvh()
+    c.scene = s
+    yield c
+    c.close()
+
+
+def records(info, n, seed):
+    rng = np.random.default_rng(seed)
+    r = np.zeros(n, sa.CACHE_RECORD)
+    lo = np.array(list(info.lo))
+    ext = 1.0 / info.scale
+    r["position"] = lo + rng.uniform(0.02, 0.98, (n, 3)) * ext
+    nrm = rng.normal(size=(n, 3))
+    r["normal"] = nrm / np.linalg.norm(nrm, axis=1, keepdims=True)
+    d = rng.normal(size=(n, 3))
+    r["direction"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    r["roughness"] = rng.uniform(0, 1, n)
+    r["target"] = rng.uniform(0, 2, (n, 3))
+    return r
+
+
+def random_params(info, seed):
+    rng = np.random.default_rng(seed)
+    grid = rng.uniform(-0.5, 0.5, info.gridParams).astype(np.float32)
+    mlp = rng.uniform(-0.25, 0.25, info.mlpParams).astype(np.float32)
+    mlp[16384:].reshape(64, 16)[:, 3:] = 0.0
+    return grid, mlp
+
+
+def test_inference_matches_oracle(ctx):
+    print("case: cache inference, 4096 records, log2 table 14", flush=True)
+    ctx.cache_create(log2_table=14, seed=7)
+    info = ctx.cache_info()
+    grid, mlp = random_params(info, 1)
+    ctx.cache_set_params(grid, mlp)
+    recs = records(info, 4096, 2)
+    gpu = ctx.cache_infer(recs)[:, :3]
+    ora = CacheOracle(info, grid, mlp, round16=True).predict(recs)
+    scale = np.abs(ora).max()
+    assert scale > 0.05
+    err = np.abs(gpu - ora).max() / scale
+    print(f"max relative error {err:.2e}")
+    assert err < 5e-3
+    exact = CacheOracle(info, grid, mlp, round16=False).predict(recs)
+    assert np.abs(gpu - exact).max() / scale < 3e-2
+
+
+def test_gradients_match_oracle(ctx):
+    print("case: cache gradients, 2048 records", flush=True)
+    ctx.cache_create(log2_table=14, seed=7)
+    info = ctx.cache_info()
+    grid, mlp = random_params(info, 3)
+    ctx.cache_set_params(grid, mlp)
+    recs = records(info, 2048, 4)
+    g_grid, g_mlp, loss = ctx.cache_gradients(recs)
+    o = CacheOracle(info, grid, mlp, round16=True)
+    og, dW, oloss = o.gradients(recs)
+    om = o.flat_mlp_grad(dW)
+    assert loss == pytest.approx(oloss, rel=1e-2)
+    rel_mlp = np.linalg.norm(g_mlp - om) / np.linalg.norm(om)
+    rel_grid = np.linalg.norm(g_grid - og.ravel()) / np.linalg.norm(og)
+    print(f"relative gradient error: mlp {rel_mlp:.2e}, grid {rel_grid:.2e}")
+    assert rel_mlp < 2e-2 and rel_grid < 2e-2
+    # per layer too (a transposed or misplaced fragment would fail here)
+    for l in range(5):
+        a = g_mlp[l * 4096:(l + 1) * 4096] if l < 4 else g_mlp[16384:]
+        b = om[l * 4096:(l + 1) * 4096] if l < 4 else om[16384:]
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) < 3e-2, l
+    # unused output columns get no gradient
+    assert np.all(g_mlp[16384:].reshape(64, 16)[:, 3:] == 0)
+
+
+def test_adam_step_matches_oracle(ctx):
+    print("case: one Adam step", flush=True)
+    ctx.cache_create(log2_table=14, seed=7, lr=1e-2)
+    info = ctx.cache_info()
+    grid, mlp = random_params(info, 5)
+    ctx.cache_set_params(grid, mlp)
+    recs = records(info, 1024, 6)
+    g_grid, g_mlp, _ = ctx.cache_gradients(recs)
+    ctx.cache_train_step(recs)
+    p_grid, p_mlp = ctx.cache_get_params()
+    n = recs.size
+    e_mlp, _, _ = adam(mlp, np.zeros_like(g_mlp), np.zeros_like(g_mlp), g_mlp / n, 1, lr=1e-2)
+    e_grid, _, _ = adam(grid, np.zeros_like(g_grid), np.zeros_like(g_grid), g_grid / n, 1, lr=1e-2, sparse=True)
+    assert np.mean(np.abs(p_mlp - e_mlp) < 1e-5) > 0.999
+    assert np.mean(np.abs(p_grid - e_grid) < 1e-5) > 0.999
+    assert ctx.cache_info().step == 1
+
+
+def test_training_reduces_loss(ctx):
+    print("case: 150 training steps on a smooth radiance field", flush=True)
+    ctx.cache_create(log2_table=16, seed=3)
+    info = ctx.cache_info()
+    rng = np.random.default_rng(8)
+    losses = []
+    for it in range(150):
+        r = records(info, 4096, 100 + it)
+        p = (r["position"] - np.array(list(info.lo))) * info.scale
+        r["target"] = 0.5 + 0.4 * np.sin(6.0 * p) * np.array([1.0, 0.7, 0.4])
+        losses.append(ctx.cache_train_step(r))
+    print(f"loss {losses[0]:.4f} -> {losses[-1]:.5f}")
+    assert np.isfinite(losses[-1])
+    assert losses[-1] < 0.1 * losses[0]
+
+
+def rmse(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)))
+
+
+def test_trained_cache_beats_plain_path_tracing_at_equal_samples():
+    print("case: config 7 closed room, cached vs plain path tracing, 8 frames of 1 spp", flush=True)
+    s = sa.SceneData.generate(7)
+    ctx = sa.Context(0)
+    ctx.set_scene(s)
+    ctx.build_bvh()
+    w, h = s.camera.dimension[0], s.camera.dimension[1]
+    sky = (0.0, 0.0, 0.0)
+    ctx.reset_accumulation()
+    ref = ctx.render(sa.render_params(s.camera, spp=1024, max_depth=8, frame=1000, sampling=1, sky=sky,
+                                      enable_tonemap=False))[..., :3].copy()
+    frames = 8
+
+    def run(mode, base):
+        ctx.reset_accumulation()
+        out = None
+        for f in range(frames):
+            p = sa.render_params(s.camera, spp=1, max_depth=8, frame=base + f, sampling=1, sky=sky,
+                                 enable_tonemap=False, cache_mode=mode, cache_query_depth=2, cache_train_paths=4096)
+            out = ctx.render(p)[..., :3].copy()
+        return out
+
+    plain = run(0, 0)
+    ctx.cache_create(log2_table=16, seed=11)
+    # warm-up: training-only frames (extended paths, no queries)
+    for f in range(300):
+        p = sa.render_params(s.camera, spp=1, max_depth=8, frame=5000 + f, sampling=1, sky=sky,
+                             enable_tonemap=False, cache_mode=3, cache_train_paths=4096)
+        ctx.render(p)
+    cnt = ctx.counters()
+    assert cnt[4] > 4096 and cnt[3] == 0  # training records, no queries in train-only mode
+    g, m = ctx.cache_get_params()
+    assert np.all(np.isfinite(g)) and np.all(np.isfinite(m))
+    cached = run(1, 0)
+    cnt = ctx.counters()
+    assert 0 < cnt[3] <= w * h  # queries answered this frame
+    assert np.all(np.isfinite(cached))
+    e_plain, e_cached = rmse(plain, ref), rmse(cached, ref)
+    print(f"RMSE vs 1024-spp reference: plain {e_plain:.4f}, cached {e_cached:.4f}, queries {cnt[3]}")
+    ctx.close()
+    assert e_cached < e_plain
