@@ -139,6 +139,63 @@ STK_API stk_status stk_get_accumulation(stk_ctx* ctx, float* out, size_t count);
  * bounce, [3] cache queries, [4] cache training records produced. */
 STK_API stk_status stk_get_counters(stk_ctx* ctx, uint32_t out[8]);
 
+/* ---- launch shape ----------------------------------------------------------
+ * No counterpart in the reference (its compute passes use a fixed 16x16
+ * workgroup, src/render/pathtracer.cpp).  blockSize 64/128/256 threads for the
+ * wavefront kernels, raygen tile edge in pixels (1 = scanline order),
+ * persistentBlocks > 0 makes the extend kernel a persistent grid of that many
+ * blocks fetching rays from a work counter, and useGraph captures one sample
+ * pass as a hipGraph replayed for every sample (plain rendering only). */
+typedef struct StkLaunchConfig {
+    uint32_t blockSize;
+    uint32_t tile;
+    uint32_t persistentBlocks;
+    uint32_t useGraph;
+} StkLaunchConfig;
+STK_API stk_status stk_set_launch_config(stk_ctx* ctx, const StkLaunchConfig* lc);
+STK_API stk_status stk_get_launch_config(stk_ctx* ctx, StkLaunchConfig* lc);
+
+/* ---- batched scenes --------------------------------------------------------
+ * Many independent scenes, each with its own view, rendered together: every
+ * wavefront kernel of a sample pass runs once over the path slots of all
+ * views.  The reference renders one scene per PtRender; a host loop over
+ * PtRender::drawFrame (src/render/ptrender.cpp:641-913) per scene is what a
+ * batch replaces.  Buffers are copied by stk_batch_add_scene; instance
+ * indexOffset/materialId/lightId are relative to the scene's own arrays. */
+typedef struct StkBatchScene {
+    const StkVertex* vb;
+    uint32_t numVertices;
+    const uint32_t* ib;
+    uint32_t numIndices;
+    const StkInstance* inst;
+    uint32_t numInstances;
+    const StkLight* lights;
+    uint32_t numLights;
+    const StkMaterial* mats;
+    uint32_t numMaterials;
+    StkPathTracerParam camera; /* matrices and dimension of the view */
+    float sky[4];              /* constant sky radiance */
+} StkBatchScene;
+typedef struct StkBatchViewInfo {
+    uint32_t pixelBase; /* first pixel of the view in the batched image */
+    uint32_t width, height;
+    uint32_t numTriangles;
+    uint32_t bvhDepth;
+    uint32_t numScenes;
+    uint32_t pad[2];
+} StkBatchViewInfo;
+STK_API stk_status stk_batch_clear(stk_ctx* ctx);
+STK_API stk_status stk_batch_add_scene(stk_ctx* ctx, const StkBatchScene* scene, uint32_t* index);
+/* Builds every scene's BVH (host threads) and uploads the merged arrays. */
+STK_API stk_status stk_batch_commit(stk_ctx* ctx, const StkBvhOptions* opts);
+STK_API stk_status stk_batch_info(stk_ctx* ctx, uint32_t scene, StkBatchViewInfo* out);
+/* stk_render over every committed view: p supplies spp, maxDepth,
+ * frameNumber, sampling, rrStartDepth, accumulation and tonemap settings
+ * (its camera, dimension and sky are ignored; cacheMode must be 0).  outRgba
+ * receives the views one after another (view i at pixelBase_i, width_i x
+ * height_i RGBA32F). */
+STK_API stk_status stk_batch_render(stk_ctx* ctx, const StkRenderParams* p, float* outRgba, int outIsDevice);
+
 /* ---- radiance cache -------------------------------------------------------
  * No counterpart in the reference renderer: an online-trained cache of
  * outgoing radiance (multi-resolution hash grid of 16 levels x 2 FP16

@@ -15,7 +15,8 @@ import numpy as np
 
 from . import _abi
 from ._abi import (BVH_NODE, CACHE_RECORD, HIT, INSTANCE, INVALID, LIGHT, MATERIAL, PACKED_NODE, RAY, TRIANGLE,
-                   VERTEX, BvhHeader, BvhOptions, CacheConfig, CacheInfo, PathTracerParam, RenderParams, SceneArrays)
+                   VERTEX, BatchScene, BatchViewInfo, BvhHeader, BvhOptions, CacheConfig, CacheInfo, LaunchConfig,
+                   PathTracerParam, RenderParams, SceneArrays)
 
 __all__ = ["lib", "StkError", "SceneData", "HostBvh", "Context", "RenderParams", "PathTracerParam", "VERTEX",
            "BVH_NODE", "INSTANCE", "LIGHT", "RAY", "HIT", "MATERIAL", "TRIANGLE", "PACKED_NODE", "INVALID",
@@ -163,6 +164,7 @@ class Context:
         if st:
             raise StkError(st, f"stk_create(device={device})")
         self._h = h
+        self._batch_views = []
 
     @property
     def handle(self):
@@ -293,6 +295,68 @@ class Context:
         out = np.zeros(8, np.uint32)
         self._check(_LIB.stk_get_counters(self._h, _ptr(out)), "stk_get_counters")
         return out
+
+    # -- launch shape --------------------------------------------------------------
+    def set_launch_config(self, block_size: int = 256, tile: int = 8, persistent_blocks: int = 0,
+                          graph: bool = False):
+        lc = LaunchConfig(block_size, tile, persistent_blocks, 1 if graph else 0)
+        self._check(_LIB.stk_set_launch_config(self._h, C.byref(lc)), "stk_set_launch_config")
+
+    def launch_config(self) -> dict:
+        lc = LaunchConfig()
+        self._check(_LIB.stk_get_launch_config(self._h, C.byref(lc)), "stk_get_launch_config")
+        return {"block_size": lc.blockSize, "tile": lc.tile, "persistent_blocks": lc.persistentBlocks,
+                "graph": bool(lc.useGraph)}
+
+    # -- batched scenes -------------------------------------------------------------
+    def batch_clear(self):
+        self._check(_LIB.stk_batch_clear(self._h), "stk_batch_clear")
+        self._batch_views = []
+
+    def batch_add(self, s: SceneData, camera: PathTracerParam | None = None, sky=None) -> int:
+        cam = camera if camera is not None else s.camera
+        b = BatchScene(_ptr(s.vb), s.vb.size, _ptr(s.ib), s.ib.size, _ptr(s.inst), s.inst.size, _ptr(s.lights),
+                       s.lights.size, _ptr(s.mats), s.mats.size)
+        C.memmove(C.byref(b.camera), C.byref(cam), C.sizeof(PathTracerParam))
+        sk = tuple(sky) if sky is not None else tuple(s.sky)
+        for i in range(4):
+            b.sky[i] = float(sk[i]) if i < len(sk) else 0.0
+        idx = C.c_uint32()
+        self._check(_LIB.stk_batch_add_scene(self._h, C.byref(b), C.byref(idx)), "stk_batch_add_scene")
+        return int(idx.value)
+
+    def batch_commit(self, max_depth: int = 32, bins: int = 32):
+        opts = BvhOptions(max_depth, bins, 1.0, 1.0)
+        self._check(_LIB.stk_batch_commit(self._h, C.byref(opts)), "stk_batch_commit")
+        self._batch_views = []
+        i = 0
+        while True:
+            info = BatchViewInfo()
+            self._check(_LIB.stk_batch_info(self._h, i, C.byref(info)), "stk_batch_info")
+            self._batch_views.append(info)
+            i += 1
+            if i >= info.numScenes:
+                break
+
+    def batch_views(self) -> list:
+        return [dict(pixel_base=v.pixelBase, width=v.width, height=v.height, triangles=v.numTriangles,
+                     depth=v.bvhDepth) for v in self._batch_views]
+
+    def batch_pixels(self) -> int:
+        return sum(v.width * v.height for v in self._batch_views)
+
+    def batch_render(self, p: RenderParams, out: np.ndarray | None = None) -> list:
+        """Renders every committed view; returns one (h, w, 4) array per view
+        (views of the flat output buffer)."""
+        n = self.batch_pixels()
+        if out is None:
+            out = np.zeros((n, 4), dtype=np.float32)
+        self._check(_LIB.stk_batch_render(self._h, C.byref(p), _ptr(out), 0), "stk_batch_render")
+        return [out[v.pixelBase:v.pixelBase + v.width * v.height].reshape(v.height, v.width, 4)
+                for v in self._batch_views]
+
+    def batch_render_device(self, p: RenderParams, d_out: int):
+        self._check(_LIB.stk_batch_render(self._h, C.byref(p), C.c_void_p(d_out), 1), "stk_batch_render")
 
     # -- radiance cache ----------------------------------------------------------
     def cache_create(self, log2_table: int = 19, base_res: int = 16, max_res: int = 2048, seed: int = 1,

@@ -77,6 +77,24 @@ class BvhHeader(C.Structure):
     _fields_ = [("lo", C.c_float * 3), ("root", C.c_int32), ("hi", C.c_float * 3), ("numTriangles", C.c_uint32)]
 
 
+class LaunchConfig(C.Structure):
+    _fields_ = [("blockSize", C.c_uint32), ("tile", C.c_uint32), ("persistentBlocks", C.c_uint32),
+                ("useGraph", C.c_uint32)]
+
+
+class BatchScene(C.Structure):
+    _fields_ = [("vb", C.c_void_p), ("numVertices", C.c_uint32), ("ib", C.c_void_p), ("numIndices", C.c_uint32),
+                ("inst", C.c_void_p), ("numInstances", C.c_uint32), ("lights", C.c_void_p),
+                ("numLights", C.c_uint32), ("mats", C.c_void_p), ("numMaterials", C.c_uint32),
+                ("camera", PathTracerParam), ("sky", C.c_float * 4)]
+
+
+class BatchViewInfo(C.Structure):
+    _fields_ = [("pixelBase", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("numTriangles", C.c_uint32), ("bvhDepth", C.c_uint32), ("numScenes", C.c_uint32),
+                ("pad", C.c_uint32 * 2)]
+
+
 class SceneArrays(C.Structure):
     _fields_ = [("vb", C.c_void_p), ("numVertices", C.c_uint32), ("ib", C.c_void_p), ("numIndices", C.c_uint32),
                 ("inst", C.c_void_p), ("numInstances", C.c_uint32), ("lights", C.c_void_p),
@@ -118,6 +136,13 @@ PROTOTYPES = [
     ("stk_get_samples", C.c_int, [P, P, C.c_size_t]),
     ("stk_get_accumulation", C.c_int, [P, P, C.c_size_t]),
     ("stk_get_counters", C.c_int, [P, P]),
+    ("stk_set_launch_config", C.c_int, [P, C.POINTER(LaunchConfig)]),
+    ("stk_get_launch_config", C.c_int, [P, C.POINTER(LaunchConfig)]),
+    ("stk_batch_clear", C.c_int, [P]),
+    ("stk_batch_add_scene", C.c_int, [P, C.POINTER(BatchScene), C.POINTER(C.c_uint32)]),
+    ("stk_batch_commit", C.c_int, [P, C.POINTER(BvhOptions)]),
+    ("stk_batch_info", C.c_int, [P, C.c_uint32, C.POINTER(BatchViewInfo)]),
+    ("stk_batch_render", C.c_int, [P, C.POINTER(RenderParams), P, C.c_int]),
     ("stk_cache_create", C.c_int, [P, C.POINTER(CacheConfig)]),
     ("stk_cache_destroy", C.c_int, [P]),
     ("stk_cache_info", C.c_int, [P, C.POINTER(CacheInfo)]),

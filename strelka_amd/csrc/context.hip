@@ -5,7 +5,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <atomic>
 #include <memory>
+#include <thread>
 #include <new>
 #include <string>
 #include <vector>
@@ -23,6 +26,20 @@ namespace {
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+};
+
+// One scene staged for a batched render (stk_batch_add_scene) and its BVH
+// (built by stk_batch_commit).
+struct BatchScene {
+    std::vector<StkVertex> vb;
+    std::vector<uint32_t> ib;
+    std::vector<StkInstance> inst;
+    std::vector<StkLight> lights;
+    std::vector<StkMaterial> mats;
+    StkPathTracerParam cam;
+    float sky[4];
+    stk::Bvh bvh;
+    std::string err;
 };
 
 bool failAllocForTest()
@@ -54,9 +71,22 @@ struct stk_ctx {
     // render state
     stk::PathState ps;
     std::vector<DevBuf> psBufs;
-    DevBuf accum, history, out, samples;
-    uint32_t width = 0, height = 0, iteration = 0;
-    size_t samplesCount = 0;
+    DevBuf accum, history, out, samples, descs;
+    uint32_t iteration = 0;
+    uint64_t imageKey = 0;
+    size_t imagePixels = 0, samplesCount = 0;
+    stk::LaunchConfig launch;
+    bool useGraph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    std::vector<uint64_t> graphKey;
+
+    // batched scenes: staged host scenes and the merged device forest
+    std::vector<BatchScene> batch;
+    bool batchReady = false;
+    std::vector<stk::SceneDesc> batchDescs;
+    DevBuf fVb, fIb, fInst, fLights, fMats, fPacked, fTris, fRefNodes;
+    uint32_t fNumMats = 0;
 
     // radiance cache
     stk::CacheBuffers cache;
@@ -69,7 +99,8 @@ struct stk_ctx {
     std::vector<DevBuf*> owned()
     {
         return {&vb, &ib, &inst, &lights, &mats, &packed, &tris, &refNodes, &rayScratch, &hitScratch,
-                &accum, &history, &out, &samples, &cGridH, &cGridP, &cGridM, &cGridV, &cGridG, &cMlpP,
+                &accum, &history, &out, &samples, &descs, &fVb, &fIb, &fInst, &fLights, &fMats, &fPacked,
+                &fTris, &fRefNodes, &cGridH, &cGridP, &cGridM, &cGridV, &cGridG, &cMlpP,
                 &cMlpM, &cMlpV, &cMlpG, &cFwd, &cBwd, &cXT, &cDyT, &cLoss, &cGridE, &cGridHE, &cMlpE,
                 &cFwdE, &cBwdE, &queryRecs, &queryOut, &trainVerts, &trainCount, &trainBatch};
     }
@@ -161,6 +192,15 @@ stk_status uploadBvh(stk_ctx* c)
 
 bool validCtx(stk_ctx* c) { return c != nullptr; }
 
+void invalidateGraph(stk_ctx* c)
+{
+    if (c->graphExec) (void)hipGraphExecDestroy(c->graphExec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graphExec = nullptr;
+    c->graph = nullptr;
+    c->graphKey.clear();
+}
+
 } // namespace
 
 extern "C" {
@@ -196,6 +236,7 @@ STK_API void stk_destroy(stk_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : c->owned()) devRelease(c, *b);
     for (DevBuf& b : c->psBufs) devRelease(c, b);
+    invalidateGraph(c);
     for (auto& ev : c->events)
         if (ev) (void)hipEventDestroy(ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -386,88 +427,117 @@ STK_API stk_status stk_generate_camera_rays_device(stk_ctx* c, const StkPathTrac
     return STK_OK;
 }
 
-static stk_status ensureRenderBuffers(stk_ctx* c, uint32_t W, uint32_t H, size_t nsamples)
+// ---- rendering ------------------------------------------------------------------
+
+static stk_status ensurePathState(stk_ctx* c, size_t slots)
 {
-    const size_t npix = (size_t)W * H;
+    if (c->ps.capacity >= slots) return STK_OK;
+    invalidateGraph(c);
     stk_status st;
-    if (c->width != W || c->height != H) {
+    for (DevBuf& b : c->psBufs) devRelease(c, b);
+    c->psBufs.assign(19, DevBuf{});
+    stk::PathState& ps = c->ps;
+    const size_t f4 = slots * sizeof(float4), u4 = slots * sizeof(uint32_t);
+    void** ptrs[] = {(void**)&ps.rayO,     (void**)&ps.rayD,     (void**)&ps.thr,     (void**)&ps.col,
+                     (void**)&ps.rng,      (void**)&ps.scene,    (void**)&ps.pixel,   (void**)&ps.hit,
+                     (void**)&ps.hitPrim,  (void**)&ps.queue[0], (void**)&ps.queue[1], (void**)&ps.shadowQ,
+                     (void**)&ps.shO,      (void**)&ps.shD,      (void**)&ps.shCd,    (void**)&ps.shCg,
+                     (void**)&ps.counters, (void**)&ps.pass,     nullptr};
+    const size_t sizes[] = {f4, f4, f4, f4, u4, u4, u4, f4, u4, u4, u4, u4, f4, f4, f4, f4, 64, 64};
+    for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); ++i) {
+        if ((st = devReserve(c, c->psBufs[i], sizes[i])) != STK_OK) {
+            ps.capacity = 0;
+            return st;
+        }
+        *ptrs[i] = c->psBufs[i].ptr;
+    }
+    ps.capacity = (uint32_t)slots;
+    return STK_OK;
+}
+
+// Image buffers; the accumulation restarts when the image layout changes.
+static stk_status ensureImage(stk_ctx* c, uint64_t key, size_t npix, size_t nsamples)
+{
+    stk_status st;
+    const size_t before[4] = {(size_t)c->history.ptr, (size_t)c->accum.ptr, (size_t)c->out.ptr,
+                              (size_t)c->samples.ptr};
+    if (c->imageKey != key) {
         c->iteration = 0;
-        c->width = W;
-        c->height = H;
+        c->imageKey = key;
         if ((st = devReserve(c, c->history, npix * sizeof(float4))) != STK_OK) return st;
         STK_HIP(c, stk::launchZero(static_cast<float4*>(c->history.ptr), npix, c->stream));
     }
     if ((st = devReserve(c, c->accum, npix * sizeof(float4))) != STK_OK) return st;
     if ((st = devReserve(c, c->out, npix * sizeof(float4))) != STK_OK) return st;
-    if (nsamples) {
-        if ((st = devReserve(c, c->samples, nsamples * sizeof(float))) != STK_OK) return st;
-    }
+    if (nsamples && (st = devReserve(c, c->samples, nsamples * sizeof(float))) != STK_OK) return st;
     c->samplesCount = nsamples;
-    if (c->ps.capacity < npix) {
-        for (DevBuf& b : c->psBufs) devRelease(c, b);
-        c->psBufs.assign(17, DevBuf{});
-        stk::PathState& ps = c->ps;
-        const size_t f4 = npix * sizeof(float4), u4 = npix * sizeof(uint32_t);
-        void** ptrs[] = {(void**)&ps.rayO, (void**)&ps.rayD, (void**)&ps.thr, (void**)&ps.col, (void**)&ps.rng,
-                         (void**)&ps.hit, (void**)&ps.hitPrim, (void**)&ps.queue[0], (void**)&ps.queue[1],
-                         (void**)&ps.shadowQ, (void**)&ps.shO, (void**)&ps.shD, (void**)&ps.shCd, (void**)&ps.shCg,
-                         (void**)&ps.counters};
-        const size_t sizes[] = {f4, f4, f4, f4, u4, f4, u4, u4, u4, u4, f4, f4, f4, f4, 64};
-        for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); ++i) {
-            if ((st = devReserve(c, c->psBufs[i], sizes[i])) != STK_OK) {
-                ps.capacity = 0;
-                return st;
-            }
-            *ptrs[i] = c->psBufs[i].ptr;
-        }
-        ps.capacity = (uint32_t)npix;
-    }
+    c->imagePixels = npix;
+    const size_t after[4] = {(size_t)c->history.ptr, (size_t)c->accum.ptr, (size_t)c->out.ptr,
+                             (size_t)c->samples.ptr};
+    if (memcmp(before, after, sizeof(before)) != 0) invalidateGraph(c);
     return STK_OK;
 }
 
-STK_API stk_status stk_render(stk_ctx* c, const StkRenderParams* p, float* outRgba, int outIsDevice)
+// Renders p->pt.spp samples of every view in `descs` (one view for
+// stk_render, many for stk_batch_render) over the forest in `base`.
+static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vector<stk::SceneDesc>& descs,
+                               const StkRenderParams* p, float* outRgba, int outIsDevice, bool allowCache,
+                               uint64_t imageKey)
 {
-    if (!validCtx(c) || !p) return STK_E_INVALID;
-    if (!c->haveBvh) return fail(c, STK_E_STATE, "stk_render: no BVH");
-    const int32_t W = p->pt.dimension[0], H = p->pt.dimension[1];
-    if (W <= 0 || H <= 0 || p->pt.spp <= 0 || p->pt.maxDepth > 64)
-        return fail(c, STK_E_INVALID, "stk_render: bad dimension, spp or maxDepth");
-    if ((uint64_t)W * (uint64_t)H > (1ull << 28)) return fail(c, STK_E_INVALID, "stk_render: image too large");
+    stk_status st;
+    if (p->pt.spp <= 0 || p->pt.maxDepth > 64) return fail(c, STK_E_INVALID, "render: bad spp or maxDepth");
+    const uint32_t T = std::max<uint32_t>(1u, c->launch.tile);
+    size_t slots = 0, npix = 0;
+    for (stk::SceneDesc& d : descs) {
+        if (d.width == 0 || d.height == 0) return fail(c, STK_E_INVALID, "render: empty view");
+        d.tilesX = (d.width + T - 1) / T;
+        const size_t tilesY = (d.height + T - 1) / T;
+        d.slotBase = (uint32_t)slots;
+        d.pixelBase = (uint32_t)npix;
+        d.cam.dimension[0] = (int32_t)d.width;
+        d.cam.dimension[1] = (int32_t)d.height;
+        d.cam.spp = p->pt.spp;
+        d.cam.stratifiedSamplingType = p->pt.stratifiedSamplingType;
+        slots += (size_t)d.tilesX * T * tilesY * T;
+        npix += (size_t)d.width * d.height;
+    }
+    if (slots > (1ull << 30) || npix > (1ull << 29)) return fail(c, STK_E_INVALID, "render: image too large");
     (void)hipSetDevice(c->device);
-    const size_t npix = (size_t)W * H;
     const size_t nsamples = p->keepSamples ? npix * (size_t)p->pt.spp * 3 : 0;
-    stk_status st = ensureRenderBuffers(c, (uint32_t)W, (uint32_t)H, nsamples);
-    if (st != STK_OK) return st;
-    stk::SceneView sv;
-    sv.bvh = bvhView(c);
-    sv.vb = static_cast<const StkVertex*>(c->vb.ptr);
-    sv.ib = static_cast<const uint32_t*>(c->ib.ptr);
-    sv.inst = static_cast<const StkInstance*>(c->inst.ptr);
-    sv.lights = static_cast<const StkLight*>(c->lights.ptr);
-    sv.numLights = c->numLights;
-    sv.mats = static_cast<const StkMaterial*>(c->mats.ptr);
-    sv.numMats = c->numMats;
-    stk::PtLaunch lp;
-    lp.pt = p->pt;
-    for (int i = 0; i < 4; ++i) lp.sky[i] = p->skyColor[i];
+    if ((st = ensurePathState(c, slots)) != STK_OK) return st;
+    if ((st = ensureImage(c, imageKey, npix, nsamples)) != STK_OK) return st;
+    const size_t descBytes = descs.size() * sizeof(stk::SceneDesc);
+    if (c->descs.bytes < descBytes) invalidateGraph(c);
+    if ((st = devUpload(c, c->descs, descs.data(), descBytes)) != STK_OK) return st;
+    stk::SceneView sv = base;
+    sv.scenes = static_cast<const stk::SceneDesc*>(c->descs.ptr);
+    sv.numScenes = (uint32_t)descs.size();
+    sv.totalSlots = (uint32_t)slots;
+    sv.tile = T;
+    stk::PtLaunch lp{};
+    lp.maxDepth = p->pt.maxDepth;
+    lp.spp = (uint32_t)p->pt.spp;
+    lp.samplingType = p->pt.stratifiedSamplingType;
     lp.rrStartDepth = p->rrStartDepth;
     lp.refCompat = p->refCompat;
     lp.invSpp = 1.0f / (float)p->pt.spp;
     stk::PtCache& pc = lp.cache;
-    pc = stk::PtCache{};
     if (p->cacheMode != 0) {
-        if (!c->haveCache) return fail(c, STK_E_STATE, "stk_render: cacheMode set but no cache (stk_cache_create)");
-        if (p->cacheMode > 3) return fail(c, STK_E_INVALID, "stk_render: cacheMode must be 0..3");
+        if (!allowCache) return fail(c, STK_E_INVALID, "render: the radiance cache serves single-scene renders");
+        if (!c->haveCache) return fail(c, STK_E_STATE, "render: cacheMode set but no cache (stk_cache_create)");
+        if (p->cacheMode > 3) return fail(c, STK_E_INVALID, "render: cacheMode must be 0..3");
         pc.mode = p->cacheMode;
         pc.queryDepth = p->cacheQueryDepth;
         const uint32_t paths = p->cacheTrainPaths ? p->cacheTrainPaths : 65536u;
-        pc.trainStride = std::max<uint32_t>(1u, (uint32_t)(npix / paths));
-        const size_t maxTrain = (npix + pc.trainStride - 1) / pc.trainStride;
-        pc.trainCap = (uint32_t)std::min<size_t>(c->cache.nCap, maxTrain * p->pt.maxDepth);
-        if ((st = devReserve(c, c->queryRecs, npix * sizeof(stk::CacheRecord))) != STK_OK) return st;
-        if ((st = devReserve(c, c->queryOut, npix * sizeof(float4))) != STK_OK) return st;
-        if ((st = devReserve(c, c->trainVerts, maxTrain * p->pt.maxDepth * 5 * sizeof(float4))) != STK_OK) return st;
-        if ((st = devReserve(c, c->trainCount, maxTrain * sizeof(uint32_t))) != STK_OK) return st;
+        pc.trainStride = std::max<uint32_t>(1u, (uint32_t)(slots / paths));
+        pc.maxTrainPaths = (uint32_t)((slots + pc.trainStride - 1) / pc.trainStride);
+        pc.trainCap = (uint32_t)std::min<size_t>(c->cache.nCap, (size_t)pc.maxTrainPaths * p->pt.maxDepth);
+        if ((st = devReserve(c, c->queryRecs, slots * sizeof(stk::CacheRecord))) != STK_OK) return st;
+        if ((st = devReserve(c, c->queryOut, slots * sizeof(float4))) != STK_OK) return st;
+        if ((st = devReserve(c, c->trainVerts, (size_t)pc.maxTrainPaths * p->pt.maxDepth * 5 * sizeof(float4))) !=
+            STK_OK)
+            return st;
+        if ((st = devReserve(c, c->trainCount, (size_t)pc.maxTrainPaths * sizeof(uint32_t))) != STK_OK) return st;
         if ((st = devReserve(c, c->trainBatch, (size_t)pc.trainCap * sizeof(stk::CacheRecord))) != STK_OK) return st;
         pc.queryRecs = static_cast<stk::CacheRecord*>(c->queryRecs.ptr);
         pc.queryOut = static_cast<float4*>(c->queryOut.ptr);
@@ -478,19 +548,51 @@ STK_API stk_status stk_render(stk_ctx* c, const StkRenderParams* p, float* outRg
     float4* accum = static_cast<float4*>(c->accum.ptr);
     STK_HIP(c, stk::launchZero(accum, npix, c->stream));
     float* samples = nsamples ? static_cast<float*>(c->samples.ptr) : nullptr;
+    // One sample pass is captured once as a graph and replayed per sample (the
+    // pass number and frame come from device memory).  Training passes of the
+    // cache keep host-side Adam state and run eagerly.
+    const bool graph = c->useGraph && pc.mode == 0;
+    if (graph) {
+        // everything the captured kernels take by value
+        std::vector<uint64_t> key = {(uint64_t)sv.bvh.packed, (uint64_t)sv.bvh.tris, (uint64_t)sv.bvh.refNodes,
+                                     (uint64_t)sv.bvh.instances, (uint64_t)sv.vb, (uint64_t)sv.ib,
+                                     (uint64_t)sv.inst, (uint64_t)sv.lights, (uint64_t)sv.mats, sv.numMats,
+                                     (uint64_t)sv.scenes, sv.numScenes, sv.totalSlots, sv.tile, lp.maxDepth,
+                                     lp.spp, lp.samplingType, lp.rrStartDepth, lp.refCompat,
+                                     c->launch.blockSize, c->launch.tile, c->launch.persistentBlocks,
+                                     (uint64_t)accum, (uint64_t)samples, (uint64_t)c->ps.rayO, c->ps.capacity};
+        if (!c->graphExec || key != c->graphKey) {
+            invalidateGraph(c);
+            STK_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = stk::launchPathTracePass(sv, lp, c->launch, c->ps, accum, samples, nullptr, c->stream);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return hipFail(c, e != hipSuccess ? e : e2, "graph capture of a path-tracing pass");
+            }
+            c->graph = g;
+            STK_HIP(c, hipGraphInstantiate(&c->graphExec, c->graph, nullptr, nullptr, 0));
+            c->graphKey = key;
+        }
+    }
     for (int s = 0; s < p->pt.spp; ++s) {
-        lp.sampleNum = (uint32_t)s;
+        uint32_t trainOffset = 0, numTrain = 0;
         if (pc.mode != 0) {
-            // a different subset of pixels trains each pass
+            // a different subset of paths trains each pass
             uint32_t h = p->pt.frameNumber * (uint32_t)p->pt.spp + (uint32_t)s;
             h ^= h >> 16;
             h *= 0x7feb352du;
             h ^= h >> 15;
-            pc.trainOffset = h % pc.trainStride;
-            pc.numTrainPaths = (uint32_t)((npix - pc.trainOffset + pc.trainStride - 1) / pc.trainStride);
+            trainOffset = h % pc.trainStride;
+            numTrain = (uint32_t)((slots - trainOffset + pc.trainStride - 1) / pc.trainStride);
         }
-        STK_HIP(c, stk::launchPathTracePass(sv, lp, c->ps, accum, samples, c->haveCache ? &c->cache : nullptr,
-                                            c->stream));
+        STK_HIP(c, stk::launchSetPass(c->ps, p->pt.frameNumber, (uint32_t)s, trainOffset, numTrain, c->stream));
+        if (graph)
+            STK_HIP(c, hipGraphLaunch(c->graphExec, c->stream));
+        else
+            STK_HIP(c, stk::launchPathTracePass(sv, lp, c->launch, c->ps, accum, samples,
+                                                c->haveCache ? &c->cache : nullptr, c->stream));
     }
     const int acc = p->enableAccumulation != 0;
     STK_HIP(c, stk::launchAccumulateTonemap(accum, static_cast<float4*>(c->history.ptr),
@@ -503,6 +605,310 @@ STK_API stk_status stk_render(stk_ctx* c, const StkRenderParams* p, float* outRg
         STK_HIP(c, hipMemcpyAsync(outRgba, c->out.ptr, npix * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
         STK_HIP(c, hipStreamSynchronize(c->stream));
     }
+    return STK_OK;
+}
+
+static stk::SceneView singleSceneView(stk_ctx* c)
+{
+    stk::SceneView sv{};
+    sv.bvh = bvhView(c);
+    sv.vb = static_cast<const StkVertex*>(c->vb.ptr);
+    sv.ib = static_cast<const uint32_t*>(c->ib.ptr);
+    sv.inst = static_cast<const StkInstance*>(c->inst.ptr);
+    sv.lights = static_cast<const StkLight*>(c->lights.ptr);
+    sv.mats = static_cast<const StkMaterial*>(c->mats.ptr);
+    sv.numMats = c->numMats;
+    return sv;
+}
+
+STK_API stk_status stk_render(stk_ctx* c, const StkRenderParams* p, float* outRgba, int outIsDevice)
+{
+    if (!validCtx(c) || !p) return STK_E_INVALID;
+    if (!c->haveBvh) return fail(c, STK_E_STATE, "stk_render: no BVH");
+    const int32_t W = p->pt.dimension[0], H = p->pt.dimension[1];
+    if (W <= 0 || H <= 0 || p->pt.spp <= 0 || p->pt.maxDepth > 64)
+        return fail(c, STK_E_INVALID, "stk_render: bad dimension, spp or maxDepth");
+    if ((uint64_t)W * (uint64_t)H > (1ull << 28)) return fail(c, STK_E_INVALID, "stk_render: image too large");
+    std::vector<stk::SceneDesc> descs(1);
+    stk::SceneDesc& d = descs[0];
+    memset(&d, 0, sizeof(d));
+    d.hdr = c->bvh.header;
+    d.refRoot = 0;
+    d.lightBase = 0;
+    d.lightCount = c->numLights;
+    d.width = (uint32_t)W;
+    d.height = (uint32_t)H;
+    for (int i = 0; i < 4; ++i) d.sky[i] = p->skyColor[i];
+    d.cam = p->pt;
+    const uint64_t key = ((uint64_t)W << 32) ^ (uint64_t)H ^ (1ull << 63);
+    return renderForest(c, singleSceneView(c), descs, p, outRgba, outIsDevice, true, key);
+}
+
+// ---- batched scenes -----------------------------------------------------------
+
+STK_API stk_status stk_batch_clear(stk_ctx* c)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    c->batch.clear();
+    c->batchDescs.clear();
+    c->batchReady = false;
+    return STK_OK;
+}
+
+STK_API stk_status stk_batch_add_scene(stk_ctx* c, const StkBatchScene* s, uint32_t* index)
+{
+    if (!validCtx(c) || !s) return STK_E_INVALID;
+    if ((s->numVertices && !s->vb) || (s->numIndices && !s->ib) || (s->numInstances && !s->inst) ||
+        (s->numLights && !s->lights) || (s->numMaterials && !s->mats) || s->numIndices % 3 != 0)
+        return fail(c, STK_E_INVALID, "stk_batch_add_scene: null buffer or index count not a multiple of 3");
+    if (s->camera.dimension[0] <= 0 || s->camera.dimension[1] <= 0 ||
+        (uint64_t)s->camera.dimension[0] * (uint64_t)s->camera.dimension[1] > (1ull << 26))
+        return fail(c, STK_E_INVALID, "stk_batch_add_scene: bad view size");
+    if (c->batch.size() >= 4096) return fail(c, STK_E_INVALID, "stk_batch_add_scene: at most 4096 scenes");
+    for (uint32_t i = 0; i < s->numInstances; ++i) {
+        const StkInstance& ic = s->inst[i];
+        if (ic.indexOffset < 0 || ic.indexCount < 0 || ic.indexCount % 3 ||
+            (uint64_t)ic.indexOffset + (uint64_t)ic.indexCount > s->numIndices)
+            return fail(c, STK_E_INVALID, "stk_batch_add_scene: instance index range outside the index buffer");
+        if (ic.lightId >= 0 && (uint32_t)ic.lightId >= s->numLights)
+            return fail(c, STK_E_INVALID, "stk_batch_add_scene: instance lightId outside the scene's lights");
+    }
+    for (uint32_t i = 0; i < s->numIndices; ++i)
+        if (s->ib[i] >= s->numVertices) return fail(c, STK_E_INVALID, "stk_batch_add_scene: index out of range");
+    for (uint32_t i = 0; i < s->numLights; ++i)
+        if (s->lights[i].type != 0) return fail(c, STK_E_INVALID, "stk_batch_add_scene: only rect lights (type 0)");
+    try {
+        BatchScene b;
+        b.vb.assign(s->vb, s->vb + s->numVertices);
+        b.ib.assign(s->ib, s->ib + s->numIndices);
+        b.inst.assign(s->inst, s->inst + s->numInstances);
+        b.lights.assign(s->lights, s->lights + s->numLights);
+        b.mats.assign(s->mats, s->mats + s->numMaterials);
+        b.cam = s->camera;
+        memcpy(b.sky, s->sky, sizeof(b.sky));
+        c->batch.push_back(std::move(b));
+    } catch (const std::bad_alloc&) {
+        return fail(c, STK_E_OOM, "stk_batch_add_scene: host allocation failed");
+    }
+    c->batchReady = false;
+    if (index) *index = (uint32_t)c->batch.size() - 1;
+    return STK_OK;
+}
+
+// Builds every staged scene's BVH (scenes in parallel, largest first) and
+// uploads the merged forest: each scene's vertices, indices, instances,
+// lights, materials, nodes and triangles are appended to one array per kind
+// with every cross reference rebased by the offsets of the scenes before it.
+STK_API stk_status stk_batch_commit(stk_ctx* c, const StkBvhOptions* opts)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    if (c->batch.empty()) return fail(c, STK_E_STATE, "stk_batch_commit: no scenes added");
+    const StkBvhOptions o = opts ? *opts : stk::defaultBvhOptions();
+    const size_t n = c->batch.size();
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(),
+              [&](size_t a, size_t b) { return c->batch[a].ib.size() > c->batch[b].ib.size(); });
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+        for (size_t k; (k = next.fetch_add(1)) < n;) {
+            BatchScene& b = c->batch[order[k]];
+            try {
+                std::vector<stk::BvhInputTri> tris;
+                b.bvh = stk::Bvh{};
+                if (!stk::flattenInstances(b.vb.data(), (uint32_t)b.vb.size(), b.ib.data(), (uint32_t)b.ib.size(),
+                                           b.inst.data(), (uint32_t)b.inst.size(), tris, b.err) ||
+                    !stk::buildBvh(tris, o, b.bvh, b.err) ||
+                    !stk::packBvh(b.vb.data(), (uint32_t)b.vb.size(), b.ib.data(), (uint32_t)b.ib.size(),
+                                  b.inst.data(), (uint32_t)b.inst.size(), b.bvh, b.err))
+                    continue;
+                if (b.bvh.depth > (uint32_t)stk::dev::kStackDepth) b.err = "tree deeper than the traversal stack";
+            } catch (const std::bad_alloc&) {
+                b.err = "host allocation failed";
+            }
+        }
+    };
+    const size_t nth = std::min<size_t>(n, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nth; ++t) pool.emplace_back(work);
+    work();
+    for (std::thread& t : pool) t.join();
+    for (size_t i = 0; i < n; ++i)
+        if (!c->batch[i].err.empty())
+            return fail(c, STK_E_INVALID, "stk_batch_commit: scene " + std::to_string(i) + ": " + c->batch[i].err);
+
+    size_t nv = 0, ni = 0, nin = 0, nl = 0, nm = 0, nn = 0, np = 0, nt = 0;
+    for (const BatchScene& b : c->batch) {
+        nv += b.vb.size();
+        ni += b.ib.size();
+        nin += b.inst.size();
+        nl += b.lights.size();
+        nm += b.mats.size();
+        nn += b.bvh.nodes.size();
+        np += b.bvh.packed.size();
+        nt += b.bvh.tris.size();
+    }
+    if (nv >= (1ull << 31) || ni >= (1ull << 31) || nn >= (1ull << 31) || np >= (1ull << 31) || nt >= (1ull << 31))
+        return fail(c, STK_E_INVALID, "stk_batch_commit: merged scene exceeds 2^31 elements");
+    std::vector<StkVertex> vb;
+    std::vector<uint32_t> ib;
+    std::vector<StkInstance> inst;
+    std::vector<StkLight> lights;
+    std::vector<StkMaterial> mats;
+    std::vector<StkBvhNode> nodes;
+    std::vector<StkPackedNode> packed;
+    std::vector<StkTriangle> tris;
+    std::vector<stk::SceneDesc> descs(n);
+    try {
+        vb.reserve(nv);
+        ib.reserve(ni);
+        inst.reserve(nin);
+        lights.reserve(nl);
+        mats.reserve(nm);
+        nodes.reserve(nn);
+        packed.reserve(np);
+        tris.reserve(nt);
+        for (size_t i = 0; i < n; ++i) {
+            const BatchScene& b = c->batch[i];
+            const uint32_t vBase = (uint32_t)vb.size(), iBase = (uint32_t)ib.size(), instBase = (uint32_t)inst.size();
+            const uint32_t lBase = (uint32_t)lights.size(), mBase = (uint32_t)mats.size();
+            const uint32_t nBase = (uint32_t)nodes.size(), pBase = (uint32_t)packed.size();
+            const uint32_t tBase = (uint32_t)tris.size();
+            vb.insert(vb.end(), b.vb.begin(), b.vb.end());
+            for (uint32_t x : b.ib) ib.push_back(x + vBase);
+            for (StkInstance ic : b.inst) {
+                ic.indexOffset += (int32_t)iBase;
+                ic.materialId = (ic.materialId >= 0 && (size_t)ic.materialId < b.mats.size())
+                                    ? ic.materialId + (int32_t)mBase
+                                    : -1;
+                if (ic.lightId >= 0) ic.lightId += (int32_t)lBase;
+                inst.push_back(ic);
+            }
+            lights.insert(lights.end(), b.lights.begin(), b.lights.end());
+            mats.insert(mats.end(), b.mats.begin(), b.mats.end());
+            // skip-link nodes: device copy carries the triangle index in maxBounds[0]
+            for (size_t k = 0; k < b.bvh.nodes.size(); ++k) {
+                StkBvhNode nd = b.bvh.nodes[k];
+                if ((uint32_t)nd.instId != STK_INVALID_INDEX) {
+                    nd.instId += (int32_t)instBase;
+                    nd.maxBounds[0] = stk::bits_to_float(b.bvh.leafToTri[k] + tBase);
+                }
+                if ((uint32_t)nd.nodeOffset != STK_INVALID_INDEX) nd.nodeOffset += (int32_t)nBase;
+                nodes.push_back(nd);
+            }
+            for (StkPackedNode pn : b.bvh.packed) {
+                for (int ch = 0; ch < 2; ++ch)
+                    pn.child[ch] = pn.child[ch] >= 0 ? pn.child[ch] + (int32_t)pBase : ~(~pn.child[ch] + (int32_t)tBase);
+                packed.push_back(pn);
+            }
+            for (StkTriangle t : b.bvh.tris) {
+                t.leaf += nBase;
+                t.instId += instBase;
+                tris.push_back(t);
+            }
+            stk::SceneDesc& d = descs[i];
+            memset(&d, 0, sizeof(d));
+            d.hdr = b.bvh.header;
+            if (d.hdr.root >= 0)
+                d.hdr.root += (int32_t)pBase;
+            else if (d.hdr.root != INT32_MIN)
+                d.hdr.root = ~(~d.hdr.root + (int32_t)tBase);
+            d.refRoot = b.bvh.nodes.empty() ? STK_INVALID_INDEX : nBase;
+            d.lightBase = lBase;
+            d.lightCount = (uint32_t)b.lights.size();
+            d.width = (uint32_t)b.cam.dimension[0];
+            d.height = (uint32_t)b.cam.dimension[1];
+            memcpy(d.sky, b.sky, sizeof(d.sky));
+            d.cam = b.cam;
+        }
+    } catch (const std::bad_alloc&) {
+        return fail(c, STK_E_OOM, "stk_batch_commit: host allocation failed");
+    }
+    (void)hipSetDevice(c->device);
+    stk_status st;
+    if ((st = devUpload(c, c->fVb, vb.data(), vb.size() * sizeof(StkVertex))) != STK_OK) return st;
+    if ((st = devUpload(c, c->fIb, ib.data(), ib.size() * sizeof(uint32_t))) != STK_OK) return st;
+    if ((st = devUpload(c, c->fInst, inst.data(), inst.size() * sizeof(StkInstance))) != STK_OK) return st;
+    if ((st = devUpload(c, c->fLights, lights.data(), lights.size() * sizeof(StkLight))) != STK_OK) return st;
+    if ((st = devUpload(c, c->fMats, mats.data(), mats.size() * sizeof(StkMaterial))) != STK_OK) return st;
+    if ((st = devUpload(c, c->fRefNodes, nodes.data(), nodes.size() * sizeof(StkBvhNode))) != STK_OK) return st;
+    if ((st = devUpload(c, c->fPacked, packed.data(), packed.size() * sizeof(StkPackedNode))) != STK_OK) return st;
+    if ((st = devUpload(c, c->fTris, tris.data(), tris.size() * sizeof(StkTriangle))) != STK_OK) return st;
+    c->fNumMats = (uint32_t)mats.size();
+    c->batchDescs = std::move(descs);
+    c->batchReady = true;
+    // the host node arrays are rebuilt by the next commit; keep only the header
+    for (BatchScene& b : c->batch) {
+        std::vector<StkBvhNode>().swap(b.bvh.nodes);
+        std::vector<StkPackedNode>().swap(b.bvh.packed);
+        std::vector<StkTriangle>().swap(b.bvh.tris);
+        std::vector<uint32_t>().swap(b.bvh.leafToTri);
+    }
+    return STK_OK;
+}
+
+STK_API stk_status stk_batch_info(stk_ctx* c, uint32_t scene, StkBatchViewInfo* out)
+{
+    if (!validCtx(c) || !out) return STK_E_INVALID;
+    if (!c->batchReady) return fail(c, STK_E_STATE, "stk_batch_info: batch not committed");
+    if (scene >= c->batchDescs.size()) return fail(c, STK_E_INVALID, "stk_batch_info: scene out of range");
+    const stk::SceneDesc& d = c->batchDescs[scene];
+    uint32_t pix = 0;
+    for (uint32_t i = 0; i < scene; ++i) pix += c->batchDescs[i].width * c->batchDescs[i].height;
+    out->pixelBase = pix;
+    out->width = d.width;
+    out->height = d.height;
+    out->numTriangles = d.hdr.numTriangles;
+    out->bvhDepth = c->batch[scene].bvh.depth;
+    out->numScenes = (uint32_t)c->batchDescs.size();
+    out->pad[0] = out->pad[1] = 0;
+    return STK_OK;
+}
+
+STK_API stk_status stk_batch_render(stk_ctx* c, const StkRenderParams* p, float* outRgba, int outIsDevice)
+{
+    if (!validCtx(c) || !p) return STK_E_INVALID;
+    if (!c->batchReady) return fail(c, STK_E_STATE, "stk_batch_render: batch not committed");
+    std::vector<stk::SceneDesc> descs = c->batchDescs;
+    for (stk::SceneDesc& d : descs) d.cam.frameNumber = p->pt.frameNumber;
+    stk::SceneView sv{};
+    sv.bvh.packed = static_cast<const float4*>(c->fPacked.ptr);
+    sv.bvh.tris = static_cast<const float4*>(c->fTris.ptr);
+    sv.bvh.refNodes = static_cast<const StkBvhNode*>(c->fRefNodes.ptr);
+    sv.bvh.instances = static_cast<const StkInstance*>(c->fInst.ptr);
+    sv.vb = static_cast<const StkVertex*>(c->fVb.ptr);
+    sv.ib = static_cast<const uint32_t*>(c->fIb.ptr);
+    sv.inst = static_cast<const StkInstance*>(c->fInst.ptr);
+    sv.lights = static_cast<const StkLight*>(c->fLights.ptr);
+    sv.mats = static_cast<const StkMaterial*>(c->fMats.ptr);
+    sv.numMats = c->fNumMats;
+    const uint64_t key = (uint64_t)(uintptr_t)c->fPacked.ptr ^ ((uint64_t)descs.size() << 48);
+    return renderForest(c, sv, descs, p, outRgba, outIsDevice, false, key);
+}
+
+STK_API stk_status stk_set_launch_config(stk_ctx* c, const StkLaunchConfig* lc)
+{
+    if (!validCtx(c) || !lc) return STK_E_INVALID;
+    if (lc->blockSize != 64 && lc->blockSize != 128 && lc->blockSize != 256)
+        return fail(c, STK_E_INVALID, "stk_set_launch_config: blockSize must be 64, 128 or 256");
+    if (lc->tile == 0 || lc->tile > 64) return fail(c, STK_E_INVALID, "stk_set_launch_config: tile must be 1..64");
+    if (lc->persistentBlocks > (1u << 20))
+        return fail(c, STK_E_INVALID, "stk_set_launch_config: persistentBlocks too large");
+    c->launch.blockSize = lc->blockSize;
+    c->launch.tile = lc->tile;
+    c->launch.persistentBlocks = lc->persistentBlocks;
+    c->useGraph = lc->useGraph != 0;
+    invalidateGraph(c);
+    return STK_OK;
+}
+
+STK_API stk_status stk_get_launch_config(stk_ctx* c, StkLaunchConfig* lc)
+{
+    if (!validCtx(c) || !lc) return STK_E_INVALID;
+    lc->blockSize = c->launch.blockSize;
+    lc->tile = c->launch.tile;
+    lc->persistentBlocks = c->launch.persistentBlocks;
+    lc->useGraph = c->useGraph ? 1u : 0u;
     return STK_OK;
 }
 
@@ -526,7 +932,7 @@ STK_API stk_status stk_get_samples(stk_ctx* c, float* out, size_t count)
 STK_API stk_status stk_get_accumulation(stk_ctx* c, float* out, size_t count)
 {
     if (!validCtx(c) || !out) return STK_E_INVALID;
-    if (count > (size_t)c->width * c->height * 4 || !c->history.ptr)
+    if (count > c->imagePixels * 4 || !c->history.ptr)
         return fail(c, STK_E_STATE, "stk_get_accumulation: nothing rendered or count too large");
     STK_HIP(c, hipMemcpyAsync(out, c->history.ptr, count * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     STK_HIP(c, hipStreamSynchronize(c->stream));

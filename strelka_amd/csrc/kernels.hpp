@@ -1,4 +1,5 @@
-// kernels.hpp - host-side launchers of the HIP kernels (kernels.hip).
+// kernels.hpp - host-side launchers of the HIP kernels (kernels.hip,
+// pathtrace.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,7 +10,46 @@
 
 namespace stk {
 
-// Device arrays of one wavefront path-tracing pass (one sample of every pixel).
+// One scene of a (possibly batched) render: the scenes' BVHs, triangles,
+// instances, lights and materials are concatenated into one forest; this
+// record locates a scene's part and carries its view.
+struct SceneDesc {
+    StkBvhHeader hdr;        // root rebased into the merged packed-node array
+    uint32_t refRoot;        // first node of the scene in the merged skip-link array
+    uint32_t lightBase, lightCount;
+    uint32_t pixelBase;      // first pixel of the view in the merged image
+    uint32_t slotBase;       // first path slot (views are padded to whole tiles)
+    uint32_t width, height;
+    uint32_t tilesX;         // tiles per row for this view's tile edge
+    float sky[4];
+    StkPathTracerParam cam;  // camera of the view (dimension = width, height)
+};
+
+struct SceneView {
+    BvhView bvh; // merged forest (bvh.hdr unused)
+    const StkVertex* vb;
+    const uint32_t* ib;
+    const StkInstance* inst;
+    const StkLight* lights;
+    const StkMaterial* mats;
+    uint32_t numMats;
+    const SceneDesc* scenes;
+    uint32_t numScenes;
+    uint32_t totalSlots;
+    uint32_t tile; // raygen tile edge in pixels (1 = scanline order)
+};
+
+// Per-pass values read from device memory, so that a captured graph of one
+// pass can be replayed for every sample and frame.
+struct PassState {
+    uint32_t frameNumber;
+    uint32_t sampleNum;
+    uint32_t trainOffset;
+    uint32_t numTrainPaths;
+};
+
+// Device arrays of one wavefront path-tracing pass (one sample of every pixel
+// of every view).
 struct PathState {
     uint32_t capacity = 0;
     float4* rayO = nullptr;  // origin.xyz, tmax
@@ -17,6 +57,8 @@ struct PathState {
     float4* thr = nullptr;   // throughput.rgb
     float4* col = nullptr;   // radiance.rgb
     uint32_t* rng = nullptr; // xorshift state
+    uint32_t* scene = nullptr; // scene of the slot
+    uint32_t* pixel = nullptr; // global pixel of the slot (0xFFFFFFFF: padding)
     float4* hit = nullptr;   // u, v, t, instId bits (queue order)
     uint32_t* hitPrim = nullptr;
     uint32_t* queue[2] = {nullptr, nullptr};
@@ -25,18 +67,8 @@ struct PathState {
     float4* shD = nullptr;
     float4* shCd = nullptr; // NEE contribution, diffuse part
     float4* shCg = nullptr; // NEE contribution, glossy part
-    uint32_t* counters = nullptr; // [0],[1] queue counts, [2] shadow count, [3..7] work counters
-};
-
-struct SceneView {
-    BvhView bvh;
-    const StkVertex* vb;
-    const uint32_t* ib;
-    const StkInstance* inst;
-    const StkLight* lights;
-    uint32_t numLights;
-    const StkMaterial* mats;
-    uint32_t numMats;
+    uint32_t* counters = nullptr; // [0],[1] queue counts, [2] shadow, [3] queries, [4] train records, [5] work
+    PassState* pass = nullptr;
 };
 
 // Radiance-cache hooks of one pass.  Training paths are the slots with
@@ -48,24 +80,30 @@ struct PtCache {
     uint32_t mode = 0; // 0 off, 1 query + train, 2 query only, 3 train only
     uint32_t queryDepth = 2;
     uint32_t trainStride = 1;
-    uint32_t trainOffset = 0;
     uint32_t trainCap = 0;
-    uint32_t numTrainPaths = 0;
-    CacheRecord* queryRecs = nullptr; // [npix]
-    float4* queryOut = nullptr;       // [npix]
-    float4* trainVerts = nullptr;     // [numTrainPaths * maxDepth * 5]
-    uint32_t* trainCount = nullptr;   // [numTrainPaths]
+    uint32_t maxTrainPaths = 0;
+    CacheRecord* queryRecs = nullptr; // [slots]
+    float4* queryOut = nullptr;       // [slots]
+    float4* trainVerts = nullptr;     // [maxTrainPaths * maxDepth * 5]
+    uint32_t* trainCount = nullptr;   // [maxTrainPaths]
     CacheRecord* trainBatch = nullptr; // [trainCap]
 };
 
 struct PtLaunch {
-    StkPathTracerParam pt;
-    float sky[4];
+    uint32_t maxDepth;
+    uint32_t spp;
+    uint32_t samplingType; // stratifiedSamplingType
     uint32_t rrStartDepth;
     uint32_t refCompat;
-    uint32_t sampleNum; // 0 .. spp-1
     float invSpp;
     PtCache cache;
+};
+
+// Launch shape of the wavefront kernels (chosen by the autotuner).
+struct LaunchConfig {
+    uint32_t blockSize = 256;      // 64, 128 or 256 threads
+    uint32_t tile = 8;             // raygen tile edge (1 = scanline)
+    uint32_t persistentBlocks = 0; // extend grid for persistent waves (0: one lane per ray)
 };
 
 hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, bool refOrder,
@@ -74,12 +112,16 @@ hipError_t launchTraceAny(const BvhView& bvh, const StkRay* rays, uint32_t n, St
                           hipStream_t s);
 hipError_t launchCameraRays(const StkPathTracerParam& p, uint32_t sampleNum, StkRay* rays, hipStream_t s);
 
-// One sample pass of the wavefront path tracer; accumulates invSpp * radiance
-// into accum (float4 per pixel) and optionally stores per-sample radiance.
-// With lp.cache.mode != 0, cb must be the context's cache: queries are answered
+// Writes the pass state (a kernel, so it can precede a graph replay).
+hipError_t launchSetPass(PathState& ps, uint32_t frame, uint32_t sampleNum, uint32_t trainOffset,
+                         uint32_t numTrainPaths, hipStream_t s);
+// One sample pass of the wavefront path tracer over every view: raygen, the
+// bounce loop and the finalize step, which adds invSpp * radiance into
+// accum[pixel] (and stores per-sample radiance when samples is set).  With
+// lp.cache.mode != 0, cb must be the context's cache: queries are answered
 // after the bounce loop and one training step runs on the pass's records.
-hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, PathState& ps, float4* accum,
-                               float* samples, CacheBuffers* cb, hipStream_t s);
+hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const LaunchConfig& lc, PathState& ps,
+                               float4* accum, float* samples, CacheBuffers* cb, hipStream_t s);
 // Plain 16-B-per-lane copy kernel (bytes must be a multiple of 16).
 hipError_t launchCopy(void* dst, const void* src, size_t bytes, hipStream_t s);
 // Zeroes n float4 (a kernel, so the call can be captured in a graph).

@@ -1,0 +1,471 @@
+// pathtrace.hip - wavefront path tracer for gfx950 over one or many views.
+//
+// Replaces the per-(pixel, sample) megakernel computeMain/pathTraceCameraRays
+// (shaders/pathtracerMdl.hlsl:204-423) with wavefront stages:
+//   reset -> raygen -> per bounce { setup, extend (closest hit), shade (NEE
+//   sample, BSDF sample, RR; compacts continuing paths and shadow rays into
+//   queues with one atomic per wave), connect (any hit, adds NEE) } ->
+//   [cache answers] -> finalize -> [cache training].
+// A path's sequence of random numbers and of additions to its radiance is the
+// reference's, so its value does not depend on the wavefront organisation or
+// on how many views share the launch.
+//
+// Many views: the scenes of all views live in one forest (concatenated node,
+// triangle, instance, light and material arrays); every path slot knows its
+// scene.  Slots are ordered by view, then by square tile, then row-major in
+// the tile, so a wave's rays are spatially coherent; views are padded to whole
+// tiles and the padding slots never enter a queue.
+#include "kernels.hpp"
+#include "pt_dev.hpp"
+
+namespace stk {
+namespace {
+
+using namespace dev;
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ float4 f4(F3 v) { return make_float4(v.x, v.y, v.z, 0.0f); }
+__device__ __forceinline__ F3 f3(float4 v) { return mk3(v.x, v.y, v.z); }
+
+// Appends lanes with pred set to a queue: one atomic per wave, order of lanes
+// kept within the wave.  Every active lane of the wave must call it.
+__device__ __forceinline__ uint32_t wave_push(uint32_t* counter, bool pred)
+{
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = (uint32_t)__shfl((int)base, (int)leader);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ uint32_t scene_of_slot(const SceneView& sc, uint32_t slot)
+{
+    uint32_t lo = 0, hi = sc.numScenes; // last scene whose slotBase <= slot
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (sc.scenes[mid].slotBase <= slot)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool is_train(const PtCache& pc, const PassState& pass, uint32_t slot)
+{
+    return (pc.mode == 1 || pc.mode == 3) && slot % pc.trainStride == pass.trainOffset;
+}
+
+__global__ void k_pass_set(PassState* pass, uint32_t frame, uint32_t sampleNum, uint32_t trainOffset,
+                           uint32_t numTrainPaths)
+{
+    if (threadIdx.x == 0) {
+        pass->frameNumber = frame;
+        pass->sampleNum = sampleNum;
+        pass->trainOffset = trainOffset;
+        pass->numTrainPaths = numTrainPaths;
+    }
+}
+
+__global__ void k_pass_reset(PathState ps)
+{
+    if (threadIdx.x < 8) ps.counters[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pt_raygen(SceneView sc, PtLaunch lp, PathState ps)
+{
+    const uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
+    bool valid = false;
+    if (slot < sc.totalSlots) {
+        const PassState pass = *ps.pass;
+        const uint32_t sid = scene_of_slot(sc, slot);
+        const SceneDesc& d = sc.scenes[sid];
+        const uint32_t T = sc.tile, local = slot - d.slotBase;
+        const uint32_t tileIdx = local / (T * T), within = local % (T * T);
+        const uint32_t px = (tileIdx % d.tilesX) * T + within % T;
+        const uint32_t py = (tileIdx / d.tilesX) * T + within / T;
+        valid = px < d.width && py < d.height;
+        ps.scene[slot] = sid;
+        ps.pixel[slot] = valid ? d.pixelBase + py * d.width + px : STK_INVALID_INDEX;
+        if (valid) {
+            const uint32_t frame = pass.frameNumber * lp.spp + pass.sampleNum;
+            uint32_t rng = init_rng(px, py, d.width, frame);
+            float4 o, dir;
+            camera_ray(d.cam, px, py, rng, frame, o, dir);
+            ps.rayO[slot] = o;
+            ps.rayD[slot] = dir;
+            ps.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+            ps.col[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ps.rng[slot] = rng;
+        } else {
+            ps.col[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        // padding slots too: the target kernel reads every training slot's count
+        if (is_train(lp.cache, pass, slot)) lp.cache.trainCount[slot / lp.cache.trainStride] = 0;
+    }
+    const uint32_t qi = wave_push(&ps.counters[0], valid);
+    if (valid) ps.queue[0][qi] = slot;
+}
+
+__global__ void k_pt_setup(PathState ps, int depth)
+{
+    if (threadIdx.x == 0) {
+        ps.counters[1 - (depth & 1)] = 0;
+        ps.counters[2] = 0;
+        ps.counters[5] = 0;
+    }
+}
+
+template <int BLOCK>
+__device__ __forceinline__ void extend_one(const SceneView& sc, PathState& ps, int cur, uint32_t q, uint32_t* stack)
+{
+    const uint32_t slot = ps.queue[cur][q];
+    const RayF r = make_ray(ps.rayO[slot], ps.rayD[slot]);
+    HitF h;
+    closest_packed(sc.bvh, sc.scenes[ps.scene[slot]].hdr, r, stack, BLOCK, h);
+    ps.hit[q] = make_float4(h.u, h.v, h.t, bits_to_float(h.inst));
+    ps.hitPrim[q] = h.prim;
+}
+
+// Closest hit of every queued path.  persistent = 0: one lane per queue
+// entry.  persistent = 1: a fixed grid of waves fetches 64 entries at a time
+// from a work counter until the queue is exhausted (every wave reaches the
+// exit test, the counter only grows).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_pt_extend(SceneView sc, PathState ps, int depth, int persistent)
+{
+    __shared__ uint32_t stack[kStackDepth * BLOCK];
+    const int cur = depth & 1;
+    const uint32_t count = ps.counters[cur];
+    if (!persistent) {
+        const uint32_t q = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+        if (q < count) extend_one<BLOCK>(sc, ps, cur, q, stack + threadIdx.x);
+        return;
+    }
+    const uint32_t lane = threadIdx.x % 64;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&ps.counters[5], 64u);
+        base = (uint32_t)__shfl((int)base, 0);
+        if (base >= count) break;
+        const uint32_t q = base + lane;
+        if (q < count) extend_one<BLOCK>(sc, ps, cur, q, stack + threadIdx.x);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, PathState ps, int depth)
+{
+    const int cur = depth & 1;
+    const uint32_t count = ps.counters[cur];
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    bool cont = false, shadow = false, cacheQuery = false;
+    uint32_t slot = 0;
+    F3 nO = mk3(0, 0, 0), nD = mk3(0, 0, 0);
+    F3 sO = mk3(0, 0, 0), sD = mk3(0, 0, 0), cd = mk3(0, 0, 0), cg = mk3(0, 0, 0);
+    float sT = 0.0f;
+    CacheRecord qrec;
+    const PtCache& pc = lp.cache;
+    if (q < count) {
+        slot = ps.queue[cur][q];
+        const SceneDesc& desc = sc.scenes[ps.scene[slot]];
+        const float4 h = ps.hit[q];
+        const uint32_t inst = float_to_bits(h.w);
+        F3 thr = f3(ps.thr[slot]);
+        F3 col = f3(ps.col[slot]);
+        uint32_t rng = ps.rng[slot];
+        if (inst == STK_INVALID_INDEX) {
+            col = col + thr * mk3(desc.sky[0], desc.sky[1], desc.sky[2]);
+            ps.col[slot] = f4(col);
+        } else {
+            const StkInstance& ic = sc.inst[inst];
+            if (ic.lightId != -1) {
+                // emitter: counted on camera rays (shaders/pathtracerMdl.hlsl:229-234);
+                // a BSDF-sampled emitter hit adds nothing and ends the path
+                if (depth == 0 && (uint32_t)ic.lightId < desc.lightBase + desc.lightCount) {
+                    col = col + thr * ld3(sc.lights[ic.lightId].color);
+                    ps.col[slot] = f4(col);
+                }
+            } else {
+                const uint32_t prim = ps.hitPrim[q];
+                const uint32_t base = (uint32_t)ic.indexOffset + prim * 3;
+                const uint32_t i0 = sc.ib[base], i1 = sc.ib[base + 1], i2 = sc.ib[base + 2];
+                const StkVertex &v0 = sc.vb[i0], &v1 = sc.vb[i1], &v2 = sc.vb[i2];
+                const F3 p0 = xform4(ic.objectToWorld, ld3(v0.position), 1.0f);
+                const F3 p1 = xform4(ic.objectToWorld, ld3(v1.position), 1.0f);
+                const F3 p2 = xform4(ic.objectToWorld, ld3(v2.position), 1.0f);
+                const F3 Ng = nrm(cross(p1 - p0, p2 - p0));
+                const F3 n0 = xform3(ic.normalMatrix, unpack_normal(v0.normal));
+                const F3 n1 = xform3(ic.normalMatrix, unpack_normal(v1.normal));
+                const F3 n2 = xform3(ic.normalMatrix, unpack_normal(v2.normal));
+                const F3 P = interp(p0, p1, p2, h.x, h.y);
+                const F3 N = nrm(interp(n0, n1, n2, h.x, h.y));
+                StkMaterial mat;
+                if (ic.materialId >= 0 && (uint32_t)ic.materialId < sc.numMats) {
+                    mat = sc.mats[ic.materialId];
+                } else {
+                    mat.type = STK_MAT_LAMBERT;
+                    mat.roughness = 1.0f;
+                    mat.albedo[0] = mat.albedo[1] = mat.albedo[2] = 0.5f;
+                    mat.albedo[3] = 1.0f;
+                }
+                const F3 k1 = -f3(ps.rayD[slot]);
+                const float rough = mat.type == STK_MAT_GGX ? mat.roughness : 1.0f;
+                const bool isTrain = is_train(pc, *ps.pass, slot);
+                if ((pc.mode == 1 || pc.mode == 2) && !isTrain && depth == (int)pc.queryDepth) {
+                    // the path ends here: its remaining radiance comes from the cache
+                    cacheQuery = true;
+                    qrec.posRough = make_float4(P.x, P.y, P.z, rough);
+                    qrec.normal = make_float4(N.x, N.y, N.z, bits_to_float(slot));
+                    qrec.dir = make_float4(k1.x, k1.y, k1.z, 0.0f);
+                    qrec.target = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                } else if (isTrain) {
+                    float4* v = pc.trainVerts + ((size_t)(slot / pc.trainStride) * lp.maxDepth + depth) * 5;
+                    v[0] = make_float4(P.x, P.y, P.z, rough);
+                    v[1] = make_float4(N.x, N.y, N.z, 0.0f);
+                    v[2] = make_float4(k1.x, k1.y, k1.z, 0.0f);
+                    v[3] = f4(col);
+                    v[4] = f4(thr);
+                    pc.trainCount[slot / pc.trainStride] = (uint32_t)depth + 1;
+                }
+                // sampleLights / estimateDirectLighting (shaders/pathtracerMdl.hlsl:31-92)
+                F3 toLight = mk3(0, 0, 0), radiance = mk3(0, 0, 0);
+                float lightPdf = 0.0f, dist = 0.0f;
+                if (desc.lightCount > 0 && !cacheQuery) {
+                    const uint32_t lid = (uint32_t)((float)desc.lightCount * rnd(rng));
+                    const float selPdf = 1.0f / (float)desc.lightCount;
+                    const StkLight& l =
+                        sc.lights[desc.lightBase + (lid < desc.lightCount ? lid : desc.lightCount - 1)];
+                    const float u0 = rnd(rng);
+                    const float u1 = rnd(rng);
+                    const LightSample ls = sample_rect_light(l, u0, u1, P);
+                    toLight = ls.L;
+                    const F3 Li = ld3(l.color);
+                    if (dot(N, toLight) > 0.0f && -dot(toLight, ls.normal) > 0.0f && Li.x != 0.0f && Li.y != 0.0f &&
+                        Li.z != 0.0f) {
+                        lightPdf = ls.pdf;
+                        dist = ls.dist;
+                        radiance = (Li * 1.0f) * saturate(dot(N, toLight)); // visibility applied by connect
+                    }
+                    lightPdf *= selPdf;
+                }
+                bool alive = !(isnan(radiance.x) || isnan(radiance.y) || isnan(radiance.z) || isnan(lightPdf));
+                if (alive && (dot(toLight, Ng) > 0.0f) && lightPdf != 0.0f) {
+                    const F3 rop = mk3(radiance.x / lightPdf, radiance.y / lightPdf, radiance.z / lightPdf);
+                    F3 bd, bg;
+                    float bpdf;
+                    bsdf_eval(mat, N, Ng, k1, toLight, bd, bg, bpdf);
+                    if (isnan(bd.x) || isnan(bd.y) || isnan(bd.z) || isnan(bg.x) || isnan(bg.y) || isnan(bg.z)) {
+                        alive = false;
+                    } else if (bpdf > 0.0f) {
+                        const float mis = (lightPdf == 0.0f) ? 1.0f : lightPdf / (lightPdf + bpdf);
+                        const F3 w = (thr * rop) * mis;
+                        cd = w * bd;
+                        cg = w * bg;
+                        shadow = true;
+                        sO = offset_ray(P, Ng);
+                        sD = toLight;
+                        sT = dist;
+                    }
+                }
+                if (alive && !cacheQuery) {
+                    const float x0 = rnd(rng);
+                    const float x1 = rnd(rng);
+                    (void)rnd(rng);
+                    (void)rnd(rng);
+                    const BsdfSample bs = bsdf_sample(mat, N, Ng, k1, x0, x1);
+                    if (bs.event != 0) {
+                        thr = thr * bs.weight;
+                        bool survive = true;
+                        if (depth > (int)lp.rrStartDepth) {
+                            const float pr = fmaxf(thr.x, fmaxf(thr.y, thr.z));
+                            if (rnd(rng) > pr)
+                                survive = false;
+                            else
+                                thr = thr * (1.0f / (pr + 1e-5f));
+                        }
+                        if (survive) {
+                            cont = true;
+                            nO = offset_ray(P, Ng);
+                            nD = bs.k2;
+                        }
+                    }
+                }
+            }
+        }
+        ps.rng[slot] = rng;
+        if (cont) {
+            ps.thr[slot] = f4(thr);
+            ps.rayO[slot] = make_float4(nO.x, nO.y, nO.z, 1e9f);
+            ps.rayD[slot] = make_float4(nD.x, nD.y, nD.z, 0.0f);
+        }
+    }
+    const uint32_t qi = wave_push(&ps.counters[1 - cur], cont);
+    if (cont) ps.queue[1 - cur][qi] = slot;
+    const uint32_t si = wave_push(&ps.counters[2], shadow);
+    if (shadow) {
+        ps.shadowQ[si] = slot;
+        ps.shO[si] = make_float4(sO.x, sO.y, sO.z, sT);
+        ps.shD[si] = make_float4(sD.x, sD.y, sD.z, 0.0f);
+        ps.shCd[si] = f4(cd);
+        ps.shCg[si] = f4(cg);
+    }
+    if (pc.mode == 1 || pc.mode == 2) {
+        const uint32_t ci = wave_push(&ps.counters[3], cacheQuery);
+        if (cacheQuery) pc.queryRecs[ci] = qrec;
+    }
+}
+
+// Shadow rays use the stackless skip-link walk: for these long incoherent
+// segments it beats the packed left-first walk (no LDS stack, so 8 waves per
+// SIMD instead of 5) while returning the same first-in-preorder hit.
+__global__ __launch_bounds__(kBlock) void k_pt_connect(SceneView sc, PathState ps)
+{
+    const uint32_t count = ps.counters[2];
+    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (s >= count) return;
+    const uint32_t slot = ps.shadowQ[s];
+    const SceneDesc& desc = sc.scenes[ps.scene[slot]];
+    const RayF r = make_ray(ps.shO[s], ps.shD[s]);
+    HitF h;
+    bool vis = !any_ref(sc.bvh, desc.hdr, desc.refRoot, r, h);
+    if (!vis && sc.inst[h.inst].lightId != -1) vis = true; // a light hit counts as visible
+    if (vis) {
+        F3 col = f3(ps.col[slot]);
+        col = col + f3(ps.shCd[s]);
+        col = col + f3(ps.shCg[s]);
+        ps.col[slot] = f4(col);
+    }
+}
+
+// Adds the cache's answer to the paths that ended at a query:
+// radiance += throughput * max(prediction, 0).
+__global__ __launch_bounds__(kBlock) void k_cache_apply(PtLaunch lp, PathState ps)
+{
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= ps.counters[3]) return;
+    const uint32_t slot = float_to_bits(lp.cache.queryRecs[q].normal.w);
+    const float4 o = lp.cache.queryOut[q];
+    const F3 thr = f3(ps.thr[slot]);
+    const F3 col = f3(ps.col[slot]) + thr * mk3(fmaxf(o.x, 0.0f), fmaxf(o.y, 0.0f), fmaxf(o.z, 0.0f));
+    ps.col[slot] = f4(col);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pt_finalize(SceneView sc, PtLaunch lp, PathState ps,
+                                                        float4* __restrict__ accum, float* __restrict__ samples)
+{
+    const uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
+    if (slot >= sc.totalSlots) return;
+    const uint32_t pixel = ps.pixel[slot];
+    if (pixel == STK_INVALID_INDEX) return;
+    const float4 c = ps.col[slot];
+    float4 a = accum[pixel];
+    // reduction.hlsl:11-32: color += invSpp * sample, in sample order
+    a.x = a.x + lp.invSpp * c.x;
+    a.y = a.y + lp.invSpp * c.y;
+    a.z = a.z + lp.invSpp * c.z;
+    a.w = 1.0f;
+    accum[pixel] = a;
+    if (samples) {
+        float* o = samples + ((size_t)pixel * lp.spp + ps.pass->sampleNum) * 3;
+        o[0] = c.x;
+        o[1] = c.y;
+        o[2] = c.z;
+    }
+}
+
+// Training targets: the radiance a training path gathered from vertex d on,
+// divided by the throughput at d - a one-sample estimate of the outgoing
+// radiance at d under the same estimator as the plain path tracer.
+__global__ __launch_bounds__(kBlock) void k_cache_targets(PtLaunch lp, PathState ps)
+{
+    const PtCache& pc = lp.cache;
+    const PassState pass = *ps.pass;
+    const uint32_t maxDepth = lp.maxDepth;
+    const uint32_t id = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t tidx = id / maxDepth, d = id % maxDepth;
+    bool emit = false;
+    CacheRecord rec;
+    if (tidx < pass.numTrainPaths && d < pc.trainCount[tidx]) {
+        const uint32_t slot = tidx * pc.trainStride + pass.trainOffset;
+        const float4* v = pc.trainVerts + ((size_t)tidx * maxDepth + d) * 5;
+        const float4 before = v[3], thr = v[4], fin = ps.col[slot];
+        if (thr.x > 1e-8f && thr.y > 1e-8f && thr.z > 1e-8f) {
+            // one-sample targets are clamped to [0, kTargetMax]: Russian-roulette
+            // survivors and near-light NEE samples can be arbitrarily large, and
+            // a few such fireflies would dominate the loss
+            const float kTargetMax = 100.0f;
+            rec.posRough = v[0];
+            rec.normal = v[1];
+            rec.dir = v[2];
+            rec.target = make_float4(fminf(fmaxf((fin.x - before.x) / thr.x, 0.0f), kTargetMax),
+                                     fminf(fmaxf((fin.y - before.y) / thr.y, 0.0f), kTargetMax),
+                                     fminf(fmaxf((fin.z - before.z) / thr.z, 0.0f), kTargetMax), 0.0f);
+            emit = isfinite(fin.x) && isfinite(fin.y) && isfinite(fin.z);
+        }
+    }
+    const uint32_t i = wave_push(&ps.counters[4], emit);
+    if (emit && i < pc.trainCap) pc.trainBatch[i] = rec;
+}
+
+inline uint32_t blocks(size_t n, uint32_t b = kBlock) { return (uint32_t)((n + b - 1) / b); }
+
+template <int BLOCK>
+void launchExtend(const SceneView& sc, PathState& ps, int depth, const LaunchConfig& lc, hipStream_t s)
+{
+    if (lc.persistentBlocks)
+        hipLaunchKernelGGL(k_pt_extend<BLOCK>, dim3(lc.persistentBlocks), dim3(BLOCK), 0, s, sc, ps, depth, 1);
+    else
+        hipLaunchKernelGGL(k_pt_extend<BLOCK>, dim3(blocks(sc.totalSlots, BLOCK)), dim3(BLOCK), 0, s, sc, ps, depth,
+                           0);
+}
+
+} // namespace
+
+hipError_t launchSetPass(PathState& ps, uint32_t frame, uint32_t sampleNum, uint32_t trainOffset,
+                         uint32_t numTrainPaths, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pass_set, dim3(1), dim3(64), 0, s, ps.pass, frame, sampleNum, trainOffset, numTrainPaths);
+    return hipGetLastError();
+}
+
+hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const LaunchConfig& lc, PathState& ps,
+                               float4* accum, float* samples, CacheBuffers* cb, hipStream_t s)
+{
+    const uint32_t n = sc.totalSlots;
+    if (n == 0) return hipSuccess;
+    const dim3 g(blocks(n)), b(kBlock);
+    const PtCache& pc = lp.cache;
+    const bool query = pc.mode == 1 || pc.mode == 2, train = pc.mode == 1 || pc.mode == 3;
+    hipLaunchKernelGGL(k_pass_reset, dim3(1), dim3(64), 0, s, ps);
+    hipLaunchKernelGGL(k_pt_raygen, g, b, 0, s, sc, lp, ps);
+    for (int depth = 0; depth < (int)lp.maxDepth; ++depth) {
+        hipLaunchKernelGGL(k_pt_setup, dim3(1), dim3(64), 0, s, ps, depth);
+        switch (lc.blockSize) {
+        case 64: launchExtend<64>(sc, ps, depth, lc, s); break;
+        case 128: launchExtend<128>(sc, ps, depth, lc, s); break;
+        default: launchExtend<256>(sc, ps, depth, lc, s); break;
+        }
+        hipLaunchKernelGGL(k_pt_shade, g, b, 0, s, sc, lp, ps, depth);
+        hipLaunchKernelGGL(k_pt_connect, g, b, 0, s, sc, ps);
+    }
+    if (query && cb) {
+        hipError_t e = cacheInfer(*cb, pc.queryRecs, n, ps.counters + 3, pc.queryOut, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_cache_apply, g, b, 0, s, lp, ps);
+    }
+    hipLaunchKernelGGL(k_pt_finalize, g, b, 0, s, sc, lp, ps, accum, samples);
+    if (train && cb && pc.maxTrainPaths > 0) {
+        hipLaunchKernelGGL(k_cache_targets, dim3(blocks((size_t)pc.maxTrainPaths * lp.maxDepth)), b, 0, s, lp, ps);
+        hipError_t e = cacheBackward(*cb, pc.trainBatch, pc.trainCap, ps.counters + 4, s);
+        if (e != hipSuccess) return e;
+        e = cacheAdam(*cb, 1.0f, ps.counters + 4, pc.trainCap, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipGetLastError();
+}
+
+} // namespace stk

@@ -127,13 +127,12 @@ __device__ __forceinline__ void miss(HitF& h)
 
 // Near-first packed traversal.  `stack` points at this lane's first slot; the
 // slots are `stride` words apart (LDS layout [depth][lane], conflict free).
-__device__ __forceinline__ bool closest_packed(const BvhView& bvh, const RayF& r, uint32_t* stack, int stride,
-                                               HitF& h)
+__device__ __forceinline__ bool closest_packed(const BvhView& bvh, const StkBvhHeader& hd, const RayF& r,
+                                               uint32_t* stack, int stride, HitF& h)
 {
     miss(h);
     bool found = false;
     int cur = -1;
-    const StkBvhHeader& hd = bvh.hdr;
     if (hd.root == INT32_MIN) return false;
     if (hd.root < 0) {
         leaf_closest(bvh, r, (uint32_t)~hd.root, found, h);
@@ -170,14 +169,16 @@ __device__ __forceinline__ bool closest_packed(const BvhView& bvh, const RayF& r
     return found;
 }
 
-// closestHit (shaders/raytracing.h:198-247), reference order, unpruned.
-__device__ __forceinline__ bool closest_ref(const BvhView& bvh, const RayF& r, HitF& h)
+// closestHit (shaders/raytracing.h:198-247), reference order, unpruned,
+// starting at skip-link node `root` (0 for a single scene).
+__device__ __forceinline__ bool closest_ref(const BvhView& bvh, const StkBvhHeader& hd, uint32_t root, const RayF& r,
+                                            HitF& h)
 {
     miss(h);
-    if (bvh.hdr.root == INT32_MIN) return false;
+    if (hd.root == INT32_MIN) return false;
     float minHit = 1e9f;
     bool found = false;
-    uint32_t idx = 0;
+    uint32_t idx = root;
     while (idx != STK_INVALID_INDEX) {
         const float4* np = reinterpret_cast<const float4*>(bvh.refNodes + idx);
         const float4 lo = np[0], hi = np[1];
@@ -213,11 +214,12 @@ __device__ __forceinline__ bool closest_ref(const BvhView& bvh, const RayF& r, H
 
 // anyHit (shaders/raytracing.h:249-286): first leaf in DFS order hit with
 // t < tmax.  h.inst/h.prim receive that leaf's ids.
-__device__ __forceinline__ bool any_ref(const BvhView& bvh, const RayF& r, HitF& h)
+__device__ __forceinline__ bool any_ref(const BvhView& bvh, const StkBvhHeader& hd, uint32_t root, const RayF& r,
+                                        HitF& h)
 {
     miss(h);
-    if (bvh.hdr.root == INT32_MIN) return false;
-    uint32_t idx = 0;
+    if (hd.root == INT32_MIN) return false;
+    uint32_t idx = root;
     while (idx != STK_INVALID_INDEX) {
         const float4* np = reinterpret_cast<const float4*>(bvh.refNodes + idx);
         const float4 lo = np[0], hi = np[1];
@@ -254,10 +256,10 @@ __device__ __forceinline__ bool any_ref(const BvhView& bvh, const RayF& r, HitF&
 // left first and a leaf right child waits on the stack while its inner left
 // sibling's subtree is walked, so leaves are tested in preorder and the walk
 // stops at the first hit, two boxes per 64-B fetch.
-__device__ __forceinline__ bool any_packed(const BvhView& bvh, const RayF& r, uint32_t* stack, int stride, HitF& h)
+__device__ __forceinline__ bool any_packed(const BvhView& bvh, const StkBvhHeader& hd, const RayF& r,
+                                           uint32_t* stack, int stride, HitF& h)
 {
     miss(h);
-    const StkBvhHeader& hd = bvh.hdr;
     if (hd.root == INT32_MIN) return false;
     int next = hd.root;
     if (hd.root >= 0) {

@@ -1,0 +1,157 @@
+"""Batched rendering, launch shapes and graph replay on the GPU.
+
+A path's random numbers and arithmetic depend only on its view, pixel, frame
+and sample, so every view of a batch must reproduce, bit for bit, what a
+single-scene render of that view gives; and every launch shape (block size,
+raygen tile, persistent extend grid, graph replay) must give the same image.
+The single-scene renders are themselves checked against the CPU oracle
+(test_gpu_pathtrace.py), which pins the batch to the oracle by transitivity;
+one view is also checked against the oracle directly here."""
+import numpy as np
+import pytest
+
+import strelka_amd as sa
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+SCENES = [(6, 0), (7, 0), (3, 10)]
+
+
+@pytest.fixture(scope="module")
+def scenes():
+    return [sa.SceneData.generate(c, v) for c, v in SCENES]
+
+
+def params(camera, sky, spp=2, frame=3, keep=True, accumulate=True):
+    return sa.render_params(camera, spp=spp, max_depth=8, frame=frame, sampling=1, sky=sky, keep_samples=keep,
+                            accumulate=accumulate, enable_tonemap=False)
+
+
+def single_render(s, spp=2, frame=3):
+    ctx = sa.Context(0)
+    ctx.set_scene(s)
+    ctx.build_bvh()
+    p = params(s.camera, s.sky, spp, frame)
+    w, h = s.camera.dimension[0], s.camera.dimension[1]
+    img = ctx.render(p)
+    smp = ctx.samples(w * h * spp * 3)
+    bvh = ctx.get_bvh()
+    ctx.close()
+    return img, smp, bvh
+
+
+def batch_context(scenes):
+    ctx = sa.Context(0)
+    ctx.batch_clear()
+    for s in scenes:
+        ctx.batch_add(s)
+    ctx.batch_commit()
+    return ctx
+
+
+def test_batch_views_equal_single_scene_renders(scenes):
+    print("case: batch of config 6, config 7 and config 3 variant 10, 2 spp, bit-exact vs single renders",
+          flush=True)
+    ctx = batch_context(scenes)
+    views = ctx.batch_views()
+    assert [v["width"] for v in views] == [s.camera.dimension[0] for s in scenes]
+    p = params(scenes[0].camera, (0, 0, 0))
+    imgs = [x.copy() for x in ctx.batch_render(p)]
+    total = ctx.batch_pixels()
+    smp = ctx.samples(total * 2 * 3)
+    ctx.close()
+    for i, s in enumerate(scenes):
+        ref_img, ref_smp, _ = single_render(s)
+        v = views[i]
+        n = v["width"] * v["height"]
+        got = smp[v["pixel_base"] * 6:(v["pixel_base"] + n) * 6]
+        print(f"view {i}: {v}, mean {ref_img[..., :3].mean():.4f}")
+        assert np.array_equal(got, ref_smp), i
+        assert np.array_equal(imgs[i], ref_img), i
+        assert ref_img[..., :3].mean() > 0.0
+
+
+def test_batch_view_matches_oracle(scenes):
+    print("case: batch view 0 (config 6) per-sample radiance vs the oracle", flush=True)
+    ctx = batch_context(scenes)
+    p = params(scenes[0].camera, (0, 0, 0), spp=2, frame=5)
+    ctx.batch_render(p)
+    v = ctx.batch_views()[0]
+    w, h = v["width"], v["height"]
+    smp = ctx.samples(ctx.batch_pixels() * 2 * 3)[:w * h * 6]
+    ctx.close()
+    s = scenes[0]
+    _, _, bvh = single_render(s, spp=1, frame=0)
+    o = Oracle.from_scene(s, bvh)
+    cpu = o.pt_frame(params(s.camera, s.sky, spp=2, frame=5), 0, 0, w, h)
+    err = np.abs(smp.reshape(-1, 3) - cpu.reshape(-1, 3)) / np.maximum(1.0, np.abs(cpu.reshape(-1, 3)))
+    frac = np.any(err > 1e-4, axis=1).mean()
+    print(f"divergent samples {frac:.5f}")
+    assert frac <= 0.005
+
+
+SHAPES = [dict(block_size=64, tile=1), dict(block_size=128, tile=16), dict(block_size=256, tile=8),
+          dict(block_size=256, tile=8, persistent_blocks=1024), dict(block_size=64, tile=4, persistent_blocks=512),
+          dict(block_size=256, tile=8, graph=True), dict(block_size=128, tile=16, persistent_blocks=2048, graph=True)]
+
+
+def test_launch_shapes_give_identical_images(scenes):
+    print("case: launch shapes over the batch, 2 spp, bit-exact", flush=True)
+    ctx = batch_context(scenes)
+    p = params(scenes[0].camera, (0, 0, 0), accumulate=False)
+    base = None
+    for shape in SHAPES:
+        print(f"shape {shape}", flush=True)
+        ctx.set_launch_config(**shape)
+        assert ctx.launch_config()["block_size"] == shape["block_size"]
+        img = np.concatenate([x.reshape(-1, 4) for x in ctx.batch_render(p)])
+        if base is None:
+            base = img.copy()
+        else:
+            assert np.array_equal(img, base), shape
+    ctx.close()
+
+
+def test_graph_replays_match_eager_frames(scenes):
+    print("case: graph-captured sample pass, 4 frames of 3 spp with eager work between", flush=True)
+    s = scenes[2]
+    frames = []
+    for graph in (False, True):
+        ctx = sa.Context(0)
+        ctx.set_scene(s)
+        ctx.build_bvh()
+        ctx.set_launch_config(block_size=256, tile=8, graph=graph)
+        rays = s.jittered_rays(9)[:4096]
+        out = []
+        for f in range(4):
+            img = ctx.render(params(s.camera, s.sky, spp=3, frame=f, keep=False)).copy()
+            hits = ctx.trace_closest(rays)  # eager work between replays
+            ctx.sync()
+            out.append((img, hits))
+        frames.append(out)
+        ctx.close()
+    for f in range(4):
+        assert np.array_equal(frames[0][f][0], frames[1][f][0]), f
+        assert np.array_equal(frames[0][f][1], frames[1][f][1]), f
+
+
+def test_batch_rejects_cache_and_bad_shapes(scenes):
+    print("case: batch error paths", flush=True)
+    ctx = sa.Context(0)
+    with pytest.raises(sa.StkError):
+        ctx.batch_render(params(scenes[0].camera, (0, 0, 0)))  # nothing committed
+    with pytest.raises(sa.StkError):
+        ctx.set_launch_config(block_size=96)
+    with pytest.raises(sa.StkError):
+        ctx.set_launch_config(tile=0)
+    ctx.batch_add(scenes[0])
+    ctx.batch_commit()
+    ctx.set_scene(scenes[0])  # the cache takes its bounds from the single-scene BVH
+    ctx.build_bvh()
+    ctx.cache_create(log2_table=12)
+    p = params(scenes[0].camera, (0, 0, 0))
+    p.cacheMode = 1
+    with pytest.raises(sa.StkError):
+        ctx.batch_render(p)
+    ctx.close()
