@@ -1,9 +1,16 @@
 """Benchmark: closest-hit throughput of the MI355X traversal on the 1M-triangle
 scene (1920x1080 jittered primary rays), plus the path tracer's samples/s.
 
-A step is one closest-hit launch over the 2,073,600 primary rays of the
-config-2 scene (1,164,290 triangles), rays and BVH resident in HBM.  Prints one
-JSON line.  Only one GPU is used: --gpus above 1 exits with a message.
+A step is one closest-hit launch over the 2,073,600 primary rays of the seeded
+heightfield + 2,000-sphere scene (1,164,290 instanced triangles), rays and BVH
+resident in HBM.  Prints one JSON line.  Only one GPU is used: --gpus above 1
+exits with a message.
+
+Other modes (one JSON line each): --workload batch (32 scenes of 10k-2M
+triangles at 960x540, 16 spp, one batched wavefront pass per sample, launch
+shape from the autotuner), --cache-study (time to an RMSE target, cached vs
+plain path tracing) and --copy-mib (plain copy kernel, calibrates the memory
+counters).
 """
 from __future__ import annotations
 
@@ -56,6 +63,105 @@ def cpu_baseline(scene, nodes, rays):
     return threads, res
 
 
+def cpu_pt_baseline(scene, nodes, params, rows: int = 64):
+    """The oracle's path tracer on the host cores over a band of the same
+    frame (full width, `rows` rows from the middle, the frame's spp)."""
+    from oracle import Oracle
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    o = Oracle.from_scene(scene, nodes)
+    w, h = params.pt.dimension[0], params.pt.dimension[1]
+    y0 = max(0, h // 2 - rows // 2)
+    t0 = time.perf_counter()
+    o.pt_frame(params, 0, y0, w, rows, threads)
+    t = time.perf_counter() - t0
+    n = w * rows * params.pt.spp
+    return {"value": round(n / t / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"{w}x{rows} band of the frame at {params.pt.spp} spp ({n} samples), oracle path tracer, "
+                      f"{cpu_model()}", "seconds": round(t, 2)}
+
+
+def pt_frame_bench(ctx, scene, spp: int, frames: int, d_out: int) -> dict:
+    """Frames of `spp` samples per pixel (8 bounces, RR after 3) of the scene's
+    view, timed with device events; rays counted by the kernels."""
+    import strelka_amd as sa
+
+    w, h = scene.camera.dimension[0], scene.camera.dimension[1]
+    p = sa.render_params(scene.camera, spp=spp, max_depth=8, frame=0, sampling=1, sky=scene.sky)
+    ctx.render_device(p, d_out)  # warm-up: buffers, graph capture
+    ctx.sync()
+    rays = [0, 0]
+    ctx.record(6)
+    for f in range(frames):
+        p.pt.frameNumber = f + 1
+        ctx.render_device(p, d_out)
+    ctx.record(7)
+    ctx.sync()
+    ms = ctx.elapsed_ms(6, 7) / frames
+    rays = ctx.ray_totals()
+    return {"frame_ms": round(ms, 3), "spp": spp, "resolution": [w, h],
+            "samples_per_s": round(w * h * spp / (ms * 1e-3), 1),
+            "mrays_per_s": round((rays[0] + rays[1]) / (ms * 1e-3) / 1e6, 1),
+            "closest_rays_per_frame": rays[0], "shadow_rays_per_frame": rays[1]}
+
+
+def batch_bench(args) -> dict:
+    """The batched workload: 32 seeded scenes of uneven size (about 10k to 2M
+    instanced triangles, log-uniform), each viewed at 960x540, 16 spp, 8
+    bounces, all views traced by the same wavefront launches; the launch shape
+    is chosen by the autotuner on a 1-spp frame first."""
+    import strelka_amd as sa
+    from strelka_amd import autotune
+
+    t0 = time.perf_counter()
+    scenes = [sa.SceneData.generate(3, v) for v in range(32)]
+    gen_s = time.perf_counter() - t0
+    ctx = sa.Context(0)
+    ctx.batch_clear()
+    for s in scenes:
+        ctx.batch_add(s)
+    t0 = time.perf_counter()
+    ctx.batch_commit()
+    build_s = time.perf_counter() - t0
+    views = ctx.batch_views()
+    npix = ctx.batch_pixels()
+    tris = [int(s.inst["indexCount"].sum()) // 3 for s in scenes]
+    print(f"batch: 32 scenes, {sum(tris)} instanced triangles, BVHs built in {build_s:.1f} s",
+          file=sys.stderr, flush=True)
+    d_out = ctx.alloc(npix * 16)
+    p = sa.render_params(scenes[0].camera, spp=1, max_depth=8, frame=0, sampling=1)
+    log = (lambda m: print(m, file=sys.stderr, flush=True))
+    tuned = autotune.tune(ctx, lambda: ctx.batch_render_device(p, d_out), total_paths=npix, log=log)
+    spp = 16
+    p = sa.render_params(scenes[0].camera, spp=spp, max_depth=8, frame=1, sampling=1)
+    ctx.batch_render_device(p, d_out)
+    ctx.sync()
+    ctx.record(8)
+    for f in range(args.steps):
+        p.pt.frameNumber = 2 + f
+        ctx.batch_render_device(p, d_out)
+    ctx.record(9)
+    ctx.sync()
+    ms = ctx.elapsed_ms(8, 9) / args.steps
+    rays = ctx.ray_totals()
+    ctx.free(d_out)
+    dev = ctx.device_info()
+    ctx.close()
+    samples = npix * spp
+    return {"metric": "samples/s, batched path tracing of 32 uneven scenes", "value": round(samples / (ms * 1e-3), 1),
+            "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(ms, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (32 seeded PCG scenes)",
+            "config": {"workload": "32 seeded scenes of 10k-2M instanced triangles, 960x540 each, 16 spp, 8 bounces, "
+                                   "one batched wavefront pass per sample", "scenes": 32, "pixels": npix, "spp": spp,
+                       "instanced_triangles": tris, "bvh_build_s": round(build_s, 2),
+                       "scene_gen_s": round(gen_s, 2)},
+            "mrays_per_s": round((rays[0] + rays[1]) / (ms * 1e-3) / 1e6, 1),
+            "rays_per_frame": {"closest": rays[0], "shadow": rays[1]},
+            "autotune": {"best": tuned["best"], "best_1spp_ms": tuned["ms"], "table": tuned["table"],
+                         "device": dev["name"], "compute_units": dev["compute_units"]}}
+
+
 def load_traffic(kernel: str):
     f = ROOT / "profiles" / f"{kernel}_traffic.json"
     if f.exists():
@@ -69,7 +175,7 @@ def load_traffic(kernel: str):
 def cache_study(args) -> dict:
     """Time to reach an RMSE target against a converged plain render, for the
     plain path tracer and the cached one (training included in its time).
-    Config-2 scene at the study resolution, 1 spp per frame, 8 bounces,
+    The 1.16M-triangle heightfield + spheres scene at the study resolution, 1 spp per frame, 8 bounces,
     RR after 3, cache queried after 2 bounces, ACES-tonemapped RMSE."""
     import strelka_amd as sa
 
@@ -146,7 +252,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--pt-frames", type=int, default=4)
+    ap.add_argument("--pt-frames", type=int, default=2)
+    ap.add_argument("--pt-spp", type=int, default=64)
+    ap.add_argument("--workload", choices=("trace", "batch"), default="trace",
+                    help="trace: closest-hit on the 1M-triangle scene (+ its 64-spp path-traced frame); "
+                         "batch: the 32-scene batched path tracer")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (calibrates memory counters)")
     ap.add_argument("--cache-study", action="store_true",
@@ -166,6 +276,10 @@ def main() -> int:
 
     if args.cache_study:
         print(json.dumps(cache_study(args)))
+        return 0
+
+    if args.workload == "batch":
+        print(json.dumps(batch_bench(args)))
         return 0
 
     if args.copy_mib:
@@ -220,17 +334,10 @@ def main() -> int:
     ctx.record(3)
     ref_kernel_ms = ctx.elapsed_ms(2, 3) / max(1, args.steps // 10)
 
-    # path tracer: 1 spp per frame at 1080p, 8 bounces, RR after 3
-    p = sa.render_params(scene.camera, spp=1, max_depth=8, frame=0, sampling=1, sky=scene.sky)
+    # path tracer on the same scene: 1080p, pt_spp samples per frame, 8 bounces, RR after 3
     d_out = ctx.alloc(n * 16)
-    ctx.render_device(p, d_out)
-    ctx.sync()
-    ctx.record(4)
-    for f in range(args.pt_frames):
-        p.pt.frameNumber = f + 1
-        ctx.render_device(p, d_out)
-    ctx.record(5)
-    pt_ms = ctx.elapsed_ms(4, 5) / max(1, args.pt_frames)
+    ctx.set_launch_config(block_size=256, tile=8, graph=True)
+    pt = pt_frame_bench(ctx, scene, args.pt_spp, max(1, args.pt_frames), d_out)
     ctx.free(d_out)
     ctx.free(d_rays)
     ctx.free(d_hits)
@@ -253,6 +360,8 @@ def main() -> int:
                "sample": f"the full {n}-ray 1080p primary frame, oracle closestHit in reference order "
                          f"(skip-link, unpruned), median of 3, {cpu_model()}",
                "pruned_mrays_per_s": round(pr["mrays_per_s"], 4)}
+        p = sa.render_params(scene.camera, spp=4, max_depth=8, frame=1, sampling=1, sky=scene.sky)
+        pt["cpu_baseline"] = cpu_pt_baseline(scene, nodes, p)
 
     out = {
         "metric": METRIC,
@@ -267,14 +376,13 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded PCG scene and rays)",
-        "config": {"workload": "config2: 1920x1080 jittered primary rays, closest hit, 1M-triangle scene",
+        "config": {"workload": "1920x1080 jittered primary rays, closest hit, 1.16M-triangle heightfield + "
+                               "2,000-sphere scene",
                    "rays": n, "triangles": ntri, "bvh_nodes": int(nodes.size), "bvh_build_s": round(build_s, 3)},
         "roofline": roof,
         "cpu_baseline": cpu,
         "extra": {"reference_order_gpu_mrays_per_s": round(n / (ref_kernel_ms * 1e-3) / 1e6, 2),
-                  "pt_1080p_1spp_frame_ms": round(pt_ms, 3),
-                  "pt_samples_per_s": round(n / (pt_ms * 1e-3), 1),
-                  "pt_max_depth": 8},
+                  "path_tracer": pt},
     }
     print(json.dumps(out))
     ctx.close()

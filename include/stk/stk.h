@@ -138,22 +138,45 @@ STK_API stk_status stk_get_accumulation(stk_ctx* ctx, float* out, size_t count);
  * [0],[1] path queues of the last two bounces, [2] shadow rays of the last
  * bounce, [3] cache queries, [4] cache training records produced. */
 STK_API stk_status stk_get_counters(stk_ctx* ctx, uint32_t out[8]);
+/* Rays traced by the last render call: [0] closest-hit (extension) rays,
+ * [1] shadow (any-hit) rays, over all its passes.  No reference counterpart
+ * (a measurement aid). */
+STK_API stk_status stk_get_ray_totals(stk_ctx* ctx, uint64_t out[2]);
 
 /* ---- launch shape ----------------------------------------------------------
  * No counterpart in the reference (its compute passes use a fixed 16x16
  * workgroup, src/render/pathtracer.cpp).  blockSize 64/128/256 threads for the
  * wavefront kernels, raygen tile edge in pixels (1 = scanline order),
  * persistentBlocks > 0 makes the extend kernel a persistent grid of that many
- * blocks fetching rays from a work counter, and useGraph captures one sample
- * pass as a hipGraph replayed for every sample (plain rendering only). */
+ * blocks fetching rays from a work counter, useGraph captures one sample pass
+ * as a hipGraph replayed for every pass (plain rendering only), and
+ * pathsInFlight > 0 (at least 4096) traces a frame's paths in passes of that
+ * many slots, so a pass's path state can stay in the last-level cache (plain
+ * rendering only; the cache trains on whole frames).  Results do not depend
+ * on any of these. */
 typedef struct StkLaunchConfig {
     uint32_t blockSize;
     uint32_t tile;
     uint32_t persistentBlocks;
     uint32_t useGraph;
+    uint32_t pathsInFlight;
+    uint32_t pad[3];
 } StkLaunchConfig;
 STK_API stk_status stk_set_launch_config(stk_ctx* ctx, const StkLaunchConfig* lc);
 STK_API stk_status stk_get_launch_config(stk_ctx* ctx, StkLaunchConfig* lc);
+/* Properties of the context's device (for sizing launches). */
+typedef struct StkDeviceInfo {
+    char name[64];
+    char arch[64]; /* e.g. "gfx950:sramecc+:xnack-" */
+    uint32_t computeUnits;
+    uint32_t wavefrontSize;
+    uint64_t globalMemBytes;
+    uint32_t l2Bytes;
+    uint32_t clockKhz;
+    uint32_t memClockKhz;
+    uint32_t memBusWidth;
+} StkDeviceInfo;
+STK_API stk_status stk_device_info(stk_ctx* ctx, StkDeviceInfo* out);
 
 /* ---- batched scenes --------------------------------------------------------
  * Many independent scenes, each with its own view, rendered together: every

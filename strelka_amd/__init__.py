@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _abi
 from ._abi import (BVH_NODE, CACHE_RECORD, HIT, INSTANCE, INVALID, LIGHT, MATERIAL, PACKED_NODE, RAY, TRIANGLE,
-                   VERTEX, BatchScene, BatchViewInfo, BvhHeader, BvhOptions, CacheConfig, CacheInfo, LaunchConfig,
+                   VERTEX, BatchScene, BatchViewInfo, BvhHeader, BvhOptions, CacheConfig, CacheInfo, DeviceInfo, LaunchConfig,
                    PathTracerParam, RenderParams, SceneArrays)
 
 __all__ = ["lib", "StkError", "SceneData", "HostBvh", "Context", "RenderParams", "PathTracerParam", "VERTEX",
@@ -296,17 +296,31 @@ class Context:
         self._check(_LIB.stk_get_counters(self._h, _ptr(out)), "stk_get_counters")
         return out
 
+    def ray_totals(self) -> tuple[int, int]:
+        """(closest-hit rays, shadow rays) traced by the last render call."""
+        out = np.zeros(2, np.uint64)
+        self._check(_LIB.stk_get_ray_totals(self._h, _ptr(out)), "stk_get_ray_totals")
+        return int(out[0]), int(out[1])
+
     # -- launch shape --------------------------------------------------------------
     def set_launch_config(self, block_size: int = 256, tile: int = 8, persistent_blocks: int = 0,
-                          graph: bool = False):
-        lc = LaunchConfig(block_size, tile, persistent_blocks, 1 if graph else 0)
+                          graph: bool = False, paths_in_flight: int = 0):
+        lc = LaunchConfig(block_size, tile, persistent_blocks, 1 if graph else 0, paths_in_flight)
         self._check(_LIB.stk_set_launch_config(self._h, C.byref(lc)), "stk_set_launch_config")
 
     def launch_config(self) -> dict:
         lc = LaunchConfig()
         self._check(_LIB.stk_get_launch_config(self._h, C.byref(lc)), "stk_get_launch_config")
         return {"block_size": lc.blockSize, "tile": lc.tile, "persistent_blocks": lc.persistentBlocks,
-                "graph": bool(lc.useGraph)}
+                "graph": bool(lc.useGraph), "paths_in_flight": lc.pathsInFlight}
+
+    def device_info(self) -> dict:
+        d = DeviceInfo()
+        self._check(_LIB.stk_device_info(self._h, C.byref(d)), "stk_device_info")
+        return {"name": d.name.decode(errors="replace"), "arch": d.arch.decode(errors="replace"),
+                "compute_units": d.computeUnits, "wavefront_size": d.wavefrontSize,
+                "global_mem_bytes": d.globalMemBytes, "l2_bytes": d.l2Bytes, "clock_khz": d.clockKhz,
+                "mem_clock_khz": d.memClockKhz, "mem_bus_width": d.memBusWidth}
 
     # -- batched scenes -------------------------------------------------------------
     def batch_clear(self):

@@ -79,7 +79,13 @@ class BvhHeader(C.Structure):
 
 class LaunchConfig(C.Structure):
     _fields_ = [("blockSize", C.c_uint32), ("tile", C.c_uint32), ("persistentBlocks", C.c_uint32),
-                ("useGraph", C.c_uint32)]
+                ("useGraph", C.c_uint32), ("pathsInFlight", C.c_uint32), ("pad", C.c_uint32 * 3)]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 64), ("computeUnits", C.c_uint32),
+                ("wavefrontSize", C.c_uint32), ("globalMemBytes", C.c_uint64), ("l2Bytes", C.c_uint32),
+                ("clockKhz", C.c_uint32), ("memClockKhz", C.c_uint32), ("memBusWidth", C.c_uint32)]
 
 
 class BatchScene(C.Structure):
@@ -136,8 +142,10 @@ PROTOTYPES = [
     ("stk_get_samples", C.c_int, [P, P, C.c_size_t]),
     ("stk_get_accumulation", C.c_int, [P, P, C.c_size_t]),
     ("stk_get_counters", C.c_int, [P, P]),
+    ("stk_get_ray_totals", C.c_int, [P, P]),
     ("stk_set_launch_config", C.c_int, [P, C.POINTER(LaunchConfig)]),
     ("stk_get_launch_config", C.c_int, [P, C.POINTER(LaunchConfig)]),
+    ("stk_device_info", C.c_int, [P, C.POINTER(DeviceInfo)]),
     ("stk_batch_clear", C.c_int, [P]),
     ("stk_batch_add_scene", C.c_int, [P, C.POINTER(BatchScene), C.POINTER(C.c_uint32)]),
     ("stk_batch_commit", C.c_int, [P, C.POINTER(BvhOptions)]),

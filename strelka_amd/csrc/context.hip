@@ -504,7 +504,12 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
     if (slots > (1ull << 30) || npix > (1ull << 29)) return fail(c, STK_E_INVALID, "render: image too large");
     (void)hipSetDevice(c->device);
     const size_t nsamples = p->keepSamples ? npix * (size_t)p->pt.spp * 3 : 0;
-    if ((st = ensurePathState(c, slots)) != STK_OK) return st;
+    // paths in flight per pass: a frame's slots are traced in passes of at most
+    // Q slots (whole waves); the cache trains on whole frames, so it takes one pass
+    size_t Q = slots;
+    if (c->launch.pathsInFlight && p->cacheMode == 0)
+        Q = std::min(slots, ((size_t)c->launch.pathsInFlight + 63) / 64 * 64);
+    if ((st = ensurePathState(c, Q)) != STK_OK) return st;
     if ((st = ensureImage(c, imageKey, npix, nsamples)) != STK_OK) return st;
     const size_t descBytes = descs.size() * sizeof(stk::SceneDesc);
     if (c->descs.bytes < descBytes) invalidateGraph(c);
@@ -521,6 +526,7 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
     lp.rrStartDepth = p->rrStartDepth;
     lp.refCompat = p->refCompat;
     lp.invSpp = 1.0f / (float)p->pt.spp;
+    lp.passSlots = (uint32_t)Q;
     stk::PtCache& pc = lp.cache;
     if (p->cacheMode != 0) {
         if (!allowCache) return fail(c, STK_E_INVALID, "render: the radiance cache serves single-scene renders");
@@ -547,6 +553,7 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
     }
     float4* accum = static_cast<float4*>(c->accum.ptr);
     STK_HIP(c, stk::launchZero(accum, npix, c->stream));
+    STK_HIP(c, stk::launchClearRayTotals(c->ps, c->stream));
     float* samples = nsamples ? static_cast<float*>(c->samples.ptr) : nullptr;
     // One sample pass is captured once as a graph and replayed per sample (the
     // pass number and frame come from device memory).  Training passes of the
@@ -558,7 +565,7 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
                                      (uint64_t)sv.bvh.instances, (uint64_t)sv.vb, (uint64_t)sv.ib,
                                      (uint64_t)sv.inst, (uint64_t)sv.lights, (uint64_t)sv.mats, sv.numMats,
                                      (uint64_t)sv.scenes, sv.numScenes, sv.totalSlots, sv.tile, lp.maxDepth,
-                                     lp.spp, lp.samplingType, lp.rrStartDepth, lp.refCompat,
+                                     lp.spp, lp.samplingType, lp.rrStartDepth, lp.refCompat, lp.passSlots,
                                      c->launch.blockSize, c->launch.tile, c->launch.persistentBlocks,
                                      (uint64_t)accum, (uint64_t)samples, (uint64_t)c->ps.rayO, c->ps.capacity};
         if (!c->graphExec || key != c->graphKey) {
@@ -577,22 +584,28 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
         }
     }
     for (int s = 0; s < p->pt.spp; ++s) {
-        uint32_t trainOffset = 0, numTrain = 0;
-        if (pc.mode != 0) {
-            // a different subset of paths trains each pass
-            uint32_t h = p->pt.frameNumber * (uint32_t)p->pt.spp + (uint32_t)s;
-            h ^= h >> 16;
-            h *= 0x7feb352du;
-            h ^= h >> 15;
-            trainOffset = h % pc.trainStride;
-            numTrain = (uint32_t)((slots - trainOffset + pc.trainStride - 1) / pc.trainStride);
+        for (size_t begin = 0; begin < slots; begin += Q) {
+            stk::PassState pass{};
+            pass.frameNumber = p->pt.frameNumber;
+            pass.sampleNum = (uint32_t)s;
+            pass.slotBegin = (uint32_t)begin;
+            pass.slotCount = (uint32_t)std::min(Q, slots - begin);
+            if (pc.mode != 0) {
+                // a different subset of paths trains each pass
+                uint32_t h = p->pt.frameNumber * (uint32_t)p->pt.spp + (uint32_t)s;
+                h ^= h >> 16;
+                h *= 0x7feb352du;
+                h ^= h >> 15;
+                pass.trainOffset = h % pc.trainStride;
+                pass.numTrainPaths = (uint32_t)((slots - pass.trainOffset + pc.trainStride - 1) / pc.trainStride);
+            }
+            STK_HIP(c, stk::launchSetPass(c->ps, pass, c->stream));
+            if (graph)
+                STK_HIP(c, hipGraphLaunch(c->graphExec, c->stream));
+            else
+                STK_HIP(c, stk::launchPathTracePass(sv, lp, c->launch, c->ps, accum, samples,
+                                                    c->haveCache ? &c->cache : nullptr, c->stream));
         }
-        STK_HIP(c, stk::launchSetPass(c->ps, p->pt.frameNumber, (uint32_t)s, trainOffset, numTrain, c->stream));
-        if (graph)
-            STK_HIP(c, hipGraphLaunch(c->graphExec, c->stream));
-        else
-            STK_HIP(c, stk::launchPathTracePass(sv, lp, c->launch, c->ps, accum, samples,
-                                                c->haveCache ? &c->cache : nullptr, c->stream));
     }
     const int acc = p->enableAccumulation != 0;
     STK_HIP(c, stk::launchAccumulateTonemap(accum, static_cast<float4*>(c->history.ptr),
@@ -894,11 +907,32 @@ STK_API stk_status stk_set_launch_config(stk_ctx* c, const StkLaunchConfig* lc)
     if (lc->tile == 0 || lc->tile > 64) return fail(c, STK_E_INVALID, "stk_set_launch_config: tile must be 1..64");
     if (lc->persistentBlocks > (1u << 20))
         return fail(c, STK_E_INVALID, "stk_set_launch_config: persistentBlocks too large");
+    if (lc->pathsInFlight != 0 && lc->pathsInFlight < 4096)
+        return fail(c, STK_E_INVALID, "stk_set_launch_config: pathsInFlight must be 0 or at least 4096");
     c->launch.blockSize = lc->blockSize;
     c->launch.tile = lc->tile;
     c->launch.persistentBlocks = lc->persistentBlocks;
+    c->launch.pathsInFlight = lc->pathsInFlight;
     c->useGraph = lc->useGraph != 0;
     invalidateGraph(c);
+    return STK_OK;
+}
+
+STK_API stk_status stk_device_info(stk_ctx* c, StkDeviceInfo* out)
+{
+    if (!validCtx(c) || !out) return STK_E_INVALID;
+    hipDeviceProp_t pr;
+    STK_HIP(c, hipGetDeviceProperties(&pr, c->device));
+    memset(out, 0, sizeof(*out));
+    snprintf(out->name, sizeof(out->name), "%s", pr.name);
+    snprintf(out->arch, sizeof(out->arch), "%s", pr.gcnArchName);
+    out->computeUnits = (uint32_t)pr.multiProcessorCount;
+    out->wavefrontSize = (uint32_t)pr.warpSize;
+    out->globalMemBytes = (uint64_t)pr.totalGlobalMem;
+    out->l2Bytes = (uint32_t)pr.l2CacheSize;
+    out->clockKhz = (uint32_t)pr.clockRate;
+    out->memClockKhz = (uint32_t)pr.memoryClockRate;
+    out->memBusWidth = (uint32_t)pr.memoryBusWidth;
     return STK_OK;
 }
 
@@ -909,6 +943,8 @@ STK_API stk_status stk_get_launch_config(stk_ctx* c, StkLaunchConfig* lc)
     lc->tile = c->launch.tile;
     lc->persistentBlocks = c->launch.persistentBlocks;
     lc->useGraph = c->useGraph ? 1u : 0u;
+    lc->pathsInFlight = c->launch.pathsInFlight;
+    lc->pad[0] = lc->pad[1] = lc->pad[2] = 0;
     return STK_OK;
 }
 
@@ -1181,6 +1217,15 @@ STK_API stk_status stk_get_counters(stk_ctx* c, uint32_t out[8])
     if (!validCtx(c) || !out) return STK_E_INVALID;
     if (!c->ps.counters) return fail(c, STK_E_STATE, "stk_get_counters: nothing rendered");
     STK_HIP(c, hipMemcpyAsync(out, c->ps.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    STK_HIP(c, hipStreamSynchronize(c->stream));
+    return STK_OK;
+}
+
+STK_API stk_status stk_get_ray_totals(stk_ctx* c, uint64_t out[2])
+{
+    if (!validCtx(c) || !out) return STK_E_INVALID;
+    if (!c->ps.counters) return fail(c, STK_E_STATE, "stk_get_ray_totals: nothing rendered");
+    STK_HIP(c, hipMemcpyAsync(out, c->ps.counters + 8, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
     STK_HIP(c, hipStreamSynchronize(c->stream));
     return STK_OK;
 }

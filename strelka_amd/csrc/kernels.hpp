@@ -46,10 +46,12 @@ struct PassState {
     uint32_t sampleNum;
     uint32_t trainOffset;
     uint32_t numTrainPaths;
+    uint32_t slotBegin; // the pass covers global slots [slotBegin, slotBegin + slotCount)
+    uint32_t slotCount;
 };
 
-// Device arrays of one wavefront path-tracing pass (one sample of every pixel
-// of every view).
+// Device arrays of one wavefront path-tracing pass (one sample of the pixels of
+// the pass's slots), indexed by the slot's position in the pass.
 struct PathState {
     uint32_t capacity = 0;
     float4* rayO = nullptr;  // origin.xyz, tmax
@@ -67,7 +69,8 @@ struct PathState {
     float4* shD = nullptr;
     float4* shCd = nullptr; // NEE contribution, diffuse part
     float4* shCg = nullptr; // NEE contribution, glossy part
-    uint32_t* counters = nullptr; // [0],[1] queue counts, [2] shadow, [3] queries, [4] train records, [5] work
+    uint32_t* counters = nullptr; // [0],[1] queue counts, [2] shadow, [3] queries, [4] train records, [5] work,
+                                  // [8..11] two 64-bit ray totals of the render call (closest, shadow)
     PassState* pass = nullptr;
 };
 
@@ -96,6 +99,7 @@ struct PtLaunch {
     uint32_t rrStartDepth;
     uint32_t refCompat;
     float invSpp;
+    uint32_t passSlots; // slots per pass (grid size of the per-slot kernels), <= PathState capacity
     PtCache cache;
 };
 
@@ -104,6 +108,7 @@ struct LaunchConfig {
     uint32_t blockSize = 256;      // 64, 128 or 256 threads
     uint32_t tile = 8;             // raygen tile edge (1 = scanline)
     uint32_t persistentBlocks = 0; // extend grid for persistent waves (0: one lane per ray)
+    uint32_t pathsInFlight = 0;    // slots per pass (0: every slot of the frame in one pass)
 };
 
 hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, bool refOrder,
@@ -113,8 +118,9 @@ hipError_t launchTraceAny(const BvhView& bvh, const StkRay* rays, uint32_t n, St
 hipError_t launchCameraRays(const StkPathTracerParam& p, uint32_t sampleNum, StkRay* rays, hipStream_t s);
 
 // Writes the pass state (a kernel, so it can precede a graph replay).
-hipError_t launchSetPass(PathState& ps, uint32_t frame, uint32_t sampleNum, uint32_t trainOffset,
-                         uint32_t numTrainPaths, hipStream_t s);
+hipError_t launchSetPass(PathState& ps, const PassState& v, hipStream_t s);
+// Clears the 64-bit ray totals (before the first pass of a render call).
+hipError_t launchClearRayTotals(PathState& ps, hipStream_t s);
 // One sample pass of the wavefront path tracer over every view: raygen, the
 // bounce loop and the finalize step, which adds invSpp * radiance into
 // accum[pixel] (and stores per-sample radiance when samples is set).  With

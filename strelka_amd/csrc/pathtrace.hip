@@ -60,15 +60,9 @@ __device__ __forceinline__ bool is_train(const PtCache& pc, const PassState& pas
     return (pc.mode == 1 || pc.mode == 3) && slot % pc.trainStride == pass.trainOffset;
 }
 
-__global__ void k_pass_set(PassState* pass, uint32_t frame, uint32_t sampleNum, uint32_t trainOffset,
-                           uint32_t numTrainPaths)
+__global__ void k_pass_set(PassState* pass, PassState v)
 {
-    if (threadIdx.x == 0) {
-        pass->frameNumber = frame;
-        pass->sampleNum = sampleNum;
-        pass->trainOffset = trainOffset;
-        pass->numTrainPaths = numTrainPaths;
-    }
+    if (threadIdx.x == 0) *pass = v;
 }
 
 __global__ void k_pass_reset(PathState ps)
@@ -76,48 +70,72 @@ __global__ void k_pass_reset(PathState ps)
     if (threadIdx.x < 8) ps.counters[threadIdx.x] = 0;
 }
 
+// Path state arrays are indexed by the slot's position in the pass (`local`);
+// a pass covers global slots [slotBegin, slotBegin + slotCount).
 __global__ __launch_bounds__(kBlock) void k_pt_raygen(SceneView sc, PtLaunch lp, PathState ps)
 {
-    const uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t local = blockIdx.x * kBlock + threadIdx.x;
+    const PassState pass = *ps.pass;
     bool valid = false;
-    if (slot < sc.totalSlots) {
-        const PassState pass = *ps.pass;
+    if (local < pass.slotCount) {
+        const uint32_t slot = pass.slotBegin + local;
         const uint32_t sid = scene_of_slot(sc, slot);
         const SceneDesc& d = sc.scenes[sid];
-        const uint32_t T = sc.tile, local = slot - d.slotBase;
-        const uint32_t tileIdx = local / (T * T), within = local % (T * T);
+        const uint32_t T = sc.tile, inView = slot - d.slotBase;
+        const uint32_t tileIdx = inView / (T * T), within = inView % (T * T);
         const uint32_t px = (tileIdx % d.tilesX) * T + within % T;
         const uint32_t py = (tileIdx / d.tilesX) * T + within / T;
         valid = px < d.width && py < d.height;
-        ps.scene[slot] = sid;
-        ps.pixel[slot] = valid ? d.pixelBase + py * d.width + px : STK_INVALID_INDEX;
+        ps.scene[local] = sid;
+        ps.pixel[local] = valid ? d.pixelBase + py * d.width + px : STK_INVALID_INDEX;
         if (valid) {
             const uint32_t frame = pass.frameNumber * lp.spp + pass.sampleNum;
             uint32_t rng = init_rng(px, py, d.width, frame);
             float4 o, dir;
             camera_ray(d.cam, px, py, rng, frame, o, dir);
-            ps.rayO[slot] = o;
-            ps.rayD[slot] = dir;
-            ps.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-            ps.col[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            ps.rng[slot] = rng;
+            ps.rayO[local] = o;
+            ps.rayD[local] = dir;
+            ps.thr[local] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+            ps.col[local] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ps.rng[local] = rng;
         } else {
-            ps.col[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ps.col[local] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
         // padding slots too: the target kernel reads every training slot's count
-        if (is_train(lp.cache, pass, slot)) lp.cache.trainCount[slot / lp.cache.trainStride] = 0;
+        // (the cache runs with one pass over all slots, so local == slot there)
+        if (is_train(lp.cache, pass, local)) lp.cache.trainCount[local / lp.cache.trainStride] = 0;
     }
     const uint32_t qi = wave_push(&ps.counters[0], valid);
-    if (valid) ps.queue[0][qi] = slot;
+    if (valid) ps.queue[0][qi] = local;
+}
+
+// Ray totals of a render call: two 64-bit words after the eight pass counters
+// (closest-hit rays, shadow rays), cleared by the host before the first pass.
+__device__ __forceinline__ unsigned long long* ray_totals(const PathState& ps)
+{
+    return reinterpret_cast<unsigned long long*>(ps.counters + 8);
 }
 
 __global__ void k_pt_setup(PathState ps, int depth)
 {
     if (threadIdx.x == 0) {
+        unsigned long long* tot = ray_totals(ps);
+        tot[0] += ps.counters[depth & 1]; // rays extended at this depth
+        tot[1] += ps.counters[2];         // shadow rays of the previous depth (0 at depth 0)
         ps.counters[1 - (depth & 1)] = 0;
         ps.counters[2] = 0;
         ps.counters[5] = 0;
     }
+}
+
+__global__ void k_pt_tally(PathState ps)
+{
+    if (threadIdx.x == 0) ray_totals(ps)[1] += ps.counters[2];
+}
+
+__global__ void k_totals_clear(PathState ps)
+{
+    if (threadIdx.x < 2) ray_totals(ps)[threadIdx.x] = 0;
 }
 
 template <int BLOCK>
@@ -354,14 +372,14 @@ __global__ __launch_bounds__(kBlock) void k_cache_apply(PtLaunch lp, PathState p
     ps.col[slot] = f4(col);
 }
 
-__global__ __launch_bounds__(kBlock) void k_pt_finalize(SceneView sc, PtLaunch lp, PathState ps,
-                                                        float4* __restrict__ accum, float* __restrict__ samples)
+__global__ __launch_bounds__(kBlock) void k_pt_finalize(PtLaunch lp, PathState ps, float4* __restrict__ accum,
+                                                        float* __restrict__ samples)
 {
-    const uint32_t slot = blockIdx.x * kBlock + threadIdx.x;
-    if (slot >= sc.totalSlots) return;
-    const uint32_t pixel = ps.pixel[slot];
+    const uint32_t local = blockIdx.x * kBlock + threadIdx.x;
+    if (local >= ps.pass->slotCount) return;
+    const uint32_t pixel = ps.pixel[local];
     if (pixel == STK_INVALID_INDEX) return;
-    const float4 c = ps.col[slot];
+    const float4 c = ps.col[local];
     float4 a = accum[pixel];
     // reduction.hlsl:11-32: color += invSpp * sample, in sample order
     a.x = a.x + lp.invSpp * c.x;
@@ -414,29 +432,33 @@ __global__ __launch_bounds__(kBlock) void k_cache_targets(PtLaunch lp, PathState
 inline uint32_t blocks(size_t n, uint32_t b = kBlock) { return (uint32_t)((n + b - 1) / b); }
 
 template <int BLOCK>
-void launchExtend(const SceneView& sc, PathState& ps, int depth, const LaunchConfig& lc, hipStream_t s)
+void launchExtend(const SceneView& sc, PathState& ps, int depth, const LaunchConfig& lc, uint32_t n, hipStream_t s)
 {
     if (lc.persistentBlocks)
         hipLaunchKernelGGL(k_pt_extend<BLOCK>, dim3(lc.persistentBlocks), dim3(BLOCK), 0, s, sc, ps, depth, 1);
     else
-        hipLaunchKernelGGL(k_pt_extend<BLOCK>, dim3(blocks(sc.totalSlots, BLOCK)), dim3(BLOCK), 0, s, sc, ps, depth,
-                           0);
+        hipLaunchKernelGGL(k_pt_extend<BLOCK>, dim3(blocks(n, BLOCK)), dim3(BLOCK), 0, s, sc, ps, depth, 0);
 }
 
 } // namespace
 
-hipError_t launchSetPass(PathState& ps, uint32_t frame, uint32_t sampleNum, uint32_t trainOffset,
-                         uint32_t numTrainPaths, hipStream_t s)
+hipError_t launchSetPass(PathState& ps, const PassState& v, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pass_set, dim3(1), dim3(64), 0, s, ps.pass, frame, sampleNum, trainOffset, numTrainPaths);
+    hipLaunchKernelGGL(k_pass_set, dim3(1), dim3(64), 0, s, ps.pass, v);
+    return hipGetLastError();
+}
+
+hipError_t launchClearRayTotals(PathState& ps, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_totals_clear, dim3(1), dim3(64), 0, s, ps);
     return hipGetLastError();
 }
 
 hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const LaunchConfig& lc, PathState& ps,
                                float4* accum, float* samples, CacheBuffers* cb, hipStream_t s)
 {
-    const uint32_t n = sc.totalSlots;
-    if (n == 0) return hipSuccess;
+    const uint32_t n = lp.passSlots;
+    if (n == 0 || n > ps.capacity) return hipErrorInvalidValue;
     const dim3 g(blocks(n)), b(kBlock);
     const PtCache& pc = lp.cache;
     const bool query = pc.mode == 1 || pc.mode == 2, train = pc.mode == 1 || pc.mode == 3;
@@ -445,19 +467,20 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
     for (int depth = 0; depth < (int)lp.maxDepth; ++depth) {
         hipLaunchKernelGGL(k_pt_setup, dim3(1), dim3(64), 0, s, ps, depth);
         switch (lc.blockSize) {
-        case 64: launchExtend<64>(sc, ps, depth, lc, s); break;
-        case 128: launchExtend<128>(sc, ps, depth, lc, s); break;
-        default: launchExtend<256>(sc, ps, depth, lc, s); break;
+        case 64: launchExtend<64>(sc, ps, depth, lc, n, s); break;
+        case 128: launchExtend<128>(sc, ps, depth, lc, n, s); break;
+        default: launchExtend<256>(sc, ps, depth, lc, n, s); break;
         }
         hipLaunchKernelGGL(k_pt_shade, g, b, 0, s, sc, lp, ps, depth);
         hipLaunchKernelGGL(k_pt_connect, g, b, 0, s, sc, ps);
     }
+    hipLaunchKernelGGL(k_pt_tally, dim3(1), dim3(64), 0, s, ps);
     if (query && cb) {
         hipError_t e = cacheInfer(*cb, pc.queryRecs, n, ps.counters + 3, pc.queryOut, s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_cache_apply, g, b, 0, s, lp, ps);
     }
-    hipLaunchKernelGGL(k_pt_finalize, g, b, 0, s, sc, lp, ps, accum, samples);
+    hipLaunchKernelGGL(k_pt_finalize, g, b, 0, s, lp, ps, accum, samples);
     if (train && cb && pc.maxTrainPaths > 0) {
         hipLaunchKernelGGL(k_cache_targets, dim3(blocks((size_t)pc.maxTrainPaths * lp.maxDepth)), b, 0, s, lp, ps);
         hipError_t e = cacheBackward(*cb, pc.trainBatch, pc.trainCap, ps.counters + 4, s);

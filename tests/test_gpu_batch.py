@@ -93,7 +93,9 @@ def test_batch_view_matches_oracle(scenes):
 
 SHAPES = [dict(block_size=64, tile=1), dict(block_size=128, tile=16), dict(block_size=256, tile=8),
           dict(block_size=256, tile=8, persistent_blocks=1024), dict(block_size=64, tile=4, persistent_blocks=512),
-          dict(block_size=256, tile=8, graph=True), dict(block_size=128, tile=16, persistent_blocks=2048, graph=True)]
+          dict(block_size=256, tile=8, graph=True), dict(block_size=128, tile=16, persistent_blocks=2048, graph=True),
+          dict(block_size=256, tile=8, paths_in_flight=4096),
+          dict(block_size=128, tile=8, paths_in_flight=100000, persistent_blocks=1024, graph=True)]
 
 
 def test_launch_shapes_give_identical_images(scenes):
@@ -101,15 +103,20 @@ def test_launch_shapes_give_identical_images(scenes):
     ctx = batch_context(scenes)
     p = params(scenes[0].camera, (0, 0, 0), accumulate=False)
     base = None
+    npix = ctx.batch_pixels()
     for shape in SHAPES:
         print(f"shape {shape}", flush=True)
         ctx.set_launch_config(**shape)
         assert ctx.launch_config()["block_size"] == shape["block_size"]
         img = np.concatenate([x.reshape(-1, 4) for x in ctx.batch_render(p)])
+        totals = ctx.ray_totals()
         if base is None:
-            base = img.copy()
+            base, base_totals = img.copy(), totals
+            # every path traces its camera ray; at most 8 extensions and 8 shadow rays per sample
+            assert npix * 2 <= totals[0] <= npix * 2 * 8 and 0 < totals[1] <= npix * 2 * 8
         else:
             assert np.array_equal(img, base), shape
+            assert totals == base_totals, shape
     ctx.close()
 
 
