@@ -105,13 +105,24 @@ def pt_frame_bench(ctx, scene, spp: int, frames: int, d_out: int) -> dict:
             "closest_rays_per_frame": rays[0], "shadow_rays_per_frame": rays[1]}
 
 
+def choose_launch(ctx, spec: str, run, total_paths: int) -> dict:
+    """The path tracer's launch shape: the autotuner's choice on `run`, or a
+    fixed shape given as JSON."""
+    from strelka_amd import autotune
+
+    if spec == "auto":
+        return autotune.tune(ctx, run, total_paths=total_paths, log=lambda m: print(m, file=sys.stderr, flush=True))
+    shape = dict(autotune.DEFAULT, **json.loads(spec))
+    ctx.set_launch_config(**shape)
+    return {"best": shape, "table": [], "ms": None, "default_ms": None}
+
+
 def batch_bench(args) -> dict:
     """The batched workload: 32 seeded scenes of uneven size (about 10k to 2M
     instanced triangles, log-uniform), each viewed at 960x540, 16 spp, 8
     bounces, all views traced by the same wavefront launches; the launch shape
     is chosen by the autotuner on a 1-spp frame first."""
     import strelka_amd as sa
-    from strelka_amd import autotune
 
     t0 = time.perf_counter()
     scenes = [sa.SceneData.generate(3, v) for v in range(32)]
@@ -130,8 +141,7 @@ def batch_bench(args) -> dict:
           file=sys.stderr, flush=True)
     d_out = ctx.alloc(npix * 16)
     p = sa.render_params(scenes[0].camera, spp=1, max_depth=8, frame=0, sampling=1)
-    log = (lambda m: print(m, file=sys.stderr, flush=True))
-    tuned = autotune.tune(ctx, lambda: ctx.batch_render_device(p, d_out), total_paths=npix, log=log)
+    tuned = choose_launch(ctx, args.launch, lambda: ctx.batch_render_device(p, d_out), npix)
     spp = 16
     p = sa.render_params(scenes[0].camera, spp=spp, max_depth=8, frame=1, sampling=1)
     ctx.batch_render_device(p, d_out)
@@ -158,7 +168,8 @@ def batch_bench(args) -> dict:
                        "scene_gen_s": round(gen_s, 2)},
             "mrays_per_s": round((rays[0] + rays[1]) / (ms * 1e-3) / 1e6, 1),
             "rays_per_frame": {"closest": rays[0], "shadow": rays[1]},
-            "autotune": {"best": tuned["best"], "best_1spp_ms": tuned["ms"], "table": tuned["table"],
+            "autotune": {"best": tuned["best"], "best_1spp_ms": tuned["ms"], "default_1spp_ms": tuned["default_ms"],
+                         "table": tuned["table"],
                          "device": dev["name"], "compute_units": dev["compute_units"]}}
 
 
@@ -254,6 +265,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pt-frames", type=int, default=2)
     ap.add_argument("--pt-spp", type=int, default=64)
+    ap.add_argument("--launch", default="auto",
+                    help="launch shape of the path tracer: 'auto' (autotuner) or a JSON object of "
+                         "Context.set_launch_config arguments")
     ap.add_argument("--workload", choices=("trace", "batch"), default="trace",
                     help="trace: closest-hit on the 1M-triangle scene (+ its 64-spp path-traced frame); "
                          "batch: the 32-scene batched path tracer")
@@ -334,10 +348,14 @@ def main() -> int:
     ctx.record(3)
     ref_kernel_ms = ctx.elapsed_ms(2, 3) / max(1, args.steps // 10)
 
-    # path tracer on the same scene: 1080p, pt_spp samples per frame, 8 bounces, RR after 3
+    # path tracer on the same scene: 1080p, pt_spp samples per frame, 8 bounces, RR after 3;
+    # launch shape from the autotuner on a 1-spp frame
     d_out = ctx.alloc(n * 16)
-    ctx.set_launch_config(block_size=256, tile=8, graph=True)
+    p1 = sa.render_params(scene.camera, spp=1, max_depth=8, frame=0, sampling=1, sky=scene.sky)
+    tuned = choose_launch(ctx, args.launch, lambda: ctx.render_device(p1, d_out), n)
     pt = pt_frame_bench(ctx, scene, args.pt_spp, max(1, args.pt_frames), d_out)
+    pt["launch"] = tuned["best"]
+    pt["autotune_1spp_ms"] = {"default": tuned.get("default_ms"), "best": tuned.get("ms")}
     ctx.free(d_out)
     ctx.free(d_rays)
     ctx.free(d_hits)

@@ -21,6 +21,16 @@
  *   config 6: a tiny scene (heightfield 16x16, 8 spheres) for smoke tests.
  *   config 7: the config-1 room closed and lit by a 1x1 ceiling light
  *             (walls albedo 0.7), 128x128: long multi-bounce paths.
+ *   config 8: convex furnace: an outward-facing cube [-1,1]^3 (flat faces,
+ *             Lambert albedo (0.8, 0.6, 0.4)) under a constant sky
+ *             (0.5, 0.7, 1.0), no lights; camera at (3, 2.4, 3.6) looking at
+ *             the origin, vfov 40, 96x96.  A path that hits the cube bounces
+ *             once and escapes, so every sample is albedo*sky or sky.
+ *   config 9: lit floor: a Lambert floor y = 0 over [-20,20]^2 (albedo 0.5)
+ *             under a 1x1 rect light centred at y = 2 facing -y (intensity
+ *             10, so radiance 10), no sky; camera at (0, 1, 0) looking down,
+ *             up -z, vfov 60, 64x64.  A pixel's value is the next-event
+ *             estimate at the floor, an integral over the light.
  */
 #ifndef STK_SCENE_H
 #define STK_SCENE_H

@@ -14,13 +14,14 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _abi
-from ._abi import (BVH_NODE, CACHE_RECORD, HIT, INSTANCE, INVALID, LIGHT, MATERIAL, PACKED_NODE, RAY, TRIANGLE,
+from ._abi import (BVH_NODE, CACHE_RECORD, HIT, INSTANCE, INVALID, LIGHT, MAT_GGX, MAT_LAMBERT, MATERIAL, PACKED_NODE,
+                   RAY, TRIANGLE,
                    VERTEX, BatchScene, BatchViewInfo, BvhHeader, BvhOptions, CacheConfig, CacheInfo, DeviceInfo, LaunchConfig,
                    PathTracerParam, RenderParams, SceneArrays)
 
 __all__ = ["lib", "StkError", "SceneData", "HostBvh", "Context", "RenderParams", "PathTracerParam", "VERTEX",
            "BVH_NODE", "INSTANCE", "LIGHT", "RAY", "HIT", "MATERIAL", "TRIANGLE", "PACKED_NODE", "INVALID",
-           "render_params", "CACHE_RECORD"]
+           "render_params", "CACHE_RECORD", "MAT_LAMBERT", "MAT_GGX"]
 
 _LIB = _abi.load()
 
@@ -304,15 +305,18 @@ class Context:
 
     # -- launch shape --------------------------------------------------------------
     def set_launch_config(self, block_size: int = 256, tile: int = 8, persistent_blocks: int = 0,
-                          graph: bool = False, paths_in_flight: int = 0):
-        lc = LaunchConfig(block_size, tile, persistent_blocks, 1 if graph else 0, paths_in_flight)
+                          graph: bool = False, paths_in_flight: int = 0, lane_refill: bool = False,
+                          refill_budget: int = 16):
+        lc = LaunchConfig(block_size, tile, persistent_blocks, 1 if graph else 0, paths_in_flight,
+                          1 if lane_refill else 0, refill_budget)
         self._check(_LIB.stk_set_launch_config(self._h, C.byref(lc)), "stk_set_launch_config")
 
     def launch_config(self) -> dict:
         lc = LaunchConfig()
         self._check(_LIB.stk_get_launch_config(self._h, C.byref(lc)), "stk_get_launch_config")
         return {"block_size": lc.blockSize, "tile": lc.tile, "persistent_blocks": lc.persistentBlocks,
-                "graph": bool(lc.useGraph), "paths_in_flight": lc.pathsInFlight}
+                "graph": bool(lc.useGraph), "paths_in_flight": lc.pathsInFlight,
+                "lane_refill": bool(lc.laneRefill), "refill_budget": lc.refillBudget}
 
     def device_info(self) -> dict:
         d = DeviceInfo()

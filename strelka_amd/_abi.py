@@ -23,6 +23,7 @@ LIGHT = np.dtype([("points", "<f4", (4, 4)), ("color", "<f4", 4), ("normal", "<f
                   ("pad", "<f4", 3)])
 RAY = np.dtype([("o", "<f4", 4), ("d", "<f4", 4)])
 HIT = np.dtype([("bary", "<f4", 2), ("t", "<f4"), ("instId", "<u4"), ("primId", "<u4")])
+MAT_LAMBERT, MAT_GGX = 0, 1  # StkMaterial.type
 MATERIAL = np.dtype([("type", "<u4"), ("roughness", "<f4"), ("pad", "<f4", 2), ("albedo", "<f4", 4)])
 TRIANGLE = np.dtype([("v0", "<f4", 3), ("leaf", "<u4"), ("e0", "<f4", 3), ("instId", "<u4"), ("e1", "<f4", 3),
                      ("primId", "<u4")])
@@ -79,7 +80,8 @@ class BvhHeader(C.Structure):
 
 class LaunchConfig(C.Structure):
     _fields_ = [("blockSize", C.c_uint32), ("tile", C.c_uint32), ("persistentBlocks", C.c_uint32),
-                ("useGraph", C.c_uint32), ("pathsInFlight", C.c_uint32), ("pad", C.c_uint32 * 3)]
+                ("useGraph", C.c_uint32), ("pathsInFlight", C.c_uint32), ("laneRefill", C.c_uint32),
+                ("refillBudget", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class DeviceInfo(C.Structure):

@@ -4,8 +4,9 @@ The reference dispatches its compute passes with one fixed 16x16 workgroup
 (src/render/pathtracer.cpp); here the shape of the wavefront launches is a
 per-device, per-workload choice: threads per block of the wavefront kernels,
 the raygen tile edge (spatial coherence of a wave's rays), a persistent grid
-for the extend kernel, the number of paths traced per pass (so a pass's path
-state fits the last-level cache) and graph replay of the pass.  None of them
+for the extend kernel with or without lane refill (and its refill interval),
+the number of paths traced per pass (so a pass's path state fits the
+last-level cache) and graph replay of the pass.  None of them
 changes a single output bit (the launch-shape tests check that), so the tuner
 only measures time: it runs the caller's workload under each candidate with
 device events on the context stream and keeps the fastest.
@@ -22,10 +23,12 @@ from typing import Callable
 
 BLOCK_SIZES = (64, 128, 256)
 TILES = (1, 4, 8, 16)
-PERSISTENT_PER_CU = (0, 4, 8)
+PERSISTENT_PER_CU = (0, 4, 8, 16)
 PATHS_IN_FLIGHT = (0, 1 << 20, 1 << 22)
+REFILL_BUDGETS = (4, 8, 16, 32)
 
-DEFAULT = {"block_size": 256, "tile": 8, "persistent_blocks": 0, "graph": True, "paths_in_flight": 0}
+DEFAULT = {"block_size": 256, "tile": 8, "persistent_blocks": 0, "graph": True, "paths_in_flight": 0,
+           "lane_refill": False, "refill_budget": 16}
 
 
 def _key(shape: dict) -> tuple:
@@ -56,6 +59,8 @@ def tune(ctx, run: Callable[[], None], total_paths: int, repeats: int = 2, sweep
         "tile": list(TILES),
         "persistent_blocks": [k * cus for k in PERSISTENT_PER_CU],
         "paths_in_flight": [q for q in PATHS_IN_FLIGHT if q == 0 or q < total_paths],
+        "lane_refill": [False, True],
+        "refill_budget": list(REFILL_BUDGETS),
         "graph": [True, False],
     }
     seen: dict = {}
@@ -69,13 +74,16 @@ def tune(ctx, run: Callable[[], None], total_paths: int, repeats: int = 2, sweep
         return seen[k]
 
     best = dict(DEFAULT)
-    best_ms = cost(best)
+    best_ms = default_ms = cost(best)
     for _ in range(sweeps):
         improved = False
         for axis, values in axes.items():
             for v in values:
                 cand = dict(best)
                 cand[axis] = v
+                if (cand["persistent_blocks"] == 0 or not cand["lane_refill"]) and \
+                        (axis == "refill_budget" or (axis == "lane_refill" and v and cand["persistent_blocks"] == 0)):
+                    continue  # refill applies to the persistent grid only, its budget to refill only
                 ms = cost(cand)
                 if ms < best_ms * 0.995:
                     best, best_ms, improved = cand, ms, True
@@ -83,7 +91,8 @@ def tune(ctx, run: Callable[[], None], total_paths: int, repeats: int = 2, sweep
             break
     ctx.set_launch_config(**best)
     table = [{"shape": dict(k), "ms": round(v, 4)} for k, v in sorted(seen.items(), key=lambda kv: kv[1])]
-    return {"best": best, "ms": round(best_ms, 4), "table": table, "compute_units": cus}
+    return {"best": best, "ms": round(best_ms, 4), "default_ms": round(default_ms, 4), "table": table,
+            "compute_units": cus}
 
 
 def load_choice(path: Path, device: str, workload: str) -> dict | None:

@@ -567,6 +567,7 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
                                      (uint64_t)sv.scenes, sv.numScenes, sv.totalSlots, sv.tile, lp.maxDepth,
                                      lp.spp, lp.samplingType, lp.rrStartDepth, lp.refCompat, lp.passSlots,
                                      c->launch.blockSize, c->launch.tile, c->launch.persistentBlocks,
+                                     c->launch.laneRefill, c->launch.refillBudget,
                                      (uint64_t)accum, (uint64_t)samples, (uint64_t)c->ps.rayO, c->ps.capacity};
         if (!c->graphExec || key != c->graphKey) {
             invalidateGraph(c);
@@ -913,6 +914,8 @@ STK_API stk_status stk_set_launch_config(stk_ctx* c, const StkLaunchConfig* lc)
     c->launch.tile = lc->tile;
     c->launch.persistentBlocks = lc->persistentBlocks;
     c->launch.pathsInFlight = lc->pathsInFlight;
+    c->launch.laneRefill = lc->laneRefill ? 1u : 0u;
+    c->launch.refillBudget = lc->refillBudget ? std::min<uint32_t>(lc->refillBudget, 4096u) : 16u;
     c->useGraph = lc->useGraph != 0;
     invalidateGraph(c);
     return STK_OK;
@@ -944,7 +947,9 @@ STK_API stk_status stk_get_launch_config(stk_ctx* c, StkLaunchConfig* lc)
     lc->persistentBlocks = c->launch.persistentBlocks;
     lc->useGraph = c->useGraph ? 1u : 0u;
     lc->pathsInFlight = c->launch.pathsInFlight;
-    lc->pad[0] = lc->pad[1] = lc->pad[2] = 0;
+    lc->laneRefill = c->launch.laneRefill;
+    lc->refillBudget = c->launch.refillBudget;
+    lc->pad = 0;
     return STK_OK;
 }
 

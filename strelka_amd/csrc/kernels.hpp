@@ -109,6 +109,8 @@ struct LaunchConfig {
     uint32_t tile = 8;             // raygen tile edge (1 = scanline)
     uint32_t persistentBlocks = 0; // extend grid for persistent waves (0: one lane per ray)
     uint32_t pathsInFlight = 0;    // slots per pass (0: every slot of the frame in one pass)
+    uint32_t laneRefill = 0;       // persistent extend: idle lanes fetch new rays mid-walk
+    uint32_t refillBudget = 16;    // node visits between refills
 };
 
 hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, bool refOrder,

@@ -149,19 +149,75 @@ __device__ __forceinline__ void extend_one(const SceneView& sc, PathState& ps, i
     ps.hitPrim[q] = h.prim;
 }
 
-// Closest hit of every queued path.  persistent = 0: one lane per queue
-// entry.  persistent = 1: a fixed grid of waves fetches 64 entries at a time
-// from a work counter until the queue is exhausted (every wave reaches the
-// exit test, the counter only grows).
+// Lane refill: each lane walks its own ray; every `budget` node visits the
+// lanes whose walk ended store their hit and take the next queue entries (one
+// atomic per wave), so a wave keeps all lanes busy while rays of very
+// different lengths finish (Aila and Laine 2009, "while-while" with dynamic
+// fetch).  The walk of a ray is the same as in extend_one, so is its hit.
+// Exit: once the counter passes the queue length the wave stops fetching and
+// leaves when its last walk ends.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_pt_extend(SceneView sc, PathState ps, int depth, int persistent)
+__device__ __forceinline__ void extend_refill(const SceneView& sc, PathState& ps, int cur, uint32_t count,
+                                              uint32_t* stack, int budget)
+{
+    const uint32_t lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    bool have = false, exhausted = false;
+    uint32_t q = 0;
+    RayF r;
+    HitF h;
+    PackedWalk w;
+    for (;;) {
+        if (!exhausted) {
+            const uint64_t need = __ballot(!have);
+            if (need) {
+                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)need) - 1;
+                const uint32_t n = (uint32_t)__popcll(need);
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&ps.counters[5], n);
+                base = (uint32_t)__shfl((int)base, (int)leader);
+                exhausted = base + n >= count;
+                if (!have) {
+                    const uint32_t idx = base + (uint32_t)__popcll(need & below);
+                    if (idx < count) {
+                        q = idx;
+                        have = true;
+                        const uint32_t slot = ps.queue[cur][q];
+                        r = make_ray(ps.rayO[slot], ps.rayD[slot]);
+                        closest_packed_begin(sc.bvh, sc.scenes[ps.scene[slot]].hdr, r, w, h);
+                    }
+                }
+            }
+        }
+        if (!__any(have)) break;
+        if (have) {
+            closest_packed_steps(sc.bvh, r, stack, BLOCK, w, h, budget);
+            if (w.cur < 0) {
+                ps.hit[q] = make_float4(h.u, h.v, h.t, bits_to_float(h.inst));
+                ps.hitPrim[q] = h.prim;
+                have = false;
+            }
+        }
+    }
+}
+
+// Closest hit of every queued path.  mode 0: one lane per queue entry.
+// mode 1: a fixed grid of waves fetches 64 entries at a time from a work
+// counter until the queue is exhausted (every wave reaches the exit test, the
+// counter only grows).  mode 2: a fixed grid with lane refill.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_pt_extend(SceneView sc, PathState ps, int depth, int mode, int budget)
 {
     __shared__ uint32_t stack[kStackDepth * BLOCK];
     const int cur = depth & 1;
     const uint32_t count = ps.counters[cur];
-    if (!persistent) {
+    if (mode == 0) {
         const uint32_t q = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
         if (q < count) extend_one<BLOCK>(sc, ps, cur, q, stack + threadIdx.x);
+        return;
+    }
+    if (mode == 2) {
+        extend_refill<BLOCK>(sc, ps, cur, count, stack + threadIdx.x, budget);
         return;
     }
     const uint32_t lane = threadIdx.x % 64;
@@ -435,9 +491,10 @@ template <int BLOCK>
 void launchExtend(const SceneView& sc, PathState& ps, int depth, const LaunchConfig& lc, uint32_t n, hipStream_t s)
 {
     if (lc.persistentBlocks)
-        hipLaunchKernelGGL(k_pt_extend<BLOCK>, dim3(lc.persistentBlocks), dim3(BLOCK), 0, s, sc, ps, depth, 1);
+        hipLaunchKernelGGL(k_pt_extend<BLOCK>, dim3(lc.persistentBlocks), dim3(BLOCK), 0, s, sc, ps, depth,
+                           lc.laneRefill ? 2 : 1, (int)lc.refillBudget);
     else
-        hipLaunchKernelGGL(k_pt_extend<BLOCK>, dim3(blocks(n, BLOCK)), dim3(BLOCK), 0, s, sc, ps, depth, 0);
+        hipLaunchKernelGGL(k_pt_extend<BLOCK>, dim3(blocks(n, BLOCK)), dim3(BLOCK), 0, s, sc, ps, depth, 0, 0);
 }
 
 } // namespace

@@ -140,9 +140,9 @@ void addQuadLight(Scene& sc, float y, float size, float intensity)
     sc.createLight(d);
 }
 
-void addInwardBox(Scene& sc, float half, int32_t material)
+void addBox(Scene& sc, float half, int32_t material, bool inward)
 {
-    // six faces, each two triangles, per-face vertices with inward normals
+    // six faces, each two triangles, per-face vertices with inward (or outward) normals
     std::vector<StkVertex> vb;
     std::vector<uint32_t> ib;
     for (int axis = 0; axis < 3; ++axis)
@@ -156,7 +156,7 @@ void addInwardBox(Scene& sc, float half, int32_t material)
                 corners[k][v] = half * sv[k];
             }
             F3 n = mk3(0, 0, 0);
-            (&n.x)[axis] = -(float)side; // inward
+            (&n.x)[axis] = inward ? -(float)side : (float)side;
             const uint32_t base = (uint32_t)vb.size();
             for (int k = 0; k < 4; ++k) {
                 StkVertex vx{};
@@ -167,7 +167,7 @@ void addInwardBox(Scene& sc, float half, int32_t material)
                 vx.tangent = packNormal(mk3(1.0f, 0.0f, 0.0f));
                 vb.push_back(vx);
             }
-            // orient the winding so the geometric normal points inwards
+            // orient the winding so the geometric normal agrees with n
             const F3 p0 = ld3(corners[0]), p1 = ld3(corners[1]), p2 = ld3(corners[2]);
             const bool flip = dot(cross(p1 - p0, p2 - p0), n) < 0.0f;
             const uint32_t t[6] = {0, 1, 2, 0, 2, 3};
@@ -178,6 +178,24 @@ void addInwardBox(Scene& sc, float half, int32_t material)
             }
         }
     const uint32_t mesh = sc.createMesh(vb, ib);
+    sc.createInstance(mesh, material, Mat4::identity());
+}
+
+void addInwardBox(Scene& sc, float half, int32_t material) { addBox(sc, half, material, true); }
+
+// Two-triangle floor y = 0 over [-extent, extent]^2, normal +y.
+void addFloor(Scene& sc, float extent, int32_t material)
+{
+    std::vector<StkVertex> vb(4);
+    const float xs[4] = {-extent, extent, extent, -extent}, zs[4] = {-extent, -extent, extent, extent};
+    for (int k = 0; k < 4; ++k) {
+        vb[k] = StkVertex{};
+        vb[k].position[0] = xs[k];
+        vb[k].position[2] = zs[k];
+        vb[k].normal = packNormal(mk3(0.0f, 1.0f, 0.0f));
+        vb[k].tangent = packNormal(mk3(1.0f, 0.0f, 0.0f));
+    }
+    const uint32_t mesh = sc.createMesh(vb, {0, 2, 1, 0, 3, 2}); // (p_b - p_a) x (p_c - p_a) points to +y
     sc.createInstance(mesh, material, Mat4::identity());
 }
 
@@ -263,6 +281,31 @@ extern "C" STK_API stk_status stk_scene_generate(uint32_t config, uint32_t varia
             s->sky[0] = 0.5f;
             s->sky[1] = 0.6f;
             s->sky[2] = 0.8f;
+        } else if (config == 8) {
+            // convex furnace: an outward-facing Lambert cube under a constant sky, no lights
+            StkMaterial m{};
+            m.type = STK_MAT_LAMBERT;
+            m.roughness = 1.0f;
+            m.albedo[0] = 0.8f;
+            m.albedo[1] = 0.6f;
+            m.albedo[2] = 0.4f;
+            m.albedo[3] = 1.0f;
+            addBox(sc, 1.0f, (int32_t)sc.addMaterial(m), false);
+            cameraMatrices(mk3(3.0f, 2.4f, 3.6f), mk3(0, 0, 0), mk3(0, 1, 0), 40.0f, 96, 96, 0.01f, 100.0f,
+                           s->camera);
+            s->sky[0] = 0.5f;
+            s->sky[1] = 0.7f;
+            s->sky[2] = 1.0f;
+        } else if (config == 9) {
+            // lit floor: Lambert floor (albedo 0.5) under a 1x1 rect light at y = 2 facing down
+            StkMaterial m{};
+            m.type = STK_MAT_LAMBERT;
+            m.roughness = 1.0f;
+            m.albedo[0] = m.albedo[1] = m.albedo[2] = 0.5f;
+            m.albedo[3] = 1.0f;
+            addFloor(sc, 20.0f, (int32_t)sc.addMaterial(m));
+            addQuadLight(sc, 2.0f, 1.0f, 10.0f);
+            cameraMatrices(mk3(0, 1.0f, 0), mk3(0, 0, 0), mk3(0, 0, -1), 60.0f, 64, 64, 0.01f, 100.0f, s->camera);
         } else {
             return STK_E_INVALID;
         }

@@ -125,23 +125,40 @@ __device__ __forceinline__ void miss(HitF& h)
     h.leaf = STK_INVALID_INDEX;
 }
 
-// Near-first packed traversal.  `stack` points at this lane's first slot; the
-// slots are `stride` words apart (LDS layout [depth][lane], conflict free).
-__device__ __forceinline__ bool closest_packed(const BvhView& bvh, const StkBvhHeader& hd, const RayF& r,
-                                               uint32_t* stack, int stride, HitF& h)
+// Near-first packed traversal, resumable: a walk is its ray, the node `cur`
+// (-1 when done), the stack depth `sp`, and the best hit so far.  `stack`
+// points at this lane's first slot; the slots are `stride` words apart (LDS
+// layout [depth][lane], conflict free).
+struct PackedWalk {
+    int cur;
+    int sp;
+    bool found;
+};
+
+// Starts a walk: handles the empty tree, a single-leaf tree and the root box.
+__device__ __forceinline__ void closest_packed_begin(const BvhView& bvh, const StkBvhHeader& hd, const RayF& r,
+                                                     PackedWalk& w, HitF& h)
 {
     miss(h);
-    bool found = false;
-    int cur = -1;
-    if (hd.root == INT32_MIN) return false;
+    w.found = false;
+    w.cur = -1;
+    w.sp = 0;
+    if (hd.root == INT32_MIN) return;
     if (hd.root < 0) {
-        leaf_closest(bvh, r, (uint32_t)~hd.root, found, h);
-        return found;
+        leaf_closest(bvh, r, (uint32_t)~hd.root, w.found, h);
+        return;
     }
     float t0;
-    if (slab(r, hd.lo[0], hd.hi[0], hd.lo[1], hd.hi[1], hd.lo[2], hd.hi[2], t0) && t0 <= r.tmax) cur = hd.root;
-    int sp = 0;
-    while (cur >= 0) {
+    if (slab(r, hd.lo[0], hd.hi[0], hd.lo[1], hd.hi[1], hd.lo[2], hd.hi[2], t0) && t0 <= r.tmax) w.cur = hd.root;
+}
+
+// Visits at most `budget` nodes (budget < 0: until the walk ends).
+__device__ __forceinline__ void closest_packed_steps(const BvhView& bvh, const RayF& r, uint32_t* stack, int stride,
+                                                     PackedWalk& w, HitF& h, int budget)
+{
+    int cur = w.cur, sp = w.sp;
+    bool found = w.found;
+    while (cur >= 0 && budget-- != 0) {
         const float4* p = bvh.packed + 4 * (size_t)cur;
         const float4 bx = p[0], by = p[1], bz = p[2];
         const int4 ch = *reinterpret_cast<const int4*>(p + 3);
@@ -166,7 +183,18 @@ __device__ __forceinline__ bool closest_packed(const BvhView& bvh, const StkBvhH
             cur = sp > 0 ? (int)stack[--sp * stride] : -1;
         }
     }
-    return found;
+    w.cur = cur;
+    w.sp = sp;
+    w.found = found;
+}
+
+__device__ __forceinline__ bool closest_packed(const BvhView& bvh, const StkBvhHeader& hd, const RayF& r,
+                                               uint32_t* stack, int stride, HitF& h)
+{
+    PackedWalk w;
+    closest_packed_begin(bvh, hd, r, w, h);
+    closest_packed_steps(bvh, r, stack, stride, w, h, -1);
+    return w.found;
 }
 
 // closestHit (shaders/raytracing.h:198-247), reference order, unpruned,
