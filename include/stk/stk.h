@@ -160,10 +160,7 @@ typedef struct StkLaunchConfig {
     uint32_t persistentBlocks;
     uint32_t useGraph;
     uint32_t pathsInFlight;
-    uint32_t laneRefill;   /* with persistentBlocks: lanes whose ray is done take a new one every
-                              refillBudget node visits instead of waiting for the wave */
-    uint32_t refillBudget; /* 0 = 16 */
-    uint32_t pad;
+    uint32_t pad[3];
 } StkLaunchConfig;
 STK_API stk_status stk_set_launch_config(stk_ctx* ctx, const StkLaunchConfig* lc);
 STK_API stk_status stk_get_launch_config(stk_ctx* ctx, StkLaunchConfig* lc);

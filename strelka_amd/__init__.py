@@ -305,18 +305,15 @@ class Context:
 
     # -- launch shape --------------------------------------------------------------
     def set_launch_config(self, block_size: int = 256, tile: int = 8, persistent_blocks: int = 0,
-                          graph: bool = False, paths_in_flight: int = 0, lane_refill: bool = False,
-                          refill_budget: int = 16):
-        lc = LaunchConfig(block_size, tile, persistent_blocks, 1 if graph else 0, paths_in_flight,
-                          1 if lane_refill else 0, refill_budget)
+                          graph: bool = False, paths_in_flight: int = 0):
+        lc = LaunchConfig(block_size, tile, persistent_blocks, 1 if graph else 0, paths_in_flight)
         self._check(_LIB.stk_set_launch_config(self._h, C.byref(lc)), "stk_set_launch_config")
 
     def launch_config(self) -> dict:
         lc = LaunchConfig()
         self._check(_LIB.stk_get_launch_config(self._h, C.byref(lc)), "stk_get_launch_config")
         return {"block_size": lc.blockSize, "tile": lc.tile, "persistent_blocks": lc.persistentBlocks,
-                "graph": bool(lc.useGraph), "paths_in_flight": lc.pathsInFlight,
-                "lane_refill": bool(lc.laneRefill), "refill_budget": lc.refillBudget}
+                "graph": bool(lc.useGraph), "paths_in_flight": lc.pathsInFlight}
 
     def device_info(self) -> dict:
         d = DeviceInfo()

@@ -4,8 +4,7 @@ The reference dispatches its compute passes with one fixed 16x16 workgroup
 (src/render/pathtracer.cpp); here the shape of the wavefront launches is a
 per-device, per-workload choice: threads per block of the wavefront kernels,
 the raygen tile edge (spatial coherence of a wave's rays), a persistent grid
-for the extend kernel with or without lane refill (and its refill interval),
-the number of paths traced per pass (so a pass's path state fits the
+for the extend kernel, the number of paths traced per pass (so a pass's path state fits the
 last-level cache) and graph replay of the pass.  None of them
 changes a single output bit (the launch-shape tests check that), so the tuner
 only measures time: it runs the caller's workload under each candidate with
@@ -25,10 +24,8 @@ BLOCK_SIZES = (64, 128, 256)
 TILES = (1, 4, 8, 16)
 PERSISTENT_PER_CU = (0, 4, 8, 16)
 PATHS_IN_FLIGHT = (0, 1 << 20, 1 << 22)
-REFILL_BUDGETS = (4, 8, 16, 32)
 
-DEFAULT = {"block_size": 256, "tile": 8, "persistent_blocks": 0, "graph": True, "paths_in_flight": 0,
-           "lane_refill": False, "refill_budget": 16}
+DEFAULT = {"block_size": 256, "tile": 8, "persistent_blocks": 0, "graph": True, "paths_in_flight": 0}
 
 
 def _key(shape: dict) -> tuple:
@@ -59,8 +56,6 @@ def tune(ctx, run: Callable[[], None], total_paths: int, repeats: int = 2, sweep
         "tile": list(TILES),
         "persistent_blocks": [k * cus for k in PERSISTENT_PER_CU],
         "paths_in_flight": [q for q in PATHS_IN_FLIGHT if q == 0 or q < total_paths],
-        "lane_refill": [False, True],
-        "refill_budget": list(REFILL_BUDGETS),
         "graph": [True, False],
     }
     seen: dict = {}
@@ -81,9 +76,6 @@ def tune(ctx, run: Callable[[], None], total_paths: int, repeats: int = 2, sweep
             for v in values:
                 cand = dict(best)
                 cand[axis] = v
-                if (cand["persistent_blocks"] == 0 or not cand["lane_refill"]) and \
-                        (axis == "refill_budget" or (axis == "lane_refill" and v and cand["persistent_blocks"] == 0)):
-                    continue  # refill applies to the persistent grid only, its budget to refill only
                 ms = cost(cand)
                 if ms < best_ms * 0.995:
                     best, best_ms, improved = cand, ms, True

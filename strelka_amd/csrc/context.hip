@@ -431,27 +431,31 @@ STK_API stk_status stk_generate_camera_rays_device(stk_ctx* c, const StkPathTrac
 
 static stk_status ensurePathState(stk_ctx* c, size_t slots)
 {
-    if (c->ps.capacity >= slots) return STK_OK;
+    const size_t qcap = (size_t)stk::kStripes * stk::stripeSegment(slots);
+    if (c->ps.capacity >= slots && c->ps.queueCapacity >= qcap) return STK_OK;
     invalidateGraph(c);
     stk_status st;
     for (DevBuf& b : c->psBufs) devRelease(c, b);
-    c->psBufs.assign(19, DevBuf{});
+    c->psBufs.assign(20, DevBuf{});
     stk::PathState& ps = c->ps;
+    ps.capacity = 0;
+    ps.queueCapacity = 0;
     const size_t f4 = slots * sizeof(float4), u4 = slots * sizeof(uint32_t);
-    void** ptrs[] = {(void**)&ps.rayO,     (void**)&ps.rayD,     (void**)&ps.thr,     (void**)&ps.col,
-                     (void**)&ps.rng,      (void**)&ps.scene,    (void**)&ps.pixel,   (void**)&ps.hit,
+    const size_t qf4 = qcap * sizeof(float4), qu4 = qcap * sizeof(uint32_t);
+    void** ptrs[] = {(void**)&ps.rayO,     (void**)&ps.rayD,     (void**)&ps.thr,      (void**)&ps.col,
+                     (void**)&ps.rng,      (void**)&ps.scene,    (void**)&ps.pixel,    (void**)&ps.hit,
                      (void**)&ps.hitPrim,  (void**)&ps.queue[0], (void**)&ps.queue[1], (void**)&ps.shadowQ,
-                     (void**)&ps.shO,      (void**)&ps.shD,      (void**)&ps.shCd,    (void**)&ps.shCg,
-                     (void**)&ps.counters, (void**)&ps.pass,     nullptr};
-    const size_t sizes[] = {f4, f4, f4, f4, u4, u4, u4, f4, u4, u4, u4, u4, f4, f4, f4, f4, 64, 64};
+                     (void**)&ps.shO,      (void**)&ps.shD,      (void**)&ps.shCd,     (void**)&ps.shCg,
+                     (void**)&ps.counters, (void**)&ps.pass,     (void**)&ps.stripes,  (void**)&ps.work};
+    const size_t sizes[] = {f4, f4, f4, f4, u4, u4, u4, qf4, qu4, qu4, qu4, qu4, qf4, qf4, qf4, qf4, 64, 64,
+                            stk::kQueueKinds * stk::kStripes * sizeof(uint32_t), stk::kStripes * sizeof(uint32_t)};
+    static_assert(sizeof(sizes) / sizeof(sizes[0]) == sizeof(ptrs) / sizeof(ptrs[0]), "path state buffers");
     for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); ++i) {
-        if ((st = devReserve(c, c->psBufs[i], sizes[i])) != STK_OK) {
-            ps.capacity = 0;
-            return st;
-        }
+        if ((st = devReserve(c, c->psBufs[i], sizes[i])) != STK_OK) return st;
         *ptrs[i] = c->psBufs[i].ptr;
     }
     ps.capacity = (uint32_t)slots;
+    ps.queueCapacity = qcap;
     return STK_OK;
 }
 
@@ -527,6 +531,7 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
     lp.refCompat = p->refCompat;
     lp.invSpp = 1.0f / (float)p->pt.spp;
     lp.passSlots = (uint32_t)Q;
+    c->ps.segCap = stk::stripeSegment(Q);
     stk::PtCache& pc = lp.cache;
     if (p->cacheMode != 0) {
         if (!allowCache) return fail(c, STK_E_INVALID, "render: the radiance cache serves single-scene renders");
@@ -566,8 +571,7 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
                                      (uint64_t)sv.inst, (uint64_t)sv.lights, (uint64_t)sv.mats, sv.numMats,
                                      (uint64_t)sv.scenes, sv.numScenes, sv.totalSlots, sv.tile, lp.maxDepth,
                                      lp.spp, lp.samplingType, lp.rrStartDepth, lp.refCompat, lp.passSlots,
-                                     c->launch.blockSize, c->launch.tile, c->launch.persistentBlocks,
-                                     c->launch.laneRefill, c->launch.refillBudget,
+                                     c->launch.blockSize, c->launch.tile, c->launch.persistentBlocks, c->ps.segCap,
                                      (uint64_t)accum, (uint64_t)samples, (uint64_t)c->ps.rayO, c->ps.capacity};
         if (!c->graphExec || key != c->graphKey) {
             invalidateGraph(c);
@@ -914,8 +918,6 @@ STK_API stk_status stk_set_launch_config(stk_ctx* c, const StkLaunchConfig* lc)
     c->launch.tile = lc->tile;
     c->launch.persistentBlocks = lc->persistentBlocks;
     c->launch.pathsInFlight = lc->pathsInFlight;
-    c->launch.laneRefill = lc->laneRefill ? 1u : 0u;
-    c->launch.refillBudget = lc->refillBudget ? std::min<uint32_t>(lc->refillBudget, 4096u) : 16u;
     c->useGraph = lc->useGraph != 0;
     invalidateGraph(c);
     return STK_OK;
@@ -947,9 +949,7 @@ STK_API stk_status stk_get_launch_config(stk_ctx* c, StkLaunchConfig* lc)
     lc->persistentBlocks = c->launch.persistentBlocks;
     lc->useGraph = c->useGraph ? 1u : 0u;
     lc->pathsInFlight = c->launch.pathsInFlight;
-    lc->laneRefill = c->launch.laneRefill;
-    lc->refillBudget = c->launch.refillBudget;
-    lc->pad = 0;
+    lc->pad[0] = lc->pad[1] = lc->pad[2] = 0;
     return STK_OK;
 }
 

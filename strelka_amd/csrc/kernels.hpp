@@ -50,10 +50,23 @@ struct PassState {
     uint32_t slotCount;
 };
 
+// Ray queues are kStripes segments of segCap entries each (see pathtrace.hip).
+constexpr uint32_t kStripes = 64;
+constexpr int kQueueKinds = 3; // path queue 0, path queue 1, shadow queue
+
+// Entries per stripe segment for a pass of n slots produced by 256-thread
+// blocks (block b appends to segment b % kStripes, at most 256 entries).
+inline uint32_t stripeSegment(size_t n) { return (uint32_t)(((n + 255) / 256 + kStripes - 1) / kStripes * 256); }
+
 // Device arrays of one wavefront path-tracing pass (one sample of the pixels of
-// the pass's slots), indexed by the slot's position in the pass.
+// the pass's slots).  Per-slot arrays are indexed by the slot's position in
+// the pass; hit, hitPrim and the queues by queue entry (stripe * segCap + i).
 struct PathState {
-    uint32_t capacity = 0;
+    uint32_t capacity = 0;      // slots
+    size_t queueCapacity = 0;   // entries of each queue-indexed array
+    uint32_t segCap = 0;        // entries per stripe segment in the current pass
+    uint32_t* stripes = nullptr; // [kQueueKinds][kStripes] entries per segment
+    uint32_t* work = nullptr;    // [kStripes] persistent-extend fetch counters
     float4* rayO = nullptr;  // origin.xyz, tmax
     float4* rayD = nullptr;  // direction.xyz
     float4* thr = nullptr;   // throughput.rgb
@@ -69,7 +82,7 @@ struct PathState {
     float4* shD = nullptr;
     float4* shCd = nullptr; // NEE contribution, diffuse part
     float4* shCg = nullptr; // NEE contribution, glossy part
-    uint32_t* counters = nullptr; // [0],[1] queue counts, [2] shadow, [3] queries, [4] train records, [5] work,
+    uint32_t* counters = nullptr; // [0],[1] queue totals, [2] shadow total, [3] queries, [4] train records,
                                   // [8..11] two 64-bit ray totals of the render call (closest, shadow)
     PassState* pass = nullptr;
 };
@@ -109,8 +122,6 @@ struct LaunchConfig {
     uint32_t tile = 8;             // raygen tile edge (1 = scanline)
     uint32_t persistentBlocks = 0; // extend grid for persistent waves (0: one lane per ray)
     uint32_t pathsInFlight = 0;    // slots per pass (0: every slot of the frame in one pass)
-    uint32_t laneRefill = 0;       // persistent extend: idle lanes fetch new rays mid-walk
-    uint32_t refillBudget = 16;    // node visits between refills
 };
 
 hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, bool refOrder,

@@ -4,8 +4,13 @@
 // (shaders/pathtracerMdl.hlsl:204-423) with wavefront stages:
 //   reset -> raygen -> per bounce { setup, extend (closest hit), shade (NEE
 //   sample, BSDF sample, RR; compacts continuing paths and shadow rays into
-//   queues with one atomic per wave), connect (any hit, adds NEE) } ->
-//   [cache answers] -> finalize -> [cache training].
+//   queues), connect (any hit, adds NEE) } -> [cache answers] -> finalize ->
+//   [cache training].
+// Queues are striped: a queue is kStripes segments of segCap entries, block b
+// of a producer appends to segment b % kStripes (one atomic per wave on that
+// segment's counter), and consumers run stripe-major grids.  Appends to one
+// counter serialise in L2 (about 11 ns each on MI355X, 0.37 ms for the 32,400
+// waves of a 1080p pass); 64 counters cut that to a few microseconds.
 // A path's sequence of random numbers and of additions to its radiance is the
 // reference's, so its value does not depend on the wavefront organisation or
 // on how many views share the launch.
@@ -65,9 +70,32 @@ __global__ void k_pass_set(PassState* pass, PassState v)
     if (threadIdx.x == 0) *pass = v;
 }
 
+// Clears the pass counters and every stripe counter (one 64-thread block).
 __global__ void k_pass_reset(PathState ps)
 {
-    if (threadIdx.x < 8) ps.counters[threadIdx.x] = 0;
+    const uint32_t t = threadIdx.x;
+    if (t < 8) ps.counters[t] = 0;
+    if (t < kStripes) {
+        for (int k = 0; k < kQueueKinds; ++k) ps.stripes[k * kStripes + t] = 0;
+        ps.work[t] = 0;
+    }
+}
+
+// Entry `e` of a stripe-major consumer grid of B-thread blocks: block b reads
+// segment b % kStripes; false past the segment's count.
+__device__ __forceinline__ bool stripe_entry(const PathState& ps, const uint32_t* counts, uint32_t B, uint32_t& e)
+{
+    const uint32_t stripe = blockIdx.x % kStripes;
+    const uint32_t local = (blockIdx.x / kStripes) * B + threadIdx.x;
+    e = stripe * ps.segCap + local;
+    return local < counts[stripe];
+}
+
+// Sum over a 64-thread block (one wave).
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o);
+    return v;
 }
 
 // Path state arrays are indexed by the slot's position in the pass (`local`);
@@ -105,8 +133,9 @@ __global__ __launch_bounds__(kBlock) void k_pt_raygen(SceneView sc, PtLaunch lp,
         // (the cache runs with one pass over all slots, so local == slot there)
         if (is_train(lp.cache, pass, local)) lp.cache.trainCount[local / lp.cache.trainStride] = 0;
     }
-    const uint32_t qi = wave_push(&ps.counters[0], valid);
-    if (valid) ps.queue[0][qi] = local;
+    const uint32_t stripe = blockIdx.x % kStripes;
+    const uint32_t qi = wave_push(&ps.stripes[stripe], valid);
+    if (valid) ps.queue[0][stripe * ps.segCap + qi] = local;
 }
 
 // Ray totals of a render call: two 64-bit words after the eight pass counters
@@ -116,21 +145,34 @@ __device__ __forceinline__ unsigned long long* ray_totals(const PathState& ps)
     return reinterpret_cast<unsigned long long*>(ps.counters + 8);
 }
 
+// Start of a bounce (one 64-thread block): tallies the rays of this depth's
+// queue and the previous depth's shadow rays, clears the next queue, the
+// shadow queue and the extend work counters.
 __global__ void k_pt_setup(PathState ps, int depth)
 {
-    if (threadIdx.x == 0) {
+    const uint32_t t = threadIdx.x;
+    const int cur = depth & 1;
+    const uint32_t nq = wave_sum(ps.stripes[cur * kStripes + t]);
+    const uint32_t ns = wave_sum(ps.stripes[2 * kStripes + t]);
+    ps.stripes[(1 - cur) * kStripes + t] = 0;
+    ps.stripes[2 * kStripes + t] = 0;
+    ps.work[t] = 0;
+    if (t == 0) {
         unsigned long long* tot = ray_totals(ps);
-        tot[0] += ps.counters[depth & 1]; // rays extended at this depth
-        tot[1] += ps.counters[2];         // shadow rays of the previous depth (0 at depth 0)
-        ps.counters[1 - (depth & 1)] = 0;
-        ps.counters[2] = 0;
-        ps.counters[5] = 0;
+        tot[0] += nq; // rays extended at this depth
+        tot[1] += ns; // shadow rays of the previous depth (0 at depth 0)
+        ps.counters[cur] = nq;
+        if (depth > 0) ps.counters[2] = ns;
     }
 }
 
 __global__ void k_pt_tally(PathState ps)
 {
-    if (threadIdx.x == 0) ray_totals(ps)[1] += ps.counters[2];
+    const uint32_t ns = wave_sum(ps.stripes[2 * kStripes + threadIdx.x]);
+    if (threadIdx.x == 0) {
+        ray_totals(ps)[1] += ns;
+        ps.counters[2] = ns;
+    }
 }
 
 __global__ void k_totals_clear(PathState ps)
@@ -139,103 +181,54 @@ __global__ void k_totals_clear(PathState ps)
 }
 
 template <int BLOCK>
-__device__ __forceinline__ void extend_one(const SceneView& sc, PathState& ps, int cur, uint32_t q, uint32_t* stack)
+__device__ __forceinline__ void extend_one(const SceneView& sc, PathState& ps, int cur, uint32_t e, uint32_t* stack)
 {
-    const uint32_t slot = ps.queue[cur][q];
+    const uint32_t slot = ps.queue[cur][e];
     const RayF r = make_ray(ps.rayO[slot], ps.rayD[slot]);
     HitF h;
     closest_packed(sc.bvh, sc.scenes[ps.scene[slot]].hdr, r, stack, BLOCK, h);
-    ps.hit[q] = make_float4(h.u, h.v, h.t, bits_to_float(h.inst));
-    ps.hitPrim[q] = h.prim;
+    ps.hit[e] = make_float4(h.u, h.v, h.t, bits_to_float(h.inst));
+    ps.hitPrim[e] = h.prim;
 }
 
-// Lane refill: each lane walks its own ray; every `budget` node visits the
-// lanes whose walk ended store their hit and take the next queue entries (one
-// atomic per wave), so a wave keeps all lanes busy while rays of very
-// different lengths finish (Aila and Laine 2009, "while-while" with dynamic
-// fetch).  The walk of a ray is the same as in extend_one, so is its hit.
-// Exit: once the counter passes the queue length the wave stops fetching and
-// leaves when its last walk ends.
+// Closest hit of every queued path.  persistent = 0: one lane per queue
+// entry, stripe-major grid.  persistent = 1: a fixed grid of waves; wave w
+// takes 64 entries at a time from segment (w + k) % kStripes, k = 0, 1, ...,
+// moving on when a segment is exhausted (a plain load checks before the
+// atomic), so every wave ends after at most kStripes segments.
 template <int BLOCK>
-__device__ __forceinline__ void extend_refill(const SceneView& sc, PathState& ps, int cur, uint32_t count,
-                                              uint32_t* stack, int budget)
-{
-    const uint32_t lane = __lane_id();
-    const uint64_t below = (1ull << lane) - 1ull;
-    bool have = false, exhausted = false;
-    uint32_t q = 0;
-    RayF r;
-    HitF h;
-    PackedWalk w;
-    for (;;) {
-        if (!exhausted) {
-            const uint64_t need = __ballot(!have);
-            if (need) {
-                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)need) - 1;
-                const uint32_t n = (uint32_t)__popcll(need);
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(&ps.counters[5], n);
-                base = (uint32_t)__shfl((int)base, (int)leader);
-                exhausted = base + n >= count;
-                if (!have) {
-                    const uint32_t idx = base + (uint32_t)__popcll(need & below);
-                    if (idx < count) {
-                        q = idx;
-                        have = true;
-                        const uint32_t slot = ps.queue[cur][q];
-                        r = make_ray(ps.rayO[slot], ps.rayD[slot]);
-                        closest_packed_begin(sc.bvh, sc.scenes[ps.scene[slot]].hdr, r, w, h);
-                    }
-                }
-            }
-        }
-        if (!__any(have)) break;
-        if (have) {
-            closest_packed_steps(sc.bvh, r, stack, BLOCK, w, h, budget);
-            if (w.cur < 0) {
-                ps.hit[q] = make_float4(h.u, h.v, h.t, bits_to_float(h.inst));
-                ps.hitPrim[q] = h.prim;
-                have = false;
-            }
-        }
-    }
-}
-
-// Closest hit of every queued path.  mode 0: one lane per queue entry.
-// mode 1: a fixed grid of waves fetches 64 entries at a time from a work
-// counter until the queue is exhausted (every wave reaches the exit test, the
-// counter only grows).  mode 2: a fixed grid with lane refill.
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_pt_extend(SceneView sc, PathState ps, int depth, int mode, int budget)
+__global__ __launch_bounds__(BLOCK) void k_pt_extend(SceneView sc, PathState ps, int depth, int persistent)
 {
     __shared__ uint32_t stack[kStackDepth * BLOCK];
     const int cur = depth & 1;
-    const uint32_t count = ps.counters[cur];
-    if (mode == 0) {
-        const uint32_t q = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
-        if (q < count) extend_one<BLOCK>(sc, ps, cur, q, stack + threadIdx.x);
-        return;
-    }
-    if (mode == 2) {
-        extend_refill<BLOCK>(sc, ps, cur, count, stack + threadIdx.x, budget);
+    const uint32_t* counts = ps.stripes + cur * kStripes;
+    if (!persistent) {
+        uint32_t e;
+        if (stripe_entry(ps, counts, BLOCK, e)) extend_one<BLOCK>(sc, ps, cur, e, stack + threadIdx.x);
         return;
     }
     const uint32_t lane = threadIdx.x % 64;
-    for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&ps.counters[5], 64u);
-        base = (uint32_t)__shfl((int)base, 0);
-        if (base >= count) break;
-        const uint32_t q = base + lane;
-        if (q < count) extend_one<BLOCK>(sc, ps, cur, q, stack + threadIdx.x);
+    const uint32_t wave = (blockIdx.x * BLOCK + threadIdx.x) / 64;
+    for (uint32_t k = 0; k < kStripes; ++k) {
+        const uint32_t sg = (wave + k) % kStripes;
+        const uint32_t cnt = counts[sg];
+        for (;;) {
+            uint32_t base = cnt;
+            if (lane == 0 && __hip_atomic_load(&ps.work[sg], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < cnt)
+                base = atomicAdd(&ps.work[sg], 64u);
+            base = (uint32_t)__shfl((int)base, 0);
+            if (base >= cnt) break;
+            const uint32_t q = base + lane;
+            if (q < cnt) extend_one<BLOCK>(sc, ps, cur, sg * ps.segCap + q, stack + threadIdx.x);
+        }
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, PathState ps, int depth)
 {
     const int cur = depth & 1;
-    const uint32_t count = ps.counters[cur];
-    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t q;
+    const bool active = stripe_entry(ps, ps.stripes + cur * kStripes, kBlock, q);
     bool cont = false, shadow = false, cacheQuery = false;
     uint32_t slot = 0;
     F3 nO = mk3(0, 0, 0), nD = mk3(0, 0, 0);
@@ -243,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, 
     float sT = 0.0f;
     CacheRecord qrec;
     const PtCache& pc = lp.cache;
-    if (q < count) {
+    if (active) {
         slot = ps.queue[cur][q];
         const SceneDesc& desc = sc.scenes[ps.scene[slot]];
         const float4 h = ps.hit[q];
@@ -377,15 +370,17 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, 
             ps.rayD[slot] = make_float4(nD.x, nD.y, nD.z, 0.0f);
         }
     }
-    const uint32_t qi = wave_push(&ps.counters[1 - cur], cont);
-    if (cont) ps.queue[1 - cur][qi] = slot;
-    const uint32_t si = wave_push(&ps.counters[2], shadow);
+    const uint32_t stripe = blockIdx.x % kStripes;
+    const uint32_t qi = wave_push(&ps.stripes[(1 - cur) * kStripes + stripe], cont);
+    if (cont) ps.queue[1 - cur][stripe * ps.segCap + qi] = slot;
+    const uint32_t si = wave_push(&ps.stripes[2 * kStripes + stripe], shadow);
     if (shadow) {
-        ps.shadowQ[si] = slot;
-        ps.shO[si] = make_float4(sO.x, sO.y, sO.z, sT);
-        ps.shD[si] = make_float4(sD.x, sD.y, sD.z, 0.0f);
-        ps.shCd[si] = f4(cd);
-        ps.shCg[si] = f4(cg);
+        const uint32_t se = stripe * ps.segCap + si;
+        ps.shadowQ[se] = slot;
+        ps.shO[se] = make_float4(sO.x, sO.y, sO.z, sT);
+        ps.shD[se] = make_float4(sD.x, sD.y, sD.z, 0.0f);
+        ps.shCd[se] = f4(cd);
+        ps.shCg[se] = f4(cg);
     }
     if (pc.mode == 1 || pc.mode == 2) {
         const uint32_t ci = wave_push(&ps.counters[3], cacheQuery);
@@ -398,9 +393,8 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, 
 // SIMD instead of 5) while returning the same first-in-preorder hit.
 __global__ __launch_bounds__(kBlock) void k_pt_connect(SceneView sc, PathState ps)
 {
-    const uint32_t count = ps.counters[2];
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (s >= count) return;
+    uint32_t s;
+    if (!stripe_entry(ps, ps.stripes + 2 * kStripes, kBlock, s)) return;
     const uint32_t slot = ps.shadowQ[s];
     const SceneDesc& desc = sc.scenes[ps.scene[slot]];
     const RayF r = make_ray(ps.shO[s], ps.shD[s]);
@@ -487,14 +481,16 @@ __global__ __launch_bounds__(kBlock) void k_cache_targets(PtLaunch lp, PathState
 
 inline uint32_t blocks(size_t n, uint32_t b = kBlock) { return (uint32_t)((n + b - 1) / b); }
 
+// Stripe-major consumer grid of B-thread blocks (segCap is a multiple of 256).
+inline dim3 stripeGrid(const PathState& ps, uint32_t B) { return dim3(kStripes * (ps.segCap / B)); }
+
 template <int BLOCK>
-void launchExtend(const SceneView& sc, PathState& ps, int depth, const LaunchConfig& lc, uint32_t n, hipStream_t s)
+void launchExtend(const SceneView& sc, PathState& ps, int depth, const LaunchConfig& lc, hipStream_t s)
 {
     if (lc.persistentBlocks)
-        hipLaunchKernelGGL(k_pt_extend<BLOCK>, dim3(lc.persistentBlocks), dim3(BLOCK), 0, s, sc, ps, depth,
-                           lc.laneRefill ? 2 : 1, (int)lc.refillBudget);
+        hipLaunchKernelGGL(k_pt_extend<BLOCK>, dim3(lc.persistentBlocks), dim3(BLOCK), 0, s, sc, ps, depth, 1);
     else
-        hipLaunchKernelGGL(k_pt_extend<BLOCK>, dim3(blocks(n, BLOCK)), dim3(BLOCK), 0, s, sc, ps, depth, 0, 0);
+        hipLaunchKernelGGL(k_pt_extend<BLOCK>, stripeGrid(ps, BLOCK), dim3(BLOCK), 0, s, sc, ps, depth, 0);
 }
 
 } // namespace
@@ -515,7 +511,9 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
                                float4* accum, float* samples, CacheBuffers* cb, hipStream_t s)
 {
     const uint32_t n = lp.passSlots;
-    if (n == 0 || n > ps.capacity) return hipErrorInvalidValue;
+    if (n == 0 || n > ps.capacity || ps.segCap % 256 != 0 || (size_t)ps.segCap * kStripes > ps.queueCapacity ||
+        (size_t)ps.segCap * kStripes < n)
+        return hipErrorInvalidValue;
     const dim3 g(blocks(n)), b(kBlock);
     const PtCache& pc = lp.cache;
     const bool query = pc.mode == 1 || pc.mode == 2, train = pc.mode == 1 || pc.mode == 3;
@@ -524,12 +522,12 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
     for (int depth = 0; depth < (int)lp.maxDepth; ++depth) {
         hipLaunchKernelGGL(k_pt_setup, dim3(1), dim3(64), 0, s, ps, depth);
         switch (lc.blockSize) {
-        case 64: launchExtend<64>(sc, ps, depth, lc, n, s); break;
-        case 128: launchExtend<128>(sc, ps, depth, lc, n, s); break;
-        default: launchExtend<256>(sc, ps, depth, lc, n, s); break;
+        case 64: launchExtend<64>(sc, ps, depth, lc, s); break;
+        case 128: launchExtend<128>(sc, ps, depth, lc, s); break;
+        default: launchExtend<256>(sc, ps, depth, lc, s); break;
         }
-        hipLaunchKernelGGL(k_pt_shade, g, b, 0, s, sc, lp, ps, depth);
-        hipLaunchKernelGGL(k_pt_connect, g, b, 0, s, sc, ps);
+        hipLaunchKernelGGL(k_pt_shade, stripeGrid(ps, kBlock), b, 0, s, sc, lp, ps, depth);
+        hipLaunchKernelGGL(k_pt_connect, stripeGrid(ps, kBlock), b, 0, s, sc, ps);
     }
     hipLaunchKernelGGL(k_pt_tally, dim3(1), dim3(64), 0, s, ps);
     if (query && cb) {

@@ -96,9 +96,8 @@ SHAPES = [dict(block_size=64, tile=1), dict(block_size=128, tile=16), dict(block
           dict(block_size=256, tile=8, graph=True), dict(block_size=128, tile=16, persistent_blocks=2048, graph=True),
           dict(block_size=256, tile=8, paths_in_flight=4096),
           dict(block_size=128, tile=8, paths_in_flight=100000, persistent_blocks=1024, graph=True),
-          dict(block_size=128, tile=8, persistent_blocks=2048, lane_refill=True, refill_budget=8),
-          dict(block_size=64, tile=4, persistent_blocks=512, lane_refill=True, refill_budget=1, graph=True),
-          dict(block_size=256, tile=16, persistent_blocks=4096, lane_refill=True, refill_budget=64)]
+          dict(block_size=128, tile=8, persistent_blocks=8192, graph=True),
+          dict(block_size=64, tile=1, persistent_blocks=1, paths_in_flight=20000)]
 
 
 def test_launch_shapes_give_identical_images(scenes):
