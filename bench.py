@@ -186,8 +186,11 @@ def load_traffic(kernel: str):
 def cache_study(args) -> dict:
     """Time to reach an RMSE target against a converged plain render, for the
     plain path tracer and the cached one (training included in its time).
-    The 1.16M-triangle heightfield + spheres scene at the study resolution, 1 spp per frame, 8 bounces,
-    RR after 3, cache queried after 2 bounces, ACES-tonemapped RMSE."""
+    The 1.16M-triangle heightfield + spheres scene at the study resolution,
+    1 spp per frame, 8 bounces, RR after 3, ACES-tonemapped RMSE.  A cached
+    run starts from a fresh cache: `warmup` frames train on the training
+    paths only (their time counts), then the accumulation restarts and every
+    frame queries the cache at vertex depth `query_depth` and trains."""
     import strelka_amd as sa
 
     scene = sa.SceneData.generate(2)
@@ -198,20 +201,21 @@ def cache_study(args) -> dict:
     ctx = sa.Context(0)
     ctx.set_scene(scene)
     ctx.build_bvh()
+    ctx.set_launch_config(block_size=64, tile=8)
     npix = W * H
-    train_paths = max(1024, int(65536 * npix / (1920 * 1080)))
+    default_paths = max(1024, int(65536 * npix / (1920 * 1080)))
     d_out = ctx.alloc(npix * 16)
     out = np.zeros((H, W, 4), np.float32)
 
-    def params(frame, mode):
+    def params(frame, mode, qd=2, paths=default_paths):
         return sa.render_params(cam, spp=1, max_depth=8, frame=frame, sampling=1, sky=scene.sky, tonemap=1,
-                                cache_mode=mode, cache_query_depth=2, cache_train_paths=train_paths)
+                                cache_mode=mode, cache_query_depth=qd, cache_train_paths=paths)
 
     ctx.reset_accumulation()
     t0 = time.perf_counter()
     for f in range(args.study_ref_spp):
         ctx.render_device(params(1_000_000 + f, 0), d_out)
-        if f % 64 == 63:
+        if f % 256 == 255:
             ctx.sync()
             print(f"reference {f + 1}/{args.study_ref_spp} spp", file=sys.stderr, flush=True)
     ctx.sync()
@@ -221,12 +225,18 @@ def cache_study(args) -> dict:
 
     checkpoints = sorted({1, 2, 4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512} | {args.study_frames})
 
-    def run(mode):
+    def run(mode, qd=2, warmup=0, paths=default_paths, label=""):
+        t_ms = 0.0
+        for f in range(warmup):
+            ctx.record(0)
+            ctx.render_device(params(500_000 + f, 4, qd, paths), d_out)
+            ctx.record(1)
+            t_ms += ctx.elapsed_ms(0, 1)
         ctx.reset_accumulation()
-        curve, t_ms = [], 0.0
+        curve = []
         for f in range(args.study_frames):
             ctx.record(0)
-            ctx.render_device(params(f, mode), d_out)
+            ctx.render_device(params(f, mode, qd, paths), d_out)
             ctx.record(1)
             t_ms += ctx.elapsed_ms(0, 1)
             if f + 1 in checkpoints:
@@ -235,14 +245,8 @@ def cache_study(args) -> dict:
                 curve.append({"frames": f + 1, "ms": round(t_ms, 2),
                               "rmse": round(float(np.sqrt(np.mean((tm - ref_tm) ** 2))), 5),
                               "rmse_linear": round(float(np.sqrt(np.mean((lin - ref_lin) ** 2))), 5)})
-                print(f"mode {mode} frame {f + 1}: {curve[-1]}", file=sys.stderr, flush=True)
+                print(f"{label} frame {f + 1}: {curve[-1]}", file=sys.stderr, flush=True)
         return curve
-
-    plain = run(0)
-    ctx.cache_create(log2_table=19, base_res=16, max_res=2048, lr=args.study_lr, ema=args.study_ema)
-    cached = run(1)
-    ctx.free(d_out)
-    ctx.close()
 
     def time_to(curve, target):
         for p in curve:
@@ -250,11 +254,26 @@ def cache_study(args) -> dict:
                 return p["ms"] / 1e3
         return None
 
-    tp, tc = time_to(plain, args.study_target), time_to(cached, args.study_target)
+    plain = run(0, label="plain")
+    tp = time_to(plain, args.study_target)
+    variants = json.loads(args.study_variants)
+    results = []
+    for v in variants:
+        ctx.cache_create(log2_table=v.get("log2_table", 19), base_res=16, max_res=v.get("max_res", 2048),
+                         lr=v.get("lr", args.study_lr), ema=v.get("ema", args.study_ema))
+        curve = run(1, v.get("query_depth", 2), v.get("warmup", 0), v.get("train_paths", default_paths),
+                    label=f"cached {v}")
+        tc = time_to(curve, args.study_target)
+        results.append({"variant": v, "time_to_target_s": tc, "speedup": (tp / tc) if (tp and tc) else None,
+                        "curve": curve})
+    ctx.free(d_out)
+    ctx.close()
+    best = min((r for r in results if r["time_to_target_s"]), key=lambda r: r["time_to_target_s"], default=None)
     return {"study": "cache", "resolution": [W, H], "target_rmse_tonemapped": args.study_target,
-            "reference_spp": args.study_ref_spp, "reference_s": round(ref_s, 2), "train_paths_per_frame": train_paths,
-            "time_to_target_s": {"plain": tp, "cached": tc},
-            "speedup": (tp / tc) if (tp and tc) else None, "plain": plain, "cached": cached}
+            "reference_spp": args.study_ref_spp, "reference_s": round(ref_s, 2),
+            "default_train_paths_per_frame": default_paths,
+            "time_to_target_s": {"plain": tp, "cached_best": best["time_to_target_s"] if best else None},
+            "speedup": best["speedup"] if best else None, "plain": plain, "cached": results}
 
 
 def main() -> int:
@@ -281,6 +300,9 @@ def main() -> int:
     ap.add_argument("--study-target", type=float, default=0.01)
     ap.add_argument("--study-lr", type=float, default=1e-2)
     ap.add_argument("--study-ema", type=float, default=0.99)
+    ap.add_argument("--study-variants", default='[{"query_depth": 2}]',
+                    help="JSON list of cached-run settings: query_depth, warmup, lr, ema, train_paths, log2_table, "
+                         "max_res")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)

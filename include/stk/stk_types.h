@@ -176,7 +176,9 @@ typedef struct StkRenderParams {
     uint32_t enableTonemap;
     uint32_t keepSamples;    /* keep per-sample radiance for stk_get_samples */
     /* radiance cache (needs stk_cache_create): 0 off (plain path tracer),
-     * 1 query and train, 2 query only (frozen cache), 3 train only */
+     * 1 query and train, 2 query only (frozen cache), 3 train only, 4 train
+     * only and trace the training paths only (warm-up; the image holds only
+     * those paths' samples) */
     uint32_t cacheMode;
     uint32_t cacheQueryDepth; /* vertex depth at which paths query the cache */
     uint32_t cacheTrainPaths; /* training paths per frame (extended to maxDepth) */

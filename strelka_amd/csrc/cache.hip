@@ -277,7 +277,8 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, float* __re
                                               float* __restrict__ gr, __half* __restrict__ half, size_t n,
                                               float gradScale, const uint32_t* __restrict__ nDev, uint32_t nLimit, float lr,
                                               float b1, float b2, float eps, float bc1, float bc2, int sparse,
-                                              float* __restrict__ ema, __half* __restrict__ emaHalf, float emaDecay)
+                                              float* __restrict__ ema, float* __restrict__ emaW,
+                                              __half* __restrict__ emaHalf, float emaDecay)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -301,7 +302,11 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, float* __re
     if (ema) {
         const float e = emaDecay * ema[i] + (1.0f - emaDecay) * pi;
         ema[i] = e;
-        if (emaHalf) emaHalf[i] = __float2half(e);
+        if (emaW) {
+            const float w = emaDecay * emaW[i] + (1.0f - emaDecay);
+            emaW[i] = w;
+            if (emaHalf) emaHalf[i] = __float2half(e / w);
+        }
     }
 }
 
@@ -313,7 +318,7 @@ __global__ __launch_bounds__(256) void k_to_half(const float* __restrict__ p, __
 
 // MFMA fragments from the FP32 master weights.
 __global__ __launch_bounds__(256) void k_pack_frags(const float* __restrict__ w, _Float16* __restrict__ fwd,
-                                                    _Float16* __restrict__ bwd)
+                                                    _Float16* __restrict__ bwd, float scale)
 {
     const int id = blockIdx.x * blockDim.x + threadIdx.x; // frag * 256 + lane * 4 + i
     if (id >= 68 * kFrag) return;
@@ -321,22 +326,25 @@ __global__ __launch_bounds__(256) void k_pack_frags(const float* __restrict__ w,
     if (f < 64) {
         const int l = f / 16, t = (f / 4) % 4, s = f % 4;
         const float* W = w + l * 4096;
-        fwd[id] = (_Float16)W[(16 * s + 4 * g + i) * 64 + 16 * t + r]; // A[out][in] = W[in][out]
-        bwd[id] = (_Float16)W[(16 * t + r) * 64 + 16 * s + 4 * g + i]; // A[in][out] = W[in][out]
+        fwd[id] = (_Float16)(scale * W[(16 * s + 4 * g + i) * 64 + 16 * t + r]); // A[out][in] = W[in][out]
+        bwd[id] = (_Float16)(scale * W[(16 * t + r) * 64 + 16 * s + 4 * g + i]); // A[in][out] = W[in][out]
     } else {
         const float* W = w + 16384;
         const int q = f - 64;
-        fwd[id] = (_Float16)W[(16 * q + 4 * g + i) * 16 + r];
-        bwd[id] = (_Float16)W[(16 * q + r) * 16 + 4 * g + i];
+        fwd[id] = (_Float16)(scale * W[(16 * q + 4 * g + i) * 16 + r]);
+        bwd[id] = (_Float16)(scale * W[(16 * q + r) * 16 + 4 * g + i]);
     }
 }
 
+// Restarts the moving average: empty average and weight; until an element is
+// updated its query copy is the current parameter.
 __global__ __launch_bounds__(256) void k_reset_ema(const float* __restrict__ p, float* __restrict__ e,
-                                                   __half* __restrict__ eh, size_t n)
+                                                   float* __restrict__ ew, __half* __restrict__ eh, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    e[i] = p[i];
+    e[i] = 0.0f;
+    if (ew) ew[i] = 0.0f;
     if (eh) eh[i] = __float2half(p[i]);
 }
 
@@ -386,31 +394,38 @@ hipError_t cacheAdam(CacheBuffers& cb, float gradScale, const uint32_t* nDev, ui
     const bool ema = cb.emaDecay > 0.0f;
     hipLaunchKernelGGL(k_adam, dim3(blocks256(cb.gridParams)), dim3(256), 0, s, cb.gridP, cb.gridM, cb.gridV, cb.gridG,
                        cb.gridH, cb.gridParams, gradScale, nDev, nLimit, cb.lr, cb.beta1, cb.beta2, cb.eps, bc1, bc2,
-                       1, ema ? cb.gridE : nullptr, ema ? cb.gridHE : nullptr, cb.emaDecay);
+                       1, ema ? cb.gridE : nullptr, ema ? cb.gridEW : nullptr, ema ? cb.gridHE : nullptr,
+                       cb.emaDecay);
     hipLaunchKernelGGL(k_adam, dim3(blocks256(kCacheMlpParams)), dim3(256), 0, s, cb.mlpP, cb.mlpM, cb.mlpV, cb.mlpG,
                        (__half*)nullptr, (size_t)kCacheMlpParams, gradScale, nDev, nLimit, cb.lr, cb.beta1, cb.beta2,
-                       cb.eps, bc1, bc2, 0, ema ? cb.mlpE : nullptr, (__half*)nullptr, cb.emaDecay);
+                       cb.eps, bc1, bc2, 0, ema ? cb.mlpE : nullptr, (float*)nullptr, (__half*)nullptr, cb.emaDecay);
     hipLaunchKernelGGL(k_pack_frags, dim3(blocks256(68 * kFrag)), dim3(256), 0, s, cb.mlpP,
-                       reinterpret_cast<_Float16*>(cb.wFwd), reinterpret_cast<_Float16*>(cb.wBwd));
-    if (ema)
+                       reinterpret_cast<_Float16*>(cb.wFwd), reinterpret_cast<_Float16*>(cb.wBwd), 1.0f);
+    if (ema) {
+        // the MLP's moving average is updated every step: weight 1 - decay^step
+        cb.emaSteps += 1;
+        const float w = 1.0f - powf(cb.emaDecay, (float)cb.emaSteps);
         hipLaunchKernelGGL(k_pack_frags, dim3(blocks256(68 * kFrag)), dim3(256), 0, s, cb.mlpE,
-                           reinterpret_cast<_Float16*>(cb.wFwdE), reinterpret_cast<_Float16*>(cb.wBwdE));
+                           reinterpret_cast<_Float16*>(cb.wFwdE), reinterpret_cast<_Float16*>(cb.wBwdE), 1.0f / w);
+    }
     return hipGetLastError();
 }
 
-hipError_t cachePack(const CacheBuffers& cb, hipStream_t s)
+hipError_t cachePack(CacheBuffers& cb, hipStream_t s)
 {
     hipLaunchKernelGGL(k_to_half, dim3(blocks256(cb.gridParams)), dim3(256), 0, s, cb.gridP, cb.gridH, cb.gridParams);
     hipLaunchKernelGGL(k_pack_frags, dim3(blocks256(68 * kFrag)), dim3(256), 0, s, cb.mlpP,
-                       reinterpret_cast<_Float16*>(cb.wFwd), reinterpret_cast<_Float16*>(cb.wBwd));
+                       reinterpret_cast<_Float16*>(cb.wFwd), reinterpret_cast<_Float16*>(cb.wBwd), 1.0f);
     if (cb.emaDecay > 0.0f) {
-        // the moving average restarts from the current parameters
+        // the moving average restarts (empty); queries read the current
+        // parameters until the first update
+        cb.emaSteps = 0;
         hipLaunchKernelGGL(k_reset_ema, dim3(blocks256(cb.gridParams)), dim3(256), 0, s, cb.gridP, cb.gridE,
-                           cb.gridHE, cb.gridParams);
+                           cb.gridEW, cb.gridHE, cb.gridParams);
         hipLaunchKernelGGL(k_reset_ema, dim3(blocks256(kCacheMlpParams)), dim3(256), 0, s, cb.mlpP, cb.mlpE,
-                           (__half*)nullptr, (size_t)kCacheMlpParams);
-        hipLaunchKernelGGL(k_pack_frags, dim3(blocks256(68 * kFrag)), dim3(256), 0, s, cb.mlpE,
-                           reinterpret_cast<_Float16*>(cb.wFwdE), reinterpret_cast<_Float16*>(cb.wBwdE));
+                           (float*)nullptr, (__half*)nullptr, (size_t)kCacheMlpParams);
+        hipLaunchKernelGGL(k_pack_frags, dim3(blocks256(68 * kFrag)), dim3(256), 0, s, cb.mlpP,
+                           reinterpret_cast<_Float16*>(cb.wFwdE), reinterpret_cast<_Float16*>(cb.wBwdE), 1.0f);
     }
     return hipGetLastError();
 }

@@ -52,8 +52,13 @@ struct CacheBuffers {
     float* mlpG = nullptr;
     __half* wFwd = nullptr;    // MFMA A fragments of W^T (forward)
     __half* wBwd = nullptr;    // MFMA A fragments of W (backward)
-    // exponential moving average of the parameters, read by queries
+    // exponential moving average of the parameters, read by queries.  The
+    // averages start at zero and are divided by their weight 1 - decay^k (k =
+    // updates of the element) so early queries are not pulled towards the
+    // initial parameters: the grid is updated sparsely, so it keeps k per
+    // element as the weight array gridEW; the MLP is updated densely (k = step).
     float emaDecay = 0.0f;     // 0: queries read the raw parameters
+    float* gridEW = nullptr;
     float* gridE = nullptr;
     __half* gridHE = nullptr;
     float* mlpE = nullptr;
@@ -65,6 +70,7 @@ struct CacheBuffers {
     uint32_t nCap = 0;         // capacity in samples (multiple of 1024)
     float* loss = nullptr;     // [0] loss sum of the last step
     uint32_t step = 0;
+    uint32_t emaSteps = 0;     // updates since the moving average restarted
     float lr = 1e-2f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-15f;
 };
 
@@ -85,7 +91,7 @@ hipError_t cacheBackward(const CacheBuffers& cb, const CacheRecord* recs, uint32
 hipError_t cacheAdam(CacheBuffers& cb, float gradScale, const uint32_t* nDev, uint32_t nLimit,
                      hipStream_t s);
 // Rebuilds the FP16 grid copy and the fragments from the FP32 masters.
-hipError_t cachePack(const CacheBuffers& cb, hipStream_t s);
+hipError_t cachePack(CacheBuffers& cb, hipStream_t s);
 hipError_t cacheZeroGrads(const CacheBuffers& cb, hipStream_t s);
 
 } // namespace stk

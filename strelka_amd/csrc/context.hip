@@ -93,7 +93,7 @@ struct stk_ctx {
     bool haveCache = false;
     StkCacheConfig cacheCfg{};
     DevBuf cGridH, cGridP, cGridM, cGridV, cGridG, cMlpP, cMlpM, cMlpV, cMlpG, cFwd, cBwd, cXT, cDyT, cLoss;
-    DevBuf cGridE, cGridHE, cMlpE, cFwdE, cBwdE;
+    DevBuf cGridE, cGridEW, cGridHE, cMlpE, cFwdE, cBwdE;
     DevBuf queryRecs, queryOut, trainVerts, trainCount, trainBatch;
 
     std::vector<DevBuf*> owned()
@@ -101,7 +101,7 @@ struct stk_ctx {
         return {&vb, &ib, &inst, &lights, &mats, &packed, &tris, &refNodes, &rayScratch, &hitScratch,
                 &accum, &history, &out, &samples, &descs, &fVb, &fIb, &fInst, &fLights, &fMats, &fPacked,
                 &fTris, &fRefNodes, &cGridH, &cGridP, &cGridM, &cGridV, &cGridG, &cMlpP,
-                &cMlpM, &cMlpV, &cMlpG, &cFwd, &cBwd, &cXT, &cDyT, &cLoss, &cGridE, &cGridHE, &cMlpE,
+                &cMlpM, &cMlpV, &cMlpG, &cFwd, &cBwd, &cXT, &cDyT, &cLoss, &cGridE, &cGridEW, &cGridHE, &cMlpE,
                 &cFwdE, &cBwdE, &queryRecs, &queryOut, &trainVerts, &trainCount, &trainBatch};
     }
 };
@@ -536,7 +536,7 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
     if (p->cacheMode != 0) {
         if (!allowCache) return fail(c, STK_E_INVALID, "render: the radiance cache serves single-scene renders");
         if (!c->haveCache) return fail(c, STK_E_STATE, "render: cacheMode set but no cache (stk_cache_create)");
-        if (p->cacheMode > 3) return fail(c, STK_E_INVALID, "render: cacheMode must be 0..3");
+        if (p->cacheMode > 4) return fail(c, STK_E_INVALID, "render: cacheMode must be 0..4");
         pc.mode = p->cacheMode;
         pc.queryDepth = p->cacheQueryDepth;
         const uint32_t paths = p->cacheTrainPaths ? p->cacheTrainPaths : 65536u;
@@ -1003,7 +1003,7 @@ STK_API stk_status stk_cache_destroy(stk_ctx* c)
     if (!validCtx(c)) return STK_E_INVALID;
     DevBuf* bufs[] = {&c->cGridH, &c->cGridP, &c->cGridM, &c->cGridV, &c->cGridG, &c->cMlpP, &c->cMlpM,
                       &c->cMlpV, &c->cMlpG, &c->cFwd, &c->cBwd, &c->cXT, &c->cDyT, &c->cLoss,
-                      &c->cGridE, &c->cGridHE, &c->cMlpE, &c->cFwdE, &c->cBwdE};
+                      &c->cGridE, &c->cGridEW, &c->cGridHE, &c->cMlpE, &c->cFwdE, &c->cBwdE};
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : bufs) devRelease(c, *b);
@@ -1071,8 +1071,8 @@ STK_API stk_status stk_cache_create(stk_ctx* c, const StkCacheConfig* cfgIn)
         struct {
             DevBuf* b;
             size_t bytes;
-        } ema[] = {{&c->cGridE, gp * 4}, {&c->cGridHE, gp * 2}, {&c->cMlpE, mp * 4}, {&c->cFwdE, 68 * 256 * 2},
-                   {&c->cBwdE, 68 * 256 * 2}};
+        } ema[] = {{&c->cGridE, gp * 4}, {&c->cGridEW, gp * 4}, {&c->cGridHE, gp * 2}, {&c->cMlpE, mp * 4},
+                   {&c->cFwdE, 68 * 256 * 2}, {&c->cBwdE, 68 * 256 * 2}};
         for (auto& a : ema) {
             stk_status st = devReserve(c, *a.b, a.bytes);
             if (st != STK_OK) {
@@ -1081,6 +1081,7 @@ STK_API stk_status stk_cache_create(stk_ctx* c, const StkCacheConfig* cfgIn)
             }
         }
         cb.gridE = static_cast<float*>(c->cGridE.ptr);
+        cb.gridEW = static_cast<float*>(c->cGridEW.ptr);
         cb.gridHE = static_cast<__half*>(c->cGridHE.ptr);
         cb.mlpE = static_cast<float*>(c->cMlpE.ptr);
         cb.wFwdE = static_cast<__half*>(c->cFwdE.ptr);

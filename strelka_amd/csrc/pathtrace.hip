@@ -29,6 +29,7 @@ namespace {
 using namespace dev;
 
 constexpr int kBlock = 256;
+constexpr float kMinAlbedo = 1e-3f; // floor of the albedo the cache's targets are divided by
 
 __device__ __forceinline__ float4 f4(F3 v) { return make_float4(v.x, v.y, v.z, 0.0f); }
 __device__ __forceinline__ F3 f3(float4 v) { return mk3(v.x, v.y, v.z); }
@@ -62,7 +63,7 @@ __device__ __forceinline__ uint32_t scene_of_slot(const SceneView& sc, uint32_t 
 
 __device__ __forceinline__ bool is_train(const PtCache& pc, const PassState& pass, uint32_t slot)
 {
-    return (pc.mode == 1 || pc.mode == 3) && slot % pc.trainStride == pass.trainOffset;
+    return (pc.mode == 1 || pc.mode == 3 || pc.mode == 4) && slot % pc.trainStride == pass.trainOffset;
 }
 
 __global__ void k_pass_set(PassState* pass, PassState v)
@@ -114,6 +115,8 @@ __global__ __launch_bounds__(kBlock) void k_pt_raygen(SceneView sc, PtLaunch lp,
         const uint32_t px = (tileIdx % d.tilesX) * T + within % T;
         const uint32_t py = (tileIdx / d.tilesX) * T + within / T;
         valid = px < d.width && py < d.height;
+        // mode 4 traces the training paths only (cache warm-up)
+        if (lp.cache.mode == 4 && !is_train(lp.cache, pass, local)) valid = false;
         ps.scene[local] = sid;
         ps.pixel[local] = valid ? d.pixelBase + py * d.width + px : STK_INVALID_INDEX;
         if (valid) {
@@ -282,19 +285,24 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, 
                 const F3 k1 = -f3(ps.rayD[slot]);
                 const float rough = mat.type == STK_MAT_GGX ? mat.roughness : 1.0f;
                 const bool isTrain = is_train(pc, *ps.pass, slot);
+                // the cache learns radiance divided by the vertex's albedo (a
+                // Lambert vertex's outgoing radiance is albedo times a smooth
+                // irradiance term), so albedo detail need not be learned
+                const F3 alb = mk3(fmaxf(mat.albedo[0], kMinAlbedo), fmaxf(mat.albedo[1], kMinAlbedo),
+                                   fmaxf(mat.albedo[2], kMinAlbedo));
                 if ((pc.mode == 1 || pc.mode == 2) && !isTrain && depth == (int)pc.queryDepth) {
                     // the path ends here: its remaining radiance comes from the cache
                     cacheQuery = true;
                     qrec.posRough = make_float4(P.x, P.y, P.z, rough);
                     qrec.normal = make_float4(N.x, N.y, N.z, bits_to_float(slot));
                     qrec.dir = make_float4(k1.x, k1.y, k1.z, 0.0f);
-                    qrec.target = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    qrec.target = make_float4(alb.x, alb.y, alb.z, 0.0f); // read back by k_cache_apply
                 } else if (isTrain) {
                     float4* v = pc.trainVerts + ((size_t)(slot / pc.trainStride) * lp.maxDepth + depth) * 5;
                     v[0] = make_float4(P.x, P.y, P.z, rough);
-                    v[1] = make_float4(N.x, N.y, N.z, 0.0f);
-                    v[2] = make_float4(k1.x, k1.y, k1.z, 0.0f);
-                    v[3] = f4(col);
+                    v[1] = make_float4(N.x, N.y, N.z, alb.x);
+                    v[2] = make_float4(k1.x, k1.y, k1.z, alb.y);
+                    v[3] = make_float4(col.x, col.y, col.z, alb.z);
                     v[4] = f4(thr);
                     pc.trainCount[slot / pc.trainStride] = (uint32_t)depth + 1;
                 }
@@ -410,15 +418,16 @@ __global__ __launch_bounds__(kBlock) void k_pt_connect(SceneView sc, PathState p
 }
 
 // Adds the cache's answer to the paths that ended at a query:
-// radiance += throughput * max(prediction, 0).
+// radiance += throughput * albedo * max(prediction, 0).
 __global__ __launch_bounds__(kBlock) void k_cache_apply(PtLaunch lp, PathState ps)
 {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
     if (q >= ps.counters[3]) return;
-    const uint32_t slot = float_to_bits(lp.cache.queryRecs[q].normal.w);
+    const CacheRecord& rec = lp.cache.queryRecs[q];
+    const uint32_t slot = float_to_bits(rec.normal.w);
     const float4 o = lp.cache.queryOut[q];
-    const F3 thr = f3(ps.thr[slot]);
-    const F3 col = f3(ps.col[slot]) + thr * mk3(fmaxf(o.x, 0.0f), fmaxf(o.y, 0.0f), fmaxf(o.z, 0.0f));
+    const F3 w = f3(ps.thr[slot]) * f3(rec.target);
+    const F3 col = f3(ps.col[slot]) + w * mk3(fmaxf(o.x, 0.0f), fmaxf(o.y, 0.0f), fmaxf(o.z, 0.0f));
     ps.col[slot] = f4(col);
 }
 
@@ -461,17 +470,18 @@ __global__ __launch_bounds__(kBlock) void k_cache_targets(PtLaunch lp, PathState
         const uint32_t slot = tidx * pc.trainStride + pass.trainOffset;
         const float4* v = pc.trainVerts + ((size_t)tidx * maxDepth + d) * 5;
         const float4 before = v[3], thr = v[4], fin = ps.col[slot];
+        const float ax = v[1].w, ay = v[2].w, az = v[3].w; // the vertex's albedo
         if (thr.x > 1e-8f && thr.y > 1e-8f && thr.z > 1e-8f) {
             // one-sample targets are clamped to [0, kTargetMax]: Russian-roulette
             // survivors and near-light NEE samples can be arbitrarily large, and
             // a few such fireflies would dominate the loss
             const float kTargetMax = 100.0f;
             rec.posRough = v[0];
-            rec.normal = v[1];
-            rec.dir = v[2];
-            rec.target = make_float4(fminf(fmaxf((fin.x - before.x) / thr.x, 0.0f), kTargetMax),
-                                     fminf(fmaxf((fin.y - before.y) / thr.y, 0.0f), kTargetMax),
-                                     fminf(fmaxf((fin.z - before.z) / thr.z, 0.0f), kTargetMax), 0.0f);
+            rec.normal = make_float4(v[1].x, v[1].y, v[1].z, 0.0f);
+            rec.dir = make_float4(v[2].x, v[2].y, v[2].z, 0.0f);
+            rec.target = make_float4(fminf(fmaxf((fin.x - before.x) / (thr.x * ax), 0.0f), kTargetMax),
+                                     fminf(fmaxf((fin.y - before.y) / (thr.y * ay), 0.0f), kTargetMax),
+                                     fminf(fmaxf((fin.z - before.z) / (thr.z * az), 0.0f), kTargetMax), 0.0f);
             emit = isfinite(fin.x) && isfinite(fin.y) && isfinite(fin.z);
         }
     }
@@ -516,7 +526,7 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
         return hipErrorInvalidValue;
     const dim3 g(blocks(n)), b(kBlock);
     const PtCache& pc = lp.cache;
-    const bool query = pc.mode == 1 || pc.mode == 2, train = pc.mode == 1 || pc.mode == 3;
+    const bool query = pc.mode == 1 || pc.mode == 2, train = pc.mode == 1 || pc.mode == 3 || pc.mode == 4;
     hipLaunchKernelGGL(k_pass_reset, dim3(1), dim3(64), 0, s, ps);
     hipLaunchKernelGGL(k_pt_raygen, g, b, 0, s, sc, lp, ps);
     for (int depth = 0; depth < (int)lp.maxDepth; ++depth) {
