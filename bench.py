@@ -369,6 +369,13 @@ def main() -> int:
         ctx.trace_closest_device(d_rays, n, d_hits, reference_order=True)
     ctx.record(3)
     ref_kernel_ms = ctx.elapsed_ms(2, 3) / max(1, args.steps // 10)
+    # packed binary-BVH walk on the same rays
+    ctx.trace_closest_device(d_rays, n, d_hits, binary=True)
+    ctx.record(2)
+    for _ in range(max(1, args.steps // 2)):
+        ctx.trace_closest_device(d_rays, n, d_hits, binary=True)
+    ctx.record(3)
+    bin_kernel_ms = ctx.elapsed_ms(2, 3) / max(1, args.steps // 2)
 
     # path tracer on the same scene: 1080p, pt_spp samples per frame, 8 bounces, RR after 3;
     # launch shape from the autotuner on a 1-spp frame
@@ -422,6 +429,7 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
         "extra": {"reference_order_gpu_mrays_per_s": round(n / (ref_kernel_ms * 1e-3) / 1e6, 2),
+                  "binary_bvh_gpu_mrays_per_s": round(n / (bin_kernel_ms * 1e-3) / 1e6, 2),
                   "path_tracer": pt},
     }
     print(json.dumps(out))

@@ -81,6 +81,9 @@ typedef struct StkBvhHeader {
  * packed near-first (closest) or packed left-first (any) walks run; they
  * return the same hit records. */
 #define STK_TRACE_REFERENCE_ORDER 0x1u
+/* Closest hit over the packed binary BVH instead of the compressed 8-wide one
+ * (same hit records; kept for measurement). */
+#define STK_TRACE_BVH2 0x2u
 
 /* ---- library / context ---------------------------------------------------- */
 STK_API const char* stk_version(void);
@@ -152,15 +155,16 @@ STK_API stk_status stk_get_ray_totals(stk_ctx* ctx, uint64_t out[2]);
  * as a hipGraph replayed for every pass (plain rendering only), and
  * pathsInFlight > 0 (at least 4096) traces a frame's paths in passes of that
  * many slots, so a pass's path state can stay in the last-level cache (plain
- * rendering only; the cache trains on whole frames).  Results do not depend
- * on any of these. */
+ * rendering only; the cache trains on whole frames), binaryWalk selects the
+ * traversal.  Results do not depend on any of these. */
 typedef struct StkLaunchConfig {
     uint32_t blockSize;
     uint32_t tile;
     uint32_t persistentBlocks;
     uint32_t useGraph;
     uint32_t pathsInFlight;
-    uint32_t pad[3];
+    uint32_t binaryWalk; /* 1: packed binary-BVH walks instead of the compressed 8-wide ones */
+    uint32_t pad[2];
 } StkLaunchConfig;
 STK_API stk_status stk_set_launch_config(stk_ctx* ctx, const StkLaunchConfig* lc);
 STK_API stk_status stk_get_launch_config(stk_ctx* ctx, StkLaunchConfig* lc);
@@ -301,6 +305,14 @@ STK_API uint32_t stk_bvh_num_triangles(const stk_bvh* b);
 STK_API const StkTriangle* stk_bvh_triangles(const stk_bvh* b);
 STK_API StkBvhHeader stk_bvh_header(const stk_bvh* b);
 STK_API uint32_t stk_bvh_depth(const stk_bvh* b);
+/* The compressed 8-wide layout the GPU traversal walks (StkWideNode), its
+ * triangles in slot order (stk_bvh_num_triangles of them), its number of
+ * inner-node levels, and the number of emitter triangles. */
+STK_API uint32_t stk_bvh_num_wide(const stk_bvh* b);
+STK_API const StkWideNode* stk_bvh_wide_nodes(const stk_bvh* b);
+STK_API const StkTriangle* stk_bvh_wide_triangles(const stk_bvh* b);
+STK_API uint32_t stk_bvh_wide_depth(const stk_bvh* b);
+STK_API uint32_t stk_bvh_num_emitters(const stk_bvh* b);
 
 #ifdef __cplusplus
 }

@@ -151,6 +151,31 @@ typedef struct StkPackedNode {
     uint32_t pad[2];
 } StkPackedNode;
 
+/* Compressed 8-wide BVH node of the MI355X traversal, 80 B (five 16-B loads).
+ * A node holds up to 8 children in slots; slot s of a node is used when bit s
+ * of vmask is set, holds an inner node when bit s of imask is set and a leaf
+ * (one triangle) otherwise.  Inner children are consecutive nodes from
+ * childBase and leaf triangles consecutive from triBase, both in slot order.
+ * Child boxes are quantised to 8 bits per plane relative to origin:
+ *   lo[a] = origin[a] + qlo[a][s] * 2^(exps[a] - 127)   (float arithmetic)
+ *   hi[a] = origin[a] + qhi[a][s] * 2^(exps[a] - 127)
+ * rounded outwards, so every decoded box contains its child's box (a leaf's
+ * box is its triangle's box widened by a relative 1e-5).  Slots are assigned
+ * by the direction of the child's centre from the node's centre (bit a of s
+ * set: the positive side of axis a), so visiting slots k ^ octant(ray),
+ * k = 0..7, goes roughly front to back. */
+typedef struct StkWideNode {
+    float origin[3];
+    uint8_t exps[3];
+    uint8_t imask;
+    uint32_t childBase;
+    uint32_t triBase;
+    uint32_t vmask; /* low 8 bits */
+    uint32_t pad;
+    uint8_t qlo[3][8];
+    uint8_t qhi[3][8];
+} StkWideNode;
+
 /* One radiance-cache sample, 64 B: the shading point, its shading normal, the
  * outgoing direction, the roughness and (for training) the radiance target.
  * payload is free for the caller (the renderer stores the path slot there). */
@@ -213,6 +238,8 @@ STK_STATIC_ASSERT(offsetof(StkPathTracerParam, len) == 304, "PathTracerParam.len
 STK_STATIC_ASSERT(sizeof(StkMaterial) == 32, "Material must be 32 B");
 STK_STATIC_ASSERT(sizeof(StkTriangle) == 48, "Triangle must be 48 B");
 STK_STATIC_ASSERT(sizeof(StkPackedNode) == 64, "PackedNode must be 64 B");
+STK_STATIC_ASSERT(sizeof(StkWideNode) == 80, "WideNode must be 80 B");
+STK_STATIC_ASSERT(offsetof(StkWideNode, qlo) == 32, "WideNode.qlo offset");
 STK_STATIC_ASSERT(sizeof(StkCacheRecord) == 64, "CacheRecord must be 64 B");
 STK_STATIC_ASSERT(sizeof(StkRenderParams) == 336 + 64, "RenderParams must be 400 B");
 

@@ -15,13 +15,13 @@ import numpy as np
 
 from . import _abi
 from ._abi import (BVH_NODE, CACHE_RECORD, HIT, INSTANCE, INVALID, LIGHT, MAT_GGX, MAT_LAMBERT, MATERIAL, PACKED_NODE,
-                   RAY, TRIANGLE,
+                   RAY, TRIANGLE, WIDE_NODE,
                    VERTEX, BatchScene, BatchViewInfo, BvhHeader, BvhOptions, CacheConfig, CacheInfo, DeviceInfo, LaunchConfig,
                    PathTracerParam, RenderParams, SceneArrays)
 
 __all__ = ["lib", "StkError", "SceneData", "HostBvh", "Context", "RenderParams", "PathTracerParam", "VERTEX",
            "BVH_NODE", "INSTANCE", "LIGHT", "RAY", "HIT", "MATERIAL", "TRIANGLE", "PACKED_NODE", "INVALID",
-           "render_params", "CACHE_RECORD", "MAT_LAMBERT", "MAT_GGX"]
+           "render_params", "CACHE_RECORD", "MAT_LAMBERT", "MAT_GGX", "WIDE_NODE"]
 
 _LIB = _abi.load()
 
@@ -104,6 +104,10 @@ class HostBvh:
         self.tris = _copy(_LIB.stk_bvh_triangles(handle), _LIB.stk_bvh_num_triangles(handle), TRIANGLE)
         self.header = _LIB.stk_bvh_header(handle)
         self.depth = int(_LIB.stk_bvh_depth(handle))
+        self.wide = _copy(_LIB.stk_bvh_wide_nodes(handle), _LIB.stk_bvh_num_wide(handle), WIDE_NODE)
+        self.wide_tris = _copy(_LIB.stk_bvh_wide_triangles(handle), self.tris.size, TRIANGLE)
+        self.wide_depth = int(_LIB.stk_bvh_wide_depth(handle))
+        self.num_emitters = int(_LIB.stk_bvh_num_emitters(handle))
 
     @classmethod
     def build(cls, scene: SceneData, max_depth: int = 32, bins: int = 32) -> "HostBvh":
@@ -212,11 +216,11 @@ class Context:
         return out
 
     # -- traversal (host buffers) ----------------------------------------------
-    def trace_closest(self, rays: np.ndarray, reference_order: bool = False) -> np.ndarray:
+    def trace_closest(self, rays: np.ndarray, reference_order: bool = False, binary: bool = False) -> np.ndarray:
         rays = np.ascontiguousarray(rays, dtype=RAY)
         hits = np.zeros(rays.size, dtype=HIT)
-        self._check(_LIB.stk_trace_closest(self._h, _ptr(rays), rays.size, _ptr(hits),
-                                           1 if reference_order else 0), "stk_trace_closest")
+        flags = (1 if reference_order else 0) | (2 if binary else 0)
+        self._check(_LIB.stk_trace_closest(self._h, _ptr(rays), rays.size, _ptr(hits), flags), "stk_trace_closest")
         return hits
 
     def trace_any(self, rays: np.ndarray, reference_order: bool = False) -> np.ndarray:
@@ -245,9 +249,11 @@ class Context:
     def copy_device(self, dst: int, src: int, nbytes: int):
         self._check(_LIB.stk_device_copy(self._h, C.c_void_p(dst), C.c_void_p(src), nbytes), "stk_device_copy")
 
-    def trace_closest_device(self, d_rays: int, n: int, d_hits: int, reference_order: bool = False):
-        self._check(_LIB.stk_trace_closest_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits),
-                                                  1 if reference_order else 0), "stk_trace_closest_device")
+    def trace_closest_device(self, d_rays: int, n: int, d_hits: int, reference_order: bool = False,
+                             binary: bool = False):
+        flags = (1 if reference_order else 0) | (2 if binary else 0)
+        self._check(_LIB.stk_trace_closest_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits), flags),
+                    "stk_trace_closest_device")
 
     def trace_any_device(self, d_rays: int, n: int, d_hits: int, reference_order: bool = False):
         self._check(_LIB.stk_trace_any_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits),
@@ -305,15 +311,17 @@ class Context:
 
     # -- launch shape --------------------------------------------------------------
     def set_launch_config(self, block_size: int = 256, tile: int = 8, persistent_blocks: int = 0,
-                          graph: bool = False, paths_in_flight: int = 0):
-        lc = LaunchConfig(block_size, tile, persistent_blocks, 1 if graph else 0, paths_in_flight)
+                          graph: bool = False, paths_in_flight: int = 0, binary_walk: bool = False):
+        lc = LaunchConfig(block_size, tile, persistent_blocks, 1 if graph else 0, paths_in_flight,
+                          1 if binary_walk else 0)
         self._check(_LIB.stk_set_launch_config(self._h, C.byref(lc)), "stk_set_launch_config")
 
     def launch_config(self) -> dict:
         lc = LaunchConfig()
         self._check(_LIB.stk_get_launch_config(self._h, C.byref(lc)), "stk_get_launch_config")
         return {"block_size": lc.blockSize, "tile": lc.tile, "persistent_blocks": lc.persistentBlocks,
-                "graph": bool(lc.useGraph), "paths_in_flight": lc.pathsInFlight}
+                "graph": bool(lc.useGraph), "paths_in_flight": lc.pathsInFlight,
+                "binary_walk": bool(lc.binaryWalk)}
 
     def device_info(self) -> dict:
         d = DeviceInfo()

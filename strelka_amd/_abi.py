@@ -28,6 +28,9 @@ MATERIAL = np.dtype([("type", "<u4"), ("roughness", "<f4"), ("pad", "<f4", 2), (
 TRIANGLE = np.dtype([("v0", "<f4", 3), ("leaf", "<u4"), ("e0", "<f4", 3), ("instId", "<u4"), ("e1", "<f4", 3),
                      ("primId", "<u4")])
 PACKED_NODE = np.dtype([("x", "<f4", 4), ("y", "<f4", 4), ("z", "<f4", 4), ("child", "<i4", 2), ("pad", "<u4", 2)])
+WIDE_NODE = np.dtype([("origin", "<f4", 3), ("exps", "u1", 3), ("imask", "u1"), ("childBase", "<u4"),
+                      ("triBase", "<u4"), ("vmask", "<u4"), ("pad", "<u4"), ("qlo", "u1", (3, 8)),
+                      ("qhi", "u1", (3, 8))])
 
 for _dt, _sz in ((VERTEX, 32), (BVH_NODE, 32), (INSTANCE, 208), (LIGHT, 112), (RAY, 32), (HIT, 20),
                  (MATERIAL, 32), (TRIANGLE, 48), (PACKED_NODE, 64)):
@@ -80,7 +83,8 @@ class BvhHeader(C.Structure):
 
 class LaunchConfig(C.Structure):
     _fields_ = [("blockSize", C.c_uint32), ("tile", C.c_uint32), ("persistentBlocks", C.c_uint32),
-                ("useGraph", C.c_uint32), ("pathsInFlight", C.c_uint32), ("pad", C.c_uint32 * 3)]
+                ("useGraph", C.c_uint32), ("pathsInFlight", C.c_uint32), ("binaryWalk", C.c_uint32),
+                ("pad", C.c_uint32 * 2)]
 
 
 class DeviceInfo(C.Structure):
@@ -178,6 +182,11 @@ PROTOTYPES = [
     ("stk_bvh_triangles", P, [P]),
     ("stk_bvh_header", BvhHeader, [P]),
     ("stk_bvh_depth", U32, [P]),
+    ("stk_bvh_num_wide", U32, [P]),
+    ("stk_bvh_wide_nodes", P, [P]),
+    ("stk_bvh_wide_triangles", P, [P]),
+    ("stk_bvh_wide_depth", U32, [P]),
+    ("stk_bvh_num_emitters", U32, [P]),
     ("stk_scene_generate", C.c_int, [U32, U32, U32, C.POINTER(P)]),
     ("stk_scene_free", None, [P]),
     ("stk_scene_get", C.c_int, [P, C.POINTER(SceneArrays)]),

@@ -440,4 +440,200 @@ bool packBvh(const StkVertex* vb, uint32_t nv, const uint32_t* ib, uint32_t ni, 
     return true;
 }
 
+// ---- compressed 8-wide layout ---------------------------------------------------
+
+namespace {
+
+struct WideChild {
+    uint32_t node; // skip-link node index
+    bool leaf;
+    float lo[3], hi[3];
+};
+
+float decodePlane(float origin, uint32_t q, float scale)
+{
+    const float prod = (float)q * scale; // exact: q < 2^8, scale a power of two
+    return origin + prod;
+}
+
+float powerOfTwo(int e) { return bits_to_float((uint32_t)e << 23); } // 2^(e - 127), 1 <= e <= 254
+
+// Smallest exponent byte e with 2^(e-127) * 253 >= extent, at least 2^-100
+// and no finer than a quarter ulp of the origin's magnitude.
+uint8_t chooseExponent(float origin, float extent)
+{
+    const double need = std::max((double)extent / 253.0, std::ldexp(1.0, -100));
+    const double ulp = std::fabs((double)origin) * std::ldexp(1.0, -25);
+    const double target = std::max(need, ulp);
+    int e = (int)std::ceil(std::log2(target)) + 127;
+    while (e > 1 && std::ldexp(1.0, e - 1 - 127) * 253.0 >= (double)extent && std::ldexp(1.0, e - 1 - 127) >= target)
+        --e;
+    while (std::ldexp(1.0, e - 127) < target) ++e;
+    return (uint8_t)std::min(254, std::max(1, e));
+}
+
+} // namespace
+
+bool buildWide(const StkInstance* inst, uint32_t ninst, Bvh& bvh, std::string& err)
+{
+    bvh.wide.clear();
+    bvh.wideTris.clear();
+    bvh.emitters.clear();
+    bvh.wideDepth = 0;
+    for (const StkTriangle& t : bvh.tris)
+        if (t.instId < ninst && inst[t.instId].lightId != -1) bvh.emitters.push_back(t);
+    const uint32_t n = (uint32_t)bvh.nodes.size();
+    if (n == 0) return true;
+    auto childOf = [&](uint32_t idx) {
+        WideChild c;
+        c.node = idx;
+        const StkBvhNode& nd = bvh.nodes[idx];
+        c.leaf = (uint32_t)nd.instId != STK_INVALID_INDEX;
+        if (!c.leaf) {
+            for (int a = 0; a < 3; ++a) {
+                c.lo[a] = nd.minBounds[a];
+                c.hi[a] = nd.maxBounds[a];
+            }
+            return c;
+        }
+        // a leaf's box: its triangle's box (the vertices as the hit test forms
+        // them), widened by 1e-5 of the larger of its extent and magnitude
+        const StkTriangle& t = bvh.tris[bvh.leafToTri[idx]];
+        for (int a = 0; a < 3; ++a) {
+            const float v0 = t.v0[a], v1 = t.v0[a] + t.e0[a], v2 = t.v0[a] + t.e1[a];
+            const float lo = std::min(v0, std::min(v1, v2)), hi = std::max(v0, std::max(v1, v2));
+            const float mag = std::max(std::fabs(lo), std::fabs(hi));
+            const float pad = 1e-5f * std::max(hi - lo, mag) + 1e-30f;
+            c.lo[a] = lo - pad;
+            c.hi[a] = hi + pad;
+        }
+        return c;
+    };
+    auto area = [](const WideChild& c) {
+        const float dx = c.hi[0] - c.lo[0], dy = c.hi[1] - c.lo[1], dz = c.hi[2] - c.lo[2];
+        return dx * dy + dy * dz + dz * dx;
+    };
+    struct Job {
+        uint32_t node; // skip-link inner node (or leaf for a single-triangle tree)
+        uint32_t wide; // wide node to fill
+        uint32_t level;
+    };
+    std::vector<Job> queue;
+    bvh.wide.emplace_back();
+    queue.push_back({0, 0, 1});
+    for (size_t qi = 0; qi < queue.size(); ++qi) {
+        const Job job = queue[qi];
+        bvh.wideDepth = std::max(bvh.wideDepth, job.level);
+        std::vector<WideChild> kids;
+        if ((uint32_t)bvh.nodes[job.node].instId != STK_INVALID_INDEX) {
+            kids.push_back(childOf(job.node)); // the whole tree is one leaf
+        } else {
+            kids.push_back(childOf(job.node + 1));
+            kids.push_back(childOf((uint32_t)bvh.nodes[job.node + 1].nodeOffset));
+        }
+        // open the inner child of largest surface area until 8 children
+        while (kids.size() < 8) {
+            int best = -1;
+            float bestArea = -1.0f;
+            for (size_t k = 0; k < kids.size(); ++k)
+                if (!kids[k].leaf && area(kids[k]) > bestArea) {
+                    bestArea = area(kids[k]);
+                    best = (int)k;
+                }
+            if (best < 0) break;
+            const uint32_t idx = kids[best].node;
+            kids[best] = childOf(idx + 1);
+            kids.push_back(childOf((uint32_t)bvh.nodes[idx + 1].nodeOffset));
+        }
+        // node box and slot assignment by the child's centre relative to the node's
+        float lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = kids[0].lo[a];
+            hi[a] = kids[0].hi[a];
+            for (const WideChild& c : kids) {
+                lo[a] = std::min(lo[a], c.lo[a]);
+                hi[a] = std::max(hi[a], c.hi[a]);
+            }
+        }
+        int slotOf[8];
+        bool slotUsed[8] = {false, false, false, false, false, false, false, false};
+        std::vector<bool> placed(kids.size(), false);
+        for (size_t round = 0; round < kids.size(); ++round) {
+            double bestScore = -1e300;
+            int bk = -1, bs = -1;
+            for (size_t k = 0; k < kids.size(); ++k) {
+                if (placed[k]) continue;
+                double d[3];
+                for (int a = 0; a < 3; ++a)
+                    d[a] = 0.5 * ((double)kids[k].lo[a] + kids[k].hi[a]) - 0.5 * ((double)lo[a] + hi[a]);
+                for (int sl = 0; sl < 8; ++sl) {
+                    if (slotUsed[sl]) continue;
+                    double score = 0.0;
+                    for (int a = 0; a < 3; ++a) score += ((sl >> a) & 1 ? d[a] : -d[a]);
+                    if (score > bestScore) {
+                        bestScore = score;
+                        bk = (int)k;
+                        bs = sl;
+                    }
+                }
+            }
+            placed[bk] = true;
+            slotUsed[bs] = true;
+            slotOf[bk] = bs;
+        }
+        int kidAt[8];
+        for (int sl = 0; sl < 8; ++sl) kidAt[sl] = -1;
+        for (size_t k = 0; k < kids.size(); ++k) kidAt[slotOf[k]] = (int)k;
+        StkWideNode w;
+        memset(&w, 0, sizeof(w));
+        float scale[3];
+        for (int a = 0; a < 3; ++a) {
+            w.origin[a] = lo[a];
+            w.exps[a] = chooseExponent(lo[a], hi[a] - lo[a]);
+            scale[a] = powerOfTwo(w.exps[a]);
+        }
+        w.childBase = (uint32_t)bvh.wide.size();
+        w.triBase = (uint32_t)bvh.wideTris.size();
+        uint32_t inner = 0;
+        for (int sl = 0; sl < 8; ++sl) {
+            if (kidAt[sl] < 0) continue;
+            const WideChild& c = kids[kidAt[sl]];
+            w.vmask |= 1u << sl;
+            for (int a = 0; a < 3; ++a) {
+                // outward rounding checked against the decode the traversal performs
+                double ql = std::floor(((double)c.lo[a] - (double)w.origin[a]) / (double)scale[a]);
+                double qh = std::ceil(((double)c.hi[a] - (double)w.origin[a]) / (double)scale[a]);
+                uint32_t qlo = (uint32_t)std::min(255.0, std::max(0.0, ql));
+                uint32_t qhi = (uint32_t)std::min(255.0, std::max(0.0, qh));
+                while (qlo > 0 && decodePlane(w.origin[a], qlo, scale[a]) > c.lo[a]) --qlo;
+                while (qhi < 255 && decodePlane(w.origin[a], qhi, scale[a]) < c.hi[a]) ++qhi;
+                if (decodePlane(w.origin[a], qlo, scale[a]) > c.lo[a] ||
+                    decodePlane(w.origin[a], qhi, scale[a]) < c.hi[a]) {
+                    err = "wide BVH: a child box does not fit its node's quantisation grid";
+                    return false;
+                }
+                w.qlo[a][sl] = (uint8_t)qlo;
+                w.qhi[a][sl] = (uint8_t)qhi;
+            }
+            if (c.leaf) {
+                bvh.wideTris.push_back(bvh.tris[bvh.leafToTri[c.node]]);
+            } else {
+                w.imask |= (uint8_t)(1u << sl);
+                ++inner;
+            }
+        }
+        const uint32_t firstChild = (uint32_t)bvh.wide.size();
+        bvh.wide.resize(bvh.wide.size() + inner);
+        uint32_t r = 0;
+        for (int sl = 0; sl < 8; ++sl)
+            if (kidAt[sl] >= 0 && !kids[kidAt[sl]].leaf) queue.push_back({kids[kidAt[sl]].node, firstChild + r++, job.level + 1});
+        bvh.wide[job.wide] = w;
+    }
+    if (bvh.wideTris.size() != bvh.tris.size()) {
+        err = "wide BVH: leaf count differs from the binary tree";
+        return false;
+    }
+    return true;
+}
+
 } // namespace stk

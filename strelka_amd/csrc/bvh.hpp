@@ -24,6 +24,11 @@ struct Bvh {
     std::vector<uint32_t> leafToTri;   // preorder node index -> triangle index
     StkBvhHeader header{};
     uint32_t depth = 0;
+    // compressed 8-wide layout (buildWide)
+    std::vector<StkWideNode> wide;     // node 0 is the root
+    std::vector<StkTriangle> wideTris; // leaf triangles in wide-node slot order
+    std::vector<StkTriangle> emitters; // triangles of emitter instances (lightId != -1)
+    uint32_t wideDepth = 0;            // inner-node levels of the wide tree
 };
 
 // createBvhBuffer (src/render/ptrender.cpp:460-512): world-space positions of
@@ -37,6 +42,10 @@ bool buildBvh(const std::vector<BvhInputTri>& tris, const StkBvhOptions& opts, B
 // Validates a skip-link array and derives packed nodes and triangles from it.
 bool packBvh(const StkVertex* vb, uint32_t nv, const uint32_t* ib, uint32_t ni, const StkInstance* inst,
              uint32_t ninst, Bvh& bvh, std::string& err);
+
+// Collapses the packed binary BVH into the compressed 8-wide layout and
+// collects the emitter triangles.  Needs packBvh's output.
+bool buildWide(const StkInstance* inst, uint32_t ninst, Bvh& bvh, std::string& err);
 
 StkBvhOptions defaultBvhOptions();
 

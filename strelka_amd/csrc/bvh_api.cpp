@@ -23,7 +23,7 @@ STK_API stk_status stk_bvh_build_host(const StkVertex* vb, uint32_t nv, const ui
         std::string err;
         const StkBvhOptions o = opts ? *opts : stk::defaultBvhOptions();
         if (!stk::flattenInstances(vb, nv, ib, ni, inst, ninst, tris, err) || !stk::buildBvh(tris, o, b->bvh, err) ||
-            !stk::packBvh(vb, nv, ib, ni, inst, ninst, b->bvh, err)) {
+            !stk::packBvh(vb, nv, ib, ni, inst, ninst, b->bvh, err) || !stk::buildWide(inst, ninst, b->bvh, err)) {
             delete b;
             return STK_E_INVALID;
         }
@@ -46,7 +46,7 @@ STK_API stk_status stk_bvh_from_nodes(const StkVertex* vb, uint32_t nv, const ui
     try {
         std::string err;
         b->bvh.nodes.assign(nodes, nodes + numNodes);
-        if (!stk::packBvh(vb, nv, ib, ni, inst, ninst, b->bvh, err)) {
+        if (!stk::packBvh(vb, nv, ib, ni, inst, ninst, b->bvh, err) || !stk::buildWide(inst, ninst, b->bvh, err)) {
             delete b;
             return STK_E_INVALID;
         }
@@ -72,5 +72,10 @@ STK_API StkBvhHeader stk_bvh_header(const stk_bvh* b)
     return b ? b->bvh.header : h;
 }
 STK_API uint32_t stk_bvh_depth(const stk_bvh* b) { return b ? b->bvh.depth : 0; }
+STK_API uint32_t stk_bvh_num_wide(const stk_bvh* b) { return b ? (uint32_t)b->bvh.wide.size() : 0; }
+STK_API const StkWideNode* stk_bvh_wide_nodes(const stk_bvh* b) { return b ? b->bvh.wide.data() : nullptr; }
+STK_API const StkTriangle* stk_bvh_wide_triangles(const stk_bvh* b) { return b ? b->bvh.wideTris.data() : nullptr; }
+STK_API uint32_t stk_bvh_wide_depth(const stk_bvh* b) { return b ? b->bvh.wideDepth : 0; }
+STK_API uint32_t stk_bvh_num_emitters(const stk_bvh* b) { return b ? (uint32_t)b->bvh.emitters.size() : 0; }
 
 } // extern "C"

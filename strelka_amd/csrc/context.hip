@@ -65,7 +65,7 @@ struct stk_ctx {
     stk::Bvh bvh;
     bool haveBvh = false;
 
-    DevBuf vb, ib, inst, lights, mats, packed, tris, refNodes;
+    DevBuf vb, ib, inst, lights, mats, packed, tris, refNodes, wideNodes, wideTris, emitters;
     DevBuf rayScratch, hitScratch;
 
     // render state
@@ -85,7 +85,7 @@ struct stk_ctx {
     std::vector<BatchScene> batch;
     bool batchReady = false;
     std::vector<stk::SceneDesc> batchDescs;
-    DevBuf fVb, fIb, fInst, fLights, fMats, fPacked, fTris, fRefNodes;
+    DevBuf fVb, fIb, fInst, fLights, fMats, fPacked, fTris, fRefNodes, fWide, fWideTris, fEmit;
     uint32_t fNumMats = 0;
 
     // radiance cache
@@ -98,9 +98,10 @@ struct stk_ctx {
 
     std::vector<DevBuf*> owned()
     {
-        return {&vb, &ib, &inst, &lights, &mats, &packed, &tris, &refNodes, &rayScratch, &hitScratch,
+        return {&vb, &ib, &inst, &lights, &mats, &packed, &tris, &refNodes, &wideNodes, &wideTris, &emitters,
+                &rayScratch, &hitScratch,
                 &accum, &history, &out, &samples, &descs, &fVb, &fIb, &fInst, &fLights, &fMats, &fPacked,
-                &fTris, &fRefNodes, &cGridH, &cGridP, &cGridM, &cGridV, &cGridG, &cMlpP,
+                &fTris, &fRefNodes, &fWide, &fWideTris, &fEmit, &cGridH, &cGridP, &cGridM, &cGridV, &cGridG, &cMlpP,
                 &cMlpM, &cMlpV, &cMlpG, &cFwd, &cBwd, &cXT, &cDyT, &cLoss, &cGridE, &cGridEW, &cGridHE, &cMlpE,
                 &cFwdE, &cBwdE, &queryRecs, &queryOut, &trainVerts, &trainCount, &trainBatch};
     }
@@ -164,6 +165,9 @@ stk_status devUpload(stk_ctx* c, DevBuf& b, const void* src, size_t bytes)
     return STK_OK;
 }
 
+// The wide walk needs one node-group stack entry per wide level.
+bool wideUsable(const stk::Bvh& b) { return !b.wide.empty() && b.wideDepth <= (uint32_t)stk::dev::kWideStack; }
+
 stk::BvhView bvhView(stk_ctx* c)
 {
     stk::BvhView v;
@@ -172,6 +176,11 @@ stk::BvhView bvhView(stk_ctx* c)
     v.refNodes = static_cast<const StkBvhNode*>(c->refNodes.ptr);
     v.instances = static_cast<const StkInstance*>(c->inst.ptr);
     v.hdr = c->bvh.header;
+    v.wide = static_cast<const float4*>(c->wideNodes.ptr);
+    v.wideTris = static_cast<const float4*>(c->wideTris.ptr);
+    v.emitters = static_cast<const float4*>(c->emitters.ptr);
+    v.wideRoot = wideUsable(c->bvh) ? 0u : STK_INVALID_INDEX;
+    v.emitCount = (uint32_t)c->bvh.emitters.size();
     return v;
 }
 
@@ -186,6 +195,11 @@ stk_status uploadBvh(stk_ctx* c)
     if ((st = devUpload(c, c->refNodes, dev.data(), dev.size() * sizeof(StkBvhNode))) != STK_OK) return st;
     if ((st = devUpload(c, c->packed, b.packed.data(), b.packed.size() * sizeof(StkPackedNode))) != STK_OK) return st;
     if ((st = devUpload(c, c->tris, b.tris.data(), b.tris.size() * sizeof(StkTriangle))) != STK_OK) return st;
+    if ((st = devUpload(c, c->wideNodes, b.wide.data(), b.wide.size() * sizeof(StkWideNode))) != STK_OK) return st;
+    if ((st = devUpload(c, c->wideTris, b.wideTris.data(), b.wideTris.size() * sizeof(StkTriangle))) != STK_OK)
+        return st;
+    if ((st = devUpload(c, c->emitters, b.emitters.data(), b.emitters.size() * sizeof(StkTriangle))) != STK_OK)
+        return st;
     c->haveBvh = true;
     return STK_OK;
 }
@@ -319,7 +333,8 @@ STK_API stk_status stk_build_bvh(stk_ctx* c, const StkBvhOptions* opts)
                                    c->hinst.data(), (uint32_t)c->hinst.size(), tris, err) ||
             !stk::buildBvh(tris, o, c->bvh, err) ||
             !stk::packBvh(c->hvb.data(), (uint32_t)c->hvb.size(), c->hib.data(), (uint32_t)c->hib.size(),
-                          c->hinst.data(), (uint32_t)c->hinst.size(), c->bvh, err))
+                          c->hinst.data(), (uint32_t)c->hinst.size(), c->bvh, err) ||
+            !stk::buildWide(c->hinst.data(), (uint32_t)c->hinst.size(), c->bvh, err))
             return fail(c, STK_E_INVALID, "stk_build_bvh: " + err);
     } catch (const std::bad_alloc&) {
         return fail(c, STK_E_OOM, "stk_build_bvh: host allocation failed");
@@ -340,7 +355,8 @@ STK_API stk_status stk_set_bvh(stk_ctx* c, const StkBvhNode* nodes, uint32_t n)
         c->bvh = stk::Bvh{};
         c->bvh.nodes.assign(nodes, nodes + n);
         if (!stk::packBvh(c->hvb.data(), (uint32_t)c->hvb.size(), c->hib.data(), (uint32_t)c->hib.size(),
-                          c->hinst.data(), (uint32_t)c->hinst.size(), c->bvh, err))
+                          c->hinst.data(), (uint32_t)c->hinst.size(), c->bvh, err) ||
+            !stk::buildWide(c->hinst.data(), (uint32_t)c->hinst.size(), c->bvh, err))
             return fail(c, STK_E_INVALID, "stk_set_bvh: " + err);
     } catch (const std::bad_alloc&) {
         return fail(c, STK_E_OOM, "stk_set_bvh: host allocation failed");
@@ -368,7 +384,7 @@ STK_API stk_status stk_trace_closest_device(stk_ctx* c, const StkRay* rays, uint
     if (!c->haveBvh) return fail(c, STK_E_STATE, "stk_trace_closest: no BVH");
     if (n && (!rays || !hits)) return fail(c, STK_E_INVALID, "stk_trace_closest: null buffer");
     (void)hipSetDevice(c->device);
-    STK_HIP(c, stk::launchTraceClosest(bvhView(c), rays, n, hits, (flags & STK_TRACE_REFERENCE_ORDER) != 0,
+    STK_HIP(c, stk::launchTraceClosest(bvhView(c), rays, n, hits, flags,
                                        c->stream));
     return STK_OK;
 }
@@ -379,7 +395,7 @@ STK_API stk_status stk_trace_any_device(stk_ctx* c, const StkRay* rays, uint32_t
     if (!c->haveBvh) return fail(c, STK_E_STATE, "stk_trace_any: no BVH");
     if (n && (!rays || !hits)) return fail(c, STK_E_INVALID, "stk_trace_any: null buffer");
     (void)hipSetDevice(c->device);
-    STK_HIP(c, stk::launchTraceAny(bvhView(c), rays, n, hits, (flags & STK_TRACE_REFERENCE_ORDER) != 0, c->stream));
+    STK_HIP(c, stk::launchTraceAny(bvhView(c), rays, n, hits, flags, c->stream));
     return STK_OK;
 }
 
@@ -397,9 +413,9 @@ static stk_status traceHost(stk_ctx* c, const StkRay* rays, uint32_t n, StkHit* 
     StkRay* dr = static_cast<StkRay*>(c->rayScratch.ptr);
     StkHit* dh = static_cast<StkHit*>(c->hitScratch.ptr);
     if (any)
-        STK_HIP(c, stk::launchTraceAny(bvhView(c), dr, n, dh, (flags & STK_TRACE_REFERENCE_ORDER) != 0, c->stream));
+        STK_HIP(c, stk::launchTraceAny(bvhView(c), dr, n, dh, flags, c->stream));
     else
-        STK_HIP(c, stk::launchTraceClosest(bvhView(c), dr, n, dh, (flags & STK_TRACE_REFERENCE_ORDER) != 0,
+        STK_HIP(c, stk::launchTraceClosest(bvhView(c), dr, n, dh, flags,
                                            c->stream));
     STK_HIP(c, hipMemcpyAsync(hits, dh, (size_t)n * sizeof(StkHit), hipMemcpyDeviceToHost, c->stream));
     STK_HIP(c, hipStreamSynchronize(c->stream));
@@ -567,11 +583,13 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
     if (graph) {
         // everything the captured kernels take by value
         std::vector<uint64_t> key = {(uint64_t)sv.bvh.packed, (uint64_t)sv.bvh.tris, (uint64_t)sv.bvh.refNodes,
+                                     (uint64_t)sv.bvh.wide, (uint64_t)sv.bvh.wideTris, (uint64_t)sv.bvh.emitters,
                                      (uint64_t)sv.bvh.instances, (uint64_t)sv.vb, (uint64_t)sv.ib,
                                      (uint64_t)sv.inst, (uint64_t)sv.lights, (uint64_t)sv.mats, sv.numMats,
                                      (uint64_t)sv.scenes, sv.numScenes, sv.totalSlots, sv.tile, lp.maxDepth,
                                      lp.spp, lp.samplingType, lp.rrStartDepth, lp.refCompat, lp.passSlots,
                                      c->launch.blockSize, c->launch.tile, c->launch.persistentBlocks, c->ps.segCap,
+                                     c->launch.binaryWalk,
                                      (uint64_t)accum, (uint64_t)samples, (uint64_t)c->ps.rayO, c->ps.capacity};
         if (!c->graphExec || key != c->graphKey) {
             invalidateGraph(c);
@@ -652,6 +670,9 @@ STK_API stk_status stk_render(stk_ctx* c, const StkRenderParams* p, float* outRg
     memset(&d, 0, sizeof(d));
     d.hdr = c->bvh.header;
     d.refRoot = 0;
+    d.wideRoot = wideUsable(c->bvh) ? 0u : STK_INVALID_INDEX;
+    d.emitBase = 0;
+    d.emitCount = (uint32_t)c->bvh.emitters.size();
     d.lightBase = 0;
     d.lightCount = c->numLights;
     d.width = (uint32_t)W;
@@ -738,7 +759,8 @@ STK_API stk_status stk_batch_commit(stk_ctx* c, const StkBvhOptions* opts)
                                            b.inst.data(), (uint32_t)b.inst.size(), tris, b.err) ||
                     !stk::buildBvh(tris, o, b.bvh, b.err) ||
                     !stk::packBvh(b.vb.data(), (uint32_t)b.vb.size(), b.ib.data(), (uint32_t)b.ib.size(),
-                                  b.inst.data(), (uint32_t)b.inst.size(), b.bvh, b.err))
+                                  b.inst.data(), (uint32_t)b.inst.size(), b.bvh, b.err) ||
+                    !stk::buildWide(b.inst.data(), (uint32_t)b.inst.size(), b.bvh, b.err))
                     continue;
                 if (b.bvh.depth > (uint32_t)stk::dev::kStackDepth) b.err = "tree deeper than the traversal stack";
             } catch (const std::bad_alloc&) {
@@ -755,7 +777,7 @@ STK_API stk_status stk_batch_commit(stk_ctx* c, const StkBvhOptions* opts)
         if (!c->batch[i].err.empty())
             return fail(c, STK_E_INVALID, "stk_batch_commit: scene " + std::to_string(i) + ": " + c->batch[i].err);
 
-    size_t nv = 0, ni = 0, nin = 0, nl = 0, nm = 0, nn = 0, np = 0, nt = 0;
+    size_t nv = 0, ni = 0, nin = 0, nl = 0, nm = 0, nn = 0, np = 0, nt = 0, nw = 0, ne = 0;
     for (const BatchScene& b : c->batch) {
         nv += b.vb.size();
         ni += b.ib.size();
@@ -765,6 +787,8 @@ STK_API stk_status stk_batch_commit(stk_ctx* c, const StkBvhOptions* opts)
         nn += b.bvh.nodes.size();
         np += b.bvh.packed.size();
         nt += b.bvh.tris.size();
+        nw += b.bvh.wide.size();
+        ne += b.bvh.emitters.size();
     }
     if (nv >= (1ull << 31) || ni >= (1ull << 31) || nn >= (1ull << 31) || np >= (1ull << 31) || nt >= (1ull << 31))
         return fail(c, STK_E_INVALID, "stk_batch_commit: merged scene exceeds 2^31 elements");
@@ -775,9 +799,13 @@ STK_API stk_status stk_batch_commit(stk_ctx* c, const StkBvhOptions* opts)
     std::vector<StkMaterial> mats;
     std::vector<StkBvhNode> nodes;
     std::vector<StkPackedNode> packed;
-    std::vector<StkTriangle> tris;
+    std::vector<StkTriangle> tris, wideTris, emit;
+    std::vector<StkWideNode> wide;
     std::vector<stk::SceneDesc> descs(n);
     try {
+        wide.reserve(nw);
+        wideTris.reserve(nt);
+        emit.reserve(ne);
         vb.reserve(nv);
         ib.reserve(ni);
         inst.reserve(nin);
@@ -819,11 +847,21 @@ STK_API stk_status stk_batch_commit(stk_ctx* c, const StkBvhOptions* opts)
                     pn.child[ch] = pn.child[ch] >= 0 ? pn.child[ch] + (int32_t)pBase : ~(~pn.child[ch] + (int32_t)tBase);
                 packed.push_back(pn);
             }
-            for (StkTriangle t : b.bvh.tris) {
+            auto rebased = [&](StkTriangle t) {
                 t.leaf += nBase;
                 t.instId += instBase;
-                tris.push_back(t);
+                return t;
+            };
+            for (const StkTriangle& t : b.bvh.tris) tris.push_back(rebased(t));
+            const uint32_t wBase = (uint32_t)wide.size(), wtBase = (uint32_t)wideTris.size();
+            const uint32_t eBase = (uint32_t)emit.size();
+            for (StkWideNode w : b.bvh.wide) {
+                w.childBase += wBase;
+                w.triBase += wtBase;
+                wide.push_back(w);
             }
+            for (const StkTriangle& t : b.bvh.wideTris) wideTris.push_back(rebased(t));
+            for (const StkTriangle& t : b.bvh.emitters) emit.push_back(rebased(t));
             stk::SceneDesc& d = descs[i];
             memset(&d, 0, sizeof(d));
             d.hdr = b.bvh.header;
@@ -832,6 +870,9 @@ STK_API stk_status stk_batch_commit(stk_ctx* c, const StkBvhOptions* opts)
             else if (d.hdr.root != INT32_MIN)
                 d.hdr.root = ~(~d.hdr.root + (int32_t)tBase);
             d.refRoot = b.bvh.nodes.empty() ? STK_INVALID_INDEX : nBase;
+            d.wideRoot = wideUsable(b.bvh) ? wBase : STK_INVALID_INDEX;
+            d.emitBase = eBase;
+            d.emitCount = (uint32_t)b.bvh.emitters.size();
             d.lightBase = lBase;
             d.lightCount = (uint32_t)b.lights.size();
             d.width = (uint32_t)b.cam.dimension[0];
@@ -852,6 +893,10 @@ STK_API stk_status stk_batch_commit(stk_ctx* c, const StkBvhOptions* opts)
     if ((st = devUpload(c, c->fRefNodes, nodes.data(), nodes.size() * sizeof(StkBvhNode))) != STK_OK) return st;
     if ((st = devUpload(c, c->fPacked, packed.data(), packed.size() * sizeof(StkPackedNode))) != STK_OK) return st;
     if ((st = devUpload(c, c->fTris, tris.data(), tris.size() * sizeof(StkTriangle))) != STK_OK) return st;
+    if ((st = devUpload(c, c->fWide, wide.data(), wide.size() * sizeof(StkWideNode))) != STK_OK) return st;
+    if ((st = devUpload(c, c->fWideTris, wideTris.data(), wideTris.size() * sizeof(StkTriangle))) != STK_OK)
+        return st;
+    if ((st = devUpload(c, c->fEmit, emit.data(), emit.size() * sizeof(StkTriangle))) != STK_OK) return st;
     c->fNumMats = (uint32_t)mats.size();
     c->batchDescs = std::move(descs);
     c->batchReady = true;
@@ -861,6 +906,9 @@ STK_API stk_status stk_batch_commit(stk_ctx* c, const StkBvhOptions* opts)
         std::vector<StkPackedNode>().swap(b.bvh.packed);
         std::vector<StkTriangle>().swap(b.bvh.tris);
         std::vector<uint32_t>().swap(b.bvh.leafToTri);
+        std::vector<StkWideNode>().swap(b.bvh.wide);
+        std::vector<StkTriangle>().swap(b.bvh.wideTris);
+        std::vector<StkTriangle>().swap(b.bvh.emitters);
     }
     return STK_OK;
 }
@@ -894,6 +942,10 @@ STK_API stk_status stk_batch_render(stk_ctx* c, const StkRenderParams* p, float*
     sv.bvh.tris = static_cast<const float4*>(c->fTris.ptr);
     sv.bvh.refNodes = static_cast<const StkBvhNode*>(c->fRefNodes.ptr);
     sv.bvh.instances = static_cast<const StkInstance*>(c->fInst.ptr);
+    sv.bvh.wide = static_cast<const float4*>(c->fWide.ptr);
+    sv.bvh.wideTris = static_cast<const float4*>(c->fWideTris.ptr);
+    sv.bvh.emitters = static_cast<const float4*>(c->fEmit.ptr);
+    sv.bvh.wideRoot = STK_INVALID_INDEX; // per scene: SceneDesc.wideRoot
     sv.vb = static_cast<const StkVertex*>(c->fVb.ptr);
     sv.ib = static_cast<const uint32_t*>(c->fIb.ptr);
     sv.inst = static_cast<const StkInstance*>(c->fInst.ptr);
@@ -918,6 +970,7 @@ STK_API stk_status stk_set_launch_config(stk_ctx* c, const StkLaunchConfig* lc)
     c->launch.tile = lc->tile;
     c->launch.persistentBlocks = lc->persistentBlocks;
     c->launch.pathsInFlight = lc->pathsInFlight;
+    c->launch.binaryWalk = lc->binaryWalk ? 1u : 0u;
     c->useGraph = lc->useGraph != 0;
     invalidateGraph(c);
     return STK_OK;
@@ -949,7 +1002,8 @@ STK_API stk_status stk_get_launch_config(stk_ctx* c, StkLaunchConfig* lc)
     lc->persistentBlocks = c->launch.persistentBlocks;
     lc->useGraph = c->useGraph ? 1u : 0u;
     lc->pathsInFlight = c->launch.pathsInFlight;
-    lc->pad[0] = lc->pad[1] = lc->pad[2] = 0;
+    lc->binaryWalk = c->launch.binaryWalk;
+    lc->pad[0] = lc->pad[1] = 0;
     return STK_OK;
 }
 

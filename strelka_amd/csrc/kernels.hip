@@ -35,15 +35,22 @@ __device__ __forceinline__ RayF load_ray(const StkRay* rays, uint32_t i)
     return make_ray(p[0], p[1]);
 }
 
+static_assert(2 * kWideStack <= kStackDepth, "the wide walk's two stacks share the packed walk's LDS");
+
+// Closest hit: the compressed 8-wide walk, or the packed BVH2 walk when asked
+// (packed != 0) or when the scene has no usable wide tree.
 __global__ __launch_bounds__(kBlock) void k_trace_closest(BvhView bvh, const StkRay* __restrict__ rays, uint32_t n,
-                                                          StkHit* __restrict__ hits)
+                                                          StkHit* __restrict__ hits, int packed)
 {
     __shared__ uint32_t stack[kStackDepth * kBlock];
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (i >= n) return;
     const RayF r = load_ray(rays, i);
     HitF h;
-    closest_packed(bvh, bvh.hdr, r, stack + threadIdx.x, kBlock, h);
+    if (packed || bvh.wideRoot == STK_INVALID_INDEX)
+        closest_packed(bvh, bvh.hdr, r, stack + threadIdx.x, kBlock, h);
+    else
+        closest_wide(bvh, bvh.wideRoot, r, stack + threadIdx.x, stack + kWideStack * kBlock + threadIdx.x, kBlock, h);
     store_hit(hits, i, h);
 }
 
@@ -168,22 +175,23 @@ inline uint32_t blocks(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlock);
 
 } // namespace
 
-hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, bool refOrder,
+hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags,
                               hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    if (refOrder)
+    if (flags & STK_TRACE_REFERENCE_ORDER)
         hipLaunchKernelGGL(k_trace_closest_ref, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
     else
-        hipLaunchKernelGGL(k_trace_closest, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
+        hipLaunchKernelGGL(k_trace_closest, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits,
+                           (flags & STK_TRACE_BVH2) ? 1 : 0);
     return hipGetLastError();
 }
 
-hipError_t launchTraceAny(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, bool refOrder,
+hipError_t launchTraceAny(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags,
                           hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    if (refOrder)
+    if (flags & STK_TRACE_REFERENCE_ORDER)
         hipLaunchKernelGGL(k_trace_any_ref, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
     else
         hipLaunchKernelGGL(k_trace_any, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);

@@ -16,6 +16,8 @@ namespace stk {
 struct SceneDesc {
     StkBvhHeader hdr;        // root rebased into the merged packed-node array
     uint32_t refRoot;        // first node of the scene in the merged skip-link array
+    uint32_t wideRoot;       // root in the merged wide-node array (STK_INVALID_INDEX: use the packed walk)
+    uint32_t emitBase, emitCount; // the scene's emitter triangles in the merged emitter array
     uint32_t lightBase, lightCount;
     uint32_t pixelBase;      // first pixel of the view in the merged image
     uint32_t slotBase;       // first path slot (views are padded to whole tiles)
@@ -122,11 +124,14 @@ struct LaunchConfig {
     uint32_t tile = 8;             // raygen tile edge (1 = scanline)
     uint32_t persistentBlocks = 0; // extend grid for persistent waves (0: one lane per ray)
     uint32_t pathsInFlight = 0;    // slots per pass (0: every slot of the frame in one pass)
+    uint32_t binaryWalk = 0;       // 1: packed BVH2 walks instead of the compressed 8-wide ones
 };
 
-hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, bool refOrder,
+// flags: STK_TRACE_REFERENCE_ORDER (skip-link walk), STK_TRACE_BVH2 (packed
+// binary walk instead of the wide one; closest hit only).
+hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags,
                               hipStream_t s);
-hipError_t launchTraceAny(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, bool refOrder,
+hipError_t launchTraceAny(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags,
                           hipStream_t s);
 hipError_t launchCameraRays(const StkPathTracerParam& p, uint32_t sampleNum, StkRay* rays, hipStream_t s);
 

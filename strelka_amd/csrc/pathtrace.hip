@@ -183,13 +183,22 @@ __global__ void k_totals_clear(PathState ps)
     if (threadIdx.x < 2) ray_totals(ps)[threadIdx.x] = 0;
 }
 
+static_assert(2 * kWideStack <= kStackDepth, "the wide walk's two stacks share the packed walk's LDS");
+
+// `stack` is this lane's column of kStackDepth words (BLOCK apart): the packed
+// walk's stack, or the wide walk's two node-group stacks of kWideStack words.
 template <int BLOCK>
-__device__ __forceinline__ void extend_one(const SceneView& sc, PathState& ps, int cur, uint32_t e, uint32_t* stack)
+__device__ __forceinline__ void extend_one(const SceneView& sc, PathState& ps, int cur, uint32_t e, uint32_t* stack,
+                                           int binary)
 {
     const uint32_t slot = ps.queue[cur][e];
     const RayF r = make_ray(ps.rayO[slot], ps.rayD[slot]);
+    const SceneDesc& d = sc.scenes[ps.scene[slot]];
     HitF h;
-    closest_packed(sc.bvh, sc.scenes[ps.scene[slot]].hdr, r, stack, BLOCK, h);
+    if (!binary && d.wideRoot != STK_INVALID_INDEX)
+        closest_wide(sc.bvh, d.wideRoot, r, stack, stack + kWideStack * BLOCK, BLOCK, h);
+    else
+        closest_packed(sc.bvh, d.hdr, r, stack, BLOCK, h);
     ps.hit[e] = make_float4(h.u, h.v, h.t, bits_to_float(h.inst));
     ps.hitPrim[e] = h.prim;
 }
@@ -200,14 +209,15 @@ __device__ __forceinline__ void extend_one(const SceneView& sc, PathState& ps, i
 // moving on when a segment is exhausted (a plain load checks before the
 // atomic), so every wave ends after at most kStripes segments.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_pt_extend(SceneView sc, PathState ps, int depth, int persistent)
+__global__ __launch_bounds__(BLOCK) void k_pt_extend(SceneView sc, PathState ps, int depth, int persistent,
+                                                     int binary)
 {
     __shared__ uint32_t stack[kStackDepth * BLOCK];
     const int cur = depth & 1;
     const uint32_t* counts = ps.stripes + cur * kStripes;
     if (!persistent) {
         uint32_t e;
-        if (stripe_entry(ps, counts, BLOCK, e)) extend_one<BLOCK>(sc, ps, cur, e, stack + threadIdx.x);
+        if (stripe_entry(ps, counts, BLOCK, e)) extend_one<BLOCK>(sc, ps, cur, e, stack + threadIdx.x, binary);
         return;
     }
     const uint32_t lane = threadIdx.x % 64;
@@ -222,7 +232,7 @@ __global__ __launch_bounds__(BLOCK) void k_pt_extend(SceneView sc, PathState ps,
             base = (uint32_t)__shfl((int)base, 0);
             if (base >= cnt) break;
             const uint32_t q = base + lane;
-            if (q < cnt) extend_one<BLOCK>(sc, ps, cur, sg * ps.segCap + q, stack + threadIdx.x);
+            if (q < cnt) extend_one<BLOCK>(sc, ps, cur, sg * ps.segCap + q, stack + threadIdx.x, binary);
         }
     }
 }
@@ -396,19 +406,30 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, 
     }
 }
 
-// Shadow rays use the stackless skip-link walk: for these long incoherent
-// segments it beats the packed left-first walk (no LDS stack, so 8 waves per
-// SIMD instead of 5) while returning the same first-in-preorder hit.
-__global__ __launch_bounds__(kBlock) void k_pt_connect(SceneView sc, PathState ps)
+// Emitter triangles a shadow ray tests before the wide walk (beyond this the
+// scene's shadow rays take the skip-link walk).
+constexpr uint32_t kMaxEmitters = 64;
+
+// Shadow rays: the wide visibility walk (the reference's first-in-preorder
+// any-hit rule, decided in any order), or the stackless skip-link walk when
+// the scene has no usable wide tree or many emitter triangles.
+__global__ __launch_bounds__(kBlock) void k_pt_connect(SceneView sc, PathState ps, int binary)
 {
+    __shared__ uint32_t stack[2 * kWideStack * kBlock];
     uint32_t s;
     if (!stripe_entry(ps, ps.stripes + 2 * kStripes, kBlock, s)) return;
     const uint32_t slot = ps.shadowQ[s];
     const SceneDesc& desc = sc.scenes[ps.scene[slot]];
     const RayF r = make_ray(ps.shO[s], ps.shD[s]);
-    HitF h;
-    bool vis = !any_ref(sc.bvh, desc.hdr, desc.refRoot, r, h);
-    if (!vis && sc.inst[h.inst].lightId != -1) vis = true; // a light hit counts as visible
+    bool vis;
+    if (!binary && desc.wideRoot != STK_INVALID_INDEX && desc.emitCount <= kMaxEmitters) {
+        vis = visible_wide(sc.bvh, desc.wideRoot, sc.bvh.emitters + 3 * (size_t)desc.emitBase, desc.emitCount, r,
+                           stack + threadIdx.x, stack + kWideStack * kBlock + threadIdx.x, kBlock);
+    } else {
+        HitF h;
+        vis = !any_ref(sc.bvh, desc.hdr, desc.refRoot, r, h);
+        if (!vis && sc.inst[h.inst].lightId != -1) vis = true; // a light hit counts as visible
+    }
     if (vis) {
         F3 col = f3(ps.col[slot]);
         col = col + f3(ps.shCd[s]);
@@ -497,10 +518,11 @@ inline dim3 stripeGrid(const PathState& ps, uint32_t B) { return dim3(kStripes *
 template <int BLOCK>
 void launchExtend(const SceneView& sc, PathState& ps, int depth, const LaunchConfig& lc, hipStream_t s)
 {
+    const int binary = lc.binaryWalk ? 1 : 0;
     if (lc.persistentBlocks)
-        hipLaunchKernelGGL(k_pt_extend<BLOCK>, dim3(lc.persistentBlocks), dim3(BLOCK), 0, s, sc, ps, depth, 1);
+        hipLaunchKernelGGL(k_pt_extend<BLOCK>, dim3(lc.persistentBlocks), dim3(BLOCK), 0, s, sc, ps, depth, 1, binary);
     else
-        hipLaunchKernelGGL(k_pt_extend<BLOCK>, stripeGrid(ps, BLOCK), dim3(BLOCK), 0, s, sc, ps, depth, 0);
+        hipLaunchKernelGGL(k_pt_extend<BLOCK>, stripeGrid(ps, BLOCK), dim3(BLOCK), 0, s, sc, ps, depth, 0, binary);
 }
 
 } // namespace
@@ -537,7 +559,7 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
         default: launchExtend<256>(sc, ps, depth, lc, s); break;
         }
         hipLaunchKernelGGL(k_pt_shade, stripeGrid(ps, kBlock), b, 0, s, sc, lp, ps, depth);
-        hipLaunchKernelGGL(k_pt_connect, stripeGrid(ps, kBlock), b, 0, s, sc, ps);
+        hipLaunchKernelGGL(k_pt_connect, stripeGrid(ps, kBlock), b, 0, s, sc, ps, lc.binaryWalk ? 1 : 0);
     }
     hipLaunchKernelGGL(k_pt_tally, dim3(1), dim3(64), 0, s, ps);
     if (query && cb) {

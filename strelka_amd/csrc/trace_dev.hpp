@@ -8,6 +8,11 @@
 //    BVH.  Leaf boxes are never tested (the reference tests a leaf's triangle
 //    whenever its parent box is entered, :211-231).
 //  * closest_ref / any_ref: the reference's stackless skip-link walk.
+//  * closest_wide / visible_wide: the compressed 8-wide tree (StkWideNode),
+//    80-B nodes with 8-bit child boxes, node-group stack entries.  Same
+//    result rule as closest_packed; visible_wide answers the shadow-ray
+//    question of the path tracer with the reference's any-hit semantics
+//    (first leaf in preorder decides) in any visiting order.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,7 +33,14 @@ struct BvhView {
     const StkBvhNode* refNodes;   // skip-link nodes; leaves carry the triangle index in maxBounds[0]
     const StkInstance* instances; // for the any-hit light test
     StkBvhHeader hdr;
+    const float4* wide;           // 5 x float4 per StkWideNode
+    const float4* wideTris;       // 3 x float4 per StkTriangle, wide slot order
+    const float4* emitters;       // 3 x float4 per emitter triangle
+    uint32_t wideRoot;            // STK_INVALID_INDEX: no usable wide tree
+    uint32_t emitCount;
 };
+
+constexpr int kWideStack = 16; // node-group entries; the builder's wide depth must not exceed it
 
 struct RayF {
     float ox, oy, oz, tmax;
@@ -328,6 +340,209 @@ __device__ __forceinline__ bool any_packed(const BvhView& bvh, const StkBvhHeade
             if (sp == 0) return false;
             next = (int)stack[--sp * stride];
         }
+    }
+}
+
+// ---- compressed 8-wide traversal ----------------------------------------------
+
+// One visited wide node: which slots' boxes the ray enters, their entry
+// distances, and the node's child ranges.  Slots are evaluated in a fixed
+// order (so byte lanes and register indices are static and the code is
+// branch-free); octant order only decides the order in which a pushed node
+// group is popped.
+struct WideVisit {
+    uint32_t childBase, triBase, imask, leafMask;
+    uint32_t innerHits, leafHits; // bit s: the box of slot s is entered
+    float t[8];                   // entry distance of slot s
+};
+
+template <int S>
+__device__ __forceinline__ float qbyte(uint32_t lo4, uint32_t hi4)
+{
+    const uint32_t w = S < 4 ? lo4 : hi4;
+    return (float)((w >> (8 * (S & 3))) & 0xFFu); // one v_cvt_f32_ubyteN
+}
+
+template <int S>
+__device__ __forceinline__ void wide_slot(const float4& w0, const uint4& q0, const uint4& q1, const uint4& q2, float sx,
+                                          float sy, float sz, const RayF& r, float lim, uint32_t& hits, float& t)
+{
+    // origin + q * scale with q * scale exact (q < 2^8, scale a power of two),
+    // so the fused form rounds once, exactly like the builder's check
+    const float lox = fmaf(qbyte<S>(q0.x, q0.y), sx, w0.x);
+    const float loy = fmaf(qbyte<S>(q0.z, q0.w), sy, w0.y);
+    const float loz = fmaf(qbyte<S>(q1.x, q1.y), sz, w0.z);
+    const float hix = fmaf(qbyte<S>(q1.z, q1.w), sx, w0.x);
+    const float hiy = fmaf(qbyte<S>(q2.x, q2.y), sy, w0.y);
+    const float hiz = fmaf(qbyte<S>(q2.z, q2.w), sz, w0.z);
+    float t0;
+    const bool in = slab(r, lox, hix, loy, hiy, loz, hiz, t0) && t0 <= r.tmax && !(t0 > lim);
+    hits |= in ? (1u << S) : 0u;
+    t = t0;
+}
+
+__device__ __forceinline__ void wide_visit(const float4* node, const RayF& r, float lim, WideVisit& v)
+{
+    const float4 w0 = node[0], w1 = node[1];
+    const uint4 q0 = *reinterpret_cast<const uint4*>(node + 2); // qlo x[0..7], qlo y[0..7]
+    const uint4 q1 = *reinterpret_cast<const uint4*>(node + 3); // qlo z[0..7], qhi x[0..7]
+    const uint4 q2 = *reinterpret_cast<const uint4*>(node + 4); // qhi y[0..7], qhi z[0..7]
+    const uint32_t meta = float_to_bits(w0.w);
+    // scale 2^(e - 127) built from the exponent byte: exact powers of two
+    const float sx = bits_to_float((meta & 0xFFu) << 23);
+    const float sy = bits_to_float(((meta >> 8) & 0xFFu) << 23);
+    const float sz = bits_to_float(((meta >> 16) & 0xFFu) << 23);
+    v.imask = meta >> 24;
+    v.childBase = float_to_bits(w1.x);
+    v.triBase = float_to_bits(w1.y);
+    const uint32_t vmask = float_to_bits(w1.z) & 0xFFu;
+    v.leafMask = vmask & ~v.imask;
+    uint32_t hits = 0;
+    wide_slot<0>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[0]);
+    wide_slot<1>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[1]);
+    wide_slot<2>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[2]);
+    wide_slot<3>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[3]);
+    wide_slot<4>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[4]);
+    wide_slot<5>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[5]);
+    wide_slot<6>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[6]);
+    wide_slot<7>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[7]);
+    hits &= vmask;
+    v.innerHits = hits & v.imask;
+    v.leafHits = hits & v.leafMask;
+}
+
+__device__ __forceinline__ uint32_t ray_octant(const RayF& r)
+{
+    return (signbit(r.dx) ? 1u : 0u) | (signbit(r.dy) ? 2u : 0u) | (signbit(r.dz) ? 4u : 0u);
+}
+
+__device__ __forceinline__ uint32_t rank_below(uint32_t mask, uint32_t s) { return __popc(mask & ((1u << s) - 1u)); }
+
+// Bit s of m moved to bit s ^ oct (an involution on 8-bit masks).
+__device__ __forceinline__ uint32_t xor_permute(uint32_t m, uint32_t oct)
+{
+    if (oct & 1u) m = ((m & 0x55u) << 1) | ((m >> 1) & 0x55u);
+    if (oct & 2u) m = ((m & 0x33u) << 2) | ((m >> 2) & 0x33u);
+    if (oct & 4u) m = ((m & 0x0Fu) << 4) | ((m >> 4) & 0x0Fu);
+    return m;
+}
+
+// Next node of the walk: the nearest entered inner child of the visited node,
+// the rest of its entered inner children pushed as one node-group entry
+// (childBase, imask | remaining << 8, remaining in visiting order k = s ^ oct);
+// otherwise the next child of the top group.  Returns false when the walk is
+// over.
+__device__ __forceinline__ bool wide_next(const WideVisit& v, uint32_t oct, uint32_t* sA, uint32_t* sB, int stride,
+                                          int& sp, uint32_t& node)
+{
+    if (v.innerHits) {
+        uint32_t m = v.innerHits;
+        int sn = 0;
+        float tn = INFINITY;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) // static indices keep t[] in registers
+            if (((m >> s) & 1u) && v.t[s] < tn) {
+                tn = v.t[s];
+                sn = s;
+            }
+        m &= ~(1u << sn);
+        if (m) {
+            sA[sp * stride] = v.childBase;
+            sB[sp * stride] = v.imask | (xor_permute(m, oct) << 8);
+            ++sp;
+        }
+        node = v.childBase + rank_below(v.imask, (uint32_t)sn);
+        return true;
+    }
+    if (sp == 0) return false;
+    const uint32_t base = sA[(sp - 1) * stride];
+    const uint32_t word = sB[(sp - 1) * stride];
+    const uint32_t imask = word & 0xFFu;
+    uint32_t m = word >> 8;
+    const int k = __ffs(m) - 1;
+    m &= m - 1u;
+    if (m)
+        sB[(sp - 1) * stride] = imask | (m << 8);
+    else
+        --sp;
+    node = base + rank_below(imask, (uint32_t)k ^ oct);
+    return true;
+}
+
+// Closest hit over the wide tree rooted at `root`.  sA/sB: this lane's first
+// slots of two LDS stacks of kWideStack words, `stride` words apart.
+__device__ __forceinline__ bool closest_wide(const BvhView& bvh, uint32_t root, const RayF& r, uint32_t* sA,
+                                             uint32_t* sB, int stride, HitF& h)
+{
+    miss(h);
+    bool found = false;
+    if (root == STK_INVALID_INDEX) return false;
+    const uint32_t oct = ray_octant(r);
+    int sp = 0;
+    uint32_t node = root;
+    for (;;) {
+        WideVisit v;
+        wide_visit(bvh.wide + 5 * (size_t)node, r, (found ? h.t : 1e9f) * kPruneSlack, v);
+        for (uint32_t m = v.leafHits; m; m &= m - 1u) {
+            const uint32_t s = (uint32_t)(__ffs(m) - 1);
+            const uint32_t tri = v.triBase + rank_below(v.leafMask, s);
+            const float4* p = bvh.wideTris + 3 * (size_t)tri;
+            const float4 a = p[0], b = p[1], c = p[2];
+            float t, u, w;
+            if (tri_test(r, a, b, c, t, u, w) && t < r.tmax) {
+                const uint32_t leaf = float_to_bits(a.w);
+                const bool acc = found ? (t < h.t || (t == h.t && leaf < h.leaf)) : (t < 1e9f);
+                if (acc) {
+                    found = true;
+                    h.t = t;
+                    h.u = u;
+                    h.v = w;
+                    h.leaf = leaf;
+                    h.inst = float_to_bits(b.w);
+                    h.prim = float_to_bits(c.w);
+                }
+            }
+        }
+        if (!wide_next(v, oct, sA, sB, stride, sp, node)) return found;
+    }
+}
+
+// Shadow-ray visibility with the reference's any-hit rule: anyHit returns the
+// first leaf in preorder whose triangle is hit with t < tmax, and the light
+// counts as visible when there is none or when that leaf belongs to an emitter
+// (shaders/pathtracerMdl.hlsl:31-92).  Equivalently: occluded iff some
+// non-emitter leaf is hit before, in preorder, every hit emitter leaf.  The
+// emitter triangles (few) are tested first; the walk then stops at the first
+// qualifying occluder, in any order.
+__device__ __forceinline__ bool visible_wide(const BvhView& bvh, uint32_t root, const float4* emit, uint32_t emitCount,
+                                             const RayF& r, uint32_t* sA, uint32_t* sB, int stride)
+{
+    uint32_t lightLeaf = STK_INVALID_INDEX;
+    for (uint32_t e = 0; e < emitCount; ++e) {
+        const float4* p = emit + 3 * (size_t)e;
+        const float4 a = p[0], b = p[1], c = p[2];
+        float t, u, w;
+        if (tri_test(r, a, b, c, t, u, w) && t < r.tmax) lightLeaf = min(lightLeaf, float_to_bits(a.w));
+    }
+    if (root == STK_INVALID_INDEX) return true;
+    const uint32_t oct = ray_octant(r);
+    int sp = 0;
+    uint32_t node = root;
+    for (;;) {
+        WideVisit v;
+        wide_visit(bvh.wide + 5 * (size_t)node, r, 1e9f * kPruneSlack, v);
+        for (uint32_t m = v.leafHits; m; m &= m - 1u) {
+            const uint32_t s = (uint32_t)(__ffs(m) - 1);
+            const uint32_t tri = v.triBase + rank_below(v.leafMask, s);
+            const float4* p = bvh.wideTris + 3 * (size_t)tri;
+            const float4 a = p[0], b = p[1], c = p[2];
+            if (float_to_bits(a.w) >= lightLeaf) continue; // preorder after a hit emitter: cannot decide
+            float t, u, w;
+            if (tri_test(r, a, b, c, t, u, w) && t < r.tmax &&
+                bvh.instances[float_to_bits(b.w)].lightId == -1)
+                return false;
+        }
+        if (!wide_next(v, oct, sA, sB, stride, sp, node)) return true;
     }
 }
 
