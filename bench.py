@@ -41,16 +41,17 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(scene, nodes, rays):
-    """The oracle's closest-hit (reference order and the kernel's pruned order)
-    on the host cores, over the same rays and BVH."""
+def cpu_baseline(scene, nodes, rays, wide, wide_tris):
+    """The oracle's closest-hit on the host cores over the same rays and BVH:
+    the reference order, the binary kernel's pruned order and the wide
+    kernel's walk (whose counters give that kernel's algorithmic bytes)."""
     from oracle import Oracle
-    from oracle.pyoracle import MODE_PRUNED, MODE_REF
+    from oracle.pyoracle import MODE_PRUNED, MODE_REF, MODE_WIDE
 
     threads = max(1, min(16, os.cpu_count() or 1))
-    o = Oracle.from_scene(scene, nodes)
+    o = Oracle.from_scene(scene, nodes, wide, wide_tris)
     res = {}
-    for name, mode in (("reference_order", MODE_REF), ("pruned", MODE_PRUNED)):
+    for name, mode in (("reference_order", MODE_REF), ("pruned", MODE_PRUNED), ("wide", MODE_WIDE)):
         o.trace(rays[:4096], mode, threads)  # warm-up
         times = []
         for _ in range(3):
@@ -392,21 +393,31 @@ def main() -> int:
     cpu = None
     roof = None
     if not args.no_cpu_baseline:
-        cores, res = cpu_baseline(scene, nodes, rays)
-        pr = res["pruned"]
-        bytes_per_ray = 32 + 20 + 64 * pr["inner_per_ray"] + 48 * pr["leaf_per_ray"]
+        hb = sa.HostBvh.build(scene)  # the same deterministic build, host side: the wide layout for the oracle
+        if hb.nodes.tobytes() != nodes.tobytes():
+            raise RuntimeError("host and context BVH builds differ")
+        cores, res = cpu_baseline(scene, nodes, rays, hb.wide, hb.wide_tris)
+        wd, pr = res["wide"], res["pruned"]
+        # algorithmic bytes of the wide kernel: ray in, hit out, one 80-B node
+        # per wide visit, one 48-B triangle per test (counts from the oracle's
+        # restatement of the same walk on the same rays)
+        bytes_per_ray = 32 + 20 + 80 * wd["inner_per_ray"] + 48 * wd["leaf_per_ray"]
         achieved = n * bytes_per_ray / (kernel_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("trace_closest_config2"),
-                "kernel": "k_trace_closest", "kernel_ms": round(kernel_ms, 4),
+                "kernel": "k_trace_closest (compressed 8-wide walk)", "kernel_ms": round(kernel_ms, 4),
                 "bytes_per_ray": round(bytes_per_ray, 1),
-                "node_fetches_per_ray": round(pr["inner_per_ray"], 2),
-                "triangle_tests_per_ray": round(pr["leaf_per_ray"], 2)}
+                "wide_node_visits_per_ray": round(wd["inner_per_ray"], 2),
+                "triangle_tests_per_ray": round(wd["leaf_per_ray"], 2),
+                "binary_walk": {"node_fetches_per_ray": round(pr["inner_per_ray"], 2),
+                                "triangle_tests_per_ray": round(pr["leaf_per_ray"], 2),
+                                "bytes_per_ray": round(32 + 20 + 64 * pr["inner_per_ray"] + 48 * pr["leaf_per_ray"], 1),
+                                "kernel_ms": round(bin_kernel_ms, 4)}}
         cpu = {"value": round(res["reference_order"]["mrays_per_s"], 4), "unit": "Mrays/s", "cores": cores,
                "kind": "port",
                "sample": f"the full {n}-ray 1080p primary frame, oracle closestHit in reference order "
                          f"(skip-link, unpruned), median of 3, {cpu_model()}",
-               "pruned_mrays_per_s": round(pr["mrays_per_s"], 4)}
+               "pruned_mrays_per_s": round(pr["mrays_per_s"], 4), "wide_mrays_per_s": round(wd["mrays_per_s"], 4)}
         p = sa.render_params(scene.camera, spp=4, max_depth=8, frame=1, sampling=1, sky=scene.sky)
         pt["cpu_baseline"] = cpu_pt_baseline(scene, nodes, p)
 

@@ -390,6 +390,124 @@ typedef struct {
 
 #define ORC_CHUNK 4096u
 
+/* ------------------------------------------------------------------------ */
+/* The compressed 8-wide walk of the GPU kernel (its design, not the          */
+/* reference's); hits must equal orc_closest_hit's.                           */
+/* ------------------------------------------------------------------------ */
+#define ORC_WIDE_STACK 64
+
+static inline float wide_scale(uint8_t e)
+{
+    const uint32_t bits = (uint32_t)e << 23;
+    float f;
+    memcpy(&f, &bits, sizeof(f));
+    return f;
+}
+
+static inline uint32_t xor_permute8(uint32_t m, uint32_t oct)
+{
+    if (oct & 1u) m = ((m & 0x55u) << 1) | ((m >> 1) & 0x55u);
+    if (oct & 2u) m = ((m & 0x33u) << 2) | ((m >> 2) & 0x33u);
+    if (oct & 4u) m = ((m & 0x0Fu) << 4) | ((m >> 4) & 0x0Fu);
+    return m;
+}
+
+static inline uint32_t popc8(uint32_t m) { return (uint32_t)__builtin_popcount(m & 0xFFu); }
+
+int orc_closest_hit_wide(const OrcScene* s, const StkRay* r, StkHit* hit, OrcStats* st)
+{
+    miss_hit(hit);
+    if (s->nwide == 0) return 0;
+    float inv[3];
+    ray_invdir(r, inv);
+    const uint32_t oct = (signbit(r->d[0]) ? 1u : 0u) | (signbit(r->d[1]) ? 2u : 0u) | (signbit(r->d[2]) ? 4u : 0u);
+    const float tmax = r->o[3];
+    int found = 0;
+    float best = 1e9f;
+    uint32_t bestLeaf = STK_INVALID_INDEX;
+    uint64_t nvisit = 0, ntri = 0;
+    uint32_t stackBase[ORC_WIDE_STACK], stackWord[ORC_WIDE_STACK];
+    int sp = 0;
+    uint32_t node = 0;
+    for (;;) {
+        const StkWideNode* w = &s->wide[node];
+        ++nvisit;
+        const float lim = (found ? best : 1e9f) * ORC_PRUNE_SLACK;
+        float sc[3], tch[8];
+        uint32_t hits = 0;
+        for (int a = 0; a < 3; ++a) sc[a] = wide_scale(w->exps[a]);
+        for (int sl = 0; sl < 8; ++sl) {
+            float lo[3], hi[3], t0;
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = fmaf((float)w->qlo[a][sl], sc[a], w->origin[a]);
+                hi[a] = fmaf((float)w->qhi[a][sl], sc[a], w->origin[a]);
+            }
+            tch[sl] = 0.0f;
+            if (orc_intersect_ray_box(r, inv, lo, hi, &t0) && t0 <= tmax && !(t0 > lim)) {
+                hits |= 1u << sl;
+                tch[sl] = t0;
+            }
+        }
+        const uint32_t vmask = w->vmask & 0xFFu, imask = w->imask, leafMask = vmask & ~imask;
+        hits &= vmask;
+        for (int sl = 0; sl < 8; ++sl) {
+            if (!((hits & leafMask) >> sl & 1u)) continue;
+            const StkTriangle* tr = &s->wideTris[w->triBase + popc8(leafMask & ((1u << sl) - 1u))];
+            float v0[3], e0[3], e1[3], t, b[2];
+            orc_get_triangle(s, tr->instId, tr->primId, v0, e0, e1);
+            ++ntri;
+            if (orc_ray_triangle(r->o, r->d, v0, e0, e1, &t, b) && t < tmax &&
+                accept(found, t, tr->leaf, best, bestLeaf)) {
+                found = 1;
+                best = t;
+                bestLeaf = tr->leaf;
+                hit->t = t;
+                hit->bary[0] = b[0];
+                hit->bary[1] = b[1];
+                hit->instId = tr->instId;
+                hit->primId = tr->primId;
+            }
+        }
+        uint32_t inner = hits & imask;
+        if (inner) {
+            int sn = 0;
+            float tn = INFINITY;
+            for (int sl = 0; sl < 8; ++sl)
+                if ((inner >> sl & 1u) && tch[sl] < tn) {
+                    tn = tch[sl];
+                    sn = sl;
+                }
+            inner &= ~(1u << sn);
+            if (inner) {
+                if (sp >= ORC_WIDE_STACK) abort();
+                stackBase[sp] = w->childBase;
+                stackWord[sp] = imask | (xor_permute8(inner, oct) << 8);
+                ++sp;
+            }
+            node = w->childBase + popc8(imask & ((1u << sn) - 1u));
+            continue;
+        }
+        if (sp == 0) break;
+        const uint32_t base = stackBase[sp - 1], word = stackWord[sp - 1];
+        const uint32_t im = word & 0xFFu;
+        uint32_t m = word >> 8;
+        const int k = __builtin_ctz(m);
+        m &= m - 1u;
+        if (m)
+            stackWord[sp - 1] = im | (m << 8);
+        else
+            --sp;
+        node = base + popc8(im & ((1u << ((uint32_t)k ^ oct)) - 1u));
+    }
+    if (st) {
+        st->inner += nvisit;
+        st->leaf += ntri;
+        st->rays += 1;
+        st->hits += (uint64_t)found;
+    }
+    return found;
+}
+
 static void* batch_worker(void* p)
 {
     BatchArg* a = (BatchArg*)p;
@@ -406,6 +524,7 @@ static void* batch_worker(void* p)
             case ORC_MODE_REF: orc_closest_hit(a->s, &a->rays[i], &a->hits[i], &a->stats); break;
             case ORC_MODE_PRUNED: orc_closest_hit_pruned(a->s, &a->rays[i], &a->hits[i], &a->stats); break;
             case ORC_MODE_BRUTE: orc_closest_hit_brute(a->s, &a->rays[i], &a->hits[i]); break;
+            case ORC_MODE_WIDE: orc_closest_hit_wide(a->s, &a->rays[i], &a->hits[i], &a->stats); break;
             default: orc_any_hit(a->s, &a->rays[i], &a->hits[i], &a->stats); break;
             }
         }

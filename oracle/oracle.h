@@ -53,6 +53,11 @@ typedef struct OrcScene {
     uint32_t nlights;
     const StkMaterial* mats;
     uint32_t nmats;
+    /* compressed 8-wide layout of the same tree (optional: for ORC_MODE_WIDE) */
+    const StkWideNode* wide;
+    uint32_t nwide;
+    const StkTriangle* wideTris;
+    uint32_t nwideTris;
 } OrcScene;
 
 /* Traversal counters: inner = inner nodes whose box was tested (reference
@@ -84,7 +89,16 @@ int orc_closest_hit_brute(const OrcScene* s, const StkRay* r, StkHit* hit);
 /* anyHit (shaders/raytracing.h:249-286). Returns 1 on hit, fills inst/prim/t. */
 int orc_any_hit(const OrcScene* s, const StkRay* r, StkHit* hit, OrcStats* st);
 
-enum { ORC_MODE_REF = 0, ORC_MODE_PRUNED = 1, ORC_MODE_BRUTE = 2, ORC_MODE_ANY = 3 };
+/* The GPU kernel's walk over the compressed 8-wide nodes (StkWideNode):
+ * child boxes decoded as origin + q * 2^(e-127) (one rounding), the slab test
+ * of intersectRayBox, leaf triangles re-formed by getTriangle from the leaf's
+ * (instance, primitive), nearest entered inner child first and the others
+ * pushed as one node group popped in octant order, the same acceptance and
+ * pruning rules as orc_closest_hit_pruned.  Counts wide-node visits (inner)
+ * and triangle tests (leaf). */
+int orc_closest_hit_wide(const OrcScene* s, const StkRay* r, StkHit* hit, OrcStats* st);
+
+enum { ORC_MODE_REF = 0, ORC_MODE_PRUNED = 1, ORC_MODE_BRUTE = 2, ORC_MODE_ANY = 3, ORC_MODE_WIDE = 4 };
 /* Batched traversal over n rays using nthreads host threads (chunks of 4096
  * rays handed out dynamically).  stats may be NULL. */
 void orc_trace_batch(const OrcScene* s, const StkRay* rays, uint32_t n, StkHit* hits, int mode, int nthreads,

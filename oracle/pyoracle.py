@@ -18,13 +18,14 @@ BVH_NODE = np.dtype([("minBounds", "<f4", 3), ("instId", "<i4"), ("maxBounds", "
 RAY = np.dtype([("o", "<f4", 4), ("d", "<f4", 4)])
 HIT = np.dtype([("bary", "<f4", 2), ("t", "<f4"), ("instId", "<u4"), ("primId", "<u4")])
 
-MODE_REF, MODE_PRUNED, MODE_BRUTE, MODE_ANY = 0, 1, 2, 3
+MODE_REF, MODE_PRUNED, MODE_BRUTE, MODE_ANY, MODE_WIDE = 0, 1, 2, 3, 4
 
 
 class OrcScene(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("nnodes", C.c_uint32), ("inst", C.c_void_p), ("ninst", C.c_uint32),
                 ("vb", C.c_void_p), ("nv", C.c_uint32), ("ib", C.c_void_p), ("ni", C.c_uint32),
-                ("lights", C.c_void_p), ("nlights", C.c_uint32), ("mats", C.c_void_p), ("nmats", C.c_uint32)]
+                ("lights", C.c_void_p), ("nlights", C.c_uint32), ("mats", C.c_void_p), ("nmats", C.c_uint32),
+                ("wide", C.c_void_p), ("nwide", C.c_uint32), ("wideTris", C.c_void_p), ("nwideTris", C.c_uint32)]
 
 
 class OracleStats(C.Structure):
@@ -100,7 +101,7 @@ def default_threads() -> int:
 class Oracle:
     """Holds the scene arrays and the skip-link BVH the oracle walks."""
 
-    def __init__(self, vb, ib, inst, nodes, lights=None, mats=None):
+    def __init__(self, vb, ib, inst, nodes, lights=None, mats=None, wide=None, wide_tris=None):
         self.lib = load()
         self.vb = np.ascontiguousarray(vb)
         self.ib = np.ascontiguousarray(ib, dtype=np.uint32)
@@ -117,11 +118,18 @@ class Oracle:
             s.lights, s.nlights = self.lights.ctypes.data, self.lights.size
         if self.mats is not None and self.mats.size:
             s.mats, s.nmats = self.mats.ctypes.data, self.mats.size
+        # the wide layout is an input (records of 80 B and 48 B), kept alive here
+        self.wide = None if wide is None else np.ascontiguousarray(wide)
+        self.wide_tris = None if wide_tris is None else np.ascontiguousarray(wide_tris)
+        if self.wide is not None and self.wide.size:
+            assert self.wide.dtype.itemsize == 80 and self.wide_tris.dtype.itemsize == 48
+            s.wide, s.nwide = self.wide.ctypes.data, self.wide.size
+            s.wideTris, s.nwideTris = self.wide_tris.ctypes.data, self.wide_tris.size
         self.scene = s
 
     @classmethod
-    def from_scene(cls, scene, nodes):
-        return cls(scene.vb, scene.ib, scene.inst, nodes, scene.lights, scene.mats)
+    def from_scene(cls, scene, nodes, wide=None, wide_tris=None):
+        return cls(scene.vb, scene.ib, scene.inst, nodes, scene.lights, scene.mats, wide, wide_tris)
 
     def trace(self, rays: np.ndarray, mode: int = MODE_REF, threads: int | None = None):
         rays = np.ascontiguousarray(rays, dtype=RAY)

@@ -1,0 +1,58 @@
+"""The GPU kernel's compressed 8-wide walk, restated in the oracle
+(orc_closest_hit_wide), must return exactly the reference walk's hits
+(orc_closest_hit, shaders/raytracing.h:198-247) on the same rays: this pins
+the wide algorithm on the CPU (quantised boxes conservative, node-group stack
+complete, acceptance rule order-free) before the GPU kernel is compared with
+it.  Its counters give the wide kernel's algorithmic bytes."""
+import numpy as np
+import pytest
+
+import strelka_amd as sa
+from oracle import Oracle
+from oracle.pyoracle import MODE_PRUNED, MODE_REF, MODE_WIDE
+
+
+def same(a, b):
+    return a.view(np.uint8).tobytes() == b.view(np.uint8).tobytes()
+
+
+def scene_oracle(config, variant=0):
+    s = sa.SceneData.generate(config, variant)
+    b = sa.HostBvh.build(s)
+    return s, b, Oracle.from_scene(s, b.nodes, b.wide, b.wide_tris)
+
+
+@pytest.mark.parametrize("config,variant,seed", [(1, 0, 0x5354524B00000001), (6, 0, 11), (7, 0, 12), (3, 10, 13)])
+def test_wide_walk_equals_reference_walk(config, variant, seed):
+    s, b, o = scene_oracle(config, variant)
+    rays = s.jittered_rays(seed)
+    ref, st_ref = o.trace(rays, MODE_REF)
+    wide, st_wide = o.trace(rays, MODE_WIDE)
+    pruned, st_pr = o.trace(rays, MODE_PRUNED)
+    assert same(ref, wide)
+    assert st_wide.hits == st_ref.hits > 0
+    # an 8-wide node visit stands for several binary node expansions
+    assert st_wide.inner < st_pr.inner
+    print(f"config {config}: per ray {st_pr.inner / rays.size:.1f} binary expansions, "
+          f"{st_wide.inner / rays.size:.1f} wide visits, {st_wide.leaf / rays.size:.1f} triangle tests")
+
+
+def test_wide_walk_secondary_rays_config2_sample():
+    """Incoherent rays (random origins in the scene box, random directions)
+    on the 1M-triangle scene: the case the GPU path tracer's bounces hit."""
+    s, b, o = scene_oracle(2)
+    rng = np.random.default_rng(5)
+    n = 20000
+    rays = np.zeros(n, sa.RAY)
+    lo, hi = np.array(b.header.lo), np.array(b.header.hi)
+    rays["o"][:, :3] = rng.uniform(lo, hi, (n, 3))
+    rays["o"][:, 3] = 1e9
+    d = rng.normal(size=(n, 3))
+    rays["d"][:, :3] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    # a few exactly axis-aligned directions (infinite inverse components)
+    rays["d"][:50, :3] = [0.0, -1.0, 0.0]
+    rays["d"][50:100, :3] = [1.0, 0.0, 0.0]
+    ref, _ = o.trace(rays, MODE_REF)
+    wide, st = o.trace(rays, MODE_WIDE)
+    assert same(ref, wide)
+    assert st.hits > n // 4
