@@ -8,7 +8,8 @@ exits with a message.
 
 Other modes (one JSON line each): --workload batch (32 scenes of 10k-2M
 triangles at 960x540, 16 spp, one batched wavefront pass per sample, launch
-shape from the autotuner), --cache-study (time to an RMSE target, cached vs
+shape from the autotuner), --workload instanced (10,000 instances of 20
+meshes, about 19 M triangles, 3840x2160, 4 spp, 6 bounces), --cache-study (time to an RMSE target, cached vs
 plain path tracing) and --copy-mib (plain copy kernel, calibrates the memory
 counters).
 """
@@ -174,6 +175,51 @@ def batch_bench(args) -> dict:
                          "device": dev["name"], "compute_units": dev["compute_units"]}}
 
 
+def instanced_bench(args) -> dict:
+    """The large instanced workload: 10,000 instances of 20 seeded meshes
+    (about 19 M triangles, flattened into one tree), 3840x2160, 4 spp per
+    frame, 6 bounces, RR after 3; launch shape from the autotuner on a
+    1-spp frame."""
+    import strelka_amd as sa
+
+    scene = sa.SceneData.generate(5)
+    ntri = int(scene.inst["indexCount"].sum()) // 3
+    ctx = sa.Context(0)
+    t0 = time.perf_counter()
+    ctx.set_scene(scene)
+    ctx.build_bvh()
+    build_s = time.perf_counter() - t0
+    print(f"instanced: {ntri} triangles, BVH built in {build_s:.1f} s", file=sys.stderr, flush=True)
+    w, h = scene.camera.dimension[0], scene.camera.dimension[1]
+    d_out = ctx.alloc(w * h * 16)
+    p1 = sa.render_params(scene.camera, spp=1, max_depth=6, frame=0, sampling=1, sky=scene.sky)
+    tuned = choose_launch(ctx, args.launch, lambda: ctx.render_device(p1, d_out), w * h)
+    spp = 4
+    p = sa.render_params(scene.camera, spp=spp, max_depth=6, frame=1, sampling=1, sky=scene.sky)
+    ctx.render_device(p, d_out)
+    ctx.sync()
+    ctx.record(8)
+    for f in range(args.steps):
+        p.pt.frameNumber = 2 + f
+        ctx.render_device(p, d_out)
+    ctx.record(9)
+    ctx.sync()
+    ms = ctx.elapsed_ms(8, 9) / args.steps
+    rays = ctx.ray_totals()
+    ctx.free(d_out)
+    ctx.close()
+    samples = w * h * spp
+    return {"metric": "samples/s, path tracing of the large instanced scene",
+            "value": round(samples / (ms * 1e-3), 1), "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": 1, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded PCG meshes and transforms)",
+            "config": {"workload": "10,000 instances of 20 seeded meshes flattened into one tree, 3840x2160, "
+                                   "4 spp, 6 bounces", "triangles": ntri, "bvh_build_s": round(build_s, 2)},
+            "mrays_per_s": round((rays[0] + rays[1]) / (ms * 1e-3) / 1e6, 1),
+            "rays_per_frame": {"closest": rays[0], "shadow": rays[1]},
+            "launch": tuned["best"], "autotune_1spp_ms": {"default": tuned.get("default_ms"), "best": tuned.get("ms")}}
+
+
 def load_traffic(kernel: str):
     f = ROOT / "profiles" / f"{kernel}_traffic.json"
     if f.exists():
@@ -288,9 +334,9 @@ def main() -> int:
     ap.add_argument("--launch", default="auto",
                     help="launch shape of the path tracer: 'auto' (autotuner) or a JSON object of "
                          "Context.set_launch_config arguments")
-    ap.add_argument("--workload", choices=("trace", "batch"), default="trace",
+    ap.add_argument("--workload", choices=("trace", "batch", "instanced"), default="trace",
                     help="trace: closest-hit on the 1M-triangle scene (+ its 64-spp path-traced frame); "
-                         "batch: the 32-scene batched path tracer")
+                         "batch: the 32-scene batched path tracer; instanced: the 19M-triangle instanced scene")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (calibrates memory counters)")
     ap.add_argument("--cache-study", action="store_true",
@@ -317,6 +363,9 @@ def main() -> int:
 
     if args.workload == "batch":
         print(json.dumps(batch_bench(args)))
+        return 0
+    if args.workload == "instanced":
+        print(json.dumps(instanced_bench(args)))
         return 0
 
     if args.copy_mib:

@@ -18,6 +18,11 @@
  *             materials, 1920x1080 camera; 1,164,290 triangles.
  *   config 3: one of the 32 uneven scenes of the batched launch (variant
  *             0..31), built as config 2 with ~N_i triangles, 960x540.
+ *   config 5: 20 seeded meshes (10 displaced icospheres of 1,280 triangles,
+ *             10 tori of 500-5,000) instanced 10,000 times with uniform random
+ *             rotations and translations in [-100,100]^3 (about 20 M
+ *             triangles), one Lambert/GGX material per instance, an 80x80 rect
+ *             light at y = 130 (radiance 8), sky (0.25, 0.3, 0.4), 3840x2160.
  *   config 6: a tiny scene (heightfield 16x16, 8 spheres) for smoke tests.
  *   config 7: the config-1 room closed and lit by a 1x1 ceiling light
  *             (walls albedo 0.7), 128x128: long multi-bounce paths.

@@ -183,6 +183,54 @@ void addBox(Scene& sc, float half, int32_t material, bool inward)
 
 void addInwardBox(Scene& sc, float half, int32_t material) { addBox(sc, half, material, true); }
 
+// Torus around the y axis: major radius R, minor radius r, U x V quads.
+void makeTorus(uint32_t U, uint32_t V, float R, float r, std::vector<StkVertex>& vb, std::vector<uint32_t>& ib)
+{
+    vb.clear();
+    ib.clear();
+    for (uint32_t i = 0; i < U; ++i)
+        for (uint32_t j = 0; j < V; ++j) {
+            const float a = 2.0f * (float)M_PI * (float)i / (float)U, b = 2.0f * (float)M_PI * (float)j / (float)V;
+            const F3 n = mk3(std::cos(a) * std::cos(b), std::sin(b), std::sin(a) * std::cos(b));
+            StkVertex v{};
+            v.position[0] = (R + r * std::cos(b)) * std::cos(a);
+            v.position[1] = r * std::sin(b);
+            v.position[2] = (R + r * std::cos(b)) * std::sin(a);
+            v.normal = packNormal(n);
+            v.tangent = packNormal(mk3(1.0f, 0.0f, 0.0f));
+            vb.push_back(v);
+        }
+    for (uint32_t i = 0; i < U; ++i)
+        for (uint32_t j = 0; j < V; ++j) {
+            const uint32_t a = i * V + j, b = ((i + 1) % U) * V + j, c = i * V + (j + 1) % V,
+                           d = ((i + 1) % U) * V + (j + 1) % V;
+            const uint32_t t[6] = {a, c, b, b, c, d};
+            ib.insert(ib.end(), t, t + 6);
+        }
+}
+
+// Uniform random rotation (Shoemake 1992, uniform unit quaternion) then a
+// translation uniform in [-extent, extent]^3.
+Mat4 randomRigid(Pcg32& rng, float extent)
+{
+    const float u1 = rng.uniform(), u2 = rng.uniform(), u3 = rng.uniform();
+    const float s1 = std::sqrt(1.0f - u1), s2 = std::sqrt(u1);
+    const float x = s1 * std::sin(2.0f * (float)M_PI * u2), y = s1 * std::cos(2.0f * (float)M_PI * u2);
+    const float z = s2 * std::sin(2.0f * (float)M_PI * u3), w = s2 * std::cos(2.0f * (float)M_PI * u3);
+    Mat4 r = Mat4::identity(); // column-major rotation of the quaternion (x, y, z, w)
+    r.m[0] = 1 - 2 * (y * y + z * z);
+    r.m[1] = 2 * (x * y + z * w);
+    r.m[2] = 2 * (x * z - y * w);
+    r.m[4] = 2 * (x * y - z * w);
+    r.m[5] = 1 - 2 * (x * x + z * z);
+    r.m[6] = 2 * (y * z + x * w);
+    r.m[8] = 2 * (x * z + y * w);
+    r.m[9] = 2 * (y * z - x * w);
+    r.m[10] = 1 - 2 * (x * x + y * y);
+    const float tx = rng.uniform(-extent, extent), ty = rng.uniform(-extent, extent), tz = rng.uniform(-extent, extent);
+    return Mat4::translate(tx, ty, tz) * r;
+}
+
 // Two-triangle floor y = 0 over [-extent, extent]^2, normal +y.
 void addFloor(Scene& sc, float extent, int32_t material)
 {
@@ -281,6 +329,52 @@ extern "C" STK_API stk_status stk_scene_generate(uint32_t config, uint32_t varia
             s->sky[0] = 0.5f;
             s->sky[1] = 0.6f;
             s->sky[2] = 0.8f;
+        } else if (config == 5) {
+            // 20 seeded meshes (10 displaced icospheres of 1,280 triangles, 10 tori
+            // of 500-5,000), 10,000 instances with random rigid transforms in
+            // [-100, 100]^3, one material per instance, a large rect light above
+            Pcg32 geo(kSeedBase + 5, 1), mat(kSeedBase + 5, 2);
+            std::vector<uint32_t> meshes;
+            for (int k = 0; k < 20; ++k) {
+                std::vector<StkVertex> vb;
+                std::vector<uint32_t> ib;
+                if (k % 2 == 0) {
+                    makeIcosphere(3, false, vb, ib);
+                    const float radius = geo.uniform(1.0f, 3.0f);
+                    for (StkVertex& v : vb) {
+                        const F3 p = ld3(v.position);
+                        const float d = radius * (1.0f + 0.15f * geo.uniform(-1.0f, 1.0f));
+                        const float l = std::sqrt(dot(p, p));
+                        v.position[0] = p.x / l * d;
+                        v.position[1] = p.y / l * d;
+                        v.position[2] = p.z / l * d;
+                    }
+                } else {
+                    const float target = geo.uniform(500.0f, 5000.0f);
+                    const uint32_t U = std::max(4u, (uint32_t)std::lround(std::sqrt(target)));
+                    const uint32_t V = std::max(3u, (uint32_t)std::lround(target / (2.0f * (float)U)));
+                    const float R = geo.uniform(1.0f, 3.0f);
+                    makeTorus(U, V, R, R * geo.uniform(0.2f, 0.4f), vb, ib);
+                }
+                meshes.push_back(sc.createMesh(vb, ib));
+            }
+            for (int k = 0; k < 10000; ++k) {
+                const uint32_t m = (uint32_t)(geo.next() % 20u);
+                const int32_t mi = (int32_t)sc.addMaterial(randomMaterial(mat));
+                sc.createInstance(meshes[m], mi, randomRigid(geo, 100.0f));
+            }
+            LightDesc d;
+            d.type = 0;
+            d.xform = Mat4::translate(0.0f, 130.0f, 0.0f) * Mat4::rotateX(-(float)M_PI / 2.0f);
+            d.width = 80.0f;
+            d.height = 80.0f;
+            d.intensity = 8.0f * 80.0f * 80.0f; // radiance 8
+            sc.createLight(d);
+            cameraMatrices(mk3(0.0f, 30.0f, 190.0f), mk3(0, 0, 0), mk3(0, 1, 0), 55.0f, 3840, 2160, 0.01f, 1000.0f,
+                           s->camera);
+            s->sky[0] = 0.25f;
+            s->sky[1] = 0.3f;
+            s->sky[2] = 0.4f;
         } else if (config == 8) {
             // convex furnace: an outward-facing Lambert cube under a constant sky, no lights
             StkMaterial m{};

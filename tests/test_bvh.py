@@ -152,3 +152,20 @@ def test_single_triangle_and_empty_scene():
                          sky=s.sky)
     e = sa.HostBvh.build(empty)
     assert e.nodes.size == 0 and e.header.root == -(2 ** 31)
+
+
+def test_instanced_scene_shape():
+    """The large instanced workload: 20 meshes, 10,000 rigid instances (plus
+    the light's), about 19 M triangles, transforms that are rotations plus a
+    translation inside [-100, 100]^3."""
+    s = sa.SceneData.generate(5)
+    inst = s.inst[s.inst["lightId"] == -1]
+    assert inst.size == 10000 and s.lights.size == 1
+    assert 15_000_000 < int(inst["indexCount"].sum()) // 3 < 25_000_000
+    m = inst["objectToWorld"].reshape(-1, 4, 4).transpose(0, 2, 1)  # column-major storage
+    R = m[:, :3, :3].astype(np.float64)
+    eye = np.einsum("nij,nkj->nik", R, R)
+    assert np.abs(eye - np.eye(3)).max() < 1e-5
+    assert np.all(np.abs(np.linalg.det(R) - 1.0) < 1e-5)
+    assert np.all(np.abs(m[:, :3, 3]) <= 100.0)
+    assert len(np.unique(inst["indexOffset"])) == 20
