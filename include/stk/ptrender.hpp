@@ -7,7 +7,7 @@
 //   PtRender::init / cleanup               stk::PtRender::init / cleanup
 //   PtRender::setScene(Scene*)             setScene(const SceneData&)   (arrays Scene::get* returns)
 //   PtRender::setActiveCameraIndex(i)      setActiveCameraIndex(i)
-//   PtRender::drawFrame(Image*)            drawFrame(Image&)            (RGBA32F, host or device)
+//   PtRender::drawFrame(Image*)            drawFrame(Image&)            (RGBA32F or RGBA8, pitch, host or device)
 //   PtRender::reloadPt()                   reloadPt()                   (restarts accumulation)
 //   SettingsManager "render/pt/..." keys   setSetting(key, value)
 //   ResourceManager::createBuffer/...      stk::ResourceManager
@@ -44,12 +44,14 @@ struct SceneData {
     std::vector<StkPathTracerParam> cameras;
 };
 
-// An RGBA32F image the frame is written to (a host array or a HIP device
-// pointer).
+// The image a frame is written to: RGBA32F or RGBA8 pixels, a row pitch (0 =
+// packed), in host memory or behind a HIP device (or mapped host) pointer.
 struct Image {
-    float* data = nullptr;
+    void* data = nullptr;
     uint32_t width = 0, height = 0;
     bool onDevice = false;
+    uint32_t format = STK_IMAGE_RGBA32F;
+    uint32_t pitchBytes = 0;
 };
 
 class PtRender {
@@ -154,7 +156,17 @@ public:
         p.enableAccumulation = mAccumulate ? 1u : 0u;
         p.enableTonemap = mTonemap ? 1u : 0u;
         for (int i = 0; i < 3; ++i) p.skyColor[i] = mSky[i];
-        return check(stk_render(mCtx, &p, result.data, result.onDevice ? 1 : 0));
+        if (result.format == STK_IMAGE_RGBA32F && result.pitchBytes == 0)
+            return check(stk_render(mCtx, &p, static_cast<float*>(result.data), result.onDevice ? 1 : 0));
+        StkImageDesc d;
+        std::memset(&d, 0, sizeof(d));
+        d.data = result.data;
+        d.width = result.width;
+        d.height = result.height;
+        d.pitchBytes = result.pitchBytes;
+        d.format = result.format;
+        d.onDevice = result.onDevice ? 1 : 0;
+        return check(stk_render_image(mCtx, &p, &d));
     }
 
     stk_ctx* context() const { return mCtx; }
@@ -181,13 +193,37 @@ private:
 // context's HIP allocations and copies.
 class ResourceManager {
 public:
+    enum class MemoryUsage { eGpuOnly, eCpuToGpu }; // device memory, or mapped pinned host memory
+
     explicit ResourceManager(stk_ctx* ctx) : mCtx(ctx) {}
-    void* createBuffer(size_t bytes)
+    void* createBuffer(size_t bytes, MemoryUsage usage = MemoryUsage::eGpuOnly)
     {
         void* p = nullptr;
+        if (usage == MemoryUsage::eCpuToGpu) {
+            if (stk_host_alloc(mCtx, bytes, &p) != STK_OK) return nullptr;
+            mMapped.push_back(p);
+            return p;
+        }
         return stk_device_alloc(mCtx, bytes, &p) == STK_OK ? p : nullptr;
     }
-    void destroyBuffer(void* p) { stk_device_free(mCtx, p); }
+    // Host address of a host-visible buffer (the same pointer: the memory is
+    // mapped into the device's address space); nullptr for device memory.
+    void* getMappedMemory(void* buffer) const
+    {
+        for (void* m : mMapped)
+            if (m == buffer) return buffer;
+        return nullptr;
+    }
+    void destroyBuffer(void* p)
+    {
+        for (size_t i = 0; i < mMapped.size(); ++i)
+            if (mMapped[i] == p) {
+                mMapped.erase(mMapped.begin() + (long)i);
+                stk_host_free(mCtx, p);
+                return;
+            }
+        stk_device_free(mCtx, p);
+    }
     stk_status copyToBuffer(void* dst, const void* src, size_t bytes) { return stk_copy_to_device(mCtx, dst, src, bytes); }
     stk_status copyFromBuffer(void* dst, const void* src, size_t bytes)
     {
@@ -198,7 +234,7 @@ public:
     Image createImage(uint32_t width, uint32_t height)
     {
         Image im;
-        im.data = static_cast<float*>(createBuffer((size_t)width * height * 16));
+        im.data = createBuffer((size_t)width * height * 16);
         im.width = im.data ? width : 0;
         im.height = im.data ? height : 0;
         im.onDevice = true;
@@ -207,6 +243,7 @@ public:
 
 private:
     stk_ctx* mCtx;
+    std::vector<void*> mMapped;
 };
 
 } // namespace stk

@@ -131,6 +131,21 @@ STK_API stk_status stk_generate_camera_rays_device(stk_ctx* ctx, const StkPathTr
  * otherwise the accumulated radiance); it may be a host or a device pointer
  * (outIsDevice).  Synchronous for host output, asynchronous otherwise. */
 STK_API stk_status stk_render(stk_ctx* ctx, const StkRenderParams* p, float* outRgba, int outIsDevice);
+/* stk_render into a caller-described image: a row pitch and RGBA32F or RGBA8
+ * pixels (the Image handed to drawFrame, src/render/ptrender.cpp:893-906, and
+ * Hydra render buffers, src/hdStrelka/RenderBuffer.cpp:78-81).  RGBA8 stores
+ * round(clamp(v, 0, 1) * 255) per channel (NaN stores 0). */
+#define STK_IMAGE_RGBA32F 0u
+#define STK_IMAGE_RGBA8 1u
+typedef struct StkImageDesc {
+    void* data;          /* first byte of row 0 */
+    uint32_t width, height;
+    uint32_t pitchBytes; /* bytes from one row to the next; 0 = width * pixel size */
+    uint32_t format;     /* STK_IMAGE_RGBA32F or STK_IMAGE_RGBA8 */
+    int32_t onDevice;    /* data is a HIP device (or mapped host) pointer */
+    uint32_t pad;
+} StkImageDesc;
+STK_API stk_status stk_render_image(stk_ctx* ctx, const StkRenderParams* p, const StkImageDesc* out);
 STK_API stk_status stk_reset_accumulation(stk_ctx* ctx);
 /* Per-sample radiance of the last stk_render call, sampleBuffer layout
  * [(pixel*spp + s)*3 + c] (shaders/pathtracerMdl.hlsl:420-422). */
@@ -274,6 +289,11 @@ STK_API stk_status stk_cache_set_params(stk_ctx* ctx, const float* grid, const f
 
 /* ---- device memory and timing helpers for callers without a HIP binding --- */
 STK_API stk_status stk_device_alloc(stk_ctx* ctx, size_t bytes, void** out);
+/* Pinned host memory the device can read and write directly (the host-visible
+ * buffers of ResourceManager::createBuffer + getMappedMemory,
+ * include/resourcemanager/resourcemanager.h:31-66). */
+STK_API stk_status stk_host_alloc(stk_ctx* ctx, size_t bytes, void** out);
+STK_API stk_status stk_host_free(stk_ctx* ctx, void* ptr);
 STK_API stk_status stk_device_free(stk_ctx* ctx, void* ptr);
 STK_API stk_status stk_copy_to_device(stk_ctx* ctx, void* dst, const void* src, size_t bytes);
 STK_API stk_status stk_copy_to_host(stk_ctx* ctx, void* dst, const void* src, size_t bytes);

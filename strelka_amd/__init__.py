@@ -16,7 +16,7 @@ import numpy as np
 from . import _abi
 from ._abi import (BVH_NODE, CACHE_RECORD, HIT, INSTANCE, INVALID, LIGHT, MAT_GGX, MAT_LAMBERT, MATERIAL, PACKED_NODE,
                    RAY, TRIANGLE, WIDE_NODE,
-                   VERTEX, BatchScene, BatchViewInfo, BvhHeader, BvhOptions, CacheConfig, CacheInfo, DeviceInfo, LaunchConfig,
+                   VERTEX, BatchScene, BatchViewInfo, BvhHeader, BvhOptions, CacheConfig, CacheInfo, DeviceInfo, ImageDesc, LaunchConfig,
                    PathTracerParam, RenderParams, SceneArrays)
 
 __all__ = ["lib", "StkError", "SceneData", "HostBvh", "Context", "RenderParams", "PathTracerParam", "VERTEX",
@@ -281,6 +281,26 @@ class Context:
             out = np.zeros((h, w, 4), dtype=np.float32)
         self._check(_LIB.stk_render(self._h, C.byref(p), _ptr(out), 0), "stk_render")
         return out
+
+    def render_image(self, p: RenderParams, rgba8: bool = False, pitch: int = 0) -> np.ndarray:
+        """stk_render_image into a host array: RGBA32F or RGBA8, rows `pitch`
+        bytes apart (0 = packed); returns the (h, w, 4) pixel view."""
+        w, h = p.pt.dimension[0], p.pt.dimension[1]
+        px = 4 if rgba8 else 16
+        row = pitch or w * px
+        buf = np.zeros(h * row, np.uint8)
+        d = ImageDesc(buf.ctypes.data, w, h, pitch, 1 if rgba8 else 0, 0)
+        self._check(_LIB.stk_render_image(self._h, C.byref(p), C.byref(d)), "stk_render_image")
+        rows = buf.reshape(h, row)[:, :w * px]
+        return (rows.copy().view(np.uint8) if rgba8 else rows.copy().view(np.float32)).reshape(h, w, 4)
+
+    def host_alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self._check(_LIB.stk_host_alloc(self._h, nbytes, C.byref(p)), "stk_host_alloc")
+        return p.value
+
+    def host_free(self, ptr: int):
+        self._check(_LIB.stk_host_free(self._h, C.c_void_p(ptr)), "stk_host_free")
 
     def render_device(self, p: RenderParams, d_out: int):
         self._check(_LIB.stk_render(self._h, C.byref(p), C.c_void_p(d_out), 1), "stk_render")

@@ -106,6 +106,11 @@ class BatchViewInfo(C.Structure):
                 ("pad", C.c_uint32 * 2)]
 
 
+class ImageDesc(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("pitchBytes", C.c_uint32),
+                ("format", C.c_uint32), ("onDevice", C.c_int32), ("pad", C.c_uint32)]
+
+
 class SceneArrays(C.Structure):
     _fields_ = [("vb", C.c_void_p), ("numVertices", C.c_uint32), ("ib", C.c_void_p), ("numIndices", C.c_uint32),
                 ("inst", C.c_void_p), ("numInstances", C.c_uint32), ("lights", C.c_void_p),
@@ -147,6 +152,9 @@ PROTOTYPES = [
     ("stk_get_samples", C.c_int, [P, P, C.c_size_t]),
     ("stk_get_accumulation", C.c_int, [P, P, C.c_size_t]),
     ("stk_get_counters", C.c_int, [P, P]),
+    ("stk_render_image", C.c_int, [P, C.POINTER(RenderParams), C.POINTER(ImageDesc)]),
+    ("stk_host_alloc", C.c_int, [P, C.c_size_t, C.POINTER(P)]),
+    ("stk_host_free", C.c_int, [P, P]),
     ("stk_get_ray_totals", C.c_int, [P, P]),
     ("stk_set_launch_config", C.c_int, [P, C.POINTER(LaunchConfig)]),
     ("stk_get_launch_config", C.c_int, [P, C.POINTER(LaunchConfig)]),

@@ -71,7 +71,7 @@ struct stk_ctx {
     // render state
     stk::PathState ps;
     std::vector<DevBuf> psBufs;
-    DevBuf accum, history, out, samples, descs;
+    DevBuf accum, history, out, samples, descs, staging;
     uint32_t iteration = 0;
     uint64_t imageKey = 0;
     size_t imagePixels = 0, samplesCount = 0;
@@ -100,7 +100,7 @@ struct stk_ctx {
     {
         return {&vb, &ib, &inst, &lights, &mats, &packed, &tris, &refNodes, &wideNodes, &wideTris, &emitters,
                 &rayScratch, &hitScratch,
-                &accum, &history, &out, &samples, &descs, &fVb, &fIb, &fInst, &fLights, &fMats, &fPacked,
+                &accum, &history, &out, &samples, &descs, &staging, &fVb, &fIb, &fInst, &fLights, &fMats, &fPacked,
                 &fTris, &fRefNodes, &fWide, &fWideTris, &fEmit, &cGridH, &cGridP, &cGridM, &cGridV, &cGridG, &cMlpP,
                 &cMlpM, &cMlpV, &cMlpG, &cFwd, &cBwd, &cXT, &cDyT, &cLoss, &cGridE, &cGridEW, &cGridHE, &cMlpE,
                 &cFwdE, &cBwdE, &queryRecs, &queryOut, &trainVerts, &trainCount, &trainBatch};
@@ -1007,6 +1007,29 @@ STK_API stk_status stk_get_launch_config(stk_ctx* c, StkLaunchConfig* lc)
     return STK_OK;
 }
 
+STK_API stk_status stk_render_image(stk_ctx* c, const StkRenderParams* p, const StkImageDesc* img)
+{
+    if (!validCtx(c) || !p || !img || !img->data) return STK_E_INVALID;
+    if (img->format != STK_IMAGE_RGBA32F && img->format != STK_IMAGE_RGBA8)
+        return fail(c, STK_E_INVALID, "stk_render_image: unknown format");
+    if ((int64_t)img->width != p->pt.dimension[0] || (int64_t)img->height != p->pt.dimension[1])
+        return fail(c, STK_E_INVALID, "stk_render_image: image size differs from the camera's dimension");
+    const size_t px = img->format == STK_IMAGE_RGBA8 ? 4 : 16;
+    const size_t row = (size_t)img->width * px;
+    const size_t pitch = img->pitchBytes ? img->pitchBytes : row;
+    if (pitch < row) return fail(c, STK_E_INVALID, "stk_render_image: pitch smaller than a row");
+    stk_status st = stk_render(c, p, nullptr, 0);
+    if (st != STK_OK) return st;
+    const size_t npix = (size_t)img->width * img->height;
+    if ((st = devReserve(c, c->staging, npix * px)) != STK_OK) return st;
+    STK_HIP(c, stk::launchPackImage(static_cast<const float4*>(c->out.ptr), (uint32_t)npix, img->format,
+                                    c->staging.ptr, c->stream));
+    STK_HIP(c, hipMemcpy2DAsync(img->data, pitch, c->staging.ptr, row, row, img->height,
+                                img->onDevice ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+    if (!img->onDevice) STK_HIP(c, hipStreamSynchronize(c->stream));
+    return STK_OK;
+}
+
 STK_API stk_status stk_reset_accumulation(stk_ctx* c)
 {
     if (!validCtx(c)) return STK_E_INVALID;
@@ -1297,6 +1320,23 @@ STK_API stk_status stk_device_alloc(stk_ctx* c, size_t bytes, void** out)
     if (failAllocForTest()) return fail(c, STK_E_OOM, "device allocation failed (test hook)");
     (void)hipSetDevice(c->device);
     STK_HIP(c, hipMalloc(out, bytes));
+    return STK_OK;
+}
+
+STK_API stk_status stk_host_alloc(stk_ctx* c, size_t bytes, void** out)
+{
+    if (!validCtx(c) || !out) return STK_E_INVALID;
+    *out = nullptr;
+    if (bytes == 0) return STK_OK;
+    (void)hipSetDevice(c->device);
+    STK_HIP(c, hipHostMalloc(out, bytes, hipHostMallocMapped | hipHostMallocPortable));
+    return STK_OK;
+}
+
+STK_API stk_status stk_host_free(stk_ctx* c, void* p)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    if (p) STK_HIP(c, hipHostFree(p));
     return STK_OK;
 }
 

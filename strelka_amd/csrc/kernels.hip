@@ -171,9 +171,34 @@ __global__ __launch_bounds__(kBlock) void k_accumulate_tonemap(const float4* __r
     out[i] = make_float4(o.x, o.y, o.z, 1.0f);
 }
 
+// RGBA32F pixels to a packed image of `format` (0 RGBA32F, 1 RGBA8).
+__global__ __launch_bounds__(kBlock) void k_pack_image(const float4* __restrict__ src, uint32_t npix, uint32_t format,
+                                                       void* __restrict__ dst)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= npix) return;
+    const float4 v = src[i];
+    if (format == 0) {
+        reinterpret_cast<float4*>(dst)[i] = v;
+        return;
+    }
+    auto q = [](float x) -> uint32_t {
+        const float c = fminf(fmaxf(x, 0.0f), 1.0f); // minNum: NaN -> 0
+        return (uint32_t)(c * 255.0f + 0.5f);
+    };
+    reinterpret_cast<uint32_t*>(dst)[i] = q(v.x) | (q(v.y) << 8) | (q(v.z) << 16) | (q(v.w) << 24);
+}
+
 inline uint32_t blocks(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 
 } // namespace
+
+hipError_t launchPackImage(const float4* src, uint32_t npix, uint32_t format, void* dst, hipStream_t s)
+{
+    if (npix == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_image, dim3(blocks(npix)), dim3(kBlock), 0, s, src, npix, format, dst);
+    return hipGetLastError();
+}
 
 hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags,
                               hipStream_t s)

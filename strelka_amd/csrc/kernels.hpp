@@ -148,6 +148,9 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
                                float4* accum, float* samples, CacheBuffers* cb, hipStream_t s);
 // Plain 16-B-per-lane copy kernel (bytes must be a multiple of 16).
 hipError_t launchCopy(void* dst, const void* src, size_t bytes, hipStream_t s);
+// RGBA32F pixels to a tightly packed RGBA32F (format 0) or RGBA8 (format 1)
+// image; RGBA8 stores round(clamp(v, 0, 1) * 255).
+hipError_t launchPackImage(const float4* src, uint32_t npix, uint32_t format, void* dst, hipStream_t s);
 // Zeroes n float4 (a kernel, so the call can be captured in a graph).
 hipError_t launchZero(float4* p, size_t n, hipStream_t s);
 // Running-mean accumulation (shaders/accumulation.hlsl:86-97) + tonemap

@@ -49,6 +49,24 @@ int main(int argc, char** argv)
             std::printf("drawFrame: %s\n", render.lastError().c_str());
             return 1;
         }
+    // an RGBA8 frame with a padded row pitch into mapped host memory
+    stk::ResourceManager rm(render.context());
+    const uint32_t pitch = img.width * 4 + 64;
+    void* mapped = rm.createBuffer((size_t)pitch * img.height, stk::ResourceManager::MemoryUsage::eCpuToGpu);
+    if (!mapped || rm.getMappedMemory(mapped) != mapped) return 1;
+    stk::Image img8;
+    img8.data = mapped;
+    img8.width = img.width;
+    img8.height = img.height;
+    img8.format = STK_IMAGE_RGBA8;
+    img8.pitchBytes = pitch;
+    if (render.drawFrame(img8) != STK_OK) {
+        std::printf("drawFrame rgba8: %s\n", render.lastError().c_str());
+        return 1;
+    }
+    const unsigned char* px = static_cast<const unsigned char*>(mapped);
+    if (px[3] != 255 || px[(size_t)pitch * (img.height - 1) + 3] != 255) return 1; // alpha of rows 0 and H-1
+    rm.destroyBuffer(mapped);
     stk_scene_free(gen);
     FILE* out = argc > 1 ? std::fopen(argv[1], "wb") : nullptr;
     if (!out) return 1;

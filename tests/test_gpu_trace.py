@@ -1,6 +1,8 @@
 """GPU traversal parity through the C ABI: hit IDs, t and barycentrics must be
 bit-identical to the CPU oracle (reference order, shaders/raytracing.h:198-286)
 on the same BVH and rays."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -176,3 +178,32 @@ def test_errors_are_reported_not_thrown(ctx):
                                  sky=s.sky))
     assert e.value.status == 1
     c.close()
+
+
+def test_render_image_pitch_and_rgba8():
+    """stk_render_image: RGBA32F with a padded pitch equals stk_render's
+    frame; RGBA8 stores round(clamp(v, 0, 1) * 255); mapped host memory works
+    as a device-visible output."""
+    print("case: stk_render_image pitch/RGBA8, config 6", flush=True)
+    s = sa.SceneData.generate(6)
+    ctx = sa.Context(0)
+    ctx.set_scene(s)
+    ctx.build_bvh()
+    w, h = s.camera.dimension[0], s.camera.dimension[1]
+
+    def params():
+        return sa.render_params(s.camera, spp=2, max_depth=8, frame=4, sampling=1, sky=s.sky, accumulate=False)
+
+    ref = ctx.render(params()).copy()
+    f32 = ctx.render_image(params(), pitch=w * 16 + 48)
+    assert np.array_equal(f32, ref)
+    u8 = ctx.render_image(params(), rgba8=True, pitch=w * 4 + 12)
+    expect = (np.clip(np.nan_to_num(ref, nan=0.0), 0, 1) * np.float32(255) + np.float32(0.5)).astype(np.uint8)
+    assert np.array_equal(u8, expect)
+    hp = ctx.host_alloc(w * h * 16)
+    ctx.render_device(params(), hp)
+    ctx.sync()
+    got = np.ctypeslib.as_array((ctypes.c_float * (w * h * 4)).from_address(hp)).reshape(h, w, 4).copy()
+    ctx.host_free(hp)
+    ctx.close()
+    assert np.array_equal(got, ref)
