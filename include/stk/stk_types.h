@@ -205,9 +205,15 @@ typedef struct StkRenderParams {
      * only and trace the training paths only (warm-up; the image holds only
      * those paths' samples) */
     uint32_t cacheMode;
-    uint32_t cacheQueryDepth; /* vertex depth at which paths query the cache */
+    /* Paths other than the training paths query the cache at the first
+     * surface vertex of depth >= cacheQueryDepth (0 = the camera ray's hit)
+     * whose roughness is >= cacheMinRoughness (Lambert counts as 1): the
+     * vertex's direct light is sampled as usual, and the radiance scattered
+     * from the BSDF-sampled direction on comes from the cache. */
+    uint32_t cacheQueryDepth;
     uint32_t cacheTrainPaths; /* training paths per frame (extended to maxDepth) */
-    uint32_t pad[3];
+    float cacheMinRoughness;  /* in [0, 1]; 0 queries at cacheQueryDepth */
+    uint32_t pad[2];
 } StkRenderParams;
 
 #ifdef __cplusplus

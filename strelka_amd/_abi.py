@@ -52,7 +52,8 @@ class RenderParams(C.Structure):
     _fields_ = [("pt", PathTracerParam), ("rrStartDepth", C.c_uint32), ("refCompat", C.c_uint32),
                 ("tonemapperType", C.c_uint32), ("enableAccumulation", C.c_uint32), ("skyColor", C.c_float * 4),
                 ("enableTonemap", C.c_uint32), ("keepSamples", C.c_uint32), ("cacheMode", C.c_uint32),
-                ("cacheQueryDepth", C.c_uint32), ("cacheTrainPaths", C.c_uint32), ("pad", C.c_uint32 * 3)]
+                ("cacheQueryDepth", C.c_uint32), ("cacheTrainPaths", C.c_uint32),
+                ("cacheMinRoughness", C.c_float), ("pad", C.c_uint32 * 2)]
 
 
 class CacheConfig(C.Structure):

@@ -555,13 +555,16 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
         if (p->cacheMode > 4) return fail(c, STK_E_INVALID, "render: cacheMode must be 0..4");
         pc.mode = p->cacheMode;
         pc.queryDepth = p->cacheQueryDepth;
+        if (!(p->cacheMinRoughness >= 0.0f && p->cacheMinRoughness <= 1.0f))
+            return fail(c, STK_E_INVALID, "render: cacheMinRoughness must be in [0, 1]");
+        pc.queryMinRough = p->cacheMinRoughness;
         const uint32_t paths = p->cacheTrainPaths ? p->cacheTrainPaths : 65536u;
         pc.trainStride = std::max<uint32_t>(1u, (uint32_t)(slots / paths));
         pc.maxTrainPaths = (uint32_t)((slots + pc.trainStride - 1) / pc.trainStride);
         pc.trainCap = (uint32_t)std::min<size_t>(c->cache.nCap, (size_t)pc.maxTrainPaths * p->pt.maxDepth);
         if ((st = devReserve(c, c->queryRecs, slots * sizeof(stk::CacheRecord))) != STK_OK) return st;
         if ((st = devReserve(c, c->queryOut, slots * sizeof(float4))) != STK_OK) return st;
-        if ((st = devReserve(c, c->trainVerts, (size_t)pc.maxTrainPaths * p->pt.maxDepth * 5 * sizeof(float4))) !=
+        if ((st = devReserve(c, c->trainVerts, (size_t)pc.maxTrainPaths * p->pt.maxDepth * 6 * sizeof(float4))) !=
             STK_OK)
             return st;
         if ((st = devReserve(c, c->trainCount, (size_t)pc.maxTrainPaths * sizeof(uint32_t))) != STK_OK) return st;

@@ -91,18 +91,21 @@ struct PathState {
 
 // Radiance-cache hooks of one pass.  Training paths are the slots with
 // slot % trainStride == trainOffset: they run to maxDepth and record every
-// surface vertex (5 float4: position+roughness, normal, outgoing direction,
-// radiance before the vertex, throughput at the vertex).  Other paths query
-// the cache at vertex depth queryDepth and end there.
+// surface vertex (6 float4: position+roughness, normal, outgoing direction,
+// radiance before the vertex, throughput at the vertex, radiance when the path
+// left the vertex).  Other paths query the cache at the first vertex of depth
+// >= queryDepth whose roughness is >= queryMinRough: they sample its direct
+// light as usual and end there, the cache supplying the rest.
 struct PtCache {
-    uint32_t mode = 0; // 0 off, 1 query + train, 2 query only, 3 train only
+    uint32_t mode = 0; // 0 off, 1 query + train, 2 query only, 3 train only, 4 warm-up
     uint32_t queryDepth = 2;
+    float queryMinRough = 0.0f;
     uint32_t trainStride = 1;
     uint32_t trainCap = 0;
     uint32_t maxTrainPaths = 0;
     CacheRecord* queryRecs = nullptr; // [slots]
     float4* queryOut = nullptr;       // [slots]
-    float4* trainVerts = nullptr;     // [maxTrainPaths * maxDepth * 5]
+    float4* trainVerts = nullptr;     // [maxTrainPaths * maxDepth * 6]
     uint32_t* trainCount = nullptr;   // [maxTrainPaths]
     CacheRecord* trainBatch = nullptr; // [trainCap]
 };

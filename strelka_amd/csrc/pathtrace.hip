@@ -30,6 +30,7 @@ using namespace dev;
 
 constexpr int kBlock = 256;
 constexpr float kMinAlbedo = 1e-3f; // floor of the albedo the cache's targets are divided by
+constexpr int kTrainWords = 6;      // float4s recorded per training vertex
 
 __device__ __forceinline__ float4 f4(F3 v) { return make_float4(v.x, v.y, v.z, 0.0f); }
 __device__ __forceinline__ F3 f3(float4 v) { return mk3(v.x, v.y, v.z); }
@@ -257,6 +258,14 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, 
         F3 thr = f3(ps.thr[slot]);
         F3 col = f3(ps.col[slot]);
         uint32_t rng = ps.rng[slot];
+        const bool isTrain = is_train(pc, *ps.pass, slot);
+        if (isTrain && depth > 0) {
+            // the radiance this path had gathered when it left vertex depth - 1
+            // (that vertex's direct light included): its training target is
+            // what the path gathers from here on
+            float4* prev = pc.trainVerts + ((size_t)(slot / pc.trainStride) * lp.maxDepth + depth - 1) * kTrainWords;
+            prev[5] = make_float4(col.x, col.y, col.z, 1.0f);
+        }
         if (inst == STK_INVALID_INDEX) {
             col = col + thr * mk3(desc.sky[0], desc.sky[1], desc.sky[2]);
             ps.col[slot] = f4(col);
@@ -294,32 +303,35 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, 
                 }
                 const F3 k1 = -f3(ps.rayD[slot]);
                 const float rough = mat.type == STK_MAT_GGX ? mat.roughness : 1.0f;
-                const bool isTrain = is_train(pc, *ps.pass, slot);
                 // the cache learns radiance divided by the vertex's albedo (a
                 // Lambert vertex's outgoing radiance is albedo times a smooth
                 // irradiance term), so albedo detail need not be learned
                 const F3 alb = mk3(fmaxf(mat.albedo[0], kMinAlbedo), fmaxf(mat.albedo[1], kMinAlbedo),
                                    fmaxf(mat.albedo[2], kMinAlbedo));
-                if ((pc.mode == 1 || pc.mode == 2) && !isTrain && depth == (int)pc.queryDepth) {
-                    // the path ends here: its remaining radiance comes from the cache
+                if ((pc.mode == 1 || pc.mode == 2) && !isTrain && depth >= (int)pc.queryDepth &&
+                    rough >= pc.queryMinRough) {
+                    // the path ends here after this vertex's direct light: the
+                    // radiance scattered from the BSDF-sampled direction on
+                    // comes from the cache
                     cacheQuery = true;
                     qrec.posRough = make_float4(P.x, P.y, P.z, rough);
                     qrec.normal = make_float4(N.x, N.y, N.z, bits_to_float(slot));
                     qrec.dir = make_float4(k1.x, k1.y, k1.z, 0.0f);
                     qrec.target = make_float4(alb.x, alb.y, alb.z, 0.0f); // read back by k_cache_apply
                 } else if (isTrain) {
-                    float4* v = pc.trainVerts + ((size_t)(slot / pc.trainStride) * lp.maxDepth + depth) * 5;
+                    float4* v = pc.trainVerts + ((size_t)(slot / pc.trainStride) * lp.maxDepth + depth) * kTrainWords;
                     v[0] = make_float4(P.x, P.y, P.z, rough);
                     v[1] = make_float4(N.x, N.y, N.z, alb.x);
                     v[2] = make_float4(k1.x, k1.y, k1.z, alb.y);
                     v[3] = make_float4(col.x, col.y, col.z, alb.z);
                     v[4] = f4(thr);
+                    v[5] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); // set when the path goes on
                     pc.trainCount[slot / pc.trainStride] = (uint32_t)depth + 1;
                 }
                 // sampleLights / estimateDirectLighting (shaders/pathtracerMdl.hlsl:31-92)
                 F3 toLight = mk3(0, 0, 0), radiance = mk3(0, 0, 0);
                 float lightPdf = 0.0f, dist = 0.0f;
-                if (desc.lightCount > 0 && !cacheQuery) {
+                if (desc.lightCount > 0) {
                     const uint32_t lid = (uint32_t)((float)desc.lightCount * rnd(rng));
                     const float selPdf = 1.0f / (float)desc.lightCount;
                     const StkLight& l =
@@ -475,9 +487,10 @@ __global__ __launch_bounds__(kBlock) void k_pt_finalize(PtLaunch lp, PathState p
     }
 }
 
-// Training targets: the radiance a training path gathered from vertex d on,
-// divided by the throughput at d - a one-sample estimate of the outgoing
-// radiance at d under the same estimator as the plain path tracer.
+// Training targets: the radiance a training path gathered after leaving vertex
+// d (everything but d's own direct-light sample), divided by the throughput at
+// d and by d's albedo - a one-sample estimate, under the plain path tracer's
+// estimator, of what a query at d stands in for.
 __global__ __launch_bounds__(kBlock) void k_cache_targets(PtLaunch lp, PathState ps)
 {
     const PtCache& pc = lp.cache;
@@ -489,8 +502,10 @@ __global__ __launch_bounds__(kBlock) void k_cache_targets(PtLaunch lp, PathState
     CacheRecord rec;
     if (tidx < pass.numTrainPaths && d < pc.trainCount[tidx]) {
         const uint32_t slot = tidx * pc.trainStride + pass.trainOffset;
-        const float4* v = pc.trainVerts + ((size_t)tidx * maxDepth + d) * 5;
-        const float4 before = v[3], thr = v[4], fin = ps.col[slot];
+        const float4* v = pc.trainVerts + ((size_t)tidx * maxDepth + d) * kTrainWords;
+        const float4 thr = v[4], fin = ps.col[slot];
+        // a path that ended at d gathered nothing after it
+        const float4 before = v[5].w != 0.0f ? v[5] : fin;
         const float ax = v[1].w, ay = v[2].w, az = v[3].w; // the vertex's albedo
         if (thr.x > 1e-8f && thr.y > 1e-8f && thr.z > 1e-8f) {
             // one-sample targets are clamped to [0, kTargetMax]: Russian-roulette
