@@ -102,6 +102,16 @@ STK_API stk_status stk_set_materials(stk_ctx* ctx, const StkMaterial* mats, uint
 /* Builds the BVH on the host (binned SAH, one triangle per leaf, preorder plus
  * skip links) and uploads it.  opts may be NULL for defaults. */
 STK_API stk_status stk_build_bvh(stk_ctx* ctx, const StkBvhOptions* opts);
+/* Builds a two-level structure instead of the flat BVH (no reference
+ * counterpart: the reference flattens every instance into one tree,
+ * src/render/ptrender.cpp:460-532): one tree per distinct mesh (instance index
+ * range) in object space and one over the instances' world boxes, for scenes
+ * of many instances of few meshes.  Renders and closest / any-hit traces then
+ * use it; hits are those of the reference walk (closestHit / anyHit) over the
+ * structure expanded into one skip-link tree, the hit test running on the
+ * world triangle getTriangle forms.  The reference-order and binary trace
+ * flags and stk_get_bvh need the flat BVH (STK_E_STATE). */
+STK_API stk_status stk_build_two_level(stk_ctx* ctx, const StkBvhOptions* opts);
 /* Uses a caller-built skip-link BVH (e.g. the reference builder's output). */
 STK_API stk_status stk_set_bvh(stk_ctx* ctx, const StkBvhNode* nodes, uint32_t numNodes);
 /* Copies the skip-link BVH of the context out (numNodes receives the count;
@@ -333,6 +343,37 @@ STK_API const StkWideNode* stk_bvh_wide_nodes(const stk_bvh* b);
 STK_API const StkTriangle* stk_bvh_wide_triangles(const stk_bvh* b);
 STK_API uint32_t stk_bvh_wide_depth(const stk_bvh* b);
 STK_API uint32_t stk_bvh_num_emitters(const stk_bvh* b);
+
+/* ---- two-level structure on the host (no device needed) ------------------------
+ * The same build stk_build_two_level runs, for tools and tests: a binary and a
+ * compressed 8-wide tree per distinct mesh (instance index range) in object
+ * space, and over the instances' world boxes.  No reference counterpart: the
+ * reference flattens instances (src/render/ptrender.cpp:460-532). */
+typedef struct stk_tlas stk_tlas;
+STK_API stk_status stk_tlas_build_host(const StkVertex* vb, uint32_t numVertices, const uint32_t* ib,
+                                       uint32_t numIndices, const StkInstance* inst, uint32_t numInstances,
+                                       const StkBvhOptions* opts, stk_tlas** out);
+STK_API void stk_tlas_free(stk_tlas* t);
+STK_API uint32_t stk_tlas_num_meshes(const stk_tlas* t);
+/* mesh of instance i, STK_INVALID_INDEX for an instance without triangles */
+STK_API uint32_t stk_tlas_instance_mesh(const stk_tlas* t, uint32_t i);
+/* a mesh's binary skip-link tree (object-space boxes; leaves: primId in
+ * minBounds[0], instId 0) */
+STK_API const StkBvhNode* stk_tlas_mesh_nodes(const stk_tlas* t, uint32_t mesh, uint32_t* count);
+/* the binary skip-link tree over the instances (leaves: instId = instance) */
+STK_API const StkBvhNode* stk_tlas_top_nodes(const stk_tlas* t, uint32_t* count);
+STK_API const StkWideNode* stk_tlas_top_wide(const stk_tlas* t, uint32_t* count);
+STK_API const StkTlasLeaf* stk_tlas_top_leaves(const stk_tlas* t, uint32_t* count);
+/* every mesh's wide tree in one array, and their leaf triangles */
+STK_API const StkWideNode* stk_tlas_mesh_wide(const stk_tlas* t, uint32_t* count);
+STK_API const StkBlasTriangle* stk_tlas_mesh_triangles(const stk_tlas* t, uint32_t* count);
+/* per instance: its leaf's preorder index in the top tree; 16 floats: the
+ * rows of the inverse transform the walk uses, then {pad, 0, 0, 0} */
+STK_API const uint32_t* stk_tlas_instance_keys(const stk_tlas* t);
+STK_API const float* stk_tlas_instance_inverses(const stk_tlas* t);
+/* world triangles of emitter instances (leaf = the mesh-tree leaf) */
+STK_API const StkTriangle* stk_tlas_emitters(const stk_tlas* t, uint32_t* count);
+STK_API void stk_tlas_depths(const stk_tlas* t, uint32_t* top, uint32_t* mesh);
 
 #ifdef __cplusplus
 }

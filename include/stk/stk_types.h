@@ -176,6 +176,31 @@ typedef struct StkWideNode {
     uint8_t qhi[3][8];
 } StkWideNode;
 
+/* Two-level (TLAS/BLAS) records, for scenes of many instances of few meshes.
+ * A BLAS is a compressed 8-wide tree (StkWideNode) over one mesh's triangles
+ * in object space; the TLAS is one over the instances' world boxes.  The hit
+ * test still runs on the world-space triangle formed exactly as getTriangle
+ * (shaders/raytracing.h:134-153) forms it, so hits are those of the
+ * reference walk over the same tree expanded into one skip-link array (each
+ * instance leaf of the TLAS replaced by its mesh's tree). */
+typedef struct StkBlasTriangle {
+    float p0[3];
+    uint32_t primId; /* triangle index within the mesh (= within each instance) */
+    float p1[3];
+    uint32_t leaf;   /* preorder index of its leaf in the mesh's binary tree */
+    float p2[3];
+    uint32_t pad;
+} StkBlasTriangle;
+
+typedef struct StkTlasLeaf {
+    float lo[3];       /* the instance's world box (holds every triangle as */
+    uint32_t instId;   /* getTriangle forms it)                             */
+    float hi[3];
+    uint32_t blasRoot; /* wide node index of the root of the instance's mesh tree */
+    uint32_t key;      /* preorder index of the instance's leaf in the TLAS binary tree */
+    uint32_t pad[3];
+} StkTlasLeaf;
+
 /* One radiance-cache sample, 64 B: the shading point, its shading normal, the
  * outgoing direction, the roughness and (for training) the radiance target.
  * payload is free for the caller (the renderer stores the path slot there). */
@@ -247,6 +272,8 @@ STK_STATIC_ASSERT(sizeof(StkPackedNode) == 64, "PackedNode must be 64 B");
 STK_STATIC_ASSERT(sizeof(StkWideNode) == 80, "WideNode must be 80 B");
 STK_STATIC_ASSERT(offsetof(StkWideNode, qlo) == 32, "WideNode.qlo offset");
 STK_STATIC_ASSERT(sizeof(StkCacheRecord) == 64, "CacheRecord must be 64 B");
+STK_STATIC_ASSERT(sizeof(StkBlasTriangle) == 48, "BlasTriangle must be 48 B");
+STK_STATIC_ASSERT(sizeof(StkTlasLeaf) == 48, "TlasLeaf must be 48 B");
 STK_STATIC_ASSERT(sizeof(StkRenderParams) == 336 + 64, "RenderParams must be 400 B");
 
 #endif /* STK_TYPES_H */

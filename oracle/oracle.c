@@ -508,6 +508,323 @@ int orc_closest_hit_wide(const OrcScene* s, const StkRay* r, StkHit* hit, OrcSta
     return found;
 }
 
+/* ---- two-level structure ------------------------------------------------------ */
+
+static void world_box(const float m[16], const float lo[3], const float hi[3], float olo[3], float ohi[3])
+{
+    for (int r = 0; r < 3; ++r) {
+        double c = m[12 + r], e = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double ck = 0.5 * ((double)lo[k] + hi[k]), ek = 0.5 * ((double)hi[k] - lo[k]);
+            c += (double)m[4 * k + r] * ck;
+            e += fabs((double)m[4 * k + r]) * ek;
+        }
+        const double l = c - e, h = c + e;
+        const double mag = fabs(l) > fabs(h) ? fabs(l) : fabs(h);
+        const double pad = 1e-5 * ((h - l) > mag ? (h - l) : mag) + 1e-30;
+        olo[r] = nextafterf((float)(l - pad), -INFINITY);
+        ohi[r] = nextafterf((float)(h + pad), INFINITY);
+    }
+}
+
+typedef struct ExpandCtx {
+    const StkBvhNode* top;
+    const StkBvhNode* meshNodes;
+    const uint32_t* meshStart;
+    const uint32_t* meshCount;
+    const uint32_t* instMesh;
+    const StkInstance* inst;
+    StkBvhNode* out;
+    uint32_t capacity;
+    int overflow;
+} ExpandCtx;
+
+static uint32_t expanded_size(const ExpandCtx* c, uint32_t ti)
+{
+    const StkBvhNode* nd = &c->top[ti];
+    if (is_leaf(nd)) return c->meshCount[c->instMesh[(uint32_t)nd->instId]];
+    const uint32_t left = ti + 1, right = (uint32_t)c->top[left].nodeOffset;
+    return 1 + expanded_size(c, left) + expanded_size(c, right);
+}
+
+static void expand_emit(ExpandCtx* c, uint32_t ti, uint32_t at, uint32_t skip)
+{
+    const StkBvhNode* nd = &c->top[ti];
+    if (is_leaf(nd)) {
+        const uint32_t i = (uint32_t)nd->instId, m = c->instMesh[i];
+        const StkBvhNode* mn = c->meshNodes + c->meshStart[m];
+        const uint32_t cnt = c->meshCount[m];
+        if ((uint64_t)at + cnt > c->capacity) {
+            c->overflow = 1;
+            return;
+        }
+        for (uint32_t k = 0; k < cnt; ++k) {
+            StkBvhNode o = mn[k];
+            if (is_leaf(&mn[k])) {
+                o.instId = (int32_t)i;
+            } else {
+                world_box(c->inst[i].objectToWorld, mn[k].minBounds, mn[k].maxBounds, o.minBounds, o.maxBounds);
+            }
+            o.nodeOffset = ((uint32_t)mn[k].nodeOffset == STK_INVALID_INDEX) ? (int32_t)skip
+                                                                            : (int32_t)(at + (uint32_t)mn[k].nodeOffset);
+            c->out[at + k] = o;
+        }
+        return;
+    }
+    if (at >= c->capacity) {
+        c->overflow = 1;
+        return;
+    }
+    c->out[at] = *nd;
+    c->out[at].nodeOffset = (int32_t)skip;
+    const uint32_t left = ti + 1, right = (uint32_t)c->top[left].nodeOffset;
+    const uint32_t sl = expanded_size(c, left);
+    expand_emit(c, left, at + 1, at + 1 + sl);
+    expand_emit(c, right, at + 1 + sl, skip);
+}
+
+uint32_t orc_expand_two_level(const StkBvhNode* top, uint32_t ntop, const StkBvhNode* meshNodes,
+                              const uint32_t* meshStart, const uint32_t* meshCount, const uint32_t* instMesh,
+                              const StkInstance* inst, StkBvhNode* out, uint32_t capacity)
+{
+    if (ntop == 0) return 0;
+    ExpandCtx c = {top, meshNodes, meshStart, meshCount, instMesh, inst, out, capacity, 0};
+    const uint32_t n = expanded_size(&c, 0);
+    if (n > capacity) return 0;
+    expand_emit(&c, 0, 0, STK_INVALID_INDEX);
+    return c.overflow ? 0 : n;
+}
+
+/* Slab tests of the 8 slots of a wide node (decoded as the kernel decodes them,
+ * widened by pad[a]); tch[s] gets the entry distance.  Returns the entered mask
+ * (t0 <= tmax and not beyond lim). */
+static uint32_t wide_slots(const StkWideNode* w, const StkRay* r, const float inv[3], float lim, const float pad[3],
+                           float tch[8])
+{
+    float sc[3];
+    uint32_t hits = 0;
+    for (int a = 0; a < 3; ++a) sc[a] = wide_scale(w->exps[a]);
+    for (int sl = 0; sl < 8; ++sl) {
+        float lo[3], hi[3], t0;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fmaf((float)w->qlo[a][sl], sc[a], w->origin[a]);
+            hi[a] = fmaf((float)w->qhi[a][sl], sc[a], w->origin[a]);
+            if (pad) {
+                lo[a] = lo[a] - pad[a];
+                hi[a] = hi[a] + pad[a];
+            }
+        }
+        tch[sl] = 0.0f;
+        if (orc_intersect_ray_box(r, inv, lo, hi, &t0) && t0 <= r->o[3] && !(t0 > lim)) {
+            hits |= 1u << sl;
+            tch[sl] = t0;
+        }
+    }
+    return hits & w->vmask & 0xFFu;
+}
+
+/* The kernel's object-space ray and box widening (trace_dev.hpp object_ray). */
+static void object_ray(const float* W, const StkRay* r, float T, StkRay* o, float inv[3], float pad[3])
+{
+    const float* w0 = W;
+    const float* w1 = W + 4;
+    const float* w2 = W + 8;
+    o->o[0] = (w0[0] * r->o[0] + w0[1] * r->o[1]) + (w0[2] * r->o[2] + w0[3]);
+    o->o[1] = (w1[0] * r->o[0] + w1[1] * r->o[1]) + (w1[2] * r->o[2] + w1[3]);
+    o->o[2] = (w2[0] * r->o[0] + w2[1] * r->o[1]) + (w2[2] * r->o[2] + w2[3]);
+    o->o[3] = r->o[3];
+    o->d[0] = (w0[0] * r->d[0] + w0[1] * r->d[1]) + w0[2] * r->d[2];
+    o->d[1] = (w1[0] * r->d[0] + w1[1] * r->d[1]) + w1[2] * r->d[2];
+    o->d[2] = (w2[0] * r->d[0] + w2[1] * r->d[1]) + w2[2] * r->d[2];
+    o->d[3] = 0.0f;
+    for (int a = 0; a < 3; ++a) inv[a] = 1.0f / o->d[a];
+    const float ax = fabsf(r->o[0]) + T * fabsf(r->d[0]), ay = fabsf(r->o[1]) + T * fabsf(r->d[1]),
+                az = fabsf(r->o[2]) + T * fabsf(r->d[2]);
+    const float k = 1.0f / 65536.0f;
+    for (int a = 0; a < 3; ++a) {
+        const float* w = W + 4 * a;
+        pad[a] = ((fabsf(w[0]) * ax + fabsf(w[1]) * ay) + (fabsf(w[2]) * az + fabsf(w[3]))) * k + W[12];
+    }
+}
+
+/* Exit distance from [lo, hi] (intersectRayBox's t1), bounded by tmax. */
+static float exit_bound(const StkRay* r, const float inv[3], const float lo[3], const float hi[3])
+{
+    float t1 = INFINITY;
+    int first = 1;
+    for (int a = 0; a < 3; ++a) {
+        const float f = (hi[a] - r->o[a]) * inv[a], n = (lo[a] - r->o[a]) * inv[a];
+        const float m = fmaxf(f, n);
+        t1 = first ? m : fminf(t1, m);
+        first = 0;
+    }
+    return fminf(t1 * 1.001f, r->o[3]);
+}
+
+static void blas_world_triangle(const float M[16], const StkBlasTriangle* t, float v0[3], float e0[3], float e1[3])
+{
+    const f3 a = xform4(M, ld(t->p0), 1.0f);
+    st3(v0, a);
+    st3(e0, sub(xform4(M, ld(t->p1), 1.0f), a));
+    st3(e1, sub(xform4(M, ld(t->p2), 1.0f), a));
+}
+
+static inline int key_less(uint32_t ikA, uint32_t leafA, uint32_t ikB, uint32_t leafB)
+{
+    return ikA < ikB || (ikA == ikB && leafA < leafB);
+}
+
+/* One wide walk with the node-group stack of the kernel (stack shared with the
+ * caller from `base`); visit(node, ...) is done by the callers below. */
+typedef struct WideStack {
+    uint32_t first[ORC_WIDE_STACK];
+    uint32_t word[ORC_WIDE_STACK];
+    int sp;
+} WideStack;
+
+static int wide_next(WideStack* s, uint32_t imask, uint32_t childBase, uint32_t innerHits, const float tch[8],
+                     uint32_t oct, int base, uint32_t* node)
+{
+    if (innerHits) {
+        int sn = 0;
+        float tn = INFINITY;
+        for (int sl = 0; sl < 8; ++sl)
+            if ((innerHits >> sl & 1u) && tch[sl] < tn) {
+                tn = tch[sl];
+                sn = sl;
+            }
+        innerHits &= ~(1u << sn);
+        if (innerHits) {
+            if (s->sp >= ORC_WIDE_STACK) abort();
+            s->first[s->sp] = childBase;
+            s->word[s->sp] = imask | (xor_permute8(innerHits, oct) << 8);
+            ++s->sp;
+        }
+        *node = childBase + popc8(imask & ((1u << sn) - 1u));
+        return 1;
+    }
+    if (s->sp == base) return 0;
+    const uint32_t first = s->first[s->sp - 1], word = s->word[s->sp - 1];
+    const uint32_t im = word & 0xFFu;
+    uint32_t m = word >> 8;
+    const int k = __builtin_ctz(m);
+    m &= m - 1u;
+    if (m)
+        s->word[s->sp - 1] = im | (m << 8);
+    else
+        --s->sp;
+    *node = first + popc8(im & ((1u << ((uint32_t)k ^ oct)) - 1u));
+    return 1;
+}
+
+static uint32_t octant_of(const float d[3])
+{
+    return (signbit(d[0]) ? 1u : 0u) | (signbit(d[1]) ? 2u : 0u) | (signbit(d[2]) ? 4u : 0u);
+}
+
+/* mode 0: closest (prune by t); mode 1: first in expanded preorder (no t
+ * pruning; instances after the best key skipped). */
+static int two_level_walk(const OrcScene* s, const StkRay* r, StkHit* hit, OrcStats* st, int anyMode)
+{
+    miss_hit(hit);
+    if (!s->tlas || s->ntlas == 0) return 0;
+    float inv[3];
+    ray_invdir(r, inv);
+    const uint32_t oct = octant_of(r->d);
+    int found = 0;
+    float best = 1e9f;
+    uint32_t bestIk = STK_INVALID_INDEX, bestLeaf = STK_INVALID_INDEX;
+    uint64_t nvisit = 0, ntri = 0;
+    WideStack stack;
+    stack.sp = 0;
+    uint32_t node = 0;
+    for (;;) {
+        const StkWideNode* w = &s->tlas[node];
+        ++nvisit;
+        float tch[8];
+        const float lim = (found && !anyMode ? best : 1e9f) * ORC_PRUNE_SLACK;
+        const uint32_t hits = wide_slots(w, r, inv, lim, NULL, tch);
+        const uint32_t imask = w->imask, leafMask = (w->vmask & 0xFFu) & ~imask;
+        uint32_t lm = hits & leafMask;
+        while (lm) {
+            int sl;
+            if (anyMode) {
+                sl = __builtin_ctz(lm);
+            } else {
+                float tn = INFINITY;
+                sl = 0;
+                for (int k = 0; k < 8; ++k)
+                    if ((lm >> k & 1u) && tch[k] < tn) {
+                        tn = tch[k];
+                        sl = k;
+                    }
+            }
+            lm &= ~(1u << sl);
+            if (!anyMode && found && tch[sl] > best * ORC_PRUNE_SLACK) continue;
+            const StkTlasLeaf* L = &s->tlasLeaves[w->triBase + popc8(leafMask & ((1u << sl) - 1u))];
+            const uint32_t inst = L->instId, ik = L->key;
+            if (anyMode && found && ik > bestIk) continue;
+            StkRay orr;
+            float oinv[3], pad[3];
+            object_ray(s->instW + 16 * (size_t)inst, r, exit_bound(r, inv, L->lo, L->hi), &orr, oinv, pad);
+            const uint32_t ooct = octant_of(orr.d);
+            const float* M = s->inst[inst].objectToWorld;
+            const int base = stack.sp;
+            uint32_t bnode = L->blasRoot;
+            for (;;) {
+                const StkWideNode* bw = &s->blas[bnode];
+                ++nvisit;
+                float btch[8];
+                const float blim = (found && !anyMode ? best : 1e9f) * ORC_PRUNE_SLACK;
+                const uint32_t bh = wide_slots(bw, &orr, oinv, blim, pad, btch);
+                const uint32_t bim = bw->imask, bleafMask = (bw->vmask & 0xFFu) & ~bim;
+                for (uint32_t m = bh & bleafMask; m; m &= m - 1u) {
+                    const int ls = __builtin_ctz(m);
+                    const StkBlasTriangle* tr = &s->blasTris[bw->triBase + popc8(bleafMask & ((1u << ls) - 1u))];
+                    if (anyMode && found && !key_less(ik, tr->leaf, bestIk, bestLeaf)) continue;
+                    float v0[3], e0[3], e1[3], t, b[2];
+                    blas_world_triangle(M, tr, v0, e0, e1);
+                    ++ntri;
+                    if (!orc_ray_triangle(r->o, r->d, v0, e0, e1, &t, b) || !(t < r->o[3])) continue;
+                    const int acc = anyMode ? 1
+                                            : (found ? (t < best || (t == best && key_less(ik, tr->leaf, bestIk, bestLeaf)))
+                                                     : (t < 1e9f));
+                    if (acc) {
+                        found = 1;
+                        best = t;
+                        bestIk = ik;
+                        bestLeaf = tr->leaf;
+                        hit->t = t;
+                        hit->bary[0] = b[0];
+                        hit->bary[1] = b[1];
+                        hit->instId = inst;
+                        hit->primId = tr->primId;
+                    }
+                }
+                if (!wide_next(&stack, bim, bw->childBase, bh & bim, btch, ooct, base, &bnode)) break;
+            }
+        }
+        if (!wide_next(&stack, imask, w->childBase, hits & imask, tch, oct, 0, &node)) break;
+    }
+    if (st) {
+        st->inner += nvisit;
+        st->leaf += ntri;
+        st->rays += 1;
+        st->hits += (uint64_t)found;
+    }
+    return found;
+}
+
+int orc_closest_two_level(const OrcScene* s, const StkRay* r, StkHit* hit, OrcStats* st)
+{
+    return two_level_walk(s, r, hit, st, 0);
+}
+
+int orc_any_two_level(const OrcScene* s, const StkRay* r, StkHit* hit, OrcStats* st)
+{
+    return two_level_walk(s, r, hit, st, 1);
+}
+
 static void* batch_worker(void* p)
 {
     BatchArg* a = (BatchArg*)p;
@@ -525,6 +842,8 @@ static void* batch_worker(void* p)
             case ORC_MODE_PRUNED: orc_closest_hit_pruned(a->s, &a->rays[i], &a->hits[i], &a->stats); break;
             case ORC_MODE_BRUTE: orc_closest_hit_brute(a->s, &a->rays[i], &a->hits[i]); break;
             case ORC_MODE_WIDE: orc_closest_hit_wide(a->s, &a->rays[i], &a->hits[i], &a->stats); break;
+            case ORC_MODE_TWO_LEVEL: orc_closest_two_level(a->s, &a->rays[i], &a->hits[i], &a->stats); break;
+            case ORC_MODE_ANY_TWO_LEVEL: orc_any_two_level(a->s, &a->rays[i], &a->hits[i], &a->stats); break;
             default: orc_any_hit(a->s, &a->rays[i], &a->hits[i], &a->stats); break;
             }
         }

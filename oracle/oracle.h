@@ -58,6 +58,14 @@ typedef struct OrcScene {
     uint32_t nwide;
     const StkTriangle* wideTris;
     uint32_t nwideTris;
+    /* two-level structure (optional: for ORC_MODE_TWO_LEVEL / ORC_MODE_ANY_TWO_LEVEL) */
+    const StkWideNode* tlas;
+    uint32_t ntlas;
+    const StkTlasLeaf* tlasLeaves;
+    const StkWideNode* blas;
+    const StkBlasTriangle* blasTris;
+    const uint32_t* instKey;
+    const float* instW; /* 16 per instance: inverse rows (3 x 4), {pad, 0, 0, 0} */
 } OrcScene;
 
 /* Traversal counters: inner = inner nodes whose box was tested (reference
@@ -98,7 +106,30 @@ int orc_any_hit(const OrcScene* s, const StkRay* r, StkHit* hit, OrcStats* st);
  * and triangle tests (leaf). */
 int orc_closest_hit_wide(const OrcScene* s, const StkRay* r, StkHit* hit, OrcStats* st);
 
-enum { ORC_MODE_REF = 0, ORC_MODE_PRUNED = 1, ORC_MODE_BRUTE = 2, ORC_MODE_ANY = 3, ORC_MODE_WIDE = 4 };
+/* Expands a two-level structure into one skip-link tree (input of the
+ * reference walks): each instance leaf of the top tree `top` (leaf instId =
+ * instance) becomes the mesh tree of that instance's mesh (meshNodes from
+ * meshStart[m], meshCount[m] nodes; leaf primId in minBounds[0]) with leaves
+ * tagged by the instance and inner boxes mapped to world boxes that hold the
+ * image of the object box (double precision, widened by 1e-5 relative, rounded
+ * outwards).  Skip links follow the preorder of the expanded tree.  Returns the
+ * node count, 0 when it exceeds capacity. */
+uint32_t orc_expand_two_level(const StkBvhNode* top, uint32_t ntop, const StkBvhNode* meshNodes,
+                              const uint32_t* meshStart, const uint32_t* meshCount, const uint32_t* instMesh,
+                              const StkInstance* inst, StkBvhNode* out, uint32_t capacity);
+
+/* The GPU kernel's two-level walk (instance tree, then each entered
+ * instance's mesh tree on the object-space ray with its widened boxes, the hit
+ * test on the world triangle getTriangle forms), accepting by t, then the
+ * expanded preorder (instance leaf, then mesh-tree leaf).  Counts wide-node
+ * visits of both levels (inner) and triangle tests (leaf). */
+int orc_closest_two_level(const OrcScene* s, const StkRay* r, StkHit* hit, OrcStats* st);
+/* anyHit's rule on the expanded tree over the two-level walk: the hit triangle
+ * first in expanded preorder. */
+int orc_any_two_level(const OrcScene* s, const StkRay* r, StkHit* hit, OrcStats* st);
+
+enum { ORC_MODE_REF = 0, ORC_MODE_PRUNED = 1, ORC_MODE_BRUTE = 2, ORC_MODE_ANY = 3, ORC_MODE_WIDE = 4,
+       ORC_MODE_TWO_LEVEL = 5, ORC_MODE_ANY_TWO_LEVEL = 6 };
 /* Batched traversal over n rays using nthreads host threads (chunks of 4096
  * rays handed out dynamically).  stats may be NULL. */
 void orc_trace_batch(const OrcScene* s, const StkRay* rays, uint32_t n, StkHit* hits, int mode, int nthreads,

@@ -19,13 +19,16 @@ RAY = np.dtype([("o", "<f4", 4), ("d", "<f4", 4)])
 HIT = np.dtype([("bary", "<f4", 2), ("t", "<f4"), ("instId", "<u4"), ("primId", "<u4")])
 
 MODE_REF, MODE_PRUNED, MODE_BRUTE, MODE_ANY, MODE_WIDE = 0, 1, 2, 3, 4
+MODE_TWO_LEVEL, MODE_ANY_TWO_LEVEL = 5, 6
 
 
 class OrcScene(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("nnodes", C.c_uint32), ("inst", C.c_void_p), ("ninst", C.c_uint32),
                 ("vb", C.c_void_p), ("nv", C.c_uint32), ("ib", C.c_void_p), ("ni", C.c_uint32),
                 ("lights", C.c_void_p), ("nlights", C.c_uint32), ("mats", C.c_void_p), ("nmats", C.c_uint32),
-                ("wide", C.c_void_p), ("nwide", C.c_uint32), ("wideTris", C.c_void_p), ("nwideTris", C.c_uint32)]
+                ("wide", C.c_void_p), ("nwide", C.c_uint32), ("wideTris", C.c_void_p), ("nwideTris", C.c_uint32),
+                ("tlas", C.c_void_p), ("ntlas", C.c_uint32), ("tlasLeaves", C.c_void_p), ("blas", C.c_void_p),
+                ("blasTris", C.c_void_p), ("instKey", C.c_void_p), ("instW", C.c_void_p)]
 
 
 class OracleStats(C.Structure):
@@ -59,6 +62,7 @@ def load() -> C.CDLL:
             "orc_closest_hit_brute": (C.c_int, [P, P, P]),
             "orc_any_hit": (C.c_int, [P, P, P, P]),
             "orc_trace_batch": (None, [P, P, U32, P, C.c_int, C.c_int, P]),
+            "orc_expand_two_level": (U32, [P, U32, P, P, P, P, P, P, U32]),
             "orc_jenkins": (U32, [U32]),
             "orc_init_rng": (U32, [U32, U32, U32, U32]),
             "orc_xorshift": (U32, [C.POINTER(U32)]),
@@ -131,6 +135,27 @@ class Oracle:
     def from_scene(cls, scene, nodes, wide=None, wide_tris=None):
         return cls(scene.vb, scene.ib, scene.inst, nodes, scene.lights, scene.mats, wide, wide_tris)
 
+    @classmethod
+    def from_two_level(cls, scene, tl):
+        """The two-level structure `tl` (arrays: top_nodes, mesh_nodes,
+        instance_mesh, top_wide, top_leaves, mesh_wide, mesh_tris,
+        instance_keys, instance_inverses) expanded into one skip-link tree for
+        the reference walks, with the two-level arrays kept for
+        MODE_TWO_LEVEL / MODE_ANY_TWO_LEVEL."""
+        nodes = expand_two_level(tl, scene.inst)
+        o = cls(scene.vb, scene.ib, scene.inst, nodes, scene.lights, scene.mats)
+        o.tl_arrays = [np.ascontiguousarray(a) for a in (tl.top_wide, tl.top_leaves, tl.mesh_wide, tl.mesh_tris,
+                                                         tl.instance_keys, tl.instance_inverses)]
+        tw, tlv, mw, mt, ik, iw = o.tl_arrays
+        assert tw.dtype.itemsize == 80 and tlv.dtype.itemsize == 48 and mt.dtype.itemsize == 48
+        s = o.scene
+        s.tlas, s.ntlas = tw.ctypes.data, tw.size
+        s.tlasLeaves, s.blas, s.blasTris = tlv.ctypes.data, mw.ctypes.data, mt.ctypes.data
+        s.instKey = ik.astype(np.uint32).ctypes.data if ik.dtype != np.uint32 else ik.ctypes.data
+        s.instW = iw.ctypes.data
+        assert iw.dtype == np.float32 and ik.dtype == np.uint32
+        return o
+
     def trace(self, rays: np.ndarray, mode: int = MODE_REF, threads: int | None = None):
         rays = np.ascontiguousarray(rays, dtype=RAY)
         hits = np.zeros(rays.size, dtype=HIT)
@@ -155,3 +180,28 @@ class Oracle:
         out = np.zeros(3, dtype=np.float32)
         self.lib.orc_pt_sample(C.byref(self.scene), C.byref(params), px, py, s, fptr(out))
         return out
+
+
+def expand_two_level(tl, inst) -> np.ndarray:
+    """orc_expand_two_level on a two-level structure's binary trees."""
+    lib = load()
+    top = np.ascontiguousarray(tl.top_nodes, dtype=BVH_NODE)
+    counts = np.array([m.size for m in tl.mesh_nodes], dtype=np.uint32)
+    starts = np.zeros(counts.size, dtype=np.uint32)
+    if counts.size:
+        starts[1:] = np.cumsum(counts)[:-1]
+    mesh = np.ascontiguousarray(np.concatenate(tl.mesh_nodes) if counts.size else np.zeros(0, BVH_NODE),
+                                dtype=BVH_NODE)
+    inst_mesh = np.ascontiguousarray(tl.instance_mesh, dtype=np.uint32)
+    inst = np.ascontiguousarray(inst)
+    total = 0
+    for i in range(inst_mesh.size):
+        if inst_mesh[i] != 0xFFFFFFFF:
+            total += int(counts[inst_mesh[i]])
+    cap = total + top.size
+    out = np.zeros(cap, dtype=BVH_NODE)
+    n = lib.orc_expand_two_level(vptr(top), top.size, vptr(mesh), vptr(starts), vptr(counts), vptr(inst_mesh),
+                                 vptr(inst), vptr(out), cap)
+    assert n > 0, "expansion failed"
+    return out[:n]
+

@@ -14,12 +14,12 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _abi
-from ._abi import (BVH_NODE, CACHE_RECORD, HIT, INSTANCE, INVALID, LIGHT, MAT_GGX, MAT_LAMBERT, MATERIAL, PACKED_NODE,
-                   RAY, TRIANGLE, WIDE_NODE,
+from ._abi import (BLAS_TRIANGLE, BVH_NODE, CACHE_RECORD, HIT, INSTANCE, INVALID, LIGHT, MAT_GGX, MAT_LAMBERT,
+                   MATERIAL, PACKED_NODE, RAY, TLAS_LEAF, TRIANGLE, WIDE_NODE,
                    VERTEX, BatchScene, BatchViewInfo, BvhHeader, BvhOptions, CacheConfig, CacheInfo, DeviceInfo, ImageDesc, LaunchConfig,
                    PathTracerParam, RenderParams, SceneArrays)
 
-__all__ = ["lib", "StkError", "SceneData", "HostBvh", "Context", "RenderParams", "PathTracerParam", "VERTEX",
+__all__ = ["lib", "StkError", "SceneData", "HostBvh", "HostTwoLevel", "Context", "RenderParams", "PathTracerParam", "VERTEX",
            "BVH_NODE", "INSTANCE", "LIGHT", "RAY", "HIT", "MATERIAL", "TRIANGLE", "PACKED_NODE", "INVALID",
            "render_params", "CACHE_RECORD", "MAT_LAMBERT", "MAT_GGX", "WIDE_NODE"]
 
@@ -135,6 +135,54 @@ class HostBvh:
             self._h = None
 
 
+class HostTwoLevel:
+    """Two-level structure built on the host (no device needed): a binary and a
+    compressed 8-wide tree per distinct mesh in object space, and over the
+    instances' world boxes (the build stk_build_two_level runs)."""
+
+    def __init__(self, handle: C.c_void_p, num_instances: int):
+        self._h = handle
+        n = C.c_uint32()
+
+        def arr(fn, dtype, *args):
+            p = fn(handle, *args, C.byref(n))
+            return _copy(p, n.value, dtype)
+
+        self.num_meshes = int(_LIB.stk_tlas_num_meshes(handle))
+        self.instance_mesh = np.array([_LIB.stk_tlas_instance_mesh(handle, i) for i in range(num_instances)],
+                                      dtype=np.uint32)
+        self.mesh_nodes = [arr(_LIB.stk_tlas_mesh_nodes, BVH_NODE, m) for m in range(self.num_meshes)]
+        self.top_nodes = arr(_LIB.stk_tlas_top_nodes, BVH_NODE)
+        self.top_wide = arr(_LIB.stk_tlas_top_wide, WIDE_NODE)
+        self.top_leaves = arr(_LIB.stk_tlas_top_leaves, TLAS_LEAF)
+        self.mesh_wide = arr(_LIB.stk_tlas_mesh_wide, WIDE_NODE)
+        self.mesh_tris = arr(_LIB.stk_tlas_mesh_triangles, BLAS_TRIANGLE)
+        self.emitters = arr(_LIB.stk_tlas_emitters, TRIANGLE)
+        self.instance_keys = _copy(_LIB.stk_tlas_instance_keys(handle), num_instances, np.dtype(np.uint32)) \
+            if num_instances else np.zeros(0, np.uint32)
+        self.instance_inverses = _copy(_LIB.stk_tlas_instance_inverses(handle), 16 * num_instances,
+                                       np.dtype(np.float32)).reshape(num_instances, 16) \
+            if num_instances else np.zeros((0, 16), np.float32)
+        top, mesh = C.c_uint32(), C.c_uint32()
+        _LIB.stk_tlas_depths(handle, C.byref(top), C.byref(mesh))
+        self.top_depth, self.mesh_depth = int(top.value), int(mesh.value)
+
+    @classmethod
+    def build(cls, scene: SceneData, max_depth: int = 32, bins: int = 32) -> "HostTwoLevel":
+        opts = BvhOptions(max_depth, bins, 1.0, 1.0)
+        h = C.c_void_p()
+        st = _LIB.stk_tlas_build_host(_ptr(scene.vb), scene.vb.size, _ptr(scene.ib), scene.ib.size,
+                                      _ptr(scene.inst), scene.inst.size, C.byref(opts), C.byref(h))
+        if st:
+            raise StkError(st, "stk_tlas_build_host")
+        return cls(h, scene.inst.size)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _LIB.stk_tlas_free(self._h)
+            self._h = None
+
+
 def render_params(camera: PathTracerParam, spp: int = 1, max_depth: int = 8, frame: int = 0,
                   sampling: int = 1, sky=(0.0, 0.0, 0.0), tonemap: int = 1, accumulate: bool = True,
                   keep_samples: bool = False, rr_start: int = 3, enable_tonemap: bool = True,
@@ -205,6 +253,11 @@ class Context:
     def build_bvh(self, max_depth: int = 32, bins: int = 32):
         opts = BvhOptions(max_depth, bins, 1.0, 1.0)
         self._check(_LIB.stk_build_bvh(self._h, C.byref(opts)), "stk_build_bvh")
+
+    def build_two_level(self, max_depth: int = 32, bins: int = 32):
+        """stk_build_two_level: per-mesh trees under a tree of instances."""
+        opts = BvhOptions(max_depth, bins, 1.0, 1.0)
+        self._check(_LIB.stk_build_two_level(self._h, C.byref(opts)), "stk_build_two_level")
 
     def set_bvh(self, nodes: np.ndarray):
         nodes = np.ascontiguousarray(nodes, dtype=BVH_NODE)

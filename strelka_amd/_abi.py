@@ -31,9 +31,14 @@ PACKED_NODE = np.dtype([("x", "<f4", 4), ("y", "<f4", 4), ("z", "<f4", 4), ("chi
 WIDE_NODE = np.dtype([("origin", "<f4", 3), ("exps", "u1", 3), ("imask", "u1"), ("childBase", "<u4"),
                       ("triBase", "<u4"), ("vmask", "<u4"), ("pad", "<u4"), ("qlo", "u1", (3, 8)),
                       ("qhi", "u1", (3, 8))])
+BLAS_TRIANGLE = np.dtype([("p0", "<f4", 3), ("primId", "<u4"), ("p1", "<f4", 3), ("leaf", "<u4"), ("p2", "<f4", 3),
+                          ("pad", "<u4")])
+TLAS_LEAF = np.dtype([("lo", "<f4", 3), ("instId", "<u4"), ("hi", "<f4", 3), ("blasRoot", "<u4"), ("key", "<u4"),
+                      ("pad", "<u4", 3)])
 
 for _dt, _sz in ((VERTEX, 32), (BVH_NODE, 32), (INSTANCE, 208), (LIGHT, 112), (RAY, 32), (HIT, 20),
-                 (MATERIAL, 32), (TRIANGLE, 48), (PACKED_NODE, 64)):
+                 (MATERIAL, 32), (TRIANGLE, 48), (PACKED_NODE, 64), (WIDE_NODE, 80), (BLAS_TRIANGLE, 48),
+                 (TLAS_LEAF, 48)):
     assert _dt.itemsize == _sz, (_dt, _sz)
 
 INVALID = 0xFFFFFFFF
@@ -196,6 +201,21 @@ PROTOTYPES = [
     ("stk_bvh_wide_triangles", P, [P]),
     ("stk_bvh_wide_depth", U32, [P]),
     ("stk_bvh_num_emitters", U32, [P]),
+    ("stk_build_two_level", C.c_int, [P, C.POINTER(BvhOptions)]),
+    ("stk_tlas_build_host", C.c_int, [P, U32, P, U32, P, U32, C.POINTER(BvhOptions), C.POINTER(P)]),
+    ("stk_tlas_free", None, [P]),
+    ("stk_tlas_num_meshes", U32, [P]),
+    ("stk_tlas_instance_mesh", U32, [P, U32]),
+    ("stk_tlas_mesh_nodes", P, [P, U32, C.POINTER(C.c_uint32)]),
+    ("stk_tlas_top_nodes", P, [P, C.POINTER(C.c_uint32)]),
+    ("stk_tlas_top_wide", P, [P, C.POINTER(C.c_uint32)]),
+    ("stk_tlas_top_leaves", P, [P, C.POINTER(C.c_uint32)]),
+    ("stk_tlas_mesh_wide", P, [P, C.POINTER(C.c_uint32)]),
+    ("stk_tlas_mesh_triangles", P, [P, C.POINTER(C.c_uint32)]),
+    ("stk_tlas_instance_keys", P, [P]),
+    ("stk_tlas_instance_inverses", P, [P]),
+    ("stk_tlas_emitters", P, [P, C.POINTER(C.c_uint32)]),
+    ("stk_tlas_depths", None, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("stk_scene_generate", C.c_int, [U32, U32, U32, C.POINTER(P)]),
     ("stk_scene_free", None, [P]),
     ("stk_scene_get", C.c_int, [P, C.POINTER(SceneArrays)]),

@@ -11,7 +11,9 @@
 #include "bvh.hpp"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
+#include <map>
 #include <cmath>
 #include <thread>
 
@@ -472,22 +474,22 @@ uint8_t chooseExponent(float origin, float extent)
     return (uint8_t)std::min(254, std::max(1, e));
 }
 
-} // namespace
-
-bool buildWide(const StkInstance* inst, uint32_t ninst, Bvh& bvh, std::string& err)
+// Collapses a binary skip-link tree (preorder, hit link i + 1, miss link
+// nodeOffset) into compressed 8-wide nodes.  leafBox(i, lo, hi) gives the box
+// of leaf node i and emitLeaf(i) appends its leaf record; leaves are emitted
+// in wide-node slot order, node by node in breadth-first order, and triBase
+// counts them from 0.
+template <class LeafBox, class EmitLeaf>
+bool collapseWide(const std::vector<StkBvhNode>& nodes, LeafBox leafBox, EmitLeaf emitLeaf,
+                  std::vector<StkWideNode>& wide, uint32_t& wideDepth, std::string& err)
 {
-    bvh.wide.clear();
-    bvh.wideTris.clear();
-    bvh.emitters.clear();
-    bvh.wideDepth = 0;
-    for (const StkTriangle& t : bvh.tris)
-        if (t.instId < ninst && inst[t.instId].lightId != -1) bvh.emitters.push_back(t);
-    const uint32_t n = (uint32_t)bvh.nodes.size();
-    if (n == 0) return true;
+    wide.clear();
+    wideDepth = 0;
+    if (nodes.empty()) return true;
     auto childOf = [&](uint32_t idx) {
         WideChild c;
         c.node = idx;
-        const StkBvhNode& nd = bvh.nodes[idx];
+        const StkBvhNode& nd = nodes[idx];
         c.leaf = (uint32_t)nd.instId != STK_INVALID_INDEX;
         if (!c.leaf) {
             for (int a = 0; a < 3; ++a) {
@@ -496,17 +498,7 @@ bool buildWide(const StkInstance* inst, uint32_t ninst, Bvh& bvh, std::string& e
             }
             return c;
         }
-        // a leaf's box: its triangle's box (the vertices as the hit test forms
-        // them), widened by 1e-5 of the larger of its extent and magnitude
-        const StkTriangle& t = bvh.tris[bvh.leafToTri[idx]];
-        for (int a = 0; a < 3; ++a) {
-            const float v0 = t.v0[a], v1 = t.v0[a] + t.e0[a], v2 = t.v0[a] + t.e1[a];
-            const float lo = std::min(v0, std::min(v1, v2)), hi = std::max(v0, std::max(v1, v2));
-            const float mag = std::max(std::fabs(lo), std::fabs(hi));
-            const float pad = 1e-5f * std::max(hi - lo, mag) + 1e-30f;
-            c.lo[a] = lo - pad;
-            c.hi[a] = hi + pad;
-        }
+        leafBox(idx, c.lo, c.hi);
         return c;
     };
     auto area = [](const WideChild& c) {
@@ -514,22 +506,23 @@ bool buildWide(const StkInstance* inst, uint32_t ninst, Bvh& bvh, std::string& e
         return dx * dy + dy * dz + dz * dx;
     };
     struct Job {
-        uint32_t node; // skip-link inner node (or leaf for a single-triangle tree)
+        uint32_t node; // skip-link inner node (or leaf for a single-leaf tree)
         uint32_t wide; // wide node to fill
         uint32_t level;
     };
     std::vector<Job> queue;
-    bvh.wide.emplace_back();
+    uint32_t emitted = 0;
+    wide.emplace_back();
     queue.push_back({0, 0, 1});
     for (size_t qi = 0; qi < queue.size(); ++qi) {
         const Job job = queue[qi];
-        bvh.wideDepth = std::max(bvh.wideDepth, job.level);
+        wideDepth = std::max(wideDepth, job.level);
         std::vector<WideChild> kids;
-        if ((uint32_t)bvh.nodes[job.node].instId != STK_INVALID_INDEX) {
+        if ((uint32_t)nodes[job.node].instId != STK_INVALID_INDEX) {
             kids.push_back(childOf(job.node)); // the whole tree is one leaf
         } else {
             kids.push_back(childOf(job.node + 1));
-            kids.push_back(childOf((uint32_t)bvh.nodes[job.node + 1].nodeOffset));
+            kids.push_back(childOf((uint32_t)nodes[job.node + 1].nodeOffset));
         }
         // open the inner child of largest surface area until 8 children
         while (kids.size() < 8) {
@@ -543,7 +536,7 @@ bool buildWide(const StkInstance* inst, uint32_t ninst, Bvh& bvh, std::string& e
             if (best < 0) break;
             const uint32_t idx = kids[best].node;
             kids[best] = childOf(idx + 1);
-            kids.push_back(childOf((uint32_t)bvh.nodes[idx + 1].nodeOffset));
+            kids.push_back(childOf((uint32_t)nodes[idx + 1].nodeOffset));
         }
         // node box and slot assignment by the child's centre relative to the node's
         float lo[3], hi[3];
@@ -592,8 +585,8 @@ bool buildWide(const StkInstance* inst, uint32_t ninst, Bvh& bvh, std::string& e
             w.exps[a] = chooseExponent(lo[a], hi[a] - lo[a]);
             scale[a] = powerOfTwo(w.exps[a]);
         }
-        w.childBase = (uint32_t)bvh.wide.size();
-        w.triBase = (uint32_t)bvh.wideTris.size();
+        w.childBase = (uint32_t)wide.size();
+        w.triBase = emitted;
         uint32_t inner = 0;
         for (int sl = 0; sl < 8; ++sl) {
             if (kidAt[sl] < 0) continue;
@@ -616,22 +609,313 @@ bool buildWide(const StkInstance* inst, uint32_t ninst, Bvh& bvh, std::string& e
                 w.qhi[a][sl] = (uint8_t)qhi;
             }
             if (c.leaf) {
-                bvh.wideTris.push_back(bvh.tris[bvh.leafToTri[c.node]]);
+                emitLeaf(c.node);
+                ++emitted;
             } else {
                 w.imask |= (uint8_t)(1u << sl);
                 ++inner;
             }
         }
-        const uint32_t firstChild = (uint32_t)bvh.wide.size();
-        bvh.wide.resize(bvh.wide.size() + inner);
+        const uint32_t firstChild = (uint32_t)wide.size();
+        wide.resize(wide.size() + inner);
         uint32_t r = 0;
         for (int sl = 0; sl < 8; ++sl)
             if (kidAt[sl] >= 0 && !kids[kidAt[sl]].leaf) queue.push_back({kids[kidAt[sl]].node, firstChild + r++, job.level + 1});
-        bvh.wide[job.wide] = w;
+        wide[job.wide] = w;
     }
+    return true;
+}
+
+// A leaf's box: its triangle's box (the vertices as given), widened by 1e-5
+// of the larger of its extent and magnitude.
+void paddedTriangleBox(const float* v0, const float* v1, const float* v2, float* lo, float* hi)
+{
+    for (int a = 0; a < 3; ++a) {
+        const float l = std::min(v0[a], std::min(v1[a], v2[a])), h = std::max(v0[a], std::max(v1[a], v2[a]));
+        const float mag = std::max(std::fabs(l), std::fabs(h));
+        const float pad = 1e-5f * std::max(h - l, mag) + 1e-30f;
+        lo[a] = l - pad;
+        hi[a] = h + pad;
+    }
+}
+
+} // namespace
+
+bool buildWide(const StkInstance* inst, uint32_t ninst, Bvh& bvh, std::string& err)
+{
+    bvh.wide.clear();
+    bvh.wideTris.clear();
+    bvh.emitters.clear();
+    bvh.wideDepth = 0;
+    for (const StkTriangle& t : bvh.tris)
+        if (t.instId < ninst && inst[t.instId].lightId != -1) bvh.emitters.push_back(t);
+    // a leaf's box: its triangle's box, the vertices as the hit test forms them
+    auto leafBox = [&](uint32_t idx, float* lo, float* hi) {
+        const StkTriangle& t = bvh.tris[bvh.leafToTri[idx]];
+        float v1[3], v2[3];
+        for (int a = 0; a < 3; ++a) {
+            v1[a] = t.v0[a] + t.e0[a];
+            v2[a] = t.v0[a] + t.e1[a];
+        }
+        paddedTriangleBox(t.v0, v1, v2, lo, hi);
+    };
+    auto emitLeaf = [&](uint32_t idx) { bvh.wideTris.push_back(bvh.tris[bvh.leafToTri[idx]]); };
+    if (!collapseWide(bvh.nodes, leafBox, emitLeaf, bvh.wide, bvh.wideDepth, err)) return false;
     if (bvh.wideTris.size() != bvh.tris.size()) {
         err = "wide BVH: leaf count differs from the binary tree";
         return false;
+    }
+    return true;
+}
+
+// ---- two-level (TLAS / BLAS) ----------------------------------------------------
+
+namespace {
+
+// The world box of a box [lo, hi] under the affine map m (column-major 4x4),
+// computed in double around the centre and rounded outwards to float, then
+// widened by 1e-5 of the larger of its extent and magnitude per axis: it holds
+// every point getTriangle's float arithmetic forms from points of [lo, hi].
+void transformBox(const float* m, const float* lo, const float* hi, float* olo, float* ohi)
+{
+    double c[3], e[3];
+    for (int a = 0; a < 3; ++a) {
+        c[a] = 0.5 * ((double)lo[a] + hi[a]);
+        e[a] = 0.5 * ((double)hi[a] - lo[a]);
+    }
+    for (int r = 0; r < 3; ++r) {
+        double cr = m[12 + r], er = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            cr += (double)m[4 * k + r] * c[k];
+            er += std::fabs((double)m[4 * k + r]) * e[k];
+        }
+        const double l = cr - er, h = cr + er;
+        const double pad = 1e-5 * std::max(h - l, std::max(std::fabs(l), std::fabs(h))) + 1e-30;
+        olo[r] = std::nextafter((float)(l - pad), -INFINITY);
+        ohi[r] = std::nextafter((float)(h + pad), INFINITY);
+    }
+}
+
+} // namespace
+
+bool buildTwoLevel(const StkVertex* vb, uint32_t nv, const uint32_t* ib, uint32_t ni, const StkInstance* inst,
+                   uint32_t ninst, const StkBvhOptions& opts, TwoLevel& out, std::string& err)
+{
+    out = TwoLevel{};
+    out.instMesh.assign(ninst, STK_INVALID_INDEX);
+    out.instKey.assign(ninst, STK_INVALID_INDEX);
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> meshOf;
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const StkInstance& ic = inst[i];
+        if (ic.indexOffset < 0 || ic.indexCount < 0 || ic.indexCount % 3 != 0 ||
+            (uint64_t)ic.indexOffset + (uint64_t)ic.indexCount > ni) {
+            err = "instance " + std::to_string(i) + " has an index range outside the index buffer";
+            return false;
+        }
+        if (ic.indexCount == 0) continue;
+        const auto key = std::make_pair((uint32_t)ic.indexOffset, (uint32_t)ic.indexCount);
+        auto it = meshOf.find(key);
+        if (it == meshOf.end()) {
+            it = meshOf.emplace(key, (uint32_t)out.meshOffset.size()).first;
+            out.meshOffset.push_back(key.first);
+            out.meshCount.push_back(key.second);
+        }
+        out.instMesh[i] = it->second;
+    }
+    const uint32_t nmesh = (uint32_t)out.meshOffset.size();
+    if (nmesh == 0) return true;
+    for (uint32_t m = 0; m < nmesh; ++m)
+        for (uint32_t k = 0; k < out.meshCount[m]; ++k)
+            if (ib[out.meshOffset[m] + k] >= nv) {
+                err = "index buffer entry outside the vertex buffer";
+                return false;
+            }
+    auto pos = [&](uint32_t m, uint32_t prim, int k) { return vb[ib[out.meshOffset[m] + 3 * prim + k]].position; };
+
+    // one binary tree and one wide tree per mesh, in object space
+    out.blasNodes.resize(nmesh);
+    out.meshRoot.resize(nmesh);
+    std::vector<std::array<float, 6>> meshBox(nmesh);
+    for (uint32_t m = 0; m < nmesh; ++m) {
+        const uint32_t ntri = out.meshCount[m] / 3;
+        std::vector<BvhInputTri> tris(ntri);
+        std::array<float, 6> box = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t p = 0; p < ntri; ++p) {
+            for (int k = 0; k < 3; ++k) {
+                tris[p].v[k] = ld3(pos(m, p, k));
+                for (int a = 0; a < 3; ++a) {
+                    box[a] = std::min(box[a], pos(m, p, k)[a]);
+                    box[3 + a] = std::max(box[3 + a], pos(m, p, k)[a]);
+                }
+            }
+            tris[p].instId = 0;
+            tris[p].primId = p;
+        }
+        meshBox[m] = box;
+        Bvh b;
+        if (!buildBvh(tris, opts, b, err)) return false;
+        std::vector<StkWideNode> wide;
+        std::vector<StkBlasTriangle> leaves;
+        uint32_t depth = 0;
+        auto leafBox = [&](uint32_t idx, float* lo, float* hi) {
+            const uint32_t p = float_to_bits(b.nodes[idx].minBounds[0]);
+            paddedTriangleBox(pos(m, p, 0), pos(m, p, 1), pos(m, p, 2), lo, hi);
+        };
+        auto emitLeaf = [&](uint32_t idx) {
+            const uint32_t p = float_to_bits(b.nodes[idx].minBounds[0]);
+            StkBlasTriangle t;
+            memset(&t, 0, sizeof(t));
+            for (int a = 0; a < 3; ++a) {
+                t.p0[a] = pos(m, p, 0)[a];
+                t.p1[a] = pos(m, p, 1)[a];
+                t.p2[a] = pos(m, p, 2)[a];
+            }
+            t.primId = p;
+            t.leaf = idx;
+            leaves.push_back(t);
+        };
+        if (!collapseWide(b.nodes, leafBox, emitLeaf, wide, depth, err)) return false;
+        const uint32_t nodeBase = (uint32_t)out.blasWide.size(), triBase = (uint32_t)out.blasTris.size();
+        for (StkWideNode& w : wide) {
+            w.childBase += nodeBase;
+            w.triBase += triBase;
+        }
+        out.meshRoot[m] = nodeBase;
+        out.blasWide.insert(out.blasWide.end(), wide.begin(), wide.end());
+        out.blasTris.insert(out.blasTris.end(), leaves.begin(), leaves.end());
+        out.blasDepth = std::max(out.blasDepth, depth);
+        out.blasNodes[m] = std::move(b.nodes);
+    }
+
+    // each instance's inverse transform as the walk uses it (rows of the
+    // linear part and translation, rounded to float) and a bound on how far
+    // it moves the image of a mesh point from the point itself
+    out.instW.assign((size_t)ninst * 16, 0.0f);
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const uint32_t m = out.instMesh[i];
+        if (m == STK_INVALID_INDEX) continue;
+        const float* M = inst[i].objectToWorld;
+        double L[3][3], Li[3][3], t[3];
+        for (int r = 0; r < 3; ++r) {
+            for (int c = 0; c < 3; ++c) L[r][c] = M[4 * c + r];
+            t[r] = M[12 + r];
+        }
+        const double det = L[0][0] * (L[1][1] * L[2][2] - L[1][2] * L[2][1]) -
+                           L[0][1] * (L[1][0] * L[2][2] - L[1][2] * L[2][0]) +
+                           L[0][2] * (L[1][0] * L[2][1] - L[1][1] * L[2][0]);
+        if (!(std::fabs(det) > 1e-30) || !std::isfinite(det)) {
+            err = "instance " + std::to_string(i) + " has a singular objectToWorld";
+            return false;
+        }
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                const int r1 = (c + 1) % 3, r2 = (c + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+                Li[r][c] = (L[r1][c1] * L[r2][c2] - L[r1][c2] * L[r2][c1]) / det;
+            }
+        float Wf[3][4];
+        for (int r = 0; r < 3; ++r) {
+            double wt = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                Wf[r][c] = (float)Li[r][c];
+                wt -= Li[r][c] * t[c];
+            }
+            Wf[r][3] = (float)wt;
+        }
+        // D(p) = (Wf Lf - I) p + (Wf t + wf) for p in the mesh box
+        double worst = 0.0;
+        for (int r = 0; r < 3; ++r) {
+            double d = 0.0, b = (double)Wf[r][3];
+            for (int c = 0; c < 3; ++c) {
+                double a = -(r == c ? 1.0 : 0.0);
+                for (int k = 0; k < 3; ++k) a += (double)Wf[r][k] * L[k][c];
+                d += std::fabs(a) * std::max(std::fabs((double)meshBox[m][c]), std::fabs((double)meshBox[m][3 + c]));
+                b += (double)Wf[r][c] * t[c];
+            }
+            worst = std::max(worst, d + std::fabs(b));
+        }
+        float* w = &out.instW[(size_t)i * 16];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 4; ++c) w[4 * r + c] = Wf[r][c];
+        w[12] = std::nextafter((float)(2.0 * worst), INFINITY);
+    }
+
+    // the instances' world boxes, one binary tree and one wide tree over them
+    std::vector<std::array<float, 6>> instBox(ninst);
+    std::vector<BvhInputTri> prims;
+    for (int a = 0; a < 3; ++a) {
+        out.lo[a] = INFINITY;
+        out.hi[a] = -INFINITY;
+    }
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const uint32_t m = out.instMesh[i];
+        if (m == STK_INVALID_INDEX) continue;
+        float* b = instBox[i].data();
+        transformBox(inst[i].objectToWorld, meshBox[m].data(), meshBox[m].data() + 3, b, b + 3);
+        BvhInputTri t;
+        t.v[0] = mk3(b[0], b[1], b[2]);
+        t.v[1] = mk3(b[3], b[4], b[5]);
+        t.v[2] = t.v[0];
+        t.instId = i;
+        t.primId = 0;
+        prims.push_back(t);
+        for (int a = 0; a < 3; ++a) {
+            out.lo[a] = std::min(out.lo[a], b[a]);
+            out.hi[a] = std::max(out.hi[a], b[3 + a]);
+        }
+    }
+    Bvh top;
+    if (!buildBvh(prims, opts, top, err)) return false;
+    auto leafBox = [&](uint32_t idx, float* lo, float* hi) {
+        const float* b = instBox[(uint32_t)top.nodes[idx].instId].data();
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = b[a];
+            hi[a] = b[3 + a];
+        }
+    };
+    auto emitLeaf = [&](uint32_t idx) {
+        const uint32_t i = (uint32_t)top.nodes[idx].instId;
+        StkTlasLeaf l;
+        memset(&l, 0, sizeof(l));
+        for (int a = 0; a < 3; ++a) {
+            l.lo[a] = instBox[i][a];
+            l.hi[a] = instBox[i][3 + a];
+        }
+        l.instId = i;
+        l.blasRoot = out.meshRoot[out.instMesh[i]];
+        l.key = idx;
+        out.tlasLeaves.push_back(l);
+    };
+    if (!collapseWide(top.nodes, leafBox, emitLeaf, out.tlasWide, out.tlasDepth, err)) return false;
+    for (uint32_t idx = 0; idx < (uint32_t)top.nodes.size(); ++idx)
+        if ((uint32_t)top.nodes[idx].instId != STK_INVALID_INDEX) out.instKey[(uint32_t)top.nodes[idx].instId] = idx;
+    out.tlasNodes = std::move(top.nodes);
+
+    // emitter triangles in world space as getTriangle forms them (shaders/raytracing.h:134-153)
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const uint32_t m = out.instMesh[i];
+        if (m == STK_INVALID_INDEX || inst[i].lightId == -1) continue;
+        const std::vector<StkBvhNode>& nodes = out.blasNodes[m];
+        for (uint32_t idx = 0; idx < (uint32_t)nodes.size(); ++idx) {
+            if ((uint32_t)nodes[idx].instId == STK_INVALID_INDEX) continue;
+            const uint32_t p = float_to_bits(nodes[idx].minBounds[0]);
+            const F3 v0 = xform4(inst[i].objectToWorld, ld3(pos(m, p, 0)), 1.0f);
+            const F3 e0 = xform4(inst[i].objectToWorld, ld3(pos(m, p, 1)), 1.0f) - v0;
+            const F3 e1 = xform4(inst[i].objectToWorld, ld3(pos(m, p, 2)), 1.0f) - v0;
+            StkTriangle t;
+            t.v0[0] = v0.x;
+            t.v0[1] = v0.y;
+            t.v0[2] = v0.z;
+            t.e0[0] = e0.x;
+            t.e0[1] = e0.y;
+            t.e0[2] = e0.z;
+            t.e1[0] = e1.x;
+            t.e1[1] = e1.y;
+            t.e1[2] = e1.z;
+            t.leaf = idx;
+            t.instId = i;
+            t.primId = p;
+            out.emitters.push_back(t);
+        }
     }
     return true;
 }

@@ -1,6 +1,7 @@
 // bvh.hpp - host BVH builder and packer.
 #pragma once
 
+#include <cstdint>
 #include <string>
 #include <vector>
 
@@ -46,6 +47,31 @@ bool packBvh(const StkVertex* vb, uint32_t nv, const uint32_t* ib, uint32_t ni, 
 // Collapses the packed binary BVH into the compressed 8-wide layout and
 // collects the emitter triangles.  Needs packBvh's output.
 bool buildWide(const StkInstance* inst, uint32_t ninst, Bvh& bvh, std::string& err);
+
+// Two-level acceleration structure: one tree per distinct mesh (index range)
+// in object space, one over the instances' world boxes.
+struct TwoLevel {
+    std::vector<uint32_t> meshOffset, meshCount;      // index range of each mesh
+    std::vector<uint32_t> instMesh;                   // mesh of each instance (STK_INVALID_INDEX: no triangles)
+    std::vector<std::vector<StkBvhNode>> blasNodes;   // binary skip-link tree per mesh, object space
+    std::vector<StkWideNode> blasWide;                // every mesh's wide tree, indices rebased
+    std::vector<StkBlasTriangle> blasTris;            // leaves of blasWide, slot order
+    std::vector<uint32_t> meshRoot;                   // wide root of each mesh in blasWide
+    uint32_t blasDepth = 0;                           // deepest mesh tree (wide levels)
+    std::vector<StkBvhNode> tlasNodes;                // binary skip-link tree over instance boxes
+    std::vector<StkWideNode> tlasWide;
+    std::vector<StkTlasLeaf> tlasLeaves;              // leaves of tlasWide, slot order
+    uint32_t tlasDepth = 0;
+    std::vector<uint32_t> instKey;                    // TLAS preorder leaf index of each instance
+    std::vector<float> instW;                         // 16 per instance: inverse rows (3 x 4), {pad, 0, 0, 0}
+    std::vector<StkTriangle> emitters;                // world triangles of emitter instances (leaf = mesh-tree leaf)
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};       // scene box
+};
+
+// Builds the two-level structure with the binned-SAH builder per mesh and over
+// the instances.  Instances with an empty index range are left out.
+bool buildTwoLevel(const StkVertex* vb, uint32_t nv, const uint32_t* ib, uint32_t ni, const StkInstance* inst,
+                   uint32_t ninst, const StkBvhOptions& opts, TwoLevel& out, std::string& err);
 
 StkBvhOptions defaultBvhOptions();
 

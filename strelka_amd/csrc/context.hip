@@ -64,8 +64,12 @@ struct stk_ctx {
 
     stk::Bvh bvh;
     bool haveBvh = false;
+    // two-level structure (stk_build_two_level): replaces the flat tree
+    stk::TwoLevel tl;
+    bool twoLevel = false;
 
     DevBuf vb, ib, inst, lights, mats, packed, tris, refNodes, wideNodes, wideTris, emitters;
+    DevBuf tlasWide, tlasLeaves, blasWide, blasTris, instKey, instW, tlEmit;
     DevBuf rayScratch, hitScratch;
 
     // render state
@@ -99,7 +103,7 @@ struct stk_ctx {
     std::vector<DevBuf*> owned()
     {
         return {&vb, &ib, &inst, &lights, &mats, &packed, &tris, &refNodes, &wideNodes, &wideTris, &emitters,
-                &rayScratch, &hitScratch,
+                &tlasWide, &tlasLeaves, &blasWide, &blasTris, &instKey, &instW, &tlEmit, &rayScratch, &hitScratch,
                 &accum, &history, &out, &samples, &descs, &staging, &fVb, &fIb, &fInst, &fLights, &fMats, &fPacked,
                 &fTris, &fRefNodes, &fWide, &fWideTris, &fEmit, &cGridH, &cGridP, &cGridM, &cGridV, &cGridG, &cMlpP,
                 &cMlpM, &cMlpV, &cMlpG, &cFwd, &cBwd, &cXT, &cDyT, &cLoss, &cGridE, &cGridEW, &cGridHE, &cMlpE,
@@ -181,6 +185,19 @@ stk::BvhView bvhView(stk_ctx* c)
     v.emitters = static_cast<const float4*>(c->emitters.ptr);
     v.wideRoot = wideUsable(c->bvh) ? 0u : STK_INVALID_INDEX;
     v.emitCount = (uint32_t)c->bvh.emitters.size();
+    v.tlas = v.tlasLeaves = v.blas = v.blasTris = v.instW = nullptr;
+    v.instKey = nullptr;
+    if (c->twoLevel) {
+        v.tlas = static_cast<const float4*>(c->tlasWide.ptr);
+        v.tlasLeaves = static_cast<const float4*>(c->tlasLeaves.ptr);
+        v.blas = static_cast<const float4*>(c->blasWide.ptr);
+        v.blasTris = static_cast<const float4*>(c->blasTris.ptr);
+        v.instKey = static_cast<const uint32_t*>(c->instKey.ptr);
+        v.instW = static_cast<const float4*>(c->instW.ptr);
+        v.emitters = static_cast<const float4*>(c->tlEmit.ptr);
+        v.emitCount = (uint32_t)c->tl.emitters.size();
+        v.wideRoot = STK_INVALID_INDEX;
+    }
     return v;
 }
 
@@ -296,6 +313,7 @@ STK_API stk_status stk_set_geometry(stk_ctx* c, const StkVertex* vb, uint32_t nv
     if ((st = devUpload(c, c->inst, inst, (size_t)ninst * sizeof(StkInstance))) != STK_OK) return st;
     c->haveGeometry = true;
     c->haveBvh = false;
+    c->twoLevel = false;
     return STK_OK;
 }
 
@@ -342,7 +360,53 @@ STK_API stk_status stk_build_bvh(stk_ctx* c, const StkBvhOptions* opts)
     if (c->bvh.depth > (uint32_t)stk::dev::kStackDepth)
         return fail(c, STK_E_INVALID, "stk_build_bvh: tree deeper than the traversal stack (32)");
     (void)hipSetDevice(c->device);
+    c->twoLevel = false;
+    invalidateGraph(c);
     return uploadBvh(c);
+}
+
+STK_API stk_status stk_build_two_level(stk_ctx* c, const StkBvhOptions* opts)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    if (!c->haveGeometry) return fail(c, STK_E_STATE, "stk_build_two_level: no geometry set");
+    std::string err;
+    const StkBvhOptions o = opts ? *opts : stk::defaultBvhOptions();
+    try {
+        if (!stk::buildTwoLevel(c->hvb.data(), (uint32_t)c->hvb.size(), c->hib.data(), (uint32_t)c->hib.size(),
+                                c->hinst.data(), (uint32_t)c->hinst.size(), o, c->tl, err))
+            return fail(c, STK_E_INVALID, "stk_build_two_level: " + err);
+    } catch (const std::bad_alloc&) {
+        return fail(c, STK_E_OOM, "stk_build_two_level: host allocation failed");
+    }
+    const stk::TwoLevel& t = c->tl;
+    if (t.tlasWide.empty()) return fail(c, STK_E_INVALID, "stk_build_two_level: no instance has triangles");
+    if (t.tlasDepth + t.blasDepth > (uint32_t)stk::dev::kWideStack)
+        return fail(c, STK_E_INVALID, "stk_build_two_level: instance and mesh trees deeper than the walk's stack (16)");
+    (void)hipSetDevice(c->device);
+    stk_status st;
+    if ((st = devUpload(c, c->tlasWide, t.tlasWide.data(), t.tlasWide.size() * sizeof(StkWideNode))) != STK_OK)
+        return st;
+    if ((st = devUpload(c, c->tlasLeaves, t.tlasLeaves.data(), t.tlasLeaves.size() * sizeof(StkTlasLeaf))) != STK_OK)
+        return st;
+    if ((st = devUpload(c, c->blasWide, t.blasWide.data(), t.blasWide.size() * sizeof(StkWideNode))) != STK_OK)
+        return st;
+    if ((st = devUpload(c, c->blasTris, t.blasTris.data(), t.blasTris.size() * sizeof(StkBlasTriangle))) != STK_OK)
+        return st;
+    if ((st = devUpload(c, c->instKey, t.instKey.data(), t.instKey.size() * sizeof(uint32_t))) != STK_OK) return st;
+    if ((st = devUpload(c, c->instW, t.instW.data(), t.instW.size() * sizeof(float))) != STK_OK) return st;
+    if ((st = devUpload(c, c->tlEmit, t.emitters.data(), t.emitters.size() * sizeof(StkTriangle))) != STK_OK)
+        return st;
+    // the flat tree goes; its header keeps the scene box (the cache's domain)
+    c->bvh = stk::Bvh{};
+    c->bvh.header.root = INT32_MIN;
+    for (int a = 0; a < 3; ++a) {
+        c->bvh.header.lo[a] = t.lo[a];
+        c->bvh.header.hi[a] = t.hi[a];
+    }
+    c->twoLevel = true;
+    c->haveBvh = true;
+    invalidateGraph(c);
+    return STK_OK;
 }
 
 STK_API stk_status stk_set_bvh(stk_ctx* c, const StkBvhNode* nodes, uint32_t n)
@@ -364,6 +428,8 @@ STK_API stk_status stk_set_bvh(stk_ctx* c, const StkBvhNode* nodes, uint32_t n)
     if (c->bvh.depth > (uint32_t)stk::dev::kStackDepth)
         return fail(c, STK_E_INVALID, "stk_set_bvh: tree deeper than the traversal stack (32)");
     (void)hipSetDevice(c->device);
+    c->twoLevel = false;
+    invalidateGraph(c);
     return uploadBvh(c);
 }
 
@@ -371,10 +437,19 @@ STK_API stk_status stk_get_bvh(stk_ctx* c, StkBvhNode* nodes, uint32_t cap, uint
 {
     if (!validCtx(c) || !n) return STK_E_INVALID;
     if (!c->haveBvh) return fail(c, STK_E_STATE, "stk_get_bvh: no BVH");
+    if (c->twoLevel) return fail(c, STK_E_STATE, "stk_get_bvh: the context holds a two-level structure");
     *n = (uint32_t)c->bvh.nodes.size();
     if (!nodes) return STK_OK;
     if (cap < *n) return fail(c, STK_E_INVALID, "stk_get_bvh: capacity too small");
     memcpy(nodes, c->bvh.nodes.data(), c->bvh.nodes.size() * sizeof(StkBvhNode));
+    return STK_OK;
+}
+
+// The reference-order and binary walks need the flat tree.
+static stk_status flatOnlyFlags(stk_ctx* c, uint32_t flags)
+{
+    if (c->twoLevel && (flags & (STK_TRACE_REFERENCE_ORDER | STK_TRACE_BVH2)))
+        return fail(c, STK_E_STATE, "trace: the reference-order and binary walks need the flat BVH (stk_build_bvh)");
     return STK_OK;
 }
 
@@ -383,6 +458,7 @@ STK_API stk_status stk_trace_closest_device(stk_ctx* c, const StkRay* rays, uint
     if (!validCtx(c)) return STK_E_INVALID;
     if (!c->haveBvh) return fail(c, STK_E_STATE, "stk_trace_closest: no BVH");
     if (n && (!rays || !hits)) return fail(c, STK_E_INVALID, "stk_trace_closest: null buffer");
+    if (flatOnlyFlags(c, flags) != STK_OK) return STK_E_STATE;
     (void)hipSetDevice(c->device);
     STK_HIP(c, stk::launchTraceClosest(bvhView(c), rays, n, hits, flags,
                                        c->stream));
@@ -394,6 +470,7 @@ STK_API stk_status stk_trace_any_device(stk_ctx* c, const StkRay* rays, uint32_t
     if (!validCtx(c)) return STK_E_INVALID;
     if (!c->haveBvh) return fail(c, STK_E_STATE, "stk_trace_any: no BVH");
     if (n && (!rays || !hits)) return fail(c, STK_E_INVALID, "stk_trace_any: null buffer");
+    if (flatOnlyFlags(c, flags) != STK_OK) return STK_E_STATE;
     (void)hipSetDevice(c->device);
     STK_HIP(c, stk::launchTraceAny(bvhView(c), rays, n, hits, flags, c->stream));
     return STK_OK;
@@ -403,6 +480,7 @@ static stk_status traceHost(stk_ctx* c, const StkRay* rays, uint32_t n, StkHit* 
 {
     if (!validCtx(c)) return STK_E_INVALID;
     if (!c->haveBvh) return fail(c, STK_E_STATE, "trace: no BVH");
+    if (flatOnlyFlags(c, flags) != STK_OK) return STK_E_STATE;
     if (n == 0) return STK_OK;
     if (!rays || !hits) return fail(c, STK_E_INVALID, "trace: null buffer");
     (void)hipSetDevice(c->device);
@@ -587,7 +665,9 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
         // everything the captured kernels take by value
         std::vector<uint64_t> key = {(uint64_t)sv.bvh.packed, (uint64_t)sv.bvh.tris, (uint64_t)sv.bvh.refNodes,
                                      (uint64_t)sv.bvh.wide, (uint64_t)sv.bvh.wideTris, (uint64_t)sv.bvh.emitters,
-                                     (uint64_t)sv.bvh.instances, (uint64_t)sv.vb, (uint64_t)sv.ib,
+                                     (uint64_t)sv.bvh.instances, (uint64_t)sv.bvh.tlas, (uint64_t)sv.bvh.tlasLeaves,
+                                     (uint64_t)sv.bvh.blas, (uint64_t)sv.bvh.blasTris, (uint64_t)sv.bvh.instKey,
+                                     (uint64_t)sv.bvh.instW, (uint64_t)sv.vb, (uint64_t)sv.ib,
                                      (uint64_t)sv.inst, (uint64_t)sv.lights, (uint64_t)sv.mats, sv.numMats,
                                      (uint64_t)sv.scenes, sv.numScenes, sv.totalSlots, sv.tile, lp.maxDepth,
                                      lp.spp, lp.samplingType, lp.rrStartDepth, lp.refCompat, lp.passSlots,
@@ -672,10 +752,10 @@ STK_API stk_status stk_render(stk_ctx* c, const StkRenderParams* p, float* outRg
     stk::SceneDesc& d = descs[0];
     memset(&d, 0, sizeof(d));
     d.hdr = c->bvh.header;
-    d.refRoot = 0;
+    d.refRoot = c->twoLevel ? STK_INVALID_INDEX : 0u;
     d.wideRoot = wideUsable(c->bvh) ? 0u : STK_INVALID_INDEX;
     d.emitBase = 0;
-    d.emitCount = (uint32_t)c->bvh.emitters.size();
+    d.emitCount = c->twoLevel ? (uint32_t)c->tl.emitters.size() : (uint32_t)c->bvh.emitters.size();
     d.lightBase = 0;
     d.lightCount = c->numLights;
     d.width = (uint32_t)W;

@@ -54,6 +54,32 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(BvhView bvh, const Stk
     store_hit(hits, i, h);
 }
 
+// Closest hit and any hit over a two-level structure.
+__global__ __launch_bounds__(kBlock) void k_trace_closest_tl(BvhView bvh, const StkRay* __restrict__ rays, uint32_t n,
+                                                             StkHit* __restrict__ hits)
+{
+    __shared__ uint32_t stack[2 * kWideStack * kBlock];
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const RayF r = load_ray(rays, i);
+    HitF h;
+    uint32_t ik;
+    closest_two_level(bvh, r, stack + threadIdx.x, stack + kWideStack * kBlock + threadIdx.x, kBlock, h, ik);
+    store_hit(hits, i, h);
+}
+
+__global__ __launch_bounds__(kBlock) void k_trace_any_tl(BvhView bvh, const StkRay* __restrict__ rays, uint32_t n,
+                                                         StkHit* __restrict__ hits)
+{
+    __shared__ uint32_t stack[2 * kWideStack * kBlock];
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const RayF r = load_ray(rays, i);
+    HitF h;
+    any_two_level(bvh, r, stack + threadIdx.x, stack + kWideStack * kBlock + threadIdx.x, kBlock, h);
+    store_hit(hits, i, h);
+}
+
 __global__ __launch_bounds__(kBlock) void k_trace_closest_ref(BvhView bvh, const StkRay* __restrict__ rays,
                                                               uint32_t n, StkHit* __restrict__ hits)
 {
@@ -204,7 +230,10 @@ hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n
                               hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    if (flags & STK_TRACE_REFERENCE_ORDER)
+    if (bvh.tlas) {
+        if (flags & (STK_TRACE_REFERENCE_ORDER | STK_TRACE_BVH2)) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_trace_closest_tl, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
+    } else if (flags & STK_TRACE_REFERENCE_ORDER)
         hipLaunchKernelGGL(k_trace_closest_ref, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
     else
         hipLaunchKernelGGL(k_trace_closest, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits,
@@ -216,7 +245,10 @@ hipError_t launchTraceAny(const BvhView& bvh, const StkRay* rays, uint32_t n, St
                           hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    if (flags & STK_TRACE_REFERENCE_ORDER)
+    if (bvh.tlas) {
+        if (flags & STK_TRACE_REFERENCE_ORDER) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_trace_any_tl, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
+    } else if (flags & STK_TRACE_REFERENCE_ORDER)
         hipLaunchKernelGGL(k_trace_any_ref, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
     else
         hipLaunchKernelGGL(k_trace_any, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);

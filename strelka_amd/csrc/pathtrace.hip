@@ -187,8 +187,9 @@ __global__ void k_totals_clear(PathState ps)
 static_assert(2 * kWideStack <= kStackDepth, "the wide walk's two stacks share the packed walk's LDS");
 
 // `stack` is this lane's column of kStackDepth words (BLOCK apart): the packed
-// walk's stack, or the wide walk's two node-group stacks of kWideStack words.
-template <int BLOCK>
+// walk's stack, or the wide walks' two node-group stacks of kWideStack words.
+// TL: the view's scene is a two-level structure.
+template <int BLOCK, bool TL>
 __device__ __forceinline__ void extend_one(const SceneView& sc, PathState& ps, int cur, uint32_t e, uint32_t* stack,
                                            int binary)
 {
@@ -196,7 +197,10 @@ __device__ __forceinline__ void extend_one(const SceneView& sc, PathState& ps, i
     const RayF r = make_ray(ps.rayO[slot], ps.rayD[slot]);
     const SceneDesc& d = sc.scenes[ps.scene[slot]];
     HitF h;
-    if (!binary && d.wideRoot != STK_INVALID_INDEX)
+    uint32_t ik;
+    if (TL)
+        closest_two_level(sc.bvh, r, stack, stack + kWideStack * BLOCK, BLOCK, h, ik);
+    else if (!binary && d.wideRoot != STK_INVALID_INDEX)
         closest_wide(sc.bvh, d.wideRoot, r, stack, stack + kWideStack * BLOCK, BLOCK, h);
     else
         closest_packed(sc.bvh, d.hdr, r, stack, BLOCK, h);
@@ -209,16 +213,16 @@ __device__ __forceinline__ void extend_one(const SceneView& sc, PathState& ps, i
 // takes 64 entries at a time from segment (w + k) % kStripes, k = 0, 1, ...,
 // moving on when a segment is exhausted (a plain load checks before the
 // atomic), so every wave ends after at most kStripes segments.
-template <int BLOCK>
+template <int BLOCK, bool TL>
 __global__ __launch_bounds__(BLOCK) void k_pt_extend(SceneView sc, PathState ps, int depth, int persistent,
                                                      int binary)
 {
-    __shared__ uint32_t stack[kStackDepth * BLOCK];
+    __shared__ uint32_t stack[(TL ? 2 * kWideStack : kStackDepth) * BLOCK];
     const int cur = depth & 1;
     const uint32_t* counts = ps.stripes + cur * kStripes;
     if (!persistent) {
         uint32_t e;
-        if (stripe_entry(ps, counts, BLOCK, e)) extend_one<BLOCK>(sc, ps, cur, e, stack + threadIdx.x, binary);
+        if (stripe_entry(ps, counts, BLOCK, e)) extend_one<BLOCK, TL>(sc, ps, cur, e, stack + threadIdx.x, binary);
         return;
     }
     const uint32_t lane = threadIdx.x % 64;
@@ -233,7 +237,7 @@ __global__ __launch_bounds__(BLOCK) void k_pt_extend(SceneView sc, PathState ps,
             base = (uint32_t)__shfl((int)base, 0);
             if (base >= cnt) break;
             const uint32_t q = base + lane;
-            if (q < cnt) extend_one<BLOCK>(sc, ps, cur, sg * ps.segCap + q, stack + threadIdx.x, binary);
+            if (q < cnt) extend_one<BLOCK, TL>(sc, ps, cur, sg * ps.segCap + q, stack + threadIdx.x, binary);
         }
     }
 }
@@ -424,7 +428,9 @@ constexpr uint32_t kMaxEmitters = 64;
 
 // Shadow rays: the wide visibility walk (the reference's first-in-preorder
 // any-hit rule, decided in any order), or the stackless skip-link walk when
-// the scene has no usable wide tree or many emitter triangles.
+// the scene has no usable wide tree or many emitter triangles; TL: the
+// two-level visibility walk.
+template <bool TL>
 __global__ __launch_bounds__(kBlock) void k_pt_connect(SceneView sc, PathState ps, int binary)
 {
     __shared__ uint32_t stack[2 * kWideStack * kBlock];
@@ -434,7 +440,10 @@ __global__ __launch_bounds__(kBlock) void k_pt_connect(SceneView sc, PathState p
     const SceneDesc& desc = sc.scenes[ps.scene[slot]];
     const RayF r = make_ray(ps.shO[s], ps.shD[s]);
     bool vis;
-    if (!binary && desc.wideRoot != STK_INVALID_INDEX && desc.emitCount <= kMaxEmitters) {
+    if (TL) {
+        vis = visible_two_level(sc.bvh, sc.bvh.emitters + 3 * (size_t)desc.emitBase, desc.emitCount, r,
+                                stack + threadIdx.x, stack + kWideStack * kBlock + threadIdx.x, kBlock);
+    } else if (!binary && desc.wideRoot != STK_INVALID_INDEX && desc.emitCount <= kMaxEmitters) {
         vis = visible_wide(sc.bvh, desc.wideRoot, sc.bvh.emitters + 3 * (size_t)desc.emitBase, desc.emitCount, r,
                            stack + threadIdx.x, stack + kWideStack * kBlock + threadIdx.x, kBlock);
     } else {
@@ -530,14 +539,25 @@ inline uint32_t blocks(size_t n, uint32_t b = kBlock) { return (uint32_t)((n + b
 // Stripe-major consumer grid of B-thread blocks (segCap is a multiple of 256).
 inline dim3 stripeGrid(const PathState& ps, uint32_t B) { return dim3(kStripes * (ps.segCap / B)); }
 
-template <int BLOCK>
-void launchExtend(const SceneView& sc, PathState& ps, int depth, const LaunchConfig& lc, hipStream_t s)
+template <int BLOCK, bool TL>
+void launchExtendWalk(const SceneView& sc, PathState& ps, int depth, const LaunchConfig& lc, hipStream_t s)
 {
     const int binary = lc.binaryWalk ? 1 : 0;
     if (lc.persistentBlocks)
-        hipLaunchKernelGGL(k_pt_extend<BLOCK>, dim3(lc.persistentBlocks), dim3(BLOCK), 0, s, sc, ps, depth, 1, binary);
+        hipLaunchKernelGGL((k_pt_extend<BLOCK, TL>), dim3(lc.persistentBlocks), dim3(BLOCK), 0, s, sc, ps, depth, 1,
+                           binary);
     else
-        hipLaunchKernelGGL(k_pt_extend<BLOCK>, stripeGrid(ps, BLOCK), dim3(BLOCK), 0, s, sc, ps, depth, 0, binary);
+        hipLaunchKernelGGL((k_pt_extend<BLOCK, TL>), stripeGrid(ps, BLOCK), dim3(BLOCK), 0, s, sc, ps, depth, 0,
+                           binary);
+}
+
+template <int BLOCK>
+void launchExtend(const SceneView& sc, PathState& ps, int depth, const LaunchConfig& lc, hipStream_t s)
+{
+    if (sc.bvh.tlas)
+        launchExtendWalk<BLOCK, true>(sc, ps, depth, lc, s);
+    else
+        launchExtendWalk<BLOCK, false>(sc, ps, depth, lc, s);
 }
 
 } // namespace
@@ -574,7 +594,10 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
         default: launchExtend<256>(sc, ps, depth, lc, s); break;
         }
         hipLaunchKernelGGL(k_pt_shade, stripeGrid(ps, kBlock), b, 0, s, sc, lp, ps, depth);
-        hipLaunchKernelGGL(k_pt_connect, stripeGrid(ps, kBlock), b, 0, s, sc, ps, lc.binaryWalk ? 1 : 0);
+        if (sc.bvh.tlas)
+            hipLaunchKernelGGL(k_pt_connect<true>, stripeGrid(ps, kBlock), b, 0, s, sc, ps, 0);
+        else
+            hipLaunchKernelGGL(k_pt_connect<false>, stripeGrid(ps, kBlock), b, 0, s, sc, ps, lc.binaryWalk ? 1 : 0);
     }
     hipLaunchKernelGGL(k_pt_tally, dim3(1), dim3(64), 0, s, ps);
     if (query && cb) {

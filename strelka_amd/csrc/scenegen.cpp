@@ -332,7 +332,9 @@ extern "C" STK_API stk_status stk_scene_generate(uint32_t config, uint32_t varia
         } else if (config == 5) {
             // 20 seeded meshes (10 displaced icospheres of 1,280 triangles, 10 tori
             // of 500-5,000), 10,000 instances with random rigid transforms in
-            // [-100, 100]^3, one material per instance, a large rect light above
+            // [-100, 100]^3, one material per instance, a large rect light above.
+            // variant n > 0: n instances in a box scaled to the same density,
+            // camera and light scaled alike, a 384x216 view
             Pcg32 geo(kSeedBase + 5, 1), mat(kSeedBase + 5, 2);
             std::vector<uint32_t> meshes;
             for (int k = 0; k < 20; ++k) {
@@ -358,20 +360,22 @@ extern "C" STK_API stk_status stk_scene_generate(uint32_t config, uint32_t varia
                 }
                 meshes.push_back(sc.createMesh(vb, ib));
             }
-            for (int k = 0; k < 10000; ++k) {
+            const uint32_t count = variant ? variant : 10000u;
+            const float k3 = std::cbrt((float)count / 10000.0f);
+            for (uint32_t k = 0; k < count; ++k) {
                 const uint32_t m = (uint32_t)(geo.next() % 20u);
                 const int32_t mi = (int32_t)sc.addMaterial(randomMaterial(mat));
-                sc.createInstance(meshes[m], mi, randomRigid(geo, 100.0f));
+                sc.createInstance(meshes[m], mi, randomRigid(geo, 100.0f * k3));
             }
             LightDesc d;
             d.type = 0;
-            d.xform = Mat4::translate(0.0f, 130.0f, 0.0f) * Mat4::rotateX(-(float)M_PI / 2.0f);
-            d.width = 80.0f;
-            d.height = 80.0f;
-            d.intensity = 8.0f * 80.0f * 80.0f; // radiance 8
+            d.xform = Mat4::translate(0.0f, 130.0f * k3, 0.0f) * Mat4::rotateX(-(float)M_PI / 2.0f);
+            d.width = 80.0f * k3;
+            d.height = 80.0f * k3;
+            d.intensity = 8.0f * d.width * d.height; // radiance 8
             sc.createLight(d);
-            cameraMatrices(mk3(0.0f, 30.0f, 190.0f), mk3(0, 0, 0), mk3(0, 1, 0), 55.0f, 3840, 2160, 0.01f, 1000.0f,
-                           s->camera);
+            cameraMatrices(mk3(0.0f, 30.0f * k3, 190.0f * k3), mk3(0, 0, 0), mk3(0, 1, 0), 55.0f, variant ? 384 : 3840,
+                           variant ? 216 : 2160, 0.01f, 1000.0f, s->camera);
             s->sky[0] = 0.25f;
             s->sky[1] = 0.3f;
             s->sky[2] = 0.4f;

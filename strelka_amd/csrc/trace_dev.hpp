@@ -38,6 +38,15 @@ struct BvhView {
     const float4* emitters;       // 3 x float4 per emitter triangle
     uint32_t wideRoot;            // STK_INVALID_INDEX: no usable wide tree
     uint32_t emitCount;
+    // two-level structure (tlas == nullptr: none); emitters then holds the
+    // emitter instances' world triangles (leaf = mesh-tree leaf)
+    const float4* tlas;           // 5 x float4 per StkWideNode, root at 0
+    const float4* tlasLeaves;     // 3 x float4 per StkTlasLeaf
+    const float4* blas;           // 5 x float4 per StkWideNode, all meshes
+    const float4* blasTris;       // 3 x float4 per StkBlasTriangle
+    const uint32_t* instKey;      // TLAS preorder leaf index of each instance
+    const float4* instW;          // 4 x float4 per instance: the rows of the inverse of
+                                  // objectToWorld (linear part | translation), then {pad, 0, 0, 0}
 };
 
 constexpr int kWideStack = 16; // node-group entries; the builder's wide depth must not exceed it
@@ -363,25 +372,37 @@ __device__ __forceinline__ float qbyte(uint32_t lo4, uint32_t hi4)
     return (float)((w >> (8 * (S & 3))) & 0xFFu); // one v_cvt_f32_ubyteN
 }
 
-template <int S>
+template <int S, bool PAD>
 __device__ __forceinline__ void wide_slot(const float4& w0, const uint4& q0, const uint4& q1, const uint4& q2, float sx,
-                                          float sy, float sz, const RayF& r, float lim, uint32_t& hits, float& t)
+                                          float sy, float sz, const float (&pad)[3], const RayF& r, float lim,
+                                          uint32_t& hits, float& t)
 {
     // origin + q * scale with q * scale exact (q < 2^8, scale a power of two),
-    // so the fused form rounds once, exactly like the builder's check
-    const float lox = fmaf(qbyte<S>(q0.x, q0.y), sx, w0.x);
-    const float loy = fmaf(qbyte<S>(q0.z, q0.w), sy, w0.y);
-    const float loz = fmaf(qbyte<S>(q1.x, q1.y), sz, w0.z);
-    const float hix = fmaf(qbyte<S>(q1.z, q1.w), sx, w0.x);
-    const float hiy = fmaf(qbyte<S>(q2.x, q2.y), sy, w0.y);
-    const float hiz = fmaf(qbyte<S>(q2.z, q2.w), sz, w0.z);
+    // so the fused form rounds once, exactly like the builder's check; PAD
+    // widens the box by pad[a] per axis
+    float lox = fmaf(qbyte<S>(q0.x, q0.y), sx, w0.x);
+    float loy = fmaf(qbyte<S>(q0.z, q0.w), sy, w0.y);
+    float loz = fmaf(qbyte<S>(q1.x, q1.y), sz, w0.z);
+    float hix = fmaf(qbyte<S>(q1.z, q1.w), sx, w0.x);
+    float hiy = fmaf(qbyte<S>(q2.x, q2.y), sy, w0.y);
+    float hiz = fmaf(qbyte<S>(q2.z, q2.w), sz, w0.z);
+    if (PAD) {
+        lox -= pad[0];
+        loy -= pad[1];
+        loz -= pad[2];
+        hix += pad[0];
+        hiy += pad[1];
+        hiz += pad[2];
+    }
     float t0;
     const bool in = slab(r, lox, hix, loy, hiy, loz, hiz, t0) && t0 <= r.tmax && !(t0 > lim);
     hits |= in ? (1u << S) : 0u;
     t = t0;
 }
 
-__device__ __forceinline__ void wide_visit(const float4* node, const RayF& r, float lim, WideVisit& v)
+template <bool PAD>
+__device__ __forceinline__ void wide_visit_t(const float4* node, const RayF& r, float lim, const float (&pad)[3],
+                                             WideVisit& v)
 {
     const float4 w0 = node[0], w1 = node[1];
     const uint4 q0 = *reinterpret_cast<const uint4*>(node + 2); // qlo x[0..7], qlo y[0..7]
@@ -398,17 +419,23 @@ __device__ __forceinline__ void wide_visit(const float4* node, const RayF& r, fl
     const uint32_t vmask = float_to_bits(w1.z) & 0xFFu;
     v.leafMask = vmask & ~v.imask;
     uint32_t hits = 0;
-    wide_slot<0>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[0]);
-    wide_slot<1>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[1]);
-    wide_slot<2>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[2]);
-    wide_slot<3>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[3]);
-    wide_slot<4>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[4]);
-    wide_slot<5>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[5]);
-    wide_slot<6>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[6]);
-    wide_slot<7>(w0, q0, q1, q2, sx, sy, sz, r, lim, hits, v.t[7]);
+    wide_slot<0, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[0]);
+    wide_slot<1, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[1]);
+    wide_slot<2, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[2]);
+    wide_slot<3, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[3]);
+    wide_slot<4, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[4]);
+    wide_slot<5, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[5]);
+    wide_slot<6, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[6]);
+    wide_slot<7, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[7]);
     hits &= vmask;
     v.innerHits = hits & v.imask;
     v.leafHits = hits & v.leafMask;
+}
+
+__device__ __forceinline__ void wide_visit(const float4* node, const RayF& r, float lim, WideVisit& v)
+{
+    const float none[3] = {0.0f, 0.0f, 0.0f};
+    wide_visit_t<false>(node, r, lim, none, v);
 }
 
 __device__ __forceinline__ uint32_t ray_octant(const RayF& r)
@@ -431,9 +458,9 @@ __device__ __forceinline__ uint32_t xor_permute(uint32_t m, uint32_t oct)
 // the rest of its entered inner children pushed as one node-group entry
 // (childBase, imask | remaining << 8, remaining in visiting order k = s ^ oct);
 // otherwise the next child of the top group.  Returns false when the walk is
-// over.
+// over (the stack is back at `base`: a nested walk stacks on its caller's).
 __device__ __forceinline__ bool wide_next(const WideVisit& v, uint32_t oct, uint32_t* sA, uint32_t* sB, int stride,
-                                          int& sp, uint32_t& node)
+                                          int& sp, uint32_t& node, int base = 0)
 {
     if (v.innerHits) {
         uint32_t m = v.innerHits;
@@ -454,8 +481,8 @@ __device__ __forceinline__ bool wide_next(const WideVisit& v, uint32_t oct, uint
         node = v.childBase + rank_below(v.imask, (uint32_t)sn);
         return true;
     }
-    if (sp == 0) return false;
-    const uint32_t base = sA[(sp - 1) * stride];
+    if (sp == base) return false;
+    const uint32_t first = sA[(sp - 1) * stride];
     const uint32_t word = sB[(sp - 1) * stride];
     const uint32_t imask = word & 0xFFu;
     uint32_t m = word >> 8;
@@ -465,7 +492,7 @@ __device__ __forceinline__ bool wide_next(const WideVisit& v, uint32_t oct, uint
         sB[(sp - 1) * stride] = imask | (m << 8);
     else
         --sp;
-    node = base + rank_below(imask, (uint32_t)k ^ oct);
+    node = first + rank_below(imask, (uint32_t)k ^ oct);
     return true;
 }
 
@@ -543,6 +570,275 @@ __device__ __forceinline__ bool visible_wide(const BvhView& bvh, uint32_t root, 
                 return false;
         }
         if (!wide_next(v, oct, sA, sB, stride, sp, node)) return true;
+    }
+}
+
+// ---- two-level walks ---------------------------------------------------------------
+
+// Exit distance of the ray from the box [lo, hi] (intersectRayBox's t1).
+__device__ __forceinline__ float slab_exit(const RayF& r, float4 lo, float4 hi)
+{
+    const float fx = (hi.x - r.ox) * r.ix, nx = (lo.x - r.ox) * r.ix;
+    const float fy = (hi.y - r.oy) * r.iy, ny = (lo.y - r.oy) * r.iy;
+    const float fz = (hi.z - r.oz) * r.iz, nz = (lo.z - r.oz) * r.iz;
+    return fminf(fmaxf(fx, nx), fminf(fmaxf(fy, ny), fmaxf(fz, nz)));
+}
+
+// The ray in an instance's object space: origin and direction through W, the
+// inverse of objectToWorld the builder computes (rows: linear part |
+// translation), the same parameter t, and the widening of the mesh tree's
+// boxes that keeps its walk conservative.  The object image of a world point
+// o + t d (t up to T, the exit from the instance's world box) is off by the
+// rounding of the transformed ray, and the world triangle getTriangle forms
+// (with its hit test's rounding) lies off the object triangle by float
+// rounding of M p and of the hit point; both stay below
+// 2^-16 (|W| (|o| + T |d|) + |w|) per axis.  W M differs from the identity
+// by the inverse's own rounding: the builder bounds what that moves the
+// mesh's points by in the record's pad term, added on top.
+struct InstRay {
+    RayF r;
+    float pad[3];
+};
+
+__device__ __forceinline__ void object_ray(const float4* W, const RayF& r, float T, InstRay& o)
+{
+    const float4 w0 = W[0], w1 = W[1], w2 = W[2], w3 = W[3];
+    o.r.ox = (w0.x * r.ox + w0.y * r.oy) + (w0.z * r.oz + w0.w);
+    o.r.oy = (w1.x * r.ox + w1.y * r.oy) + (w1.z * r.oz + w1.w);
+    o.r.oz = (w2.x * r.ox + w2.y * r.oy) + (w2.z * r.oz + w2.w);
+    o.r.tmax = r.tmax;
+    o.r.dx = (w0.x * r.dx + w0.y * r.dy) + w0.z * r.dz;
+    o.r.dy = (w1.x * r.dx + w1.y * r.dy) + w1.z * r.dz;
+    o.r.dz = (w2.x * r.dx + w2.y * r.dy) + w2.z * r.dz;
+    o.r.ix = 1.0f / o.r.dx;
+    o.r.iy = 1.0f / o.r.dy;
+    o.r.iz = 1.0f / o.r.dz;
+    const float ax = fabsf(r.ox) + T * fabsf(r.dx), ay = fabsf(r.oy) + T * fabsf(r.dy),
+                az = fabsf(r.oz) + T * fabsf(r.dz);
+    const float k = 1.0f / 65536.0f;
+    o.pad[0] = ((fabsf(w0.x) * ax + fabsf(w0.y) * ay) + (fabsf(w0.z) * az + fabsf(w0.w))) * k + w3.x;
+    o.pad[1] = ((fabsf(w1.x) * ax + fabsf(w1.y) * ay) + (fabsf(w1.z) * az + fabsf(w1.w))) * k + w3.x;
+    o.pad[2] = ((fabsf(w2.x) * ax + fabsf(w2.y) * ay) + (fabsf(w2.z) * az + fabsf(w2.w))) * k + w3.x;
+}
+
+// The widening's parameter bound for an instance entered through its TLAS
+// leaf: the exit from the leaf's world box, at most tmax (an unbounded or
+// undefined exit falls back to tmax).
+__device__ __forceinline__ float exit_bound(const RayF& r, float4 lo, float4 hi)
+{
+    return fminf(slab_exit(r, lo, hi) * 1.001f, r.tmax);
+}
+
+// getTriangle (shaders/raytracing.h:134-153) on a mesh-tree leaf record: the
+// world triangle {v0, e0, e1} from the object positions through objectToWorld.
+__device__ __forceinline__ void world_triangle(const float* M, const float4* p, float4& a, float4& b, float4& c)
+{
+    const float4 p0 = p[0], p1 = p[1], p2 = p[2];
+    const F3 v0 = xform4(M, mk3(p0.x, p0.y, p0.z), 1.0f);
+    const F3 e0 = xform4(M, mk3(p1.x, p1.y, p1.z), 1.0f) - v0;
+    const F3 e1 = xform4(M, mk3(p2.x, p2.y, p2.z), 1.0f) - v0;
+    a = make_float4(v0.x, v0.y, v0.z, p0.w); // .w: primId
+    b = make_float4(e0.x, e0.y, e0.z, p1.w); // .w: mesh-tree leaf
+    c = make_float4(e1.x, e1.y, e1.z, 0.0f);
+}
+
+// Order of the leaves of the expanded tree: the instance's TLAS leaf, then the
+// triangle's leaf in its mesh tree.
+__device__ __forceinline__ bool key_less(uint32_t ikA, uint32_t leafA, uint32_t ikB, uint32_t leafB)
+{
+    return ikA < ikB || (ikA == ikB && leafA < leafB);
+}
+
+// Nearest entered slot of m and its entry distance (static indices keep t[]
+// in registers).
+__device__ __forceinline__ int nearest_slot(const WideVisit& v, uint32_t m, float& tn)
+{
+    int sn = 0;
+    tn = INFINITY;
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+        if (((m >> s) & 1u) && v.t[s] < tn) {
+            tn = v.t[s];
+            sn = s;
+        }
+    return sn;
+}
+
+// Closest hit over the two-level structure, with the reference walk's
+// acceptance over the expanded tree: min t, ties to the leaf first in the
+// expanded preorder.  Instances at a TLAS node are entered nearest first; each
+// mesh walk stacks its node groups on the TLAS walk's stack.  h.leaf is the
+// mesh-tree leaf and hIk the instance's TLAS leaf of the hit.
+__device__ __forceinline__ bool closest_two_level(const BvhView& bvh, const RayF& r, uint32_t* sA, uint32_t* sB,
+                                                  int stride, HitF& h, uint32_t& hIk)
+{
+    miss(h);
+    hIk = STK_INVALID_INDEX;
+    bool found = false;
+    const uint32_t oct = ray_octant(r);
+    int sp = 0;
+    uint32_t node = 0;
+    for (;;) {
+        WideVisit v;
+        wide_visit(bvh.tlas + 5 * (size_t)node, r, (found ? h.t : 1e9f) * kPruneSlack, v);
+        for (uint32_t m = v.leafHits; m;) {
+            float ts;
+            const int s = nearest_slot(v, m, ts);
+            m &= ~(1u << s);
+            if (found && ts > h.t * kPruneSlack) continue;
+            const float4* L = bvh.tlasLeaves + 3 * (size_t)(v.triBase + rank_below(v.leafMask, (uint32_t)s));
+            const float4 L0 = L[0], L1 = L[1], L2 = L[2];
+            const uint32_t inst = float_to_bits(L0.w), ik = float_to_bits(L2.x);
+            const StkInstance& ic = bvh.instances[inst];
+            InstRay o;
+            object_ray(bvh.instW + 4 * (size_t)inst, r, exit_bound(r, L0, L1), o);
+            const uint32_t ooct = ray_octant(o.r);
+            const int base = sp;
+            uint32_t bnode = float_to_bits(L1.w);
+            for (;;) {
+                WideVisit bv;
+                wide_visit_t<true>(bvh.blas + 5 * (size_t)bnode, o.r, (found ? h.t : 1e9f) * kPruneSlack, o.pad, bv);
+                for (uint32_t lm = bv.leafHits; lm; lm &= lm - 1u) {
+                    const uint32_t ls = (uint32_t)(__ffs(lm) - 1);
+                    float4 a, b, c;
+                    world_triangle(ic.objectToWorld, bvh.blasTris + 3 * (size_t)(bv.triBase + rank_below(bv.leafMask, ls)),
+                                   a, b, c);
+                    float t, u, w;
+                    if (tri_test(r, a, b, c, t, u, w) && t < r.tmax) {
+                        const uint32_t leaf = float_to_bits(b.w);
+                        const bool acc = found ? (t < h.t || (t == h.t && key_less(ik, leaf, hIk, h.leaf))) : (t < 1e9f);
+                        if (acc) {
+                            found = true;
+                            h.t = t;
+                            h.u = u;
+                            h.v = w;
+                            h.leaf = leaf;
+                            h.inst = inst;
+                            h.prim = float_to_bits(a.w);
+                            hIk = ik;
+                        }
+                    }
+                }
+                if (!wide_next(bv, ooct, sA, sB, stride, sp, bnode, base)) break;
+            }
+        }
+        if (!wide_next(v, oct, sA, sB, stride, sp, node)) return found;
+    }
+}
+
+// Shadow-ray visibility over the two-level structure with the reference's
+// any-hit rule on the expanded tree (as visible_wide): the emitter triangles
+// first (lightKey: the first hit one in expanded preorder), then any walk
+// order, occluded by a hit non-emitter triangle ordered before lightKey.
+__device__ __forceinline__ bool visible_two_level(const BvhView& bvh, const float4* emit, uint32_t emitCount,
+                                                  const RayF& r, uint32_t* sA, uint32_t* sB, int stride)
+{
+    uint32_t lightIk = STK_INVALID_INDEX, lightLeaf = STK_INVALID_INDEX;
+    for (uint32_t e = 0; e < emitCount; ++e) {
+        const float4* p = emit + 3 * (size_t)e;
+        const float4 a = p[0], b = p[1], c = p[2];
+        float t, u, w;
+        if (tri_test(r, a, b, c, t, u, w) && t < r.tmax) {
+            const uint32_t ik = bvh.instKey[float_to_bits(b.w)], leaf = float_to_bits(a.w);
+            if (key_less(ik, leaf, lightIk, lightLeaf)) {
+                lightIk = ik;
+                lightLeaf = leaf;
+            }
+        }
+    }
+    const uint32_t oct = ray_octant(r);
+    int sp = 0;
+    uint32_t node = 0;
+    for (;;) {
+        WideVisit v;
+        wide_visit(bvh.tlas + 5 * (size_t)node, r, 1e9f * kPruneSlack, v);
+        for (uint32_t m = v.leafHits; m; m &= m - 1u) {
+            const uint32_t s = (uint32_t)(__ffs(m) - 1);
+            const float4* L = bvh.tlasLeaves + 3 * (size_t)(v.triBase + rank_below(v.leafMask, s));
+            const float4 L0 = L[0], L1 = L[1], L2 = L[2];
+            const uint32_t inst = float_to_bits(L0.w), ik = float_to_bits(L2.x);
+            // an emitter's hit triangles all come at or after lightKey; so do
+            // those of an instance whose TLAS leaf follows the light's
+            if (ik > lightIk) continue;
+            const StkInstance& ic = bvh.instances[inst];
+            if (ic.lightId != -1) continue;
+            InstRay o;
+            object_ray(bvh.instW + 4 * (size_t)inst, r, exit_bound(r, L0, L1), o);
+            const uint32_t ooct = ray_octant(o.r);
+            const int base = sp;
+            uint32_t bnode = float_to_bits(L1.w);
+            for (;;) {
+                WideVisit bv;
+                wide_visit_t<true>(bvh.blas + 5 * (size_t)bnode, o.r, 1e9f * kPruneSlack, o.pad, bv);
+                for (uint32_t lm = bv.leafHits; lm; lm &= lm - 1u) {
+                    const uint32_t ls = (uint32_t)(__ffs(lm) - 1);
+                    float4 a, b, c;
+                    world_triangle(ic.objectToWorld, bvh.blasTris + 3 * (size_t)(bv.triBase + rank_below(bv.leafMask, ls)),
+                                   a, b, c);
+                    if (ik == lightIk && float_to_bits(b.w) >= lightLeaf) continue;
+                    float t, u, w;
+                    if (tri_test(r, a, b, c, t, u, w) && t < r.tmax) return false;
+                }
+                if (!wide_next(bv, ooct, sA, sB, stride, sp, bnode, base)) break;
+            }
+        }
+        if (!wide_next(v, oct, sA, sB, stride, sp, node)) return true;
+    }
+}
+
+// anyHit (shaders/raytracing.h:249-286) over the expanded tree: the first leaf
+// in expanded preorder whose triangle is hit with t < tmax, found by walking
+// every entered box (no order of the wide walk follows the preorder).
+__device__ __forceinline__ bool any_two_level(const BvhView& bvh, const RayF& r, uint32_t* sA, uint32_t* sB,
+                                              int stride, HitF& h)
+{
+    miss(h);
+    bool found = false;
+    uint32_t bestIk = STK_INVALID_INDEX;
+    const uint32_t oct = ray_octant(r);
+    int sp = 0;
+    uint32_t node = 0;
+    for (;;) {
+        WideVisit v;
+        wide_visit(bvh.tlas + 5 * (size_t)node, r, 1e9f * kPruneSlack, v);
+        for (uint32_t m = v.leafHits; m; m &= m - 1u) {
+            const uint32_t s = (uint32_t)(__ffs(m) - 1);
+            const float4* L = bvh.tlasLeaves + 3 * (size_t)(v.triBase + rank_below(v.leafMask, s));
+            const float4 L0 = L[0], L1 = L[1], L2 = L[2];
+            const uint32_t inst = float_to_bits(L0.w), ik = float_to_bits(L2.x);
+            if (found && ik > bestIk) continue;
+            const StkInstance& ic = bvh.instances[inst];
+            InstRay o;
+            object_ray(bvh.instW + 4 * (size_t)inst, r, exit_bound(r, L0, L1), o);
+            const uint32_t ooct = ray_octant(o.r);
+            const int base = sp;
+            uint32_t bnode = float_to_bits(L1.w);
+            for (;;) {
+                WideVisit bv;
+                wide_visit_t<true>(bvh.blas + 5 * (size_t)bnode, o.r, 1e9f * kPruneSlack, o.pad, bv);
+                for (uint32_t lm = bv.leafHits; lm; lm &= lm - 1u) {
+                    const uint32_t ls = (uint32_t)(__ffs(lm) - 1);
+                    float4 a, b, c;
+                    world_triangle(ic.objectToWorld, bvh.blasTris + 3 * (size_t)(bv.triBase + rank_below(bv.leafMask, ls)),
+                                   a, b, c);
+                    const uint32_t leaf = float_to_bits(b.w);
+                    if (found && !key_less(ik, leaf, bestIk, h.leaf)) continue;
+                    float t, u, w;
+                    if (tri_test(r, a, b, c, t, u, w) && t < r.tmax) {
+                        found = true;
+                        bestIk = ik;
+                        h.t = t;
+                        h.u = u;
+                        h.v = w;
+                        h.leaf = leaf;
+                        h.inst = inst;
+                        h.prim = float_to_bits(a.w);
+                    }
+                }
+                if (!wide_next(bv, ooct, sA, sB, stride, sp, bnode, base)) break;
+            }
+        }
+        if (!wide_next(v, oct, sA, sB, stride, sp, node)) return found;
     }
 }
 
