@@ -674,137 +674,145 @@ static inline int key_less(uint32_t ikA, uint32_t leafA, uint32_t ikB, uint32_t 
     return ikA < ikB || (ikA == ikB && leafA < leafB);
 }
 
-/* One wide walk with the node-group stack of the kernel (stack shared with the
- * caller from `base`); visit(node, ...) is done by the callers below. */
-typedef struct WideStack {
-    uint32_t first[ORC_WIDE_STACK];
-    uint32_t word[ORC_WIDE_STACK];
-    int sp;
-} WideStack;
-
-static int wide_next(WideStack* s, uint32_t imask, uint32_t childBase, uint32_t innerHits, const float tch[8],
-                     uint32_t oct, int base, uint32_t* node)
-{
-    if (innerHits) {
-        int sn = 0;
-        float tn = INFINITY;
-        for (int sl = 0; sl < 8; ++sl)
-            if ((innerHits >> sl & 1u) && tch[sl] < tn) {
-                tn = tch[sl];
-                sn = sl;
-            }
-        innerHits &= ~(1u << sn);
-        if (innerHits) {
-            if (s->sp >= ORC_WIDE_STACK) abort();
-            s->first[s->sp] = childBase;
-            s->word[s->sp] = imask | (xor_permute8(innerHits, oct) << 8);
-            ++s->sp;
-        }
-        *node = childBase + popc8(imask & ((1u << sn) - 1u));
-        return 1;
-    }
-    if (s->sp == base) return 0;
-    const uint32_t first = s->first[s->sp - 1], word = s->word[s->sp - 1];
-    const uint32_t im = word & 0xFFu;
-    uint32_t m = word >> 8;
-    const int k = __builtin_ctz(m);
-    m &= m - 1u;
-    if (m)
-        s->word[s->sp - 1] = im | (m << 8);
-    else
-        --s->sp;
-    *node = first + popc8(im & ((1u << ((uint32_t)k ^ oct)) - 1u));
-    return 1;
-}
-
 static uint32_t octant_of(const float d[3])
 {
     return (signbit(d[0]) ? 1u : 0u) | (signbit(d[1]) ? 2u : 0u) | (signbit(d[2]) ? 4u : 0u);
 }
 
-/* mode 0: closest (prune by t); mode 1: first in expanded preorder (no t
- * pruning; instances after the best key skipped). */
+/* Sets up the walk of instance-tree leaf record li's mesh tree (0: skipped). */
+static int enter_instance(const OrcScene* s, const StkRay* r, const float inv[3], uint32_t li, int anyMode, int found,
+                          uint32_t bestIk, uint32_t* inst, uint32_t* ik, StkRay* c, float cinv[3], float pad[3],
+                          uint32_t* oct, uint32_t* node)
+{
+    const StkTlasLeaf* L = &s->tlasLeaves[li];
+    if (anyMode && found && L->key > bestIk) return 0;
+    *inst = L->instId;
+    *ik = L->key;
+    object_ray(s->instW + 16 * (size_t)L->instId, r, exit_bound(r, inv, L->lo, L->hi), c, cinv, pad);
+    *oct = octant_of(c->d);
+    *node = L->blasRoot;
+    return 1;
+}
+
+/* The kernel's single-loop two-level walk (trace_dev.hpp two_level_walk):
+ * stack entries are node groups of kind 0 (instance tree), 1 (mesh tree of
+ * the current instance) or 2 (entered instances of an instance-tree node, slot
+ * order); a node's instances are taken before its inner children.
+ * anyMode: first hit in expanded preorder, no t pruning. */
 static int two_level_walk(const OrcScene* s, const StkRay* r, StkHit* hit, OrcStats* st, int anyMode)
 {
     miss_hit(hit);
     if (!s->tlas || s->ntlas == 0) return 0;
     float inv[3];
     ray_invdir(r, inv);
-    const uint32_t oct = octant_of(r->d);
+    const uint32_t octW = octant_of(r->d);
     int found = 0;
     float best = 1e9f;
     uint32_t bestIk = STK_INVALID_INDEX, bestLeaf = STK_INVALID_INDEX;
     uint64_t nvisit = 0, ntri = 0;
-    WideStack stack;
-    stack.sp = 0;
-    uint32_t node = 0;
+    uint32_t stFirst[ORC_WIDE_STACK], stWord[ORC_WIDE_STACK];
+    int sp = 0;
+    StkRay c = *r;
+    float cinv[3] = {inv[0], inv[1], inv[2]}, pad[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t oct = octW, inst = 0, ik = 0, node = 0;
+    int inMesh = 0;
     for (;;) {
-        const StkWideNode* w = &s->tlas[node];
+        const StkWideNode* w = inMesh ? &s->blas[node] : &s->tlas[node];
         ++nvisit;
         float tch[8];
-        const float lim = (found && !anyMode ? best : 1e9f) * ORC_PRUNE_SLACK;
-        const uint32_t hits = wide_slots(w, r, inv, lim, NULL, tch);
+        const float lim = ((!anyMode && found) ? best : 1e9f) * ORC_PRUNE_SLACK;
+        const uint32_t hits = wide_slots(w, &c, cinv, lim, pad, tch);
         const uint32_t imask = w->imask, leafMask = (w->vmask & 0xFFu) & ~imask;
-        uint32_t lm = hits & leafMask;
-        while (lm) {
-            int sl;
-            if (anyMode) {
-                sl = __builtin_ctz(lm);
-            } else {
-                float tn = INFINITY;
-                sl = 0;
-                for (int k = 0; k < 8; ++k)
-                    if ((lm >> k & 1u) && tch[k] < tn) {
-                        tn = tch[k];
-                        sl = k;
-                    }
-            }
-            lm &= ~(1u << sl);
-            if (!anyMode && found && tch[sl] > best * ORC_PRUNE_SLACK) continue;
-            const StkTlasLeaf* L = &s->tlasLeaves[w->triBase + popc8(leafMask & ((1u << sl) - 1u))];
-            const uint32_t inst = L->instId, ik = L->key;
-            if (anyMode && found && ik > bestIk) continue;
-            StkRay orr;
-            float oinv[3], pad[3];
-            object_ray(s->instW + 16 * (size_t)inst, r, exit_bound(r, inv, L->lo, L->hi), &orr, oinv, pad);
-            const uint32_t ooct = octant_of(orr.d);
+        uint32_t inner = hits & imask;
+        const uint32_t leafHits = hits & leafMask;
+        if (inMesh) {
             const float* M = s->inst[inst].objectToWorld;
-            const int base = stack.sp;
-            uint32_t bnode = L->blasRoot;
-            for (;;) {
-                const StkWideNode* bw = &s->blas[bnode];
-                ++nvisit;
-                float btch[8];
-                const float blim = (found && !anyMode ? best : 1e9f) * ORC_PRUNE_SLACK;
-                const uint32_t bh = wide_slots(bw, &orr, oinv, blim, pad, btch);
-                const uint32_t bim = bw->imask, bleafMask = (bw->vmask & 0xFFu) & ~bim;
-                for (uint32_t m = bh & bleafMask; m; m &= m - 1u) {
-                    const int ls = __builtin_ctz(m);
-                    const StkBlasTriangle* tr = &s->blasTris[bw->triBase + popc8(bleafMask & ((1u << ls) - 1u))];
-                    if (anyMode && found && !key_less(ik, tr->leaf, bestIk, bestLeaf)) continue;
-                    float v0[3], e0[3], e1[3], t, b[2];
-                    blas_world_triangle(M, tr, v0, e0, e1);
-                    ++ntri;
-                    if (!orc_ray_triangle(r->o, r->d, v0, e0, e1, &t, b) || !(t < r->o[3])) continue;
-                    const int acc = anyMode ? 1
-                                            : (found ? (t < best || (t == best && key_less(ik, tr->leaf, bestIk, bestLeaf)))
-                                                     : (t < 1e9f));
-                    if (acc) {
-                        found = 1;
-                        best = t;
-                        bestIk = ik;
-                        bestLeaf = tr->leaf;
-                        hit->t = t;
-                        hit->bary[0] = b[0];
-                        hit->bary[1] = b[1];
-                        hit->instId = inst;
-                        hit->primId = tr->primId;
-                    }
+            for (uint32_t m = leafHits; m; m &= m - 1u) {
+                const int ls = __builtin_ctz(m);
+                const StkBlasTriangle* tr = &s->blasTris[w->triBase + popc8(leafMask & ((1u << ls) - 1u))];
+                if (anyMode && found && !key_less(ik, tr->leaf, bestIk, bestLeaf)) continue;
+                float v0[3], e0[3], e1[3], t, b[2];
+                blas_world_triangle(M, tr, v0, e0, e1);
+                ++ntri;
+                if (!orc_ray_triangle(r->o, r->d, v0, e0, e1, &t, b) || !(t < r->o[3])) continue;
+                const int acc = anyMode ? 1
+                                        : (found ? (t < best || (t == best && key_less(ik, tr->leaf, bestIk, bestLeaf)))
+                                                 : (t < 1e9f));
+                if (acc) {
+                    found = 1;
+                    best = t;
+                    bestIk = ik;
+                    bestLeaf = tr->leaf;
+                    hit->t = t;
+                    hit->bary[0] = b[0];
+                    hit->bary[1] = b[1];
+                    hit->instId = inst;
+                    hit->primId = tr->primId;
                 }
-                if (!wide_next(&stack, bim, bw->childBase, bh & bim, btch, ooct, base, &bnode)) break;
             }
+        } else if (leafHits) {
+            if (sp + 2 > ORC_WIDE_STACK) abort();
+            if (inner) {
+                stFirst[sp] = w->childBase;
+                stWord[sp] = imask | (xor_permute8(inner, oct) << 8);
+                ++sp;
+                inner = 0;
+            }
+            stFirst[sp] = w->triBase;
+            stWord[sp] = leafMask | (leafHits << 8) | (2u << 16);
+            ++sp;
         }
-        if (!wide_next(&stack, imask, w->childBase, hits & imask, tch, oct, 0, &node)) break;
+        if (inner) {
+            int sn = 0;
+            float tn = INFINITY;
+            for (int sl = 0; sl < 8; ++sl)
+                if ((inner >> sl & 1u) && tch[sl] < tn) {
+                    tn = tch[sl];
+                    sn = sl;
+                }
+            const uint32_t rest = inner & ~(1u << sn);
+            if (rest) {
+                if (sp >= ORC_WIDE_STACK) abort();
+                stFirst[sp] = w->childBase;
+                stWord[sp] = imask | (xor_permute8(rest, oct) << 8) | ((inMesh ? 1u : 0u) << 16);
+                ++sp;
+            }
+            node = w->childBase + popc8(imask & ((1u << sn) - 1u));
+            continue;
+        }
+        int go = 0;
+        while (sp > 0) {
+            const uint32_t first = stFirst[sp - 1], word = stWord[sp - 1];
+            const uint32_t mask = word & 0xFFu, kind = word >> 16;
+            uint32_t m = (word >> 8) & 0xFFu;
+            const uint32_t k = (uint32_t)__builtin_ctz(m);
+            m &= m - 1u;
+            if (m)
+                stWord[sp - 1] = (word & 0xFFFF00FFu) | (m << 8);
+            else
+                --sp;
+            if (kind == 2) {
+                if (!enter_instance(s, r, inv, first + popc8(mask & ((1u << k) - 1u)), anyMode, found, bestIk, &inst,
+                                    &ik, &c, cinv, pad, &oct, &node))
+                    continue;
+                inMesh = 1;
+                go = 1;
+                break;
+            }
+            if (kind == 0 && inMesh) {
+                c = *r;
+                for (int a = 0; a < 3; ++a) {
+                    cinv[a] = inv[a];
+                    pad[a] = 0.0f;
+                }
+                oct = octW;
+                inMesh = 0;
+            }
+            node = first + popc8(mask & ((1u << (k ^ oct)) - 1u));
+            go = 1;
+            break;
+        }
+        if (!go) break;
     }
     if (st) {
         st->inner += nvisit;

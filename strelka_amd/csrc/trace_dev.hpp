@@ -649,87 +649,159 @@ __device__ __forceinline__ bool key_less(uint32_t ikA, uint32_t leafA, uint32_t 
     return ikA < ikB || (ikA == ikB && leafA < leafB);
 }
 
-// Nearest entered slot of m and its entry distance (static indices keep t[]
-// in registers).
-__device__ __forceinline__ int nearest_slot(const WideVisit& v, uint32_t m, float& tn)
-{
-    int sn = 0;
-    tn = INFINITY;
-#pragma unroll
-    for (int s = 0; s < 8; ++s)
-        if (((m >> s) & 1u) && v.t[s] < tn) {
-            tn = v.t[s];
-            sn = s;
-        }
-    return sn;
-}
+// The two-level walks run as one loop of wide-node visits: the current ray is
+// the world ray in the instance tree and the instance's object ray (with its
+// box widening) in a mesh tree, so every iteration of a wave does the same
+// kind of work.  Stack entries are node groups tagged by kind: 0 instance-tree
+// nodes, 1 mesh-tree nodes of the current instance, 2 entered instances of an
+// instance-tree node (slot order).  A node's entered instances are taken
+// before its inner children; popping an instance sets up its object ray,
+// popping an instance-tree group from inside a mesh tree returns to the world
+// ray.  (Entering the nearest instance at once, as inner children are, costs
+// more in registers and divergence than its ordering saves.)  Modes: closest hit (t, then the expanded preorder: instance leaf, then
+// mesh-tree leaf), visibility (occluded by a non-emitter triangle ordered
+// before the first hit emitter triangle, lightIk / lightLeaf) and any hit
+// (the hit triangle first in expanded preorder).
+enum { kTlClosest = 0, kTlVisible = 1, kTlAny = 2 };
 
-// Closest hit over the two-level structure, with the reference walk's
-// acceptance over the expanded tree: min t, ties to the leaf first in the
-// expanded preorder.  Instances at a TLAS node are entered nearest first; each
-// mesh walk stacks its node groups on the TLAS walk's stack.  h.leaf is the
-// mesh-tree leaf and hIk the instance's TLAS leaf of the hit.
-__device__ __forceinline__ bool closest_two_level(const BvhView& bvh, const RayF& r, uint32_t* sA, uint32_t* sB,
-                                                  int stride, HitF& h, uint32_t& hIk)
+template <int MODE>
+__device__ __forceinline__ bool two_level_walk(const BvhView& bvh, const RayF& r, uint32_t* sA, uint32_t* sB,
+                                               int stride, HitF& h, uint32_t& hIk, uint32_t lightIk,
+                                               uint32_t lightLeaf)
 {
     miss(h);
     hIk = STK_INVALID_INDEX;
     bool found = false;
-    const uint32_t oct = ray_octant(r);
+    const uint32_t octW = ray_octant(r);
+    RayF c = r;
+    float pad[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t oct = octW, inst = 0, ik = 0;
+    bool inMesh = false;
     int sp = 0;
     uint32_t node = 0;
+    // enters the instance of instance-tree leaf record li (false: skipped)
+    auto enter = [&](uint32_t li) -> bool {
+        const float4* L = bvh.tlasLeaves + 3 * (size_t)li;
+        const float4 L0 = L[0], L1 = L[1], L2 = L[2];
+        const uint32_t ni = float_to_bits(L0.w), nk = float_to_bits(L2.x);
+        // visibility: an emitter's hit triangles, and every triangle of an
+        // instance whose leaf follows the light's, come after the light in
+        // preorder; any hit: nothing after the best
+        if (MODE == kTlVisible && (nk > lightIk || bvh.instances[ni].lightId != -1)) return false;
+        if (MODE == kTlAny && found && nk > hIk) return false;
+        inst = ni;
+        ik = nk;
+        InstRay o;
+        object_ray(bvh.instW + 4 * (size_t)ni, r, exit_bound(r, L0, L1), o);
+        c = o.r;
+        pad[0] = o.pad[0];
+        pad[1] = o.pad[1];
+        pad[2] = o.pad[2];
+        oct = ray_octant(c);
+        inMesh = true;
+        node = float_to_bits(L1.w);
+        return true;
+    };
     for (;;) {
         WideVisit v;
-        wide_visit(bvh.tlas + 5 * (size_t)node, r, (found ? h.t : 1e9f) * kPruneSlack, v);
-        for (uint32_t m = v.leafHits; m;) {
-            float ts;
-            const int s = nearest_slot(v, m, ts);
-            m &= ~(1u << s);
-            if (found && ts > h.t * kPruneSlack) continue;
-            const float4* L = bvh.tlasLeaves + 3 * (size_t)(v.triBase + rank_below(v.leafMask, (uint32_t)s));
-            const float4 L0 = L[0], L1 = L[1], L2 = L[2];
-            const uint32_t inst = float_to_bits(L0.w), ik = float_to_bits(L2.x);
-            const StkInstance& ic = bvh.instances[inst];
-            InstRay o;
-            object_ray(bvh.instW + 4 * (size_t)inst, r, exit_bound(r, L0, L1), o);
-            const uint32_t ooct = ray_octant(o.r);
-            const int base = sp;
-            uint32_t bnode = float_to_bits(L1.w);
-            for (;;) {
-                WideVisit bv;
-                wide_visit_t<true>(bvh.blas + 5 * (size_t)bnode, o.r, (found ? h.t : 1e9f) * kPruneSlack, o.pad, bv);
-                for (uint32_t lm = bv.leafHits; lm; lm &= lm - 1u) {
-                    const uint32_t ls = (uint32_t)(__ffs(lm) - 1);
-                    float4 a, b, c;
-                    world_triangle(ic.objectToWorld, bvh.blasTris + 3 * (size_t)(bv.triBase + rank_below(bv.leafMask, ls)),
-                                   a, b, c);
-                    float t, u, w;
-                    if (tri_test(r, a, b, c, t, u, w) && t < r.tmax) {
-                        const uint32_t leaf = float_to_bits(b.w);
-                        const bool acc = found ? (t < h.t || (t == h.t && key_less(ik, leaf, hIk, h.leaf))) : (t < 1e9f);
-                        if (acc) {
-                            found = true;
-                            h.t = t;
-                            h.u = u;
-                            h.v = w;
-                            h.leaf = leaf;
-                            h.inst = inst;
-                            h.prim = float_to_bits(a.w);
-                            hIk = ik;
-                        }
+        const float lim = ((MODE == kTlClosest && found) ? h.t : 1e9f) * kPruneSlack;
+        wide_visit_t<true>((inMesh ? bvh.blas : bvh.tlas) + 5 * (size_t)node, c, lim, pad, v);
+        uint32_t inner = v.innerHits;
+        if (inMesh) {
+            const float* M = bvh.instances[inst].objectToWorld;
+            for (uint32_t lm = v.leafHits; lm; lm &= lm - 1u) {
+                const uint32_t ls = (uint32_t)(__ffs(lm) - 1);
+                float4 a, b, cc;
+                world_triangle(M, bvh.blasTris + 3 * (size_t)(v.triBase + rank_below(v.leafMask, ls)), a, b, cc);
+                const uint32_t leaf = float_to_bits(b.w);
+                if (MODE == kTlVisible && ik == lightIk && leaf >= lightLeaf) continue;
+                if (MODE == kTlAny && found && !key_less(ik, leaf, hIk, h.leaf)) continue;
+                float t, u, w;
+                if (tri_test(r, a, b, cc, t, u, w) && t < r.tmax) {
+                    if (MODE == kTlVisible) return false;
+                    const bool acc = MODE == kTlAny
+                                         ? true
+                                         : (found ? (t < h.t || (t == h.t && key_less(ik, leaf, hIk, h.leaf)))
+                                                  : (t < 1e9f));
+                    if (acc) {
+                        found = true;
+                        h.t = t;
+                        h.u = u;
+                        h.v = w;
+                        h.leaf = leaf;
+                        h.inst = inst;
+                        h.prim = float_to_bits(a.w);
+                        hIk = ik;
                     }
                 }
-                if (!wide_next(bv, ooct, sA, sB, stride, sp, bnode, base)) break;
             }
+        } else if (v.leafHits) {
+            if (inner) {
+                sA[sp * stride] = v.childBase;
+                sB[sp * stride] = v.imask | (xor_permute(inner, oct) << 8);
+                ++sp;
+                inner = 0;
+            }
+            sA[sp * stride] = v.triBase;
+            sB[sp * stride] = v.leafMask | (v.leafHits << 8) | (2u << 16);
+            ++sp;
         }
-        if (!wide_next(v, oct, sA, sB, stride, sp, node)) return found;
+        if (inner) {
+            int sn = 0;
+            float tn = INFINITY;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) // static indices keep t[] in registers
+                if (((inner >> s) & 1u) && v.t[s] < tn) {
+                    tn = v.t[s];
+                    sn = s;
+                }
+            const uint32_t rest = inner & ~(1u << sn);
+            if (rest) {
+                sA[sp * stride] = v.childBase;
+                sB[sp * stride] = v.imask | (xor_permute(rest, oct) << 8) | ((inMesh ? 1u : 0u) << 16);
+                ++sp;
+            }
+            node = v.childBase + rank_below(v.imask, (uint32_t)sn);
+            continue;
+        }
+        for (;;) {
+            if (sp == 0) return MODE == kTlVisible ? true : found;
+            const uint32_t first = sA[(sp - 1) * stride], word = sB[(sp - 1) * stride];
+            const uint32_t mask = word & 0xFFu, kind = word >> 16;
+            uint32_t m = (word >> 8) & 0xFFu;
+            const uint32_t k = (uint32_t)(__ffs(m) - 1);
+            m &= m - 1u;
+            if (m)
+                sB[(sp - 1) * stride] = (word & 0xFFFF00FFu) | (m << 8);
+            else
+                --sp;
+            if (kind == 2) {
+                if (enter(first + rank_below(mask, k))) break;
+                continue;
+            }
+            if (kind == 0 && inMesh) {
+                c = r;
+                pad[0] = pad[1] = pad[2] = 0.0f;
+                oct = octW;
+                inMesh = false;
+            }
+            node = first + rank_below(mask, k ^ oct);
+            break;
+        }
     }
 }
 
-// Shadow-ray visibility over the two-level structure with the reference's
-// any-hit rule on the expanded tree (as visible_wide): the emitter triangles
-// first (lightKey: the first hit one in expanded preorder), then any walk
-// order, occluded by a hit non-emitter triangle ordered before lightKey.
+// Closest hit over the two-level structure; h.leaf is the mesh-tree leaf and
+// hIk the instance's leaf in the instance tree.
+__device__ __forceinline__ bool closest_two_level(const BvhView& bvh, const RayF& r, uint32_t* sA, uint32_t* sB,
+                                                  int stride, HitF& h, uint32_t& hIk)
+{
+    return two_level_walk<kTlClosest>(bvh, r, sA, sB, stride, h, hIk, 0u, 0u);
+}
+
+// Shadow-ray visibility with the reference's any-hit rule on the expanded
+// tree (as visible_wide): the emitter triangles first (the first hit one in
+// expanded preorder), then the walk in any order.
 __device__ __forceinline__ bool visible_two_level(const BvhView& bvh, const float4* emit, uint32_t emitCount,
                                                   const RayF& r, uint32_t* sA, uint32_t* sB, int stride)
 {
@@ -746,44 +818,9 @@ __device__ __forceinline__ bool visible_two_level(const BvhView& bvh, const floa
             }
         }
     }
-    const uint32_t oct = ray_octant(r);
-    int sp = 0;
-    uint32_t node = 0;
-    for (;;) {
-        WideVisit v;
-        wide_visit(bvh.tlas + 5 * (size_t)node, r, 1e9f * kPruneSlack, v);
-        for (uint32_t m = v.leafHits; m; m &= m - 1u) {
-            const uint32_t s = (uint32_t)(__ffs(m) - 1);
-            const float4* L = bvh.tlasLeaves + 3 * (size_t)(v.triBase + rank_below(v.leafMask, s));
-            const float4 L0 = L[0], L1 = L[1], L2 = L[2];
-            const uint32_t inst = float_to_bits(L0.w), ik = float_to_bits(L2.x);
-            // an emitter's hit triangles all come at or after lightKey; so do
-            // those of an instance whose TLAS leaf follows the light's
-            if (ik > lightIk) continue;
-            const StkInstance& ic = bvh.instances[inst];
-            if (ic.lightId != -1) continue;
-            InstRay o;
-            object_ray(bvh.instW + 4 * (size_t)inst, r, exit_bound(r, L0, L1), o);
-            const uint32_t ooct = ray_octant(o.r);
-            const int base = sp;
-            uint32_t bnode = float_to_bits(L1.w);
-            for (;;) {
-                WideVisit bv;
-                wide_visit_t<true>(bvh.blas + 5 * (size_t)bnode, o.r, 1e9f * kPruneSlack, o.pad, bv);
-                for (uint32_t lm = bv.leafHits; lm; lm &= lm - 1u) {
-                    const uint32_t ls = (uint32_t)(__ffs(lm) - 1);
-                    float4 a, b, c;
-                    world_triangle(ic.objectToWorld, bvh.blasTris + 3 * (size_t)(bv.triBase + rank_below(bv.leafMask, ls)),
-                                   a, b, c);
-                    if (ik == lightIk && float_to_bits(b.w) >= lightLeaf) continue;
-                    float t, u, w;
-                    if (tri_test(r, a, b, c, t, u, w) && t < r.tmax) return false;
-                }
-                if (!wide_next(bv, ooct, sA, sB, stride, sp, bnode, base)) break;
-            }
-        }
-        if (!wide_next(v, oct, sA, sB, stride, sp, node)) return true;
-    }
+    HitF h;
+    uint32_t hIk;
+    return two_level_walk<kTlVisible>(bvh, r, sA, sB, stride, h, hIk, lightIk, lightLeaf);
 }
 
 // anyHit (shaders/raytracing.h:249-286) over the expanded tree: the first leaf
@@ -792,54 +829,8 @@ __device__ __forceinline__ bool visible_two_level(const BvhView& bvh, const floa
 __device__ __forceinline__ bool any_two_level(const BvhView& bvh, const RayF& r, uint32_t* sA, uint32_t* sB,
                                               int stride, HitF& h)
 {
-    miss(h);
-    bool found = false;
-    uint32_t bestIk = STK_INVALID_INDEX;
-    const uint32_t oct = ray_octant(r);
-    int sp = 0;
-    uint32_t node = 0;
-    for (;;) {
-        WideVisit v;
-        wide_visit(bvh.tlas + 5 * (size_t)node, r, 1e9f * kPruneSlack, v);
-        for (uint32_t m = v.leafHits; m; m &= m - 1u) {
-            const uint32_t s = (uint32_t)(__ffs(m) - 1);
-            const float4* L = bvh.tlasLeaves + 3 * (size_t)(v.triBase + rank_below(v.leafMask, s));
-            const float4 L0 = L[0], L1 = L[1], L2 = L[2];
-            const uint32_t inst = float_to_bits(L0.w), ik = float_to_bits(L2.x);
-            if (found && ik > bestIk) continue;
-            const StkInstance& ic = bvh.instances[inst];
-            InstRay o;
-            object_ray(bvh.instW + 4 * (size_t)inst, r, exit_bound(r, L0, L1), o);
-            const uint32_t ooct = ray_octant(o.r);
-            const int base = sp;
-            uint32_t bnode = float_to_bits(L1.w);
-            for (;;) {
-                WideVisit bv;
-                wide_visit_t<true>(bvh.blas + 5 * (size_t)bnode, o.r, 1e9f * kPruneSlack, o.pad, bv);
-                for (uint32_t lm = bv.leafHits; lm; lm &= lm - 1u) {
-                    const uint32_t ls = (uint32_t)(__ffs(lm) - 1);
-                    float4 a, b, c;
-                    world_triangle(ic.objectToWorld, bvh.blasTris + 3 * (size_t)(bv.triBase + rank_below(bv.leafMask, ls)),
-                                   a, b, c);
-                    const uint32_t leaf = float_to_bits(b.w);
-                    if (found && !key_less(ik, leaf, bestIk, h.leaf)) continue;
-                    float t, u, w;
-                    if (tri_test(r, a, b, c, t, u, w) && t < r.tmax) {
-                        found = true;
-                        bestIk = ik;
-                        h.t = t;
-                        h.u = u;
-                        h.v = w;
-                        h.leaf = leaf;
-                        h.inst = inst;
-                        h.prim = float_to_bits(a.w);
-                    }
-                }
-                if (!wide_next(bv, ooct, sA, sB, stride, sp, bnode, base)) break;
-            }
-        }
-        if (!wide_next(v, oct, sA, sB, stride, sp, node)) return found;
-    }
+    uint32_t hIk;
+    return two_level_walk<kTlAny>(bvh, r, sA, sB, stride, h, hIk, 0u, 0u);
 }
 
 // XCD-aware block order: hardware deals workgroups round-robin over the 8
