@@ -281,10 +281,10 @@ def cache_study(args) -> dict:
     d_out = ctx.alloc(npix * 16)
     out = np.zeros((H, W, 4), np.float32)
 
-    def params(frame, mode, qd=2, paths=default_paths, min_rough=0.0):
+    def params(frame, mode, qd=2, paths=default_paths, min_rough=0.0, steps=1):
         return sa.render_params(cam, spp=1, max_depth=8, frame=frame, sampling=1, sky=scene.sky, tonemap=1,
                                 cache_mode=mode, cache_query_depth=qd, cache_train_paths=paths,
-                                cache_min_roughness=min_rough)
+                                cache_min_roughness=min_rough, cache_train_steps=steps)
 
     ctx.reset_accumulation()
     t0 = time.perf_counter()
@@ -300,18 +300,18 @@ def cache_study(args) -> dict:
 
     checkpoints = sorted({1, 2, 4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512} | {args.study_frames})
 
-    def run(mode, qd=2, warmup=0, paths=default_paths, min_rough=0.0, label=""):
+    def run(mode, qd=2, warmup=0, paths=default_paths, min_rough=0.0, steps=1, label=""):
         t_ms = 0.0
         for f in range(warmup):
             ctx.record(0)
-            ctx.render_device(params(500_000 + f, 4, qd, paths, min_rough), d_out)
+            ctx.render_device(params(500_000 + f, 4, qd, paths, min_rough, steps), d_out)
             ctx.record(1)
             t_ms += ctx.elapsed_ms(0, 1)
         ctx.reset_accumulation()
         curve = []
         for f in range(args.study_frames):
             ctx.record(0)
-            ctx.render_device(params(f, mode, qd, paths, min_rough), d_out)
+            ctx.render_device(params(f, mode, qd, paths, min_rough, steps), d_out)
             ctx.record(1)
             t_ms += ctx.elapsed_ms(0, 1)
             if f + 1 in checkpoints:
@@ -337,7 +337,7 @@ def cache_study(args) -> dict:
         ctx.cache_create(log2_table=v.get("log2_table", 19), base_res=16, max_res=v.get("max_res", 2048),
                          lr=v.get("lr", args.study_lr), ema=v.get("ema", args.study_ema))
         curve = run(1, v.get("query_depth", 2), v.get("warmup", 0), v.get("train_paths", default_paths),
-                    v.get("min_rough", 0.0), label=f"cached {v}")
+                    v.get("min_rough", 0.0), v.get("train_steps", 1), label=f"cached {v}")
         tc = time_to(curve, args.study_target)
         results.append({"variant": v, "time_to_target_s": tc, "speedup": (tp / tc) if (tp and tc) else None,
                         "curve": curve})
@@ -376,7 +376,8 @@ def main() -> int:
     ap.add_argument("--study-lr", type=float, default=1e-2)
     ap.add_argument("--study-ema", type=float, default=0.99)
     ap.add_argument("--study-variants", default='[{"query_depth": 2}]',
-                    help="JSON list of cached-run settings: query_depth, min_rough, warmup, lr, ema, train_paths, log2_table, "
+                    help="JSON list of cached-run settings: query_depth, min_rough, train_steps, warmup, lr, ema, train_paths, "
+                         "log2_table, "
                          "max_res")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

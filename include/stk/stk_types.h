@@ -238,7 +238,8 @@ typedef struct StkRenderParams {
     uint32_t cacheQueryDepth;
     uint32_t cacheTrainPaths; /* training paths per frame (extended to maxDepth) */
     float cacheMinRoughness;  /* in [0, 1]; 0 queries at cacheQueryDepth */
-    uint32_t pad[2];
+    uint32_t cacheTrainSteps; /* optimiser steps per frame over the frame's training records (0 = 1, <= 64) */
+    uint32_t pad[1];
 } StkRenderParams;
 
 #ifdef __cplusplus

@@ -187,7 +187,7 @@ def render_params(camera: PathTracerParam, spp: int = 1, max_depth: int = 8, fra
                   sampling: int = 1, sky=(0.0, 0.0, 0.0), tonemap: int = 1, accumulate: bool = True,
                   keep_samples: bool = False, rr_start: int = 3, enable_tonemap: bool = True,
                   cache_mode: int = 0, cache_query_depth: int = 2, cache_train_paths: int = 65536,
-                  cache_min_roughness: float = 0.0) -> RenderParams:
+                  cache_min_roughness: float = 0.0, cache_train_steps: int = 1) -> RenderParams:
     p = RenderParams()
     C.memmove(C.byref(p.pt), C.byref(camera), C.sizeof(PathTracerParam))
     p.pt.spp = spp
@@ -204,6 +204,7 @@ def render_params(camera: PathTracerParam, spp: int = 1, max_depth: int = 8, fra
     p.cacheQueryDepth = cache_query_depth
     p.cacheTrainPaths = cache_train_paths
     p.cacheMinRoughness = cache_min_roughness
+    p.cacheTrainSteps = cache_train_steps
     for i in range(3):
         p.skyColor[i] = float(sky[i])
     p.skyColor[3] = 0.0

@@ -58,7 +58,7 @@ class RenderParams(C.Structure):
                 ("tonemapperType", C.c_uint32), ("enableAccumulation", C.c_uint32), ("skyColor", C.c_float * 4),
                 ("enableTonemap", C.c_uint32), ("keepSamples", C.c_uint32), ("cacheMode", C.c_uint32),
                 ("cacheQueryDepth", C.c_uint32), ("cacheTrainPaths", C.c_uint32),
-                ("cacheMinRoughness", C.c_float), ("pad", C.c_uint32 * 2)]
+                ("cacheMinRoughness", C.c_float), ("cacheTrainSteps", C.c_uint32), ("pad", C.c_uint32 * 1)]
 
 
 class CacheConfig(C.Structure):

@@ -636,6 +636,8 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
         if (!(p->cacheMinRoughness >= 0.0f && p->cacheMinRoughness <= 1.0f))
             return fail(c, STK_E_INVALID, "render: cacheMinRoughness must be in [0, 1]");
         pc.queryMinRough = p->cacheMinRoughness;
+        if (p->cacheTrainSteps > 64) return fail(c, STK_E_INVALID, "render: cacheTrainSteps must be at most 64");
+        pc.trainSteps = p->cacheTrainSteps ? p->cacheTrainSteps : 1u;
         const uint32_t paths = p->cacheTrainPaths ? p->cacheTrainPaths : 65536u;
         pc.trainStride = std::max<uint32_t>(1u, (uint32_t)(slots / paths));
         pc.maxTrainPaths = (uint32_t)((slots + pc.trainStride - 1) / pc.trainStride);

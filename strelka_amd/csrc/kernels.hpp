@@ -100,6 +100,7 @@ struct PtCache {
     uint32_t mode = 0; // 0 off, 1 query + train, 2 query only, 3 train only, 4 warm-up
     uint32_t queryDepth = 2;
     float queryMinRough = 0.0f;
+    uint32_t trainSteps = 1; // optimiser steps per frame over the frame's records
     uint32_t trainStride = 1;
     uint32_t trainCap = 0;
     uint32_t maxTrainPaths = 0;

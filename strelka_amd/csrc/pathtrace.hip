@@ -608,10 +608,13 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
     hipLaunchKernelGGL(k_pt_finalize, g, b, 0, s, lp, ps, accum, samples);
     if (train && cb && pc.maxTrainPaths > 0) {
         hipLaunchKernelGGL(k_cache_targets, dim3(blocks((size_t)pc.maxTrainPaths * lp.maxDepth)), b, 0, s, lp, ps);
-        hipError_t e = cacheBackward(*cb, pc.trainBatch, pc.trainCap, ps.counters + 4, s);
-        if (e != hipSuccess) return e;
-        e = cacheAdam(*cb, 1.0f, ps.counters + 4, pc.trainCap, s);
-        if (e != hipSuccess) return e;
+        // trainSteps optimiser steps over the frame's records
+        for (uint32_t k = 0; k < pc.trainSteps; ++k) {
+            hipError_t e = cacheBackward(*cb, pc.trainBatch, pc.trainCap, ps.counters + 4, s);
+            if (e != hipSuccess) return e;
+            e = cacheAdam(*cb, 1.0f, ps.counters + 4, pc.trainCap, s);
+            if (e != hipSuccess) return e;
+        }
     }
     return hipGetLastError();
 }
