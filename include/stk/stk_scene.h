@@ -23,6 +23,8 @@
  *             rotations and translations in [-100,100]^3 (about 20 M
  *             triangles), one Lambert/GGX material per instance, an 80x80 rect
  *             light at y = 130 (radiance 8), sky (0.25, 0.3, 0.4), 3840x2160.
+ *             variant n > 0: n instances in a box scaled by cbrt(n / 10,000)
+ *             (same density; light and camera scaled alike), 384x216.
  *   config 6: a tiny scene (heightfield 16x16, 8 spheres) for smoke tests.
  *   config 7: the config-1 room closed and lit by a 1x1 ceiling light
  *             (walls albedo 0.7), 128x128: long multi-bounce paths.
