@@ -9,6 +9,7 @@ CPU code.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -181,6 +182,61 @@ class HostTwoLevel:
         if getattr(self, "_h", None):
             _LIB.stk_tlas_free(self._h)
             self._h = None
+
+
+def save_bvh(path, nodes: np.ndarray) -> None:
+    """stk_bvh_save_nodes: the skip-link nodes to a BVH file."""
+    nodes = np.ascontiguousarray(nodes, dtype=BVH_NODE)
+    st = _LIB.stk_bvh_save_nodes(_ptr(nodes), nodes.size, os.fsencode(path))
+    if st:
+        raise StkError(st, f"stk_bvh_save_nodes({path})")
+
+
+def load_bvh(path) -> np.ndarray:
+    """stk_bvh_load_nodes: the skip-link nodes of a BVH file."""
+    n = C.c_uint32()
+    st = _LIB.stk_bvh_load_nodes(os.fsencode(path), None, 0, C.byref(n))
+    if st:
+        raise StkError(st, f"stk_bvh_load_nodes({path})")
+    nodes = np.zeros(n.value, dtype=BVH_NODE)
+    st = _LIB.stk_bvh_load_nodes(os.fsencode(path), _ptr(nodes), nodes.size, C.byref(n))
+    if st:
+        raise StkError(st, f"stk_bvh_load_nodes({path})")
+    return nodes
+
+
+def write_pfm(path, rgba: np.ndarray) -> None:
+    """stk_image_write_pfm: an (h, w, 4) float32 image, top row first."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.float32)
+    h, w = rgba.shape[0], rgba.shape[1]
+    assert rgba.shape == (h, w, 4)
+    st = _LIB.stk_image_write_pfm(os.fsencode(path), _ptr(rgba), w, h)
+    if st:
+        raise StkError(st, f"stk_image_write_pfm({path})")
+
+
+def read_pfm(path) -> np.ndarray:
+    """stk_image_read_pfm: an (h, w, 4) float32 image, top row first."""
+    w, h = C.c_uint32(), C.c_uint32()
+    st = _LIB.stk_image_read_pfm(os.fsencode(path), None, 0, C.byref(w), C.byref(h))
+    if st:
+        raise StkError(st, f"stk_image_read_pfm({path})")
+    out = np.zeros((h.value, w.value, 4), np.float32)
+    st = _LIB.stk_image_read_pfm(os.fsencode(path), _ptr(out), out.size, C.byref(w), C.byref(h))
+    if st:
+        raise StkError(st, f"stk_image_read_pfm({path})")
+    return out
+
+
+def rmse(a: np.ndarray, b: np.ndarray, tonemapped: bool = True) -> float:
+    """Root-mean-square difference of two images' RGB channels; with
+    tonemapped, both are first clamped to [0, 1] (the display range of an
+    ACES-tonemapped frame)."""
+    x = np.asarray(a, np.float64)[..., :3]
+    y = np.asarray(b, np.float64)[..., :3]
+    if tonemapped:
+        x, y = np.clip(x, 0.0, 1.0), np.clip(y, 0.0, 1.0)
+    return float(np.sqrt(np.mean((x - y) ** 2)))
 
 
 def render_params(camera: PathTracerParam, spp: int = 1, max_depth: int = 8, frame: int = 0,

@@ -202,6 +202,10 @@ PROTOTYPES = [
     ("stk_bvh_wide_depth", U32, [P]),
     ("stk_bvh_num_emitters", U32, [P]),
     ("stk_build_two_level", C.c_int, [P, C.POINTER(BvhOptions)]),
+    ("stk_bvh_save_nodes", C.c_int, [P, U32, C.c_char_p]),
+    ("stk_bvh_load_nodes", C.c_int, [C.c_char_p, P, U32, C.POINTER(C.c_uint32)]),
+    ("stk_image_write_pfm", C.c_int, [C.c_char_p, P, U32, U32]),
+    ("stk_image_read_pfm", C.c_int, [C.c_char_p, P, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("stk_tlas_build_host", C.c_int, [P, U32, P, U32, P, U32, C.POINTER(BvhOptions), C.POINTER(P)]),
     ("stk_tlas_free", None, [P]),
     ("stk_tlas_num_meshes", U32, [P]),
