@@ -55,7 +55,8 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(BvhView bvh, const Stk
 }
 
 // Closest hit and any hit over a two-level structure.
-__global__ __launch_bounds__(kBlock) void k_trace_closest_tl(BvhView bvh, const StkRay* __restrict__ rays, uint32_t n,
+// five waves per SIMD: the two-level walk fits 96 registers without spills
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_trace_closest_tl(BvhView bvh, const StkRay* __restrict__ rays, uint32_t n,
                                                              StkHit* __restrict__ hits)
 {
     __shared__ uint32_t stack[2 * kWideStack * kBlock];

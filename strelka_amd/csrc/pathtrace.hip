@@ -214,7 +214,7 @@ __device__ __forceinline__ void extend_one(const SceneView& sc, PathState& ps, i
 // moving on when a segment is exhausted (a plain load checks before the
 // atomic), so every wave ends after at most kStripes segments.
 template <int BLOCK, bool TL>
-__global__ __launch_bounds__(BLOCK) void k_pt_extend(SceneView sc, PathState ps, int depth, int persistent,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pt_extend(SceneView sc, PathState ps, int depth, int persistent,
                                                      int binary)
 {
     __shared__ uint32_t stack[(TL ? 2 * kWideStack : kStackDepth) * BLOCK];
@@ -431,7 +431,7 @@ constexpr uint32_t kMaxEmitters = 64;
 // the scene has no usable wide tree or many emitter triangles; TL: the
 // two-level visibility walk.
 template <bool TL>
-__global__ __launch_bounds__(kBlock) void k_pt_connect(SceneView sc, PathState ps, int binary)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pt_connect(SceneView sc, PathState ps, int binary)
 {
     __shared__ uint32_t stack[2 * kWideStack * kBlock];
     uint32_t s;
