@@ -596,23 +596,24 @@ uint32_t orc_expand_two_level(const StkBvhNode* top, uint32_t ntop, const StkBvh
 }
 
 /* Slab tests of the 8 slots of a wide node (decoded as the kernel decodes them,
- * widened by pad[a]); tch[s] gets the entry distance.  Returns the entered mask
- * (t0 <= tmax and not beyond lim). */
+ * widened by pad[a]: the origin moved out by pad before the decode); tch[s]
+ * gets the entry distance.  Returns the entered mask (t0 <= tmax and not
+ * beyond lim). */
 static uint32_t wide_slots(const StkWideNode* w, const StkRay* r, const float inv[3], float lim, const float pad[3],
                            float tch[8])
 {
-    float sc[3];
+    float sc[3], olo[3], ohi[3];
     uint32_t hits = 0;
-    for (int a = 0; a < 3; ++a) sc[a] = wide_scale(w->exps[a]);
+    for (int a = 0; a < 3; ++a) {
+        sc[a] = wide_scale(w->exps[a]);
+        olo[a] = pad ? w->origin[a] - pad[a] : w->origin[a];
+        ohi[a] = pad ? w->origin[a] + pad[a] : w->origin[a];
+    }
     for (int sl = 0; sl < 8; ++sl) {
         float lo[3], hi[3], t0;
         for (int a = 0; a < 3; ++a) {
-            lo[a] = fmaf((float)w->qlo[a][sl], sc[a], w->origin[a]);
-            hi[a] = fmaf((float)w->qhi[a][sl], sc[a], w->origin[a]);
-            if (pad) {
-                lo[a] = lo[a] - pad[a];
-                hi[a] = hi[a] + pad[a];
-            }
+            lo[a] = fmaf((float)w->qlo[a][sl], sc[a], olo[a]);
+            hi[a] = fmaf((float)w->qhi[a][sl], sc[a], ohi[a]);
         }
         tch[sl] = 0.0f;
         if (orc_intersect_ray_box(r, inv, lo, hi, &t0) && t0 <= r->o[3] && !(t0 > lim)) {

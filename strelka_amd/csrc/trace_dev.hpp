@@ -372,34 +372,29 @@ __device__ __forceinline__ float qbyte(uint32_t lo4, uint32_t hi4)
     return (float)((w >> (8 * (S & 3))) & 0xFFu); // one v_cvt_f32_ubyteN
 }
 
-template <int S, bool PAD>
-__device__ __forceinline__ void wide_slot(const float4& w0, const uint4& q0, const uint4& q1, const uint4& q2, float sx,
-                                          float sy, float sz, const float (&pad)[3], const RayF& r, float lim,
-                                          uint32_t& hits, float& t)
+template <int S>
+__device__ __forceinline__ void wide_slot(const uint4& q0, const uint4& q1, const uint4& q2, float sx, float sy,
+                                          float sz, const float (&lo)[3], const float (&hi)[3], const RayF& r,
+                                          float lim, uint32_t& hits, float& t)
 {
     // origin + q * scale with q * scale exact (q < 2^8, scale a power of two),
-    // so the fused form rounds once, exactly like the builder's check; PAD
-    // widens the box by pad[a] per axis
-    float lox = fmaf(qbyte<S>(q0.x, q0.y), sx, w0.x);
-    float loy = fmaf(qbyte<S>(q0.z, q0.w), sy, w0.y);
-    float loz = fmaf(qbyte<S>(q1.x, q1.y), sz, w0.z);
-    float hix = fmaf(qbyte<S>(q1.z, q1.w), sx, w0.x);
-    float hiy = fmaf(qbyte<S>(q2.x, q2.y), sy, w0.y);
-    float hiz = fmaf(qbyte<S>(q2.z, q2.w), sz, w0.z);
-    if (PAD) {
-        lox -= pad[0];
-        loy -= pad[1];
-        loz -= pad[2];
-        hix += pad[0];
-        hiy += pad[1];
-        hiz += pad[2];
-    }
+    // so the fused form rounds once, exactly like the builder's check; lo / hi
+    // are the node origin, or the origin moved out by a widening
+    const float lox = fmaf(qbyte<S>(q0.x, q0.y), sx, lo[0]);
+    const float loy = fmaf(qbyte<S>(q0.z, q0.w), sy, lo[1]);
+    const float loz = fmaf(qbyte<S>(q1.x, q1.y), sz, lo[2]);
+    const float hix = fmaf(qbyte<S>(q1.z, q1.w), sx, hi[0]);
+    const float hiy = fmaf(qbyte<S>(q2.x, q2.y), sy, hi[1]);
+    const float hiz = fmaf(qbyte<S>(q2.z, q2.w), sz, hi[2]);
     float t0;
     const bool in = slab(r, lox, hix, loy, hiy, loz, hiz, t0) && t0 <= r.tmax && !(t0 > lim);
     hits |= in ? (1u << S) : 0u;
     t = t0;
 }
 
+// PAD: every child box widened by pad[a] per axis (the origin moved out by
+// pad before the decode: one more rounding, far inside the widening).
+// pad = 0 decodes exactly like the unwidened form.
 template <bool PAD>
 __device__ __forceinline__ void wide_visit_t(const float4* node, const RayF& r, float lim, const float (&pad)[3],
                                              WideVisit& v)
@@ -418,15 +413,23 @@ __device__ __forceinline__ void wide_visit_t(const float4* node, const RayF& r, 
     v.triBase = float_to_bits(w1.y);
     const uint32_t vmask = float_to_bits(w1.z) & 0xFFu;
     v.leafMask = vmask & ~v.imask;
+    float lo[3] = {w0.x, w0.y, w0.z}, hi[3] = {w0.x, w0.y, w0.z};
+    if (PAD) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = lo[a] - pad[a];
+            hi[a] = hi[a] + pad[a];
+        }
+    }
     uint32_t hits = 0;
-    wide_slot<0, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[0]);
-    wide_slot<1, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[1]);
-    wide_slot<2, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[2]);
-    wide_slot<3, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[3]);
-    wide_slot<4, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[4]);
-    wide_slot<5, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[5]);
-    wide_slot<6, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[6]);
-    wide_slot<7, PAD>(w0, q0, q1, q2, sx, sy, sz, pad, r, lim, hits, v.t[7]);
+    wide_slot<0>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[0]);
+    wide_slot<1>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[1]);
+    wide_slot<2>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[2]);
+    wide_slot<3>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[3]);
+    wide_slot<4>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[4]);
+    wide_slot<5>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[5]);
+    wide_slot<6>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[6]);
+    wide_slot<7>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[7]);
     hits &= vmask;
     v.innerHits = hits & v.imask;
     v.leafHits = hits & v.leafMask;
