@@ -102,6 +102,16 @@ STK_API stk_status stk_set_materials(stk_ctx* ctx, const StkMaterial* mats, uint
 /* Builds the BVH on the host (binned SAH, one triangle per leaf, preorder plus
  * skip links) and uploads it.  opts may be NULL for defaults. */
 STK_API stk_status stk_build_bvh(stk_ctx* ctx, const StkBvhOptions* opts);
+/* Builds the flat BVH on the device instead of the host (no reference
+ * counterpart; the reference builds with Embree on the host,
+ * src/render/bvh.cpp:107-167): a linear BVH (Morton-ordered centroids, binary
+ * radix tree, one triangle per leaf) in the same skip-link preorder layout,
+ * with the packed and compressed 8-wide layouts derived on the device.  Hits
+ * are the reference walk's over that tree; its shape differs from
+ * stk_build_bvh's SAH tree.  A tree deeper than the binary walk's stack (32)
+ * is walked by the wide walk (closest hit, renders) or the stackless
+ * skip-link walk (any hit); the binary-walk flag is then refused. */
+STK_API stk_status stk_build_bvh_gpu(stk_ctx* ctx);
 /* Builds a two-level structure instead of the flat BVH (no reference
  * counterpart: the reference flattens every instance into one tree,
  * src/render/ptrender.cpp:460-532): one tree per distinct mesh (instance index

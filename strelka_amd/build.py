@@ -20,7 +20,7 @@ LIB = PKG / "libstrelka_amd.so"
 ARCH = os.environ.get("STK_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["bvh.cpp", "bvh_api.cpp", "files.cpp", "scene.cpp", "scenegen.cpp"]
-HIP_SOURCES = ["kernels.hip", "pathtrace.hip", "context.hip", "cache.hip"]
+HIP_SOURCES = ["kernels.hip", "pathtrace.hip", "context.hip", "cache.hip", "gpubuild.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fvisibility=hidden",
           "-Wall", "-Wno-unused-function"]

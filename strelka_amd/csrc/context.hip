@@ -18,6 +18,7 @@
 #include "../../include/stk/stk.h"
 #include "bvh.hpp"
 #include "cache.hpp"
+#include "gpubuild.hpp"
 #include "kernels.hpp"
 
 namespace {
@@ -67,6 +68,11 @@ struct stk_ctx {
     // two-level structure (stk_build_two_level): replaces the flat tree
     stk::TwoLevel tl;
     bool twoLevel = false;
+    // the flat tree's device arrays as the walks see them: counts of the wide
+    // nodes and emitter triangles, and whether the tree was built on the GPU
+    // (then the host-side arrays of bvh are empty and stk_get_bvh downloads)
+    uint32_t numNodes = 0, numWide = 0, numEmit = 0;
+    bool gpuBuilt = false;
 
     DevBuf vb, ib, inst, lights, mats, packed, tris, refNodes, wideNodes, wideTris, emitters;
     DevBuf tlasWide, tlasLeaves, blasWide, blasTris, instKey, instW, tlEmit;
@@ -172,6 +178,9 @@ stk_status devUpload(stk_ctx* c, DevBuf& b, const void* src, size_t bytes)
 // The wide walk needs one node-group stack entry per wide level.
 bool wideUsable(const stk::Bvh& b) { return !b.wide.empty() && b.wideDepth <= (uint32_t)stk::dev::kWideStack; }
 
+// The same for the context's flat tree (host-built or GPU-built).
+bool ctxWideUsable(const stk_ctx* c) { return c->numWide > 0 && c->bvh.wideDepth <= (uint32_t)stk::dev::kWideStack; }
+
 stk::BvhView bvhView(stk_ctx* c)
 {
     stk::BvhView v;
@@ -183,8 +192,9 @@ stk::BvhView bvhView(stk_ctx* c)
     v.wide = static_cast<const float4*>(c->wideNodes.ptr);
     v.wideTris = static_cast<const float4*>(c->wideTris.ptr);
     v.emitters = static_cast<const float4*>(c->emitters.ptr);
-    v.wideRoot = wideUsable(c->bvh) ? 0u : STK_INVALID_INDEX;
-    v.emitCount = (uint32_t)c->bvh.emitters.size();
+    v.wideRoot = ctxWideUsable(c) ? 0u : STK_INVALID_INDEX;
+    v.emitCount = c->numEmit;
+    v.packedOk = c->bvh.depth <= (uint32_t)stk::dev::kStackDepth ? 1u : 0u;
     v.tlas = v.tlasLeaves = v.blas = v.blasTris = v.instW = nullptr;
     v.instKey = nullptr;
     if (c->twoLevel) {
@@ -217,6 +227,10 @@ stk_status uploadBvh(stk_ctx* c)
         return st;
     if ((st = devUpload(c, c->emitters, b.emitters.data(), b.emitters.size() * sizeof(StkTriangle))) != STK_OK)
         return st;
+    c->numNodes = (uint32_t)b.nodes.size();
+    c->numWide = (uint32_t)b.wide.size();
+    c->numEmit = (uint32_t)b.emitters.size();
+    c->gpuBuilt = false;
     c->haveBvh = true;
     return STK_OK;
 }
@@ -403,6 +417,7 @@ STK_API stk_status stk_build_two_level(stk_ctx* c, const StkBvhOptions* opts)
         c->bvh.header.lo[a] = t.lo[a];
         c->bvh.header.hi[a] = t.hi[a];
     }
+    c->gpuBuilt = false;
     c->twoLevel = true;
     c->haveBvh = true;
     invalidateGraph(c);
@@ -438,10 +453,86 @@ STK_API stk_status stk_get_bvh(stk_ctx* c, StkBvhNode* nodes, uint32_t cap, uint
     if (!validCtx(c) || !n) return STK_E_INVALID;
     if (!c->haveBvh) return fail(c, STK_E_STATE, "stk_get_bvh: no BVH");
     if (c->twoLevel) return fail(c, STK_E_STATE, "stk_get_bvh: the context holds a two-level structure");
+    if (c->gpuBuilt) {
+        // the device copy: leaves carry their triangle index in maxBounds[0],
+        // which the reference layout leaves zero
+        *n = c->numNodes;
+        if (!nodes) return STK_OK;
+        if (cap < *n) return fail(c, STK_E_INVALID, "stk_get_bvh: capacity too small");
+        (void)hipSetDevice(c->device);
+        STK_HIP(c, hipMemcpyAsync(nodes, c->refNodes.ptr, (size_t)*n * sizeof(StkBvhNode), hipMemcpyDeviceToHost,
+                                  c->stream));
+        STK_HIP(c, hipStreamSynchronize(c->stream));
+        for (uint32_t i = 0; i < *n; ++i)
+            if ((uint32_t)nodes[i].instId != STK_INVALID_INDEX) nodes[i].maxBounds[0] = 0.0f;
+        return STK_OK;
+    }
     *n = (uint32_t)c->bvh.nodes.size();
     if (!nodes) return STK_OK;
     if (cap < *n) return fail(c, STK_E_INVALID, "stk_get_bvh: capacity too small");
     memcpy(nodes, c->bvh.nodes.data(), c->bvh.nodes.size() * sizeof(StkBvhNode));
+    return STK_OK;
+}
+
+STK_API stk_status stk_build_bvh_gpu(stk_ctx* c)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    if (!c->haveGeometry) return fail(c, STK_E_STATE, "stk_build_bvh_gpu: no geometry set");
+    const uint32_t ninst = (uint32_t)c->hinst.size();
+    std::vector<uint32_t> start(ninst + 1, 0);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < ninst; ++i) {
+        start[i] = (uint32_t)total;
+        total += (uint64_t)(c->hinst[i].indexCount / 3);
+    }
+    if (total >= (1ull << 30)) return fail(c, STK_E_INVALID, "stk_build_bvh_gpu: too many triangles");
+    start[ninst] = (uint32_t)total;
+    (void)hipSetDevice(c->device);
+    std::vector<DevBuf> temps;
+    auto alloc = [&](int slot, size_t bytes) -> void* {
+        DevBuf* b = nullptr;
+        switch (slot) {
+        case stk::kGbRefNodes: b = &c->refNodes; break;
+        case stk::kGbPacked: b = &c->packed; break;
+        case stk::kGbTris: b = &c->tris; break;
+        case stk::kGbWide: b = &c->wideNodes; break;
+        case stk::kGbWideTris: b = &c->wideTris; break;
+        case stk::kGbEmitters: b = &c->emitters; break;
+        default: {
+            const size_t k = (size_t)(slot - stk::kGbFirstTemp);
+            if (temps.size() <= k) temps.resize(k + 1);
+            b = &temps[k];
+        }
+        }
+        if (devReserve(c, *b, std::max<size_t>(bytes, 16)) != STK_OK) return nullptr;
+        return b->ptr;
+    };
+    stk::GpuBvhResult r;
+    std::string err;
+    c->haveBvh = false;
+    const hipError_t e = stk::gpuBuildBvh(static_cast<const StkVertex*>(c->vb.ptr), static_cast<const uint32_t*>(c->ib.ptr),
+                                          static_cast<const StkInstance*>(c->inst.ptr), start.data(), ninst,
+                                          (uint32_t)total, alloc, r, err, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    for (DevBuf& b : temps) (void)devRelease(c, b);
+    if (e != hipSuccess) {
+        if (e == hipErrorOutOfMemory) return fail(c, STK_E_OOM, "stk_build_bvh_gpu: " + err);
+        if (!err.empty()) return fail(c, STK_E_INVALID, "stk_build_bvh_gpu: " + err);
+        return hipFail(c, e, "stk_build_bvh_gpu");
+    }
+    if (total > 0 && r.wideDepth > (uint32_t)stk::dev::kWideStack)
+        return fail(c, STK_E_INVALID, "stk_build_bvh_gpu: wide tree deeper than the walk's stack (16)");
+    c->bvh = stk::Bvh{};
+    c->bvh.header = r.header;
+    c->bvh.depth = r.depth;
+    c->bvh.wideDepth = r.wideDepth;
+    c->numNodes = r.numNodes;
+    c->numWide = r.numWide;
+    c->numEmit = r.numEmit;
+    c->gpuBuilt = true;
+    c->twoLevel = false;
+    c->haveBvh = true;
+    invalidateGraph(c);
     return STK_OK;
 }
 
@@ -450,6 +541,8 @@ static stk_status flatOnlyFlags(stk_ctx* c, uint32_t flags)
 {
     if (c->twoLevel && (flags & (STK_TRACE_REFERENCE_ORDER | STK_TRACE_BVH2)))
         return fail(c, STK_E_STATE, "trace: the reference-order and binary walks need the flat BVH (stk_build_bvh)");
+    if (!c->twoLevel && c->bvh.depth > (uint32_t)stk::dev::kStackDepth && (flags & STK_TRACE_BVH2))
+        return fail(c, STK_E_STATE, "trace: the tree is deeper than the binary walk's stack (32)");
     return STK_OK;
 }
 
@@ -755,9 +848,9 @@ STK_API stk_status stk_render(stk_ctx* c, const StkRenderParams* p, float* outRg
     memset(&d, 0, sizeof(d));
     d.hdr = c->bvh.header;
     d.refRoot = c->twoLevel ? STK_INVALID_INDEX : 0u;
-    d.wideRoot = wideUsable(c->bvh) ? 0u : STK_INVALID_INDEX;
+    d.wideRoot = (!c->twoLevel && ctxWideUsable(c)) ? 0u : STK_INVALID_INDEX;
     d.emitBase = 0;
-    d.emitCount = c->twoLevel ? (uint32_t)c->tl.emitters.size() : (uint32_t)c->bvh.emitters.size();
+    d.emitCount = c->twoLevel ? (uint32_t)c->tl.emitters.size() : c->numEmit;
     d.lightBase = 0;
     d.lightCount = c->numLights;
     d.width = (uint32_t)W;
@@ -1031,6 +1124,7 @@ STK_API stk_status stk_batch_render(stk_ctx* c, const StkRenderParams* p, float*
     sv.bvh.wideTris = static_cast<const float4*>(c->fWideTris.ptr);
     sv.bvh.emitters = static_cast<const float4*>(c->fEmit.ptr);
     sv.bvh.wideRoot = STK_INVALID_INDEX; // per scene: SceneDesc.wideRoot
+    sv.bvh.packedOk = 1;                  // host-built trees fit the packed walk's stack
     sv.vb = static_cast<const StkVertex*>(c->fVb.ptr);
     sv.ib = static_cast<const uint32_t*>(c->fIb.ptr);
     sv.inst = static_cast<const StkInstance*>(c->fInst.ptr);

@@ -234,11 +234,14 @@ hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n
     if (bvh.tlas) {
         if (flags & (STK_TRACE_REFERENCE_ORDER | STK_TRACE_BVH2)) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_trace_closest_tl, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
-    } else if (flags & STK_TRACE_REFERENCE_ORDER)
+    } else if (!bvh.packedOk && (flags & STK_TRACE_BVH2)) {
+        return hipErrorInvalidValue; // deeper than the packed walk's stack
+    } else if ((flags & STK_TRACE_REFERENCE_ORDER) || (!bvh.packedOk && bvh.wideRoot == STK_INVALID_INDEX)) {
         hipLaunchKernelGGL(k_trace_closest_ref, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
-    else
+    } else {
         hipLaunchKernelGGL(k_trace_closest, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits,
                            (flags & STK_TRACE_BVH2) ? 1 : 0);
+    }
     return hipGetLastError();
 }
 
@@ -249,7 +252,7 @@ hipError_t launchTraceAny(const BvhView& bvh, const StkRay* rays, uint32_t n, St
     if (bvh.tlas) {
         if (flags & STK_TRACE_REFERENCE_ORDER) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_trace_any_tl, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
-    } else if (flags & STK_TRACE_REFERENCE_ORDER)
+    } else if ((flags & STK_TRACE_REFERENCE_ORDER) || !bvh.packedOk)
         hipLaunchKernelGGL(k_trace_any_ref, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
     else
         hipLaunchKernelGGL(k_trace_any, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);

@@ -542,7 +542,8 @@ inline dim3 stripeGrid(const PathState& ps, uint32_t B) { return dim3(kStripes *
 template <int BLOCK, bool TL>
 void launchExtendWalk(const SceneView& sc, PathState& ps, int depth, const LaunchConfig& lc, hipStream_t s)
 {
-    const int binary = lc.binaryWalk ? 1 : 0;
+    // the binary walk only where its stack holds the tree
+    const int binary = (lc.binaryWalk && sc.bvh.packedOk) ? 1 : 0;
     if (lc.persistentBlocks)
         hipLaunchKernelGGL((k_pt_extend<BLOCK, TL>), dim3(lc.persistentBlocks), dim3(BLOCK), 0, s, sc, ps, depth, 1,
                            binary);

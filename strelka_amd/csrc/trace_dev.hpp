@@ -38,6 +38,7 @@ struct BvhView {
     const float4* emitters;       // 3 x float4 per emitter triangle
     uint32_t wideRoot;            // STK_INVALID_INDEX: no usable wide tree
     uint32_t emitCount;
+    uint32_t packedOk;            // 0: deeper than the packed walks' stack (kStackDepth)
     // two-level structure (tlas == nullptr: none); emitters then holds the
     // emitter instances' world triangles (leaf = mesh-tree leaf)
     const float4* tlas;           // 5 x float4 per StkWideNode, root at 0
