@@ -196,9 +196,15 @@ def instanced_bench(args) -> dict:
     spp = 4
 
     def measure(structure: str) -> dict:
+        if structure == "flat_device":
+            ctx.build_bvh_gpu()  # first call loads the build kernels
+            ctx.sync()
         t0 = time.perf_counter()
         if structure == "two_level":
             ctx.build_two_level()
+        elif structure == "flat_device":
+            ctx.build_bvh_gpu()
+            ctx.sync()
         else:
             ctx.build_bvh()
         build_s = time.perf_counter() - t0
@@ -235,6 +241,7 @@ def instanced_bench(args) -> dict:
 
     two = measure("two_level")
     flat = measure("flat")
+    flat_dev = measure("flat_device")
     for b in (d_out, d_rays, d_hits):
         ctx.free(b)
     ctx.close()
@@ -244,7 +251,7 @@ def instanced_bench(args) -> dict:
             "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded PCG meshes and transforms)",
             "config": {"workload": "10,000 instances of 20 seeded meshes under a tree of instances, 3840x2160, "
                                    "4 spp, 6 bounces", "triangles": ntri, "build_s": two["build_s"]},
-            "two_level": two, "flat": flat}
+            "two_level": two, "flat": flat, "flat_device_built": flat_dev}
 
 
 def load_traffic(kernel: str):
@@ -465,6 +472,26 @@ def main() -> int:
     pt["launch"] = tuned["best"]
     pt["autotune_1spp_ms"] = {"default": tuned.get("default_ms"), "best": tuned.get("ms")}
     ctx.free(d_out)
+
+    # the same scene with the tree built on the device (linear BVH): build
+    # time and closest-hit rate of the wide walk over it
+    ctx.build_bvh_gpu()
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.build_bvh_gpu()
+    ctx.sync()
+    dev_build_s = time.perf_counter() - t0
+    for _ in range(3):
+        ctx.trace_closest_device(d_rays, n, d_hits)
+    ctx.sync()
+    ctx.record(2)
+    for _ in range(args.steps):
+        ctx.trace_closest_device(d_rays, n, d_hits)
+    ctx.record(3)
+    ctx.sync()
+    dev_kernel_ms = ctx.elapsed_ms(2, 3) / args.steps
+    device_build = {"build_s": round(dev_build_s, 4), "wide_kernel_ms": round(dev_kernel_ms, 4),
+                    "mrays_per_s": round(n / (dev_kernel_ms * 1e-3) / 1e6, 2)}
     ctx.free(d_rays)
     ctx.free(d_hits)
 
@@ -519,6 +546,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "extra": {"reference_order_gpu_mrays_per_s": round(n / (ref_kernel_ms * 1e-3) / 1e6, 2),
                   "binary_bvh_gpu_mrays_per_s": round(n / (bin_kernel_ms * 1e-3) / 1e6, 2),
+                  "device_built_tree": device_build,
                   "path_tracer": pt},
     }
     print(json.dumps(out))
