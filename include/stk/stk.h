@@ -199,7 +199,11 @@ typedef struct StkLaunchConfig {
     uint32_t useGraph;
     uint32_t pathsInFlight;
     uint32_t binaryWalk; /* 1: packed binary-BVH walks instead of the compressed 8-wide ones */
-    uint32_t pad[2];
+    /* threads per stripe of the extend / shade / connect grids (0: one per
+     * possible queue entry); a capped grid loops over its stripe's entries,
+     * so late bounces with short queues launch few blocks */
+    uint32_t stripeThreads;
+    uint32_t pad[1];
 } StkLaunchConfig;
 STK_API stk_status stk_set_launch_config(stk_ctx* ctx, const StkLaunchConfig* lc);
 STK_API stk_status stk_get_launch_config(stk_ctx* ctx, StkLaunchConfig* lc);

@@ -5,7 +5,8 @@ The reference dispatches its compute passes with one fixed 16x16 workgroup
 per-device, per-workload choice: threads per block of the wavefront kernels,
 the raygen tile edge (spatial coherence of a wave's rays), a persistent grid
 for the extend kernel, the traversal (compressed 8-wide or packed binary
-BVH), the number of paths traced per pass (so a pass's path state fits the
+BVH), a cap on the threads per queue stripe of the consumer grids (fewer
+blocks that loop, for short late-bounce queues), the number of paths traced per pass (so a pass's path state fits the
 last-level cache) and graph replay of the pass.  None of them
 changes a single output bit (the launch-shape tests check that), so the tuner
 only measures time: it runs the caller's workload under each candidate with
@@ -25,9 +26,10 @@ BLOCK_SIZES = (64, 128, 256)
 TILES = (1, 4, 8, 16)
 PERSISTENT_PER_CU = (0, 4, 8, 16)
 PATHS_IN_FLIGHT = (0, 1 << 20, 1 << 22)
+STRIPE_THREADS_PER_CU = (0, 8, 16, 32)  # x CUs threads per queue stripe (0: one per entry)
 
 DEFAULT = {"block_size": 256, "tile": 8, "persistent_blocks": 0, "graph": True, "paths_in_flight": 0,
-           "binary_walk": False}
+           "binary_walk": False, "stripe_threads": 0}
 
 
 def _key(shape: dict) -> tuple:
@@ -60,6 +62,7 @@ def tune(ctx, run: Callable[[], None], total_paths: int, repeats: int = 2, sweep
         "paths_in_flight": [q for q in PATHS_IN_FLIGHT if q == 0 or q < total_paths],
         "graph": [True, False],
         "binary_walk": [False, True],
+        "stripe_threads": [k * cus for k in STRIPE_THREADS_PER_CU],
     }
     seen: dict = {}
 

@@ -342,6 +342,11 @@ bool packBvh(const StkVertex* vb, uint32_t nv, const uint32_t* ib, uint32_t ni, 
             continue;
         }
         ++ninner;
+        for (int a = 0; a < 3; ++a)
+            if (!std::isfinite(nd.minBounds[a]) || !std::isfinite(nd.maxBounds[a]) || nd.minBounds[a] > nd.maxBounds[a]) {
+                err = "BVH inner node " + std::to_string(it.idx) + " has a non-finite or inverted box";
+                return false;
+            }
         const uint32_t left = it.idx + 1;
         if (left >= n) {
             err = "BVH inner node without children";
@@ -467,6 +472,7 @@ uint8_t chooseExponent(float origin, float extent)
     const double need = std::max((double)extent / 253.0, std::ldexp(1.0, -100));
     const double ulp = std::fabs((double)origin) * std::ldexp(1.0, -25);
     const double target = std::max(need, ulp);
+    if (!(target < 1e300)) return 254; // non-finite boxes (rejected before the collapse)
     int e = (int)std::ceil(std::log2(target)) + 127;
     while (e > 1 && std::ldexp(1.0, e - 1 - 127) * 253.0 >= (double)extent && std::ldexp(1.0, e - 1 - 127) >= target)
         --e;
@@ -563,7 +569,7 @@ bool collapseWide(const std::vector<StkBvhNode>& nodes, LeafBox leafBox, EmitLea
                     if (slotUsed[sl]) continue;
                     double score = 0.0;
                     for (int a = 0; a < 3; ++a) score += ((sl >> a) & 1 ? d[a] : -d[a]);
-                    if (score > bestScore) {
+                    if (score > bestScore || bk < 0) { // the first free pair stands in for non-finite scores
                         bestScore = score;
                         bk = (int)k;
                         bs = sl;

@@ -767,7 +767,7 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
                                      (uint64_t)sv.scenes, sv.numScenes, sv.totalSlots, sv.tile, lp.maxDepth,
                                      lp.spp, lp.samplingType, lp.rrStartDepth, lp.refCompat, lp.passSlots,
                                      c->launch.blockSize, c->launch.tile, c->launch.persistentBlocks, c->ps.segCap,
-                                     c->launch.binaryWalk,
+                                     c->launch.binaryWalk, c->launch.stripeThreads,
                                      (uint64_t)accum, (uint64_t)samples, (uint64_t)c->ps.rayO, c->ps.capacity};
         if (!c->graphExec || key != c->graphKey) {
             invalidateGraph(c);
@@ -1105,7 +1105,7 @@ STK_API stk_status stk_batch_info(stk_ctx* c, uint32_t scene, StkBatchViewInfo* 
     out->numTriangles = d.hdr.numTriangles;
     out->bvhDepth = c->batch[scene].bvh.depth;
     out->numScenes = (uint32_t)c->batchDescs.size();
-    out->pad[0] = out->pad[1] = 0;
+    memset(out->pad, 0, sizeof(out->pad));
     return STK_OK;
 }
 
@@ -1150,6 +1150,7 @@ STK_API stk_status stk_set_launch_config(stk_ctx* c, const StkLaunchConfig* lc)
     c->launch.persistentBlocks = lc->persistentBlocks;
     c->launch.pathsInFlight = lc->pathsInFlight;
     c->launch.binaryWalk = lc->binaryWalk ? 1u : 0u;
+    c->launch.stripeThreads = lc->stripeThreads;
     c->useGraph = lc->useGraph != 0;
     invalidateGraph(c);
     return STK_OK;
@@ -1182,7 +1183,8 @@ STK_API stk_status stk_get_launch_config(stk_ctx* c, StkLaunchConfig* lc)
     lc->useGraph = c->useGraph ? 1u : 0u;
     lc->pathsInFlight = c->launch.pathsInFlight;
     lc->binaryWalk = c->launch.binaryWalk;
-    lc->pad[0] = lc->pad[1] = 0;
+    lc->stripeThreads = c->launch.stripeThreads;
+    memset(lc->pad, 0, sizeof(lc->pad));
     return STK_OK;
 }
 

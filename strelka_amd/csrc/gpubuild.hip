@@ -177,7 +177,7 @@ __device__ __forceinline__ int prefix(const uint64_t* K, int n, int a, int b)
 
 __global__ __launch_bounds__(kB) void k_radix_tree(const uint64_t* __restrict__ K, int n, uint32_t* __restrict__ childL,
                                                    uint32_t* __restrict__ childR, uint32_t* __restrict__ parentInner,
-                                                   uint32_t* __restrict__ parentLeaf)
+                                                   uint32_t* __restrict__ parentLeaf, uint32_t* __restrict__ count)
 {
     const int i = (int)(blockIdx.x * kB + threadIdx.x);
     if (i >= n - 1) return;
@@ -201,6 +201,7 @@ __global__ __launch_bounds__(kB) void k_radix_tree(const uint64_t* __restrict__ 
     const uint32_t right = (max(i, j) == gamma + 1) ? ((uint32_t)(gamma + 1) | kLeafBit) : (uint32_t)(gamma + 1);
     childL[i] = left;
     childR[i] = right;
+    count[i] = (uint32_t)(max(i, j) - min(i, j) + 1); // node i covers the sorted leaves [min(i, j), max(i, j)]
     if (left & kLeafBit)
         parentLeaf[left & ~kLeafBit] = (uint32_t)i;
     else
@@ -218,36 +219,70 @@ __global__ __launch_bounds__(kB) void k_refit(uint32_t n, const uint32_t* __rest
                                               const uint32_t* __restrict__ childL, const uint32_t* __restrict__ childR,
                                               const uint32_t* __restrict__ parentInner,
                                               const uint32_t* __restrict__ parentLeaf, float4* nodeLo, float4* nodeHi,
-                                              uint32_t* count, uint32_t* flags)
+                                              uint32_t* flags)
 {
     const uint32_t k = blockIdx.x * kB + threadIdx.x;
     if (k >= n) return;
     uint32_t p = parentLeaf[k];
     while (p != kNone) {
-        // the second child to arrive finishes p; acquire/release at agent
-        // scope makes the first child's box visible across the XCDs' caches
-        const uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        // The second child to arrive finishes p.  Full agent-scope fences on
+        // both sides of the arrival: this lane's box stores have completed
+        // and are written back before it, and the sibling's are read after
+        // it, across the XCDs' caches.
+        __threadfence();
+        const uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (old == 0) return;
+        __threadfence();
         float4 lo[2], hi[2];
-        uint32_t cnt = 0;
         const uint32_t kids[2] = {childL[p], childR[p]};
         for (int c = 0; c < 2; ++c) {
             if (kids[c] & kLeafBit) {
                 const uint32_t t = vals[kids[c] & ~kLeafBit];
                 lo[c] = triLo[t];
                 hi[c] = triHi[t];
-                cnt += 1;
             } else {
                 lo[c] = nodeLo[kids[c]];
                 hi[c] = nodeHi[kids[c]];
-                cnt += count[kids[c]];
             }
         }
         nodeLo[p] = make_float4(fminf(lo[0].x, lo[1].x), fminf(lo[0].y, lo[1].y), fminf(lo[0].z, lo[1].z), 0.0f);
         nodeHi[p] = make_float4(fmaxf(hi[0].x, hi[1].x), fmaxf(hi[0].y, hi[1].y), fmaxf(hi[0].z, hi[1].z), 0.0f);
-        count[p] = cnt;
         p = parentInner[p];
     }
+}
+
+// Checks the finished inner nodes: finite boxes holding their children's, and
+// leaf counts adding up; any failure sets *error (the build then fails
+// instead of emitting a tree the walks would mistrust).
+__global__ __launch_bounds__(kB) void k_check_inner(uint32_t ninner, const uint32_t* __restrict__ vals,
+                                                    const float4* __restrict__ triLo, const float4* __restrict__ triHi,
+                                                    const uint32_t* __restrict__ childL,
+                                                    const uint32_t* __restrict__ childR,
+                                                    const float4* __restrict__ nodeLo, const float4* __restrict__ nodeHi,
+                                                    const uint32_t* __restrict__ count, uint32_t* error)
+{
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= ninner) return;
+    const float4 lo = nodeLo[i], hi = nodeHi[i];
+    bool ok = isfinite(lo.x) && isfinite(lo.y) && isfinite(lo.z) && isfinite(hi.x) && isfinite(hi.y) &&
+              isfinite(hi.z) && lo.x <= hi.x && lo.y <= hi.y && lo.z <= hi.z;
+    uint32_t cnt = 0;
+    const uint32_t kids[2] = {childL[i], childR[i]};
+    for (int c = 0; c < 2; ++c) {
+        float4 l, h;
+        if (kids[c] & kLeafBit) {
+            const uint32_t t = vals[kids[c] & ~kLeafBit];
+            l = triLo[t];
+            h = triHi[t];
+            cnt += 1;
+        } else {
+            l = nodeLo[kids[c]];
+            h = nodeHi[kids[c]];
+            cnt += count[kids[c]];
+        }
+        ok = ok && l.x >= lo.x && l.y >= lo.y && l.z >= lo.z && h.x <= hi.x && h.y <= hi.y && h.z <= hi.z;
+    }
+    if (!ok || cnt != count[i]) atomicOr(error, 2u);
 }
 
 // 6. preorder, one level of inner nodes per launch
@@ -396,6 +431,7 @@ __device__ uint32_t choose_exponent(float origin, float extent)
     const double need = fmax((double)extent / 253.0, ldexp(1.0, -100));
     const double ulp = fabs((double)origin) * ldexp(1.0, -25);
     const double target = fmax(need, ulp);
+    if (!(target < 1e300)) return 254; // non-finite boxes are rejected by the inner-node check
     int e = (int)ceil(log2(target)) + 127;
     while (e > 1 && ldexp(1.0, e - 1 - 127) * 253.0 >= (double)extent && ldexp(1.0, e - 1 - 127) >= target) --e;
     while (ldexp(1.0, e - 127) < target) ++e;
@@ -459,7 +495,7 @@ __global__ __launch_bounds__(64) void k_wide_plan(const uint32_t* __restrict__ j
                 if ((slotUsed >> sl) & 1u) continue;
                 double score = 0.0;
                 for (int a = 0; a < 3; ++a) score += ((sl >> a) & 1 ? d[a] : -d[a]);
-                if (score > bestScore) {
+                if (score > bestScore || bk < 0) { // the first free pair stands in for non-finite scores
                     bestScore = score;
                     bk = k;
                     bs = sl;
@@ -619,11 +655,21 @@ hipError_t gpuBuildBvh(const StkVertex* vb, const uint32_t* ib, const StkInstanc
         hipLaunchKernelGGL(k_set_u32, dim3(std::min<uint32_t>(blocksFor(ninner), 1024)), dim3(kB), 0, s, parentLeaf,
                            kNone, (size_t)n);
         hipLaunchKernelGGL(k_radix_tree, dim3(blocksFor(ninner)), dim3(kB), 0, s, keys, (int)n, childL, childR,
-                           parentInner, parentLeaf);
+                           parentInner, parentLeaf, count);
         hipLaunchKernelGGL(k_set_u32, dim3(std::min<uint32_t>(blocksFor(ninner), 1024)), dim3(kB), 0, s, flags, 0u,
                            (size_t)ninner);
         hipLaunchKernelGGL(k_refit, dim3(blocksFor(n)), dim3(kB), 0, s, n, vals, triLo, triHi, childL, childR,
-                           parentInner, parentLeaf, nodeLo, nodeHi, count, flags);
+                           parentInner, parentLeaf, nodeLo, nodeHi, flags);
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(kB), 0, s, counters + 2, 0u, (size_t)1);
+        hipLaunchKernelGGL(k_check_inner, dim3(blocksFor(ninner)), dim3(kB), 0, s, ninner, vals, triLo, triHi, childL,
+                           childR, nodeLo, nodeHi, count, counters + 2);
+        uint32_t checked = 0;
+        if ((e = hipMemcpyAsync(&checked, counters + 2, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        if (checked != 0) {
+            err = "inner node boxes or leaf counts are inconsistent";
+            return hipErrorInvalidValue;
+        }
         // preorder, level by level from the root (pre[0] = 0)
         STK_GB_ALLOC(uint32_t, frontA, ninner, temp((size_t)ninner * 4))
         STK_GB_ALLOC(uint32_t, frontB, ninner, temp((size_t)ninner * 4))
@@ -668,7 +714,7 @@ hipError_t gpuBuildBvh(const StkVertex* vb, const uint32_t* ib, const StkInstanc
         return e;
     STK_GB_ALLOC(void, scanTemp, scanBytes, temp(scanBytes))
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(kB), 0, s, jobsA, n > 1 ? 0u : kLeafBit, (size_t)1);
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(kB), 0, s, counters + 2, 0u, (size_t)1);
+    if (n == 1) hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(kB), 0, s, counters + 2, 0u, (size_t)1);
     uint32_t nj = 1, levelStart = 0, triStart = 0, wideDepth = 0;
     uint32_t* jc = jobsA;
     uint32_t* jn = jobsB;
@@ -683,12 +729,16 @@ hipError_t gpuBuildBvh(const StkVertex* vb, const uint32_t* ib, const StkInstanc
             return e;
         hipLaunchKernelGGL(k_wide_emit, dim3(blocksFor(nj)), dim3(kB), 0, s, nj, levelStart, triStart, plan, kidRefs,
                            innerOff, leafOff, tris, wide, wideTris, jn);
+        // (on a failed copy, wait for the ones queued before returning: they
+        // write this frame's stack)
         uint32_t last[4];
-        if ((e = hipMemcpyAsync(&last[0], innerOff + nj - 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(&last[1], innerCnt + nj - 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(&last[2], leafOff + nj - 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(&last[3], leafCnt + nj - 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        e = hipMemcpyAsync(&last[0], innerOff + nj - 1, 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(&last[1], innerCnt + nj - 1, 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(&last[2], leafOff + nj - 1, 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(&last[3], leafCnt + nj - 1, 4, hipMemcpyDeviceToHost, s);
+        const hipError_t es = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+        if ((e = es) != hipSuccess) return e;
         ++wideDepth;
         levelStart += nj;
         triStart += last[2] + last[3];
@@ -700,13 +750,17 @@ hipError_t gpuBuildBvh(const StkVertex* vb, const uint32_t* ib, const StkInstanc
         std::swap(jc, jn);
     }
     uint32_t tail[3];
-    if ((e = hipMemcpyAsync(tail, counters, 12, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
     float4 rootBox[2] = {};
-    if (n > 1) {
-        if ((e = hipMemcpyAsync(&rootBox[0], nodeLo, 16, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(&rootBox[1], nodeHi, 16, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    e = hipMemcpyAsync(tail, counters, 12, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && n > 1) e = hipMemcpyAsync(&rootBox[0], nodeLo, 16, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && n > 1) e = hipMemcpyAsync(&rootBox[1], nodeHi, 16, hipMemcpyDeviceToHost, s);
+    const hipError_t es = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    if ((e = es) != hipSuccess) return e;
+    if (tail[2] & 2u) {
+        err = "inner node boxes or leaf counts are inconsistent";
+        return hipErrorInvalidValue;
     }
-    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
     if (tail[2] != 0) {
         err = "wide collapse: a child box does not fit its node's quantisation grid";
         return hipErrorInvalidValue;

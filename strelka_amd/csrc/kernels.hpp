@@ -129,6 +129,7 @@ struct LaunchConfig {
     uint32_t persistentBlocks = 0; // extend grid for persistent waves (0: one lane per ray)
     uint32_t pathsInFlight = 0;    // slots per pass (0: every slot of the frame in one pass)
     uint32_t binaryWalk = 0;       // 1: packed BVH2 walks instead of the compressed 8-wide ones
+    uint32_t stripeThreads = 0;    // threads per stripe of the consumer grids (0: one per entry)
 };
 
 // flags: STK_TRACE_REFERENCE_ORDER (skip-link walk), STK_TRACE_BVH2 (packed

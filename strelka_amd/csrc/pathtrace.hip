@@ -83,15 +83,13 @@ __global__ void k_pass_reset(PathState ps)
     }
 }
 
-// Entry `e` of a stripe-major consumer grid of B-thread blocks: block b reads
-// segment b % kStripes; false past the segment's count.
-__device__ __forceinline__ bool stripe_entry(const PathState& ps, const uint32_t* counts, uint32_t B, uint32_t& e)
-{
-    const uint32_t stripe = blockIdx.x % kStripes;
-    const uint32_t local = (blockIdx.x / kStripes) * B + threadIdx.x;
-    e = stripe * ps.segCap + local;
-    return local < counts[stripe];
-}
+// Stripe-major consumer grids of B-thread blocks: block b reads segment
+// b % kStripes, entries (b / kStripes) * B + lane, then every
+// (gridDim / kStripes) * B further while the segment has entries (a grid
+// capped below one thread per possible entry loops).  The loop bound is
+// uniform over the block, so wave-wide appends inside see whole waves.
+__device__ __forceinline__ uint32_t stripe_first(uint32_t B) { return (blockIdx.x / kStripes) * B; }
+__device__ __forceinline__ uint32_t stripe_step(uint32_t B) { return (gridDim.x / kStripes) * B; }
 
 // Sum over a 64-thread block (one wave).
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
@@ -221,8 +219,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) v
     const int cur = depth & 1;
     const uint32_t* counts = ps.stripes + cur * kStripes;
     if (!persistent) {
-        uint32_t e;
-        if (stripe_entry(ps, counts, BLOCK, e)) extend_one<BLOCK, TL>(sc, ps, cur, e, stack + threadIdx.x, binary);
+        const uint32_t stripe = blockIdx.x % kStripes, cnt = counts[stripe];
+        for (uint32_t base = stripe_first(BLOCK); base < cnt; base += stripe_step(BLOCK)) {
+            const uint32_t local = base + threadIdx.x;
+            if (local < cnt)
+                extend_one<BLOCK, TL>(sc, ps, cur, stripe * ps.segCap + local, stack + threadIdx.x, binary);
+        }
         return;
     }
     const uint32_t lane = threadIdx.x % 64;
@@ -242,11 +244,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) v
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, PathState ps, int depth)
+// One queue entry of the shade step (active: q is a valid entry; every lane of
+// the block calls it, for the wave-wide appends at the end).
+__device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& lp, PathState& ps, int depth, bool active,
+                                          uint32_t q)
 {
     const int cur = depth & 1;
-    uint32_t q;
-    const bool active = stripe_entry(ps, ps.stripes + cur * kStripes, kBlock, q);
     bool cont = false, shadow = false, cacheQuery = false;
     uint32_t slot = 0;
     F3 nO = mk3(0, 0, 0), nD = mk3(0, 0, 0);
@@ -422,20 +425,23 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, 
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, PathState ps, int depth)
+{
+    const uint32_t stripe = blockIdx.x % kStripes, cnt = ps.stripes[(depth & 1) * kStripes + stripe];
+    for (uint32_t base = stripe_first(kBlock); base < cnt; base += stripe_step(kBlock)) {
+        const uint32_t local = base + threadIdx.x;
+        shade_one(sc, lp, ps, depth, local < cnt, stripe * ps.segCap + local);
+    }
+}
+
 // Emitter triangles a shadow ray tests before the wide walk (beyond this the
 // scene's shadow rays take the skip-link walk).
 constexpr uint32_t kMaxEmitters = 64;
 
-// Shadow rays: the wide visibility walk (the reference's first-in-preorder
-// any-hit rule, decided in any order), or the stackless skip-link walk when
-// the scene has no usable wide tree or many emitter triangles; TL: the
-// two-level visibility walk.
+// One shadow-ray entry s of the connect step.
 template <bool TL>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pt_connect(SceneView sc, PathState ps, int binary)
+__device__ __forceinline__ void connect_one(const SceneView& sc, PathState& ps, uint32_t s, uint32_t* stack, int binary)
 {
-    __shared__ uint32_t stack[2 * kWideStack * kBlock];
-    uint32_t s;
-    if (!stripe_entry(ps, ps.stripes + 2 * kStripes, kBlock, s)) return;
     const uint32_t slot = ps.shadowQ[s];
     const SceneDesc& desc = sc.scenes[ps.scene[slot]];
     const RayF r = make_ray(ps.shO[s], ps.shD[s]);
@@ -458,6 +464,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
         ps.col[slot] = f4(col);
     }
 }
+
+// Shadow rays: the wide visibility walk (the reference's first-in-preorder
+// any-hit rule, decided in any order), or the stackless skip-link walk when
+// the scene has no usable wide tree or many emitter triangles; TL: the
+// two-level visibility walk.
+template <bool TL>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pt_connect(SceneView sc, PathState ps, int binary)
+{
+    __shared__ uint32_t stack[2 * kWideStack * kBlock];
+    const uint32_t stripe = blockIdx.x % kStripes, cnt = ps.stripes[2 * kStripes + stripe];
+    for (uint32_t base = stripe_first(kBlock); base < cnt; base += stripe_step(kBlock)) {
+        const uint32_t local = base + threadIdx.x;
+        if (local < cnt) connect_one<TL>(sc, ps, stripe * ps.segCap + local, stack, binary);
+    }
+}
+
 
 // Adds the cache's answer to the paths that ended at a query:
 // radiance += throughput * albedo * max(prediction, 0).
@@ -537,7 +559,12 @@ __global__ __launch_bounds__(kBlock) void k_cache_targets(PtLaunch lp, PathState
 inline uint32_t blocks(size_t n, uint32_t b = kBlock) { return (uint32_t)((n + b - 1) / b); }
 
 // Stripe-major consumer grid of B-thread blocks (segCap is a multiple of 256).
-inline dim3 stripeGrid(const PathState& ps, uint32_t B) { return dim3(kStripes * (ps.segCap / B)); }
+inline dim3 stripeGrid(const PathState& ps, uint32_t B, uint32_t stripeThreads = 0)
+{
+    uint32_t perStripe = ps.segCap / B;
+    if (stripeThreads) perStripe = std::min(perStripe, std::max(1u, stripeThreads / B));
+    return dim3(kStripes * perStripe);
+}
 
 template <int BLOCK, bool TL>
 void launchExtendWalk(const SceneView& sc, PathState& ps, int depth, const LaunchConfig& lc, hipStream_t s)
@@ -548,7 +575,8 @@ void launchExtendWalk(const SceneView& sc, PathState& ps, int depth, const Launc
         hipLaunchKernelGGL((k_pt_extend<BLOCK, TL>), dim3(lc.persistentBlocks), dim3(BLOCK), 0, s, sc, ps, depth, 1,
                            binary);
     else
-        hipLaunchKernelGGL((k_pt_extend<BLOCK, TL>), stripeGrid(ps, BLOCK), dim3(BLOCK), 0, s, sc, ps, depth, 0,
+        hipLaunchKernelGGL((k_pt_extend<BLOCK, TL>), stripeGrid(ps, BLOCK, lc.stripeThreads), dim3(BLOCK), 0, s, sc,
+                           ps, depth, 0,
                            binary);
 }
 
@@ -594,11 +622,12 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
         case 128: launchExtend<128>(sc, ps, depth, lc, s); break;
         default: launchExtend<256>(sc, ps, depth, lc, s); break;
         }
-        hipLaunchKernelGGL(k_pt_shade, stripeGrid(ps, kBlock), b, 0, s, sc, lp, ps, depth);
+        hipLaunchKernelGGL(k_pt_shade, stripeGrid(ps, kBlock, lc.stripeThreads), b, 0, s, sc, lp, ps, depth);
         if (sc.bvh.tlas)
-            hipLaunchKernelGGL(k_pt_connect<true>, stripeGrid(ps, kBlock), b, 0, s, sc, ps, 0);
+            hipLaunchKernelGGL(k_pt_connect<true>, stripeGrid(ps, kBlock, lc.stripeThreads), b, 0, s, sc, ps, 0);
         else
-            hipLaunchKernelGGL(k_pt_connect<false>, stripeGrid(ps, kBlock), b, 0, s, sc, ps, lc.binaryWalk ? 1 : 0);
+            hipLaunchKernelGGL(k_pt_connect<false>, stripeGrid(ps, kBlock, lc.stripeThreads), b, 0, s, sc, ps,
+                               lc.binaryWalk ? 1 : 0);
     }
     hipLaunchKernelGGL(k_pt_tally, dim3(1), dim3(64), 0, s, ps);
     if (query && cb) {

@@ -89,3 +89,34 @@ def test_reference_icosphere_faces():
     # last column emits (10,2,1) and (10,11,3) (src/scene/scene.cpp:275-304)
     assert fixed.inst[0]["indexCount"] == ref.inst[0]["indexCount"] == 3840
     assert fixed.ib.tobytes() != ref.ib.tobytes()
+
+
+HEADER_STRUCTS = {
+    "StkPathTracerParam": "PathTracerParam", "StkRenderParams": "RenderParams", "StkCacheConfig": "CacheConfig",
+    "StkCacheInfo": "CacheInfo", "StkBvhOptions": "BvhOptions", "StkBvhHeader": "BvhHeader",
+    "StkLaunchConfig": "LaunchConfig", "StkDeviceInfo": "DeviceInfo", "StkBatchScene": "BatchScene",
+    "StkBatchViewInfo": "BatchViewInfo", "StkImageDesc": "ImageDesc",
+}
+
+
+def test_ctypes_records_match_the_header(tmp_path):
+    """Every ctypes record has the size and field offsets the C compiler gives
+    the header's struct, so no entry point writes past a caller's record."""
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "stk/stk.h"', "int main(void) {"]
+    for c_name, py_name in HEADER_STRUCTS.items():
+        lines.append(f'printf("{c_name} %zu\\n", sizeof({c_name}));')
+        for f, *_ in getattr(_abi, py_name)._fields_:
+            lines.append(f'printf("{c_name}.{f} %zu\\n", offsetof({c_name}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "sizes.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "sizes"
+    import subprocess
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    for c_name, py_name in HEADER_STRUCTS.items():
+        cls = getattr(_abi, py_name)
+        assert int(got[c_name]) == C.sizeof(cls), c_name
+        for f, *_ in cls._fields_:
+            assert int(got[f"{c_name}.{f}"]) == getattr(cls, f).offset, (c_name, f)

@@ -169,3 +169,20 @@ def test_instanced_scene_shape():
     assert np.all(np.abs(np.linalg.det(R) - 1.0) < 1e-5)
     assert np.all(np.abs(m[:, :3, 3]) <= 100.0)
     assert len(np.unique(inst["indexOffset"])) == 20
+
+
+def test_non_finite_inner_box_is_rejected():
+    """A skip-link array whose structure is valid but an inner box is NaN must
+    be refused with an error (the wide collapse would otherwise assign slots
+    from non-finite scores)."""
+    s = sa.SceneData.generate(1)
+    nodes = sa.HostBvh.build(s).nodes.copy()
+    inner = np.nonzero(nodes["instId"] == -1)[0]
+    nodes["minBounds"][inner[3]] = np.nan
+    nodes["maxBounds"][inner[3]] = np.nan
+    with pytest.raises(sa.StkError):
+        sa.HostBvh.from_nodes(s, nodes)
+    nodes = sa.HostBvh.build(s).nodes.copy()
+    nodes["maxBounds"][inner[4], 0] = np.inf
+    with pytest.raises(sa.StkError):
+        sa.HostBvh.from_nodes(s, nodes)

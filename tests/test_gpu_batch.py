@@ -7,6 +7,8 @@ raygen tile, persistent extend grid, graph replay) must give the same image.
 The single-scene renders are themselves checked against the CPU oracle
 (test_gpu_pathtrace.py), which pins the batch to the oracle by transitivity;
 one view is also checked against the oracle directly here."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -97,7 +99,9 @@ SHAPES = [dict(block_size=64, tile=1), dict(block_size=128, tile=16), dict(block
           dict(block_size=256, tile=8, paths_in_flight=4096),
           dict(block_size=128, tile=8, paths_in_flight=100000, persistent_blocks=1024, graph=True),
           dict(block_size=128, tile=8, persistent_blocks=8192, graph=True),
-          dict(block_size=64, tile=1, persistent_blocks=1, paths_in_flight=20000)]
+          dict(block_size=64, tile=1, persistent_blocks=1, paths_in_flight=20000),
+          dict(block_size=64, tile=8, stripe_threads=64), dict(block_size=256, tile=8, stripe_threads=2048, graph=True),
+          dict(block_size=128, tile=4, stripe_threads=4096, paths_in_flight=20000, persistent_blocks=512)]
 
 
 def test_launch_shapes_give_identical_images(scenes):
@@ -119,6 +123,21 @@ def test_launch_shapes_give_identical_images(scenes):
         else:
             assert np.array_equal(img, base), shape
             assert totals == base_totals, shape
+    ctx.close()
+
+
+def test_record_queries_write_only_their_record(scenes):
+    """stk_get_launch_config and stk_batch_info fill the caller's record and
+    nothing after it (a canary block follows each record)."""
+    print("case: launch-config and view-info records followed by canaries", flush=True)
+    ctx = batch_context(scenes)
+    ctx.set_launch_config(block_size=128, tile=4, stripe_threads=64)
+    for cls, call in ((sa._abi.LaunchConfig, lambda p: sa.lib().stk_get_launch_config(ctx._h, p)),
+                      (sa._abi.BatchViewInfo, lambda p: sa.lib().stk_batch_info(ctx._h, 1, p))):
+        n = C.sizeof(cls)
+        buf = (C.c_uint8 * (n + 64))(*([0xA5] * (n + 64)))
+        assert call(C.cast(buf, C.POINTER(cls))) == 0
+        assert bytes(buf[n:]) == b"\xa5" * 64, cls.__name__
     ctx.close()
 
 
