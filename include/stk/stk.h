@@ -203,7 +203,11 @@ typedef struct StkLaunchConfig {
      * possible queue entry); a capped grid loops over its stripe's entries,
      * so late bounces with short queues launch few blocks */
     uint32_t stripeThreads;
-    uint32_t pad[1];
+    /* 1..4 (0 means 1): with n > 1, pass k of a frame runs on stream k % n,
+     * each stream with its own path state, so that one pass's last bounces
+     * overlap the next passes' first ones; the passes still add into the
+     * image in sample order (plain rendering only; no graph replay then) */
+    uint32_t passStreams;
 } StkLaunchConfig;
 STK_API stk_status stk_set_launch_config(stk_ctx* ctx, const StkLaunchConfig* lc);
 STK_API stk_status stk_get_launch_config(stk_ctx* ctx, StkLaunchConfig* lc);

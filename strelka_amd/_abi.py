@@ -90,7 +90,7 @@ class BvhHeader(C.Structure):
 class LaunchConfig(C.Structure):
     _fields_ = [("blockSize", C.c_uint32), ("tile", C.c_uint32), ("persistentBlocks", C.c_uint32),
                 ("useGraph", C.c_uint32), ("pathsInFlight", C.c_uint32), ("binaryWalk", C.c_uint32),
-                ("stripeThreads", C.c_uint32), ("pad", C.c_uint32 * 1)]
+                ("stripeThreads", C.c_uint32), ("passStreams", C.c_uint32)]
 
 
 class DeviceInfo(C.Structure):

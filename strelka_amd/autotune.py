@@ -6,8 +6,10 @@ per-device, per-workload choice: threads per block of the wavefront kernels,
 the raygen tile edge (spatial coherence of a wave's rays), a persistent grid
 for the extend kernel, the traversal (compressed 8-wide or packed binary
 BVH), a cap on the threads per queue stripe of the consumer grids (fewer
-blocks that loop, for short late-bounce queues), the number of paths traced per pass (so a pass's path state fits the
-last-level cache) and graph replay of the pass.  None of them
+blocks that loop, for short late-bounce queues), the number of paths traced
+per pass (so a pass's path state fits the last-level cache), the number of
+pass streams (overlapping a pass's last bounces with the next passes) and
+graph replay of the pass.  None of them
 changes a single output bit (the launch-shape tests check that), so the tuner
 only measures time: it runs the caller's workload under each candidate with
 device events on the context stream and keeps the fastest.
@@ -29,7 +31,7 @@ PATHS_IN_FLIGHT = (0, 1 << 20, 1 << 22)
 STRIPE_THREADS_PER_CU = (0, 8, 16, 32)  # x CUs threads per queue stripe (0: one per entry)
 
 DEFAULT = {"block_size": 256, "tile": 8, "persistent_blocks": 0, "graph": True, "paths_in_flight": 0,
-           "binary_walk": False, "stripe_threads": 0}
+           "binary_walk": False, "stripe_threads": 0, "pass_streams": 1}
 
 
 def _key(shape: dict) -> tuple:
@@ -63,6 +65,7 @@ def tune(ctx, run: Callable[[], None], total_paths: int, repeats: int = 2, sweep
         "graph": [True, False],
         "binary_walk": [False, True],
         "stripe_threads": [k * cus for k in STRIPE_THREADS_PER_CU],
+        "pass_streams": [1, 2, 3],
     }
     seen: dict = {}
 

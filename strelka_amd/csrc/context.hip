@@ -23,6 +23,9 @@
 
 namespace {
 
+// Most streams a render call spreads its passes over (StkLaunchConfig::passStreams).
+constexpr int kMaxPassStreams = 4;
+
 // A device allocation owned by a context.
 struct DevBuf {
     void* ptr = nullptr;
@@ -78,9 +81,21 @@ struct stk_ctx {
     DevBuf tlasWide, tlasLeaves, blasWide, blasTris, instKey, instW, tlEmit;
     DevBuf rayScratch, hitScratch;
 
-    // render state
+    // render state.  With passStreams = n > 1, pass k of a render call runs on
+    // pass stream k % n with that stream's path state: stream 0 is `stream`
+    // with `ps`, streams 1.. are xStream[i - 1] with xPs[i - 1].  finEv[i]
+    // marks the finalize of the last pass on stream i, joinEv[i] the joins.
     stk::PathState ps;
     std::vector<DevBuf> psBufs;
+    stk::PathState xPs[kMaxPassStreams - 1];
+    std::vector<DevBuf> xPsBufs[kMaxPassStreams - 1];
+    hipStream_t xStream[kMaxPassStreams - 1] = {};
+    hipEvent_t finEv[kMaxPassStreams] = {}, joinEv[kMaxPassStreams] = {};
+    uint32_t lastStreams = 1;    // pass streams of the last render call
+    uint32_t lastPassStream = 0; // stream of its last pass
+
+    stk::PathState& passState(uint32_t i) { return i ? xPs[i - 1] : ps; }
+    hipStream_t passStream(uint32_t i) const { return i ? xStream[i - 1] : stream; }
     DevBuf accum, history, out, samples, descs, staging;
     uint32_t iteration = 0;
     uint64_t imageKey = 0;
@@ -279,11 +294,21 @@ STK_API void stk_destroy(stk_ctx* c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (hipStream_t s : c->xStream)
+        if (s) (void)hipStreamSynchronize(s);
     for (DevBuf* b : c->owned()) devRelease(c, *b);
     for (DevBuf& b : c->psBufs) devRelease(c, b);
+    for (auto& bufs : c->xPsBufs)
+        for (DevBuf& b : bufs) devRelease(c, b);
     invalidateGraph(c);
     for (auto& ev : c->events)
         if (ev) (void)hipEventDestroy(ev);
+    for (int i = 0; i < kMaxPassStreams; ++i) {
+        if (c->finEv[i]) (void)hipEventDestroy(c->finEv[i]);
+        if (c->joinEv[i]) (void)hipEventDestroy(c->joinEv[i]);
+    }
+    for (hipStream_t s : c->xStream)
+        if (s) (void)hipStreamDestroy(s);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -616,15 +641,16 @@ STK_API stk_status stk_generate_camera_rays_device(stk_ctx* c, const StkPathTrac
 
 // ---- rendering ------------------------------------------------------------------
 
-static stk_status ensurePathState(stk_ctx* c, size_t slots)
+static stk_status ensurePathState(stk_ctx* c, stk::PathState& ps, std::vector<DevBuf>& bufs, size_t slots)
 {
     const size_t qcap = (size_t)stk::kStripes * stk::stripeSegment(slots);
-    if (c->ps.capacity >= slots && c->ps.queueCapacity >= qcap) return STK_OK;
+    if (ps.capacity >= slots && ps.queueCapacity >= qcap) return STK_OK;
     invalidateGraph(c);
     stk_status st;
-    for (DevBuf& b : c->psBufs) devRelease(c, b);
-    c->psBufs.assign(20, DevBuf{});
-    stk::PathState& ps = c->ps;
+    for (hipStream_t s : c->xStream)
+        if (s) (void)hipStreamSynchronize(s);
+    for (DevBuf& b : bufs) devRelease(c, b);
+    bufs.assign(20, DevBuf{});
     ps.capacity = 0;
     ps.queueCapacity = 0;
     const size_t f4 = slots * sizeof(float4), u4 = slots * sizeof(uint32_t);
@@ -638,8 +664,8 @@ static stk_status ensurePathState(stk_ctx* c, size_t slots)
                             stk::kQueueKinds * stk::kStripes * sizeof(uint32_t), stk::kStripes * sizeof(uint32_t)};
     static_assert(sizeof(sizes) / sizeof(sizes[0]) == sizeof(ptrs) / sizeof(ptrs[0]), "path state buffers");
     for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); ++i) {
-        if ((st = devReserve(c, c->psBufs[i], sizes[i])) != STK_OK) return st;
-        *ptrs[i] = c->psBufs[i].ptr;
+        if ((st = devReserve(c, bufs[i], sizes[i])) != STK_OK) return st;
+        *ptrs[i] = bufs[i].ptr;
     }
     ps.capacity = (uint32_t)slots;
     ps.queueCapacity = qcap;
@@ -700,7 +726,19 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
     size_t Q = slots;
     if (c->launch.pathsInFlight && p->cacheMode == 0)
         Q = std::min(slots, ((size_t)c->launch.pathsInFlight + 63) / 64 * 64);
-    if ((st = ensurePathState(c, Q)) != STK_OK) return st;
+    if ((st = ensurePathState(c, c->ps, c->psBufs, Q)) != STK_OK) return st;
+    // several pass streams: plain rendering only (the cache's training passes
+    // depend on each other)
+    const uint32_t nstreams = p->cacheMode == 0 ? std::max<uint32_t>(1u, c->launch.passStreams) : 1u;
+    for (uint32_t i = 1; i < nstreams; ++i) {
+        if ((st = ensurePathState(c, c->xPs[i - 1], c->xPsBufs[i - 1], Q)) != STK_OK) return st;
+        if (!c->xStream[i - 1]) STK_HIP(c, hipStreamCreateWithFlags(&c->xStream[i - 1], hipStreamNonBlocking));
+    }
+    if (nstreams > 1 && !c->finEv[0])
+        for (int i = 0; i < kMaxPassStreams; ++i) {
+            STK_HIP(c, hipEventCreateWithFlags(&c->finEv[i], hipEventDisableTiming));
+            STK_HIP(c, hipEventCreateWithFlags(&c->joinEv[i], hipEventDisableTiming));
+        }
     if ((st = ensureImage(c, imageKey, npix, nsamples)) != STK_OK) return st;
     const size_t descBytes = descs.size() * sizeof(stk::SceneDesc);
     if (c->descs.bytes < descBytes) invalidateGraph(c);
@@ -719,6 +757,7 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
     lp.invSpp = 1.0f / (float)p->pt.spp;
     lp.passSlots = (uint32_t)Q;
     c->ps.segCap = stk::stripeSegment(Q);
+    for (stk::PathState& x : c->xPs) x.segCap = c->ps.segCap;
     stk::PtCache& pc = lp.cache;
     if (p->cacheMode != 0) {
         if (!allowCache) return fail(c, STK_E_INVALID, "render: the radiance cache serves single-scene renders");
@@ -751,11 +790,17 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
     float4* accum = static_cast<float4*>(c->accum.ptr);
     STK_HIP(c, stk::launchZero(accum, npix, c->stream));
     STK_HIP(c, stk::launchClearRayTotals(c->ps, c->stream));
+    if (nstreams > 1) {
+        // the pass streams start after the context stream's set-up work
+        for (uint32_t i = 1; i < nstreams; ++i) STK_HIP(c, stk::launchClearRayTotals(c->passState(i), c->stream));
+        STK_HIP(c, hipEventRecord(c->joinEv[0], c->stream));
+        for (uint32_t i = 1; i < nstreams; ++i) STK_HIP(c, hipStreamWaitEvent(c->passStream(i), c->joinEv[0], 0));
+    }
     float* samples = nsamples ? static_cast<float*>(c->samples.ptr) : nullptr;
     // One sample pass is captured once as a graph and replayed per sample (the
     // pass number and frame come from device memory).  Training passes of the
     // cache keep host-side Adam state and run eagerly.
-    const bool graph = c->useGraph && pc.mode == 0;
+    const bool graph = c->useGraph && pc.mode == 0 && nstreams == 1;
     if (graph) {
         // everything the captured kernels take by value
         std::vector<uint64_t> key = {(uint64_t)sv.bvh.packed, (uint64_t)sv.bvh.tris, (uint64_t)sv.bvh.refNodes,
@@ -784,8 +829,9 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
             c->graphKey = key;
         }
     }
+    uint32_t k = 0; // pass number of this call
     for (int s = 0; s < p->pt.spp; ++s) {
-        for (size_t begin = 0; begin < slots; begin += Q) {
+        for (size_t begin = 0; begin < slots; begin += Q, ++k) {
             stk::PassState pass{};
             pass.frameNumber = p->pt.frameNumber;
             pass.sampleNum = (uint32_t)s;
@@ -800,6 +846,17 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
                 pass.trainOffset = h % pc.trainStride;
                 pass.numTrainPaths = (uint32_t)((slots - pass.trainOffset + pc.trainStride - 1) / pc.trainStride);
             }
+            if (nstreams > 1) {
+                // pass k on stream k % n; each finalize waits for the previous
+                // pass's, so every pixel still sums its samples in order
+                const uint32_t i = k % nstreams;
+                hipStream_t ps_stream = c->passStream(i);
+                STK_HIP(c, stk::launchSetPass(c->passState(i), pass, ps_stream));
+                STK_HIP(c, stk::launchPathTracePass(sv, lp, c->launch, c->passState(i), accum, samples, nullptr,
+                                                    ps_stream, k ? c->finEv[(k - 1) % nstreams] : nullptr,
+                                                    c->finEv[i]));
+                continue;
+            }
             STK_HIP(c, stk::launchSetPass(c->ps, pass, c->stream));
             if (graph)
                 STK_HIP(c, hipGraphLaunch(c->graphExec, c->stream));
@@ -808,6 +865,12 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
                                                     c->haveCache ? &c->cache : nullptr, c->stream));
         }
     }
+    for (uint32_t i = 1; i < nstreams; ++i) {
+        STK_HIP(c, hipEventRecord(c->joinEv[i], c->passStream(i)));
+        STK_HIP(c, hipStreamWaitEvent(c->stream, c->joinEv[i], 0));
+    }
+    c->lastStreams = nstreams;
+    c->lastPassStream = k ? (k - 1) % nstreams : 0;
     const int acc = p->enableAccumulation != 0;
     STK_HIP(c, stk::launchAccumulateTonemap(accum, static_cast<float4*>(c->history.ptr),
                                              outIsDevice && outRgba ? reinterpret_cast<float4*>(outRgba)
@@ -1143,6 +1206,8 @@ STK_API stk_status stk_set_launch_config(stk_ctx* c, const StkLaunchConfig* lc)
     if (lc->tile == 0 || lc->tile > 64) return fail(c, STK_E_INVALID, "stk_set_launch_config: tile must be 1..64");
     if (lc->persistentBlocks > (1u << 20))
         return fail(c, STK_E_INVALID, "stk_set_launch_config: persistentBlocks too large");
+    if (lc->passStreams > kMaxPassStreams)
+        return fail(c, STK_E_INVALID, "stk_set_launch_config: passStreams must be 0..4");
     if (lc->pathsInFlight != 0 && lc->pathsInFlight < 4096)
         return fail(c, STK_E_INVALID, "stk_set_launch_config: pathsInFlight must be 0 or at least 4096");
     c->launch.blockSize = lc->blockSize;
@@ -1151,6 +1216,7 @@ STK_API stk_status stk_set_launch_config(stk_ctx* c, const StkLaunchConfig* lc)
     c->launch.pathsInFlight = lc->pathsInFlight;
     c->launch.binaryWalk = lc->binaryWalk ? 1u : 0u;
     c->launch.stripeThreads = lc->stripeThreads;
+    c->launch.passStreams = std::max<uint32_t>(1u, lc->passStreams);
     c->useGraph = lc->useGraph != 0;
     invalidateGraph(c);
     return STK_OK;
@@ -1184,7 +1250,7 @@ STK_API stk_status stk_get_launch_config(stk_ctx* c, StkLaunchConfig* lc)
     lc->pathsInFlight = c->launch.pathsInFlight;
     lc->binaryWalk = c->launch.binaryWalk;
     lc->stripeThreads = c->launch.stripeThreads;
-    memset(lc->pad, 0, sizeof(lc->pad));
+    lc->passStreams = c->launch.passStreams;
     return STK_OK;
 }
 
@@ -1480,7 +1546,8 @@ STK_API stk_status stk_get_counters(stk_ctx* c, uint32_t out[8])
 {
     if (!validCtx(c) || !out) return STK_E_INVALID;
     if (!c->ps.counters) return fail(c, STK_E_STATE, "stk_get_counters: nothing rendered");
-    STK_HIP(c, hipMemcpyAsync(out, c->ps.counters, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    STK_HIP(c, hipMemcpyAsync(out, c->passState(c->lastPassStream).counters, 8 * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, c->stream));
     STK_HIP(c, hipStreamSynchronize(c->stream));
     return STK_OK;
 }
@@ -1490,7 +1557,15 @@ STK_API stk_status stk_get_ray_totals(stk_ctx* c, uint64_t out[2])
     if (!validCtx(c) || !out) return STK_E_INVALID;
     if (!c->ps.counters) return fail(c, STK_E_STATE, "stk_get_ray_totals: nothing rendered");
     STK_HIP(c, hipMemcpyAsync(out, c->ps.counters + 8, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    uint64_t more[kMaxPassStreams - 1][2] = {};
+    for (uint32_t i = 1; i < c->lastStreams; ++i)
+        STK_HIP(c, hipMemcpyAsync(more[i - 1], c->passState(i).counters + 8, sizeof(more[0]), hipMemcpyDeviceToHost,
+                                  c->stream));
     STK_HIP(c, hipStreamSynchronize(c->stream));
+    for (uint32_t i = 1; i < c->lastStreams; ++i) {
+        out[0] += more[i - 1][0];
+        out[1] += more[i - 1][1];
+    }
     return STK_OK;
 }
 

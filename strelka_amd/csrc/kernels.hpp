@@ -130,6 +130,7 @@ struct LaunchConfig {
     uint32_t pathsInFlight = 0;    // slots per pass (0: every slot of the frame in one pass)
     uint32_t binaryWalk = 0;       // 1: packed BVH2 walks instead of the compressed 8-wide ones
     uint32_t stripeThreads = 0;    // threads per stripe of the consumer grids (0: one per entry)
+    uint32_t passStreams = 1;      // n > 1: pass k runs on stream k % n
 };
 
 // flags: STK_TRACE_REFERENCE_ORDER (skip-link walk), STK_TRACE_BVH2 (packed
@@ -149,8 +150,12 @@ hipError_t launchClearRayTotals(PathState& ps, hipStream_t s);
 // accum[pixel] (and stores per-sample radiance when samples is set).  With
 // lp.cache.mode != 0, cb must be the context's cache: queries are answered
 // after the bounce loop and one training step runs on the pass's records.
+// finalizeAfter (optional): the finalize step waits for this event (the
+// previous pass's finalize on another stream); finalizeDone (optional) is
+// recorded after it.
 hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const LaunchConfig& lc, PathState& ps,
-                               float4* accum, float* samples, CacheBuffers* cb, hipStream_t s);
+                               float4* accum, float* samples, CacheBuffers* cb, hipStream_t s,
+                               hipEvent_t finalizeAfter = nullptr, hipEvent_t finalizeDone = nullptr);
 // Plain 16-B-per-lane copy kernel (bytes must be a multiple of 16).
 hipError_t launchCopy(void* dst, const void* src, size_t bytes, hipStream_t s);
 // RGBA32F pixels to a tightly packed RGBA32F (format 0) or RGBA8 (format 1)

@@ -604,7 +604,8 @@ hipError_t launchClearRayTotals(PathState& ps, hipStream_t s)
 }
 
 hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const LaunchConfig& lc, PathState& ps,
-                               float4* accum, float* samples, CacheBuffers* cb, hipStream_t s)
+                               float4* accum, float* samples, CacheBuffers* cb, hipStream_t s,
+                               hipEvent_t finalizeAfter, hipEvent_t finalizeDone)
 {
     const uint32_t n = lp.passSlots;
     if (n == 0 || n > ps.capacity || ps.segCap % 256 != 0 || (size_t)ps.segCap * kStripes > ps.queueCapacity ||
@@ -635,7 +636,15 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_cache_apply, g, b, 0, s, lp, ps);
     }
+    if (finalizeAfter) {
+        hipError_t e = hipStreamWaitEvent(s, finalizeAfter, 0);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_pt_finalize, g, b, 0, s, lp, ps, accum, samples);
+    if (finalizeDone) {
+        hipError_t e = hipEventRecord(finalizeDone, s);
+        if (e != hipSuccess) return e;
+    }
     if (train && cb && pc.maxTrainPaths > 0) {
         hipLaunchKernelGGL(k_cache_targets, dim3(blocks((size_t)pc.maxTrainPaths * lp.maxDepth)), b, 0, s, lp, ps);
         // trainSteps optimiser steps over the frame's records

@@ -101,7 +101,12 @@ SHAPES = [dict(block_size=64, tile=1), dict(block_size=128, tile=16), dict(block
           dict(block_size=128, tile=8, persistent_blocks=8192, graph=True),
           dict(block_size=64, tile=1, persistent_blocks=1, paths_in_flight=20000),
           dict(block_size=64, tile=8, stripe_threads=64), dict(block_size=256, tile=8, stripe_threads=2048, graph=True),
-          dict(block_size=128, tile=4, stripe_threads=4096, paths_in_flight=20000, persistent_blocks=512)]
+          dict(block_size=128, tile=4, stripe_threads=4096, paths_in_flight=20000, persistent_blocks=512),
+          dict(block_size=64, tile=8, pass_streams=2), dict(block_size=256, tile=8, pass_streams=2, graph=True),
+          dict(block_size=128, tile=8, pass_streams=2, paths_in_flight=20000, persistent_blocks=1024),
+          dict(block_size=64, tile=4, pass_streams=3, paths_in_flight=8192),
+          dict(block_size=256, tile=8, pass_streams=4, paths_in_flight=50000, stripe_threads=4096),
+          dict(block_size=64, tile=8, pass_streams=1, graph=True)]
 
 
 def test_launch_shapes_give_identical_images(scenes):
