@@ -282,7 +282,9 @@ def cache_study(args) -> dict:
     ctx = sa.Context(0)
     ctx.set_scene(scene)
     ctx.build_bvh()
-    ctx.set_launch_config(block_size=64, tile=8)
+    # two pass streams: plain frames overlap each other (cached frames run on
+    # one stream: each trains on the previous frame's network)
+    ctx.set_launch_config(block_size=64, tile=8, pass_streams=2)
     npix = W * H
     default_paths = max(1024, int(65536 * npix / (1920 * 1080)))
     d_out = ctx.alloc(npix * 16)
@@ -308,26 +310,33 @@ def cache_study(args) -> dict:
     checkpoints = sorted({1, 2, 4, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512} | {args.study_frames})
 
     def run(mode, qd=2, warmup=0, paths=default_paths, min_rough=0.0, steps=1, label=""):
+        # frames are issued back to back and timed between checkpoints (the
+        # image read-back at a checkpoint is not timed)
         t_ms = 0.0
-        for f in range(warmup):
+        if warmup:
+            ctx.sync()
             ctx.record(0)
-            ctx.render_device(params(500_000 + f, 4, qd, paths, min_rough, steps), d_out)
+            for f in range(warmup):
+                ctx.render_device(params(500_000 + f, 4, qd, paths, min_rough, steps), d_out)
             ctx.record(1)
             t_ms += ctx.elapsed_ms(0, 1)
         ctx.reset_accumulation()
         curve = []
+        ctx.sync()
+        ctx.record(0)
         for f in range(args.study_frames):
-            ctx.record(0)
             ctx.render_device(params(f, mode, qd, paths, min_rough, steps), d_out)
-            ctx.record(1)
-            t_ms += ctx.elapsed_ms(0, 1)
             if f + 1 in checkpoints:
+                ctx.record(1)
+                t_ms += ctx.elapsed_ms(0, 1)
                 tm = ctx.download(d_out, out.copy())[..., :3]
                 lin = ctx.accumulation(W, H)[..., :3]
                 curve.append({"frames": f + 1, "ms": round(t_ms, 2),
                               "rmse": round(float(np.sqrt(np.mean((tm - ref_tm) ** 2))), 5),
                               "rmse_linear": round(float(np.sqrt(np.mean((lin - ref_lin) ** 2))), 5)})
                 print(f"{label} frame {f + 1}: {curve[-1]}", file=sys.stderr, flush=True)
+                ctx.sync()
+                ctx.record(0)
         return curve
 
     def time_to(curve, target):
@@ -382,6 +391,9 @@ def main() -> int:
     ap.add_argument("--study-target", type=float, default=0.01)
     ap.add_argument("--study-lr", type=float, default=1e-2)
     ap.add_argument("--study-ema", type=float, default=0.99)
+    ap.add_argument("--hw-queues", type=int, default=8,
+                    help="hardware queues HIP gives this process (GPU_MAX_HW_QUEUES, raised to this value before "
+                         "HIP starts): the context stream plus up to four pass streams")
     ap.add_argument("--study-variants", default='[{"query_depth": 2}]',
                     help="JSON list of cached-run settings: query_depth, min_rough, train_steps, warmup, lr, ema, train_paths, "
                          "log2_table, "
@@ -390,6 +402,10 @@ def main() -> int:
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         return 2
+    # HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 by
+    # default); streams beyond that share queues and lose their overlap
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < args.hw_queues:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
 
     import strelka_amd as sa
 
@@ -471,6 +487,7 @@ def main() -> int:
     tuned = choose_launch(ctx, args.launch, lambda: ctx.render_device(p1, d_out), n)
     pt = pt_frame_bench(ctx, scene, args.pt_spp, max(1, args.pt_frames), d_out)
     pt["launch"] = tuned["best"]
+    pt["hw_queues"] = int(os.environ["GPU_MAX_HW_QUEUES"])
     pt["autotune_4spp_ms"] = {"default": tuned.get("default_ms"), "best": tuned.get("ms")}
     ctx.free(d_out)
 

@@ -65,7 +65,7 @@ def tune(ctx, run: Callable[[], None], total_paths: int, repeats: int = 2, sweep
         "graph": [True, False],
         "binary_walk": [False, True],
         "stripe_threads": [k * cus for k in STRIPE_THREADS_PER_CU],
-        "pass_streams": [1, 2, 3],
+        "pass_streams": [1, 2, 3, 4],
     }
     seen: dict = {}
 

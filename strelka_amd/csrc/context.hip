@@ -81,22 +81,28 @@ struct stk_ctx {
     DevBuf tlasWide, tlasLeaves, blasWide, blasTris, instKey, instW, tlEmit;
     DevBuf rayScratch, hitScratch;
 
-    // render state.  With passStreams = n > 1, pass k of a render call runs on
-    // pass stream k % n with that stream's path state: stream 0 is `stream`
-    // with `ps`, streams 1.. are xStream[i - 1] with xPs[i - 1].  finEv[i]
-    // marks the finalize of the last pass on stream i, joinEv[i] the joins.
+    // render state.  One pass stream (passStreams = 1): every pass runs on
+    // `stream` with `ps`.  n > 1 pass streams: the passes run on xStream[i]
+    // with xPs[i], in turn across render calls, and `stream` only takes each
+    // call's accumulate / tonemap step, after joining the pass streams, so a
+    // call's passes overlap the previous call's last ones.  Calls alternate
+    // between two accumulation buffers (accum, accum2); accumFree[l] marks
+    // the last read of buffer l.  finEv[i]: finalize of the last pass on
+    // xStream[i]; joinEv[i]: joins.
     stk::PathState ps;
     std::vector<DevBuf> psBufs;
-    stk::PathState xPs[kMaxPassStreams - 1];
-    std::vector<DevBuf> xPsBufs[kMaxPassStreams - 1];
-    hipStream_t xStream[kMaxPassStreams - 1] = {};
-    hipEvent_t finEv[kMaxPassStreams] = {}, joinEv[kMaxPassStreams] = {};
+    stk::PathState xPs[kMaxPassStreams];
+    std::vector<DevBuf> xPsBufs[kMaxPassStreams];
+    hipStream_t xStream[kMaxPassStreams] = {};
+    hipEvent_t finEv[kMaxPassStreams] = {}, joinEv[kMaxPassStreams] = {}, accumFree[2] = {};
+    bool accumFreeSet[2] = {false, false};
+    uint32_t renderCalls = 0, passTurn = 0; // multi-stream calls so far, next pass stream
     uint32_t lastStreams = 1;    // pass streams of the last render call
     uint32_t lastPassStream = 0; // stream of its last pass
+    std::vector<stk::SceneDesc> descsHost; // what descs holds
 
-    stk::PathState& passState(uint32_t i) { return i ? xPs[i - 1] : ps; }
-    hipStream_t passStream(uint32_t i) const { return i ? xStream[i - 1] : stream; }
-    DevBuf accum, history, out, samples, descs, staging;
+    stk::PathState& passState(uint32_t i) { return lastStreams > 1 ? xPs[i] : ps; }
+    DevBuf accum, accum2, history, out, samples, descs, staging;
     uint32_t iteration = 0;
     uint64_t imageKey = 0;
     size_t imagePixels = 0, samplesCount = 0;
@@ -125,7 +131,7 @@ struct stk_ctx {
     {
         return {&vb, &ib, &inst, &lights, &mats, &packed, &tris, &refNodes, &wideNodes, &wideTris, &emitters,
                 &tlasWide, &tlasLeaves, &blasWide, &blasTris, &instKey, &instW, &tlEmit, &rayScratch, &hitScratch,
-                &accum, &history, &out, &samples, &descs, &staging, &fVb, &fIb, &fInst, &fLights, &fMats, &fPacked,
+                &accum, &accum2, &history, &out, &samples, &descs, &staging, &fVb, &fIb, &fInst, &fLights, &fMats, &fPacked,
                 &fTris, &fRefNodes, &fWide, &fWideTris, &fEmit, &cGridH, &cGridP, &cGridM, &cGridV, &cGridG, &cMlpP,
                 &cMlpM, &cMlpV, &cMlpG, &cFwd, &cBwd, &cXT, &cDyT, &cLoss, &cGridE, &cGridEW, &cGridHE, &cMlpE,
                 &cFwdE, &cBwdE, &queryRecs, &queryOut, &trainVerts, &trainCount, &trainBatch};
@@ -156,6 +162,9 @@ stk_status hipFail(stk_ctx* c, hipError_t e, const char* what)
 stk_status devRelease(stk_ctx* c, DevBuf& b)
 {
     if (b.ptr) {
+        // the pass streams may still read any buffer
+        for (hipStream_t s : c->xStream)
+            if (s) (void)hipStreamSynchronize(s);
         hipError_t e = hipFree(b.ptr);
         b.ptr = nullptr;
         b.bytes = 0;
@@ -307,6 +316,8 @@ STK_API void stk_destroy(stk_ctx* c)
         if (c->finEv[i]) (void)hipEventDestroy(c->finEv[i]);
         if (c->joinEv[i]) (void)hipEventDestroy(c->joinEv[i]);
     }
+    for (auto& ev : c->accumFree)
+        if (ev) (void)hipEventDestroy(ev);
     for (hipStream_t s : c->xStream)
         if (s) (void)hipStreamDestroy(s);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -726,23 +737,39 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
     size_t Q = slots;
     if (c->launch.pathsInFlight && p->cacheMode == 0)
         Q = std::min(slots, ((size_t)c->launch.pathsInFlight + 63) / 64 * 64);
-    if ((st = ensurePathState(c, c->ps, c->psBufs, Q)) != STK_OK) return st;
     // several pass streams: plain rendering only (the cache's training passes
     // depend on each other)
     const uint32_t nstreams = p->cacheMode == 0 ? std::max<uint32_t>(1u, c->launch.passStreams) : 1u;
-    for (uint32_t i = 1; i < nstreams; ++i) {
-        if ((st = ensurePathState(c, c->xPs[i - 1], c->xPsBufs[i - 1], Q)) != STK_OK) return st;
-        if (!c->xStream[i - 1]) STK_HIP(c, hipStreamCreateWithFlags(&c->xStream[i - 1], hipStreamNonBlocking));
+    const bool multi = nstreams > 1;
+    if (!multi && (st = ensurePathState(c, c->ps, c->psBufs, Q)) != STK_OK) return st;
+    for (uint32_t i = 0; multi && i < nstreams; ++i) {
+        if ((st = ensurePathState(c, c->xPs[i], c->xPsBufs[i], Q)) != STK_OK) return st;
+        if (!c->xStream[i]) STK_HIP(c, hipStreamCreateWithFlags(&c->xStream[i], hipStreamNonBlocking));
     }
-    if (nstreams > 1 && !c->finEv[0])
+    if (multi && !c->finEv[0]) {
         for (int i = 0; i < kMaxPassStreams; ++i) {
             STK_HIP(c, hipEventCreateWithFlags(&c->finEv[i], hipEventDisableTiming));
             STK_HIP(c, hipEventCreateWithFlags(&c->joinEv[i], hipEventDisableTiming));
         }
+        for (auto& ev : c->accumFree) STK_HIP(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+    // multi-stream calls alternate between the two accumulation buffers
+    const uint32_t lane = multi ? (c->renderCalls & 1u) : 0u;
+    DevBuf& accBuf = lane ? c->accum2 : c->accum;
     if ((st = ensureImage(c, imageKey, npix, nsamples)) != STK_OK) return st;
+    if ((st = devReserve(c, accBuf, npix * sizeof(float4))) != STK_OK) return st;
+    // the views' records; the kernels take the frame number from the pass
+    // state, so the records of a progressive render do not change per frame
+    // and are uploaded (a synchronising copy) only when they do
+    for (stk::SceneDesc& d : descs) d.cam.frameNumber = 0;
     const size_t descBytes = descs.size() * sizeof(stk::SceneDesc);
     if (c->descs.bytes < descBytes) invalidateGraph(c);
-    if ((st = devUpload(c, c->descs, descs.data(), descBytes)) != STK_OK) return st;
+    if (!c->descs.ptr || c->descsHost.size() != descs.size() ||
+        memcmp(c->descsHost.data(), descs.data(), descBytes) != 0) {
+        c->descsHost.clear();
+        if ((st = devUpload(c, c->descs, descs.data(), descBytes)) != STK_OK) return st;
+        c->descsHost = descs;
+    }
     stk::SceneView sv = base;
     sv.scenes = static_cast<const stk::SceneDesc*>(c->descs.ptr);
     sv.numScenes = (uint32_t)descs.size();
@@ -787,20 +814,31 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
         pc.trainCount = static_cast<uint32_t*>(c->trainCount.ptr);
         pc.trainBatch = static_cast<stk::CacheRecord*>(c->trainBatch.ptr);
     }
-    float4* accum = static_cast<float4*>(c->accum.ptr);
-    STK_HIP(c, stk::launchZero(accum, npix, c->stream));
-    STK_HIP(c, stk::launchClearRayTotals(c->ps, c->stream));
-    if (nstreams > 1) {
-        // the pass streams start after the context stream's set-up work
-        for (uint32_t i = 1; i < nstreams; ++i) STK_HIP(c, stk::launchClearRayTotals(c->passState(i), c->stream));
-        STK_HIP(c, hipEventRecord(c->joinEv[0], c->stream));
-        for (uint32_t i = 1; i < nstreams; ++i) STK_HIP(c, hipStreamWaitEvent(c->passStream(i), c->joinEv[0], 0));
+    float4* accum = static_cast<float4*>(accBuf.ptr);
+    const uint32_t first = c->passTurn % std::max(1u, nstreams); // pass stream of this call's first pass
+    if (!multi) {
+        STK_HIP(c, stk::launchZero(accum, npix, c->stream));
+        STK_HIP(c, stk::launchClearRayTotals(c->ps, c->stream));
+    } else {
+        // After a single-stream call the pass streams wait for the context
+        // stream; after a multi-stream call they go on at once (what a pass
+        // reads is uploaded with a synchronising copy; what it writes is its
+        // stream's path state, this call's accumulation buffer and the
+        // samples, ordered below).
+        if (c->lastStreams == 1) {
+            STK_HIP(c, hipEventRecord(c->joinEv[0], c->stream));
+            for (uint32_t i = 0; i < nstreams; ++i) STK_HIP(c, hipStreamWaitEvent(c->xStream[i], c->joinEv[0], 0));
+        }
+        for (uint32_t i = 0; i < nstreams; ++i) STK_HIP(c, stk::launchClearRayTotals(c->xPs[i], c->xStream[i]));
+        // the buffer is free once the call before last has tonemapped it
+        if (c->accumFreeSet[lane]) STK_HIP(c, hipStreamWaitEvent(c->xStream[first], c->accumFree[lane], 0));
+        STK_HIP(c, stk::launchZero(accum, npix, c->xStream[first]));
     }
     float* samples = nsamples ? static_cast<float*>(c->samples.ptr) : nullptr;
     // One sample pass is captured once as a graph and replayed per sample (the
     // pass number and frame come from device memory).  Training passes of the
     // cache keep host-side Adam state and run eagerly.
-    const bool graph = c->useGraph && pc.mode == 0 && nstreams == 1;
+    const bool graph = c->useGraph && pc.mode == 0 && !multi;
     if (graph) {
         // everything the captured kernels take by value
         std::vector<uint64_t> key = {(uint64_t)sv.bvh.packed, (uint64_t)sv.bvh.tris, (uint64_t)sv.bvh.refNodes,
@@ -846,15 +884,19 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
                 pass.trainOffset = h % pc.trainStride;
                 pass.numTrainPaths = (uint32_t)((slots - pass.trainOffset + pc.trainStride - 1) / pc.trainStride);
             }
-            if (nstreams > 1) {
-                // pass k on stream k % n; each finalize waits for the previous
-                // pass's, so every pixel still sums its samples in order
-                const uint32_t i = k % nstreams;
-                hipStream_t ps_stream = c->passStream(i);
-                STK_HIP(c, stk::launchSetPass(c->passState(i), pass, ps_stream));
-                STK_HIP(c, stk::launchPathTracePass(sv, lp, c->launch, c->passState(i), accum, samples, nullptr,
-                                                    ps_stream, k ? c->finEv[(k - 1) % nstreams] : nullptr,
+            if (multi) {
+                // the pass streams in turn; each finalize waits for the
+                // previous one (of this call or, after a multi-stream call,
+                // of that call: the samples buffer is shared), so every pixel
+                // sums its samples in order and no sample is overwritten late
+                const uint32_t i = (first + k) % nstreams;
+                const bool chained = k > 0 || c->lastStreams > 1;
+                STK_HIP(c, stk::launchSetPass(c->xPs[i], pass, c->xStream[i]));
+                STK_HIP(c, stk::launchPathTracePass(sv, lp, c->launch, c->xPs[i], accum, samples, nullptr,
+                                                    c->xStream[i], chained ? c->finEv[c->lastPassStream] : nullptr,
                                                     c->finEv[i]));
+                c->lastPassStream = i;
+                c->lastStreams = nstreams;
                 continue;
             }
             STK_HIP(c, stk::launchSetPass(c->ps, pass, c->stream));
@@ -865,12 +907,17 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
                                                     c->haveCache ? &c->cache : nullptr, c->stream));
         }
     }
-    for (uint32_t i = 1; i < nstreams; ++i) {
-        STK_HIP(c, hipEventRecord(c->joinEv[i], c->passStream(i)));
-        STK_HIP(c, hipStreamWaitEvent(c->stream, c->joinEv[i], 0));
+    if (multi) {
+        for (uint32_t i = 0; i < nstreams; ++i) {
+            STK_HIP(c, hipEventRecord(c->joinEv[i], c->xStream[i]));
+            STK_HIP(c, hipStreamWaitEvent(c->stream, c->joinEv[i], 0));
+        }
+        c->passTurn += k;
+        ++c->renderCalls;
+    } else {
+        c->lastStreams = 1;
+        c->lastPassStream = 0;
     }
-    c->lastStreams = nstreams;
-    c->lastPassStream = k ? (k - 1) % nstreams : 0;
     const int acc = p->enableAccumulation != 0;
     STK_HIP(c, stk::launchAccumulateTonemap(accum, static_cast<float4*>(c->history.ptr),
                                              outIsDevice && outRgba ? reinterpret_cast<float4*>(outRgba)
@@ -878,6 +925,10 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
                                              (uint32_t)npix, c->iteration, acc, p->enableTonemap != 0,
                                              p->tonemapperType, c->stream));
     if (acc) ++c->iteration;
+    if (c->accumFree[lane]) {
+        STK_HIP(c, hipEventRecord(c->accumFree[lane], c->stream));
+        c->accumFreeSet[lane] = true;
+    }
     if (outRgba && !outIsDevice) {
         STK_HIP(c, hipMemcpyAsync(outRgba, c->out.ptr, npix * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
         STK_HIP(c, hipStreamSynchronize(c->stream));
@@ -1545,7 +1596,7 @@ STK_API stk_status stk_cache_set_params(stk_ctx* c, const float* grid, const flo
 STK_API stk_status stk_get_counters(stk_ctx* c, uint32_t out[8])
 {
     if (!validCtx(c) || !out) return STK_E_INVALID;
-    if (!c->ps.counters) return fail(c, STK_E_STATE, "stk_get_counters: nothing rendered");
+    if (!c->passState(c->lastPassStream).counters) return fail(c, STK_E_STATE, "stk_get_counters: nothing rendered");
     STK_HIP(c, hipMemcpyAsync(out, c->passState(c->lastPassStream).counters, 8 * sizeof(uint32_t),
                               hipMemcpyDeviceToHost, c->stream));
     STK_HIP(c, hipStreamSynchronize(c->stream));
@@ -1555,8 +1606,9 @@ STK_API stk_status stk_get_counters(stk_ctx* c, uint32_t out[8])
 STK_API stk_status stk_get_ray_totals(stk_ctx* c, uint64_t out[2])
 {
     if (!validCtx(c) || !out) return STK_E_INVALID;
-    if (!c->ps.counters) return fail(c, STK_E_STATE, "stk_get_ray_totals: nothing rendered");
-    STK_HIP(c, hipMemcpyAsync(out, c->ps.counters + 8, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    if (!c->passState(0).counters) return fail(c, STK_E_STATE, "stk_get_ray_totals: nothing rendered");
+    STK_HIP(c, hipMemcpyAsync(out, c->passState(0).counters + 8, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                              c->stream));
     uint64_t more[kMaxPassStreams - 1][2] = {};
     for (uint32_t i = 1; i < c->lastStreams; ++i)
         STK_HIP(c, hipMemcpyAsync(more[i - 1], c->passState(i).counters + 8, sizeof(more[0]), hipMemcpyDeviceToHost,

@@ -131,6 +131,47 @@ def test_launch_shapes_give_identical_images(scenes):
     ctx.close()
 
 
+def test_pass_streams_across_render_calls(scenes):
+    """Progressive frames issued back to back (device output, no host wait)
+    with several pass streams, switching stream counts and single-stream graph
+    replay on the way, give every frame, the last frame's samples and its ray
+    totals bit for bit as one stream does."""
+    print("case: config 3 variant 10, 6 accumulated frames of 2 spp, pass streams 3/3/1/2/2/4", flush=True)
+    s = scenes[2]
+    w, h = s.camera.dimension[0], s.camera.dimension[1]
+    shapes = [dict(block_size=64, tile=8, pass_streams=3, paths_in_flight=4096),
+              dict(block_size=64, tile=8, pass_streams=3, paths_in_flight=4096),
+              dict(block_size=64, tile=8, graph=True),
+              dict(block_size=128, tile=4, pass_streams=2),
+              dict(block_size=128, tile=4, pass_streams=2, paths_in_flight=8192),
+              dict(block_size=64, tile=8, pass_streams=4, paths_in_flight=4096)]
+    ref = []
+    ctx = sa.Context(0)
+    ctx.set_scene(s)
+    ctx.build_bvh()
+    for f in range(len(shapes)):
+        ref.append(ctx.render(params(s.camera, s.sky, spp=2, frame=f)).copy())
+    ref_smp = ctx.samples(w * h * 2 * 3)
+    ref_totals = ctx.ray_totals()
+    ctx.close()
+    ctx = sa.Context(0)
+    ctx.set_scene(s)
+    ctx.build_bvh()
+    outs = [ctx.alloc(w * h * 16) for _ in shapes]
+    for f, shape in enumerate(shapes):
+        ctx.set_launch_config(**shape)
+        ctx.render_device(params(s.camera, s.sky, spp=2, frame=f), outs[f])
+    smp = ctx.samples(w * h * 2 * 3)
+    totals = ctx.ray_totals()
+    for f, d in enumerate(outs):
+        img = ctx.download(d, np.zeros((h, w, 4), np.float32))
+        assert np.array_equal(img, ref[f]), f
+        ctx.free(d)
+    assert np.array_equal(smp, ref_smp)
+    assert totals == ref_totals
+    ctx.close()
+
+
 def test_record_queries_write_only_their_record(scenes):
     """stk_get_launch_config and stk_batch_info fill the caller's record and
     nothing after it (a canary block follows each record)."""
