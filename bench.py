@@ -274,9 +274,9 @@ def cache_study(args) -> dict:
     frame queries the cache at vertex depth `query_depth` and trains."""
     import strelka_amd as sa
 
-    scene = sa.SceneData.generate(2)
-    W, H = args.study_res
+    scene = sa.SceneData.generate(args.study_scene)
     cam = scene.camera
+    W, H = args.study_res if args.study_res else (cam.dimension[0], cam.dimension[1])
     cam.dimension[0], cam.dimension[1] = W, H
     cam.invDimension[0], cam.invDimension[1] = 1.0 / W, 1.0 / H
     ctx = sa.Context(0)
@@ -352,7 +352,8 @@ def cache_study(args) -> dict:
     for v in variants:
         ctx.cache_create(log2_table=v.get("log2_table", 19), base_res=16, max_res=v.get("max_res", 2048),
                          lr=v.get("lr", args.study_lr), ema=v.get("ema", args.study_ema))
-        curve = run(1, v.get("query_depth", 2), v.get("warmup", 0), v.get("train_paths", default_paths),
+        curve = run(2 if v.get("freeze") else 1, v.get("query_depth", 2), v.get("warmup", 0),
+                    v.get("train_paths", default_paths),
                     v.get("min_rough", 0.0), v.get("train_steps", 1), label=f"cached {v}")
         tc = time_to(curve, args.study_target)
         results.append({"variant": v, "time_to_target_s": tc, "speedup": (tp / tc) if (tp and tc) else None,
@@ -360,7 +361,7 @@ def cache_study(args) -> dict:
     ctx.free(d_out)
     ctx.close()
     best = min((r for r in results if r["time_to_target_s"]), key=lambda r: r["time_to_target_s"], default=None)
-    return {"study": "cache", "resolution": [W, H], "target_rmse_tonemapped": args.study_target,
+    return {"study": "cache", "scene": args.study_scene, "resolution": [W, H], "target_rmse_tonemapped": args.study_target,
             "reference_spp": args.study_ref_spp, "reference_s": round(ref_s, 2),
             "default_train_paths_per_frame": default_paths,
             "time_to_target_s": {"plain": tp, "cached_best": best["time_to_target_s"] if best else None},
@@ -385,7 +386,11 @@ def main() -> int:
                     help="only run the plain copy kernel over this many MiB (calibrates memory counters)")
     ap.add_argument("--cache-study", action="store_true",
                     help="only run the time-to-RMSE study of the cached vs plain path tracer")
-    ap.add_argument("--study-res", type=int, nargs=2, default=(960, 540))
+    ap.add_argument("--study-res", type=int, nargs=2, default=None,
+                    help="resolution of the study (default: 960x540 for scene 2, else the scene's own)")
+    ap.add_argument("--study-scene", type=int, default=2,
+                    help="seeded scene of the study: 2 (the 1M-triangle heightfield + spheres, open sky) or 7 "
+                         "(a closed room lit by a small ceiling light)")
     ap.add_argument("--study-frames", type=int, default=256)
     ap.add_argument("--study-ref-spp", type=int, default=2048)
     ap.add_argument("--study-target", type=float, default=0.01)
@@ -395,7 +400,8 @@ def main() -> int:
                     help="hardware queues HIP gives this process (GPU_MAX_HW_QUEUES, raised to this value before "
                          "HIP starts): the context stream plus up to four pass streams")
     ap.add_argument("--study-variants", default='[{"query_depth": 2}]',
-                    help="JSON list of cached-run settings: query_depth, min_rough, train_steps, warmup, lr, ema, train_paths, "
+                    help="JSON list of cached-run settings: query_depth, min_rough, train_steps, warmup, freeze (no "
+                         "training after the warm-up), lr, ema, train_paths, "
                          "log2_table, "
                          "max_res")
     args = ap.parse_args()
@@ -410,6 +416,8 @@ def main() -> int:
     import strelka_amd as sa
 
     if args.cache_study:
+        if args.study_res is None and args.study_scene == 2:
+            args.study_res = (960, 540)
         print(json.dumps(cache_study(args)))
         return 0
 
