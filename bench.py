@@ -351,7 +351,8 @@ def cache_study(args) -> dict:
     results = []
     for v in variants:
         ctx.cache_create(log2_table=v.get("log2_table", 19), base_res=16, max_res=v.get("max_res", 2048),
-                         lr=v.get("lr", args.study_lr), ema=v.get("ema", args.study_ema))
+                         lr=v.get("lr", args.study_lr), ema=v.get("ema", args.study_ema),
+                         lr_half_life=v.get("lr_half_life", 0.0), min_lr=v.get("min_lr", 0.0))
         curve = run(2 if v.get("freeze") else 1, v.get("query_depth", 2), v.get("warmup", 0),
                     v.get("train_paths", default_paths),
                     v.get("min_rough", 0.0), v.get("train_steps", 1), label=f"cached {v}")
@@ -399,9 +400,11 @@ def main() -> int:
     ap.add_argument("--hw-queues", type=int, default=8,
                     help="hardware queues HIP gives this process (GPU_MAX_HW_QUEUES, raised to this value before "
                          "HIP starts): the context stream plus up to four pass streams")
-    ap.add_argument("--study-variants", default='[{"query_depth": 2}]',
+    ap.add_argument("--study-variants",
+                    default='[{"query_depth": 0, "warmup": 64, "train_steps": 2, "train_paths": 16384, '
+                            '"lr_half_life": 128, "min_lr": 0.0002}]',
                     help="JSON list of cached-run settings: query_depth, min_rough, train_steps, warmup, freeze (no "
-                         "training after the warm-up), lr, ema, train_paths, "
+                         "training after the warm-up), lr, lr_half_life (Adam steps), min_lr, ema, train_paths, "
                          "log2_table, "
                          "max_res")
     args = ap.parse_args()

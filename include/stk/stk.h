@@ -282,7 +282,10 @@ typedef struct StkCacheConfig {
     uint32_t maxBatch;       /* capacity of one training step, in records */
     float emaDecay;          /* queries read an exponential moving average of the
                                 weights with this decay (0 = the raw weights) */
-    uint32_t pad[2];
+    /* learning-rate decay: step k (from 1) uses
+     * max(minLearningRate, learningRate * 2^(-(k - 1) / lrHalfLife)); 0 = none */
+    float lrHalfLife;
+    float minLearningRate;
 } StkCacheConfig;
 
 typedef struct StkCacheInfo {

@@ -522,8 +522,10 @@ class Context:
     # -- radiance cache ----------------------------------------------------------
     def cache_create(self, log2_table: int = 19, base_res: int = 16, max_res: int = 2048, seed: int = 1,
                      lr: float = 1e-2, beta1: float = 0.9, beta2: float = 0.99, eps: float = 1e-15,
-                     max_batch: int = 1 << 18, ema: float = 0.99):
-        cfg = CacheConfig(log2_table, base_res, max_res, seed, lr, beta1, beta2, eps, max_batch, ema)
+                     max_batch: int = 1 << 18, ema: float = 0.99, lr_half_life: float = 0.0,
+                     min_lr: float = 0.0):
+        cfg = CacheConfig(log2_table, base_res, max_res, seed, lr, beta1, beta2, eps, max_batch, ema, lr_half_life,
+                          min_lr)
         self._check(_LIB.stk_cache_create(self._h, C.byref(cfg)), "stk_cache_create")
 
     def cache_info(self) -> CacheInfo:

@@ -64,7 +64,8 @@ class RenderParams(C.Structure):
 class CacheConfig(C.Structure):
     _fields_ = [("log2TableSize", C.c_uint32), ("baseResolution", C.c_uint32), ("maxResolution", C.c_uint32),
                 ("seed", C.c_uint32), ("learningRate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
-                ("epsilon", C.c_float), ("maxBatch", C.c_uint32), ("emaDecay", C.c_float), ("pad", C.c_uint32 * 2)]
+                ("epsilon", C.c_float), ("maxBatch", C.c_uint32), ("emaDecay", C.c_float), ("lrHalfLife", C.c_float),
+                ("minLearningRate", C.c_float)]
 
 
 class CacheInfo(C.Structure):

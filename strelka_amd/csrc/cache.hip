@@ -389,15 +389,18 @@ hipError_t cacheBackward(const CacheBuffers& cb, const CacheRecord* recs, uint32
 hipError_t cacheAdam(CacheBuffers& cb, float gradScale, const uint32_t* nDev, uint32_t nLimit, hipStream_t s)
 {
     cb.step += 1;
+    const float lr = cb.lrHalfLife > 0.0f
+                         ? fmaxf(cb.lrMin, cb.lr * exp2f(-(float)(cb.step - 1) / cb.lrHalfLife))
+                         : cb.lr;
     const float bc1 = 1.0f - powf(cb.beta1, (float)cb.step);
     const float bc2 = 1.0f - powf(cb.beta2, (float)cb.step);
     const bool ema = cb.emaDecay > 0.0f;
     hipLaunchKernelGGL(k_adam, dim3(blocks256(cb.gridParams)), dim3(256), 0, s, cb.gridP, cb.gridM, cb.gridV, cb.gridG,
-                       cb.gridH, cb.gridParams, gradScale, nDev, nLimit, cb.lr, cb.beta1, cb.beta2, cb.eps, bc1, bc2,
+                       cb.gridH, cb.gridParams, gradScale, nDev, nLimit, lr, cb.beta1, cb.beta2, cb.eps, bc1, bc2,
                        1, ema ? cb.gridE : nullptr, ema ? cb.gridEW : nullptr, ema ? cb.gridHE : nullptr,
                        cb.emaDecay);
     hipLaunchKernelGGL(k_adam, dim3(blocks256(kCacheMlpParams)), dim3(256), 0, s, cb.mlpP, cb.mlpM, cb.mlpV, cb.mlpG,
-                       (__half*)nullptr, (size_t)kCacheMlpParams, gradScale, nDev, nLimit, cb.lr, cb.beta1, cb.beta2,
+                       (__half*)nullptr, (size_t)kCacheMlpParams, gradScale, nDev, nLimit, lr, cb.beta1, cb.beta2,
                        cb.eps, bc1, bc2, 0, ema ? cb.mlpE : nullptr, (float*)nullptr, (__half*)nullptr, cb.emaDecay);
     hipLaunchKernelGGL(k_pack_frags, dim3(blocks256(68 * kFrag)), dim3(256), 0, s, cb.mlpP,
                        reinterpret_cast<_Float16*>(cb.wFwd), reinterpret_cast<_Float16*>(cb.wBwd), 1.0f);

@@ -72,6 +72,7 @@ struct CacheBuffers {
     uint32_t step = 0;
     uint32_t emaSteps = 0;     // updates since the moving average restarted
     float lr = 1e-2f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-15f;
+    float lrHalfLife = 0.0f, lrMin = 0.0f; // learning-rate decay (0: constant)
 };
 
 // In every call, n is the host-side maximum record count; when nDev is not

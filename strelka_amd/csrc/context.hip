@@ -1394,7 +1394,9 @@ STK_API stk_status stk_cache_create(stk_ctx* c, const StkCacheConfig* cfgIn)
     StkCacheConfig cfg = cfgIn ? *cfgIn : defaultCacheConfig();
     if (cfg.log2TableSize < 8 || cfg.log2TableSize > 24 || cfg.baseResolution < 2 ||
         cfg.maxResolution < cfg.baseResolution || cfg.maxResolution > (1u << 20) || cfg.maxBatch == 0 ||
-        cfg.maxBatch > (1u << 24) || !(cfg.learningRate > 0.0f) || !(cfg.emaDecay >= 0.0f && cfg.emaDecay < 1.0f))
+        cfg.maxBatch > (1u << 24) || !(cfg.learningRate > 0.0f) || !(cfg.emaDecay >= 0.0f && cfg.emaDecay < 1.0f) ||
+        !(cfg.lrHalfLife >= 0.0f && cfg.lrHalfLife < 1e9f) ||
+        !(cfg.minLearningRate >= 0.0f && cfg.minLearningRate <= cfg.learningRate))
         return fail(c, STK_E_INVALID, "stk_cache_create: configuration out of range");
     stk_cache_destroy(c);
     stk::CacheBuffers& cb = c->cache;
@@ -1420,6 +1422,8 @@ STK_API stk_status stk_cache_create(stk_ctx* c, const StkCacheConfig* cfgIn)
     cb.gridParams = (size_t)stk::kCacheLevels * T * stk::kCacheFeatures;
     cb.nCap = (cfg.maxBatch + 1023u) & ~1023u;
     cb.lr = cfg.learningRate;
+    cb.lrHalfLife = cfg.lrHalfLife;
+    cb.lrMin = cfg.minLearningRate;
     cb.beta1 = cfg.beta1;
     cb.beta2 = cfg.beta2;
     cb.eps = cfg.epsilon;
