@@ -589,8 +589,7 @@ STK_API stk_status stk_trace_closest_device(stk_ctx* c, const StkRay* rays, uint
     if (n && (!rays || !hits)) return fail(c, STK_E_INVALID, "stk_trace_closest: null buffer");
     if (flatOnlyFlags(c, flags) != STK_OK) return STK_E_STATE;
     (void)hipSetDevice(c->device);
-    STK_HIP(c, stk::launchTraceClosest(bvhView(c), rays, n, hits, flags,
-                                       c->stream));
+    STK_HIP(c, stk::launchTraceClosest(bvhView(c), rays, n, hits, flags, c->stream));
     return STK_OK;
 }
 
@@ -619,11 +618,11 @@ static stk_status traceHost(stk_ctx* c, const StkRay* rays, uint32_t n, StkHit* 
     STK_HIP(c, hipMemcpyAsync(c->rayScratch.ptr, rays, (size_t)n * sizeof(StkRay), hipMemcpyHostToDevice, c->stream));
     StkRay* dr = static_cast<StkRay*>(c->rayScratch.ptr);
     StkHit* dh = static_cast<StkHit*>(c->hitScratch.ptr);
-    if (any)
+    if (any) {
         STK_HIP(c, stk::launchTraceAny(bvhView(c), dr, n, dh, flags, c->stream));
-    else
-        STK_HIP(c, stk::launchTraceClosest(bvhView(c), dr, n, dh, flags,
-                                           c->stream));
+    } else {
+        STK_HIP(c, stk::launchTraceClosest(bvhView(c), dr, n, dh, flags, c->stream));
+    }
     STK_HIP(c, hipMemcpyAsync(hits, dh, (size_t)n * sizeof(StkHit), hipMemcpyDeviceToHost, c->stream));
     STK_HIP(c, hipStreamSynchronize(c->stream));
     return STK_OK;
