@@ -196,14 +196,15 @@ def instanced_bench(args) -> dict:
     spp = 4
 
     def measure(structure: str) -> dict:
-        if structure == "flat_device":
-            ctx.build_bvh_gpu()  # first call loads the build kernels
+        method = {"flat_device": "lbvh", "flat_device_ploc": "ploc"}.get(structure)
+        if method:
+            ctx.build_bvh_gpu(method)  # first call loads the build kernels
             ctx.sync()
         t0 = time.perf_counter()
         if structure == "two_level":
             ctx.build_two_level()
-        elif structure == "flat_device":
-            ctx.build_bvh_gpu()
+        elif method:
+            ctx.build_bvh_gpu(method)
             ctx.sync()
         else:
             ctx.build_bvh()
@@ -242,6 +243,7 @@ def instanced_bench(args) -> dict:
     two = measure("two_level")
     flat = measure("flat")
     flat_dev = measure("flat_device")
+    flat_ploc = measure("flat_device_ploc")
     for b in (d_out, d_rays, d_hits):
         ctx.free(b)
     ctx.close()
@@ -251,7 +253,7 @@ def instanced_bench(args) -> dict:
             "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded PCG meshes and transforms)",
             "config": {"workload": "10,000 instances of 20 seeded meshes under a tree of instances, 3840x2160, "
                                    "4 spp, 6 bounces", "triangles": ntri, "build_s": two["build_s"]},
-            "two_level": two, "flat": flat, "flat_device_built": flat_dev}
+            "two_level": two, "flat": flat, "flat_device_built": flat_dev, "flat_device_clustered": flat_ploc}
 
 
 def load_traffic(kernel: str):
@@ -502,25 +504,28 @@ def main() -> int:
     pt["autotune_4spp_ms"] = {"default": tuned.get("default_ms"), "best": tuned.get("ms")}
     ctx.free(d_out)
 
-    # the same scene with the tree built on the device (linear BVH): build
-    # time and closest-hit rate of the wide walk over it
-    ctx.build_bvh_gpu()
-    ctx.sync()
-    t0 = time.perf_counter()
-    ctx.build_bvh_gpu()
-    ctx.sync()
-    dev_build_s = time.perf_counter() - t0
-    for _ in range(3):
-        ctx.trace_closest_device(d_rays, n, d_hits)
-    ctx.sync()
-    ctx.record(2)
-    for _ in range(args.steps):
-        ctx.trace_closest_device(d_rays, n, d_hits)
-    ctx.record(3)
-    ctx.sync()
-    dev_kernel_ms = ctx.elapsed_ms(2, 3) / args.steps
-    device_build = {"build_s": round(dev_build_s, 4), "wide_kernel_ms": round(dev_kernel_ms, 4),
-                    "mrays_per_s": round(n / (dev_kernel_ms * 1e-3) / 1e6, 2)}
+    # the same scene with the tree built on the device (linear BVH, then
+    # locally-ordered clustering): build time and closest-hit rate of the wide
+    # walk over it
+    device_build = {}
+    for method in ("lbvh", "ploc"):
+        ctx.build_bvh_gpu(method)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.build_bvh_gpu(method)
+        ctx.sync()
+        dev_build_s = time.perf_counter() - t0
+        for _ in range(3):
+            ctx.trace_closest_device(d_rays, n, d_hits)
+        ctx.sync()
+        ctx.record(2)
+        for _ in range(args.steps):
+            ctx.trace_closest_device(d_rays, n, d_hits)
+        ctx.record(3)
+        ctx.sync()
+        dev_kernel_ms = ctx.elapsed_ms(2, 3) / args.steps
+        device_build[method] = {"build_s": round(dev_build_s, 4), "wide_kernel_ms": round(dev_kernel_ms, 4),
+                                "mrays_per_s": round(n / (dev_kernel_ms * 1e-3) / 1e6, 2)}
     ctx.free(d_rays)
     ctx.free(d_hits)
 

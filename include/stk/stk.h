@@ -112,6 +112,14 @@ STK_API stk_status stk_build_bvh(stk_ctx* ctx, const StkBvhOptions* opts);
  * is walked by the wide walk (closest hit, renders) or the stackless
  * skip-link walk (any hit); the binary-walk flag is then refused. */
 STK_API stk_status stk_build_bvh_gpu(stk_ctx* ctx);
+/* stk_build_bvh_gpu with a choice of topology: STK_GPU_BUILD_LBVH (the binary
+ * radix tree above) or STK_GPU_BUILD_PLOC (parallel locally-ordered
+ * clustering, Meister and Bittner 2018: clusters in Morton order merge with
+ * the neighbour, within 32 positions, whose merged box has the least area;
+ * a tree of lower area cost than the radix tree, built in the same time). */
+#define STK_GPU_BUILD_LBVH 0u
+#define STK_GPU_BUILD_PLOC 1u
+STK_API stk_status stk_build_bvh_gpu_method(stk_ctx* ctx, uint32_t method);
 /* Builds a two-level structure instead of the flat BVH (no reference
  * counterpart: the reference flattens every instance into one tree,
  * src/render/ptrender.cpp:460-532): one tree per distinct mesh (instance index

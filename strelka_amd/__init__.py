@@ -311,9 +311,11 @@ class Context:
         opts = BvhOptions(max_depth, bins, 1.0, 1.0)
         self._check(_LIB.stk_build_bvh(self._h, C.byref(opts)), "stk_build_bvh")
 
-    def build_bvh_gpu(self):
-        """stk_build_bvh_gpu: the flat tree built on the device (linear BVH)."""
-        self._check(_LIB.stk_build_bvh_gpu(self._h), "stk_build_bvh_gpu")
+    def build_bvh_gpu(self, method: str = "lbvh"):
+        """stk_build_bvh_gpu_method: the flat tree built on the device, as a
+        linear BVH ("lbvh") or by locally-ordered clustering ("ploc")."""
+        m = {"lbvh": 0, "ploc": 1}[method]
+        self._check(_LIB.stk_build_bvh_gpu_method(self._h, m), "stk_build_bvh_gpu_method")
 
     def build_two_level(self, max_depth: int = 32, bins: int = 32):
         """stk_build_two_level: per-mesh trees under a tree of instances."""

@@ -204,6 +204,7 @@ PROTOTYPES = [
     ("stk_bvh_num_emitters", U32, [P]),
     ("stk_build_two_level", C.c_int, [P, C.POINTER(BvhOptions)]),
     ("stk_build_bvh_gpu", C.c_int, [P]),
+    ("stk_build_bvh_gpu_method", C.c_int, [P, C.c_uint32]),
     ("stk_bvh_save_nodes", C.c_int, [P, U32, C.c_char_p]),
     ("stk_bvh_load_nodes", C.c_int, [C.c_char_p, P, U32, C.POINTER(C.c_uint32)]),
     ("stk_image_write_pfm", C.c_int, [C.c_char_p, P, U32, U32]),

@@ -510,9 +510,13 @@ STK_API stk_status stk_get_bvh(stk_ctx* c, StkBvhNode* nodes, uint32_t cap, uint
     return STK_OK;
 }
 
-STK_API stk_status stk_build_bvh_gpu(stk_ctx* c)
+STK_API stk_status stk_build_bvh_gpu(stk_ctx* c) { return stk_build_bvh_gpu_method(c, STK_GPU_BUILD_LBVH); }
+
+STK_API stk_status stk_build_bvh_gpu_method(stk_ctx* c, uint32_t method)
 {
     if (!validCtx(c)) return STK_E_INVALID;
+    if (method != STK_GPU_BUILD_LBVH && method != STK_GPU_BUILD_PLOC)
+        return fail(c, STK_E_INVALID, "stk_build_bvh_gpu: unknown method");
     if (!c->haveGeometry) return fail(c, STK_E_STATE, "stk_build_bvh_gpu: no geometry set");
     const uint32_t ninst = (uint32_t)c->hinst.size();
     std::vector<uint32_t> start(ninst + 1, 0);
@@ -548,7 +552,7 @@ STK_API stk_status stk_build_bvh_gpu(stk_ctx* c)
     c->haveBvh = false;
     const hipError_t e = stk::gpuBuildBvh(static_cast<const StkVertex*>(c->vb.ptr), static_cast<const uint32_t*>(c->ib.ptr),
                                           static_cast<const StkInstance*>(c->inst.ptr), start.data(), ninst,
-                                          (uint32_t)total, alloc, r, err, c->stream);
+                                          (uint32_t)total, method, alloc, r, err, c->stream);
     (void)hipStreamSynchronize(c->stream);
     for (DevBuf& b : temps) (void)devRelease(c, b);
     if (e != hipSuccess) {

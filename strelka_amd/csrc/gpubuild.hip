@@ -13,6 +13,13 @@
 //      index), parents of every node;
 //   5. boxes and leaf counts bottom-up (the second child to arrive at a node
 //      finishes it; agent-scope acquire/release atomics order the writes);
+//   4'-5'. or, for the clustered tree (kGpuBuildPloc), parallel locally-ordered
+//      clustering (Meister and Bittner 2018, "Parallel Locally-Ordered
+//      Clustering for Bounding Volume Hierarchy Construction"): the clusters
+//      in Morton order each pick the neighbour within kPlocRadius positions
+//      whose merged box has the smallest surface area, mutual pairs merge
+//      into a new inner node, and the survivors are compacted in order, until
+//      one cluster is left;
 //   6. preorder indices top-down, one kernel per level: left child p + 1,
 //      right child p + 2 x leaves(left); skip link = p + nodes of the subtree;
 //   7. the skip-link, packed and triangle arrays;
@@ -211,6 +218,130 @@ __global__ __launch_bounds__(kB) void k_radix_tree(const uint64_t* __restrict__ 
     else
         parentInner[right] = (uint32_t)i;
     if (i == 0) parentInner[0] = kNone;
+}
+
+// 4'-5'. clustered tree.  Surface area without the factor 2 (only compared).
+constexpr int kPlocRadius = 32;
+
+__device__ __forceinline__ float half_area(float4 lo, float4 hi)
+{
+    const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+    return (dx * dy + dy * dz) + dz * dx;
+}
+
+__device__ __forceinline__ float4 box_min(float4 a, float4 b)
+{
+    return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.0f);
+}
+
+__device__ __forceinline__ float4 box_max(float4 a, float4 b)
+{
+    return make_float4(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z), 0.0f);
+}
+
+// clusters = the sorted leaves
+__global__ __launch_bounds__(kB) void k_ploc_init(uint32_t n, const uint32_t* __restrict__ vals,
+                                                  const float4* __restrict__ triLo, const float4* __restrict__ triHi,
+                                                  uint32_t* __restrict__ C, float4* __restrict__ lo,
+                                                  float4* __restrict__ hi, uint32_t* __restrict__ cnt)
+{
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t t = vals[k];
+    C[k] = k | kLeafBit;
+    lo[k] = make_float4(triLo[t].x, triLo[t].y, triLo[t].z, 0.0f);
+    hi[k] = make_float4(triHi[t].x, triHi[t].y, triHi[t].z, 0.0f);
+    cnt[k] = 1;
+}
+
+// Order of candidate pairs {i, j} after the area: nearer positions first,
+// then pairs whose lower position is even, then the lower position.  A
+// strict total order on pairs, so the least pair of a round is mutual (every
+// round merges), and equal boxes pair up as (0, 1), (2, 3), ...
+__device__ __forceinline__ uint32_t pair_rank(uint32_t i, uint32_t j)
+{
+    const uint32_t lo = min(i, j);
+    return (max(i, j) - lo) * 2u + (lo & 1u);
+}
+
+// nearest neighbour of each cluster within the radius
+__global__ __launch_bounds__(kB) void k_ploc_nn(uint32_t m, const float4* __restrict__ lo,
+                                                const float4* __restrict__ hi, uint32_t* __restrict__ nn)
+{
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= m) return;
+    const float4 li = lo[i], hi_ = hi[i];
+    const uint32_t j0 = i > (uint32_t)kPlocRadius ? i - kPlocRadius : 0u;
+    const uint32_t j1 = min(m - 1, i + (uint32_t)kPlocRadius);
+    float best = INFINITY;
+    uint32_t bj = i == 0 ? 1u : i - 1, bk = 0xFFFFFFFFu;
+    for (uint32_t j = j0; j <= j1; ++j) {
+        if (j == i) continue;
+        const float a = half_area(box_min(li, lo[j]), box_max(hi_, hi[j]));
+        const uint32_t k = pair_rank(i, j);
+        // (for equal area and rank the lower position is the smaller j: j
+        // rises through the window)
+        if (a < best || (a == best && k < bk)) {
+            best = a;
+            bk = k;
+            bj = j;
+        }
+    }
+    nn[i] = bj;
+}
+
+// merge[i]: i merges with its neighbour into a new node (mutual, i the
+// smaller); keep[i]: position i survives (all but the larger of a pair)
+__global__ __launch_bounds__(kB) void k_ploc_flags(uint32_t m, const uint32_t* __restrict__ nn,
+                                                   uint32_t* __restrict__ merge, uint32_t* __restrict__ keep)
+{
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t j = nn[i];
+    const bool mutual = nn[j] == i;
+    merge[i] = (mutual && i < j) ? 1u : 0u;
+    keep[i] = (mutual && i > j) ? 0u : 1u;
+}
+
+// writes the next round's clusters and the new inner nodes; the k-th merge of
+// the build gets inner index ninner - 1 - k, so the last one is the root, 0
+__global__ __launch_bounds__(kB) void k_ploc_write(uint32_t m, uint32_t base, uint32_t ninner,
+                                                   const uint32_t* __restrict__ C, const float4* __restrict__ lo,
+                                                   const float4* __restrict__ hi, const uint32_t* __restrict__ cnt,
+                                                   const uint32_t* __restrict__ nn, const uint32_t* __restrict__ merge,
+                                                   const uint32_t* __restrict__ keep,
+                                                   const uint32_t* __restrict__ mrank, const uint32_t* __restrict__ pos,
+                                                   uint32_t* __restrict__ C2, float4* __restrict__ lo2,
+                                                   float4* __restrict__ hi2, uint32_t* __restrict__ cnt2,
+                                                   uint32_t* __restrict__ childL, uint32_t* __restrict__ childR,
+                                                   uint32_t* __restrict__ count, float4* __restrict__ nodeLo,
+                                                   float4* __restrict__ nodeHi, uint32_t* __restrict__ mOut)
+{
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= m) return;
+    if (i == m - 1) *mOut = pos[i] + keep[i];
+    if (!keep[i]) return;
+    const uint32_t p = pos[i];
+    if (merge[i]) {
+        const uint32_t j = nn[i];
+        const uint32_t id = ninner - 1 - (base + mrank[i]);
+        const float4 l = box_min(lo[i], lo[j]), h = box_max(hi[i], hi[j]);
+        const uint32_t c = cnt[i] + cnt[j];
+        childL[id] = C[i];
+        childR[id] = C[j];
+        count[id] = c;
+        nodeLo[id] = l;
+        nodeHi[id] = h;
+        C2[p] = id;
+        lo2[p] = l;
+        hi2[p] = h;
+        cnt2[p] = c;
+    } else {
+        C2[p] = C[i];
+        lo2[p] = lo[i];
+        hi2[p] = hi[i];
+        cnt2[p] = cnt[i];
+    }
 }
 
 // 5. boxes and leaf counts bottom-up
@@ -583,8 +714,9 @@ __global__ void k_set_u32(uint32_t* p, uint32_t v, size_t n)
 } // namespace
 
 hipError_t gpuBuildBvh(const StkVertex* vb, const uint32_t* ib, const StkInstance* inst, const uint32_t* instStart,
-                       uint32_t ninst, uint32_t ntri, const std::function<void*(int, size_t)>& alloc,
-                       GpuBvhResult& out, std::string& err, hipStream_t s)
+                       uint32_t ninst, uint32_t ntri, uint32_t method,
+                       const std::function<void*(int, size_t)>& alloc, GpuBvhResult& out, std::string& err,
+                       hipStream_t s)
 {
     out = GpuBvhResult{};
     const uint32_t n = ntri;
@@ -652,14 +784,74 @@ hipError_t gpuBuildBvh(const StkVertex* vb, const uint32_t* ib, const StkInstanc
         const uint32_t zero = 0;
         if ((e = hipMemcpyAsync(preLeaf, &zero, 4, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
     } else {
-        hipLaunchKernelGGL(k_set_u32, dim3(std::min<uint32_t>(blocksFor(ninner), 1024)), dim3(kB), 0, s, parentLeaf,
-                           kNone, (size_t)n);
-        hipLaunchKernelGGL(k_radix_tree, dim3(blocksFor(ninner)), dim3(kB), 0, s, keys, (int)n, childL, childR,
-                           parentInner, parentLeaf, count);
-        hipLaunchKernelGGL(k_set_u32, dim3(std::min<uint32_t>(blocksFor(ninner), 1024)), dim3(kB), 0, s, flags, 0u,
-                           (size_t)ninner);
-        hipLaunchKernelGGL(k_refit, dim3(blocksFor(n)), dim3(kB), 0, s, n, vals, triLo, triHi, childL, childR,
-                           parentInner, parentLeaf, nodeLo, nodeHi, flags);
+        if (method == kGpuBuildPloc) {
+            // clusters, double-buffered, and the per-round arrays
+            STK_GB_ALLOC(uint32_t, cA, n, temp((size_t)n * 4))
+            STK_GB_ALLOC(uint32_t, cB, n, temp((size_t)n * 4))
+            STK_GB_ALLOC(float4, loA, n, temp((size_t)n * 16))
+            STK_GB_ALLOC(float4, loB, n, temp((size_t)n * 16))
+            STK_GB_ALLOC(float4, hiA, n, temp((size_t)n * 16))
+            STK_GB_ALLOC(float4, hiB, n, temp((size_t)n * 16))
+            STK_GB_ALLOC(uint32_t, nA, n, temp((size_t)n * 4))
+            STK_GB_ALLOC(uint32_t, nB, n, temp((size_t)n * 4))
+            STK_GB_ALLOC(uint32_t, nn, n, temp((size_t)n * 4))
+            STK_GB_ALLOC(uint32_t, mergeF, n, temp((size_t)n * 4))
+            STK_GB_ALLOC(uint32_t, keepF, n, temp((size_t)n * 4))
+            STK_GB_ALLOC(uint32_t, mrank, n, temp((size_t)n * 4))
+            STK_GB_ALLOC(uint32_t, pos, n, temp((size_t)n * 4))
+            size_t pBytes = 0;
+            if ((e = rocprim::exclusive_scan(nullptr, pBytes, mergeF, mrank, 0u, n, rocprim::plus<uint32_t>(), s)) !=
+                hipSuccess)
+                return e;
+            STK_GB_ALLOC(void, pTemp, pBytes, temp(pBytes))
+            hipLaunchKernelGGL(k_ploc_init, dim3(blocksFor(n)), dim3(kB), 0, s, n, vals, triLo, triHi, cA, loA, hiA,
+                               nA);
+            uint32_t m = n, base = 0, rounds = 0;
+            while (m > 1) {
+                if (++rounds > 4096) {
+                    (void)hipStreamSynchronize(s);
+                    err = "clustering did not converge (non-finite triangle boxes?)";
+                    return hipErrorInvalidValue;
+                }
+                hipLaunchKernelGGL(k_ploc_nn, dim3(blocksFor(m)), dim3(kB), 0, s, m, loA, hiA, nn);
+                hipLaunchKernelGGL(k_ploc_flags, dim3(blocksFor(m)), dim3(kB), 0, s, m, nn, mergeF, keepF);
+                if ((e = rocprim::exclusive_scan(pTemp, pBytes, mergeF, mrank, 0u, m, rocprim::plus<uint32_t>(), s)) !=
+                    hipSuccess)
+                    return e;
+                if ((e = rocprim::exclusive_scan(pTemp, pBytes, keepF, pos, 0u, m, rocprim::plus<uint32_t>(), s)) !=
+                    hipSuccess)
+                    return e;
+                hipLaunchKernelGGL(k_ploc_write, dim3(blocksFor(m)), dim3(kB), 0, s, m, base, ninner, cA, loA, hiA, nA,
+                                   nn, mergeF, keepF, mrank, pos, cB, loB, hiB, nB, childL, childR, count, nodeLo,
+                                   nodeHi, counters + 3);
+                uint32_t m2 = 0;
+                if ((e = hipMemcpyAsync(&m2, counters + 3, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) {
+                    (void)hipStreamSynchronize(s);
+                    return e;
+                }
+                if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+                if (m2 >= m || m2 == 0) {
+                    err = "clustering made no progress";
+                    return hipErrorInvalidValue;
+                }
+                base += m - m2;
+                m = m2;
+                std::swap(cA, cB);
+                std::swap(loA, loB);
+                std::swap(hiA, hiB);
+                std::swap(nA, nB);
+            }
+            out.rounds = rounds;
+        } else {
+            hipLaunchKernelGGL(k_set_u32, dim3(std::min<uint32_t>(blocksFor(ninner), 1024)), dim3(kB), 0, s,
+                               parentLeaf, kNone, (size_t)n);
+            hipLaunchKernelGGL(k_radix_tree, dim3(blocksFor(ninner)), dim3(kB), 0, s, keys, (int)n, childL, childR,
+                               parentInner, parentLeaf, count);
+            hipLaunchKernelGGL(k_set_u32, dim3(std::min<uint32_t>(blocksFor(ninner), 1024)), dim3(kB), 0, s, flags, 0u,
+                               (size_t)ninner);
+            hipLaunchKernelGGL(k_refit, dim3(blocksFor(n)), dim3(kB), 0, s, n, vals, triLo, triHi, childL, childR,
+                               parentInner, parentLeaf, nodeLo, nodeHi, flags);
+        }
         hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(kB), 0, s, counters + 2, 0u, (size_t)1);
         hipLaunchKernelGGL(k_check_inner, dim3(blocksFor(ninner)), dim3(kB), 0, s, ninner, vals, triLo, triHi, childL,
                            childR, nodeLo, nodeHi, count, counters + 2);

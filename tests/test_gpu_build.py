@@ -1,9 +1,9 @@
-"""The flat tree built on the device (stk_build_bvh_gpu, a linear BVH): its
-skip-link array is a valid reference layout (the host packer accepts it and
-the reference walk over it equals brute force), and every GPU walk over it -
-the wide and binary closest-hit walks, the any-hit walk, the path tracer -
-returns exactly what the oracle's reference walks return over the same
-nodes."""
+"""The flat tree built on the device (stk_build_bvh_gpu_method: a linear BVH
+or a tree from locally-ordered clustering): its skip-link array is a valid
+reference layout (the host packer accepts it and the reference walk over it
+equals brute force), and every GPU walk over it - the wide and binary
+closest-hit walks, the any-hit walk, the path tracer - returns exactly what
+the oracle's reference walks return over the same nodes."""
 import time
 
 import numpy as np
@@ -29,12 +29,23 @@ def ctx():
     c.close()
 
 
+def sah_cost(nodes):
+    """Sum of inner-node box areas over the root's (the walk cost the SAH
+    estimates, up to the leaf term every one-triangle-per-leaf tree shares)."""
+    inner = nodes["instId"] == -1
+    lo, hi = nodes["minBounds"][inner].astype(np.float64), nodes["maxBounds"][inner].astype(np.float64)
+    d = hi - lo
+    area = d[:, 0] * d[:, 1] + d[:, 1] * d[:, 2] + d[:, 2] * d[:, 0]
+    return float(area.sum() / area[0])
+
+
+@pytest.mark.parametrize("method", ["lbvh", "ploc"])
 @pytest.mark.parametrize("config", [1, 6, 7])
-def test_gpu_tree_is_a_valid_reference_layout(ctx, config):
-    print(f"case: config {config}, device build, closest and any hits", flush=True)
+def test_gpu_tree_is_a_valid_reference_layout(ctx, config, method):
+    print(f"case: config {config}, device build ({method}), closest and any hits", flush=True)
     s = sa.SceneData.generate(config)
     ctx.set_scene(s)
-    ctx.build_bvh_gpu()
+    ctx.build_bvh_gpu(method)
     nodes = ctx.get_bvh()
     assert nodes.size == 2 * int(s.inst["indexCount"].sum() // 3) - 1
     sa.HostBvh.from_nodes(s, nodes)  # the host packer validates preorder, skip links and leaves
@@ -53,17 +64,20 @@ def test_gpu_tree_is_a_valid_reference_layout(ctx, config):
     assert np.array_equal(a_gpu[["instId", "primId", "t"]], a_ref[["instId", "primId", "t"]])
 
 
-def test_gpu_tree_config2_and_render(ctx):
-    print("case: 1.16M-triangle scene, device build, 1080p rays subsample, 16x16x4 render crop", flush=True)
+@pytest.mark.parametrize("method", ["lbvh", "ploc"])
+def test_gpu_tree_config2_and_render(ctx, method):
+    print(f"case: 1.16M-triangle scene, device build ({method}), 1080p rays subsample, 16x16x4 render crop",
+          flush=True)
     s = sa.SceneData.generate(2)
     ctx.set_scene(s)
     t0 = time.perf_counter()
-    ctx.build_bvh_gpu()
+    ctx.build_bvh_gpu(method)
     print(f"device build {time.perf_counter() - t0:.3f} s (first call includes kernel loading)")
     t0 = time.perf_counter()
-    ctx.build_bvh_gpu()
+    ctx.build_bvh_gpu(method)
     print(f"device build {time.perf_counter() - t0:.3f} s")
     nodes = ctx.get_bvh()
+    print(f"inner-area cost {sah_cost(nodes):.1f}")
     o = Oracle.from_scene(s, nodes)
     rays = s.jittered_rays(0x5354524B00000002)[::37]
     assert same(ctx.trace_closest(rays), o.trace(rays, MODE_REF)[0])
@@ -77,3 +91,17 @@ def test_gpu_tree_config2_and_render(ctx):
     frac, _ = compare_samples(gpu[y0:y0 + 16, x0:x0 + 16].reshape(-1), cpu)
     print(f"divergent samples {frac:.5f}")
     assert frac <= MAX_DIVERGENT
+
+
+def test_clustered_tree_is_cheaper_than_the_linear_one(ctx):
+    print("case: config 2, inner-area cost of the clustered and linear device trees", flush=True)
+    s = sa.SceneData.generate(2)
+    ctx.set_scene(s)
+    costs = {}
+    for method in ("lbvh", "ploc"):
+        ctx.build_bvh_gpu(method)
+        costs[method] = sah_cost(ctx.get_bvh())
+    ctx.build_bvh()
+    costs["host sah"] = sah_cost(ctx.get_bvh())
+    print(costs)
+    assert costs["ploc"] < costs["lbvh"]
