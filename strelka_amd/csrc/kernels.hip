@@ -37,20 +37,31 @@ __device__ __forceinline__ RayF load_ray(const StkRay* rays, uint32_t i)
 
 static_assert(2 * kWideStack <= kStackDepth, "the wide walk's two stacks share the packed walk's LDS");
 
-// Closest hit: the compressed 8-wide walk, or the packed BVH2 walk when asked
-// (packed != 0) or when the scene has no usable wide tree.
+// Closest hit over the compressed 8-wide tree (two LDS node-group stacks per
+// lane).  The binary walk has a kernel of its own (56 registers instead of
+// the wide walk's 67).
 __global__ __launch_bounds__(kBlock) void k_trace_closest(BvhView bvh, const StkRay* __restrict__ rays, uint32_t n,
-                                                          StkHit* __restrict__ hits, int packed)
+                                                          StkHit* __restrict__ hits)
+{
+    __shared__ uint32_t stack[2 * kWideStack * kBlock];
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const RayF r = load_ray(rays, i);
+    HitF h;
+    closest_wide(bvh, bvh.wideRoot, r, stack + threadIdx.x, stack + kWideStack * kBlock + threadIdx.x, kBlock, h);
+    store_hit(hits, i, h);
+}
+
+// Closest hit over the packed binary tree.
+__global__ __launch_bounds__(kBlock) void k_trace_closest_packed(BvhView bvh, const StkRay* __restrict__ rays,
+                                                                 uint32_t n, StkHit* __restrict__ hits)
 {
     __shared__ uint32_t stack[kStackDepth * kBlock];
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (i >= n) return;
     const RayF r = load_ray(rays, i);
     HitF h;
-    if (packed || bvh.wideRoot == STK_INVALID_INDEX)
-        closest_packed(bvh, bvh.hdr, r, stack + threadIdx.x, kBlock, h);
-    else
-        closest_wide(bvh, bvh.wideRoot, r, stack + threadIdx.x, stack + kWideStack * kBlock + threadIdx.x, kBlock, h);
+    closest_packed(bvh, bvh.hdr, r, stack + threadIdx.x, kBlock, h);
     store_hit(hits, i, h);
 }
 
@@ -238,9 +249,10 @@ hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n
         return hipErrorInvalidValue; // deeper than the packed walk's stack
     } else if ((flags & STK_TRACE_REFERENCE_ORDER) || (!bvh.packedOk && bvh.wideRoot == STK_INVALID_INDEX)) {
         hipLaunchKernelGGL(k_trace_closest_ref, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
+    } else if ((flags & STK_TRACE_BVH2) || bvh.wideRoot == STK_INVALID_INDEX) {
+        hipLaunchKernelGGL(k_trace_closest_packed, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
     } else {
-        hipLaunchKernelGGL(k_trace_closest, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits,
-                           (flags & STK_TRACE_BVH2) ? 1 : 0);
+        hipLaunchKernelGGL(k_trace_closest, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
     }
     return hipGetLastError();
 }
