@@ -1404,11 +1404,14 @@ STK_API stk_status stk_cache_create(stk_ctx* c, const StkCacheConfig* cfgIn)
     stk_cache_destroy(c);
     stk::CacheBuffers& cb = c->cache;
     stk::GridDesc& gd = cb.grid;
+    // the scene box: the flat tree's root box, or the box a two-level
+    // structure keeps in the header (its root field is then unset)
     const StkBvhHeader& h = c->bvh.header;
+    const bool box = h.root >= 0 || c->twoLevel;
     float ext = 0.0f;
     for (int a = 0; a < 3; ++a) {
-        gd.lo[a] = (h.root >= 0) ? h.lo[a] : 0.0f;
-        if (h.root >= 0) ext = std::max(ext, h.hi[a] - h.lo[a]);
+        gd.lo[a] = box ? h.lo[a] : 0.0f;
+        if (box) ext = std::max(ext, h.hi[a] - h.lo[a]);
     }
     gd.scale = ext > 0.0f ? 1.0f / ext : 1.0f;
     const uint64_t T = 1ull << cfg.log2TableSize;

@@ -99,3 +99,28 @@ def test_full_instanced_scene_matches_flat_tree(ctx):
     differ = (a["instId"] != b["instId"]) | (a["primId"] != b["primId"])
     print(f"rays {rays.size}, hits {(a['instId'] != INVALID).sum()}, differing triangle at equal t {differ.sum()}")
     assert differ.sum() <= rays.size // 10000
+
+
+def test_cache_over_two_level_scene(ctx, small):
+    """The radiance cache of a two-level scene covers the scene box (the same
+    domain the flat tree of that scene gives it) and serves a cached render."""
+    print("case: 60-instance scene, cache domain and a cached 2-frame render over the two-level structure",
+          flush=True)
+    s, _, _ = small
+    ctx.set_scene(s)
+    ctx.build_bvh()
+    ctx.cache_create(log2_table=14)
+    flat = ctx.cache_info()
+    ctx.build_two_level()
+    ctx.cache_create(log2_table=14)
+    tl = ctx.cache_info()
+    assert abs(tl.scale / flat.scale - 1.0) < 1e-3, (tl.scale, flat.scale)
+    assert np.allclose(list(tl.lo), list(flat.lo), atol=1e-3 / flat.scale)
+    w, h = s.camera.dimension[0], s.camera.dimension[1]
+    for f in range(2):
+        p = sa.render_params(s.camera, spp=1, max_depth=6, frame=f, sampling=1, sky=s.sky, cache_mode=1,
+                             cache_query_depth=1, cache_train_paths=2048)
+        img = ctx.render(p)
+    cnt = ctx.counters()
+    assert 0 < cnt[3] <= w * h and cnt[4] > 0
+    assert np.all(np.isfinite(img))
