@@ -240,19 +240,21 @@ def instanced_bench(args) -> dict:
                 "primary_kernel_ms": round(trace_ms, 4), "launch": tuned["best"],
                 "autotune_4spp_ms": {"default": tuned.get("default_ms"), "best": tuned.get("ms")}}
 
-    two = measure("two_level")
-    flat = measure("flat")
-    flat_dev = measure("flat_device")
-    flat_ploc = measure("flat_device_ploc")
+    want = set(args.structures.split(","))
+    two = measure("two_level") if "two_level" in want else None
+    flat = measure("flat") if "flat" in want else None
+    flat_dev = measure("flat_device") if "flat_device" in want else None
+    flat_ploc = measure("flat_device_ploc") if "flat_device_ploc" in want else None
+    head = next(r for r in (two, flat, flat_dev, flat_ploc) if r is not None)
     for b in (d_out, d_rays, d_hits):
         ctx.free(b)
     ctx.close()
     return {"metric": "samples/s, path tracing of the large instanced scene",
-            "value": two["samples_per_s"], "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": 1, "ms_per_step": two["frame_ms"], "higher_is_better": True, "scaling": "weak",
+            "value": head["samples_per_s"], "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": 1, "ms_per_step": head["frame_ms"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded PCG meshes and transforms)",
             "config": {"workload": "10,000 instances of 20 seeded meshes under a tree of instances, 3840x2160, "
-                                   "4 spp, 6 bounces", "triangles": ntri, "build_s": two["build_s"]},
+                                   "4 spp, 6 bounces", "triangles": ntri, "build_s": head["build_s"]},
             "two_level": two, "flat": flat, "flat_device_built": flat_dev, "flat_device_clustered": flat_ploc}
 
 
@@ -283,7 +285,12 @@ def cache_study(args) -> dict:
     cam.invDimension[0], cam.invDimension[1] = 1.0 / W, 1.0 / H
     ctx = sa.Context(0)
     ctx.set_scene(scene)
-    ctx.build_bvh()
+    if args.study_structure == "two_level":
+        ctx.build_two_level()
+    elif args.study_structure == "device":
+        ctx.build_bvh_gpu()
+    else:
+        ctx.build_bvh()
     # two pass streams: plain frames overlap each other (cached frames run on
     # one stream: each trains on the previous frame's network)
     ctx.set_launch_config(block_size=64, tile=8, pass_streams=2)
@@ -293,17 +300,20 @@ def cache_study(args) -> dict:
     out = np.zeros((H, W, 4), np.float32)
 
     def params(frame, mode, qd=2, paths=default_paths, min_rough=0.0, steps=1):
-        return sa.render_params(cam, spp=1, max_depth=8, frame=frame, sampling=1, sky=scene.sky, tonemap=1,
+        return sa.render_params(cam, spp=args.study_spp, max_depth=args.study_depth, frame=frame, sampling=1,
+                                sky=scene.sky, tonemap=1,
                                 cache_mode=mode, cache_query_depth=qd, cache_train_paths=paths,
                                 cache_min_roughness=min_rough, cache_train_steps=steps)
 
     ctx.reset_accumulation()
     t0 = time.perf_counter()
-    for f in range(args.study_ref_spp):
+    ref_frames = (args.study_ref_spp + args.study_spp - 1) // args.study_spp
+    for f in range(ref_frames):
         ctx.render_device(params(1_000_000 + f, 0), d_out)
-        if f % 256 == 255:
+        if f % 64 == 63:
             ctx.sync()
-            print(f"reference {f + 1}/{args.study_ref_spp} spp", file=sys.stderr, flush=True)
+            print(f"reference {(f + 1) * args.study_spp}/{ref_frames * args.study_spp} spp", file=sys.stderr,
+                  flush=True)
     ctx.sync()
     ref_s = time.perf_counter() - t0
     ref_tm = ctx.download(d_out, out.copy())[..., :3]
@@ -364,7 +374,8 @@ def cache_study(args) -> dict:
     ctx.free(d_out)
     ctx.close()
     best = min((r for r in results if r["time_to_target_s"]), key=lambda r: r["time_to_target_s"], default=None)
-    return {"study": "cache", "scene": args.study_scene, "resolution": [W, H], "target_rmse_tonemapped": args.study_target,
+    return {"study": "cache", "scene": args.study_scene, "structure": args.study_structure,
+            "spp_per_frame": args.study_spp, "max_depth": args.study_depth, "resolution": [W, H], "target_rmse_tonemapped": args.study_target,
             "reference_spp": args.study_ref_spp, "reference_s": round(ref_s, 2),
             "default_train_paths_per_frame": default_paths,
             "time_to_target_s": {"plain": tp, "cached_best": best["time_to_target_s"] if best else None},
@@ -387,6 +398,8 @@ def main() -> int:
     ap.add_argument("--workload", choices=("trace", "batch", "instanced"), default="trace",
                     help="trace: closest-hit on the 1M-triangle scene (+ its 64-spp path-traced frame); "
                          "batch: the 32-scene batched path tracer; instanced: the 19M-triangle instanced scene")
+    ap.add_argument("--structures", default="two_level,flat,flat_device,flat_device_ploc",
+                    help="instanced workload: the structures to measure (the first one gives value)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (calibrates memory counters)")
     ap.add_argument("--cache-study", action="store_true",
@@ -397,6 +410,10 @@ def main() -> int:
                     help="seeded scene of the study: 2 (the 1M-triangle heightfield + spheres, open sky) or 7 "
                          "(a closed room lit by a small ceiling light)")
     ap.add_argument("--study-frames", type=int, default=256)
+    ap.add_argument("--study-spp", type=int, default=1, help="samples per pixel per frame of the study")
+    ap.add_argument("--study-depth", type=int, default=8, help="bounces of the study's paths")
+    ap.add_argument("--study-structure", choices=("flat", "two_level", "device"), default="flat",
+                    help="acceleration structure of the study's scene")
     ap.add_argument("--study-ref-spp", type=int, default=2048)
     ap.add_argument("--study-target", type=float, default=0.01)
     ap.add_argument("--study-lr", type=float, default=1e-2)
