@@ -870,9 +870,10 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
             c->graphKey = key;
         }
     }
-    uint32_t k = 0; // pass number of this call
-    for (int s = 0; s < p->pt.spp; ++s) {
-        for (size_t begin = 0; begin < slots; begin += Q, ++k) {
+    uint32_t k = 0;              // pass number of this call
+    hipError_t issued = hipSuccess; // first failure of the multi-stream passes
+    for (int s = 0; s < p->pt.spp && issued == hipSuccess; ++s) {
+        for (size_t begin = 0; begin < slots && issued == hipSuccess; begin += Q, ++k) {
             stk::PassState pass{};
             pass.frameNumber = p->pt.frameNumber;
             pass.sampleNum = (uint32_t)s;
@@ -894,10 +895,11 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
                 // sums its samples in order and no sample is overwritten late
                 const uint32_t i = (first + k) % nstreams;
                 const bool chained = k > 0 || c->lastStreams > 1;
-                STK_HIP(c, stk::launchSetPass(c->xPs[i], pass, c->xStream[i]));
-                STK_HIP(c, stk::launchPathTracePass(sv, lp, c->launch, c->xPs[i], accum, samples, nullptr,
-                                                    c->xStream[i], chained ? c->finEv[c->lastPassStream] : nullptr,
-                                                    c->finEv[i]));
+                issued = stk::launchSetPass(c->xPs[i], pass, c->xStream[i]);
+                if (issued == hipSuccess)
+                    issued = stk::launchPathTracePass(sv, lp, c->launch, c->xPs[i], accum, samples, nullptr,
+                                                      c->xStream[i],
+                                                      chained ? c->finEv[c->lastPassStream] : nullptr, c->finEv[i]);
                 c->lastPassStream = i;
                 c->lastStreams = nstreams;
                 continue;
@@ -911,12 +913,21 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
         }
     }
     if (multi) {
+        // join the pass streams into the context stream; after a failed
+        // launch also wait for them, so nothing of this call is left running
+        hipError_t joined = hipSuccess;
         for (uint32_t i = 0; i < nstreams; ++i) {
-            STK_HIP(c, hipEventRecord(c->joinEv[i], c->xStream[i]));
-            STK_HIP(c, hipStreamWaitEvent(c->stream, c->joinEv[i], 0));
+            hipError_t e = hipEventRecord(c->joinEv[i], c->xStream[i]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, c->joinEv[i], 0);
+            if (e != hipSuccess && joined == hipSuccess) joined = e;
         }
         c->passTurn += k;
         ++c->renderCalls;
+        if (issued != hipSuccess || joined != hipSuccess) {
+            for (uint32_t i = 0; i < nstreams; ++i) (void)hipStreamSynchronize(c->xStream[i]);
+            (void)hipStreamSynchronize(c->stream);
+            return hipFail(c, issued != hipSuccess ? issued : joined, "render: a path-tracing pass");
+        }
     } else {
         c->lastStreams = 1;
         c->lastPassStream = 0;
