@@ -99,6 +99,14 @@ STK_API stk_status stk_set_geometry(stk_ctx* ctx, const StkVertex* vb, uint32_t 
                                     uint32_t numIndices, const StkInstance* inst, uint32_t numInstances);
 STK_API stk_status stk_set_lights(stk_ctx* ctx, const StkLight* lights, uint32_t numLights);
 STK_API stk_status stk_set_materials(stk_ctx* ctx, const StkMaterial* mats, uint32_t numMaterials);
+/* Textures the materials' albedoTexture fields refer to (no counterpart in
+ * the non-MDL reference path; the reference's MDL materials sample textures
+ * through shaders/mdl_runtime.hlsl).  texels: numTexels RGBA32F values (4
+ * floats each); every texture must lie inside them, and be at most
+ * 16384 x 16384.  Single-scene renders only (a batch refuses textured
+ * materials). */
+STK_API stk_status stk_set_textures(stk_ctx* ctx, const StkTexture* textures, uint32_t numTextures,
+                                    const float* texels, uint64_t numTexels);
 /* Builds the BVH on the host (binned SAH, one triangle per leaf, preorder plus
  * skip links) and uploads it.  opts may be NULL for defaults. */
 STK_API stk_status stk_build_bvh(stk_ctx* ctx, const StkBvhOptions* opts);

@@ -113,15 +113,31 @@ typedef struct StkPathTracerParam {
 
 /* Material record of this library, 32 B.  The reference binds MDL argument
  * blocks here (shaders/pathtracerMdl.hlsl:276); this library evaluates the
- * non-MDL Lambert and GGX models instead. */
+ * non-MDL Lambert and GGX models instead, with an optional albedo texture:
+ * albedo = albedo[0..2] x the texture's rgb at the hit's texture coordinates
+ * (the vertices' packed uv, shaders/pack.h:15-23, interpolated as
+ * shaders/pathtracerMdl.hlsl:261-265). */
 enum { STK_MAT_LAMBERT = 0, STK_MAT_GGX = 1 };
 typedef struct StkMaterial {
     uint32_t type;
-    float roughness; /* GGX alpha = roughness^2 */
-    float pad0;
+    float roughness;        /* GGX alpha = roughness^2 */
+    uint32_t albedoTexture; /* 0: none; k: texture k - 1 of stk_set_textures */
     float pad1;
     float albedo[4];
 } StkMaterial;
+
+/* A texture of stk_set_textures: width x height RGBA32F texels, row-major
+ * (row 0 at v = 0), from texel `offset` of the texel array.  Lookups follow
+ * the reference's tex_lookup_float4_2d (shaders/mdl_runtime.hlsl:124-207):
+ * wrap (repeat, or clamp to the edge texels' centres), the quintic
+ * "smootherstep" of the fractional texel position, then a bilinear blend. */
+enum { STK_TEX_REPEAT = 0, STK_TEX_CLAMP = 1 };
+typedef struct StkTexture {
+    uint32_t width;
+    uint32_t height;
+    uint32_t offset; /* first texel (RGBA32F, 16 B) of the texture */
+    uint32_t wrap;   /* STK_TEX_REPEAT or STK_TEX_CLAMP, both axes */
+} StkTexture;
 
 /* Pre-transformed world-space triangle, 48 B: {v0, e0, e1} exactly as
  * getTriangle (shaders/raytracing.h:134-153) computes them, plus the ids the
@@ -268,6 +284,8 @@ STK_STATIC_ASSERT(offsetof(StkPathTracerParam, dimension) == 272, "PathTracerPar
 STK_STATIC_ASSERT(offsetof(StkPathTracerParam, frameNumber) == 288, "PathTracerParam.frameNumber offset");
 STK_STATIC_ASSERT(offsetof(StkPathTracerParam, len) == 304, "PathTracerParam.len offset");
 STK_STATIC_ASSERT(sizeof(StkMaterial) == 32, "Material must be 32 B");
+STK_STATIC_ASSERT(offsetof(StkMaterial, albedoTexture) == 8, "Material.albedoTexture offset");
+STK_STATIC_ASSERT(sizeof(StkTexture) == 16, "Texture must be 16 B");
 STK_STATIC_ASSERT(sizeof(StkTriangle) == 48, "Triangle must be 48 B");
 STK_STATIC_ASSERT(sizeof(StkPackedNode) == 64, "PackedNode must be 64 B");
 STK_STATIC_ASSERT(sizeof(StkWideNode) == 80, "WideNode must be 80 B");

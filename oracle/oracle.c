@@ -1304,7 +1304,57 @@ static inline f3 interp(f3 a, f3 b, f3 c, float u, float v)
 /* Path tracer: pathTraceCameraRays (shaders/pathtracerMdl.hlsl:204-404)      */
 /* with estimateDirectLighting/sampleLights (:31-92).                         */
 /* ------------------------------------------------------------------------ */
-static const StkMaterial k_default_material = {STK_MAT_LAMBERT, 1.0f, 0.0f, 0.0f, {0.5f, 0.5f, 0.5f, 1.0f}};
+static const StkMaterial k_default_material = {STK_MAT_LAMBERT, 1.0f, 0u, 0.0f, {0.5f, 0.5f, 0.5f, 1.0f}};
+
+/* unpackUV (shaders/pack.h:15-23) */
+static inline void unpack_uv(uint32_t p, float* s, float* t)
+{
+    *t = (float)((p & 0xffff0000u) >> 16) / 16383.99999f * 20.0f - 10.0f;
+    *s = (float)(p & 0x0000ffffu) / 16383.99999f * 20.0f - 10.0f;
+}
+
+/* One axis of tex_lookup_float4_2d (shaders/mdl_runtime.hlsl:173-207): the
+ * clamp of apply_wrap_and_crop (:93-120) for a clamped texture, the quintic
+ * smoothing of apply_smootherstep_filter (:125-140), then the bilinear
+ * filter's texel coordinate x * n - 0.5. */
+static float tex_axis(float c, uint32_t n, int clampEdge)
+{
+    const float fn = (float)n;
+    if (clampEdge) {
+        const float h = 0.5f / fn;
+        c = fminf(fmaxf(c, h), 1.0f - h);
+    }
+    float r = c * fn + 0.5f;
+    const float i = floorf(r);
+    float f = r - i;
+    f = f * f * f * (f * (f * 6.0f - 15.0f) + 10.0f);
+    r = i + f;
+    return ((r - 0.5f) / fn) * fn - 0.5f;
+}
+
+static int tex_wrap(int i, int n, int clampEdge)
+{
+    if (clampEdge) return i < 0 ? 0 : (i >= n ? n - 1 : i);
+    const int m = i % n;
+    return m < 0 ? m + n : m;
+}
+
+void orc_texture_lookup(const StkTexture* tx, const float* texels, float s, float t, float out[3])
+{
+    const int ce = tx->wrap == STK_TEX_CLAMP;
+    const float x = tex_axis(s, tx->width, ce), y = tex_axis(t, tx->height, ce);
+    const float x0 = floorf(x), y0 = floorf(y);
+    const float fx = x - x0, fy = y - y0;
+    const int w = (int)tx->width, h = (int)tx->height;
+    const int ix0 = tex_wrap((int)x0, w, ce), ix1 = tex_wrap((int)x0 + 1, w, ce);
+    const int iy0 = tex_wrap((int)y0, h, ce), iy1 = tex_wrap((int)y0 + 1, h, ce);
+    const float* a = texels + 4 * ((size_t)tx->offset + (size_t)iy0 * tx->width + ix0);
+    const float* b = texels + 4 * ((size_t)tx->offset + (size_t)iy0 * tx->width + ix1);
+    const float* c = texels + 4 * ((size_t)tx->offset + (size_t)iy1 * tx->width + ix0);
+    const float* d = texels + 4 * ((size_t)tx->offset + (size_t)iy1 * tx->width + ix1);
+    const float gx = 1.0f - fx, gy = 1.0f - fy;
+    for (int k = 0; k < 3; ++k) out[k] = (a[k] * gx + b[k] * fx) * gy + (c[k] * gx + d[k] * fx) * fy;
+}
 
 void orc_pt_sample(const OrcScene* s, const StkRenderParams* rp, uint32_t px, uint32_t py, uint32_t sampleNum,
                    float out[3])
@@ -1346,9 +1396,21 @@ void orc_pt_sample(const OrcScene* s, const StkRenderParams* rp, uint32_t px, ui
         const f3 n2 = xform3(ic->normalMatrix, unpack_normal(s->vb[i2].normal));
         const f3 P = interp(p0, p1, p2, hit.bary[0], hit.bary[1]);
         const f3 N = nrm(interp(n0, n1, n2, hit.bary[0], hit.bary[1]));
-        const StkMaterial* mat = (ic->materialId >= 0 && (uint32_t)ic->materialId < s->nmats)
-                                     ? &s->mats[ic->materialId]
-                                     : &k_default_material;
+        StkMaterial matv = (ic->materialId >= 0 && (uint32_t)ic->materialId < s->nmats) ? s->mats[ic->materialId]
+                                                                                         : k_default_material;
+        if (matv.albedoTexture != 0 && matv.albedoTexture <= s->ntextures) {
+            /* texture coordinates as shaders/pathtracerMdl.hlsl:261-265 */
+            float s0, t0, s1, t1, s2, t2, tc[3];
+            unpack_uv(s->vb[i0].uv, &s0, &t0);
+            unpack_uv(s->vb[i1].uv, &s1, &t1);
+            unpack_uv(s->vb[i2].uv, &s2, &t2);
+            const float w = 1.0f - hit.bary[0] - hit.bary[1];
+            const float su = (s0 * w + s1 * hit.bary[0]) + s2 * hit.bary[1];
+            const float tv = (t0 * w + t1 * hit.bary[0]) + t2 * hit.bary[1];
+            orc_texture_lookup(&s->textures[matv.albedoTexture - 1], s->texels, su, tv, tc);
+            for (int k = 0; k < 3; ++k) matv.albedo[k] = matv.albedo[k] * tc[k];
+        }
+        const StkMaterial* mat = &matv;
         const f3 k1 = neg(ld(ray.d));
 
         /* sampleLights + estimateDirectLighting */

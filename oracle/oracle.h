@@ -53,6 +53,10 @@ typedef struct OrcScene {
     uint32_t nlights;
     const StkMaterial* mats;
     uint32_t nmats;
+    /* albedo textures (optional): texels RGBA32F, 4 floats each */
+    const StkTexture* textures;
+    uint32_t ntextures;
+    const float* texels;
     /* compressed 8-wide layout of the same tree (optional: for ORC_MODE_WIDE) */
     const StkWideNode* wide;
     uint32_t nwide;
@@ -162,6 +166,11 @@ void orc_bsdf_sample(const StkMaterial* m, const float N[3], const float Ng[3], 
 /* diffuse/glossy outputs are bsdf * dot(N, k2), pdf is the solid-angle pdf. */
 void orc_bsdf_eval(const StkMaterial* m, const float N[3], const float Ng[3], const float k1[3], const float k2[3],
                    float diffuse[3], float glossy[3], float* pdf);
+
+/* Albedo texture lookup (rgb) at texture coordinates (s, t): the reference's
+ * tex_lookup_float4_2d (shaders/mdl_runtime.hlsl:173-207) with its
+ * smootherstep filter, the bilinear blend restated in binary32. */
+void orc_texture_lookup(const StkTexture* tx, const float* texels, float s, float t, float out[3]);
 
 /* ---- path tracer (shaders/pathtracerMdl.hlsl:204-423) -------------------- */
 /* One (pixel, sample) path; seeds the RNG with frameNumber*spp + sampleNum. */

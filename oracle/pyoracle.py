@@ -26,6 +26,7 @@ class OrcScene(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("nnodes", C.c_uint32), ("inst", C.c_void_p), ("ninst", C.c_uint32),
                 ("vb", C.c_void_p), ("nv", C.c_uint32), ("ib", C.c_void_p), ("ni", C.c_uint32),
                 ("lights", C.c_void_p), ("nlights", C.c_uint32), ("mats", C.c_void_p), ("nmats", C.c_uint32),
+                ("textures", C.c_void_p), ("ntextures", C.c_uint32), ("texels", C.c_void_p),
                 ("wide", C.c_void_p), ("nwide", C.c_uint32), ("wideTris", C.c_void_p), ("nwideTris", C.c_uint32),
                 ("tlas", C.c_void_p), ("ntlas", C.c_uint32), ("tlasLeaves", C.c_void_p), ("blas", C.c_void_p),
                 ("blasTris", C.c_void_p), ("instKey", C.c_void_p), ("instW", C.c_void_p)]
@@ -76,6 +77,7 @@ def load() -> C.CDLL:
             "orc_bsdf_eval": (None, [P, FP, FP, FP, FP, FP, FP, FP]),
             "orc_pt_sample": (None, [P, P, U32, U32, U32, FP]),
             "orc_pt_frame": (None, [P, P, U32, U32, U32, U32, FP, C.c_int]),
+            "orc_texture_lookup": (None, [P, FP, C.c_float, C.c_float, FP]),
             "orc_reduce": (None, [FP, U32, U32, FP]),
             "orc_accumulate": (None, [FP, FP, U32, U32, FP]),
             "orc_tonemap": (None, [FP, U32, U32, FP]),
@@ -97,6 +99,16 @@ def vptr(a):
     return None if a is None else C.c_void_p(a.ctypes.data)
 
 
+def texture_lookup(texture: np.ndarray, texels: np.ndarray, s: float, t: float) -> np.ndarray:
+    """orc_texture_lookup: rgb of one StkTexture record (16-B structured
+    scalar) over RGBA32F texels at texture coordinates (s, t)."""
+    rec = np.ascontiguousarray(np.asarray(texture).reshape(1))
+    tx = np.ascontiguousarray(texels, dtype=np.float32)
+    out = np.zeros(3, np.float32)
+    load().orc_texture_lookup(rec.ctypes.data, fptr(tx), C.c_float(s), C.c_float(t), fptr(out))
+    return out
+
+
 def default_threads() -> int:
     n = os.cpu_count() or 1
     return max(1, min(n, 64))
@@ -105,7 +117,8 @@ def default_threads() -> int:
 class Oracle:
     """Holds the scene arrays and the skip-link BVH the oracle walks."""
 
-    def __init__(self, vb, ib, inst, nodes, lights=None, mats=None, wide=None, wide_tris=None):
+    def __init__(self, vb, ib, inst, nodes, lights=None, mats=None, wide=None, wide_tris=None, textures=None,
+                 texels=None):
         self.lib = load()
         self.vb = np.ascontiguousarray(vb)
         self.ib = np.ascontiguousarray(ib, dtype=np.uint32)
@@ -122,6 +135,12 @@ class Oracle:
             s.lights, s.nlights = self.lights.ctypes.data, self.lights.size
         if self.mats is not None and self.mats.size:
             s.mats, s.nmats = self.mats.ctypes.data, self.mats.size
+        # albedo textures (StkTexture records, RGBA32F texels), kept alive here
+        self.textures = None if textures is None else np.ascontiguousarray(textures)
+        self.texels = None if texels is None else np.ascontiguousarray(texels, dtype=np.float32)
+        if self.textures is not None and self.textures.size:
+            assert self.textures.dtype.itemsize == 16
+            s.textures, s.ntextures, s.texels = self.textures.ctypes.data, self.textures.size, self.texels.ctypes.data
         # the wide layout is an input (records of 80 B and 48 B), kept alive here
         self.wide = None if wide is None else np.ascontiguousarray(wide)
         self.wide_tris = None if wide_tris is None else np.ascontiguousarray(wide_tris)
@@ -133,7 +152,8 @@ class Oracle:
 
     @classmethod
     def from_scene(cls, scene, nodes, wide=None, wide_tris=None):
-        return cls(scene.vb, scene.ib, scene.inst, nodes, scene.lights, scene.mats, wide, wide_tris)
+        return cls(scene.vb, scene.ib, scene.inst, nodes, scene.lights, scene.mats, wide, wide_tris,
+                   getattr(scene, "textures", None), getattr(scene, "texels", None))
 
     @classmethod
     def from_two_level(cls, scene, tl):

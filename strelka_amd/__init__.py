@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _abi
 from ._abi import (BLAS_TRIANGLE, BVH_NODE, CACHE_RECORD, HIT, INSTANCE, INVALID, LIGHT, MAT_GGX, MAT_LAMBERT,
-                   MATERIAL, PACKED_NODE, RAY, TLAS_LEAF, TRIANGLE, WIDE_NODE,
+                   MATERIAL, PACKED_NODE, RAY, TEXTURE, TLAS_LEAF, TRIANGLE, WIDE_NODE,
                    VERTEX, BatchScene, BatchViewInfo, BvhHeader, BvhOptions, CacheConfig, CacheInfo, DeviceInfo, ImageDesc, LaunchConfig,
                    PathTracerParam, RenderParams, SceneArrays)
 
@@ -60,6 +60,8 @@ class SceneData:
     camera: PathTracerParam
     sky: tuple
     _handle: C.c_void_p | None = None
+    textures: np.ndarray | None = None  # TEXTURE records (albedo textures of the materials)
+    texels: np.ndarray | None = None    # RGBA32F texels, shape (n, 4)
 
     @property
     def num_triangles(self) -> int:
@@ -306,6 +308,15 @@ class Context:
                                           s.inst.size), "stk_set_geometry")
         self._check(_LIB.stk_set_lights(self._h, _ptr(s.lights), s.lights.size), "stk_set_lights")
         self._check(_LIB.stk_set_materials(self._h, _ptr(s.mats), s.mats.size), "stk_set_materials")
+        if s.textures is not None:
+            self.set_textures(s.textures, s.texels)
+
+    def set_textures(self, textures: np.ndarray, texels: np.ndarray):
+        """stk_set_textures: TEXTURE records over RGBA32F texels (n x 4)."""
+        textures = np.ascontiguousarray(textures, dtype=_abi.TEXTURE)
+        texels = np.ascontiguousarray(texels, dtype=np.float32).reshape(-1, 4)
+        self._check(_LIB.stk_set_textures(self._h, _ptr(textures), textures.size, _ptr(texels), texels.shape[0]),
+                    "stk_set_textures")
 
     def build_bvh(self, max_depth: int = 32, bins: int = 32):
         opts = BvhOptions(max_depth, bins, 1.0, 1.0)

@@ -24,7 +24,10 @@ LIGHT = np.dtype([("points", "<f4", (4, 4)), ("color", "<f4", 4), ("normal", "<f
 RAY = np.dtype([("o", "<f4", 4), ("d", "<f4", 4)])
 HIT = np.dtype([("bary", "<f4", 2), ("t", "<f4"), ("instId", "<u4"), ("primId", "<u4")])
 MAT_LAMBERT, MAT_GGX = 0, 1  # StkMaterial.type
-MATERIAL = np.dtype([("type", "<u4"), ("roughness", "<f4"), ("pad", "<f4", 2), ("albedo", "<f4", 4)])
+MATERIAL = np.dtype([("type", "<u4"), ("roughness", "<f4"), ("albedoTexture", "<u4"), ("pad", "<f4"),
+                     ("albedo", "<f4", 4)])
+TEXTURE = np.dtype([("width", "<u4"), ("height", "<u4"), ("offset", "<u4"), ("wrap", "<u4")])
+assert MATERIAL.itemsize == 32 and TEXTURE.itemsize == 16
 TRIANGLE = np.dtype([("v0", "<f4", 3), ("leaf", "<u4"), ("e0", "<f4", 3), ("instId", "<u4"), ("e1", "<f4", 3),
                      ("primId", "<u4")])
 PACKED_NODE = np.dtype([("x", "<f4", 4), ("y", "<f4", 4), ("z", "<f4", 4), ("child", "<i4", 2), ("pad", "<u4", 2)])
@@ -145,6 +148,7 @@ PROTOTYPES = [
     ("stk_stream", P, [P]),
     ("stk_set_geometry", C.c_int, [P, P, U32, P, U32, P, U32]),
     ("stk_set_lights", C.c_int, [P, P, U32]),
+    ("stk_set_textures", C.c_int, [P, P, C.c_uint32, P, C.c_uint64]),
     ("stk_set_materials", C.c_int, [P, P, U32]),
     ("stk_build_bvh", C.c_int, [P, C.POINTER(BvhOptions)]),
     ("stk_set_bvh", C.c_int, [P, P, U32]),

@@ -77,7 +77,8 @@ struct stk_ctx {
     uint32_t numNodes = 0, numWide = 0, numEmit = 0;
     bool gpuBuilt = false;
 
-    DevBuf vb, ib, inst, lights, mats, packed, tris, refNodes, wideNodes, wideTris, emitters;
+    DevBuf vb, ib, inst, lights, mats, packed, tris, refNodes, wideNodes, wideTris, emitters, textures, texels;
+    uint32_t numTextures = 0;
     DevBuf tlasWide, tlasLeaves, blasWide, blasTris, instKey, instW, tlEmit;
     DevBuf rayScratch, hitScratch;
 
@@ -129,7 +130,7 @@ struct stk_ctx {
 
     std::vector<DevBuf*> owned()
     {
-        return {&vb, &ib, &inst, &lights, &mats, &packed, &tris, &refNodes, &wideNodes, &wideTris, &emitters,
+        return {&vb, &ib, &inst, &lights, &mats, &textures, &texels, &packed, &tris, &refNodes, &wideNodes, &wideTris, &emitters,
                 &tlasWide, &tlasLeaves, &blasWide, &blasTris, &instKey, &instW, &tlEmit, &rayScratch, &hitScratch,
                 &accum, &accum2, &history, &out, &samples, &descs, &staging, &fVb, &fIb, &fInst, &fLights, &fMats, &fPacked,
                 &fTris, &fRefNodes, &fWide, &fWideTris, &fEmit, &cGridH, &cGridP, &cGridM, &cGridV, &cGridG, &cMlpP,
@@ -387,6 +388,29 @@ STK_API stk_status stk_set_materials(stk_ctx* c, const StkMaterial* m, uint32_t 
     stk_status st = devUpload(c, c->mats, m, (size_t)n * sizeof(StkMaterial));
     if (st == STK_OK) c->numMats = n;
     return st;
+}
+
+STK_API stk_status stk_set_textures(stk_ctx* c, const StkTexture* t, uint32_t n, const float* texels,
+                                    uint64_t numTexels)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    if ((n && !t) || (numTexels && !texels)) return fail(c, STK_E_INVALID, "stk_set_textures: null buffer");
+    if (numTexels > (1ull << 31)) return fail(c, STK_E_INVALID, "stk_set_textures: too many texels");
+    for (uint32_t i = 0; i < n; ++i) {
+        const StkTexture& x = t[i];
+        if (x.width == 0 || x.height == 0 || x.width > 16384 || x.height > 16384 || x.wrap > STK_TEX_CLAMP ||
+            (uint64_t)x.offset + (uint64_t)x.width * x.height > numTexels)
+            return fail(c, STK_E_INVALID, "stk_set_textures: texture " + std::to_string(i) +
+                                              " empty, too large, with an unknown wrap or outside the texels");
+    }
+    (void)hipSetDevice(c->device);
+    invalidateGraph(c);
+    c->numTextures = 0;
+    stk_status st;
+    if ((st = devUpload(c, c->textures, t, (size_t)n * sizeof(StkTexture))) != STK_OK) return st;
+    if ((st = devUpload(c, c->texels, texels, (size_t)numTexels * 4 * sizeof(float))) != STK_OK) return st;
+    c->numTextures = n;
+    return STK_OK;
 }
 
 STK_API stk_status stk_build_bvh(stk_ctx* c, const StkBvhOptions* opts)
@@ -850,6 +874,7 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
                                      (uint64_t)sv.bvh.blas, (uint64_t)sv.bvh.blasTris, (uint64_t)sv.bvh.instKey,
                                      (uint64_t)sv.bvh.instW, (uint64_t)sv.vb, (uint64_t)sv.ib,
                                      (uint64_t)sv.inst, (uint64_t)sv.lights, (uint64_t)sv.mats, sv.numMats,
+                                     (uint64_t)sv.textures, sv.numTextures, (uint64_t)sv.texels,
                                      (uint64_t)sv.scenes, sv.numScenes, sv.totalSlots, sv.tile, lp.maxDepth,
                                      lp.spp, lp.samplingType, lp.rrStartDepth, lp.refCompat, lp.passSlots,
                                      c->launch.blockSize, c->launch.tile, c->launch.persistentBlocks, c->ps.segCap,
@@ -960,6 +985,9 @@ static stk::SceneView singleSceneView(stk_ctx* c)
     sv.lights = static_cast<const StkLight*>(c->lights.ptr);
     sv.mats = static_cast<const StkMaterial*>(c->mats.ptr);
     sv.numMats = c->numMats;
+    sv.textures = static_cast<const StkTexture*>(c->textures.ptr);
+    sv.numTextures = c->numTextures;
+    sv.texels = static_cast<const float4*>(c->texels.ptr);
     return sv;
 }
 
@@ -1010,6 +1038,9 @@ STK_API stk_status stk_batch_add_scene(stk_ctx* c, const StkBatchScene* s, uint3
         (uint64_t)s->camera.dimension[0] * (uint64_t)s->camera.dimension[1] > (1ull << 26))
         return fail(c, STK_E_INVALID, "stk_batch_add_scene: bad view size");
     if (c->batch.size() >= 4096) return fail(c, STK_E_INVALID, "stk_batch_add_scene: at most 4096 scenes");
+    for (uint32_t i = 0; i < s->numMaterials; ++i)
+        if (s->mats[i].albedoTexture != 0)
+            return fail(c, STK_E_INVALID, "stk_batch_add_scene: textured materials serve single-scene renders");
     for (uint32_t i = 0; i < s->numInstances; ++i) {
         const StkInstance& ic = s->inst[i];
         if (ic.indexOffset < 0 || ic.indexCount < 0 || ic.indexCount % 3 ||
@@ -1259,6 +1290,9 @@ STK_API stk_status stk_batch_render(stk_ctx* c, const StkRenderParams* p, float*
     sv.lights = static_cast<const StkLight*>(c->fLights.ptr);
     sv.mats = static_cast<const StkMaterial*>(c->fMats.ptr);
     sv.numMats = c->fNumMats;
+    sv.textures = nullptr;
+    sv.numTextures = 0;
+    sv.texels = nullptr;
     const uint64_t key = (uint64_t)(uintptr_t)c->fPacked.ptr ^ ((uint64_t)descs.size() << 48);
     return renderForest(c, sv, descs, p, outRgba, outIsDevice, false, key);
 }

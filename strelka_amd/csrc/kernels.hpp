@@ -35,6 +35,9 @@ struct SceneView {
     const StkLight* lights;
     const StkMaterial* mats;
     uint32_t numMats;
+    const StkTexture* textures; // albedo textures (single-scene renders)
+    uint32_t numTextures;
+    const float4* texels;
     const SceneDesc* scenes;
     uint32_t numScenes;
     uint32_t totalSlots;

@@ -226,6 +226,59 @@ __device__ __forceinline__ F3 interp(F3 a, F3 b, F3 c, float u, float v)
     return (a * w + b * u) + c * v;
 }
 
+// ---- albedo textures ---------------------------------------------------------
+// unpackUV (shaders/pack.h:15-23)
+__device__ __forceinline__ void unpack_uv(uint32_t p, float& s, float& t)
+{
+    t = (float)((p & 0xffff0000u) >> 16) / 16383.99999f * 20.0f - 10.0f;
+    s = (float)(p & 0x0000ffffu) / 16383.99999f * 20.0f - 10.0f;
+}
+
+// One axis of the lookup: the clamp of a clamped texture
+// (apply_wrap_and_crop, shaders/mdl_runtime.hlsl:93-120), then the quintic
+// smoothing of the fractional texel position (apply_smootherstep_filter,
+// :125-140).  Returns the filter's texel coordinate (x * n - 0.5).
+__device__ __forceinline__ float tex_axis(float c, uint32_t n, bool clampEdge)
+{
+    const float fn = (float)n;
+    if (clampEdge) {
+        const float h = 0.5f / fn;
+        c = fminf(fmaxf(c, h), 1.0f - h);
+    }
+    float r = c * fn + 0.5f;
+    const float i = floorf(r);
+    float f = r - i;
+    f = f * f * f * (f * (f * 6.0f - 15.0f) + 10.0f);
+    r = i + f;
+    return ((r - 0.5f) / fn) * fn - 0.5f;
+}
+
+__device__ __forceinline__ int tex_wrap(int i, int n, bool clampEdge)
+{
+    if (clampEdge) return i < 0 ? 0 : (i >= n ? n - 1 : i);
+    const int m = i % n;
+    return m < 0 ? m + n : m;
+}
+
+// Bilinear RGB of texture tx at (s, t).
+__device__ __forceinline__ F3 tex_lookup(const StkTexture& tx, const float4* __restrict__ texels, float s, float t)
+{
+    const bool ce = tx.wrap == STK_TEX_CLAMP;
+    const float x = tex_axis(s, tx.width, ce), y = tex_axis(t, tx.height, ce);
+    const float x0 = floorf(x), y0 = floorf(y);
+    const float fx = x - x0, fy = y - y0;
+    const int w = (int)tx.width, h = (int)tx.height;
+    const int ix0 = tex_wrap((int)x0, w, ce), ix1 = tex_wrap((int)x0 + 1, w, ce);
+    const int iy0 = tex_wrap((int)y0, h, ce), iy1 = tex_wrap((int)y0 + 1, h, ce);
+    const float4* row0 = texels + tx.offset + (size_t)iy0 * tx.width;
+    const float4* row1 = texels + tx.offset + (size_t)iy1 * tx.width;
+    const float4 a = row0[ix0], b = row0[ix1], c = row1[ix0], d = row1[ix1];
+    const float gx = 1.0f - fx, gy = 1.0f - fy;
+    return mk3((a.x * gx + b.x * fx) * gy + (c.x * gx + d.x * fx) * fy,
+               (a.y * gx + b.y * fx) * gy + (c.y * gx + d.y * fx) * fy,
+               (a.z * gx + b.z * fx) * gy + (c.z * gx + d.z * fx) * fy);
+}
+
 // ---- BSDFs --------------------------------------------------------------------
 struct BsdfSample {
     F3 k2;

@@ -120,3 +120,26 @@ def test_ctypes_records_match_the_header(tmp_path):
         assert int(got[c_name]) == C.sizeof(cls), c_name
         for f, *_ in cls._fields_:
             assert int(got[f"{c_name}.{f}"]) == getattr(cls, f).offset, (c_name, f)
+
+
+def test_oracle_scene_record_matches_its_header(tmp_path):
+    """oracle/pyoracle.OrcScene has the size and field offsets of oracle.h's
+    OrcScene (the oracle reads the scene through it)."""
+    import subprocess
+
+    from oracle import pyoracle
+    fields = [f for f, *_ in pyoracle.OrcScene._fields_]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "oracle.h"', "int main(void) {",
+             'printf("size %zu\\n", sizeof(OrcScene));']
+    lines += [f'printf("{f} %zu\\n", offsetof(OrcScene, {f}));' for f in fields]
+    lines.append("return 0; }")
+    src = tmp_path / "orc.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "orc"
+    subprocess.run(["gcc", "-I", str(ROOT / "oracle"), "-I", str(ROOT / "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    assert int(got["size"]) == C.sizeof(pyoracle.OrcScene)
+    for f in fields:
+        assert int(got[f]) == getattr(pyoracle.OrcScene, f).offset, f
