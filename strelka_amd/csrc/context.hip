@@ -96,6 +96,12 @@ struct stk_ctx {
     std::vector<DevBuf> xPsBufs[kMaxPassStreams];
     hipStream_t xStream[kMaxPassStreams] = {};
     hipEvent_t finEv[kMaxPassStreams] = {}, joinEv[kMaxPassStreams] = {}, accumFree[2] = {};
+    // radiance cache with several pass streams: each stream's query and
+    // training-vertex buffers, and cacheEv[i], the end of the cache steps of
+    // the last pass on xStream[i] (the next pass's inference waits for it)
+    DevBuf xQueryRecs[kMaxPassStreams], xQueryOut[kMaxPassStreams], xTrainVerts[kMaxPassStreams],
+        xTrainCount[kMaxPassStreams];
+    hipEvent_t cacheEv[kMaxPassStreams] = {};
     bool accumFreeSet[2] = {false, false};
     uint32_t renderCalls = 0, passTurn = 0; // multi-stream calls so far, next pass stream
     uint32_t lastStreams = 1;    // pass streams of the last render call
@@ -135,7 +141,10 @@ struct stk_ctx {
                 &accum, &accum2, &history, &out, &samples, &descs, &staging, &fVb, &fIb, &fInst, &fLights, &fMats, &fPacked,
                 &fTris, &fRefNodes, &fWide, &fWideTris, &fEmit, &cGridH, &cGridP, &cGridM, &cGridV, &cGridG, &cMlpP,
                 &cMlpM, &cMlpV, &cMlpG, &cFwd, &cBwd, &cXT, &cDyT, &cLoss, &cGridE, &cGridEW, &cGridHE, &cMlpE,
-                &cFwdE, &cBwdE, &queryRecs, &queryOut, &trainVerts, &trainCount, &trainBatch};
+                &cFwdE, &cBwdE, &queryRecs, &queryOut, &trainVerts, &trainCount, &trainBatch,
+                &xQueryRecs[0], &xQueryRecs[1], &xQueryRecs[2], &xQueryRecs[3], &xQueryOut[0], &xQueryOut[1],
+                &xQueryOut[2], &xQueryOut[3], &xTrainVerts[0], &xTrainVerts[1], &xTrainVerts[2], &xTrainVerts[3],
+                &xTrainCount[0], &xTrainCount[1], &xTrainCount[2], &xTrainCount[3]};
     }
 };
 
@@ -316,6 +325,7 @@ STK_API void stk_destroy(stk_ctx* c)
     for (int i = 0; i < kMaxPassStreams; ++i) {
         if (c->finEv[i]) (void)hipEventDestroy(c->finEv[i]);
         if (c->joinEv[i]) (void)hipEventDestroy(c->joinEv[i]);
+        if (c->cacheEv[i]) (void)hipEventDestroy(c->cacheEv[i]);
     }
     for (auto& ev : c->accumFree)
         if (ev) (void)hipEventDestroy(ev);
@@ -764,9 +774,9 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
     size_t Q = slots;
     if (c->launch.pathsInFlight && p->cacheMode == 0)
         Q = std::min(slots, ((size_t)c->launch.pathsInFlight + 63) / 64 * 64);
-    // several pass streams: plain rendering only (the cache's training passes
-    // depend on each other)
-    const uint32_t nstreams = p->cacheMode == 0 ? std::max<uint32_t>(1u, c->launch.passStreams) : 1u;
+    // several pass streams (with the cache, a pass's bounces overlap the
+    // previous pass's inference and training, which its own wait for)
+    const uint32_t nstreams = std::max<uint32_t>(1u, c->launch.passStreams);
     const bool multi = nstreams > 1;
     if (!multi && (st = ensurePathState(c, c->ps, c->psBufs, Q)) != STK_OK) return st;
     for (uint32_t i = 0; multi && i < nstreams; ++i) {
@@ -777,6 +787,7 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
         for (int i = 0; i < kMaxPassStreams; ++i) {
             STK_HIP(c, hipEventCreateWithFlags(&c->finEv[i], hipEventDisableTiming));
             STK_HIP(c, hipEventCreateWithFlags(&c->joinEv[i], hipEventDisableTiming));
+            STK_HIP(c, hipEventCreateWithFlags(&c->cacheEv[i], hipEventDisableTiming));
         }
         for (auto& ev : c->accumFree) STK_HIP(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     }
@@ -828,17 +839,23 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
         pc.trainStride = std::max<uint32_t>(1u, (uint32_t)(slots / paths));
         pc.maxTrainPaths = (uint32_t)((slots + pc.trainStride - 1) / pc.trainStride);
         pc.trainCap = (uint32_t)std::min<size_t>(c->cache.nCap, (size_t)pc.maxTrainPaths * p->pt.maxDepth);
-        if ((st = devReserve(c, c->queryRecs, slots * sizeof(stk::CacheRecord))) != STK_OK) return st;
-        if ((st = devReserve(c, c->queryOut, slots * sizeof(float4))) != STK_OK) return st;
-        if ((st = devReserve(c, c->trainVerts, (size_t)pc.maxTrainPaths * p->pt.maxDepth * 6 * sizeof(float4))) !=
-            STK_OK)
-            return st;
-        if ((st = devReserve(c, c->trainCount, (size_t)pc.maxTrainPaths * sizeof(uint32_t))) != STK_OK) return st;
+        // per pass stream (one set on a single stream): queries, training vertices
+        for (uint32_t i = 0; i < nstreams; ++i) {
+            DevBuf& qr = multi ? c->xQueryRecs[i] : c->queryRecs;
+            DevBuf& qo = multi ? c->xQueryOut[i] : c->queryOut;
+            DevBuf& tv = multi ? c->xTrainVerts[i] : c->trainVerts;
+            DevBuf& tc = multi ? c->xTrainCount[i] : c->trainCount;
+            if ((st = devReserve(c, qr, slots * sizeof(stk::CacheRecord))) != STK_OK) return st;
+            if ((st = devReserve(c, qo, slots * sizeof(float4))) != STK_OK) return st;
+            if ((st = devReserve(c, tv, (size_t)pc.maxTrainPaths * p->pt.maxDepth * 6 * sizeof(float4))) != STK_OK)
+                return st;
+            if ((st = devReserve(c, tc, (size_t)pc.maxTrainPaths * sizeof(uint32_t))) != STK_OK) return st;
+        }
         if ((st = devReserve(c, c->trainBatch, (size_t)pc.trainCap * sizeof(stk::CacheRecord))) != STK_OK) return st;
-        pc.queryRecs = static_cast<stk::CacheRecord*>(c->queryRecs.ptr);
-        pc.queryOut = static_cast<float4*>(c->queryOut.ptr);
-        pc.trainVerts = static_cast<float4*>(c->trainVerts.ptr);
-        pc.trainCount = static_cast<uint32_t*>(c->trainCount.ptr);
+        pc.queryRecs = static_cast<stk::CacheRecord*>((multi ? c->xQueryRecs[0] : c->queryRecs).ptr);
+        pc.queryOut = static_cast<float4*>((multi ? c->xQueryOut[0] : c->queryOut).ptr);
+        pc.trainVerts = static_cast<float4*>((multi ? c->xTrainVerts[0] : c->trainVerts).ptr);
+        pc.trainCount = static_cast<uint32_t*>((multi ? c->xTrainCount[0] : c->trainCount).ptr);
         pc.trainBatch = static_cast<stk::CacheRecord*>(c->trainBatch.ptr);
     }
     float4* accum = static_cast<float4*>(accBuf.ptr);
@@ -920,11 +937,19 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
                 // sums its samples in order and no sample is overwritten late
                 const uint32_t i = (first + k) % nstreams;
                 const bool chained = k > 0 || c->lastStreams > 1;
+                stk::PtLaunch lpi = lp;
+                if (pc.mode != 0) {
+                    lpi.cache.queryRecs = static_cast<stk::CacheRecord*>(c->xQueryRecs[i].ptr);
+                    lpi.cache.queryOut = static_cast<float4*>(c->xQueryOut[i].ptr);
+                    lpi.cache.trainVerts = static_cast<float4*>(c->xTrainVerts[i].ptr);
+                    lpi.cache.trainCount = static_cast<uint32_t*>(c->xTrainCount[i].ptr);
+                }
                 issued = stk::launchSetPass(c->xPs[i], pass, c->xStream[i]);
                 if (issued == hipSuccess)
-                    issued = stk::launchPathTracePass(sv, lp, c->launch, c->xPs[i], accum, samples, nullptr,
-                                                      c->xStream[i],
-                                                      chained ? c->finEv[c->lastPassStream] : nullptr, c->finEv[i]);
+                    issued = stk::launchPathTracePass(
+                        sv, lpi, c->launch, c->xPs[i], accum, samples, c->haveCache ? &c->cache : nullptr,
+                        c->xStream[i], chained ? c->finEv[c->lastPassStream] : nullptr, c->finEv[i],
+                        chained ? c->cacheEv[c->lastPassStream] : nullptr, c->cacheEv[i]);
                 c->lastPassStream = i;
                 c->lastStreams = nstreams;
                 continue;

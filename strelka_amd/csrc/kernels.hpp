@@ -155,10 +155,13 @@ hipError_t launchClearRayTotals(PathState& ps, hipStream_t s);
 // after the bounce loop and one training step runs on the pass's records.
 // finalizeAfter (optional): the finalize step waits for this event (the
 // previous pass's finalize on another stream); finalizeDone (optional) is
-// recorded after it.
+// recorded after it.  cacheAfter (optional): the cache's inference and
+// training steps wait for it (the previous pass's training on another
+// stream); cacheDone (optional) is recorded after them.
 hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const LaunchConfig& lc, PathState& ps,
                                float4* accum, float* samples, CacheBuffers* cb, hipStream_t s,
-                               hipEvent_t finalizeAfter = nullptr, hipEvent_t finalizeDone = nullptr);
+                               hipEvent_t finalizeAfter = nullptr, hipEvent_t finalizeDone = nullptr,
+                               hipEvent_t cacheAfter = nullptr, hipEvent_t cacheDone = nullptr);
 // Plain 16-B-per-lane copy kernel (bytes must be a multiple of 16).
 hipError_t launchCopy(void* dst, const void* src, size_t bytes, hipStream_t s);
 // RGBA32F pixels to a tightly packed RGBA32F (format 0) or RGBA8 (format 1)

@@ -619,7 +619,8 @@ hipError_t launchClearRayTotals(PathState& ps, hipStream_t s)
 
 hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const LaunchConfig& lc, PathState& ps,
                                float4* accum, float* samples, CacheBuffers* cb, hipStream_t s,
-                               hipEvent_t finalizeAfter, hipEvent_t finalizeDone)
+                               hipEvent_t finalizeAfter, hipEvent_t finalizeDone, hipEvent_t cacheAfter,
+                               hipEvent_t cacheDone)
 {
     const uint32_t n = lp.passSlots;
     if (n == 0 || n > ps.capacity || ps.segCap % 256 != 0 || (size_t)ps.segCap * kStripes > ps.queueCapacity ||
@@ -645,6 +646,11 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
                                lc.binaryWalk ? 1 : 0);
     }
     hipLaunchKernelGGL(k_pt_tally, dim3(1), dim3(64), 0, s, ps);
+    if (cacheAfter && cb && (query || train)) {
+        // the network and its optimiser state as the previous pass's training left them
+        hipError_t e = hipStreamWaitEvent(s, cacheAfter, 0);
+        if (e != hipSuccess) return e;
+    }
     if (query && cb) {
         hipError_t e = cacheInfer(*cb, pc.queryRecs, n, ps.counters + 3, pc.queryOut, s);
         if (e != hipSuccess) return e;
@@ -668,6 +674,10 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
             e = cacheAdam(*cb, 1.0f, ps.counters + 4, pc.trainCap, s);
             if (e != hipSuccess) return e;
         }
+    }
+    if (cacheDone) {
+        hipError_t e = hipEventRecord(cacheDone, s);
+        if (e != hipSuccess) return e;
     }
     return hipGetLastError();
 }

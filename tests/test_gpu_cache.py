@@ -167,3 +167,48 @@ def test_trained_cache_beats_plain_path_tracing_at_equal_samples():
     print(f"RMSE vs 1024-spp reference: plain {e_plain:.4f}, cached {e_cached:.4f}, queries {cnt[3]}")
     ctx.close()
     assert e_cached < e_plain
+
+
+def test_cached_frames_on_pass_streams_equal_one_stream():
+    """With several pass streams a pass's bounces overlap the previous pass's
+    inference and training, which its own still follow.  Query-only frames
+    over a fixed network and train-only frames (their images do not read the
+    network) are bit-identical to one stream; query-and-train frames are close
+    to one stream's (training itself is not bit-reproducible: records are
+    appended and grid gradients scattered with atomics)."""
+    print("case: config 7, cached frames of 2 spp, pass streams 1 / 2 / 3", flush=True)
+    s = sa.SceneData.generate(7)
+    w, h = s.camera.dimension[0], s.camera.dimension[1]
+
+    def run(streams, mode, frames=3):
+        ctx = sa.Context(0)
+        ctx.set_scene(s)
+        ctx.build_bvh()
+        ctx.set_launch_config(block_size=64, tile=8, pass_streams=streams)
+        ctx.cache_create(log2_table=14, seed=5)
+        if mode == 2:
+            info = ctx.cache_info()
+            grid, mlp = random_params(info, 3)
+            ctx.cache_set_params(grid, mlp)
+        imgs = []
+        for f in range(frames):
+            p = sa.render_params(s.camera, spp=2, max_depth=8, frame=f, sampling=1, sky=(0.0, 0.0, 0.0),
+                                 keep_samples=True, cache_mode=mode, cache_query_depth=1, cache_train_paths=2048,
+                                 cache_train_steps=2)
+            imgs.append(ctx.render(p).copy())
+        smp = ctx.samples(w * h * 2 * 3)
+        ctx.close()
+        return imgs, smp
+
+    for mode in (2, 3):
+        base = run(1, mode)
+        for streams in (2, 3):
+            got = run(streams, mode)
+            for f in range(3):
+                assert np.array_equal(got[0][f], base[0][f]), (mode, streams, f)
+            assert np.array_equal(got[1], base[1]), (mode, streams)
+    one = run(1, 1)[0][-1][..., :3]
+    two = run(2, 1)[0][-1][..., :3]
+    rel = np.abs(two - one).mean() / one.mean()
+    print(f"query + train: mean relative difference between one and two streams {rel:.2e}")
+    assert rel < 0.02
