@@ -222,7 +222,8 @@ typedef struct StkLaunchConfig {
     /* 1..4 (0 means 1): with n > 1, pass k of a frame runs on stream k % n,
      * each stream with its own path state, so that one pass's last bounces
      * overlap the next passes' first ones; the passes still add into the
-     * image in sample order (plain rendering only; no graph replay then) */
+     * image in sample order; with the radiance cache a pass's inference and
+     * training also wait for the previous pass's (no graph replay then) */
     uint32_t passStreams;
 } StkLaunchConfig;
 STK_API stk_status stk_set_launch_config(stk_ctx* ctx, const StkLaunchConfig* lc);
