@@ -113,16 +113,17 @@ typedef struct StkPathTracerParam {
 
 /* Material record of this library, 32 B.  The reference binds MDL argument
  * blocks here (shaders/pathtracerMdl.hlsl:276); this library evaluates the
- * non-MDL Lambert and GGX models instead, with an optional albedo texture:
- * albedo = albedo[0..2] x the texture's rgb at the hit's texture coordinates
- * (the vertices' packed uv, shaders/pack.h:15-23, interpolated as
- * shaders/pathtracerMdl.hlsl:261-265). */
+ * non-MDL Lambert and GGX models instead, with optional textures looked up at
+ * the hit's texture coordinates (the vertices' packed uv, shaders/pack.h:15-23,
+ * interpolated as shaders/pathtracerMdl.hlsl:261-265): albedo = albedo[0..2] x
+ * the albedo texture's rgb, roughness = roughness x the roughness texture's
+ * red channel. */
 enum { STK_MAT_LAMBERT = 0, STK_MAT_GGX = 1 };
 typedef struct StkMaterial {
     uint32_t type;
-    float roughness;        /* GGX alpha = roughness^2 */
-    uint32_t albedoTexture; /* 0: none; k: texture k - 1 of stk_set_textures */
-    float pad1;
+    float roughness;           /* GGX alpha = roughness^2 */
+    uint32_t albedoTexture;    /* 0: none; k: texture k - 1 of stk_set_textures */
+    uint32_t roughnessTexture; /* 0: none; k: roughness x texture k - 1's red channel */
     float albedo[4];
 } StkMaterial;
 
@@ -285,6 +286,7 @@ STK_STATIC_ASSERT(offsetof(StkPathTracerParam, frameNumber) == 288, "PathTracerP
 STK_STATIC_ASSERT(offsetof(StkPathTracerParam, len) == 304, "PathTracerParam.len offset");
 STK_STATIC_ASSERT(sizeof(StkMaterial) == 32, "Material must be 32 B");
 STK_STATIC_ASSERT(offsetof(StkMaterial, albedoTexture) == 8, "Material.albedoTexture offset");
+STK_STATIC_ASSERT(offsetof(StkMaterial, roughnessTexture) == 12, "Material.roughnessTexture offset");
 STK_STATIC_ASSERT(sizeof(StkTexture) == 16, "Texture must be 16 B");
 STK_STATIC_ASSERT(sizeof(StkTriangle) == 48, "Triangle must be 48 B");
 STK_STATIC_ASSERT(sizeof(StkPackedNode) == 64, "PackedNode must be 64 B");

@@ -1304,7 +1304,7 @@ static inline f3 interp(f3 a, f3 b, f3 c, float u, float v)
 /* Path tracer: pathTraceCameraRays (shaders/pathtracerMdl.hlsl:204-404)      */
 /* with estimateDirectLighting/sampleLights (:31-92).                         */
 /* ------------------------------------------------------------------------ */
-static const StkMaterial k_default_material = {STK_MAT_LAMBERT, 1.0f, 0u, 0.0f, {0.5f, 0.5f, 0.5f, 1.0f}};
+static const StkMaterial k_default_material = {STK_MAT_LAMBERT, 1.0f, 0u, 0u, {0.5f, 0.5f, 0.5f, 1.0f}};
 
 /* unpackUV (shaders/pack.h:15-23) */
 static inline void unpack_uv(uint32_t p, float* s, float* t)
@@ -1398,7 +1398,9 @@ void orc_pt_sample(const OrcScene* s, const StkRenderParams* rp, uint32_t px, ui
         const f3 N = nrm(interp(n0, n1, n2, hit.bary[0], hit.bary[1]));
         StkMaterial matv = (ic->materialId >= 0 && (uint32_t)ic->materialId < s->nmats) ? s->mats[ic->materialId]
                                                                                          : k_default_material;
-        if (matv.albedoTexture != 0 && matv.albedoTexture <= s->ntextures) {
+        const int albTex = matv.albedoTexture != 0 && matv.albedoTexture <= s->ntextures;
+        const int roughTex = matv.roughnessTexture != 0 && matv.roughnessTexture <= s->ntextures;
+        if (albTex || roughTex) {
             /* texture coordinates as shaders/pathtracerMdl.hlsl:261-265 */
             float s0, t0, s1, t1, s2, t2, tc[3];
             unpack_uv(s->vb[i0].uv, &s0, &t0);
@@ -1407,8 +1409,14 @@ void orc_pt_sample(const OrcScene* s, const StkRenderParams* rp, uint32_t px, ui
             const float w = 1.0f - hit.bary[0] - hit.bary[1];
             const float su = (s0 * w + s1 * hit.bary[0]) + s2 * hit.bary[1];
             const float tv = (t0 * w + t1 * hit.bary[0]) + t2 * hit.bary[1];
-            orc_texture_lookup(&s->textures[matv.albedoTexture - 1], s->texels, su, tv, tc);
-            for (int k = 0; k < 3; ++k) matv.albedo[k] = matv.albedo[k] * tc[k];
+            if (albTex) {
+                orc_texture_lookup(&s->textures[matv.albedoTexture - 1], s->texels, su, tv, tc);
+                for (int k = 0; k < 3; ++k) matv.albedo[k] = matv.albedo[k] * tc[k];
+            }
+            if (roughTex) {
+                orc_texture_lookup(&s->textures[matv.roughnessTexture - 1], s->texels, su, tv, tc);
+                matv.roughness = matv.roughness * tc[0];
+            }
         }
         const StkMaterial* mat = &matv;
         const f3 k1 = neg(ld(ray.d));

@@ -24,7 +24,7 @@ LIGHT = np.dtype([("points", "<f4", (4, 4)), ("color", "<f4", 4), ("normal", "<f
 RAY = np.dtype([("o", "<f4", 4), ("d", "<f4", 4)])
 HIT = np.dtype([("bary", "<f4", 2), ("t", "<f4"), ("instId", "<u4"), ("primId", "<u4")])
 MAT_LAMBERT, MAT_GGX = 0, 1  # StkMaterial.type
-MATERIAL = np.dtype([("type", "<u4"), ("roughness", "<f4"), ("albedoTexture", "<u4"), ("pad", "<f4"),
+MATERIAL = np.dtype([("type", "<u4"), ("roughness", "<f4"), ("albedoTexture", "<u4"), ("roughnessTexture", "<u4"),
                      ("albedo", "<f4", 4)])
 TEXTURE = np.dtype([("width", "<u4"), ("height", "<u4"), ("offset", "<u4"), ("wrap", "<u4")])
 assert MATERIAL.itemsize == 32 and TEXTURE.itemsize == 16

@@ -1064,7 +1064,7 @@ STK_API stk_status stk_batch_add_scene(stk_ctx* c, const StkBatchScene* s, uint3
         return fail(c, STK_E_INVALID, "stk_batch_add_scene: bad view size");
     if (c->batch.size() >= 4096) return fail(c, STK_E_INVALID, "stk_batch_add_scene: at most 4096 scenes");
     for (uint32_t i = 0; i < s->numMaterials; ++i)
-        if (s->mats[i].albedoTexture != 0)
+        if (s->mats[i].albedoTexture != 0 || s->mats[i].roughnessTexture != 0)
             return fail(c, STK_E_INVALID, "stk_batch_add_scene: textured materials serve single-scene renders");
     for (uint32_t i = 0; i < s->numInstances; ++i) {
         const StkInstance& ic = s->inst[i];

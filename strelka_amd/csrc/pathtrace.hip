@@ -305,11 +305,13 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
                 } else {
                     mat.type = STK_MAT_LAMBERT;
                     mat.roughness = 1.0f;
-                    mat.albedoTexture = 0;
+                    mat.albedoTexture = mat.roughnessTexture = 0;
                     mat.albedo[0] = mat.albedo[1] = mat.albedo[2] = 0.5f;
                     mat.albedo[3] = 1.0f;
                 }
-                if (mat.albedoTexture != 0 && mat.albedoTexture <= sc.numTextures) {
+                const bool albTex = mat.albedoTexture != 0 && mat.albedoTexture <= sc.numTextures;
+                const bool roughTex = mat.roughnessTexture != 0 && mat.roughnessTexture <= sc.numTextures;
+                if (albTex || roughTex) {
                     // texture coordinates as shaders/pathtracerMdl.hlsl:261-265
                     float s0, t0, s1, t1, s2, t2;
                     unpack_uv(v0.uv, s0, t0);
@@ -317,10 +319,14 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
                     unpack_uv(v2.uv, s2, t2);
                     const float w = 1.0f - h.x - h.y;
                     const float s = (s0 * w + s1 * h.x) + s2 * h.y, t = (t0 * w + t1 * h.x) + t2 * h.y;
-                    const F3 tc = tex_lookup(sc.textures[mat.albedoTexture - 1], sc.texels, s, t);
-                    mat.albedo[0] = mat.albedo[0] * tc.x;
-                    mat.albedo[1] = mat.albedo[1] * tc.y;
-                    mat.albedo[2] = mat.albedo[2] * tc.z;
+                    if (albTex) {
+                        const F3 tc = tex_lookup(sc.textures[mat.albedoTexture - 1], sc.texels, s, t);
+                        mat.albedo[0] = mat.albedo[0] * tc.x;
+                        mat.albedo[1] = mat.albedo[1] * tc.y;
+                        mat.albedo[2] = mat.albedo[2] * tc.z;
+                    }
+                    if (roughTex)
+                        mat.roughness = mat.roughness * tex_lookup(sc.textures[mat.roughnessTexture - 1], sc.texels, s, t).x;
                 }
                 const F3 k1 = -f3(ps.rayD[slot]);
                 const float rough = mat.type == STK_MAT_GGX ? mat.roughness : 1.0f;

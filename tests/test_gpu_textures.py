@@ -1,4 +1,5 @@
-"""Albedo textures on the GPU (stk_set_textures + StkMaterial.albedoTexture):
+"""Material textures on the GPU (stk_set_textures + StkMaterial.albedoTexture
+and roughnessTexture):
 per-sample radiance against the oracle path tracer with the same textures, a
 white 1x1 texture leaving every sample bit-identical to the untextured
 render, and the error paths."""
@@ -13,11 +14,12 @@ from test_gpu_pathtrace import MAX_DIVERGENT, compare_samples
 pytestmark = pytest.mark.gpu
 
 
-def textured(textures, texels, pick):
+def textured(textures, texels, pick, rough=lambda m: 0):
     s = sa.SceneData.generate(6)
     mats = s.mats.copy()
     for m in range(mats.size):
         mats["albedoTexture"][m] = pick(m)
+        mats["roughnessTexture"][m] = rough(m)
     s.mats, s.textures, s.texels = mats, textures, texels
     return s
 
@@ -57,7 +59,8 @@ def render_samples(s):
 def test_textured_samples_match_oracle():
     print("case: config 6 with a checker (repeat) and a gradient (clamp) texture, 2 spp, crop vs oracle", flush=True)
     textures, texels = checker_and_gradient()
-    s = textured(textures, texels, lambda m: 1 + m % 2)
+    s = textured(textures, texels, lambda m: 1 + m % 2, rough=lambda m: 2 - m % 2)
+    assert np.any(s.mats["type"] == sa.MAT_GGX)
     smp, nodes = render_samples(s)
     o = Oracle.from_scene(s, nodes)
     h, w = smp.shape[:2]
@@ -76,7 +79,7 @@ def test_white_texture_changes_nothing():
     textures = np.zeros(1, sa.TEXTURE)
     textures[0] = (1, 1, 0, 0)
     white = np.ones((1, 4), np.float32)
-    s = textured(textures, white, lambda m: 1)
+    s = textured(textures, white, lambda m: 1, rough=lambda m: 1)
     smp, _ = render_samples(s)
     plain, _ = render_samples(sa.SceneData.generate(6))
     assert np.array_equal(smp, plain)
