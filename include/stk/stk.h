@@ -288,7 +288,8 @@ STK_API stk_status stk_batch_render(stk_ctx* ctx, const StkRenderParams* p, floa
  * outgoing radiance (multi-resolution hash grid of 16 levels x 2 FP16
  * features feeding a 64-wide MLP with 4 ReLU hidden layers on MFMA, FP16
  * weights/activations, FP32 accumulation, Adam).  The plain path tracer
- * (cacheMode 0) stays available as the fallback and the judge. */
+ * (cacheMode 0) stays available as the fallback and as the reference the
+ * cache's bias is measured against. */
 typedef struct StkCacheConfig {
     uint32_t log2TableSize;  /* hash-table entries per level: 2^19 */
     uint32_t baseResolution; /* coarsest grid resolution: 16 */
