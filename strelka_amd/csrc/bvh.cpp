@@ -480,11 +480,92 @@ uint8_t chooseExponent(float origin, float extent)
     return (uint8_t)std::min(254, std::max(1, e));
 }
 
+// Which binary nodes become wide nodes: the collapse that minimises the summed
+// surface area of the wide nodes (the walk's expected node tests; with one
+// triangle per leaf slot the triangle tests do not depend on the collapse), by
+// dynamic programming over the binary tree (Ylitie, Karras and Laine 2017,
+// "Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs").
+// F(c, a): least cost of the subtree of c in at most a slots of its parent -
+// 0 for a leaf, SA(c) + G(c) for c kept as one wide node, or c opened and its
+// two children given a1 + a2 <= a slots; G(c): least cost inside a wide node
+// at c (its children in at most 8 slots).
+class WideCollapsePlan {
+public:
+    explicit WideCollapsePlan(const std::vector<StkBvhNode>& nodes) : nodes_(nodes)
+    {
+        const size_t n = nodes.size();
+        F_.assign(n * 8, 0.0);
+        openSplit_.assign(n * 8, 0);
+        innerSplit_.assign(n, 0);
+        // children have larger preorder indices than their parent
+        for (size_t i = n; i-- > 0;) {
+            if (isLeaf((uint32_t)i)) continue;
+            const uint32_t L = (uint32_t)i + 1, R = (uint32_t)nodes[i + 1].nodeOffset;
+            double g = kInf;
+            for (int a1 = 1; a1 <= 7; ++a1) {
+                const double c = F(L, a1) + F(R, 8 - a1);
+                if (c < g) {
+                    g = c;
+                    innerSplit_[i] = (uint8_t)a1;
+                }
+            }
+            const double keep = area((uint32_t)i) + g;
+            for (int a = 1; a <= 8; ++a) {
+                double best = keep;
+                uint8_t split = 0; // 0: kept as one wide node
+                for (int a1 = 1; a1 < a; ++a1) {
+                    const double c = F(L, a1) + F(R, a - a1);
+                    if (c < best) {
+                        best = c;
+                        split = (uint8_t)a1;
+                    }
+                }
+                F_[i * 8 + (a - 1)] = best;
+                openSplit_[i * 8 + (a - 1)] = split;
+            }
+        }
+    }
+    // The slots of a wide node at inner node i: binary nodes, in preorder.
+    void slotsOf(uint32_t i, std::vector<uint32_t>& out) const
+    {
+        out.clear();
+        const uint32_t L = i + 1, R = (uint32_t)nodes_[i + 1].nodeOffset;
+        expand(L, innerSplit_[i], out);
+        expand(R, 8 - innerSplit_[i], out);
+    }
+
+private:
+    static constexpr double kInf = 1e300;
+    bool isLeaf(uint32_t i) const { return (uint32_t)nodes_[i].instId != STK_INVALID_INDEX; }
+    double F(uint32_t i, int a) const { return isLeaf(i) ? 0.0 : F_[(size_t)i * 8 + (a - 1)]; }
+    double area(uint32_t i) const
+    {
+        const StkBvhNode& nd = nodes_[i];
+        const double dx = (double)nd.maxBounds[0] - nd.minBounds[0], dy = (double)nd.maxBounds[1] - nd.minBounds[1],
+                     dz = (double)nd.maxBounds[2] - nd.minBounds[2];
+        const double s = dx * dy + dy * dz + dz * dx;
+        return std::isfinite(s) ? s : kInf / 1e10; // non-finite boxes are rejected later; keep the sums finite
+    }
+    void expand(uint32_t c, int a, std::vector<uint32_t>& out) const
+    {
+        if (isLeaf(c) || a <= 1 || openSplit_[(size_t)c * 8 + (a - 1)] == 0) {
+            out.push_back(c);
+            return;
+        }
+        const int a1 = openSplit_[(size_t)c * 8 + (a - 1)];
+        expand(c + 1, a1, out);
+        expand((uint32_t)nodes_[c + 1].nodeOffset, a - a1, out);
+    }
+    const std::vector<StkBvhNode>& nodes_;
+    std::vector<double> F_;
+    std::vector<uint8_t> openSplit_, innerSplit_;
+};
+
 // Collapses a binary skip-link tree (preorder, hit link i + 1, miss link
 // nodeOffset) into compressed 8-wide nodes.  leafBox(i, lo, hi) gives the box
 // of leaf node i and emitLeaf(i) appends its leaf record; leaves are emitted
 // in wide-node slot order, node by node in breadth-first order, and triBase
-// counts them from 0.
+// counts them from 0.  Each wide node's children follow WideCollapsePlan.
 template <class LeafBox, class EmitLeaf>
 bool collapseWide(const std::vector<StkBvhNode>& nodes, LeafBox leafBox, EmitLeaf emitLeaf,
                   std::vector<StkWideNode>& wide, uint32_t& wideDepth, std::string& err)
@@ -492,6 +573,8 @@ bool collapseWide(const std::vector<StkBvhNode>& nodes, LeafBox leafBox, EmitLea
     wide.clear();
     wideDepth = 0;
     if (nodes.empty()) return true;
+    const WideCollapsePlan plan(nodes);
+    std::vector<uint32_t> slotNodes;
     auto childOf = [&](uint32_t idx) {
         WideChild c;
         c.node = idx;
@@ -506,10 +589,6 @@ bool collapseWide(const std::vector<StkBvhNode>& nodes, LeafBox leafBox, EmitLea
         }
         leafBox(idx, c.lo, c.hi);
         return c;
-    };
-    auto area = [](const WideChild& c) {
-        const float dx = c.hi[0] - c.lo[0], dy = c.hi[1] - c.lo[1], dz = c.hi[2] - c.lo[2];
-        return dx * dy + dy * dz + dz * dx;
     };
     struct Job {
         uint32_t node; // skip-link inner node (or leaf for a single-leaf tree)
@@ -527,22 +606,8 @@ bool collapseWide(const std::vector<StkBvhNode>& nodes, LeafBox leafBox, EmitLea
         if ((uint32_t)nodes[job.node].instId != STK_INVALID_INDEX) {
             kids.push_back(childOf(job.node)); // the whole tree is one leaf
         } else {
-            kids.push_back(childOf(job.node + 1));
-            kids.push_back(childOf((uint32_t)nodes[job.node + 1].nodeOffset));
-        }
-        // open the inner child of largest surface area until 8 children
-        while (kids.size() < 8) {
-            int best = -1;
-            float bestArea = -1.0f;
-            for (size_t k = 0; k < kids.size(); ++k)
-                if (!kids[k].leaf && area(kids[k]) > bestArea) {
-                    bestArea = area(kids[k]);
-                    best = (int)k;
-                }
-            if (best < 0) break;
-            const uint32_t idx = kids[best].node;
-            kids[best] = childOf(idx + 1);
-            kids.push_back(childOf((uint32_t)nodes[idx + 1].nodeOffset));
+            plan.slotsOf(job.node, slotNodes);
+            for (uint32_t c : slotNodes) kids.push_back(childOf(c));
         }
         // node box and slot assignment by the child's centre relative to the node's
         float lo[3], hi[3];
