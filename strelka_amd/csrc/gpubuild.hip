@@ -23,8 +23,10 @@
 //   6. preorder indices top-down, one kernel per level: left child p + 1,
 //      right child p + 2 x leaves(left); skip link = p + nodes of the subtree;
 //   7. the skip-link, packed and triangle arrays;
-//   8. the 8-wide collapse one wide level at a time: each job opens its
-//      largest-area inner descendants up to 8 children, assigns octant slots,
+//   8. the 8-wide collapse: a bottom-up dynamic program picks which binary
+//      nodes become wide nodes (least summed wide-node area), then one wide
+//      level at a time each job opens its descendants as planned, assigns
+//      octant slots,
 //      quantises outwards (checked against the float decode), then an
 //      exclusive scan of child / leaf counts places the next level.
 #include "gpubuild.hpp"
@@ -549,12 +551,6 @@ __device__ void child_box(uint32_t ref, const float4* nodeLo, const float4* node
     }
 }
 
-__device__ float child_area(const WideChildRef& c)
-{
-    const float dx = c.hi[0] - c.lo[0], dy = c.hi[1] - c.lo[1], dz = c.hi[2] - c.lo[2];
-    return dx * dy + dy * dz + dz * dx;
-}
-
 // Smallest exponent byte e with 2^(e-127) * 253 >= extent, at least 2^-100
 // and no finer than a quarter ulp of the origin's magnitude.
 __device__ uint32_t choose_exponent(float origin, float extent)
@@ -569,9 +565,88 @@ __device__ uint32_t choose_exponent(float origin, float extent)
     return (uint32_t)min(254, max(1, e));
 }
 
+// Parent links from child links (the clustered build writes only the latter).
+__global__ __launch_bounds__(kB) void k_parents(uint32_t ninner, const uint32_t* __restrict__ childL,
+                                                const uint32_t* __restrict__ childR,
+                                                uint32_t* __restrict__ parentInner, uint32_t* __restrict__ parentLeaf)
+{
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= ninner) return;
+    const uint32_t kids[2] = {childL[i], childR[i]};
+    for (int c = 0; c < 2; ++c) {
+        if (kids[c] & kLeafBit)
+            parentLeaf[kids[c] & ~kLeafBit] = i;
+        else
+            parentInner[kids[c]] = i;
+    }
+    if (i == 0) parentInner[0] = kNone;
+}
+
+// Which binary nodes become wide nodes: the collapse of least summed wide-node
+// surface area, by dynamic programming bottom-up (Ylitie, Karras and Laine
+// 2017, "Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide
+// BVHs"), the plan the host collapse follows, in single precision.  For inner
+// node p and a = 1..8 slots of its parent: F(p, a) = least cost of p's subtree
+// in at most a slots (a leaf costs 0), where p either stays one wide node
+// (area(p) + the best split of 8 slots between its children) or is opened
+// (a1 + a2 = a slots to its children).  openSplit(p, a) = a1 of the opened
+// choice, 0 when kept; innerSplit(p) = the children's split inside a wide node
+// at p.  The second child to arrive finishes its parent, as in k_refit.
+__global__ __launch_bounds__(kB) void k_wide_cost(uint32_t n, const uint32_t* __restrict__ childL,
+                                                  const uint32_t* __restrict__ childR,
+                                                  const uint32_t* __restrict__ parentInner,
+                                                  const uint32_t* __restrict__ parentLeaf,
+                                                  const float4* __restrict__ nodeLo, const float4* __restrict__ nodeHi,
+                                                  float* F, uint8_t* __restrict__ openSplit,
+                                                  uint8_t* __restrict__ innerSplit, uint32_t* flags)
+{
+    const uint32_t k = blockIdx.x * kB + threadIdx.x;
+    if (k >= n) return;
+    uint32_t p = parentLeaf[k];
+    while (p != kNone) {
+        __threadfence();
+        const uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) return;
+        __threadfence();
+        const uint32_t L = childL[p], R = childR[p];
+        float fl[8], fr[8];
+        for (int a = 0; a < 8; ++a) {
+            fl[a] = (L & kLeafBit) ? 0.0f : F[(size_t)L * 8 + a];
+            fr[a] = (R & kLeafBit) ? 0.0f : F[(size_t)R * 8 + a];
+        }
+        float g = fl[0] + fr[6];
+        int gs = 1; // a1 in 1..7 whatever the sums (non-finite ones compare false)
+        for (int a1 = 2; a1 <= 7; ++a1) {
+            const float c = fl[a1 - 1] + fr[7 - a1];
+            if (c < g) {
+                g = c;
+                gs = a1;
+            }
+        }
+        innerSplit[p] = (uint8_t)gs;
+        const float keep = half_area(nodeLo[p], nodeHi[p]) + g;
+        for (int a = 1; a <= 8; ++a) {
+            float best = keep;
+            int split = 0;
+            for (int a1 = 1; a1 < a; ++a1) {
+                const float c = fl[a1 - 1] + fr[a - a1 - 1];
+                if (c < best) {
+                    best = c;
+                    split = a1;
+                }
+            }
+            F[(size_t)p * 8 + (a - 1)] = best;
+            openSplit[(size_t)p * 8 + (a - 1)] = (uint8_t)split;
+        }
+        p = parentInner[p];
+    }
+}
+
 __global__ __launch_bounds__(64) void k_wide_plan(const uint32_t* __restrict__ jobs, uint32_t nj,
                                                   const uint32_t* __restrict__ childL,
                                                   const uint32_t* __restrict__ childR,
+                                                  const uint8_t* __restrict__ openSplit,
+                                                  const uint8_t* __restrict__ innerSplit,
                                                   const float4* __restrict__ nodeLo, const float4* __restrict__ nodeHi,
                                                   const StkTriangle* __restrict__ tris, StkWideNode* __restrict__ plan,
                                                   uint32_t* __restrict__ kidRefs, uint32_t* __restrict__ innerCnt,
@@ -585,22 +660,30 @@ __global__ __launch_bounds__(64) void k_wide_plan(const uint32_t* __restrict__ j
     if (job & kLeafBit) {
         child_box(job, nodeLo, nodeHi, tris, kids[nk++]); // the whole tree is one leaf
     } else {
-        child_box(childL[job], nodeLo, nodeHi, tris, kids[nk++]);
-        child_box(childR[job], nodeLo, nodeHi, tris, kids[nk++]);
-    }
-    // open the inner child of largest surface area until 8 children
-    while (nk < 8) {
-        int best = -1;
-        float bestArea = -1.0f;
-        for (int k = 0; k < nk; ++k)
-            if (!(kids[k].ref & kLeafBit) && child_area(kids[k]) > bestArea) {
-                bestArea = child_area(kids[k]);
-                best = k;
+        // the children in preorder as the plan opens them: every stacked or
+        // taken entry holds at least one of the 8 slots, so neither exceeds 8
+        uint32_t stRef[8];
+        int stSlots[8];
+        int sp = 0;
+        const int is = min(7, max(1, (int)innerSplit[job]));
+        stRef[sp] = childR[job];
+        stSlots[sp++] = 8 - is;
+        stRef[sp] = childL[job];
+        stSlots[sp++] = is;
+        while (sp > 0) {
+            --sp;
+            const uint32_t r = stRef[sp];
+            const int a = stSlots[sp];
+            const int split = ((r & kLeafBit) || a <= 1) ? 0 : openSplit[(size_t)r * 8 + (a - 1)];
+            if (split <= 0 || split >= a) {
+                child_box(r, nodeLo, nodeHi, tris, kids[nk++]);
+                continue;
             }
-        if (best < 0) break;
-        const uint32_t idx = kids[best].ref;
-        child_box(childL[idx], nodeLo, nodeHi, tris, kids[best]);
-        child_box(childR[idx], nodeLo, nodeHi, tris, kids[nk++]);
+            stRef[sp] = childR[r];
+            stSlots[sp++] = a - split;
+            stRef[sp] = childL[r];
+            stSlots[sp++] = split;
+        }
     }
     float lo[3], hi[3];
     for (int a = 0; a < 3; ++a) {
@@ -888,8 +971,24 @@ hipError_t gpuBuildBvh(const StkVertex* vb, const uint32_t* ib, const StkInstanc
     hipLaunchKernelGGL(k_emit_leaves, dim3(blocksFor(n)), dim3(kB), 0, s, n, vals, preLeaf, trisIn, inst, tris, ref,
                        emit, counters + 1);
 
-    // the 8-wide collapse, one level at a time
+    // the 8-wide collapse, one level at a time, following the plan of least
+    // summed wide-node area
     const size_t wideCap = std::max<size_t>(ninner, 1);
+    STK_GB_ALLOC(float, planCost, wideCap * 8, temp(wideCap * 32))
+    STK_GB_ALLOC(uint8_t, openSplit, wideCap * 8, temp(wideCap * 8))
+    STK_GB_ALLOC(uint8_t, innerSplit, wideCap, temp(wideCap))
+    if (n > 1) {
+        if (method == kGpuBuildPloc) {
+            hipLaunchKernelGGL(k_set_u32, dim3(std::min<uint32_t>(blocksFor(n), 1024)), dim3(kB), 0, s, parentLeaf,
+                               kNone, (size_t)n);
+            hipLaunchKernelGGL(k_parents, dim3(blocksFor(ninner)), dim3(kB), 0, s, ninner, childL, childR,
+                               parentInner, parentLeaf);
+        }
+        hipLaunchKernelGGL(k_set_u32, dim3(std::min<uint32_t>(blocksFor(ninner), 1024)), dim3(kB), 0, s, flags, 0u,
+                           (size_t)ninner);
+        hipLaunchKernelGGL(k_wide_cost, dim3(blocksFor(n)), dim3(kB), 0, s, n, childL, childR, parentInner,
+                           parentLeaf, nodeLo, nodeHi, planCost, openSplit, innerSplit, flags);
+    }
     STK_GB_ALLOC(StkWideNode, wide, wideCap, alloc(kGbWide, wideCap * sizeof(StkWideNode)))
     STK_GB_ALLOC(StkTriangle, wideTris, n, alloc(kGbWideTris, (size_t)n * sizeof(StkTriangle)))
     STK_GB_ALLOC(uint32_t, jobsA, wideCap, temp(wideCap * 4))
@@ -911,8 +1010,8 @@ hipError_t gpuBuildBvh(const StkVertex* vb, const uint32_t* ib, const StkInstanc
     uint32_t* jc = jobsA;
     uint32_t* jn = jobsB;
     while (nj > 0) {
-        hipLaunchKernelGGL(k_wide_plan, dim3((nj + 63) / 64), dim3(64), 0, s, jc, nj, childL, childR, nodeLo, nodeHi,
-                           tris, plan, kidRefs, innerCnt, leafCnt, counters + 2);
+        hipLaunchKernelGGL(k_wide_plan, dim3((nj + 63) / 64), dim3(64), 0, s, jc, nj, childL, childR, openSplit,
+                           innerSplit, nodeLo, nodeHi, tris, plan, kidRefs, innerCnt, leafCnt, counters + 2);
         if ((e = rocprim::exclusive_scan(scanTemp, scanBytes, innerCnt, innerOff, 0u, nj, rocprim::plus<uint32_t>(),
                                          s)) != hipSuccess)
             return e;
