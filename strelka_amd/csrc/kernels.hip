@@ -18,6 +18,9 @@ namespace {
 using namespace dev;
 
 constexpr int kBlock = 256;
+// One wave per block for the wide closest-hit walk: its LDS stacks bound the
+// occupancy whatever the block size, and single-wave blocks retire one by one.
+constexpr int kWideBlock = 64;
 
 __device__ __forceinline__ void store_hit(StkHit* hits, uint32_t i, const HitF& h)
 {
@@ -40,15 +43,16 @@ static_assert(2 * kWideStack <= kStackDepth, "the wide walk's two stacks share t
 // Closest hit over the compressed 8-wide tree (two LDS node-group stacks per
 // lane).  The binary walk has a kernel of its own (56 registers instead of
 // the wide walk's 67).
-__global__ __launch_bounds__(kBlock) void k_trace_closest(BvhView bvh, const StkRay* __restrict__ rays, uint32_t n,
-                                                          StkHit* __restrict__ hits)
+__global__ __launch_bounds__(kWideBlock) void k_trace_closest(BvhView bvh, const StkRay* __restrict__ rays,
+                                                              uint32_t n, StkHit* __restrict__ hits)
 {
-    __shared__ uint32_t stack[2 * kWideStack * kBlock];
-    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    __shared__ uint32_t stack[2 * kWideStack * kWideBlock];
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kWideBlock + threadIdx.x;
     if (i >= n) return;
     const RayF r = load_ray(rays, i);
     HitF h;
-    closest_wide(bvh, bvh.wideRoot, r, stack + threadIdx.x, stack + kWideStack * kBlock + threadIdx.x, kBlock, h);
+    closest_wide(bvh, bvh.wideRoot, r, stack + threadIdx.x, stack + kWideStack * kWideBlock + threadIdx.x, kWideBlock,
+                 h);
     store_hit(hits, i, h);
 }
 
@@ -252,7 +256,8 @@ hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n
     } else if ((flags & STK_TRACE_BVH2) || bvh.wideRoot == STK_INVALID_INDEX) {
         hipLaunchKernelGGL(k_trace_closest_packed, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
     } else {
-        hipLaunchKernelGGL(k_trace_closest, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
+        hipLaunchKernelGGL(k_trace_closest, dim3((uint32_t)(((size_t)n + kWideBlock - 1) / kWideBlock)), dim3(kWideBlock), 0,
+                           s, bvh, rays, n, hits);
     }
     return hipGetLastError();
 }
