@@ -18,9 +18,9 @@ namespace {
 using namespace dev;
 
 constexpr int kBlock = 256;
-// One wave per block for the wide closest-hit walk: its LDS stacks bound the
+// One wave per block for the walks with LDS stacks: the stacks bound the
 // occupancy whatever the block size, and single-wave blocks retire one by one.
-constexpr int kWideBlock = 64;
+constexpr int kWalkBlock = 64;
 
 __device__ __forceinline__ void store_hit(StkHit* hits, uint32_t i, const HitF& h)
 {
@@ -43,56 +43,56 @@ static_assert(2 * kWideStack <= kStackDepth, "the wide walk's two stacks share t
 // Closest hit over the compressed 8-wide tree (two LDS node-group stacks per
 // lane).  The binary walk has a kernel of its own (56 registers instead of
 // the wide walk's 67).
-__global__ __launch_bounds__(kWideBlock) void k_trace_closest(BvhView bvh, const StkRay* __restrict__ rays,
+__global__ __launch_bounds__(kWalkBlock) void k_trace_closest(BvhView bvh, const StkRay* __restrict__ rays,
                                                               uint32_t n, StkHit* __restrict__ hits)
 {
-    __shared__ uint32_t stack[2 * kWideStack * kWideBlock];
-    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kWideBlock + threadIdx.x;
+    __shared__ uint32_t stack[2 * kWideStack * kWalkBlock];
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kWalkBlock + threadIdx.x;
     if (i >= n) return;
     const RayF r = load_ray(rays, i);
     HitF h;
-    closest_wide(bvh, bvh.wideRoot, r, stack + threadIdx.x, stack + kWideStack * kWideBlock + threadIdx.x, kWideBlock,
+    closest_wide(bvh, bvh.wideRoot, r, stack + threadIdx.x, stack + kWideStack * kWalkBlock + threadIdx.x, kWalkBlock,
                  h);
     store_hit(hits, i, h);
 }
 
 // Closest hit over the packed binary tree.
-__global__ __launch_bounds__(kBlock) void k_trace_closest_packed(BvhView bvh, const StkRay* __restrict__ rays,
+__global__ __launch_bounds__(kWalkBlock) void k_trace_closest_packed(BvhView bvh, const StkRay* __restrict__ rays,
                                                                  uint32_t n, StkHit* __restrict__ hits)
 {
-    __shared__ uint32_t stack[kStackDepth * kBlock];
-    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    __shared__ uint32_t stack[kStackDepth * kWalkBlock];
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kWalkBlock + threadIdx.x;
     if (i >= n) return;
     const RayF r = load_ray(rays, i);
     HitF h;
-    closest_packed(bvh, bvh.hdr, r, stack + threadIdx.x, kBlock, h);
+    closest_packed(bvh, bvh.hdr, r, stack + threadIdx.x, kWalkBlock, h);
     store_hit(hits, i, h);
 }
 
 // Closest hit and any hit over a two-level structure.
 // five waves per SIMD: the two-level walk fits 96 registers without spills
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_trace_closest_tl(BvhView bvh, const StkRay* __restrict__ rays, uint32_t n,
+__global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_trace_closest_tl(BvhView bvh, const StkRay* __restrict__ rays, uint32_t n,
                                                              StkHit* __restrict__ hits)
 {
-    __shared__ uint32_t stack[2 * kWideStack * kBlock];
-    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    __shared__ uint32_t stack[2 * kWideStack * kWalkBlock];
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kWalkBlock + threadIdx.x;
     if (i >= n) return;
     const RayF r = load_ray(rays, i);
     HitF h;
     uint32_t ik;
-    closest_two_level(bvh, r, stack + threadIdx.x, stack + kWideStack * kBlock + threadIdx.x, kBlock, h, ik);
+    closest_two_level(bvh, r, stack + threadIdx.x, stack + kWideStack * kWalkBlock + threadIdx.x, kWalkBlock, h, ik);
     store_hit(hits, i, h);
 }
 
-__global__ __launch_bounds__(kBlock) void k_trace_any_tl(BvhView bvh, const StkRay* __restrict__ rays, uint32_t n,
+__global__ __launch_bounds__(kWalkBlock) void k_trace_any_tl(BvhView bvh, const StkRay* __restrict__ rays, uint32_t n,
                                                          StkHit* __restrict__ hits)
 {
-    __shared__ uint32_t stack[2 * kWideStack * kBlock];
-    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    __shared__ uint32_t stack[2 * kWideStack * kWalkBlock];
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kWalkBlock + threadIdx.x;
     if (i >= n) return;
     const RayF r = load_ray(rays, i);
     HitF h;
-    any_two_level(bvh, r, stack + threadIdx.x, stack + kWideStack * kBlock + threadIdx.x, kBlock, h);
+    any_two_level(bvh, r, stack + threadIdx.x, stack + kWideStack * kWalkBlock + threadIdx.x, kWalkBlock, h);
     store_hit(hits, i, h);
 }
 
@@ -107,15 +107,15 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest_ref(BvhView bvh, const
     store_hit(hits, i, h);
 }
 
-__global__ __launch_bounds__(kBlock) void k_trace_any(BvhView bvh, const StkRay* __restrict__ rays, uint32_t n,
+__global__ __launch_bounds__(kWalkBlock) void k_trace_any(BvhView bvh, const StkRay* __restrict__ rays, uint32_t n,
                                                       StkHit* __restrict__ hits)
 {
-    __shared__ uint32_t stack[kStackDepth * kBlock];
-    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    __shared__ uint32_t stack[kStackDepth * kWalkBlock];
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kWalkBlock + threadIdx.x;
     if (i >= n) return;
     const RayF r = load_ray(rays, i);
     HitF h;
-    any_packed(bvh, bvh.hdr, r, stack + threadIdx.x, kBlock, h);
+    any_packed(bvh, bvh.hdr, r, stack + threadIdx.x, kWalkBlock, h);
     h.u = 0.0f;
     h.v = 0.0f;
     store_hit(hits, i, h);
@@ -232,6 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_pack_image(const float4* __restrict_
 }
 
 inline uint32_t blocks(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+inline uint32_t walkBlocks(size_t n) { return (uint32_t)((n + kWalkBlock - 1) / kWalkBlock); }
 
 } // namespace
 
@@ -248,16 +249,15 @@ hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n
     if (n == 0) return hipSuccess;
     if (bvh.tlas) {
         if (flags & (STK_TRACE_REFERENCE_ORDER | STK_TRACE_BVH2)) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_trace_closest_tl, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
+        hipLaunchKernelGGL(k_trace_closest_tl, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh, rays, n, hits);
     } else if (!bvh.packedOk && (flags & STK_TRACE_BVH2)) {
         return hipErrorInvalidValue; // deeper than the packed walk's stack
     } else if ((flags & STK_TRACE_REFERENCE_ORDER) || (!bvh.packedOk && bvh.wideRoot == STK_INVALID_INDEX)) {
         hipLaunchKernelGGL(k_trace_closest_ref, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
     } else if ((flags & STK_TRACE_BVH2) || bvh.wideRoot == STK_INVALID_INDEX) {
-        hipLaunchKernelGGL(k_trace_closest_packed, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
+        hipLaunchKernelGGL(k_trace_closest_packed, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh, rays, n, hits);
     } else {
-        hipLaunchKernelGGL(k_trace_closest, dim3((uint32_t)(((size_t)n + kWideBlock - 1) / kWideBlock)), dim3(kWideBlock), 0,
-                           s, bvh, rays, n, hits);
+        hipLaunchKernelGGL(k_trace_closest, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh, rays, n, hits);
     }
     return hipGetLastError();
 }
@@ -268,11 +268,11 @@ hipError_t launchTraceAny(const BvhView& bvh, const StkRay* rays, uint32_t n, St
     if (n == 0) return hipSuccess;
     if (bvh.tlas) {
         if (flags & STK_TRACE_REFERENCE_ORDER) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_trace_any_tl, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
+        hipLaunchKernelGGL(k_trace_any_tl, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh, rays, n, hits);
     } else if ((flags & STK_TRACE_REFERENCE_ORDER) || !bvh.packedOk)
         hipLaunchKernelGGL(k_trace_any_ref, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
     else
-        hipLaunchKernelGGL(k_trace_any, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
+        hipLaunchKernelGGL(k_trace_any, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh, rays, n, hits);
     return hipGetLastError();
 }
 
