@@ -229,6 +229,7 @@ stk::BvhView bvhView(stk_ctx* c)
     v.wideRoot = ctxWideUsable(c) ? 0u : STK_INVALID_INDEX;
     v.emitCount = c->numEmit;
     v.packedOk = c->bvh.depth <= (uint32_t)stk::dev::kStackDepth ? 1u : 0u;
+    v.wideDepth = c->bvh.wideDepth;
     v.tlas = v.tlasLeaves = v.blas = v.blasTris = v.instW = nullptr;
     v.instKey = nullptr;
     if (c->twoLevel) {
@@ -1308,6 +1309,7 @@ STK_API stk_status stk_batch_render(stk_ctx* c, const StkRenderParams* p, float*
     sv.bvh.wideTris = static_cast<const float4*>(c->fWideTris.ptr);
     sv.bvh.emitters = static_cast<const float4*>(c->fEmit.ptr);
     sv.bvh.wideRoot = STK_INVALID_INDEX; // per scene: SceneDesc.wideRoot
+    sv.bvh.wideDepth = (uint32_t)stk::dev::kWideStack;
     sv.bvh.packedOk = 1;                  // host-built trees fit the packed walk's stack
     sv.vb = static_cast<const StkVertex*>(c->fVb.ptr);
     sv.ib = static_cast<const uint32_t*>(c->fIb.ptr);

@@ -21,6 +21,10 @@ constexpr int kBlock = 256;
 // One wave per block for the walks with LDS stacks: the stacks bound the
 // occupancy whatever the block size, and single-wave blocks retire one by one.
 constexpr int kWalkBlock = 64;
+// Stack entries of the wide closest-hit walk over trees of at most this many
+// levels: 5.5 KiB of LDS per wave instead of 8, so the 67-register walk runs 7
+// waves per SIMD instead of 5.
+constexpr int kWideStackShallow = 11;
 
 __device__ __forceinline__ void store_hit(StkHit* hits, uint32_t i, const HitF& h)
 {
@@ -43,16 +47,19 @@ static_assert(2 * kWideStack <= kStackDepth, "the wide walk's two stacks share t
 // Closest hit over the compressed 8-wide tree (two LDS node-group stacks per
 // lane).  The binary walk has a kernel of its own (56 registers instead of
 // the wide walk's 67).
+// S: stack entries per lane; a walk holds at most one entry per level below
+// the root, so a tree of at most S levels fits.  Fewer entries mean less LDS
+// per wave and more waves per CU.
+template <int S>
 __global__ __launch_bounds__(kWalkBlock) void k_trace_closest(BvhView bvh, const StkRay* __restrict__ rays,
                                                               uint32_t n, StkHit* __restrict__ hits)
 {
-    __shared__ uint32_t stack[2 * kWideStack * kWalkBlock];
+    __shared__ uint32_t stack[2 * S * kWalkBlock];
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kWalkBlock + threadIdx.x;
     if (i >= n) return;
     const RayF r = load_ray(rays, i);
     HitF h;
-    closest_wide(bvh, bvh.wideRoot, r, stack + threadIdx.x, stack + kWideStack * kWalkBlock + threadIdx.x, kWalkBlock,
-                 h);
+    closest_wide(bvh, bvh.wideRoot, r, stack + threadIdx.x, stack + S * kWalkBlock + threadIdx.x, kWalkBlock, h);
     store_hit(hits, i, h);
 }
 
@@ -257,7 +264,12 @@ hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n
     } else if ((flags & STK_TRACE_BVH2) || bvh.wideRoot == STK_INVALID_INDEX) {
         hipLaunchKernelGGL(k_trace_closest_packed, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh, rays, n, hits);
     } else {
-        hipLaunchKernelGGL(k_trace_closest, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh, rays, n, hits);
+        if (bvh.wideDepth <= (uint32_t)kWideStackShallow)
+            hipLaunchKernelGGL(k_trace_closest<kWideStackShallow>, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh,
+                               rays, n, hits);
+        else
+            hipLaunchKernelGGL(k_trace_closest<kWideStack>, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh, rays, n,
+                               hits);
     }
     return hipGetLastError();
 }

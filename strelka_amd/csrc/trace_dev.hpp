@@ -39,6 +39,7 @@ struct BvhView {
     uint32_t wideRoot;            // STK_INVALID_INDEX: no usable wide tree
     uint32_t emitCount;
     uint32_t packedOk;            // 0: deeper than the packed walks' stack (kStackDepth)
+    uint32_t wideDepth;           // levels of the wide tree (a walk holds at most one stack entry per level)
     // two-level structure (tlas == nullptr: none); emitters then holds the
     // emitter instances' world triangles (leaf = mesh-tree leaf)
     const float4* tlas;           // 5 x float4 per StkWideNode, root at 0
