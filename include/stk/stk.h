@@ -144,6 +144,23 @@ STK_API stk_status stk_set_bvh(stk_ctx* ctx, const StkBvhNode* nodes, uint32_t n
  * nodes may be NULL to query the count). */
 STK_API stk_status stk_get_bvh(stk_ctx* ctx, StkBvhNode* nodes, uint32_t capacity, uint32_t* numNodes);
 
+/* What the context traces: the flat tree's header and depths, or a two-level
+ * structure.  No counterpart in the reference (its BVH lives in a Vulkan
+ * buffer, src/render/ptrender.cpp:516-532); for tests and tools that check
+ * which walk a tree takes. */
+typedef struct StkBvhInfo {
+    StkBvhHeader header;
+    uint32_t depth;       /* binary tree depth (root 0) */
+    uint32_t wideDepth;   /* levels of the 8-wide tree (root = level 1) */
+    uint32_t numNodes;    /* skip-link nodes */
+    uint32_t numWide;     /* 8-wide nodes */
+    uint32_t numEmitters; /* emitter triangles tested first by shadow rays */
+    uint32_t twoLevel;    /* 1: instance tree over mesh trees */
+    uint32_t gpuBuilt;    /* 1: built by stk_build_bvh_gpu* */
+    uint32_t walkStack;   /* stack entries of the closest-hit kernel it takes (0: no wide walk) */
+} StkBvhInfo;
+STK_API stk_status stk_get_bvh_info(stk_ctx* ctx, StkBvhInfo* out);
+
 /* ---- traversal ------------------------------------------------------------ */
 /* Host buffers; synchronous. */
 STK_API stk_status stk_trace_closest(stk_ctx* ctx, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags);

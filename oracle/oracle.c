@@ -9,6 +9,7 @@
  * (see oracle/build.py).
  */
 #include "oracle.h"
+#include "../strelka_amd/csrc/fmath.h"
 
 #include <math.h>
 #include <pthread.h>
@@ -324,6 +325,30 @@ int orc_closest_hit_brute(const OrcScene* s, const StkRay* r, StkHit* hit)
     for (uint32_t i = 0; i < s->nnodes; ++i)
         if (is_leaf(&s->nodes[i])) leaf_test(s, r, i, &found, &best, &bestLeaf, hit);
     return found;
+}
+
+/* The any-hit rule by brute force: the first leaf of the preorder array whose
+ * triangle is hit with t < tmax.  The walk below visits leaves in preorder and
+ * only skips subtrees whose box it misses or enters beyond tmax; a triangle
+ * hit at t < tmax lies in every ancestor box, so both give the same leaf
+ * except where a slab test rejects a box that holds a valid hit. */
+int orc_any_hit_brute(const OrcScene* s, const StkRay* r, StkHit* hit)
+{
+    miss_hit(hit);
+    for (uint32_t i = 0; i < s->nnodes; ++i) {
+        const StkBvhNode* n = &s->nodes[i];
+        if (!is_leaf(n)) continue;
+        const uint32_t prim = as_uint(n->minBounds[0]);
+        float v0[3], e0[3], e1[3], t, b[2];
+        orc_get_triangle(s, (uint32_t)n->instId, prim, v0, e0, e1);
+        if (orc_ray_triangle(r->o, r->d, v0, e0, e1, &t, b) && t < r->o[3]) {
+            hit->t = t;
+            hit->instId = (uint32_t)n->instId;
+            hit->primId = prim;
+            return 1;
+        }
+    }
+    return 0;
 }
 
 /* anyHit (shaders/raytracing.h:249-286): first leaf in DFS order whose
@@ -853,6 +878,7 @@ static void* batch_worker(void* p)
             case ORC_MODE_WIDE: orc_closest_hit_wide(a->s, &a->rays[i], &a->hits[i], &a->stats); break;
             case ORC_MODE_TWO_LEVEL: orc_closest_two_level(a->s, &a->rays[i], &a->hits[i], &a->stats); break;
             case ORC_MODE_ANY_TWO_LEVEL: orc_any_two_level(a->s, &a->rays[i], &a->hits[i], &a->stats); break;
+            case ORC_MODE_ANY_BRUTE: orc_any_hit_brute(a->s, &a->rays[i], &a->hits[i]); break;
             default: orc_any_hit(a->s, &a->rays[i], &a->hits[i], &a->stats); break;
             }
         }
@@ -1017,6 +1043,11 @@ static void pixel_jitter(const StkPathTracerParam* p, uint32_t s, uint32_t* rng,
 }
 
 /* generateCameraRay (shaders/pathtracerMdl.hlsl:165-202). */
+void orc_pixel_jitter(const StkPathTracerParam* p, uint32_t s, uint32_t* rng, float* jx, float* jy)
+{
+    pixel_jitter(p, s, rng, jx, jy);
+}
+
 void orc_camera_ray(const StkPathTracerParam* p, uint32_t px, uint32_t py, uint32_t* rng, uint32_t s, StkRay* ray)
 {
     float jx, jy;
@@ -1092,8 +1123,8 @@ static SphQuad sphquad_init(const StkLight* l, f3 o)
     const f3 v10 = mk(q.x1, q.y0, q.z0), v11 = mk(q.x1, q.y1, q.z0);
     const f3 n0 = nrm(cross(v00, v10)), n1 = nrm(cross(v10, v11));
     const f3 n2 = nrm(cross(v11, v01)), n3 = nrm(cross(v01, v00));
-    const float g0 = acosf(-dot(n0, n1)), g1 = acosf(-dot(n1, n2));
-    const float g2 = acosf(-dot(n2, n3)), g3 = acosf(-dot(n3, n0));
+    const float g0 = stk_acosf(-dot(n0, n1)), g1 = stk_acosf(-dot(n1, n2));
+    const float g2 = stk_acosf(-dot(n2, n3)), g3 = stk_acosf(-dot(n3, n0));
     q.b0 = n0.z;
     q.b1 = n2.z;
     q.b0sq = q.b0 * q.b0;
@@ -1105,7 +1136,7 @@ static SphQuad sphquad_init(const StkLight* l, f3 o)
 static f3 sphquad_sample(const SphQuad* q, float u, float v)
 {
     const float au = u * q->S + q->k;
-    const float fu = (cosf(au) * q->b0 - q->b1) / sinf(au);
+    const float fu = (stk_cosf(au) * q->b0 - q->b1) / stk_sinf(au);
     float cu = 1.0f / sqrtf(fu * fu + q->b0sq) * (fu > 0 ? 1.0f : -1.0f);
     cu = clampf(cu, -1.0f, 1.0f);
     float xu = -(cu * q->z0) / sqrtf(1.0f - cu * cu);
@@ -1168,7 +1199,7 @@ static inline f3 sample_cosine(float u, float v)
     const float cosTheta = sqrtf(saturate(u));
     const float sinTheta = sqrtf(saturate(1.0f - cosTheta * cosTheta));
     const float phi = 2.0f * ORC_PI * v;
-    return mk(cosf(phi) * sinTheta, sinf(phi) * sinTheta, cosTheta + 1e-6f);
+    return mk(stk_cosf(phi) * sinTheta, stk_sinf(phi) * sinTheta, cosTheta + 1e-6f);
 }
 
 static inline f3 sample_ggx(float u, float v, float alpha)
@@ -1176,7 +1207,7 @@ static inline f3 sample_ggx(float u, float v, float alpha)
     const float cosTheta = sqrtf((1.0f - u) / (u * (alpha * alpha - 1.0f) + 1.0f));
     const float sinTheta = sqrtf(saturate(1.0f - cosTheta * cosTheta));
     const float phi = 2.0f * ORC_PI * v;
-    return mk(cosf(phi) * sinTheta, sinf(phi) * sinTheta, cosTheta + 1e-6f);
+    return mk(stk_cosf(phi) * sinTheta, stk_sinf(phi) * sinTheta, cosTheta + 1e-6f);
 }
 
 static inline float ggx_d(float a2, f3 n, f3 m)
@@ -1200,7 +1231,7 @@ static inline float ggx_g1(float a2, f3 w, f3 n, f3 m)
 
 static inline f3 schlick(f3 f0, float c)
 {
-    const float w = powf(1.0f - c, 5.0f);
+    const float w = stk_pow5f(1.0f - c);
     return mk(f0.x + (1.0f - f0.x) * w, f0.y + (1.0f - f0.y) * w, f0.z + (1.0f - f0.z) * w);
 }
 
@@ -1627,5 +1658,22 @@ void orc_tonemap(const float* in, uint32_t npix, uint32_t type, float* out)
         out[i * 4 + 1] = r.y;
         out[i * 4 + 2] = r.z;
         out[i * 4 + 3] = 1.0f;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* The shared binary32 transcendentals (strelka_amd/csrc/fmath.h), exposed so */
+/* the tests can pin them against binary64 libm.                              */
+/* op: 0 sin, 1 cos, 2 acos, 3 pow5                                           */
+/* ------------------------------------------------------------------------ */
+void orc_fmath(uint32_t op, const float* in, uint32_t n, float* out)
+{
+    for (uint32_t i = 0; i < n; ++i) {
+        switch (op) {
+        case 0: out[i] = stk_sinf(in[i]); break;
+        case 1: out[i] = stk_cosf(in[i]); break;
+        case 2: out[i] = stk_acosf(in[i]); break;
+        default: out[i] = stk_pow5f(in[i]); break;
+        }
     }
 }

@@ -100,6 +100,8 @@ int orc_closest_hit_pruned(const OrcScene* s, const StkRay* r, StkHit* hit, OrcS
 int orc_closest_hit_brute(const OrcScene* s, const StkRay* r, StkHit* hit);
 /* anyHit (shaders/raytracing.h:249-286). Returns 1 on hit, fills inst/prim/t. */
 int orc_any_hit(const OrcScene* s, const StkRay* r, StkHit* hit, OrcStats* st);
+/* The first leaf in preorder (array order) whose triangle is hit with t < tmax. */
+int orc_any_hit_brute(const OrcScene* s, const StkRay* r, StkHit* hit);
 
 /* The GPU kernel's walk over the compressed 8-wide nodes (StkWideNode):
  * child boxes decoded as origin + q * 2^(e-127) (one rounding), the slab test
@@ -133,7 +135,7 @@ int orc_closest_two_level(const OrcScene* s, const StkRay* r, StkHit* hit, OrcSt
 int orc_any_two_level(const OrcScene* s, const StkRay* r, StkHit* hit, OrcStats* st);
 
 enum { ORC_MODE_REF = 0, ORC_MODE_PRUNED = 1, ORC_MODE_BRUTE = 2, ORC_MODE_ANY = 3, ORC_MODE_WIDE = 4,
-       ORC_MODE_TWO_LEVEL = 5, ORC_MODE_ANY_TWO_LEVEL = 6 };
+       ORC_MODE_TWO_LEVEL = 5, ORC_MODE_ANY_TWO_LEVEL = 6, ORC_MODE_ANY_BRUTE = 7 };
 /* Batched traversal over n rays using nthreads host threads (chunks of 4096
  * rays handed out dynamically).  stats may be NULL. */
 void orc_trace_batch(const OrcScene* s, const StkRay* rays, uint32_t n, StkHit* hits, int mode, int nthreads,
@@ -149,6 +151,9 @@ float orc_randfloat(uint32_t i, uint32_t p);
 
 /* ---- camera / lights / BSDF ---------------------------------------------- */
 void orc_camera_ray(const StkPathTracerParam* p, uint32_t px, uint32_t py, uint32_t* rng, uint32_t s, StkRay* ray);
+/* The sub-pixel position of sample s of stratifiedSamplingType (0 centre,
+ * 1 random, 2 stratifiedSampling1, 3 stratifiedSamplingOptimized). */
+void orc_pixel_jitter(const StkPathTracerParam* p, uint32_t s, uint32_t* rng, float* jx, float* jy);
 void orc_offset_ray(const float p[3], const float n[3], float out[3]);
 /* Fills point, L, distToLight, normal, area, pdf of SampleRectLight
  * (shaders/lights.h:284-301) and the solid angle S of the SphQuad. */
@@ -187,6 +192,9 @@ void orc_pt_frame(const OrcScene* s, const StkRenderParams* rp, uint32_t x0, uin
 void orc_reduce(const float* samples, uint32_t npix, uint32_t spp, float* rgba);
 void orc_accumulate(const float* cur, const float* hist, uint32_t npix, uint32_t iteration, float* out);
 void orc_tonemap(const float* in, uint32_t npix, uint32_t type, float* out);
+
+/* The transcendentals both sides share (op 0 sin, 1 cos, 2 acos, 3 x^5). */
+void orc_fmath(uint32_t op, const float* in, uint32_t n, float* out);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,7 @@ RAY = np.dtype([("o", "<f4", 4), ("d", "<f4", 4)])
 HIT = np.dtype([("bary", "<f4", 2), ("t", "<f4"), ("instId", "<u4"), ("primId", "<u4")])
 
 MODE_REF, MODE_PRUNED, MODE_BRUTE, MODE_ANY, MODE_WIDE = 0, 1, 2, 3, 4
-MODE_TWO_LEVEL, MODE_ANY_TWO_LEVEL = 5, 6
+MODE_TWO_LEVEL, MODE_ANY_TWO_LEVEL, MODE_ANY_BRUTE = 5, 6, 7
 
 
 class OrcScene(C.Structure):
@@ -71,6 +71,7 @@ def load() -> C.CDLL:
             "orc_permute": (U32, [U32, U32, U32]),
             "orc_randfloat": (F, [U32, U32]),
             "orc_camera_ray": (None, [P, U32, U32, C.POINTER(U32), U32, P]),
+            "orc_pixel_jitter": (None, [P, U32, C.POINTER(U32), FP, FP]),
             "orc_offset_ray": (None, [FP, FP, FP]),
             "orc_sample_rect_light": (None, [P, FP, FP, FP, FP, FP, FP, FP, FP, FP]),
             "orc_bsdf_sample": (None, [P, FP, FP, FP, FP, P]),
@@ -81,6 +82,7 @@ def load() -> C.CDLL:
             "orc_reduce": (None, [FP, U32, U32, FP]),
             "orc_accumulate": (None, [FP, FP, U32, U32, FP]),
             "orc_tonemap": (None, [FP, U32, U32, FP]),
+            "orc_fmath": (None, [U32, FP, U32, FP]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -106,6 +108,14 @@ def texture_lookup(texture: np.ndarray, texels: np.ndarray, s: float, t: float) 
     tx = np.ascontiguousarray(texels, dtype=np.float32)
     out = np.zeros(3, np.float32)
     load().orc_texture_lookup(rec.ctypes.data, fptr(tx), C.c_float(s), C.c_float(t), fptr(out))
+    return out
+
+
+def fmath(op: int, x: np.ndarray) -> np.ndarray:
+    """The shared binary32 transcendentals: op 0 sin, 1 cos, 2 acos, 3 x^5."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros_like(x)
+    load().orc_fmath(op, fptr(x), x.size, fptr(out))
     return out
 
 

@@ -17,7 +17,7 @@ import numpy as np
 from . import _abi
 from ._abi import (BLAS_TRIANGLE, BVH_NODE, CACHE_RECORD, HIT, INSTANCE, INVALID, LIGHT, MAT_GGX, MAT_LAMBERT,
                    MATERIAL, PACKED_NODE, RAY, TEXTURE, TLAS_LEAF, TRIANGLE, WIDE_NODE,
-                   VERTEX, BatchScene, BatchViewInfo, BvhHeader, BvhOptions, CacheConfig, CacheInfo, DeviceInfo, ImageDesc, LaunchConfig,
+                   VERTEX, BatchScene, BatchViewInfo, BvhHeader, BvhInfo, BvhOptions, CacheConfig, CacheInfo, DeviceInfo, ImageDesc, LaunchConfig,
                    PathTracerParam, RenderParams, SceneArrays)
 
 __all__ = ["lib", "StkError", "SceneData", "HostBvh", "HostTwoLevel", "Context", "RenderParams", "PathTracerParam", "VERTEX",
@@ -343,6 +343,13 @@ class Context:
         out = np.zeros(n.value, dtype=BVH_NODE)
         self._check(_LIB.stk_get_bvh(self._h, _ptr(out), out.size, C.byref(n)), "stk_get_bvh")
         return out
+
+    def bvh_info(self) -> BvhInfo:
+        """stk_get_bvh_info: depths and sizes of what the context traces, and
+        the stack entries of the closest-hit kernel its tree takes."""
+        info = BvhInfo()
+        self._check(_LIB.stk_get_bvh_info(self._h, C.byref(info)), "stk_get_bvh_info")
+        return info
 
     # -- traversal (host buffers) ----------------------------------------------
     def trace_closest(self, rays: np.ndarray, reference_order: bool = False, binary: bool = False) -> np.ndarray:

@@ -91,6 +91,12 @@ class BvhHeader(C.Structure):
     _fields_ = [("lo", C.c_float * 3), ("root", C.c_int32), ("hi", C.c_float * 3), ("numTriangles", C.c_uint32)]
 
 
+class BvhInfo(C.Structure):
+    _fields_ = [("header", BvhHeader), ("depth", C.c_uint32), ("wideDepth", C.c_uint32), ("numNodes", C.c_uint32),
+                ("numWide", C.c_uint32), ("numEmitters", C.c_uint32), ("twoLevel", C.c_uint32),
+                ("gpuBuilt", C.c_uint32), ("walkStack", C.c_uint32)]
+
+
 class LaunchConfig(C.Structure):
     _fields_ = [("blockSize", C.c_uint32), ("tile", C.c_uint32), ("persistentBlocks", C.c_uint32),
                 ("useGraph", C.c_uint32), ("pathsInFlight", C.c_uint32), ("binaryWalk", C.c_uint32),
@@ -153,6 +159,7 @@ PROTOTYPES = [
     ("stk_build_bvh", C.c_int, [P, C.POINTER(BvhOptions)]),
     ("stk_set_bvh", C.c_int, [P, P, U32]),
     ("stk_get_bvh", C.c_int, [P, P, U32, C.POINTER(U32)]),
+    ("stk_get_bvh_info", C.c_int, [P, P]),
     ("stk_trace_closest", C.c_int, [P, P, U32, P, U32]),
     ("stk_trace_any", C.c_int, [P, P, U32, P, U32]),
     ("stk_trace_closest_device", C.c_int, [P, P, U32, P, U32]),

@@ -246,9 +246,18 @@ stk::BvhView bvhView(stk_ctx* c)
     return v;
 }
 
-stk_status uploadBvh(stk_ctx* c)
+// Uploads a validated host tree and makes it the context's flat tree.  The
+// context's tree is replaced only once every device array holds the new one;
+// a failed upload leaves the context without a tree (the device arrays are
+// then a mix of old and new), never with a host description of one tree over
+// the device arrays of another.
+stk_status uploadBvh(stk_ctx* c, stk::Bvh&& b)
 {
-    stk::Bvh& b = c->bvh;
+    c->haveBvh = false;
+    c->twoLevel = false;
+    c->numWide = 0;
+    c->numNodes = 0;
+    c->numEmit = 0;
     // device copy of the skip-link array: leaves carry their triangle index in maxBounds[0]
     std::vector<StkBvhNode> dev(b.nodes);
     for (size_t i = 0; i < dev.size(); ++i)
@@ -262,9 +271,10 @@ stk_status uploadBvh(stk_ctx* c)
         return st;
     if ((st = devUpload(c, c->emitters, b.emitters.data(), b.emitters.size() * sizeof(StkTriangle))) != STK_OK)
         return st;
-    c->numNodes = (uint32_t)b.nodes.size();
-    c->numWide = (uint32_t)b.wide.size();
-    c->numEmit = (uint32_t)b.emitters.size();
+    c->bvh = std::move(b);
+    c->numNodes = (uint32_t)c->bvh.nodes.size();
+    c->numWide = (uint32_t)c->bvh.wide.size();
+    c->numEmit = (uint32_t)c->bvh.emitters.size();
     c->gpuBuilt = false;
     c->haveBvh = true;
     return STK_OK;
@@ -431,23 +441,24 @@ STK_API stk_status stk_build_bvh(stk_ctx* c, const StkBvhOptions* opts)
     std::vector<stk::BvhInputTri> tris;
     std::string err;
     const StkBvhOptions o = opts ? *opts : stk::defaultBvhOptions();
+    // built aside: a failed build leaves the context's tree as it was
+    stk::Bvh b;
     try {
         if (!stk::flattenInstances(c->hvb.data(), (uint32_t)c->hvb.size(), c->hib.data(), (uint32_t)c->hib.size(),
                                    c->hinst.data(), (uint32_t)c->hinst.size(), tris, err) ||
-            !stk::buildBvh(tris, o, c->bvh, err) ||
+            !stk::buildBvh(tris, o, b, err) ||
             !stk::packBvh(c->hvb.data(), (uint32_t)c->hvb.size(), c->hib.data(), (uint32_t)c->hib.size(),
-                          c->hinst.data(), (uint32_t)c->hinst.size(), c->bvh, err) ||
-            !stk::buildWide(c->hinst.data(), (uint32_t)c->hinst.size(), c->bvh, err))
+                          c->hinst.data(), (uint32_t)c->hinst.size(), b, err) ||
+            !stk::buildWide(c->hinst.data(), (uint32_t)c->hinst.size(), b, err))
             return fail(c, STK_E_INVALID, "stk_build_bvh: " + err);
     } catch (const std::bad_alloc&) {
         return fail(c, STK_E_OOM, "stk_build_bvh: host allocation failed");
     }
-    if (c->bvh.depth > (uint32_t)stk::dev::kStackDepth)
+    if (b.depth > (uint32_t)stk::dev::kStackDepth)
         return fail(c, STK_E_INVALID, "stk_build_bvh: tree deeper than the traversal stack (32)");
     (void)hipSetDevice(c->device);
-    c->twoLevel = false;
     invalidateGraph(c);
-    return uploadBvh(c);
+    return uploadBvh(c, std::move(b));
 }
 
 STK_API stk_status stk_build_two_level(stk_ctx* c, const StkBvhOptions* opts)
@@ -456,18 +467,23 @@ STK_API stk_status stk_build_two_level(stk_ctx* c, const StkBvhOptions* opts)
     if (!c->haveGeometry) return fail(c, STK_E_STATE, "stk_build_two_level: no geometry set");
     std::string err;
     const StkBvhOptions o = opts ? *opts : stk::defaultBvhOptions();
+    // built aside: a failed build leaves the context's structure as it was
+    stk::TwoLevel t;
     try {
         if (!stk::buildTwoLevel(c->hvb.data(), (uint32_t)c->hvb.size(), c->hib.data(), (uint32_t)c->hib.size(),
-                                c->hinst.data(), (uint32_t)c->hinst.size(), o, c->tl, err))
+                                c->hinst.data(), (uint32_t)c->hinst.size(), o, t, err))
             return fail(c, STK_E_INVALID, "stk_build_two_level: " + err);
     } catch (const std::bad_alloc&) {
         return fail(c, STK_E_OOM, "stk_build_two_level: host allocation failed");
     }
-    const stk::TwoLevel& t = c->tl;
     if (t.tlasWide.empty()) return fail(c, STK_E_INVALID, "stk_build_two_level: no instance has triangles");
     if (t.tlasDepth + t.blasDepth > (uint32_t)stk::dev::kWideStack)
         return fail(c, STK_E_INVALID, "stk_build_two_level: instance and mesh trees deeper than the walk's stack (16)");
     (void)hipSetDevice(c->device);
+    // from here on the device arrays change: without a tree until all are written
+    c->haveBvh = false;
+    c->twoLevel = false;
+    c->numWide = 0;
     stk_status st;
     if ((st = devUpload(c, c->tlasWide, t.tlasWide.data(), t.tlasWide.size() * sizeof(StkWideNode))) != STK_OK)
         return st;
@@ -481,12 +497,13 @@ STK_API stk_status stk_build_two_level(stk_ctx* c, const StkBvhOptions* opts)
     if ((st = devUpload(c, c->instW, t.instW.data(), t.instW.size() * sizeof(float))) != STK_OK) return st;
     if ((st = devUpload(c, c->tlEmit, t.emitters.data(), t.emitters.size() * sizeof(StkTriangle))) != STK_OK)
         return st;
+    c->tl = std::move(t);
     // the flat tree goes; its header keeps the scene box (the cache's domain)
     c->bvh = stk::Bvh{};
     c->bvh.header.root = INT32_MIN;
     for (int a = 0; a < 3; ++a) {
-        c->bvh.header.lo[a] = t.lo[a];
-        c->bvh.header.hi[a] = t.hi[a];
+        c->bvh.header.lo[a] = c->tl.lo[a];
+        c->bvh.header.hi[a] = c->tl.hi[a];
     }
     c->gpuBuilt = false;
     c->twoLevel = true;
@@ -501,22 +518,22 @@ STK_API stk_status stk_set_bvh(stk_ctx* c, const StkBvhNode* nodes, uint32_t n)
     if (!c->haveGeometry) return fail(c, STK_E_STATE, "stk_set_bvh: no geometry set");
     if (n && !nodes) return fail(c, STK_E_INVALID, "stk_set_bvh: null buffer");
     std::string err;
+    // validated aside: a rejected tree leaves the context's tree as it was
+    stk::Bvh b;
     try {
-        c->bvh = stk::Bvh{};
-        c->bvh.nodes.assign(nodes, nodes + n);
+        b.nodes.assign(nodes, nodes + n);
         if (!stk::packBvh(c->hvb.data(), (uint32_t)c->hvb.size(), c->hib.data(), (uint32_t)c->hib.size(),
-                          c->hinst.data(), (uint32_t)c->hinst.size(), c->bvh, err) ||
-            !stk::buildWide(c->hinst.data(), (uint32_t)c->hinst.size(), c->bvh, err))
+                          c->hinst.data(), (uint32_t)c->hinst.size(), b, err) ||
+            !stk::buildWide(c->hinst.data(), (uint32_t)c->hinst.size(), b, err))
             return fail(c, STK_E_INVALID, "stk_set_bvh: " + err);
     } catch (const std::bad_alloc&) {
         return fail(c, STK_E_OOM, "stk_set_bvh: host allocation failed");
     }
-    if (c->bvh.depth > (uint32_t)stk::dev::kStackDepth)
+    if (b.depth > (uint32_t)stk::dev::kStackDepth)
         return fail(c, STK_E_INVALID, "stk_set_bvh: tree deeper than the traversal stack (32)");
     (void)hipSetDevice(c->device);
-    c->twoLevel = false;
     invalidateGraph(c);
-    return uploadBvh(c);
+    return uploadBvh(c, std::move(b));
 }
 
 STK_API stk_status stk_get_bvh(stk_ctx* c, StkBvhNode* nodes, uint32_t cap, uint32_t* n)
@@ -542,6 +559,23 @@ STK_API stk_status stk_get_bvh(stk_ctx* c, StkBvhNode* nodes, uint32_t cap, uint
     if (!nodes) return STK_OK;
     if (cap < *n) return fail(c, STK_E_INVALID, "stk_get_bvh: capacity too small");
     memcpy(nodes, c->bvh.nodes.data(), c->bvh.nodes.size() * sizeof(StkBvhNode));
+    return STK_OK;
+}
+
+STK_API stk_status stk_get_bvh_info(stk_ctx* c, StkBvhInfo* out)
+{
+    if (!validCtx(c) || !out) return STK_E_INVALID;
+    if (!c->haveBvh) return fail(c, STK_E_STATE, "stk_get_bvh_info: no BVH");
+    memset(out, 0, sizeof(*out));
+    out->header = c->bvh.header;
+    out->depth = c->bvh.depth;
+    out->wideDepth = c->twoLevel ? c->tl.tlasDepth + c->tl.blasDepth : c->bvh.wideDepth;
+    out->numNodes = c->numNodes;
+    out->numWide = c->twoLevel ? (uint32_t)(c->tl.tlasWide.size() + c->tl.blasWide.size()) : c->numWide;
+    out->numEmitters = c->twoLevel ? (uint32_t)c->tl.emitters.size() : c->numEmit;
+    out->twoLevel = c->twoLevel ? 1u : 0u;
+    out->gpuBuilt = c->gpuBuilt ? 1u : 0u;
+    out->walkStack = stk::closestWalkStack(bvhView(c));
     return STK_OK;
 }
 

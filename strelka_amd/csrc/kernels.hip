@@ -47,9 +47,11 @@ static_assert(2 * kWideStack <= kStackDepth, "the wide walk's two stacks share t
 // Closest hit over the compressed 8-wide tree (two LDS node-group stacks per
 // lane).  The binary walk has a kernel of its own (56 registers instead of
 // the wide walk's 67).
-// S: stack entries per lane; a walk holds at most one entry per level below
-// the root, so a tree of at most S levels fits.  Fewer entries mean less LDS
-// per wave and more waves per CU.
+// S: stack entries per lane.  At a node of level L (the root is level 1) the
+// stack holds at most one group of remaining siblings per level 2..L, and only
+// a node with inner children pushes, so a walk over a tree of D levels holds
+// at most D - 1 entries: a tree of at most S + 1 levels fits.  Fewer entries
+// mean less LDS per wave and more waves per CU.
 template <int S>
 __global__ __launch_bounds__(kWalkBlock) void k_trace_closest(BvhView bvh, const StkRay* __restrict__ rays,
                                                               uint32_t n, StkHit* __restrict__ hits)
@@ -264,7 +266,7 @@ hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n
     } else if ((flags & STK_TRACE_BVH2) || bvh.wideRoot == STK_INVALID_INDEX) {
         hipLaunchKernelGGL(k_trace_closest_packed, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh, rays, n, hits);
     } else {
-        if (bvh.wideDepth <= (uint32_t)kWideStackShallow)
+        if (closestWalkStack(bvh) == (uint32_t)kWideStackShallow)
             hipLaunchKernelGGL(k_trace_closest<kWideStackShallow>, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh,
                                rays, n, hits);
         else
@@ -272,6 +274,12 @@ hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n
                                hits);
     }
     return hipGetLastError();
+}
+
+uint32_t closestWalkStack(const BvhView& bvh)
+{
+    if (bvh.tlas || bvh.wideRoot == STK_INVALID_INDEX) return 0;
+    return bvh.wideDepth <= (uint32_t)kWideStackShallow + 1 ? (uint32_t)kWideStackShallow : (uint32_t)kWideStack;
 }
 
 hipError_t launchTraceAny(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags,

@@ -136,6 +136,11 @@ struct LaunchConfig {
     uint32_t passStreams = 1;      // n > 1: pass k runs on stream k % n
 };
 
+// Stack entries per lane of the wide closest-hit kernel a flat tree takes (11
+// for trees of at most 12 wide levels, else 16); 0 when the tree has no wide
+// walk (two-level structures, or no wide tree).
+uint32_t closestWalkStack(const BvhView& bvh);
+
 // flags: STK_TRACE_REFERENCE_ORDER (skip-link walk), STK_TRACE_BVH2 (packed
 // binary walk instead of the wide one; closest hit only).
 hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags,

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/stk/stk.h"
+#include "fmath.h"
 #include "geom.hpp"
 
 namespace stk {
@@ -179,14 +180,14 @@ __device__ __forceinline__ LightSample sample_rect_light(const StkLight& l, floa
     const F3 v00 = mk3(x0, y0, z0), v01 = mk3(x0, y1, z0), v10 = mk3(x1, y0, z0), v11 = mk3(x1, y1, z0);
     const F3 n0 = nrm(cross(v00, v10)), n1 = nrm(cross(v10, v11));
     const F3 n2 = nrm(cross(v11, v01)), n3 = nrm(cross(v01, v00));
-    const float g0 = acosf(-dot(n0, n1)), g1 = acosf(-dot(n1, n2));
-    const float g2 = acosf(-dot(n2, n3)), g3 = acosf(-dot(n3, n0));
+    const float g0 = stk_acosf(-dot(n0, n1)), g1 = stk_acosf(-dot(n1, n2));
+    const float g2 = stk_acosf(-dot(n2, n3)), g3 = stk_acosf(-dot(n3, n0));
     const float b0 = n0.z, b1 = n2.z, b0sq = b0 * b0;
     const float k = 2.0f * kPi - g2 - g3;
     const float S = g0 + g1 - k;
     // SphQuadSample (shaders/lights.h:168-194)
     const float au = u * S + k;
-    const float fu = (cosf(au) * b0 - b1) / sinf(au);
+    const float fu = (stk_cosf(au) * b0 - b1) / stk_sinf(au);
     float cu = 1.0f / sqrtf(fu * fu + b0sq) * (fu > 0 ? 1.0f : -1.0f);
     cu = fminf(fmaxf(cu, -1.0f), 1.0f);
     float xu = -(cu * z0) / sqrtf(1.0f - cu * cu);
@@ -318,7 +319,7 @@ __device__ __forceinline__ float ggx_g1(float a2, F3 w, F3 n, F3 m)
 }
 __device__ __forceinline__ F3 schlick(F3 f0, float c)
 {
-    const float w = powf(1.0f - c, 5.0f);
+    const float w = stk_pow5f(1.0f - c);
     return mk3(f0.x + (1.0f - f0.x) * w, f0.y + (1.0f - f0.y) * w, f0.z + (1.0f - f0.z) * w);
 }
 
@@ -339,7 +340,7 @@ __device__ __forceinline__ BsdfSample bsdf_sample(const StkMaterial& m, F3 N, F3
         const float cosT = sqrtf((1.0f - x0) / (x0 * (alpha * alpha - 1.0f) + 1.0f));
         const float sinT = sqrtf(saturate(1.0f - cosT * cosT));
         const float phi = 2.0f * kPi * x1;
-        const F3 hl = mk3(cosf(phi) * sinT, sinf(phi) * sinT, cosT + 1e-6f);
+        const F3 hl = mk3(stk_cosf(phi) * sinT, stk_sinf(phi) * sinT, cosT + 1e-6f);
         const F3 h = nrm(to_world(T, B, N, hl));
         const float k1h = dot(k1, h);
         const F3 k2 = h * (2.0f * k1h) - k1;
@@ -358,7 +359,7 @@ __device__ __forceinline__ BsdfSample bsdf_sample(const StkMaterial& m, F3 N, F3
     const float cosT = sqrtf(saturate(x0));
     const float sinT = sqrtf(saturate(1.0f - cosT * cosT));
     const float phi = 2.0f * kPi * x1;
-    const F3 k2 = to_world(T, B, N, mk3(cosf(phi) * sinT, sinf(phi) * sinT, cosT + 1e-6f));
+    const F3 k2 = to_world(T, B, N, mk3(stk_cosf(phi) * sinT, stk_sinf(phi) * sinT, cosT + 1e-6f));
     if (dot(k2, Ng) <= 0.0f) return s;
     s.k2 = k2;
     s.pdf = kInvPi * (saturate(dot(N, k2)) + 1e-6f);

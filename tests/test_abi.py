@@ -95,7 +95,7 @@ HEADER_STRUCTS = {
     "StkPathTracerParam": "PathTracerParam", "StkRenderParams": "RenderParams", "StkCacheConfig": "CacheConfig",
     "StkCacheInfo": "CacheInfo", "StkBvhOptions": "BvhOptions", "StkBvhHeader": "BvhHeader",
     "StkLaunchConfig": "LaunchConfig", "StkDeviceInfo": "DeviceInfo", "StkBatchScene": "BatchScene",
-    "StkBatchViewInfo": "BatchViewInfo", "StkImageDesc": "ImageDesc",
+    "StkBatchViewInfo": "BatchViewInfo", "StkImageDesc": "ImageDesc", "StkBvhInfo": "BvhInfo",
 }
 
 

@@ -106,7 +106,8 @@ def test_gpu_path_samples_match_golden(gold):
     x, y, w, h = m["crop"]
     got = smp[y:y + h, x:x + w].reshape(-1, 3)
     ref = g["pt"].reshape(-1, 3)
-    # per channel within 1e-4 (relative above 1); at most 0.5% of samples may
-    # take another branch (device transcendentals differ from glibc by ulps)
+    # every sample within 1e-4 per channel (relative above 1): kernels and
+    # oracle share their transcendentals, so no path takes another branch
     err = np.abs(got - ref) / np.maximum(1.0, np.abs(ref))
-    assert np.mean(np.any(err > 1e-4, axis=1)) <= 0.005
+    print(f"max error {float(err.max()):.3e}")
+    assert not np.any(err > 1e-4)

@@ -90,7 +90,7 @@ def test_batch_view_matches_oracle(scenes):
     err = np.abs(smp.reshape(-1, 3) - cpu.reshape(-1, 3)) / np.maximum(1.0, np.abs(cpu.reshape(-1, 3)))
     frac = np.any(err > 1e-4, axis=1).mean()
     print(f"divergent samples {frac:.5f}")
-    assert frac <= 0.005
+    assert frac == 0
 
 
 SHAPES = [dict(block_size=64, tile=1), dict(block_size=128, tile=16), dict(block_size=256, tile=8),

@@ -77,7 +77,12 @@ def test_gpu_tree_config2_and_render(ctx, method):
     ctx.build_bvh_gpu(method)
     print(f"device build {time.perf_counter() - t0:.3f} s")
     nodes = ctx.get_bvh()
-    print(f"inner-area cost {sah_cost(nodes):.1f}")
+    info = ctx.bvh_info()
+    print(f"inner-area cost {sah_cost(nodes):.1f}, wide levels {info.wideDepth}, walk stack {info.walkStack}")
+    assert info.gpuBuilt == 1 and info.walkStack == (11 if info.wideDepth <= 12 else 16)
+    if method == "lbvh":
+        # the linear tree is deeper: it is the case that runs the 16-entry kernel
+        assert info.walkStack == 16
     o = Oracle.from_scene(s, nodes)
     rays = s.jittered_rays(0x5354524B00000002)[::37]
     assert same(ctx.trace_closest(rays), o.trace(rays, MODE_REF)[0])

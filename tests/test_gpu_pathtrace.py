@@ -11,12 +11,12 @@ from oracle import Oracle, load
 
 pytestmark = pytest.mark.gpu
 
-# per-sample tolerance: |gpu - cpu| <= 1e-4 * max(1, |cpu|) per channel; the
-# transcendental functions differ by a few ulp between the device library and
-# glibc, so a path may rarely take another branch (a sample near a threshold);
-# at most 0.5% of samples may do so.
+# per-sample tolerance: |gpu - cpu| <= 1e-4 * max(1, |cpu|) per channel on
+# every sample.  Kernels and oracle share their transcendentals
+# (strelka_amd/csrc/fmath.h) and evaluate the same IEEE binary32 operations
+# in the same order, so no path takes another branch.
 TOL = 1e-4
-MAX_DIVERGENT = 0.005
+MAX_DIVERGENT = 0
 
 
 @pytest.fixture(scope="module")
@@ -31,20 +31,24 @@ def compare_samples(gpu, cpu):
     c = cpu.reshape(-1, 3)
     err = np.abs(g - c) / np.maximum(1.0, np.abs(c))
     bad = np.any(err > TOL, axis=1)
+    print(f"samples {g.shape[0]}, beyond 1e-4: {int(bad.sum())}, max error {float(err.max()):.3e}, "
+          f"bit-identical {float(np.mean(np.all(g == c, axis=1))):.4f}")
     return bad.mean(), err
 
 
-def test_per_sample_radiance_small_scene(ctx):
-    print("case: config 6 scene, 64x48, 4 spp, 8 bounces", flush=True)
+@pytest.mark.parametrize("sampling", [0, 1, 2, 3])
+def test_per_sample_radiance_small_scene(ctx, sampling):
+    spp = 16 if sampling >= 2 else 4
+    print(f"case: config 6 scene, 8 bounces, {spp} spp, sampling type {sampling}", flush=True)
     s = sa.SceneData.generate(6)
     ctx.set_scene(s)
     ctx.build_bvh()
     o = Oracle.from_scene(s, ctx.get_bvh())
-    p = sa.render_params(s.camera, spp=4, max_depth=8, frame=2, sampling=1, sky=s.sky, keep_samples=True)
+    p = sa.render_params(s.camera, spp=spp, max_depth=8, frame=2, sampling=sampling, sky=s.sky, keep_samples=True)
     w, h = p.pt.dimension[0], p.pt.dimension[1]
     ctx.reset_accumulation()
     ctx.render(p)
-    gpu = ctx.samples(w * h * 4 * 3)
+    gpu = ctx.samples(w * h * spp * 3)
     cpu = o.pt_frame(p, 0, 0, w, h)
     frac, err = compare_samples(gpu, cpu)
     print(f"divergent samples {frac:.5f}, median err {np.median(err):.2e}")
@@ -72,7 +76,11 @@ def test_per_sample_radiance_config2_crop(ctx):
 
 
 def test_reduction_accumulation_tonemap(ctx):
-    print("case: post passes, config 6, 3 frames", flush=True)
+    """Reduction, running-mean accumulation and all three tonemappers
+    (Reinhard, the reference's default in src/hdRunner/main.cpp:422; ACES
+    fitted; ACESFilm) bit for bit, over frames that continue one
+    accumulation."""
+    print("case: post passes, config 6, 6 frames, tonemappers 0, 1, 2", flush=True)
     s = sa.SceneData.generate(6)
     ctx.set_scene(s)
     ctx.build_bvh()
@@ -81,9 +89,10 @@ def test_reduction_accumulation_tonemap(ctx):
     npix = w * h
     ctx.reset_accumulation()
     hist = np.zeros((npix, 4), np.float32)
-    for frame in range(3):
+    for frame in range(6):
+        kind = frame % 3
         p = sa.render_params(s.camera, spp=2, max_depth=4, frame=frame, sampling=1, sky=s.sky, keep_samples=True,
-                             tonemap=1)
+                             tonemap=kind)
         out = ctx.render(p)
         samples = ctx.samples(npix * 2 * 3)
         cur = np.zeros((npix, 4), np.float32)
@@ -93,10 +102,12 @@ def test_reduction_accumulation_tonemap(ctx):
                            frame, acc.ctypes.data_as(C.POINTER(C.c_float)))
         hist = acc
         tm = np.zeros((npix, 4), np.float32)
-        lib.orc_tonemap(acc.ctypes.data_as(C.POINTER(C.c_float)), npix, 1, tm.ctypes.data_as(C.POINTER(C.c_float)))
+        lib.orc_tonemap(acc.ctypes.data_as(C.POINTER(C.c_float)), npix, kind,
+                        tm.ctypes.data_as(C.POINTER(C.c_float)))
         # same samples in, same op order: the post passes are bit-exact
+        # (Reinhard keeps the reference's NaN for black pixels)
         assert np.array_equal(ctx.accumulation(w, h).reshape(-1, 4)[:, :3], acc[:, :3])
-        assert np.array_equal(out.reshape(-1, 4)[:, :3], tm[:, :3])
+        assert np.array_equal(out.reshape(-1, 4)[:, :3], tm[:, :3], equal_nan=True), kind
 
 
 def test_progressive_accumulation_converges(ctx):

@@ -64,6 +64,10 @@ def test_config1_all_rays_bit_exact(ctx):
 def test_config2_subsample_bit_exact_and_full_frame_orders_agree(ctx):
     print("case: config 2, 1080p jittered rays", flush=True)
     s, o = setup(ctx, 2)
+    info = ctx.bvh_info()
+    print(f"SAH tree: depth {info.depth}, wide levels {info.wideDepth}, walk stack {info.walkStack}")
+    # this tree is walked by the 11-entry kernel (at most 12 wide levels)
+    assert info.wideDepth <= 12 and info.walkStack == 11
     rays = s.jittered_rays(0x5354524B00000002)
     sub = rays[::37].copy()
     ref, _ = o.trace(sub, MODE_PRUNED)
@@ -90,11 +94,15 @@ def test_config2_subsample_bit_exact_and_full_frame_orders_agree(ctx):
     assert np.mean(full["instId"] != INVALID) > 0.9999
 
 
-def test_device_entry_points_and_camera_rays(ctx):
-    print("case: device buffers + camera ray kernel, config 1", flush=True)
+@pytest.mark.parametrize("sampling", [0, 1, 2, 3])
+def test_device_entry_points_and_camera_rays(ctx, sampling):
+    """Every stratifiedSamplingType (0 centre, the reference's default in
+    src/hdRunner/main.cpp:419; 1 random; 2 stratified; 3 correlated
+    multi-jittered) gives the oracle's generateCameraRay bit for bit."""
+    print(f"case: device buffers + camera ray kernel, config 1, sampling type {sampling}", flush=True)
     s, o = setup(ctx, 1)
     cam = s.camera
-    cam.stratifiedSamplingType = 1
+    cam.stratifiedSamplingType = sampling
     cam.frameNumber = 3
     cam.spp = 2
     n = cam.dimension[0] * cam.dimension[1]
@@ -108,7 +116,7 @@ def test_device_entry_points_and_camera_rays(ctx):
         from oracle import load
         lib = load()
         exp = np.zeros(1, dtype=sa.RAY)
-        for i in list(range(0, n, 997)) + [n - 1]:
+        for i in list(range(0, n, 61)) + [n - 1]:
             px, py = i % cam.dimension[0], i // cam.dimension[0]
             frame = cam.frameNumber * cam.spp + 1
             st = C.c_uint32(lib.orc_init_rng(px, py, cam.dimension[0], frame))
@@ -207,3 +215,30 @@ def test_render_image_pitch_and_rgba8():
     ctx.host_free(hp)
     ctx.close()
     assert np.array_equal(got, ref)
+
+
+def test_failed_tree_updates_keep_the_previous_tree(ctx):
+    """A rejected stk_set_bvh, stk_build_bvh or stk_build_two_level leaves the
+    context tracing the tree it had (host description and device arrays of the
+    same tree), so the next trace returns the previous hits."""
+    print("case: config 1, rejected set_bvh / build_bvh / build_two_level between two traces", flush=True)
+    s, o = setup(ctx, 1)
+    rays = s.jittered_rays(0x5354524B00000001)[::8].copy()
+    before = ctx.trace_closest(rays)
+    info0 = ctx.bvh_info()
+    nodes = ctx.get_bvh()
+    broken = nodes.copy()
+    broken[1]["nodeOffset"] = 10 ** 8  # a skip link past the array
+    with pytest.raises(sa.StkError) as e:
+        ctx.set_bvh(broken)
+    assert e.value.status == 1
+    assert same(ctx.trace_closest(rays), before)
+    with pytest.raises(sa.StkError):
+        ctx.build_bvh(max_depth=40)  # deeper than the walks' stacks allow
+    assert same(ctx.trace_closest(rays), before)
+    info1 = ctx.bvh_info()
+    assert (info1.wideDepth, info1.numWide, info1.numNodes, info1.walkStack) == \
+        (info0.wideDepth, info0.numWide, info0.numNodes, info0.walkStack)
+    # a valid replacement still takes effect
+    ctx.set_bvh(nodes)
+    assert same(ctx.trace_closest(rays), before)

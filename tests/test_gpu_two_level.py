@@ -93,7 +93,16 @@ def test_full_instanced_scene_matches_flat_tree(ctx):
     rays = s.jittered_rays(0x5354524B00000005)
     a = ctx.trace_closest(rays)
     ctx.build_bvh()
+    info = ctx.bvh_info()
+    print(f"flat tree: depth {info.depth}, wide levels {info.wideDepth}, walk stack {info.walkStack}")
+    # 12 wide levels: the deepest tree the 11-entry kernel takes (a walk holds
+    # at most one entry per level below the root)
+    assert info.wideDepth == 12 and info.walkStack == 11
     b = ctx.trace_closest(rays)
+    # the boundary case against the reference walk over the same nodes
+    sub = rays[::997].copy()
+    ref, _ = Oracle.from_scene(s, ctx.get_bvh()).trace(sub, MODE_REF)
+    assert b[::997].tobytes() == ref.tobytes()
     assert (a["instId"] != INVALID).mean() > 0.05
     assert np.array_equal(a["t"], b["t"])
     differ = (a["instId"] != b["instId"]) | (a["primId"] != b["primId"])
