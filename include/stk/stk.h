@@ -165,6 +165,18 @@ STK_API stk_status stk_get_bvh_info(stk_ctx* ctx, StkBvhInfo* out);
 /* Host buffers; synchronous. */
 STK_API stk_status stk_trace_closest(stk_ctx* ctx, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags);
 STK_API stk_status stk_trace_any(stk_ctx* ctx, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags);
+/* stk_trace_closest without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (serving loops,
+ * harnesses that forbid hipMalloc).  The rays and hits are staged through one
+ * static device buffer of the library (52 MiB on each GPU it runs on, held
+ * until the process ends) instead of buffers the context allocates, so at most
+ * STK_NOALLOC_MAX_RAYS rays per call (more: STK_E_INVALID, nothing traced).
+ * Same results as stk_trace_closest; returns when the hits are in `hits`.
+ * Every context on one GPU shares that GPU's buffer: calls that overlap on one
+ * GPU (from several threads or contexts) race. */
+#define STK_NOALLOC_MAX_RAYS 1048576u
+STK_API stk_status stk_trace_closest_noalloc(stk_ctx* ctx, const StkRay* rays, uint32_t n, StkHit* hits,
+                                             uint32_t flags);
 /* Device buffers; enqueued on the context stream, asynchronous. */
 STK_API stk_status stk_trace_closest_device(stk_ctx* ctx, const StkRay* dRays, uint32_t n, StkHit* dHits,
                                             uint32_t flags);

@@ -359,6 +359,17 @@ class Context:
         self._check(_LIB.stk_trace_closest(self._h, _ptr(rays), rays.size, _ptr(hits), flags), "stk_trace_closest")
         return hits
 
+    def trace_closest_noalloc(self, rays: np.ndarray, reference_order: bool = False,
+                              binary: bool = False) -> np.ndarray:
+        """stk_trace_closest_noalloc: the same hits, staged through the
+        library's static device buffer (at most 1,048,576 rays per call)."""
+        rays = np.ascontiguousarray(rays, dtype=RAY)
+        hits = np.zeros(rays.size, dtype=HIT)
+        flags = (1 if reference_order else 0) | (2 if binary else 0)
+        self._check(_LIB.stk_trace_closest_noalloc(self._h, _ptr(rays), rays.size, _ptr(hits), flags),
+                    "stk_trace_closest_noalloc")
+        return hits
+
     def trace_any(self, rays: np.ndarray, reference_order: bool = False) -> np.ndarray:
         rays = np.ascontiguousarray(rays, dtype=RAY)
         hits = np.zeros(rays.size, dtype=HIT)

@@ -161,6 +161,7 @@ PROTOTYPES = [
     ("stk_get_bvh", C.c_int, [P, P, U32, C.POINTER(U32)]),
     ("stk_get_bvh_info", C.c_int, [P, P]),
     ("stk_trace_closest", C.c_int, [P, P, U32, P, U32]),
+    ("stk_trace_closest_noalloc", C.c_int, [P, P, U32, P, U32]),
     ("stk_trace_any", C.c_int, [P, P, U32, P, U32]),
     ("stk_trace_closest_device", C.c_int, [P, P, U32, P, U32]),
     ("stk_trace_any_device", C.c_int, [P, P, U32, P, U32]),

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <atomic>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <new>
 #include <string>
@@ -45,6 +46,41 @@ struct BatchScene {
     stk::Bvh bvh;
     std::string err;
 };
+
+// ---- static staging for stk_trace_closest_noalloc ----------------------------
+// The one entry point that allocates nothing stages its rays and hits through
+// this file-scope device array instead of the context's allocations: rays in
+// the first 32 MiB, hits in the next 20 MiB.  Every GPU the library runs on
+// holds its own copy (52 MiB) until the process ends.  In host code `scratch`
+// names a host-side stand-in, never the device copy: host code uses the
+// per-device address looked up by setupScratch.
+constexpr uint32_t kNoAllocMaxRays = 1u << 20;
+constexpr size_t kNoAllocRayBytes = (size_t)kNoAllocMaxRays * sizeof(StkRay);
+constexpr size_t kNoAllocBytes = kNoAllocRayBytes + (size_t)kNoAllocMaxRays * sizeof(StkHit);
+static_assert(kNoAllocRayBytes % 256 == 0, "the hit records start 256-B aligned");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratchMu;
+std::atomic<void*> g_scratchPtr[kMaxScratchDevices];
+hipError_t g_scratchErr[kMaxScratchDevices];
+
+// With `device` current: looks up that device's copy of `scratch`, once per
+// device (a failed lookup is retried by the next context created there).
+void setupScratch(int device)
+{
+    if (device < 0 || device >= kMaxScratchDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (g_scratchPtr[device].load() != nullptr) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratchErr[device] = e;
+    if (e != hipSuccess || p == nullptr) {
+        (void)hipGetLastError();
+        return;
+    }
+    g_scratchPtr[device].store(p);
+}
 
 bool failAllocForTest()
 {
@@ -315,6 +351,7 @@ STK_API stk_status stk_create(int hipDevice, stk_ctx** out)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return STK_E_DEVICE;
     for (auto& ev : c->events)
         if (hipEventCreate(&ev) != hipSuccess) return STK_E_DEVICE;
+    setupScratch(hipDevice);
     *out = c.release();
     return STK_OK;
 }
@@ -709,6 +746,33 @@ STK_API stk_status stk_trace_closest(stk_ctx* c, const StkRay* rays, uint32_t n,
 STK_API stk_status stk_trace_any(stk_ctx* c, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags)
 {
     return traceHost(c, rays, n, hits, 1, flags);
+}
+
+STK_API stk_status stk_trace_closest_noalloc(stk_ctx* c, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    if (!c->haveBvh) return fail(c, STK_E_STATE, "stk_trace_closest_noalloc: no BVH");
+    if (flatOnlyFlags(c, flags) != STK_OK) return STK_E_STATE;
+    if (n > kNoAllocMaxRays)
+        return fail(c, STK_E_INVALID, "stk_trace_closest_noalloc: more rays than the static staging buffer holds "
+                                      "(1048576); split the batch or use stk_trace_closest");
+    if (n == 0) return STK_OK;
+    if (!rays || !hits) return fail(c, STK_E_INVALID, "stk_trace_closest_noalloc: null buffer");
+    void* base = (c->device >= 0 && c->device < kMaxScratchDevices) ? g_scratchPtr[c->device].load() : nullptr;
+    if (!base)
+        return fail(c, STK_E_DEVICE, std::string("stk_trace_closest_noalloc: no static staging buffer on this device: ") +
+                                         hipGetErrorString(c->device < kMaxScratchDevices ? g_scratchErr[c->device]
+                                                                                          : hipErrorInvalidDevice));
+    (void)hipSetDevice(c->device);
+    // every ray is copied in and every hit written by the kernel: nothing of a
+    // previous call's contents is read
+    StkRay* dr = static_cast<StkRay*>(base);
+    StkHit* dh = reinterpret_cast<StkHit*>(static_cast<unsigned char*>(base) + kNoAllocRayBytes);
+    STK_HIP(c, hipMemcpyAsync(dr, rays, (size_t)n * sizeof(StkRay), hipMemcpyHostToDevice, c->stream));
+    STK_HIP(c, stk::launchTraceClosest(bvhView(c), dr, n, dh, flags, c->stream));
+    STK_HIP(c, hipMemcpyAsync(hits, dh, (size_t)n * sizeof(StkHit), hipMemcpyDeviceToHost, c->stream));
+    STK_HIP(c, hipStreamSynchronize(c->stream));
+    return STK_OK;
 }
 
 STK_API stk_status stk_generate_camera_rays_device(stk_ctx* c, const StkPathTracerParam* p, uint32_t sampleNum,
