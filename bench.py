@@ -42,15 +42,35 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+# Host threads of the CPU baseline: every core this process may run on, up to
+# the host-CPU share of one GPU on the GPU boxes (16 threads); nproc and the
+# affinity mask are reported beside it.
+CPU_SHARE_PER_GPU = 16
+
+
+def cpu_threads() -> dict:
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"nproc": os.cpu_count() or 1, "affinity": allowed, "threads": max(1, min(allowed, CPU_SHARE_PER_GPU))}
+
+
 def cpu_baseline(scene, nodes, rays, wide, wide_tris):
     """The oracle's closest-hit on the host cores over the same rays and BVH:
     the reference order, the binary kernel's pruned order and the wide
-    kernel's walk (whose counters give that kernel's algorithmic bytes)."""
+    kernel's walk (whose counters give that kernel's algorithmic bytes).
+    Timed on a -march=native build of the oracle made on this host, after
+    checking that its hits equal the portable build's."""
     from oracle import Oracle
     from oracle.pyoracle import MODE_PRUNED, MODE_REF, MODE_WIDE
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    o = Oracle.from_scene(scene, nodes, wide, wide_tris)
+    threads = cpu_threads()["threads"]
+    o = Oracle.from_scene(scene, nodes, wide, wide_tris, native=True)
+    portable = Oracle.from_scene(scene, nodes, wide, wide_tris)
+    check = rays[::97].copy()
+    if o.trace(check, MODE_REF, threads)[0].tobytes() != portable.trace(check, MODE_REF, threads)[0].tobytes():
+        raise RuntimeError("the -march=native oracle build gives other hits than the portable build")
     res = {}
     for name, mode in (("reference_order", MODE_REF), ("pruned", MODE_PRUNED), ("wide", MODE_WIDE)):
         o.trace(rays[:4096], mode, threads)  # warm-up
@@ -70,8 +90,8 @@ def cpu_pt_baseline(scene, nodes, params, rows: int = 64):
     frame (full width, `rows` rows from the middle, the frame's spp)."""
     from oracle import Oracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    o = Oracle.from_scene(scene, nodes)
+    threads = cpu_threads()["threads"]
+    o = Oracle.from_scene(scene, nodes, native=True)
     w, h = params.pt.dimension[0], params.pt.dimension[1]
     y0 = max(0, h // 2 - rows // 2)
     t0 = time.perf_counter()
@@ -79,8 +99,8 @@ def cpu_pt_baseline(scene, nodes, params, rows: int = 64):
     t = time.perf_counter() - t0
     n = w * rows * params.pt.spp
     return {"value": round(n / t / 1e6, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"{w}x{rows} band of the frame at {params.pt.spp} spp ({n} samples), oracle path tracer, "
-                      f"{cpu_model()}", "seconds": round(t, 2)}
+            "sample": f"{w}x{rows} band of the frame at {params.pt.spp} spp ({n} samples), oracle path tracer "
+                      f"(-march=native build)", "cpu_model": cpu_model(), "seconds": round(t, 2)}
 
 
 def pt_frame_bench(ctx, scene, spp: int, frames: int, d_out: int) -> dict:
@@ -268,6 +288,40 @@ def load_traffic(kernel: str):
     return None
 
 
+def load_counters(kernel: str) -> dict | None:
+    """Per-launch counter figures of a kernel from its committed profile
+    (profiles/<kernel>_counters.json, written from rocprofv3 --pmc runs)."""
+    f = ROOT / "profiles" / f"{kernel}_counters.json"
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text())
+    except (OSError, ValueError):
+        return None
+
+
+def limiter(counters: dict | None, traffic: float | None, kernel_ms: float) -> dict | None:
+    """The resource that binds the kernel, from its counters: the VALU issue
+    fraction (SIMD quad-cycles in which at least one VALU instruction issues,
+    (SQ_ACTIVE_INST_VALU - SQ_ACTIVE_INST_VALU2) / (4 CUs x GRBM_GUI_ACTIVE per
+    XCD / 4)), the measured HBM fraction (measured bytes per launch over this
+    run's launch time and the 8 TB/s peak) and the L2 hit rate; the binding
+    resource is the one nearest its limit."""
+    if not counters:
+        return None
+    valu = counters.get("valu_issue_quad_cycle_frac")
+    hbm = (traffic / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None
+    cands = {k: v for k, v in (("valu_issue", valu), ("hbm", hbm)) if v is not None}
+    resource = max(cands, key=cands.get) if cands else None
+    return {"resource": resource, "frac": round(cands[resource], 4) if resource else None,
+            "valu_busy": counters.get("valu_busy"), "valu_dual_issue_share": counters.get("valu_dual_issue_share"),
+            "valu_thread_utilization": counters.get("valu_utilization"),
+            "hbm_measured_frac": round(hbm, 4) if hbm is not None else None,
+            "l2_hit_rate": counters.get("l2_hit_rate"), "l2_requests_per_ray": counters.get("l2_requests_per_ray"),
+            "valu_instructions_per_ray": counters.get("valu_insts_per_ray"),
+            "source": counters.get("source")}
+
+
 def cache_study(args) -> dict:
     """Time to reach an RMSE target against a converged plain render, for the
     plain path tracer and the cached one (training included in its time).
@@ -400,6 +454,8 @@ def main() -> int:
                          "batch: the 32-scene batched path tracer; instanced: the 19M-triangle instanced scene")
     ap.add_argument("--structures", default="two_level,flat,flat_device,flat_device_ploc",
                     help="instanced workload: the structures to measure (the first one gives value)")
+    ap.add_argument("--trace-only", action="store_true",
+                    help="only the timed closest-hit launches (for counter profiles of that kernel)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run the plain copy kernel over this many MiB (calibrates memory counters)")
     ap.add_argument("--cache-study", action="store_true",
@@ -506,6 +562,11 @@ def main() -> int:
     wall = time.perf_counter() - t0
     kernel_ms = ctx.elapsed_ms(0, 1) / args.steps
     mrays = n * args.steps / wall / 1e6
+    if args.trace_only:
+        print(json.dumps({"metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "kernel_ms": round(kernel_ms, 4),
+                          "steps": args.steps, "rays": n, "walk_stack": ctx.bvh_info().walkStack}))
+        ctx.close()
+        return 0
 
     # reference-order (unpruned skip-link) GPU kernel on the same rays
     ctx.record(2)
@@ -571,8 +632,10 @@ def main() -> int:
         # restatement of the same walk on the same rays)
         bytes_per_ray = 32 + 20 + 80 * wd["inner_per_ray"] + 48 * wd["leaf_per_ray"]
         achieved = n * bytes_per_ray / (kernel_ms * 1e-3) / 1e9
+        traffic = load_traffic("trace_closest_config2")
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("trace_closest_config2"),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "limiter": limiter(load_counters("trace_closest_config2"), traffic, kernel_ms),
                 "kernel": "k_trace_closest (compressed 8-wide walk)", "kernel_ms": round(kernel_ms, 4),
                 "bytes_per_ray": round(bytes_per_ray, 1),
                 "wide_node_visits_per_ray": round(wd["inner_per_ray"], 2),
@@ -584,7 +647,8 @@ def main() -> int:
         cpu = {"value": round(res["reference_order"]["mrays_per_s"], 4), "unit": "Mrays/s", "cores": cores,
                "kind": "port",
                "sample": f"the full {n}-ray 1080p primary frame, oracle closestHit in reference order "
-                         f"(skip-link, unpruned), median of 3, {cpu_model()}",
+                         f"(skip-link, unpruned), median of 3, -march=native build",
+               "cpu_model": cpu_model(), "host_threads": cpu_threads(),
                "pruned_mrays_per_s": round(pr["mrays_per_s"], 4), "wide_mrays_per_s": round(wd["mrays_per_s"], 4)}
         p = sa.render_params(scene.camera, spp=4, max_depth=8, frame=1, sampling=1, sky=scene.sky)
         pt["cpu_baseline"] = cpu_pt_baseline(scene, nodes, p)

@@ -44,52 +44,66 @@ def build_if_missing() -> Path:
 
 
 _lib = None
+_native = None
+
+
+def load_native() -> C.CDLL:
+    """The -march=native timing build (compiled on first use on this host)."""
+    global _native
+    if _native is None:
+        from . import build as _b
+        path = _b.build_native()
+        _native = _bind(C.CDLL(str(path)))
+    return _native
 
 
 def load() -> C.CDLL:
     global _lib
     if _lib is None:
         build_if_missing()
-        lib = C.CDLL(str(LIB_PATH))
-        P, U32, F = C.c_void_p, C.c_uint32, C.c_float
-        FP = C.POINTER(C.c_float)
-        sigs = {
-            "orc_intersect_ray_box": (C.c_int, [P, FP, FP, FP, FP]),
-            "orc_ray_triangle": (C.c_int, [FP, FP, FP, FP, FP, FP, FP]),
-            "orc_transform_point": (None, [FP, FP, FP]),
-            "orc_get_triangle": (None, [P, U32, U32, FP, FP, FP]),
-            "orc_closest_hit": (C.c_int, [P, P, P, P]),
-            "orc_closest_hit_pruned": (C.c_int, [P, P, P, P]),
-            "orc_closest_hit_brute": (C.c_int, [P, P, P]),
-            "orc_any_hit": (C.c_int, [P, P, P, P]),
-            "orc_trace_batch": (None, [P, P, U32, P, C.c_int, C.c_int, P]),
-            "orc_expand_two_level": (U32, [P, U32, P, P, P, P, P, P, U32]),
-            "orc_jenkins": (U32, [U32]),
-            "orc_init_rng": (U32, [U32, U32, U32, U32]),
-            "orc_xorshift": (U32, [C.POINTER(U32)]),
-            "orc_rand": (F, [C.POINTER(U32)]),
-            "orc_permute": (U32, [U32, U32, U32]),
-            "orc_randfloat": (F, [U32, U32]),
-            "orc_camera_ray": (None, [P, U32, U32, C.POINTER(U32), U32, P]),
-            "orc_pixel_jitter": (None, [P, U32, C.POINTER(U32), FP, FP]),
-            "orc_offset_ray": (None, [FP, FP, FP]),
-            "orc_sample_rect_light": (None, [P, FP, FP, FP, FP, FP, FP, FP, FP, FP]),
-            "orc_bsdf_sample": (None, [P, FP, FP, FP, FP, P]),
-            "orc_bsdf_eval": (None, [P, FP, FP, FP, FP, FP, FP, FP]),
-            "orc_pt_sample": (None, [P, P, U32, U32, U32, FP]),
-            "orc_pt_frame": (None, [P, P, U32, U32, U32, U32, FP, C.c_int]),
-            "orc_texture_lookup": (None, [P, FP, C.c_float, C.c_float, FP]),
-            "orc_reduce": (None, [FP, U32, U32, FP]),
-            "orc_accumulate": (None, [FP, FP, U32, U32, FP]),
-            "orc_tonemap": (None, [FP, U32, U32, FP]),
-            "orc_fmath": (None, [U32, FP, U32, FP]),
-        }
-        for name, (res, args) in sigs.items():
-            fn = getattr(lib, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = lib
+        _lib = _bind(C.CDLL(str(LIB_PATH)))
     return _lib
+
+
+def _bind(lib: C.CDLL) -> C.CDLL:
+    P, U32, F = C.c_void_p, C.c_uint32, C.c_float
+    FP = C.POINTER(C.c_float)
+    sigs = {
+        "orc_intersect_ray_box": (C.c_int, [P, FP, FP, FP, FP]),
+        "orc_ray_triangle": (C.c_int, [FP, FP, FP, FP, FP, FP, FP]),
+        "orc_transform_point": (None, [FP, FP, FP]),
+        "orc_get_triangle": (None, [P, U32, U32, FP, FP, FP]),
+        "orc_closest_hit": (C.c_int, [P, P, P, P]),
+        "orc_closest_hit_pruned": (C.c_int, [P, P, P, P]),
+        "orc_closest_hit_brute": (C.c_int, [P, P, P]),
+        "orc_any_hit": (C.c_int, [P, P, P, P]),
+        "orc_trace_batch": (None, [P, P, U32, P, C.c_int, C.c_int, P]),
+        "orc_expand_two_level": (U32, [P, U32, P, P, P, P, P, P, U32]),
+        "orc_jenkins": (U32, [U32]),
+        "orc_init_rng": (U32, [U32, U32, U32, U32]),
+        "orc_xorshift": (U32, [C.POINTER(U32)]),
+        "orc_rand": (F, [C.POINTER(U32)]),
+        "orc_permute": (U32, [U32, U32, U32]),
+        "orc_randfloat": (F, [U32, U32]),
+        "orc_camera_ray": (None, [P, U32, U32, C.POINTER(U32), U32, P]),
+        "orc_pixel_jitter": (None, [P, U32, C.POINTER(U32), FP, FP]),
+        "orc_offset_ray": (None, [FP, FP, FP]),
+        "orc_sample_rect_light": (None, [P, FP, FP, FP, FP, FP, FP, FP, FP, FP]),
+        "orc_bsdf_sample": (None, [P, FP, FP, FP, FP, P]),
+        "orc_bsdf_eval": (None, [P, FP, FP, FP, FP, FP, FP, FP]),
+        "orc_pt_sample": (None, [P, P, U32, U32, U32, FP]),
+        "orc_pt_frame": (None, [P, P, U32, U32, U32, U32, FP, C.c_int]),
+        "orc_texture_lookup": (None, [P, FP, C.c_float, C.c_float, FP]),
+        "orc_reduce": (None, [FP, U32, U32, FP]),
+        "orc_accumulate": (None, [FP, FP, U32, U32, FP]),
+        "orc_tonemap": (None, [FP, U32, U32, FP]),
+        "orc_fmath": (None, [U32, FP, U32, FP]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
 
 
 def fptr(a: np.ndarray):
@@ -128,8 +142,8 @@ class Oracle:
     """Holds the scene arrays and the skip-link BVH the oracle walks."""
 
     def __init__(self, vb, ib, inst, nodes, lights=None, mats=None, wide=None, wide_tris=None, textures=None,
-                 texels=None):
-        self.lib = load()
+                 texels=None, native: bool = False):
+        self.lib = load_native() if native else load()
         self.vb = np.ascontiguousarray(vb)
         self.ib = np.ascontiguousarray(ib, dtype=np.uint32)
         self.inst = np.ascontiguousarray(inst)
@@ -161,9 +175,9 @@ class Oracle:
         self.scene = s
 
     @classmethod
-    def from_scene(cls, scene, nodes, wide=None, wide_tris=None):
+    def from_scene(cls, scene, nodes, wide=None, wide_tris=None, native: bool = False):
         return cls(scene.vb, scene.ib, scene.inst, nodes, scene.lights, scene.mats, wide, wide_tris,
-                   getattr(scene, "textures", None), getattr(scene, "texels", None))
+                   getattr(scene, "textures", None), getattr(scene, "texels", None), native)
 
     @classmethod
     def from_two_level(cls, scene, tl):
