@@ -127,6 +127,15 @@ def pt_frame_bench(ctx, scene, spp: int, frames: int, d_out: int) -> dict:
             "closest_rays_per_frame": rays[0], "shadow_rays_per_frame": rays[1]}
 
 
+def with_graph_state(shape: dict) -> dict:
+    """The launch shape plus whether a captured graph is replayed: the library
+    replays one only when every pass runs on one stream (pass_streams 1) and the
+    radiance cache is off."""
+    out = dict(shape)
+    out["graph_replayed"] = bool(shape.get("graph")) and int(shape.get("pass_streams", 1)) <= 1
+    return out
+
+
 def choose_launch(ctx, spec: str, run, total_paths: int) -> dict:
     """The path tracer's launch shape: the autotuner's choice on `run`, or a
     fixed shape given as JSON."""
@@ -190,7 +199,7 @@ def batch_bench(args) -> dict:
                        "scene_gen_s": round(gen_s, 2)},
             "mrays_per_s": round((rays[0] + rays[1]) / (ms * 1e-3) / 1e6, 1),
             "rays_per_frame": {"closest": rays[0], "shadow": rays[1]},
-            "autotune": {"best": tuned["best"], "best_4spp_ms": tuned["ms"], "default_4spp_ms": tuned["default_ms"],
+            "autotune": {"best": with_graph_state(tuned["best"]), "best_4spp_ms": tuned["ms"], "default_4spp_ms": tuned["default_ms"],
                          "table": tuned["table"],
                          "device": dev["name"], "compute_units": dev["compute_units"]}}
 
@@ -257,7 +266,7 @@ def instanced_bench(args) -> dict:
                 "mrays_per_s": round((rays[0] + rays[1]) / (ms * 1e-3) / 1e6, 1),
                 "rays_per_frame": {"closest": rays[0], "shadow": rays[1]},
                 "primary_closest_mrays_per_s": round(cam_rays.size / (trace_ms * 1e-3) / 1e6, 1),
-                "primary_kernel_ms": round(trace_ms, 4), "launch": tuned["best"],
+                "primary_kernel_ms": round(trace_ms, 4), "launch": with_graph_state(tuned["best"]),
                 "autotune_4spp_ms": {"default": tuned.get("default_ms"), "best": tuned.get("ms")}}
 
     want = set(args.structures.split(","))
@@ -589,7 +598,7 @@ def main() -> int:
     p1 = sa.render_params(scene.camera, spp=4, max_depth=8, frame=0, sampling=1, sky=scene.sky)
     tuned = choose_launch(ctx, args.launch, lambda: ctx.render_device(p1, d_out), n)
     pt = pt_frame_bench(ctx, scene, args.pt_spp, max(1, args.pt_frames), d_out)
-    pt["launch"] = tuned["best"]
+    pt["launch"] = with_graph_state(tuned["best"])
     pt["hw_queues"] = int(os.environ["GPU_MAX_HW_QUEUES"])
     pt["autotune_4spp_ms"] = {"default": tuned.get("default_ms"), "best": tuned.get("ms")}
     ctx.free(d_out)
