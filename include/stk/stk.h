@@ -364,6 +364,12 @@ STK_API stk_status stk_cache_gradients(stk_ctx* ctx, const StkCacheRecord* dRecs
 /* FP32 master parameters in/out (host arrays of gridParams / mlpParams). */
 STK_API stk_status stk_cache_get_params(stk_ctx* ctx, float* grid, float* mlp);
 STK_API stk_status stk_cache_set_params(stk_ctx* ctx, const float* grid, const float* mlp);
+/* The records the last cached render produced (no reference counterpart):
+ * which = 0 the cache queries of its last pass (targets unused; normal.w holds
+ * the path slot), 1 the training records of the frame with their targets.
+ * out may be NULL to query *n; waits for the render. */
+STK_API stk_status stk_cache_last_records(stk_ctx* ctx, uint32_t which, StkCacheRecord* out, uint32_t capacity,
+                                          uint32_t* n);
 
 /* ---- device memory and timing helpers for callers without a HIP binding --- */
 STK_API stk_status stk_device_alloc(stk_ctx* ctx, size_t bytes, void** out);

@@ -611,6 +611,18 @@ class Context:
         self._check(_LIB.stk_cache_get_params(self._h, _ptr(g), _ptr(m)), "stk_cache_get_params")
         return g, m
 
+    def cache_last_records(self, which: str = "train") -> np.ndarray:
+        """stk_cache_last_records: the last cached render's query ("query") or
+        training ("train") records."""
+        w = {"query": 0, "train": 1}[which]
+        n = C.c_uint32()
+        self._check(_LIB.stk_cache_last_records(self._h, w, None, 0, C.byref(n)), "stk_cache_last_records")
+        out = np.zeros(n.value, CACHE_RECORD)
+        if n.value:
+            self._check(_LIB.stk_cache_last_records(self._h, w, _ptr(out), out.size, C.byref(n)),
+                        "stk_cache_last_records")
+        return out[:n.value]
+
     def cache_set_params(self, grid: np.ndarray | None, mlp: np.ndarray | None):
         grid = None if grid is None else np.ascontiguousarray(grid, np.float32)
         mlp = None if mlp is None else np.ascontiguousarray(mlp, np.float32)

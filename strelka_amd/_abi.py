@@ -191,6 +191,7 @@ PROTOTYPES = [
     ("stk_cache_gradients", C.c_int, [P, P, U32, P, P, C.POINTER(C.c_float)]),
     ("stk_cache_get_params", C.c_int, [P, P, P]),
     ("stk_cache_set_params", C.c_int, [P, P, P]),
+    ("stk_cache_last_records", C.c_int, [P, U32, P, U32, C.POINTER(U32)]),
     ("stk_device_alloc", C.c_int, [P, C.c_size_t, C.POINTER(P)]),
     ("stk_device_free", C.c_int, [P, P]),
     ("stk_copy_to_device", C.c_int, [P, P, P, C.c_size_t]),

@@ -169,6 +169,7 @@ struct stk_ctx {
     DevBuf cGridH, cGridP, cGridM, cGridV, cGridG, cMlpP, cMlpM, cMlpV, cMlpG, cFwd, cBwd, cXT, cDyT, cLoss;
     DevBuf cGridE, cGridEW, cGridHE, cMlpE, cFwdE, cBwdE;
     DevBuf queryRecs, queryOut, trainVerts, trainCount, trainBatch;
+    uint32_t lastTrainCap = 0; // training records the last cached render could hold
 
     std::vector<DevBuf*> owned()
     {
@@ -951,6 +952,7 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
             if ((st = devReserve(c, tc, (size_t)pc.maxTrainPaths * sizeof(uint32_t))) != STK_OK) return st;
         }
         if ((st = devReserve(c, c->trainBatch, (size_t)pc.trainCap * sizeof(stk::CacheRecord))) != STK_OK) return st;
+        c->lastTrainCap = pc.trainCap;
         pc.queryRecs = static_cast<stk::CacheRecord*>((multi ? c->xQueryRecs[0] : c->queryRecs).ptr);
         pc.queryOut = static_cast<float4*>((multi ? c->xQueryOut[0] : c->queryOut).ptr);
         pc.trainVerts = static_cast<float4*>((multi ? c->xTrainVerts[0] : c->trainVerts).ptr);
@@ -1779,6 +1781,33 @@ STK_API stk_status stk_get_counters(stk_ctx* c, uint32_t out[8])
     if (!c->passState(c->lastPassStream).counters) return fail(c, STK_E_STATE, "stk_get_counters: nothing rendered");
     STK_HIP(c, hipMemcpyAsync(out, c->passState(c->lastPassStream).counters, 8 * sizeof(uint32_t),
                               hipMemcpyDeviceToHost, c->stream));
+    STK_HIP(c, hipStreamSynchronize(c->stream));
+    return STK_OK;
+}
+
+STK_API stk_status stk_cache_last_records(stk_ctx* c, uint32_t which, StkCacheRecord* out, uint32_t capacity,
+                                          uint32_t* n)
+{
+    if (!validCtx(c) || !n || which > 1) return STK_E_INVALID;
+    if (!c->haveCache) return fail(c, STK_E_STATE, "stk_cache_last_records: no cache");
+    const stk::PathState& ps = c->passState(c->lastPassStream);
+    const DevBuf& buf = which == 0 ? (c->lastStreams > 1 ? c->xQueryRecs[c->lastPassStream] : c->queryRecs)
+                                   : c->trainBatch;
+    if (!ps.counters || !buf.ptr) return fail(c, STK_E_STATE, "stk_cache_last_records: no cached render yet");
+    (void)hipSetDevice(c->device);
+    for (hipStream_t s : c->xStream)
+        if (s) STK_HIP(c, hipStreamSynchronize(s));
+    uint32_t cnt[8];
+    STK_HIP(c, hipMemcpyAsync(cnt, ps.counters, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    STK_HIP(c, hipStreamSynchronize(c->stream));
+    const size_t held = buf.bytes / sizeof(stk::CacheRecord);
+    uint32_t count = which == 0 ? cnt[3] : std::min(cnt[4], c->lastTrainCap);
+    count = (uint32_t)std::min<size_t>(count, held);
+    *n = count;
+    if (!out || count == 0) return STK_OK;
+    if (capacity < count) return fail(c, STK_E_INVALID, "stk_cache_last_records: capacity too small");
+    STK_HIP(c, hipMemcpyAsync(out, buf.ptr, (size_t)count * sizeof(stk::CacheRecord), hipMemcpyDeviceToHost,
+                              c->stream));
     STK_HIP(c, hipStreamSynchronize(c->stream));
     return STK_OK;
 }

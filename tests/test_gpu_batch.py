@@ -229,3 +229,46 @@ def test_batch_rejects_cache_and_bad_shapes(scenes):
     with pytest.raises(sa.StkError):
         ctx.batch_render(p)
     ctx.close()
+
+
+def test_config3_all_32_scenes_against_oracle_and_single_renders():
+    """Config 3 at its real workload: the 32 seeded scenes of 10k-2M triangles
+    (generate(3, v), v = 0..31), every view 960x540, 8 bounces, one batch.
+    A 16x16 crop of every view equals the oracle per sample, and four views
+    (the smallest, the largest and two between) equal their single-scene
+    renders bit for bit (samples and accumulated image)."""
+    print("case: config 3, 32 scenes at 960x540, 2 spp, 8 bounces; oracle crop per view, 4 single renders",
+          flush=True)
+    scenes = [sa.SceneData.generate(3, v) for v in range(32)]
+    tris = [int(s.inst["indexCount"].sum()) // 3 for s in scenes]
+    print(f"instanced triangles: min {min(tris)}, max {max(tris)}, total {sum(tris)}", flush=True)
+    assert min(tris) < 40_000 and max(tris) > 1_000_000
+    ctx = batch_context(scenes)
+    views = ctx.batch_views()
+    assert len(views) == 32 and all((v["width"], v["height"]) == (960, 540) for v in views)
+    p = params(scenes[0].camera, (0, 0, 0), spp=2, frame=7)
+    imgs = [x.copy() for x in ctx.batch_render(p)]
+    total = ctx.batch_pixels()
+    smp = ctx.samples(total * 2 * 3)
+    ctx.close()
+    x0, y0 = 472, 262
+    worst = 0.0
+    for i, s in enumerate(scenes):
+        v = views[i]
+        n = v["width"] * v["height"]
+        view = smp[v["pixel_base"] * 6:(v["pixel_base"] + n) * 6].reshape(v["height"], v["width"], 2, 3)
+        crop = view[y0:y0 + 16, x0:x0 + 16].reshape(-1, 3)
+        o = Oracle.from_scene(s, sa.HostBvh.build(s).nodes)
+        cpu = o.pt_frame(params(s.camera, s.sky, spp=2, frame=7), x0, y0, 16, 16).reshape(-1, 3)
+        err = np.abs(crop - cpu) / np.maximum(1.0, np.abs(cpu))
+        worst = max(worst, float(err.max()))
+        assert not np.any(err > 1e-4), (i, float(err.max()))
+    print(f"32 views x 512 samples: max error {worst:.3e}")
+    order = np.argsort(tris)
+    for i in (int(order[0]), int(order[10]), int(order[21]), int(order[-1])):
+        ref_img, ref_smp, _ = single_render(scenes[i], spp=2, frame=7)
+        v = views[i]
+        n = v["width"] * v["height"]
+        assert np.array_equal(smp[v["pixel_base"] * 6:(v["pixel_base"] + n) * 6], ref_smp), i
+        assert np.array_equal(imgs[i], ref_img), i
+        print(f"view {i} ({tris[i]} triangles) equals its single render")

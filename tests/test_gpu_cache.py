@@ -126,6 +126,74 @@ def rmse(a, b):
     return float(np.sqrt(np.mean((a - b) ** 2)))
 
 
+def test_config4_cache_on_the_1m_triangle_scene():
+    """Config 4 at its real workload: the 1.16M-triangle scene at 1920x1080,
+    a 16-level grid of 2^19 entries per level, 2^16 training paths per frame,
+    queries after 2 bounces, 32 frames of 1 spp.  On the records the renderer
+    itself produced, GPU inference and gradients equal the FP64 oracle's (FP16
+    rounding emulated), and training lowers the served predictions' squared
+    error against the one-sample targets, on the first frame's records and on
+    each frame's fresh ones (the relative L2 loss the optimiser minimises is
+    dominated by a few large one-sample targets and is not monotone)."""
+    print("case: config 4, 1080p, 2^19 grid, 65,536 training paths, query depth 2, 32 frames", flush=True)
+    s = sa.SceneData.generate(2)
+    ctx = sa.Context(0)
+    ctx.set_scene(s)
+    ctx.build_bvh()
+    ctx.cache_create(log2_table=19)
+    info = ctx.cache_info()
+    assert info.log2TableSize == 19 and info.gridParams == 16 * (1 << 19) * 2
+    w, h = s.camera.dimension[0], s.camera.dimension[1]
+
+    def mse(r):
+        return float(np.mean((ctx.cache_infer(r)[:, :3] - r["target"]) ** 2))
+
+    evals, first, fresh = None, [], []
+    for f in range(32):
+        p = sa.render_params(s.camera, spp=1, max_depth=8, frame=f, sampling=1, sky=s.sky, cache_mode=1,
+                             cache_query_depth=2, cache_train_paths=65536)
+        img = ctx.render(p)
+        recs = ctx.cache_last_records("train")
+        assert recs.size > 65536  # several vertices per training path
+        sub = recs[:: max(1, recs.size // 16384)][:16384].copy()
+        if f == 0:
+            evals = sub
+        if f < 4 or f >= 28:
+            first.append(mse(evals))
+            fresh.append(mse(sub))
+    print(f"squared error on frame 0's records, frames 1-4 / 29-32: {np.mean(first[:4]):.4f} / "
+          f"{np.mean(first[4:]):.4f}; on each frame's own records: {np.mean(fresh[:4]):.4f} / "
+          f"{np.mean(fresh[4:]):.4f}")
+    assert np.all(np.isfinite(img))
+    assert np.mean(first[4:]) < 0.7 * np.mean(first[:4]) and np.mean(fresh[4:]) < 0.7 * np.mean(fresh[:4])
+    cnt = ctx.counters()
+    queries = ctx.cache_last_records("query")
+    train = ctx.cache_last_records("train")
+    print(f"queries {queries.size} (counter {cnt[3]}), training records {train.size} (counter {cnt[4]})")
+    assert 0 < queries.size == cnt[3] <= w * h and train.size > 0
+    grid, mlp = ctx.cache_get_params()
+    ctx.cache_set_params(grid, mlp)  # serve the trained weights themselves (the moving average restarts)
+    o = CacheOracle(info, grid, mlp, round16=True)
+    q = queries[:: max(1, queries.size // 4096)][:4096].copy()
+    gpu = ctx.cache_infer(q)[:, :3]
+    ora = o.predict(q)
+    scale = np.abs(ora).max()
+    err = np.abs(gpu - ora).max() / scale
+    print(f"inference on renderer queries: max relative error {err:.2e} (scale {scale:.3f})")
+    assert scale > 0.01 and err < 5e-3
+    t = train[:: max(1, train.size // 2048)][:2048].copy()
+    g_grid, g_mlp, loss = ctx.cache_gradients(t)
+    og, dW, oloss = o.gradients(t)
+    om = o.flat_mlp_grad(dW)
+    rel_mlp = np.linalg.norm(g_mlp - om) / np.linalg.norm(om)
+    rel_grid = np.linalg.norm(g_grid - og.ravel()) / np.linalg.norm(og)
+    print(f"gradients on renderer training records: mlp {rel_mlp:.2e}, grid {rel_grid:.2e}, "
+          f"loss {loss:.4f} vs {oloss:.4f}")
+    assert loss == pytest.approx(oloss, rel=1e-2)
+    assert rel_mlp < 2e-2 and rel_grid < 2e-2
+    ctx.close()
+
+
 def test_trained_cache_beats_plain_path_tracing_at_equal_samples():
     print("case: config 7 closed room, cached vs plain path tracing, 8 frames of 1 spp", flush=True)
     s = sa.SceneData.generate(7)
