@@ -26,7 +26,7 @@ FLAGS = ["-O3", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-sh
 def _compile(out: Path, extra: list[str]) -> Path:
     OUT.mkdir(exist_ok=True)
     tmp = out.with_suffix(f".so.tmp{os.getpid()}")
-    cmd = ["gcc", *FLAGS, *extra, str(HERE / "oracle.c"), "-o", str(tmp), "-lm"]
+    cmd = ["gcc", *FLAGS, *extra, str(HERE / "oracle.c"), str(HERE / "oracle_fp64.c"), "-o", str(tmp), "-lm"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)

@@ -401,6 +401,35 @@ int orc_any_hit(const OrcScene* s, const StkRay* r, StkHit* hit, OrcStats* st)
     return found;
 }
 
+/* The shadow test of the corrected estimator: whether a triangle of a
+ * non-emitter instance is hit with t < tmax (the anyHit walk of
+ * shaders/raytracing.h:249-286 continued past emitter triangles). */
+int orc_occluded(const OrcScene* s, const StkRay* r)
+{
+    if (s->nnodes == 0) return 0;
+    float inv[3];
+    ray_invdir(r, inv);
+    uint32_t idx = 0;
+    while (idx != STK_INVALID_INDEX) {
+        const StkBvhNode* n = &s->nodes[idx];
+        if (is_leaf(n)) {
+            const uint32_t prim = as_uint(n->minBounds[0]);
+            float v0[3], e0[3], e1[3], t, b[2];
+            orc_get_triangle(s, (uint32_t)n->instId, prim, v0, e0, e1);
+            if (orc_ray_triangle(r->o, r->d, v0, e0, e1, &t, b) && t < r->o[3] && s->inst[n->instId].lightId == -1)
+                return 1;
+        } else {
+            float boxT;
+            if (orc_intersect_ray_box(r, inv, n->minBounds, n->maxBounds, &boxT)) {
+                idx = boxT > r->o[3] ? (uint32_t)n->nodeOffset : idx + 1;
+                continue;
+            }
+        }
+        idx = (uint32_t)n->nodeOffset;
+    }
+    return 0;
+}
+
 /* ---- threaded batch ------------------------------------------------------- */
 typedef struct {
     const OrcScene* s;
@@ -1151,6 +1180,13 @@ static f3 sphquad_sample(const SphQuad* q, float u, float v)
     return add(add(add(q->o, scl(q->x, xu)), scl(q->y, yv)), scl(q->z, q->z0));
 }
 
+/* calcLightNormal of a rect (shaders/lights.h:230-246): the direction it emits along. */
+static f3 rect_light_normal(const StkLight* l)
+{
+    const f3 p0 = ld(l->points[0]), p1 = ld(l->points[1]), p3 = ld(l->points[3]);
+    return neg(nrm(cross(sub(p1, p0), sub(p3, p0))));
+}
+
 void orc_sample_rect_light(const StkLight* l, const float u[2], const float hitPoint[3], float point[3], float L[3],
                            float* dist, float normal[3], float* area, float* pdf, float* solidAngle)
 {
@@ -1400,6 +1436,8 @@ void orc_pt_sample(const OrcScene* s, const StkRenderParams* rp, uint32_t px, ui
     const f3 sky = ld(rp->skyColor);
     int specularBounce = 0;
     const int maxDepth = (int)p->maxDepth;
+    const int ref = rp->refCompat != 0;
+    float prevPdf = 0.0f; /* solid-angle pdf of the BSDF sample that made `ray` (corrected estimator) */
     for (int depth = 0; depth < maxDepth; ++depth) {
         StkHit hit;
         if (!orc_closest_hit(s, &ray, &hit, NULL)) {
@@ -1408,11 +1446,25 @@ void orc_pt_sample(const OrcScene* s, const StkRenderParams* rp, uint32_t px, ui
         }
         const StkInstance* ic = &s->inst[hit.instId];
         if (ic->lightId != -1) {
-            /* emitter: counted on camera rays and after specular bounces
-             * (:229-234); a BSDF-sampled emitter hit after a diffuse or glossy
-             * bounce adds nothing and ends the path (the reference indexes
-             * material -1 there). */
-            if (depth == 0 || specularBounce) color = add(color, mul(thr, ld(s->lights[ic->lightId].color)));
+            const StkLight* l = &s->lights[ic->lightId];
+            if (ref) {
+                /* emitter: counted on camera rays and after specular bounces
+                 * (:229-234); a BSDF-sampled emitter hit after a diffuse or glossy
+                 * bounce adds nothing and ends the path (the reference indexes
+                 * material -1 there). */
+                if (depth == 0 || specularBounce) color = add(color, mul(thr, ld(l->color)));
+            } else if (dot(ld(ray.d), rect_light_normal(l)) < 0.0f) {
+                /* corrected: the front face emits; after a bounce the hit is
+                 * weighted against the previous vertex's light sample (balance
+                 * heuristic, both pdfs in solid angle) */
+                float w = 1.0f;
+                if (depth > 0) {
+                    const SphQuad q = sphquad_init(l, ld(ray.o));
+                    const float lpdf = q.S > 0.0f ? (1.0f / (float)s->nlights) / q.S : 0.0f;
+                    w = prevPdf / (prevPdf + lpdf);
+                }
+                color = add(color, scl(mul(thr, ld(l->color)), w));
+            }
             break;
         }
         const uint32_t i0 = s->ib[(uint32_t)ic->indexOffset + hit.primId * 3 + 0];
@@ -1462,8 +1514,8 @@ void orc_pt_sample(const OrcScene* s, const StkRenderParams* rp, uint32_t px, ui
             const StkLight* l = &s->lights[lid < s->nlights ? lid : s->nlights - 1];
             const float u0 = orc_rand(&rng);
             const float u1 = orc_rand(&rng);
-            float uu[2] = {u0, u1}, pt[3], L[3], dist, ln[3], area, pdf;
-            orc_sample_rect_light(l, uu, (const float*)&P, pt, L, &dist, ln, &area, &pdf, NULL);
+            float uu[2] = {u0, u1}, pt[3], L[3], dist, ln[3], area, pdf, solid;
+            orc_sample_rect_light(l, uu, (const float*)&P, pt, L, &dist, ln, &area, &pdf, &solid);
             toLight = ld(L);
             const f3 Li = ld(l->color);
             if (dot(N, toLight) > 0.0f && -dot(toLight, ld(ln)) > 0.0f && Li.x != 0.0f && Li.y != 0.0f &&
@@ -1479,10 +1531,19 @@ void orc_pt_sample(const OrcScene* s, const StkRenderParams* rp, uint32_t px, ui
                 sr.d[2] = toLight.z;
                 sr.d[3] = 0.0f;
                 StkHit sh;
-                float vis = orc_any_hit(s, &sr, &sh, NULL) ? 0.0f : 1.0f;
-                if (vis == 0.0f && s->inst[sh.instId].lightId != -1) vis = 1.0f; /* Q5 */
-                lightPdf = pdf;
-                radiance = scl(mul(scl(Li, vis), mk(1, 1, 1)), saturate(dot(N, toLight)));
+                float vis;
+                if (ref) {
+                    vis = orc_any_hit(s, &sr, &sh, NULL) ? 0.0f : 1.0f;
+                    if (vis == 0.0f && s->inst[sh.instId].lightId != -1) vis = 1.0f; /* Q5 */
+                    /* area-measure pdf against solid-angle samples (Q6); the cosine
+                     * the BSDF evaluation carries is applied again */
+                    lightPdf = pdf;
+                    radiance = scl(mul(scl(Li, vis), mk(1, 1, 1)), saturate(dot(N, toLight)));
+                } else {
+                    vis = orc_occluded(s, &sr) ? 0.0f : 1.0f;
+                    lightPdf = solid > 0.0f ? 1.0f / solid : 0.0f;
+                    radiance = scl(Li, vis);
+                }
             }
             lightPdf *= selPdf;
         }
@@ -1513,10 +1574,18 @@ void orc_pt_sample(const OrcScene* s, const StkRenderParams* rp, uint32_t px, ui
         const f3 nextO = offset_ray(P, Ng);
         thr = mul(thr, ld(bs.bsdf_over_pdf));
         if (depth > (int)rp->rrStartDepth) {
-            const float pr = fmaxf(thr.x, fmaxf(thr.y, thr.z));
-            if (orc_rand(&rng) > pr) break;
-            thr = scl(thr, 1.0f / (pr + 1e-5f));
+            if (ref) {
+                /* p = max(rgb), unclamped, and 1 / (p + 1e-5) (Q9) */
+                const float pr = fmaxf(thr.x, fmaxf(thr.y, thr.z));
+                if (orc_rand(&rng) > pr) break;
+                thr = scl(thr, 1.0f / (pr + 1e-5f));
+            } else {
+                const float pr = fminf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 1.0f);
+                if (orc_rand(&rng) > pr || !(pr > 0.0f)) break;
+                thr = scl(thr, 1.0f / pr);
+            }
         }
+        prevPdf = bs.pdf;
         ray.o[0] = nextO.x;
         ray.o[1] = nextO.y;
         ray.o[2] = nextO.z;

@@ -193,6 +193,19 @@ void orc_reduce(const float* samples, uint32_t npix, uint32_t spp, float* rgba);
 void orc_accumulate(const float* cur, const float* hist, uint32_t npix, uint32_t iteration, float* out);
 void orc_tonemap(const float* in, uint32_t npix, uint32_t type, float* out);
 
+/* The binary64 twin of orc_pt_sample (oracle_fp64.c): the same estimator
+ * (refCompat 1: the reference's; 0: the corrected one) and random numbers,
+ * every geometric and shading quantity in binary64. */
+void orc_pt_sample_fp64(const OrcScene* s, const StkRenderParams* rp, uint32_t px, uint32_t py, uint32_t sampleNum,
+                        double out[3]);
+/* Per-pixel sums and sums of squares (w x h x 3) of the pixel's rp->pt.spp
+ * binary64 samples, threaded over pixels. */
+void orc_pt_frame_fp64(const OrcScene* s, const StkRenderParams* rp, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                       double* sum, double* sumsq, int nthreads);
+/* Whether a triangle of a non-emitter instance is hit with t < tmax (the
+ * corrected estimator's shadow test). */
+int orc_occluded(const OrcScene* s, const StkRay* r);
+
 /* The transcendentals both sides share (op 0 sin, 1 cos, 2 acos, 3 x^5). */
 void orc_fmath(uint32_t op, const float* in, uint32_t n, float* out);
 

@@ -98,6 +98,10 @@ def _bind(lib: C.CDLL) -> C.CDLL:
         "orc_accumulate": (None, [FP, FP, U32, U32, FP]),
         "orc_tonemap": (None, [FP, U32, U32, FP]),
         "orc_fmath": (None, [U32, FP, U32, FP]),
+        "orc_pt_sample_fp64": (None, [P, P, U32, U32, U32, C.POINTER(C.c_double)]),
+        "orc_pt_frame_fp64": (None, [P, P, U32, U32, U32, U32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.c_int]),
+        "orc_occluded": (C.c_int, [P, P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -219,6 +223,19 @@ class Oracle:
         self.lib.orc_pt_frame(C.byref(self.scene), C.byref(params), x0, y0, w, h, fptr(out),
                               threads or default_threads())
         return out
+
+    def pt_frame_fp64(self, params, x0: int, y0: int, w: int, h: int, threads: int | None = None):
+        """Per-pixel mean and variance of the mean (h x w x 3, float64) of the
+        binary64 twin's params.pt.spp samples per pixel."""
+        n = params.pt.spp
+        s1 = np.zeros(w * h * 3, np.float64)
+        s2 = np.zeros(w * h * 3, np.float64)
+        dp = C.POINTER(C.c_double)
+        self.lib.orc_pt_frame_fp64(C.byref(self.scene), C.byref(params), x0, y0, w, h, s1.ctypes.data_as(dp),
+                                   s2.ctypes.data_as(dp), threads or default_threads())
+        mean = s1 / n
+        var = np.maximum(s2 / n - mean * mean, 0.0) * n / max(n - 1, 1)
+        return mean.reshape(h, w, 3), (var / n).reshape(h, w, 3)
 
     def pt_sample(self, params, px: int, py: int, s: int) -> np.ndarray:
         out = np.zeros(3, dtype=np.float32)

@@ -245,7 +245,12 @@ def render_params(camera: PathTracerParam, spp: int = 1, max_depth: int = 8, fra
                   sampling: int = 1, sky=(0.0, 0.0, 0.0), tonemap: int = 1, accumulate: bool = True,
                   keep_samples: bool = False, rr_start: int = 3, enable_tonemap: bool = True,
                   cache_mode: int = 0, cache_query_depth: int = 2, cache_train_paths: int = 65536,
-                  cache_min_roughness: float = 0.0, cache_train_steps: int = 1) -> RenderParams:
+                  cache_min_roughness: float = 0.0, cache_train_steps: int = 1,
+                  ref_compat: bool = True) -> RenderParams:
+    """StkRenderParams for a render call.  ref_compat: reproduce the
+    reference estimator exactly (True) or run the corrected one (False: the
+    light sample's solid-angle pdf, only non-emitters occlude, BSDF-sampled
+    light hits weighted by MIS, Russian roulette with p <= 1)."""
     p = RenderParams()
     C.memmove(C.byref(p.pt), C.byref(camera), C.sizeof(PathTracerParam))
     p.pt.spp = spp
@@ -253,7 +258,7 @@ def render_params(camera: PathTracerParam, spp: int = 1, max_depth: int = 8, fra
     p.pt.frameNumber = frame
     p.pt.stratifiedSamplingType = sampling
     p.rrStartDepth = rr_start
-    p.refCompat = 1
+    p.refCompat = 1 if ref_compat else 0
     p.tonemapperType = tonemap
     p.enableAccumulation = 1 if accumulate else 0
     p.enableTonemap = 1 if enable_tonemap else 0

@@ -253,6 +253,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
     bool cont = false, shadow = false, cacheQuery = false;
     uint32_t slot = 0;
     F3 nO = mk3(0, 0, 0), nD = mk3(0, 0, 0);
+    float nPdf = 0.0f; // solid-angle pdf of the sampled direction (corrected estimator's MIS)
     F3 sO = mk3(0, 0, 0), sD = mk3(0, 0, 0), cd = mk3(0, 0, 0), cg = mk3(0, 0, 0);
     float sT = 0.0f;
     CacheRecord qrec;
@@ -279,11 +280,30 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
         } else {
             const StkInstance& ic = sc.inst[inst];
             if (ic.lightId != -1) {
-                // emitter: counted on camera rays (shaders/pathtracerMdl.hlsl:229-234);
-                // a BSDF-sampled emitter hit adds nothing and ends the path
-                if (depth == 0 && (uint32_t)ic.lightId < desc.lightBase + desc.lightCount) {
-                    col = col + thr * ld3(sc.lights[ic.lightId].color);
-                    ps.col[slot] = f4(col);
+                const bool own = (uint32_t)ic.lightId < desc.lightBase + desc.lightCount;
+                if (lp.refCompat) {
+                    // emitter: counted on camera rays (shaders/pathtracerMdl.hlsl:229-234);
+                    // a BSDF-sampled emitter hit adds nothing and ends the path
+                    if (depth == 0 && own) {
+                        col = col + thr * ld3(sc.lights[ic.lightId].color);
+                        ps.col[slot] = f4(col);
+                    }
+                } else if (own) {
+                    // corrected: the front face emits; after a bounce the hit is
+                    // weighted against the light sample of the previous vertex
+                    // (balance heuristic, both pdfs in solid angle)
+                    const StkLight& l = sc.lights[ic.lightId];
+                    if (dot(f3(ps.rayD[slot]), rect_light_normal(l)) < 0.0f) {
+                        float w = 1.0f;
+                        if (depth > 0) {
+                            const float bpdf = ps.rayD[slot].w;
+                            const float S = sphquad_init(l, f3(ps.rayO[slot])).S;
+                            const float lpdf = S > 0.0f ? (1.0f / (float)desc.lightCount) / S : 0.0f;
+                            w = bpdf / (bpdf + lpdf);
+                        }
+                        col = col + (thr * ld3(l.color)) * w;
+                        ps.col[slot] = f4(col);
+                    }
                 }
             } else {
                 const uint32_t prim = ps.hitPrim[q];
@@ -370,9 +390,16 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
                     const F3 Li = ld3(l.color);
                     if (dot(N, toLight) > 0.0f && -dot(toLight, ls.normal) > 0.0f && Li.x != 0.0f && Li.y != 0.0f &&
                         Li.z != 0.0f) {
-                        lightPdf = ls.pdf;
                         dist = ls.dist;
-                        radiance = (Li * 1.0f) * saturate(dot(N, toLight)); // visibility applied by connect
+                        if (lp.refCompat) {
+                            // the area-measure pdf against solid-angle samples (Q6), and
+                            // the cosine the BSDF evaluation already carries applied again
+                            lightPdf = ls.pdf;
+                            radiance = (Li * 1.0f) * saturate(dot(N, toLight)); // visibility applied by connect
+                        } else {
+                            lightPdf = ls.solidAngle > 0.0f ? 1.0f / ls.solidAngle : 0.0f;
+                            radiance = Li;
+                        }
                     }
                     lightPdf *= selPdf;
                 }
@@ -405,16 +432,26 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
                         thr = thr * bs.weight;
                         bool survive = true;
                         if (depth > (int)lp.rrStartDepth) {
-                            const float pr = fmaxf(thr.x, fmaxf(thr.y, thr.z));
-                            if (rnd(rng) > pr)
-                                survive = false;
-                            else
-                                thr = thr * (1.0f / (pr + 1e-5f));
+                            if (lp.refCompat) {
+                                // p = max(rgb), unclamped, and 1 / (p + 1e-5) (Q9)
+                                const float pr = fmaxf(thr.x, fmaxf(thr.y, thr.z));
+                                if (rnd(rng) > pr)
+                                    survive = false;
+                                else
+                                    thr = thr * (1.0f / (pr + 1e-5f));
+                            } else {
+                                const float pr = fminf(fmaxf(thr.x, fmaxf(thr.y, thr.z)), 1.0f);
+                                if (rnd(rng) > pr || !(pr > 0.0f))
+                                    survive = false;
+                                else
+                                    thr = thr * (1.0f / pr);
+                            }
                         }
                         if (survive) {
                             cont = true;
                             nO = offset_ray(P, Ng);
                             nD = bs.k2;
+                            nPdf = lp.refCompat ? 0.0f : bs.pdf;
                         }
                     }
                 }
@@ -424,7 +461,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
         if (cont) {
             ps.thr[slot] = f4(thr);
             ps.rayO[slot] = make_float4(nO.x, nO.y, nO.z, 1e9f);
-            ps.rayD[slot] = make_float4(nD.x, nD.y, nD.z, 0.0f);
+            ps.rayD[slot] = make_float4(nD.x, nD.y, nD.z, nPdf);
         }
     }
     const uint32_t stripe = blockIdx.x % kStripes;
@@ -459,19 +496,26 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, 
 constexpr uint32_t kMaxEmitters = 64;
 
 // One shadow-ray entry s of the connect step.
+// refCompat: the reference's rule (the first hit leaf in preorder decides,
+// and a light hit counts as visible, Q5); otherwise any non-emitter triangle
+// before tmax occludes (the emitter walks with no emitter triangles given).
 template <bool TL>
-__device__ __forceinline__ void connect_one(const SceneView& sc, PathState& ps, uint32_t s, uint32_t* stack, int binary)
+__device__ __forceinline__ void connect_one(const SceneView& sc, PathState& ps, uint32_t s, uint32_t* stack, int binary,
+                                            int refCompat)
 {
     const uint32_t slot = ps.shadowQ[s];
     const SceneDesc& desc = sc.scenes[ps.scene[slot]];
     const RayF r = make_ray(ps.shO[s], ps.shD[s]);
+    const uint32_t emitCount = refCompat ? desc.emitCount : 0u;
     bool vis;
     if (TL) {
-        vis = visible_two_level(sc.bvh, sc.bvh.emitters + 3 * (size_t)desc.emitBase, desc.emitCount, r,
+        vis = visible_two_level(sc.bvh, sc.bvh.emitters + 3 * (size_t)desc.emitBase, emitCount, r,
                                 stack + threadIdx.x, stack + kWideStack * kBlock + threadIdx.x, kBlock);
     } else if (!binary && desc.wideRoot != STK_INVALID_INDEX && desc.emitCount <= kMaxEmitters) {
-        vis = visible_wide(sc.bvh, desc.wideRoot, sc.bvh.emitters + 3 * (size_t)desc.emitBase, desc.emitCount, r,
+        vis = visible_wide(sc.bvh, desc.wideRoot, sc.bvh.emitters + 3 * (size_t)desc.emitBase, emitCount, r,
                            stack + threadIdx.x, stack + kWideStack * kBlock + threadIdx.x, kBlock);
+    } else if (!refCompat) {
+        vis = !occluded_ref(sc.bvh, desc.hdr, desc.refRoot, r);
     } else {
         HitF h;
         vis = !any_ref(sc.bvh, desc.hdr, desc.refRoot, r, h);
@@ -490,13 +534,14 @@ __device__ __forceinline__ void connect_one(const SceneView& sc, PathState& ps, 
 // the scene has no usable wide tree or many emitter triangles; TL: the
 // two-level visibility walk.
 template <bool TL>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pt_connect(SceneView sc, PathState ps, int binary)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pt_connect(SceneView sc, PathState ps, int binary,
+                                                                                         int refCompat)
 {
     __shared__ uint32_t stack[2 * kWideStack * kBlock];
     const uint32_t stripe = blockIdx.x % kStripes, cnt = ps.stripes[2 * kStripes + stripe];
     for (uint32_t base = stripe_first(kBlock); base < cnt; base += stripe_step(kBlock)) {
         const uint32_t local = base + threadIdx.x;
-        if (local < cnt) connect_one<TL>(sc, ps, stripe * ps.segCap + local, stack, binary);
+        if (local < cnt) connect_one<TL>(sc, ps, stripe * ps.segCap + local, stack, binary, refCompat);
     }
 }
 
@@ -645,11 +690,12 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
         default: launchExtend<256>(sc, ps, depth, lc, s); break;
         }
         hipLaunchKernelGGL(k_pt_shade, stripeGrid(ps, kBlock, lc.stripeThreads), b, 0, s, sc, lp, ps, depth);
+        const int rc = lp.refCompat ? 1 : 0;
         if (sc.bvh.tlas)
-            hipLaunchKernelGGL(k_pt_connect<true>, stripeGrid(ps, kBlock, lc.stripeThreads), b, 0, s, sc, ps, 0);
+            hipLaunchKernelGGL(k_pt_connect<true>, stripeGrid(ps, kBlock, lc.stripeThreads), b, 0, s, sc, ps, 0, rc);
         else
             hipLaunchKernelGGL(k_pt_connect<false>, stripeGrid(ps, kBlock, lc.stripeThreads), b, 0, s, sc, ps,
-                               lc.binaryWalk ? 1 : 0);
+                               lc.binaryWalk ? 1 : 0, rc);
     }
     hipLaunchKernelGGL(k_pt_tally, dim3(1), dim3(64), 0, s, ps);
     if (cacheAfter && cb && (query || train)) {

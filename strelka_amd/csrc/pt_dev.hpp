@@ -156,50 +156,85 @@ struct LightSample {
     F3 L;
     float dist;
     F3 normal;
-    float pdf;
+    float pdf;        // the reference's area-measure pdf d^2 / (cos_l A) (shaders/lights.h:298)
+    float solidAngle; // S: the samples are uniform over this solid angle, pdf 1 / S
 };
 
-__device__ __forceinline__ LightSample sample_rect_light(const StkLight& l, float u, float v, F3 o)
+// The spherical rectangle of a rect light seen from o (SphQuad init,
+// shaders/lights.h:102-166).
+struct SphQuadF {
+    F3 X, Y, Z;
+    float z0, z0sq, x0, y0, y0sq, x1, y1, y1sq, b0, b1, b0sq, k, S;
+};
+
+__device__ __forceinline__ SphQuadF sphquad_init(const StkLight& l, F3 o)
 {
+    SphQuadF q;
     const F3 p0 = ld3(l.points[0]), p1 = ld3(l.points[1]), p3 = ld3(l.points[3]);
     const F3 ex = p1 - p0, ey = p3 - p0;
     const float exl = len3(ex), eyl = len3(ey);
-    const F3 X = mk3(ex.x / exl, ex.y / exl, ex.z / exl);
-    const F3 Y = mk3(ey.x / eyl, ey.y / eyl, ey.z / eyl);
-    F3 Z = cross(X, Y);
+    q.X = mk3(ex.x / exl, ex.y / exl, ex.z / exl);
+    q.Y = mk3(ey.x / eyl, ey.y / eyl, ey.z / eyl);
+    q.Z = cross(q.X, q.Y);
     const F3 d = p0 - o;
-    float z0 = dot(d, Z);
-    if (z0 > 0) {
-        Z = Z * -1.0f;
-        z0 *= -1.0f;
+    q.z0 = dot(d, q.Z);
+    if (q.z0 > 0) {
+        q.Z = q.Z * -1.0f;
+        q.z0 *= -1.0f;
     }
-    const float z0sq = z0 * z0;
-    const float x0 = dot(d, X), y0 = dot(d, Y);
-    const float x1 = x0 + exl, y1 = y0 + eyl;
-    const float y0sq = y0 * y0, y1sq = y1 * y1;
-    const F3 v00 = mk3(x0, y0, z0), v01 = mk3(x0, y1, z0), v10 = mk3(x1, y0, z0), v11 = mk3(x1, y1, z0);
+    q.z0sq = q.z0 * q.z0;
+    q.x0 = dot(d, q.X);
+    q.y0 = dot(d, q.Y);
+    q.x1 = q.x0 + exl;
+    q.y1 = q.y0 + eyl;
+    q.y0sq = q.y0 * q.y0;
+    q.y1sq = q.y1 * q.y1;
+    const F3 v00 = mk3(q.x0, q.y0, q.z0), v01 = mk3(q.x0, q.y1, q.z0), v10 = mk3(q.x1, q.y0, q.z0),
+             v11 = mk3(q.x1, q.y1, q.z0);
     const F3 n0 = nrm(cross(v00, v10)), n1 = nrm(cross(v10, v11));
     const F3 n2 = nrm(cross(v11, v01)), n3 = nrm(cross(v01, v00));
     const float g0 = stk_acosf(-dot(n0, n1)), g1 = stk_acosf(-dot(n1, n2));
     const float g2 = stk_acosf(-dot(n2, n3)), g3 = stk_acosf(-dot(n3, n0));
-    const float b0 = n0.z, b1 = n2.z, b0sq = b0 * b0;
-    const float k = 2.0f * kPi - g2 - g3;
-    const float S = g0 + g1 - k;
-    // SphQuadSample (shaders/lights.h:168-194)
-    const float au = u * S + k;
-    const float fu = (stk_cosf(au) * b0 - b1) / stk_sinf(au);
-    float cu = 1.0f / sqrtf(fu * fu + b0sq) * (fu > 0 ? 1.0f : -1.0f);
+    q.b0 = n0.z;
+    q.b1 = n2.z;
+    q.b0sq = q.b0 * q.b0;
+    q.k = 2.0f * kPi - g2 - g3;
+    q.S = g0 + g1 - q.k;
+    return q;
+}
+
+// SphQuadSample (shaders/lights.h:168-194): the point of the rect at (u, v).
+__device__ __forceinline__ F3 sphquad_sample(const SphQuadF& q, F3 o, float u, float v)
+{
+    const float au = u * q.S + q.k;
+    const float fu = (stk_cosf(au) * q.b0 - q.b1) / stk_sinf(au);
+    float cu = 1.0f / sqrtf(fu * fu + q.b0sq) * (fu > 0 ? 1.0f : -1.0f);
     cu = fminf(fmaxf(cu, -1.0f), 1.0f);
-    float xu = -(cu * z0) / sqrtf(1.0f - cu * cu);
-    xu = fminf(fmaxf(xu, x0), x1);
-    const float dd = sqrtf(xu * xu + z0sq);
-    const float h0 = y0 / sqrtf(dd * dd + y0sq);
-    const float h1 = y1 / sqrtf(dd * dd + y1sq);
+    float xu = -(cu * q.z0) / sqrtf(1.0f - cu * cu);
+    xu = fminf(fmaxf(xu, q.x0), q.x1);
+    const float dd = sqrtf(xu * xu + q.z0sq);
+    const float h0 = q.y0 / sqrtf(dd * dd + q.y0sq);
+    const float h1 = q.y1 / sqrtf(dd * dd + q.y1sq);
     const float hv = h0 + v * (h1 - h0);
     const float hv2 = hv * hv;
-    const float yv = (hv < 1.0f - 1e-5f) ? (hv * dd) / sqrtf(1.0f - hv2) : y1;
-    const F3 pt = ((o + X * xu) + Y * yv) + Z * z0;
+    const float yv = (hv < 1.0f - 1e-5f) ? (hv * dd) / sqrtf(1.0f - hv2) : q.y1;
+    return ((o + q.X * xu) + q.Y * yv) + q.Z * q.z0;
+}
+
+// calcLightNormal of a rect (shaders/lights.h:230-246): it emits along this.
+__device__ __forceinline__ F3 rect_light_normal(const StkLight& l)
+{
+    const F3 p0 = ld3(l.points[0]), p1 = ld3(l.points[1]), p3 = ld3(l.points[3]);
+    return -nrm(cross(p1 - p0, p3 - p0));
+}
+
+__device__ __forceinline__ LightSample sample_rect_light(const StkLight& l, float u, float v, F3 o)
+{
+    const SphQuadF q = sphquad_init(l, o);
+    const F3 pt = sphquad_sample(q, o, u, v);
     // fillLightData (:274-282)
+    const F3 p0 = ld3(l.points[0]), p1 = ld3(l.points[1]), p3 = ld3(l.points[3]);
+    const F3 ex = p1 - p0, ey = p3 - p0;
     const float A = len3(cross(ex, ey));
     const F3 nl = -nrm(cross(ex, ey));
     const F3 toL = pt - o;
@@ -209,6 +244,7 @@ __device__ __forceinline__ LightSample sample_rect_light(const StkLight& l, floa
     ls.dist = lenL;
     ls.normal = nl;
     ls.pdf = lenL * lenL / (-dot(ls.L, nl) * A); // area-measure pdf of the reference (:298)
+    ls.solidAngle = q.S;
     return ls;
 }
 

@@ -302,6 +302,38 @@ __device__ __forceinline__ bool any_ref(const BvhView& bvh, const StkBvhHeader& 
     return false;
 }
 
+// Whether a triangle of a non-emitter instance is hit with t < tmax: the
+// skip-link walk of anyHit that does not stop at emitter triangles (the
+// shadow test of the corrected estimator, where only non-emitters occlude).
+__device__ __forceinline__ bool occluded_ref(const BvhView& bvh, const StkBvhHeader& hd, uint32_t root,
+                                             const RayF& r)
+{
+    if (hd.root == INT32_MIN) return false;
+    uint32_t idx = root;
+    while (idx != STK_INVALID_INDEX) {
+        const float4* np = reinterpret_cast<const float4*>(bvh.refNodes + idx);
+        const float4 lo = np[0], hi = np[1];
+        const uint32_t inst = float_to_bits(lo.w);
+        const uint32_t skip = float_to_bits(hi.w);
+        if (inst != STK_INVALID_INDEX) {
+            const uint32_t tri = float_to_bits(hi.x);
+            const float4* p = bvh.tris + 3 * (size_t)tri;
+            const float4 a = p[0], b = p[1], c = p[2];
+            float t, u, v;
+            if (tri_test(r, a, b, c, t, u, v) && t < r.tmax && bvh.instances[inst].lightId == -1) return true;
+            idx = skip;
+            continue;
+        }
+        float bt;
+        if (slab(r, lo.x, hi.x, lo.y, hi.y, lo.z, hi.z, bt)) {
+            idx = (bt > r.tmax) ? skip : idx + 1;
+            continue;
+        }
+        idx = skip;
+    }
+    return false;
+}
+
 // anyHit with the reference's result - the first leaf in DFS preorder whose
 // triangle is hit with t < tmax - over the packed nodes: children are visited
 // left first and a leaf right child waits on the stack while its inner left

@@ -93,10 +93,22 @@ def nee_lambert_expectation(P, N, corners, radiance, albedo, light_normal):
     return np.sum(np.where(ok, c * cos_l / r2, 0.0) * ws, axis=-1) / S
 
 
-def floor_expectations(s, pixels, sub=2):
+def lambert_reflected(P, N, corners, radiance, albedo):
+    """Radiance a Lambert surface at each point P reflects from a rect emitter
+    of constant radiance: albedo / pi * L * integral of cos over the rect's
+    solid angle (Lambert's polygon formula), what an unbiased estimator
+    converges to."""
+    P = np.asarray(P, np.float64).reshape(-1, 3)
+    proj = np.array([projected_solid_angle(p, N, corners) for p in P])
+    return albedo / np.pi * radiance * proj
+
+
+def floor_expectations(s, pixels, sub=2, corrected=False):
     """Closed-form pixel means of the lit-floor scene: the footprint of each
     pixel (sub x sub Gauss points) mapped to the floor by the camera's
-    matrices, then the next-event expectation at each point."""
+    matrices, then at each point the expectation of the reference's estimator
+    (next-event estimate with its quirks) or, corrected, the reflected
+    radiance itself."""
     cam = s.camera
     W, H = cam.dimension[0], cam.dimension[1]
     Cm = np.array(cam.clipToView, np.float64).reshape(4, 4)  # column-major: Cm[col][row]
@@ -118,5 +130,8 @@ def floor_expectations(s, pixels, sub=2):
             v = ndc @ Cm  # clipToView * ndc (Cm holds columns)
             d = v[:, :3] @ Vm[:3, :3]  # viewToWorld * (v, 0)
             P = eye + d * (-eye[1] / d[:, 1])[:, None]
-            acc += wx * wy * nee_lambert_expectation(P, N, corners, radiance, albedo, lightN)
+            if corrected:
+                acc += wx * wy * lambert_reflected(P, N, corners, radiance, albedo)
+            else:
+                acc += wx * wy * nee_lambert_expectation(P, N, corners, radiance, albedo, lightN)
     return acc

@@ -33,15 +33,19 @@ def test_convex_furnace_samples_are_exact_on_gpu():
     assert absorbed.mean() < 1e-3
 
 
-def test_lit_floor_converges_to_fp64_quadrature_on_gpu():
+@pytest.mark.parametrize("ref_compat", [True, False])
+def test_lit_floor_converges_to_fp64_quadrature_on_gpu(ref_compat):
     """64 frames of 64 spp of every pixel of the lit floor: each pixel's
     t statistic (mean minus closed form over the standard error of the 64
     frame means, 63 degrees of freedom) must stay below the two-sided
     quantile for a family-wise error of 1e-3 over the 4,096 pixels, and the
-    mean relative deviation over the image (a bias) below 0.1%."""
+    mean relative deviation over the image (a bias) below 0.1%.  The closed
+    form is the reference estimator's expectation (ref_compat) or, for the
+    corrected estimator, the radiance the floor reflects (Lambert's polygon
+    formula for the light)."""
     from scipy.stats import t as student_t
 
-    print("case: lit floor (config 9), 64x64, 64 x 64 spp vs FP64 quadrature", flush=True)
+    print(f"case: lit floor (config 9), 64x64, 64 x 64 spp vs FP64 quadrature, ref_compat={ref_compat}", flush=True)
     s = sa.SceneData.generate(9)
     ctx = sa.Context(0)
     ctx.set_scene(s)
@@ -50,14 +54,14 @@ def test_lit_floor_converges_to_fp64_quadrature_on_gpu():
     frames = []
     for f in range(64):
         p = sa.render_params(s.camera, spp=64, max_depth=8, frame=f, sampling=1, sky=(0, 0, 0),
-                             accumulate=False, enable_tonemap=False)
+                             accumulate=False, enable_tonemap=False, ref_compat=ref_compat)
         frames.append(ctx.render(p)[..., 0].astype(np.float64).copy())
     ctx.close()
     frames = np.array(frames)
     mean = frames.mean(axis=0)
     se = frames.std(axis=0, ddof=1) / np.sqrt(len(frames))
     pix = [(x, y) for y in range(h) for x in range(w)]
-    expect = floor_expectations(s, pix).reshape(h, w)
+    expect = floor_expectations(s, pix, corrected=not ref_compat).reshape(h, w)
     z = (mean - expect) / np.maximum(se, 1e-12)
     rel = (mean - expect) / expect
     print(f"max |z| {np.abs(z).max():.2f}, mean relative deviation {rel.mean():.2e}, "

@@ -56,13 +56,43 @@ def test_per_sample_radiance_small_scene(ctx, sampling):
     assert np.mean(cpu) > 0.01  # the scene is lit
 
 
-def test_per_sample_radiance_config2_crop(ctx):
-    print("case: config 2 scene, 1080p, 1 spp render, 16x16 crop x 4 spp checked", flush=True)
+def test_per_sample_radiance_corrected_estimator(ctx):
+    """refCompat = 0 (the light sample's solid-angle pdf, only non-emitters
+    occlude, BSDF-sampled light hits weighted by MIS, Russian roulette with
+    p <= 1) against the oracle's corrected estimator, every sample."""
+    print("case: config 6 and config 7 scenes, 8 bounces, 4 spp, corrected estimator", flush=True)
+    for config in (6, 7):
+        s = sa.SceneData.generate(config)
+        ctx.set_scene(s)
+        ctx.build_bvh()
+        o = Oracle.from_scene(s, ctx.get_bvh())
+        p = sa.render_params(s.camera, spp=4, max_depth=8, frame=5, sampling=1, sky=s.sky, keep_samples=True,
+                             ref_compat=False)
+        w, h = p.pt.dimension[0], p.pt.dimension[1]
+        ctx.reset_accumulation()
+        ctx.render(p)
+        gpu = ctx.samples(w * h * 4 * 3)
+        cpu = o.pt_frame(p, 0, 0, w, h)
+        frac, err = compare_samples(gpu, cpu)
+        assert frac <= MAX_DIVERGENT
+        # the corrected estimator differs from the reference's on these scenes
+        p.refCompat = 1
+        assert not np.array_equal(o.pt_frame(p, 0, 0, min(w, 16), min(h, 16)),
+                                  o.pt_frame(sa.render_params(s.camera, spp=4, max_depth=8, frame=5, sampling=1,
+                                                              sky=s.sky, ref_compat=False), 0, 0, min(w, 16),
+                                             min(h, 16)))
+
+
+@pytest.mark.parametrize("ref_compat", [True, False])
+def test_per_sample_radiance_config2_crop(ctx, ref_compat):
+    print(f"case: config 2 scene, 1080p, 4 spp render, 16x16 crop x 4 spp checked, ref_compat={ref_compat}",
+          flush=True)
     s = sa.SceneData.generate(2)
     ctx.set_scene(s)
     ctx.build_bvh()
     o = Oracle.from_scene(s, ctx.get_bvh())
-    p = sa.render_params(s.camera, spp=4, max_depth=8, frame=0, sampling=1, sky=s.sky, keep_samples=True)
+    p = sa.render_params(s.camera, spp=4, max_depth=8, frame=0, sampling=1, sky=s.sky, keep_samples=True,
+                         ref_compat=ref_compat)
     w, h = p.pt.dimension[0], p.pt.dimension[1]
     ctx.reset_accumulation()
     ctx.render(p)

@@ -252,3 +252,32 @@ def test_lit_floor_converges_to_quadrature():
         print(f"pixel {px},{py}: oracle {mean:.6f} +- {se:.6f}, closed form {e:.6f}")
         assert abs(mean - e) < 5 * se + 1e-5 * e
         assert se < 0.01 * e  # the comparison is informative
+
+
+def test_corrected_estimator_converges_to_the_reflected_radiance():
+    """With refCompat = 0 the estimator is unbiased: on the lit floor each
+    pixel's mean is the radiance the Lambert floor reflects from the rect
+    light, albedo / pi * L * (Lambert's polygon formula for the light seen
+    from the floor), for the binary32 oracle and for its binary64 twin, within
+    5 standard errors.  The reference's estimator (refCompat = 1) does not
+    converge there (its light pdf is in area measure and its next-event term
+    carries cos^2): it lands at least 20 standard errors off."""
+    s, o = scene_oracle(9)
+    spp = 2048
+    pixels = [(32, 32), (10, 50), (60, 3), (45, 20)]
+    truth = floor_expectations(s, pixels, corrected=True)
+    quirky = floor_expectations(s, pixels)
+    fixed = sa.render_params(s.camera, spp=spp, max_depth=8, frame=0, sampling=1, sky=(0, 0, 0), ref_compat=False)
+    ref = sa.render_params(s.camera, spp=spp, max_depth=8, frame=0, sampling=1, sky=(0, 0, 0))
+    for (px, py), e, q in zip(pixels, truth, quirky):
+        smp = o.pt_frame(fixed, px, py, 1, 1).reshape(-1, 3)[:, 0].astype(np.float64)
+        mean, se = smp.mean(), smp.std() / np.sqrt(spp)
+        m64, v64 = o.pt_frame_fp64(fixed, px, py, 1, 1)
+        m64, se64 = float(m64[0, 0, 0]), float(np.sqrt(v64[0, 0, 0]))
+        rsm = o.pt_frame(ref, px, py, 1, 1).reshape(-1, 3)[:, 0].astype(np.float64)
+        print(f"pixel {px},{py}: corrected {mean:.6f} +- {se:.6f} (fp64 {m64:.6f} +- {se64:.6f}), "
+              f"reflected radiance {e:.6f}; reference estimator {rsm.mean():.6f} (its expectation {q:.6f})")
+        assert abs(mean - e) < 5 * se + 1e-5 * e
+        assert abs(m64 - e) < 5 * se64 + 1e-5 * e
+        assert se < 0.01 * e
+        assert abs(rsm.mean() - e) > 20 * rsm.std() / np.sqrt(spp)
