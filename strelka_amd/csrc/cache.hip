@@ -9,6 +9,8 @@
 // (8 B per lane per MFMA).
 #include <hip/hip_fp16.h>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "cache.hpp"
 
 namespace stk {
@@ -146,6 +148,68 @@ __global__ __launch_bounds__(256) void k_cache_infer(GridDesc gd, const __half* 
     }
 }
 
+// Training records are walked in the order of a 21-bit Morton code of their
+// normalised position (7 bits per axis), so the 16 samples of a wave lie close
+// together and share most cells of the coarse grid levels.
+constexpr int kMortonBits = 7;
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v)
+{
+    uint32_t x = 0;
+    for (int b = 0; b < kMortonBits; ++b) x |= ((v >> b) & 1u) << (3 * b);
+    return x;
+}
+
+// key[i]: Morton code of record i, or 1 << 21 past the record count (sorted
+// last); val[i] = i.
+__global__ __launch_bounds__(256) void k_cache_keys(GridDesc gd, const CacheRecord* __restrict__ recs, uint32_t n,
+                                                    const uint32_t* __restrict__ nDev, uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t N = nDev ? (*nDev < n ? *nDev : n) : n;
+    uint32_t key = 1u << (3 * kMortonBits);
+    if (i < N) {
+        float px, py, pz;
+        normalised(gd, recs[i].posRough, px, py, pz);
+        const float m = (float)((1 << kMortonBits) - 1);
+        key = spread3((uint32_t)(px * m)) | (spread3((uint32_t)(py * m)) << 1) | (spread3((uint32_t)(pz * m)) << 2);
+    }
+    keys[i] = key;
+    vals[i] = i;
+}
+
+// Grid levels of at most this many cells per axis merge their gradient
+// contributions within the wave before the atomics.
+constexpr uint32_t kMergeRes = 160;
+
+// Adds (v0, v1) to tab[2e], tab[2e + 1] for every active lane, after summing
+// the values of the wave's active lanes that hold the same key (a key names
+// one entry of one table; lanes of one key hold the same tab and e): one
+// atomic per distinct key.  Every lane of the wave must call it.
+__device__ __forceinline__ void wave_add_by_key(float* tab, uint32_t key, uint32_t e, bool active, float v0, float v1)
+{
+    const uint32_t lane = __lane_id();
+    uint64_t todo = __ballot(active);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t lk = (uint32_t)__shfl((int)key, leader);
+        const bool mine = ((todo >> lane) & 1ull) && key == lk;
+        const uint64_t m = __ballot(mine);
+        float s0 = mine ? v0 : 0.0f, s1 = mine ? v1 : 0.0f;
+        for (int off = 32; off > 0; off >>= 1) {
+            s0 += __shfl_xor(s0, off);
+            s1 += __shfl_xor(s1, off);
+        }
+        if ((int)lane == leader) {
+            atomicAdd(tab + 2 * e, s0);
+            atomicAdd(tab + 2 * e + 1, s1);
+        }
+        todo &= ~m;
+    }
+}
+
 // Fused encode + forward + relative-L2 loss + backward for 16 samples per
 // wave.  Saves every layer's input (xT) and output gradient (dyT), transposed
 // to [row][sample], for the weight-gradient GEMM, and scatters the gradient of
@@ -153,19 +217,21 @@ __global__ __launch_bounds__(256) void k_cache_infer(GridDesc gd, const __half* 
 __global__ __launch_bounds__(256) void k_cache_train(GridDesc gd, const __half* __restrict__ grid,
                                                      const __half* __restrict__ wFwd,
                                                      const __half* __restrict__ wBwd,
-                                                     const CacheRecord* __restrict__ recs, uint32_t n,
+                                                     const CacheRecord* __restrict__ recs,
+                                                     const uint32_t* __restrict__ order, uint32_t n,
                                                      const uint32_t* __restrict__ nDev, uint32_t nCap,
                                                      _Float16* __restrict__ xT, _Float16* __restrict__ dyT,
-                                                     float* __restrict__ gridG, float* __restrict__ lossSum)
+                                                     float* __restrict__ gridG, float* __restrict__ gridGDense,
+                                                     float* __restrict__ lossSum)
 {
     const uint32_t N = nDev ? (*nDev < n ? *nDev : n) : n;
     const int lane = threadIdx.x % kWave, r = lane % 16, g = lane / 16;
     const uint32_t grp = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (grp * 16 >= N) return;
-    const uint32_t idx = grp * 16 + r;
+    const uint32_t idx = grp * 16 + r; // column of the saved activations (sorted order)
     const bool valid = idx < N;
     CacheRecord rec;
-    if (valid) rec = recs[idx];
+    if (valid) rec = recs[order[idx]];
     h4 xs[5][4];
     input_frags(gd, grid, rec, g, valid, xs[0]);
     const f4 o = forward(wFwd, lane, xs);
@@ -206,23 +272,59 @@ __global__ __launch_bounds__(256) void k_cache_train(GridDesc gd, const __half* 
         }
         for (int t = 0; t < 4; ++t) dcur[t] = dnext[t];
     }
-    // gradient of the grid features: rows 0..31 of W0 * dY0
-    if (!valid) return;
-    float px, py, pz;
-    normalised(gd, rec.posRough, px, py, pz);
+    // gradient of the grid features: rows 0..31 of W0 * dY0 (every lane takes
+    // part: the coarse levels merge across the wave)
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    if (valid) normalised(gd, rec.posRough, px, py, pz);
     for (int t = 0; t < 2; ++t) {
         f4 a = f4{0, 0, 0, 0};
         for (int s = 0; s < 4; ++s) a = mfma(load_frag(wBwd, t * 4 + s, lane), dcur[s], a);
         for (int k = 0; k < 2; ++k) {
             const int lvl = 8 * t + 2 * g + k;
             const float d0 = a[2 * k], d1 = a[2 * k + 1];
-            if (d0 == 0.0f && d1 == 0.0f) continue;
-            float* tab = gridG + ((size_t)lvl << gd.log2Table) * 2;
-            for_corners(gd, lvl, px, py, pz, [&](uint32_t e, float w) {
-                atomicAdd(tab + 2 * e, w * d0);
-                atomicAdd(tab + 2 * e + 1, w * d1);
-            });
+            const bool act = valid && !(d0 == 0.0f && d1 == 0.0f);
+            const bool merge = gd.res[lvl] <= kMergeRes;
+            float* tab = gd.dense[lvl]
+                             ? gridGDense + ((size_t)(blockIdx.x % kDenseCopies) * gd.denseEntries + gd.denseBase[lvl]) * 2
+                             : gridG + ((size_t)lvl << gd.log2Table) * 2;
+            if (__ballot(merge) != 0) {
+                for_corners(gd, lvl, px, py, pz, [&](uint32_t e, float w) {
+                    wave_add_by_key(tab, e * 4u + (uint32_t)g, e, act && merge, w * d0, w * d1);
+                    if (act && !merge) {
+                        atomicAdd(tab + 2 * e, w * d0);
+                        atomicAdd(tab + 2 * e + 1, w * d1);
+                    }
+                });
+            } else if (act) {
+                for_corners(gd, lvl, px, py, pz, [&](uint32_t e, float w) {
+                    atomicAdd(tab + 2 * e, w * d0);
+                    atomicAdd(tab + 2 * e + 1, w * d1);
+                });
+            }
         }
+    }
+}
+
+// Folds the dense levels' gradient copies into gridG and clears them: one
+// thread per (dense entry, feature), the only writer of its gridG element.
+__global__ __launch_bounds__(256) void k_dense_fold(GridDesc gd, float* __restrict__ gridGDense,
+                                                    float* __restrict__ gridG)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; // dense entry * 2 + feature
+    if (i >= gd.denseEntries * 2) return;
+    const uint32_t e = i / 2;
+    int lvl = 0;
+    for (int l = 0; l < kCacheLevels; ++l)
+        if (gd.dense[l] && gd.denseBase[l] <= e) lvl = l;
+    float sum = 0.0f;
+    for (int k = 0; k < kDenseCopies; ++k) {
+        float* c = gridGDense + (size_t)k * gd.denseEntries * 2 + i;
+        sum += *c;
+        *c = 0.0f;
+    }
+    if (sum != 0.0f) {
+        float* g = gridG + ((size_t)lvl << gd.log2Table) * 2 + (size_t)(e - gd.denseBase[lvl]) * 2 + (i & 1u);
+        *g += sum;
     }
 }
 
@@ -376,10 +478,20 @@ hipError_t cacheBackward(const CacheBuffers& cb, const CacheRecord* recs, uint32
     const uint32_t nMax = n < cb.nCap ? n : cb.nCap;
     if (nMax == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(64), 0, s, cb.loss, (size_t)1);
+    // the records' spatial order
+    hipLaunchKernelGGL(k_cache_keys, dim3(blocks256(nMax)), dim3(256), 0, s, cb.grid, recs, nMax, nDev, cb.sortKeys,
+                       cb.sortVals);
+    size_t bytes = cb.sortBytes;
+    hipError_t e = rocprim::radix_sort_pairs(cb.sortTemp, bytes, cb.sortKeys, cb.sortKeys + cb.nCap, cb.sortVals,
+                                             cb.sortVals + cb.nCap, nMax, 0, 3 * kMortonBits + 1, s);
+    if (e != hipSuccess) return e;
     const uint32_t groups = (nMax + 15) / 16;
     hipLaunchKernelGGL(k_cache_train, dim3((groups + 3) / 4), dim3(256), 0, s, cb.grid, cb.gridH, cb.wFwd, cb.wBwd,
-                       recs, nMax, nDev, cb.nCap, reinterpret_cast<_Float16*>(cb.xT),
-                       reinterpret_cast<_Float16*>(cb.dyT), cb.gridG, cb.loss);
+                       recs, cb.sortVals + cb.nCap, nMax, nDev, cb.nCap, reinterpret_cast<_Float16*>(cb.xT),
+                       reinterpret_cast<_Float16*>(cb.dyT), cb.gridG, cb.gridGDense, cb.loss);
+    if (cb.grid.denseEntries)
+        hipLaunchKernelGGL(k_dense_fold, dim3(blocks256((size_t)cb.grid.denseEntries * 2)), dim3(256), 0, s, cb.grid,
+                           cb.gridGDense, cb.gridG);
     const uint32_t chunks = (nMax + 1023) / 1024;
     hipLaunchKernelGGL(k_cache_dw, dim3(68, chunks), dim3(64), 0, s, reinterpret_cast<const _Float16*>(cb.xT),
                        reinterpret_cast<const _Float16*>(cb.dyT), nMax, nDev, cb.nCap, cb.mlpG);
@@ -431,6 +543,13 @@ hipError_t cachePack(CacheBuffers& cb, hipStream_t s)
                            reinterpret_cast<_Float16*>(cb.wFwdE), reinterpret_cast<_Float16*>(cb.wBwdE), 1.0f);
     }
     return hipGetLastError();
+}
+
+hipError_t cacheSortBytes(uint32_t nCap, size_t& bytes)
+{
+    bytes = 0;
+    return rocprim::radix_sort_pairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                     (uint32_t*)nullptr, nCap, 0, 3 * kMortonBits + 1);
 }
 
 hipError_t cacheZeroGrads(const CacheBuffers& cb, hipStream_t s)

@@ -19,11 +19,21 @@ constexpr int kCacheOut = 16;      // output layer width (3 used: RGB)
 // and one 64x16 output matrix, no biases (feature 39 is a constant 1).
 constexpr int kCacheMlpParams = kCacheHidden * kCacheWidth * kCacheWidth + kCacheWidth * kCacheOut;
 
+// Copies of the dense levels' gradient table: training block b adds into copy
+// b % kDenseCopies.  Blocks are dispatched to the 8 XCDs in turn, so each copy
+// is written by one XCD and stays in its L2 (the dense levels of a 2^19-entry
+// grid take 2.6 MB); the copies are folded into gridG after the scatter.
+// Every sample touches the same few thousand coarse entries, and one shared
+// copy serialised those atomics.
+constexpr int kDenseCopies = 8;
+
 struct GridDesc {
     float lo[3];
     float scale;                 // normalised position = (p - lo) * scale, in [0,1]
     uint32_t res[kCacheLevels];  // cells per unit of normalised coordinate
     uint32_t dense[kCacheLevels];
+    uint32_t denseBase[kCacheLevels]; // first entry of a dense level in the dense gradient copies
+    uint32_t denseEntries;       // entries of all dense levels
     uint32_t tableMask;          // entries per level - 1 (power of two)
     uint32_t log2Table;
 };
@@ -44,6 +54,7 @@ struct CacheBuffers {
     float* gridM = nullptr;    // Adam first moment
     float* gridV = nullptr;    // Adam second moment
     float* gridG = nullptr;    // gradient accumulator
+    float* gridGDense = nullptr; // [kDenseCopies][denseEntries][feature] dense-level gradients (zero between steps)
     size_t gridParams = 0;
     // MLP: fp32 master weights W[l][in][out]
     float* mlpP = nullptr;
@@ -68,6 +79,10 @@ struct CacheBuffers {
     __half* xT = nullptr;      // saved layer inputs  [5][64][nCap]
     __half* dyT = nullptr;     // saved layer output gradients [4][64][nCap] + [16][nCap]
     uint32_t nCap = 0;         // capacity in samples (multiple of 1024)
+    uint32_t* sortKeys = nullptr; // [2][nCap] spatial sort of the training records: keys
+    uint32_t* sortVals = nullptr; // [2][nCap] and record indices (the second half: sorted)
+    void* sortTemp = nullptr;
+    size_t sortBytes = 0;
     float* loss = nullptr;     // [0] loss sum of the last step
     uint32_t step = 0;
     uint32_t emaSteps = 0;     // updates since the moving average restarted
@@ -94,5 +109,7 @@ hipError_t cacheAdam(CacheBuffers& cb, float gradScale, const uint32_t* nDev, ui
 // Rebuilds the FP16 grid copy and the fragments from the FP32 masters.
 hipError_t cachePack(CacheBuffers& cb, hipStream_t s);
 hipError_t cacheZeroGrads(const CacheBuffers& cb, hipStream_t s);
+// Temporary bytes of the training records' sort for nCap records.
+hipError_t cacheSortBytes(uint32_t nCap, size_t& bytes);
 
 } // namespace stk

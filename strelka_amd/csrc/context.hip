@@ -167,7 +167,7 @@ struct stk_ctx {
     bool haveCache = false;
     StkCacheConfig cacheCfg{};
     DevBuf cGridH, cGridP, cGridM, cGridV, cGridG, cMlpP, cMlpM, cMlpV, cMlpG, cFwd, cBwd, cXT, cDyT, cLoss;
-    DevBuf cGridE, cGridEW, cGridHE, cMlpE, cFwdE, cBwdE;
+    DevBuf cGridE, cGridEW, cGridHE, cMlpE, cFwdE, cBwdE, cGridGD, cSortKeys, cSortVals, cSortTemp;
     DevBuf queryRecs, queryOut, trainVerts, trainCount, trainBatch;
     uint32_t lastTrainCap = 0; // training records the last cached render could hold
 
@@ -178,7 +178,7 @@ struct stk_ctx {
                 &accum, &accum2, &history, &out, &samples, &descs, &staging, &fVb, &fIb, &fInst, &fLights, &fMats, &fPacked,
                 &fTris, &fRefNodes, &fWide, &fWideTris, &fEmit, &cGridH, &cGridP, &cGridM, &cGridV, &cGridG, &cMlpP,
                 &cMlpM, &cMlpV, &cMlpG, &cFwd, &cBwd, &cXT, &cDyT, &cLoss, &cGridE, &cGridEW, &cGridHE, &cMlpE,
-                &cFwdE, &cBwdE, &queryRecs, &queryOut, &trainVerts, &trainCount, &trainBatch,
+                &cFwdE, &cBwdE, &cGridGD, &cSortKeys, &cSortVals, &cSortTemp, &queryRecs, &queryOut, &trainVerts, &trainCount, &trainBatch,
                 &xQueryRecs[0], &xQueryRecs[1], &xQueryRecs[2], &xQueryRecs[3], &xQueryOut[0], &xQueryOut[1],
                 &xQueryOut[2], &xQueryOut[3], &xTrainVerts[0], &xTrainVerts[1], &xTrainVerts[2], &xTrainVerts[3],
                 &xTrainCount[0], &xTrainCount[1], &xTrainCount[2], &xTrainCount[3]};
@@ -1553,7 +1553,8 @@ STK_API stk_status stk_cache_destroy(stk_ctx* c)
     if (!validCtx(c)) return STK_E_INVALID;
     DevBuf* bufs[] = {&c->cGridH, &c->cGridP, &c->cGridM, &c->cGridV, &c->cGridG, &c->cMlpP, &c->cMlpM,
                       &c->cMlpV, &c->cMlpG, &c->cFwd, &c->cBwd, &c->cXT, &c->cDyT, &c->cLoss,
-                      &c->cGridE, &c->cGridEW, &c->cGridHE, &c->cMlpE, &c->cFwdE, &c->cBwdE};
+                      &c->cGridE, &c->cGridEW, &c->cGridHE, &c->cMlpE, &c->cFwdE, &c->cBwdE, &c->cGridGD,
+                      &c->cSortKeys, &c->cSortVals, &c->cSortTemp};
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : bufs) devRelease(c, *b);
@@ -1597,6 +1598,12 @@ STK_API stk_status stk_cache_create(stk_ctx* c, const StkCacheConfig* cfgIn)
         const uint64_t s = (uint64_t)res + 1;
         gd.dense[l] = (s * s * s <= T) ? 1u : 0u;
     }
+    gd.denseEntries = 0;
+    for (int l = 0; l < stk::kCacheLevels; ++l) {
+        const uint64_t s = (uint64_t)gd.res[l] + 1;
+        gd.denseBase[l] = gd.dense[l] ? gd.denseEntries : 0xFFFFFFFFu;
+        if (gd.dense[l]) gd.denseEntries += (uint32_t)(s * s * s);
+    }
     cb.gridParams = (size_t)stk::kCacheLevels * T * stk::kCacheFeatures;
     cb.nCap = (cfg.maxBatch + 1023u) & ~1023u;
     cb.lr = cfg.learningRate;
@@ -1607,6 +1614,9 @@ STK_API stk_status stk_cache_create(stk_ctx* c, const StkCacheConfig* cfgIn)
     cb.eps = cfg.epsilon;
     cb.step = 0;
     const size_t gp = cb.gridParams, mp = stk::kCacheMlpParams;
+    size_t sortBytes = 0;
+    if (stk::cacheSortBytes(cb.nCap, sortBytes) != hipSuccess)
+        return fail(c, STK_E_DEVICE, "stk_cache_create: sort storage query failed");
     struct {
         DevBuf* b;
         size_t bytes;
@@ -1614,7 +1624,10 @@ STK_API stk_status stk_cache_create(stk_ctx* c, const StkCacheConfig* cfgIn)
                   {&c->cGridG, gp * 4},  {&c->cMlpP, mp * 4},  {&c->cMlpM, mp * 4},    {&c->cMlpV, mp * 4},
                   {&c->cMlpG, mp * 4},   {&c->cFwd, 68 * 256 * 2}, {&c->cBwd, 68 * 256 * 2},
                   {&c->cXT, (size_t)5 * 64 * cb.nCap * 2}, {&c->cDyT, (size_t)272 * cb.nCap * 2},
-                  {&c->cLoss, 64}};
+                  {&c->cLoss, 64},
+                  {&c->cGridGD, (size_t)stk::kDenseCopies * gd.denseEntries * 2 * 4 + 16},
+                  {&c->cSortKeys, (size_t)2 * cb.nCap * 4}, {&c->cSortVals, (size_t)2 * cb.nCap * 4},
+                  {&c->cSortTemp, sortBytes + 16}};
     (void)hipSetDevice(c->device);
     for (auto& a : allocs) {
         stk_status st = devReserve(c, *a.b, a.bytes);
@@ -1649,6 +1662,11 @@ STK_API stk_status stk_cache_create(stk_ctx* c, const StkCacheConfig* cfgIn)
     cb.gridM = static_cast<float*>(c->cGridM.ptr);
     cb.gridV = static_cast<float*>(c->cGridV.ptr);
     cb.gridG = static_cast<float*>(c->cGridG.ptr);
+    cb.gridGDense = static_cast<float*>(c->cGridGD.ptr);
+    cb.sortKeys = static_cast<uint32_t*>(c->cSortKeys.ptr);
+    cb.sortVals = static_cast<uint32_t*>(c->cSortVals.ptr);
+    cb.sortTemp = c->cSortTemp.ptr;
+    cb.sortBytes = sortBytes;
     cb.mlpP = static_cast<float*>(c->cMlpP.ptr);
     cb.mlpM = static_cast<float*>(c->cMlpM.ptr);
     cb.mlpV = static_cast<float*>(c->cMlpV.ptr);
@@ -1675,6 +1693,7 @@ STK_API stk_status stk_cache_create(stk_ctx* c, const StkCacheConfig* cfgIn)
     STK_HIP(c, hipMemsetAsync(cb.gridM, 0, gp * 4, c->stream));
     STK_HIP(c, hipMemsetAsync(cb.gridV, 0, gp * 4, c->stream));
     STK_HIP(c, hipMemsetAsync(cb.gridG, 0, gp * 4, c->stream));
+    STK_HIP(c, hipMemsetAsync(cb.gridGDense, 0, (size_t)stk::kDenseCopies * gd.denseEntries * 2 * 4, c->stream));
     STK_HIP(c, hipMemsetAsync(cb.mlpM, 0, mp * 4, c->stream));
     STK_HIP(c, hipMemsetAsync(cb.mlpV, 0, mp * 4, c->stream));
     STK_HIP(c, hipMemsetAsync(cb.mlpG, 0, mp * 4, c->stream));
