@@ -445,6 +445,31 @@ def cache_study(args) -> dict:
             "speedup": best["speedup"] if best else None, "plain": plain, "cached": results}
 
 
+# The cached estimator the default run times against plain path tracing: the
+# cache answers at the second path vertex (query depth 1) after 16 training-only
+# frames, one optimiser step per frame over 16,384 training paths, learning
+# rate 1e-2 halving every 128 steps down to 2e-4.
+TIME_TO_RMSE_VARIANT = {"query_depth": 1, "warmup": 16, "train_steps": 1, "train_paths": 16384,
+                        "lr_half_life": 128, "min_lr": 0.0002}
+
+
+def time_to_rmse_leg() -> dict:
+    """Bounded time-to-RMSE measurement for the default run: the 1M-triangle
+    scene at 960x540, 1 spp per frame, 8 bounces, up to 256 frames each of the
+    plain and the cached path tracer, ACES-tonemapped RMSE against a 2,048-spp
+    plain render (the same protocol as --cache-study, one cached variant)."""
+    ns = argparse.Namespace(study_scene=2, study_res=(960, 540), study_structure="flat", study_spp=1, study_depth=8,
+                            study_ref_spp=2048, study_frames=256, study_target=0.01, study_lr=1e-2,
+                            study_ema=0.99, study_variants=json.dumps([TIME_TO_RMSE_VARIANT]))
+    r = cache_study(ns)
+    cached = r["cached"][0]
+    return {"plain_s": r["time_to_target_s"]["plain"], "cached_s": cached["time_to_target_s"],
+            "speedup": cached["speedup"], "floor": cached["curve"][-1]["rmse"],
+            "plain_rmse_at_end": r["plain"][-1]["rmse"], "frames": ns.study_frames, "target": ns.study_target,
+            "resolution": list(ns.study_res), "reference_spp": ns.study_ref_spp, "variant": TIME_TO_RMSE_VARIANT,
+            "estimator": "reference (refCompat 1) for both"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -463,6 +488,8 @@ def main() -> int:
                          "batch: the 32-scene batched path tracer; instanced: the 19M-triangle instanced scene")
     ap.add_argument("--structures", default="two_level,flat,flat_device,flat_device_ploc",
                     help="instanced workload: the structures to measure (the first one gives value)")
+    ap.add_argument("--no-time-to-rmse", action="store_true",
+                    help="skip the time-to-RMSE leg (cached against plain path tracing) of the default run")
     ap.add_argument("--trace-only", action="store_true",
                     help="only the timed closest-hit launches (for counter profiles of that kernel)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -662,6 +689,8 @@ def main() -> int:
         p = sa.render_params(scene.camera, spp=4, max_depth=8, frame=1, sampling=1, sky=scene.sky)
         pt["cpu_baseline"] = cpu_pt_baseline(scene, nodes, p)
 
+    ttr = None if args.no_time_to_rmse else time_to_rmse_leg()
+
     out = {
         "metric": METRIC,
         "value": round(mrays, 2),
@@ -684,7 +713,8 @@ def main() -> int:
                   "reference_order_gpu_mrays_per_s": round(n / (ref_kernel_ms * 1e-3) / 1e6, 2),
                   "binary_bvh_gpu_mrays_per_s": round(n / (bin_kernel_ms * 1e-3) / 1e6, 2),
                   "device_built_tree": device_build,
-                  "path_tracer": pt},
+                  "path_tracer": pt,
+                  "time_to_rmse": ttr},
     }
     print(json.dumps(out))
     ctx.close()
