@@ -5,7 +5,8 @@
 //
 //   reference                              facade
 //   PtRender::init / cleanup               stk::PtRender::init / cleanup
-//   PtRender::setScene(Scene*)             setScene(const SceneData&)   (arrays Scene::get* returns)
+//   PtRender::setScene(Scene*)             setScene(const Scene&)       (the container of scene.hpp)
+//                                          setScene(const SceneData&)   (arrays Scene::get* returns)
 //   PtRender::setActiveCameraIndex(i)      setActiveCameraIndex(i)
 //   PtRender::drawFrame(Image*)            drawFrame(Image&)            (RGBA32F or RGBA8, pitch, host or device)
 //   PtRender::reloadPt()                   reloadPt()                   (restarts accumulation)
@@ -22,6 +23,7 @@
 #include <string>
 #include <vector>
 
+#include "scene.hpp"
 #include "stk.h"
 
 namespace stk {
@@ -92,6 +94,22 @@ public:
         if ((st = check(stk_set_materials(mCtx, s.materials, s.numMaterials))) != STK_OK) return st;
         if ((st = check(stk_build_bvh(mCtx, nullptr))) != STK_OK) return st;
         mCameras = s.cameras;
+        mCamera = 0;
+        mFrame = 0;
+        return check(stk_reset_accumulation(mCtx));
+    }
+
+    // The scene container of scene.hpp: its buffers go to the device, the
+    // BVH is built over them, its cameras become the selectable cameras and
+    // its sky the miss radiance.
+    stk_status setScene(const Scene& s)
+    {
+        if (!mCtx) return STK_E_STATE;
+        if (!s.valid()) return STK_E_INVALID;
+        stk_status st;
+        if ((st = check(stk_set_scene(mCtx, s.handle(), nullptr))) != STK_OK) return st;
+        mCameras = s.getCameras();
+        s.getSky(mSky);
         mCamera = 0;
         mFrame = 0;
         return check(stk_reset_accumulation(mCtx));

@@ -110,6 +110,16 @@ STK_API stk_status stk_set_textures(stk_ctx* ctx, const StkTexture* textures, ui
 /* Builds the BVH on the host (binned SAH, one triangle per leaf, preorder plus
  * skip links) and uploads it.  opts may be NULL for defaults. */
 STK_API stk_status stk_build_bvh(stk_ctx* ctx, const StkBvhOptions* opts);
+/* The whole scene of a stk_scene container (stk_scene.h: generated, or built
+ * with stk_scene_create_mesh / _instance / _light / _add_material): uploads its
+ * geometry, instances, lights and materials and builds the flat BVH with opts
+ * (NULL = defaults) - what oka::PtRender does with the oka::Scene handed to
+ * setScene (include/render/ptrender.h:64, src/render/ptrender.cpp:656-667). */
+#ifndef STK_SCENE_HANDLE
+#define STK_SCENE_HANDLE
+typedef struct stk_scene stk_scene;
+#endif
+STK_API stk_status stk_set_scene(stk_ctx* ctx, const stk_scene* scene, const StkBvhOptions* opts);
 /* Builds the flat BVH on the device instead of the host (no reference
  * counterpart; the reference builds with Embree on the host,
  * src/render/bvh.cpp:107-167): a linear BVH (Morton-ordered centroids, binary

@@ -20,7 +20,7 @@ from ._abi import (BLAS_TRIANGLE, BVH_NODE, CACHE_RECORD, HIT, INSTANCE, INVALID
                    VERTEX, BatchScene, BatchViewInfo, BvhHeader, BvhInfo, BvhOptions, CacheConfig, CacheInfo, DeviceInfo, ImageDesc, LaunchConfig,
                    PathTracerParam, RenderParams, SceneArrays)
 
-__all__ = ["lib", "StkError", "SceneData", "HostBvh", "HostTwoLevel", "Context", "RenderParams", "PathTracerParam", "VERTEX",
+__all__ = ["lib", "StkError", "SceneData", "Scene", "look_at", "pack_normal", "pack_uv", "HostBvh", "HostTwoLevel", "Context", "RenderParams", "PathTracerParam", "VERTEX",
            "BVH_NODE", "INSTANCE", "LIGHT", "RAY", "HIT", "MATERIAL", "TRIANGLE", "PACKED_NODE", "INVALID",
            "render_params", "CACHE_RECORD", "MAT_LAMBERT", "MAT_GGX", "WIDE_NODE"]
 
@@ -92,6 +92,143 @@ class SceneData:
 
     def __del__(self):
         if self._handle:
+            _LIB.stk_scene_free(self._handle)
+            self._handle = None
+
+
+def look_at(eye, target, up, vfov_deg: float, width: int, height: int, znear: float = 0.01,
+            zfar: float = 100.0) -> PathTracerParam:
+    """stk_camera_look_at: oka::Camera's view and reverse-Z projection
+    matrices for a look-at camera (src/scene/camera.cpp:61-131)."""
+    f3 = C.c_float * 3
+    cam = PathTracerParam()
+    st = _LIB.stk_camera_look_at(f3(*eye), f3(*target), f3(*up), vfov_deg, width, height, znear, zfar,
+                                 C.byref(cam))
+    if st:
+        raise StkError(st, "stk_camera_look_at")
+    return cam
+
+
+def pack_normal(x: float, y: float, z: float) -> int:
+    """The vertex record's 10:10:10 normal packing (src/hdStrelka/RenderPass.cpp:50-57)."""
+    return int(_LIB.stk_pack_normal(x, y, z))
+
+
+def pack_uv(u: float, v: float) -> int:
+    """The vertex record's 16:16 UV packing (src/hdStrelka/RenderPass.cpp:59-64)."""
+    return int(_LIB.stk_pack_uv(u, v))
+
+
+def _mat16(m) -> np.ndarray:
+    """A 4x4 transform as the column-major float[16] the C ABI takes: a (4, 4)
+    array is read as a matrix (rows, columns); a flat 16-array is taken as
+    already column-major."""
+    a = np.asarray(m, dtype=np.float32)
+    if a.shape == (4, 4):
+        a = a.T
+    a = np.ascontiguousarray(a.reshape(16), dtype=np.float32)
+    return a
+
+
+class Scene:
+    """The scene container of include/stk/stk_scene.h, built through oka::Scene's
+    calls (include/scene/scene.h:231-389): meshes, instances, materials,
+    lights and cameras.  Every call raises StkError when the library refuses
+    it; ids count from 0 in creation order."""
+
+    def __init__(self):
+        self._handle = C.c_void_p()
+        st = _LIB.stk_scene_create(C.byref(self._handle))
+        if st:
+            raise StkError(st, "stk_scene_create")
+
+    def _call(self, fn, what, *args):
+        st = fn(self._handle, *args)
+        if st:
+            raise StkError(st, what)
+
+    def create_mesh(self, vb: np.ndarray, ib) -> int:
+        vb = np.ascontiguousarray(vb, dtype=VERTEX)
+        ib = np.ascontiguousarray(ib, dtype=np.uint32)
+        out = C.c_uint32()
+        self._call(_LIB.stk_scene_create_mesh, "stk_scene_create_mesh", _ptr(vb), vb.size, _ptr(ib), ib.size,
+                   C.byref(out))
+        return out.value
+
+    def create_instance(self, mesh_id: int, material_id: int, transform) -> int:
+        m = _mat16(transform)
+        out = C.c_uint32()
+        self._call(_LIB.stk_scene_create_instance, "stk_scene_create_instance", mesh_id, material_id, _ptr(m),
+                   C.byref(out))
+        return out.value
+
+    def update_instance_transform(self, inst_id: int, transform):
+        m = _mat16(transform)
+        self._call(_LIB.stk_scene_update_instance_transform, "stk_scene_update_instance_transform", inst_id, _ptr(m))
+
+    def add_material(self, mat) -> int:
+        rec = np.ascontiguousarray(np.asarray(mat, dtype=MATERIAL).reshape(1))
+        out = C.c_uint32()
+        self._call(_LIB.stk_scene_add_material, "stk_scene_add_material", _ptr(rec), C.byref(out))
+        return out.value
+
+    @staticmethod
+    def light_desc(type: int = 0, xform=None, color=(1.0, 1.0, 1.0), intensity: float = 1.0, width: float = 1.0,
+                   height: float = 1.0, radius: float = 1.0) -> _abi.LightDesc:
+        d = _abi.LightDesc()
+        d.type = type
+        m = _mat16(np.eye(4, dtype=np.float32) if xform is None else xform)
+        for i in range(16):
+            d.xform[i] = float(m[i])
+        for i in range(3):
+            d.color[i] = color[i]
+        d.intensity, d.width, d.height, d.radius = intensity, width, height, radius
+        return d
+
+    def create_light(self, desc: _abi.LightDesc) -> int:
+        out = C.c_uint32()
+        self._call(_LIB.stk_scene_create_light, "stk_scene_create_light", C.byref(desc), C.byref(out))
+        return out.value
+
+    def update_light(self, light_id: int, desc: _abi.LightDesc):
+        self._call(_LIB.stk_scene_update_light, "stk_scene_update_light", light_id, C.byref(desc))
+
+    def light_instance(self, light_id: int) -> int:
+        out = C.c_uint32()
+        self._call(_LIB.stk_scene_light_instance, "stk_scene_light_instance", light_id, C.byref(out))
+        return out.value
+
+    def add_camera(self, camera: PathTracerParam) -> int:
+        out = C.c_uint32()
+        self._call(_LIB.stk_scene_add_camera, "stk_scene_add_camera", C.byref(camera), C.byref(out))
+        return out.value
+
+    def update_camera(self, camera_id: int, camera: PathTracerParam):
+        self._call(_LIB.stk_scene_update_camera, "stk_scene_update_camera", camera_id, C.byref(camera))
+
+    def camera_count(self) -> int:
+        return int(_LIB.stk_scene_camera_count(self._handle))
+
+    def camera(self, camera_id: int) -> PathTracerParam:
+        cam = PathTracerParam()
+        self._call(_LIB.stk_scene_get_camera, "stk_scene_get_camera", camera_id, C.byref(cam))
+        return cam
+
+    def set_sky(self, r: float, g: float, b: float):
+        self._call(_LIB.stk_scene_set_sky, "stk_scene_set_sky", r, g, b)
+
+    def data(self) -> "SceneData":
+        """Copies of the container's arrays (stk_scene_get) as a SceneData."""
+        a = SceneArrays()
+        self._call(_LIB.stk_scene_get, "stk_scene_get", C.byref(a))
+        cam = PathTracerParam()
+        C.memmove(C.byref(cam), C.byref(a.camera), C.sizeof(PathTracerParam))
+        return SceneData(vb=_copy(a.vb, a.numVertices, VERTEX), ib=_copy(a.ib, a.numIndices, np.dtype("<u4")),
+                         inst=_copy(a.inst, a.numInstances, INSTANCE), lights=_copy(a.lights, a.numLights, LIGHT),
+                         mats=_copy(a.mats, a.numMaterials, MATERIAL), camera=cam, sky=tuple(a.sky))
+
+    def __del__(self):
+        if getattr(self, "_handle", None):
             _LIB.stk_scene_free(self._handle)
             self._handle = None
 
@@ -315,6 +452,12 @@ class Context:
         self._check(_LIB.stk_set_materials(self._h, _ptr(s.mats), s.mats.size), "stk_set_materials")
         if s.textures is not None:
             self.set_textures(s.textures, s.texels)
+
+    def load_scene(self, s, max_depth: int = 32, bins: int = 32):
+        """stk_set_scene: the whole container (a Scene, or a generated
+        SceneData) goes to the device and the flat BVH is built over it."""
+        opts = BvhOptions(max_depth, bins, 1.0, 1.0)
+        self._check(_LIB.stk_set_scene(self._h, s._handle, C.byref(opts)), "stk_set_scene")
 
     def set_textures(self, textures: np.ndarray, texels: np.ndarray):
         """stk_set_textures: TEXTURE records over RGBA32F texels (n x 4)."""

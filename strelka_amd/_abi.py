@@ -135,6 +135,12 @@ class SceneArrays(C.Structure):
                 ("sky", C.c_float * 4)]
 
 
+class LightDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("xform", C.c_float * 16), ("color", C.c_float * 3), ("intensity", C.c_float),
+                ("width", C.c_float), ("height", C.c_float), ("radius", C.c_float)]
+
+
+assert C.sizeof(LightDesc) == 96
 assert C.sizeof(PathTracerParam) == 336
 assert C.sizeof(RenderParams) == 336 + 64
 assert C.sizeof(CacheConfig) == 48
@@ -240,6 +246,24 @@ PROTOTYPES = [
     ("stk_scene_free", None, [P]),
     ("stk_scene_get", C.c_int, [P, C.POINTER(SceneArrays)]),
     ("stk_scene_jittered_rays", C.c_int, [P, C.c_uint64, P, U32]),
+    ("stk_scene_create", C.c_int, [C.POINTER(P)]),
+    ("stk_scene_create_mesh", C.c_int, [P, P, U32, P, U32, C.POINTER(U32)]),
+    ("stk_scene_create_instance", C.c_int, [P, U32, C.c_int32, P, C.POINTER(U32)]),
+    ("stk_scene_update_instance_transform", C.c_int, [P, U32, P]),
+    ("stk_scene_add_material", C.c_int, [P, P, C.POINTER(U32)]),
+    ("stk_scene_create_light", C.c_int, [P, C.POINTER(LightDesc), C.POINTER(U32)]),
+    ("stk_scene_update_light", C.c_int, [P, U32, C.POINTER(LightDesc)]),
+    ("stk_scene_light_instance", C.c_int, [P, U32, C.POINTER(U32)]),
+    ("stk_scene_add_camera", C.c_int, [P, C.POINTER(PathTracerParam), C.POINTER(U32)]),
+    ("stk_scene_update_camera", C.c_int, [P, U32, C.POINTER(PathTracerParam)]),
+    ("stk_scene_camera_count", U32, [P]),
+    ("stk_scene_get_camera", C.c_int, [P, U32, C.POINTER(PathTracerParam)]),
+    ("stk_scene_set_sky", C.c_int, [P, C.c_float, C.c_float, C.c_float]),
+    ("stk_camera_look_at", C.c_int, [C.c_float * 3, C.c_float * 3, C.c_float * 3, C.c_float, U32, U32, C.c_float,
+                                     C.c_float, C.POINTER(PathTracerParam)]),
+    ("stk_pack_normal", U32, [C.c_float, C.c_float, C.c_float]),
+    ("stk_pack_uv", U32, [C.c_float, C.c_float]),
+    ("stk_set_scene", C.c_int, [P, P, C.POINTER(BvhOptions)]),
 ]
 
 

@@ -17,6 +17,7 @@
 #include <cmath>
 
 #include "../../include/stk/stk.h"
+#include "../../include/stk/stk_scene.h"
 #include "bvh.hpp"
 #include "cache.hpp"
 #include "gpubuild.hpp"
@@ -497,6 +498,29 @@ STK_API stk_status stk_build_bvh(stk_ctx* c, const StkBvhOptions* opts)
     (void)hipSetDevice(c->device);
     invalidateGraph(c);
     return uploadBvh(c, std::move(b));
+}
+
+// The renderer's one-time scene read (src/render/ptrender.cpp:656-667 and
+// createInstanceBuffer, 560-601): the container's global buffers, instance
+// records, lights and materials go to the device and the flat BVH is built
+// over them.  The scene may be freed or changed afterwards; a later change
+// takes effect at the next stk_set_scene.
+STK_API stk_status stk_set_scene(stk_ctx* c, const stk_scene* s, const StkBvhOptions* opts)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    StkSceneArrays a;
+    stk_status st = stk_scene_get(s, &a);
+    if (st != STK_OK) return fail(c, st, "stk_set_scene: null scene or host allocation failed");
+    for (uint32_t i = 0; i < a.numInstances; ++i)
+        if (a.inst[i].materialId >= (int32_t)a.numMaterials)
+            return fail(c, STK_E_INVALID, "stk_set_scene: instance " + std::to_string(i) +
+                                              " names material " + std::to_string(a.inst[i].materialId) +
+                                              " of " + std::to_string(a.numMaterials));
+    if ((st = stk_set_geometry(c, a.vb, a.numVertices, a.ib, a.numIndices, a.inst, a.numInstances)) != STK_OK)
+        return st;
+    if ((st = stk_set_lights(c, a.lights, a.numLights)) != STK_OK) return st;
+    if ((st = stk_set_materials(c, a.mats, a.numMaterials)) != STK_OK) return st;
+    return stk_build_bvh(c, opts);
 }
 
 STK_API stk_status stk_build_two_level(stk_ctx* c, const StkBvhOptions* opts)

@@ -179,13 +179,13 @@ uint32_t Scene::createIcosphereMesh(int subdivisions, bool refCompat)
     return createMesh(vb, ib);
 }
 
-uint32_t Scene::createLight(const LightDesc& desc)
+// The GPU light record of a description and the transform of its emitting
+// mesh (Scene::updateLight, src/scene/scene.cpp:452-511; the mesh transform of
+// Scene::createLight, src/scene/scene.cpp:418-450).
+static StkLight lightRecord(const LightDesc& desc, Mat4& meshXform)
 {
     StkLight l{};
-    const uint32_t lightId = (uint32_t)lights_.size();
     float intensityPerPoint = desc.intensity;
-    Mat4 meshXform = desc.xform;
-    uint32_t meshId = 0;
     if (desc.type == 0) {
         const Mat4 local = desc.xform * Mat4::scale(desc.width, desc.height, 1.0f);
         const float corners[4][3] = {{0.5f, 0.5f, 0.0f}, {-0.5f, 0.5f, 0.0f}, {-0.5f, -0.5f, 0.0f}, {0.5f, -0.5f, 0.0f}};
@@ -198,7 +198,6 @@ uint32_t Scene::createLight(const LightDesc& desc)
         }
         l.type = 0;
         intensityPerPoint /= (desc.width * desc.height);
-        meshId = createRectLightMesh();
         meshXform = local;
     } else if (desc.type == 1) {
         const Mat4 local = desc.xform * Mat4::scale(desc.radius, desc.radius, desc.radius);
@@ -218,7 +217,6 @@ uint32_t Scene::createLight(const LightDesc& desc)
         l.normal[2] = nz.z;
         l.type = 1;
         intensityPerPoint /= (float)(M_PI * desc.radius * desc.radius);
-        meshId = createDiscLightMesh();
         meshXform = local;
     } else {
         l.points[0][0] = desc.radius;
@@ -229,14 +227,42 @@ uint32_t Scene::createLight(const LightDesc& desc)
         l.points[1][3] = 1.0f;
         l.type = 2;
         intensityPerPoint /= (float)(4.0 * M_PI * desc.radius * desc.radius);
-        meshId = createIcosphereMesh(3, false);
         meshXform = desc.xform * Mat4::scale(desc.radius, desc.radius, desc.radius);
     }
     for (int c = 0; c < 3; ++c) l.color[c] = desc.color[c] * intensityPerPoint;
     l.color[3] = 1.0f * intensityPerPoint;
+    return l;
+}
+
+uint32_t Scene::createLight(const LightDesc& desc)
+{
+    const uint32_t lightId = (uint32_t)lights_.size();
+    Mat4 meshXform = desc.xform;
+    const StkLight l = lightRecord(desc, meshXform);
+    const uint32_t meshId = desc.type == 0   ? createRectLightMesh()
+                            : desc.type == 1 ? createDiscLightMesh()
+                                             : createIcosphereMesh(3, false);
     lights_.push_back(l);
-    createInstance(meshId, -1, meshXform, (int32_t)lightId);
+    lightDescs_.push_back(desc);
+    lightInstance_.push_back(createInstance(meshId, -1, meshXform, (int32_t)lightId));
     return lightId;
+}
+
+bool Scene::updateLight(uint32_t lightId, const LightDesc& desc)
+{
+    if (lightId >= lights_.size() || desc.type != lightDescs_[lightId].type) return false;
+    Mat4 meshXform = desc.xform;
+    lights_[lightId] = lightRecord(desc, meshXform);
+    lightDescs_[lightId] = desc;
+    instances_[lightInstance_[lightId]].transform = meshXform;
+    return true;
+}
+
+bool Scene::updateInstanceTransform(uint32_t instId, const Mat4& transform)
+{
+    if (instId >= instances_.size()) return false;
+    instances_[instId].transform = transform;
+    return true;
 }
 
 std::vector<StkInstance> Scene::getInstanceConstants() const

@@ -62,6 +62,11 @@ public:
     uint32_t createInstance(uint32_t meshId, int32_t materialId, const Mat4& transform, int32_t lightId = -1);
     // Scene::createLight / updateLight (src/scene/scene.cpp:418-511).
     uint32_t createLight(const LightDesc& desc);
+    // Scene::updateLight (src/scene/scene.cpp:452-511): the light record and
+    // the transform of its mesh instance; the light type cannot change.
+    bool updateLight(uint32_t lightId, const LightDesc& desc);
+    // Scene::updateInstanceTransform (include/scene/scene.h:389).
+    bool updateInstanceTransform(uint32_t instId, const Mat4& transform);
     uint32_t addMaterial(const StkMaterial& m);
 
     // createRectLightMesh (src/scene/scene.cpp:117-140)
@@ -81,6 +86,7 @@ public:
     // createInstanceBuffer (src/render/ptrender.cpp:560-601).
     std::vector<StkInstance> getInstanceConstants() const;
     size_t numInstances() const { return instances_.size(); }
+    uint32_t lightInstance(uint32_t lightId) const { return lightInstance_[lightId]; }
 
 private:
     std::vector<StkVertex> vertices_;
@@ -88,6 +94,8 @@ private:
     std::vector<Mesh> meshes_;
     std::vector<Instance> instances_;
     std::vector<StkLight> lights_;
+    std::vector<LightDesc> lightDescs_;
+    std::vector<uint32_t> lightInstance_; // Scene::mLightIdToInstanceId
     std::vector<StkMaterial> materials_;
     int32_t rectLightMesh_ = -1;
 };

@@ -1,5 +1,6 @@
 // scenegen.cpp - seeded synthetic workloads (see include/stk/stk_scene.h).
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <new>
 #include <vector>
@@ -252,10 +253,22 @@ void addFloor(Scene& sc, float extent, int32_t material)
 
 struct stk_scene {
     stk::Scene scene;
-    std::vector<StkInstance> inst;
-    StkPathTracerParam camera{};
+    // instance records (transforms, their inverses, index ranges), rebuilt by
+    // stk_scene_get after the instances or lights change
+    mutable std::vector<StkInstance> inst;
+    mutable bool instDirty = false;
+    std::vector<StkPathTracerParam> cameras; // Scene::mCameras; camera 0 is the generator's
     float sky[4] = {0, 0, 0, 0};
-    uint32_t config = 0;
+    uint32_t config = 0;                     // 0 for a scene built through the calls below
+
+    const std::vector<StkInstance>& instances() const
+    {
+        if (instDirty) {
+            inst = scene.getInstanceConstants();
+            instDirty = false;
+        }
+        return inst;
+    }
 };
 
 using namespace stk;
@@ -269,6 +282,7 @@ extern "C" STK_API stk_status stk_scene_generate(uint32_t config, uint32_t varia
     s->config = config;
     Scene& sc = s->scene;
     const bool refIco = (flags & STK_SCENE_REF_ICOSPHERE) != 0;
+    StkPathTracerParam cam{};
     try {
         if (config == 1 || config == 7) {
             StkMaterial sphere{};
@@ -283,7 +297,7 @@ extern "C" STK_API stk_status stk_scene_generate(uint32_t config, uint32_t varia
             addInwardBox(sc, 2.0f, mw);
             const uint32_t res = config == 7 ? 128 : 256;
             if (config == 7) addQuadLight(sc, 1.9f, 1.0f, 20.0f); // closed room lit from the ceiling
-            cameraMatrices(mk3(0, 0, 1.9f), mk3(0, 0, 0), mk3(0, 1, 0), 60.0f, res, res, 0.01f, 100.0f, s->camera);
+            cameraMatrices(mk3(0, 0, 1.9f), mk3(0, 0, 0), mk3(0, 1, 0), 60.0f, res, res, 0.01f, 100.0f, cam);
         } else if (config == 2 || config == 3 || config == 6) {
             uint32_t hfRes = 512, spheres = 2000, width = 1920, height = 1080;
             float extent = 10.0f, lightY = 8.0f, lightSize = 2.0f, intensity = 100.0f;
@@ -321,11 +335,11 @@ extern "C" STK_API stk_status stk_scene_generate(uint32_t config, uint32_t varia
             addQuadLight(sc, lightY, lightSize, intensity);
             if (config == 6)
                 cameraMatrices(mk3(0, 3.0f, 6.0f), mk3(0, 0.3f, 0), mk3(0, 1, 0), 55.0f, width, height, 0.01f, 100.0f,
-                               s->camera);
+                               cam);
             else
                 // 60 degrees below the horizon; the whole frame falls on the heightfield
                 cameraMatrices(mk3(0, 17.0f, 9.5f), mk3(0, 0.0f, -0.3f), mk3(0, 1, 0), 24.0f, width, height, 0.01f,
-                               1000.0f, s->camera);
+                               1000.0f, cam);
             s->sky[0] = 0.5f;
             s->sky[1] = 0.6f;
             s->sky[2] = 0.8f;
@@ -375,7 +389,7 @@ extern "C" STK_API stk_status stk_scene_generate(uint32_t config, uint32_t varia
             d.intensity = 8.0f * d.width * d.height; // radiance 8
             sc.createLight(d);
             cameraMatrices(mk3(0.0f, 30.0f * k3, 190.0f * k3), mk3(0, 0, 0), mk3(0, 1, 0), 55.0f, variant ? 384 : 3840,
-                           variant ? 216 : 2160, 0.01f, 1000.0f, s->camera);
+                           variant ? 216 : 2160, 0.01f, 1000.0f, cam);
             s->sky[0] = 0.25f;
             s->sky[1] = 0.3f;
             s->sky[2] = 0.4f;
@@ -390,7 +404,7 @@ extern "C" STK_API stk_status stk_scene_generate(uint32_t config, uint32_t varia
             m.albedo[3] = 1.0f;
             addBox(sc, 1.0f, (int32_t)sc.addMaterial(m), false);
             cameraMatrices(mk3(3.0f, 2.4f, 3.6f), mk3(0, 0, 0), mk3(0, 1, 0), 40.0f, 96, 96, 0.01f, 100.0f,
-                           s->camera);
+                           cam);
             s->sky[0] = 0.5f;
             s->sky[1] = 0.7f;
             s->sky[2] = 1.0f;
@@ -403,10 +417,11 @@ extern "C" STK_API stk_status stk_scene_generate(uint32_t config, uint32_t varia
             m.albedo[3] = 1.0f;
             addFloor(sc, 20.0f, (int32_t)sc.addMaterial(m));
             addQuadLight(sc, 2.0f, 1.0f, 10.0f);
-            cameraMatrices(mk3(0, 1.0f, 0), mk3(0, 0, 0), mk3(0, 0, -1), 60.0f, 64, 64, 0.01f, 100.0f, s->camera);
+            cameraMatrices(mk3(0, 1.0f, 0), mk3(0, 0, 0), mk3(0, 0, -1), 60.0f, 64, 64, 0.01f, 100.0f, cam);
         } else {
             return STK_E_INVALID;
         }
+        s->cameras.push_back(cam);
         s->inst = sc.getInstanceConstants();
     } catch (const std::bad_alloc&) {
         return STK_E_OOM;
@@ -421,6 +436,11 @@ extern "C" STK_API stk_status stk_scene_get(const stk_scene* s, StkSceneArrays* 
 {
     if (!s || !out) return STK_E_INVALID;
     const Scene& sc = s->scene;
+    try {
+        s->instances();
+    } catch (const std::bad_alloc&) {
+        return STK_E_OOM;
+    }
     out->vb = sc.getVertices().data();
     out->numVertices = (uint32_t)sc.getVertices().size();
     out->ib = sc.getIndices().data();
@@ -433,15 +453,18 @@ extern "C" STK_API stk_status stk_scene_get(const stk_scene* s, StkSceneArrays* 
     out->numMaterials = (uint32_t)sc.getMaterials().size();
     out->numTriangles = out->numIndices / 3;
     out->pad = 0;
-    out->camera = s->camera;
+    if (s->cameras.empty())
+        std::memset(&out->camera, 0, sizeof(out->camera));
+    else
+        out->camera = s->cameras[0];
     for (int i = 0; i < 4; ++i) out->sky[i] = s->sky[i];
     return STK_OK;
 }
 
 extern "C" STK_API stk_status stk_scene_jittered_rays(const stk_scene* s, uint64_t seed, StkRay* rays, uint32_t n)
 {
-    if (!s || !rays) return STK_E_INVALID;
-    const StkPathTracerParam& p = s->camera;
+    if (!s || !rays || s->cameras.empty()) return STK_E_INVALID;
+    const StkPathTracerParam& p = s->cameras[0];
     const uint32_t W = (uint32_t)p.dimension[0], H = (uint32_t)p.dimension[1];
     if ((uint64_t)W * H != n) return STK_E_INVALID;
     Pcg32 rng(seed, 3);
@@ -468,3 +491,187 @@ extern "C" STK_API stk_status stk_scene_jittered_rays(const stk_scene* s, uint64
         }
     return STK_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Scene building through the C ABI: the calls oka::Scene offers its producers
+// (include/scene/scene.h:231-389, src/scene/scene.cpp), over the same container
+// the generators fill.  Each returns STK_E_INVALID for a null or out-of-range
+// argument and STK_E_OOM when the host allocation fails; ids count from 0 in
+// creation order.
+
+namespace {
+
+Mat4 loadMat4(const float* m)
+{
+    Mat4 r;
+    std::memcpy(r.m, m, sizeof(r.m));
+    return r;
+}
+
+bool finite16(const float* m)
+{
+    for (int i = 0; i < 16; ++i)
+        if (!std::isfinite(m[i])) return false;
+    return true;
+}
+
+bool toLightDesc(const StkLightDesc* d, LightDesc& out)
+{
+    if (!d || d->type < 0 || d->type > 2 || !finite16(d->xform)) return false;
+    if (d->type == 0 && !(d->width > 0.0f && d->height > 0.0f)) return false;
+    if (d->type != 0 && !(d->radius > 0.0f)) return false;
+    if (!std::isfinite(d->intensity)) return false;
+    out.type = d->type;
+    out.xform = loadMat4(d->xform);
+    for (int k = 0; k < 3; ++k) out.color[k] = d->color[k];
+    out.intensity = d->intensity;
+    out.width = d->width;
+    out.height = d->height;
+    out.radius = d->radius;
+    return true;
+}
+
+} // namespace
+
+extern "C" STK_API stk_status stk_scene_create(stk_scene** out)
+{
+    if (!out) return STK_E_INVALID;
+    *out = new (std::nothrow) stk_scene();
+    return *out ? STK_OK : STK_E_OOM;
+}
+
+extern "C" STK_API stk_status stk_scene_create_mesh(stk_scene* s, const StkVertex* vb, uint32_t numVertices,
+                                                    const uint32_t* ib, uint32_t numIndices, uint32_t* meshId)
+{
+    if (!s || !meshId || !numVertices || !vb || !numIndices || !ib || numIndices % 3) return STK_E_INVALID;
+    for (uint32_t i = 0; i < numIndices; ++i)
+        if (ib[i] >= numVertices) return STK_E_INVALID;
+    const uint64_t total = (uint64_t)s->scene.getVertices().size() + numVertices;
+    if (total > 0xFFFFFFFFull || (uint64_t)s->scene.getIndices().size() + numIndices > 0x7FFFFFFFull)
+        return STK_E_INVALID;
+    try {
+        *meshId = s->scene.createMesh(std::vector<StkVertex>(vb, vb + numVertices),
+                                      std::vector<uint32_t>(ib, ib + numIndices));
+    } catch (const std::bad_alloc&) {
+        return STK_E_OOM;
+    }
+    return STK_OK;
+}
+
+extern "C" STK_API stk_status stk_scene_create_instance(stk_scene* s, uint32_t meshId, int32_t materialId,
+                                                        const float transform[16], uint32_t* instId)
+{
+    if (!s || !instId || !transform || meshId >= s->scene.getMeshes().size() || materialId < -1 ||
+        !finite16(transform))
+        return STK_E_INVALID;
+    try {
+        *instId = s->scene.createInstance(meshId, materialId, loadMat4(transform));
+    } catch (const std::bad_alloc&) {
+        return STK_E_OOM;
+    }
+    s->instDirty = true;
+    return STK_OK;
+}
+
+extern "C" STK_API stk_status stk_scene_update_instance_transform(stk_scene* s, uint32_t instId,
+                                                                  const float transform[16])
+{
+    if (!s || !transform || !finite16(transform)) return STK_E_INVALID;
+    if (!s->scene.updateInstanceTransform(instId, loadMat4(transform))) return STK_E_INVALID;
+    s->instDirty = true;
+    return STK_OK;
+}
+
+extern "C" STK_API stk_status stk_scene_add_material(stk_scene* s, const StkMaterial* m, uint32_t* materialId)
+{
+    if (!s || !m || !materialId) return STK_E_INVALID;
+    try {
+        *materialId = s->scene.addMaterial(*m);
+    } catch (const std::bad_alloc&) {
+        return STK_E_OOM;
+    }
+    return STK_OK;
+}
+
+extern "C" STK_API stk_status stk_scene_create_light(stk_scene* s, const StkLightDesc* desc, uint32_t* lightId)
+{
+    LightDesc d;
+    if (!s || !lightId || !toLightDesc(desc, d)) return STK_E_INVALID;
+    try {
+        *lightId = s->scene.createLight(d);
+    } catch (const std::bad_alloc&) {
+        return STK_E_OOM;
+    }
+    s->instDirty = true;
+    return STK_OK;
+}
+
+extern "C" STK_API stk_status stk_scene_update_light(stk_scene* s, uint32_t lightId, const StkLightDesc* desc)
+{
+    LightDesc d;
+    if (!s || !toLightDesc(desc, d) || !s->scene.updateLight(lightId, d)) return STK_E_INVALID;
+    s->instDirty = true;
+    return STK_OK;
+}
+
+extern "C" STK_API stk_status stk_scene_light_instance(const stk_scene* s, uint32_t lightId, uint32_t* instId)
+{
+    if (!s || !instId || lightId >= s->scene.getLights().size()) return STK_E_INVALID;
+    *instId = s->scene.lightInstance(lightId);
+    return STK_OK;
+}
+
+extern "C" STK_API stk_status stk_scene_add_camera(stk_scene* s, const StkPathTracerParam* camera, uint32_t* cameraId)
+{
+    if (!s || !camera || !cameraId || camera->dimension[0] <= 0 || camera->dimension[1] <= 0) return STK_E_INVALID;
+    try {
+        s->cameras.push_back(*camera);
+    } catch (const std::bad_alloc&) {
+        return STK_E_OOM;
+    }
+    *cameraId = (uint32_t)s->cameras.size() - 1;
+    return STK_OK;
+}
+
+extern "C" STK_API stk_status stk_scene_update_camera(stk_scene* s, uint32_t cameraId,
+                                                      const StkPathTracerParam* camera)
+{
+    if (!s || !camera || cameraId >= s->cameras.size() || camera->dimension[0] <= 0 || camera->dimension[1] <= 0)
+        return STK_E_INVALID;
+    s->cameras[cameraId] = *camera;
+    return STK_OK;
+}
+
+extern "C" STK_API uint32_t stk_scene_camera_count(const stk_scene* s) { return s ? (uint32_t)s->cameras.size() : 0; }
+
+extern "C" STK_API stk_status stk_scene_get_camera(const stk_scene* s, uint32_t cameraId, StkPathTracerParam* out)
+{
+    if (!s || !out || cameraId >= s->cameras.size()) return STK_E_INVALID;
+    *out = s->cameras[cameraId];
+    return STK_OK;
+}
+
+extern "C" STK_API stk_status stk_scene_set_sky(stk_scene* s, float r, float g, float b)
+{
+    if (!s || !std::isfinite(r) || !std::isfinite(g) || !std::isfinite(b)) return STK_E_INVALID;
+    s->sky[0] = r;
+    s->sky[1] = g;
+    s->sky[2] = b;
+    s->sky[3] = 0.0f;
+    return STK_OK;
+}
+
+extern "C" STK_API stk_status stk_camera_look_at(const float eye[3], const float target[3], const float up[3],
+                                                 float vfovDeg, uint32_t width, uint32_t height, float znear,
+                                                 float zfar, StkPathTracerParam* out)
+{
+    if (!eye || !target || !up || !out || !width || !height || !(vfovDeg > 0.0f && vfovDeg < 180.0f) ||
+        !(znear > 0.0f && zfar > znear))
+        return STK_E_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    cameraMatrices(ld3(eye), ld3(target), ld3(up), vfovDeg, width, height, znear, zfar, *out);
+    return STK_OK;
+}
+
+extern "C" STK_API uint32_t stk_pack_normal(float x, float y, float z) { return packNormal(mk3(x, y, z)); }
+extern "C" STK_API uint32_t stk_pack_uv(float u, float v) { return packUV(u, v); }

@@ -1,14 +1,16 @@
-// sanitize_driver.cpp - drives the host half of the library (scene container,
-// workload generators, BVH builder and packers, two-level build, file
+// sanitize_driver.cpp - drives the host half of the library (scene container
+// and its building calls, workload generators, BVH builder and packers, two-level build, file
 // formats, host C-ABI entry points) and the CPU oracle, for a build under
 // -fsanitize=address,undefined (tests/test_sanitizers.py).  No device code:
 // the HIP sources are not part of this build.  Exits 0 when every check
 // passes; a sanitizer report aborts the process.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/stk/stk.h"
@@ -180,6 +182,91 @@ void images()
     check(stk_image_read_pfm(path.c_str(), back.data(), 3, &rw, &rh) != STK_OK, "short image buffer refused");
 }
 
+// The scene-building calls: a generated workload re-entered from its own
+// arrays gives the same arrays; updates and refused calls leave the container
+// consistent.
+void sceneApi()
+{
+    Scene gen(7);
+    const StkSceneArrays& g = gen.a;
+    stk_scene* s = nullptr;
+    check(stk_scene_create(&s) == STK_OK, "stk_scene_create");
+    if (!s) return;
+    uint32_t id = 0;
+    for (uint32_t i = 0; i < g.numMaterials; ++i)
+        check(stk_scene_add_material(s, &g.mats[i], &id) == STK_OK && id == i, "add material");
+    std::vector<std::pair<int32_t, uint32_t>> meshes; // (indexOffset, mesh id)
+    for (uint32_t i = 0; i < g.numInstances; ++i) {
+        const StkInstance& in = g.inst[i];
+        if (in.lightId >= 0) continue;
+        uint32_t mesh = 0xFFFFFFFFu;
+        for (auto& m : meshes)
+            if (m.first == in.indexOffset) mesh = m.second;
+        if (mesh == 0xFFFFFFFFu) {
+            uint32_t lo = 0xFFFFFFFFu, hi = 0;
+            for (int32_t k = 0; k < in.indexCount; ++k) {
+                lo = std::min(lo, g.ib[in.indexOffset + k]);
+                hi = std::max(hi, g.ib[in.indexOffset + k]);
+            }
+            std::vector<uint32_t> local(g.ib + in.indexOffset, g.ib + in.indexOffset + in.indexCount);
+            for (uint32_t& x : local) x -= lo;
+            check(stk_scene_create_mesh(s, g.vb + lo, hi - lo + 1, local.data(), (uint32_t)local.size(), &mesh) ==
+                      STK_OK,
+                  "create mesh");
+            meshes.push_back({in.indexOffset, mesh});
+        }
+        check(stk_scene_create_instance(s, mesh, in.materialId, in.objectToWorld, &id) == STK_OK, "create instance");
+    }
+    for (uint32_t i = 0; i < g.numInstances; ++i) {
+        const StkInstance& in = g.inst[i];
+        if (in.lightId < 0) continue;
+        StkLightDesc d;
+        std::memset(&d, 0, sizeof(d));
+        std::memcpy(d.xform, in.objectToWorld, sizeof(d.xform));
+        d.color[0] = d.color[1] = d.color[2] = 1.0f;
+        d.intensity = g.lights[in.lightId].color[0];
+        d.width = d.height = d.radius = 1.0f;
+        check(stk_scene_create_light(s, &d, &id) == STK_OK, "create light");
+    }
+    check(stk_scene_add_camera(s, &g.camera, &id) == STK_OK && id == 0, "add camera");
+    StkSceneArrays a;
+    check(stk_scene_get(s, &a) == STK_OK, "get rebuilt");
+    check(a.numVertices == g.numVertices && std::memcmp(a.vb, g.vb, sizeof(StkVertex) * g.numVertices) == 0,
+          "rebuilt vertices");
+    check(a.numIndices == g.numIndices && std::memcmp(a.ib, g.ib, 4 * (size_t)g.numIndices) == 0, "rebuilt indices");
+    check(a.numInstances == g.numInstances &&
+              std::memcmp(a.inst, g.inst, sizeof(StkInstance) * g.numInstances) == 0,
+          "rebuilt instances");
+    check(a.numLights == g.numLights && std::memcmp(a.lights, g.lights, sizeof(StkLight) * g.numLights) == 0,
+          "rebuilt lights");
+    // updates, then refused calls
+    StkLightDesc d;
+    std::memset(&d, 0, sizeof(d));
+    for (int k = 0; k < 4; ++k) d.xform[k * 5] = 1.0f;
+    d.width = d.height = 2.0f;
+    d.intensity = 4.0f;
+    check(stk_scene_update_light(s, 0, &d) == STK_OK, "update light");
+    d.type = 2;
+    check(stk_scene_update_light(s, 0, &d) != STK_OK, "light type change refused");
+    check(stk_scene_update_instance_transform(s, 0, d.xform) == STK_OK, "update transform");
+    check(stk_scene_update_instance_transform(s, 1u << 30, d.xform) != STK_OK, "unknown instance refused");
+    const uint32_t bad[3] = {0, 1, 9};
+    check(stk_scene_create_mesh(s, g.vb, 3, bad, 3, &id) != STK_OK, "index past the vertices refused");
+    check(stk_scene_create_instance(s, 1u << 20, -1, d.xform, &id) != STK_OK, "unknown mesh refused");
+    d.type = 1;
+    d.radius = 0.5f;
+    check(stk_scene_create_light(s, &d, &id) == STK_OK, "disc light");
+    d.type = 2;
+    check(stk_scene_create_light(s, &d, &id) == STK_OK, "sphere light");
+    check(stk_scene_get(s, &a) == STK_OK && a.numLights == g.numLights + 2, "lights after updates");
+    StkPathTracerParam cam;
+    const float eye[3] = {0, 0, 3}, target[3] = {0, 0, 0}, up[3] = {0, 1, 0};
+    check(stk_camera_look_at(eye, target, up, 50.0f, 16, 8, 0.01f, 10.0f, &cam) == STK_OK, "look at");
+    check(stk_scene_update_camera(s, 0, &cam) == STK_OK && stk_scene_camera_count(s) == 1, "update camera");
+    check(stk_scene_get_camera(s, 3, &cam) != STK_OK, "unknown camera refused");
+    stk_scene_free(s);
+}
+
 void errorPaths()
 {
     stk_bvh* b = nullptr;
@@ -207,6 +294,7 @@ int main()
     flatScene(7, 0, true);
     flatScene(9, 0, true);
     twoLevel();
+    sceneApi();
     images();
     errorPaths();
     if (failures) {

@@ -96,13 +96,15 @@ HEADER_STRUCTS = {
     "StkCacheInfo": "CacheInfo", "StkBvhOptions": "BvhOptions", "StkBvhHeader": "BvhHeader",
     "StkLaunchConfig": "LaunchConfig", "StkDeviceInfo": "DeviceInfo", "StkBatchScene": "BatchScene",
     "StkBatchViewInfo": "BatchViewInfo", "StkImageDesc": "ImageDesc", "StkBvhInfo": "BvhInfo",
+    "StkLightDesc": "LightDesc", "StkSceneArrays": "SceneArrays",
 }
 
 
 def test_ctypes_records_match_the_header(tmp_path):
     """Every ctypes record has the size and field offsets the C compiler gives
     the header's struct, so no entry point writes past a caller's record."""
-    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "stk/stk.h"', "int main(void) {"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "stk/stk.h"', '#include "stk/stk_scene.h"',
+             "int main(void) {"]
     for c_name, py_name in HEADER_STRUCTS.items():
         lines.append(f'printf("{c_name} %zu\\n", sizeof({c_name}));')
         for f, *_ in getattr(_abi, py_name)._fields_:
