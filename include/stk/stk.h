@@ -222,11 +222,58 @@ typedef struct StkImageDesc {
 } StkImageDesc;
 STK_API stk_status stk_render_image(stk_ctx* ctx, const StkRenderParams* p, const StkImageDesc* out);
 STK_API stk_status stk_reset_accumulation(stk_ctx* ctx);
+/* Watchdog (no reference counterpart: the reference has no failure detection).
+ * With milliseconds > 0, a stk_render / stk_batch_render frame still running
+ * that long after the call began is abandoned: a host thread sets an abort
+ * word in pinned memory; the path tracer's one-wave bookkeeping kernels copy
+ * it to the pass's device word at the start of every bounce, every later
+ * kernel of the pass returns at entry, a persistent extend stops taking work
+ * within 16 fetches per wave, and the accumulation step returns without
+ * touching the image.  The call then returns STK_E_TIMEOUT with the
+ * accumulated image and iteration count as they were before it (the output
+ * buffer's contents are unspecified); a radiance cache keeps the optimiser
+ * steps that finished before the abort and drops the rest.  While a deadline
+ * is armed every render call waits for its frame.  0 (the default) disarms.
+ * stk_trace_* calls are single bounded launches and are not watched. */
+STK_API stk_status stk_set_watchdog(stk_ctx* ctx, uint32_t milliseconds);
+/* Sets the abort word now, from any thread: the frame in flight (or, if none
+ * is, the next one) is abandoned as above and its render call returns
+ * STK_E_TIMEOUT; when that call had already returned (device output, no
+ * deadline armed), the next render call undoes the frame's accumulation step
+ * and returns STK_E_TIMEOUT without rendering.  With several pass streams
+ * (which overlap consecutive calls) and device output, an abort can also cut
+ * an earlier call's frame: the accumulation then restarts. */
+STK_API stk_status stk_abort(stk_ctx* ctx);
 /* Per-sample radiance of the last stk_render call, sampleBuffer layout
  * [(pixel*spp + s)*3 + c] (shaders/pathtracerMdl.hlsl:420-422). */
 STK_API stk_status stk_get_samples(stk_ctx* ctx, float* out, size_t count);
 /* The accumulated (not tonemapped) RGBA32F image. */
 STK_API stk_status stk_get_accumulation(stk_ctx* ctx, float* out, size_t count);
+
+/* Checkpoint / resume of the progressive state (no reference counterpart: the
+ * reference keeps the accumulation image and mPtIteration only in GPU memory,
+ * src/render/ptrender.cpp:714-726,839).  stk_checkpoint_save writes, to one
+ * file: the accumulated image of the last stk_render (running mean, RGBA32F),
+ * its iteration count and image layout; and, when the context has a radiance
+ * cache, its configuration, step counter, FP32 weights, Adam moments and
+ * moving averages.  userFrame is stored for the caller (the frameNumber to
+ * continue from).  The file is written aside and renamed over path.
+ * stk_checkpoint_load restores that state into a context whose scene (and, for
+ * the cache section, whose cache: stk_cache_create with the same
+ * configuration over the same scene bounds) the caller has set up again; the
+ * next stk_render of the same image size continues the accumulation.  A
+ * damaged or truncated file, or a cache of another configuration, is refused
+ * (STK_E_INVALID / STK_E_STATE) before any state changes. */
+typedef struct StkCheckpointInfo {
+    uint32_t sections;  /* bit 0 accumulation, bit 1 cache */
+    uint32_t iteration; /* frames accumulated */
+    uint64_t pixels;    /* pixels of the accumulated image */
+    uint64_t userFrame;
+    uint32_t cacheStep; /* Adam steps taken */
+    uint32_t pad;
+} StkCheckpointInfo;
+STK_API stk_status stk_checkpoint_save(stk_ctx* ctx, const char* path, uint64_t userFrame);
+STK_API stk_status stk_checkpoint_load(stk_ctx* ctx, const char* path, StkCheckpointInfo* info);
 /* Queue counters of the last wavefront pass (host copy of 8 words):
  * [0],[1] path queues of the last two bounces, [2] shadow rays of the last
  * bounce, [3] cache queries, [4] cache training records produced. */

@@ -600,6 +600,30 @@ class Context:
     def render_device(self, p: RenderParams, d_out: int):
         self._check(_LIB.stk_render(self._h, C.byref(p), C.c_void_p(d_out), 1), "stk_render")
 
+    def set_watchdog(self, milliseconds: int):
+        """stk_set_watchdog: frames running longer than this are abandoned
+        (the render call raises StkError with STK_E_TIMEOUT); 0 disarms."""
+        self._check(_LIB.stk_set_watchdog(self._h, milliseconds), "stk_set_watchdog")
+
+    def abort(self):
+        """stk_abort: abandon the frame in flight (or the next one)."""
+        self._check(_LIB.stk_abort(self._h), "stk_abort")
+
+    def checkpoint_save(self, path, user_frame: int = 0):
+        """stk_checkpoint_save: the accumulated image, its iteration count and,
+        with a cache, the cache's weights, Adam moments, averages and step."""
+        self._check(_LIB.stk_checkpoint_save(self._h, os.fsencode(str(path)), C.c_uint64(user_frame)),
+                    "stk_checkpoint_save")
+
+    def checkpoint_load(self, path) -> dict:
+        """stk_checkpoint_load: restores a checkpoint into this context (same
+        scene; same cache configuration for the cache section)."""
+        info = _abi.CheckpointInfo()
+        self._check(_LIB.stk_checkpoint_load(self._h, os.fsencode(str(path)), C.byref(info)), "stk_checkpoint_load")
+        return {"accumulation": bool(info.sections & 1), "cache": bool(info.sections & 2),
+                "iteration": info.iteration, "pixels": info.pixels, "user_frame": info.userFrame,
+                "cache_step": info.cacheStep}
+
     def reset_accumulation(self):
         self._check(_LIB.stk_reset_accumulation(self._h), "stk_reset_accumulation")
 

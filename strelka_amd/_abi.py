@@ -135,6 +135,11 @@ class SceneArrays(C.Structure):
                 ("sky", C.c_float * 4)]
 
 
+class CheckpointInfo(C.Structure):
+    _fields_ = [("sections", C.c_uint32), ("iteration", C.c_uint32), ("pixels", C.c_uint64),
+                ("userFrame", C.c_uint64), ("cacheStep", C.c_uint32), ("pad", C.c_uint32)]
+
+
 class LightDesc(C.Structure):
     _fields_ = [("type", C.c_int32), ("xform", C.c_float * 16), ("color", C.c_float * 3), ("intensity", C.c_float),
                 ("width", C.c_float), ("height", C.c_float), ("radius", C.c_float)]
@@ -264,6 +269,10 @@ PROTOTYPES = [
     ("stk_pack_normal", U32, [C.c_float, C.c_float, C.c_float]),
     ("stk_pack_uv", U32, [C.c_float, C.c_float]),
     ("stk_set_scene", C.c_int, [P, P, C.POINTER(BvhOptions)]),
+    ("stk_checkpoint_save", C.c_int, [P, C.c_char_p, C.c_uint64]),
+    ("stk_set_watchdog", C.c_int, [P, U32]),
+    ("stk_abort", C.c_int, [P]),
+    ("stk_checkpoint_load", C.c_int, [P, C.c_char_p, C.POINTER(CheckpointInfo)]),
 ]
 
 

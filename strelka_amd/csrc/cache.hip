@@ -380,10 +380,12 @@ __global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, float* __re
                                               float gradScale, const uint32_t* __restrict__ nDev, uint32_t nLimit, float lr,
                                               float b1, float b2, float eps, float bc1, float bc2, int sparse,
                                               float* __restrict__ ema, float* __restrict__ emaW,
-                                              __half* __restrict__ emaHalf, float emaDecay)
+                                              __half* __restrict__ emaHalf, float emaDecay,
+                                              const uint32_t* __restrict__ abortWord, uint32_t* __restrict__ stepsDone)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || (abortWord && *abortWord)) return;
+    if (stepsDone && i == 0) atomicAdd(stepsDone, 1u);
     const float raw = gr[i];
     if (sparse && raw == 0.0f) return;
     if (!isfinite(raw)) {
@@ -450,6 +452,19 @@ __global__ __launch_bounds__(256) void k_reset_ema(const float* __restrict__ p, 
     if (eh) eh[i] = __float2half(p[i]);
 }
 
+// The query copy of a restored moving average: the bias-corrected average of
+// an element that has been updated, else the parameter itself (what
+// k_reset_ema and k_adam leave behind).
+__global__ __launch_bounds__(256) void k_restore_ema_half(const float* __restrict__ p, const float* __restrict__ e,
+                                                          const float* __restrict__ ew, __half* __restrict__ eh,
+                                                          size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float w = ew[i];
+    eh[i] = __float2half(w > 0.0f ? e[i] / w : p[i]);
+}
+
 __global__ __launch_bounds__(256) void k_fill_zero(float* __restrict__ p, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -498,7 +513,8 @@ hipError_t cacheBackward(const CacheBuffers& cb, const CacheRecord* recs, uint32
     return hipGetLastError();
 }
 
-hipError_t cacheAdam(CacheBuffers& cb, float gradScale, const uint32_t* nDev, uint32_t nLimit, hipStream_t s)
+hipError_t cacheAdam(CacheBuffers& cb, float gradScale, const uint32_t* nDev, uint32_t nLimit,
+                     const uint32_t* abortWord, hipStream_t s)
 {
     cb.step += 1;
     const float lr = cb.lrHalfLife > 0.0f
@@ -510,10 +526,11 @@ hipError_t cacheAdam(CacheBuffers& cb, float gradScale, const uint32_t* nDev, ui
     hipLaunchKernelGGL(k_adam, dim3(blocks256(cb.gridParams)), dim3(256), 0, s, cb.gridP, cb.gridM, cb.gridV, cb.gridG,
                        cb.gridH, cb.gridParams, gradScale, nDev, nLimit, lr, cb.beta1, cb.beta2, cb.eps, bc1, bc2,
                        1, ema ? cb.gridE : nullptr, ema ? cb.gridEW : nullptr, ema ? cb.gridHE : nullptr,
-                       cb.emaDecay);
+                       cb.emaDecay, abortWord, (uint32_t*)nullptr);
     hipLaunchKernelGGL(k_adam, dim3(blocks256(kCacheMlpParams)), dim3(256), 0, s, cb.mlpP, cb.mlpM, cb.mlpV, cb.mlpG,
                        (__half*)nullptr, (size_t)kCacheMlpParams, gradScale, nDev, nLimit, lr, cb.beta1, cb.beta2,
-                       cb.eps, bc1, bc2, 0, ema ? cb.mlpE : nullptr, (float*)nullptr, (__half*)nullptr, cb.emaDecay);
+                       cb.eps, bc1, bc2, 0, ema ? cb.mlpE : nullptr, (float*)nullptr, (__half*)nullptr, cb.emaDecay,
+                       abortWord, cb.stepsDone);
     hipLaunchKernelGGL(k_pack_frags, dim3(blocks256(68 * kFrag)), dim3(256), 0, s, cb.mlpP,
                        reinterpret_cast<_Float16*>(cb.wFwd), reinterpret_cast<_Float16*>(cb.wBwd), 1.0f);
     if (ema) {
@@ -542,6 +559,27 @@ hipError_t cachePack(CacheBuffers& cb, hipStream_t s)
         hipLaunchKernelGGL(k_pack_frags, dim3(blocks256(68 * kFrag)), dim3(256), 0, s, cb.mlpP,
                            reinterpret_cast<_Float16*>(cb.wFwdE), reinterpret_cast<_Float16*>(cb.wBwdE), 1.0f);
     }
+    return hipGetLastError();
+}
+
+hipError_t cacheRestore(CacheBuffers& cb, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_to_half, dim3(blocks256(cb.gridParams)), dim3(256), 0, s, cb.gridP, cb.gridH, cb.gridParams);
+    hipLaunchKernelGGL(k_pack_frags, dim3(blocks256(68 * kFrag)), dim3(256), 0, s, cb.mlpP,
+                       reinterpret_cast<_Float16*>(cb.wFwd), reinterpret_cast<_Float16*>(cb.wBwd), 1.0f);
+    if (cb.emaDecay > 0.0f) {
+        hipLaunchKernelGGL(k_restore_ema_half, dim3(blocks256(cb.gridParams)), dim3(256), 0, s, cb.gridP, cb.gridE,
+                           cb.gridEW, cb.gridHE, cb.gridParams);
+        // the MLP average as k_adam leaves it after emaSteps updates (the raw
+        // weights before the first)
+        const bool avg = cb.emaSteps > 0;
+        const float w = avg ? 1.0f - powf(cb.emaDecay, (float)cb.emaSteps) : 1.0f;
+        hipLaunchKernelGGL(k_pack_frags, dim3(blocks256(68 * kFrag)), dim3(256), 0, s, avg ? cb.mlpE : cb.mlpP,
+                           reinterpret_cast<_Float16*>(cb.wFwdE), reinterpret_cast<_Float16*>(cb.wBwdE), 1.0f / w);
+    }
+    hipLaunchKernelGGL(k_fill_zero, dim3(blocks256(cb.gridParams)), dim3(256), 0, s, cb.gridG, cb.gridParams);
+    hipLaunchKernelGGL(k_fill_zero, dim3(blocks256(kCacheMlpParams)), dim3(256), 0, s, cb.mlpG,
+                       (size_t)kCacheMlpParams);
     return hipGetLastError();
 }
 

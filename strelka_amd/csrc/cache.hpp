@@ -84,6 +84,8 @@ struct CacheBuffers {
     void* sortTemp = nullptr;
     size_t sortBytes = 0;
     float* loss = nullptr;     // [0] loss sum of the last step
+    uint32_t* stepsDone = nullptr; // device count of the optimiser steps applied (equals step between frames;
+                                   // fewer after a frame the watchdog abandoned)
     uint32_t step = 0;
     uint32_t emaSteps = 0;     // updates since the moving average restarted
     float lr = 1e-2f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-15f;
@@ -103,11 +105,18 @@ hipError_t cacheBackward(const CacheBuffers& cb, const CacheRecord* recs, uint32
                          hipStream_t s);
 // One Adam step on every parameter from the accumulated gradients times
 // gradScale (divided by *nDev when given), then refreshes the FP16 copies and
-// MFMA fragments.
+// MFMA fragments.  abortWord (may be null): a nonzero word leaves every
+// parameter, moment and gradient as it was (a frame the watchdog abandoned;
+// the caller then restores step / emaSteps and calls cacheRestore).
 hipError_t cacheAdam(CacheBuffers& cb, float gradScale, const uint32_t* nDev, uint32_t nLimit,
-                     hipStream_t s);
+                     const uint32_t* abortWord, hipStream_t s);
 // Rebuilds the FP16 grid copy and the fragments from the FP32 masters.
 hipError_t cachePack(CacheBuffers& cb, hipStream_t s);
+// After the FP32 state (masters, moments, moving averages, their weights,
+// step and emaSteps) was written back from a checkpoint: rebuilds every
+// derived copy without restarting the moving average, and zeroes the
+// gradient accumulators.
+hipError_t cacheRestore(CacheBuffers& cb, hipStream_t s);
 hipError_t cacheZeroGrads(const CacheBuffers& cb, hipStream_t s);
 // Temporary bytes of the training records' sort for nCap records.
 hipError_t cacheSortBytes(uint32_t nCap, size_t& bytes);

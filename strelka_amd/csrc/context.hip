@@ -7,6 +7,8 @@
 #include <cstring>
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <condition_variable>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -171,6 +173,24 @@ struct stk_ctx {
     DevBuf cGridE, cGridEW, cGridHE, cMlpE, cFwdE, cBwdE, cGridGD, cSortKeys, cSortVals, cSortTemp;
     DevBuf queryRecs, queryOut, trainVerts, trainCount, trainBatch;
     uint32_t lastTrainCap = 0; // training records the last cached render could hold
+
+    // watchdog (stk_set_watchdog / stk_abort): the abort word in mapped
+    // pinned host memory (host and device addresses), the device word the
+    // accumulation step reads (abortAny), and a thread that sets the host word
+    // when an armed deadline passes.  The state of the call before the last
+    // one lets a later call undo a frame that was abandoned after its call
+    // had returned (device output, no watchdog armed).
+    uint32_t* abortHost = nullptr;
+    uint32_t* abortHostDev = nullptr;
+    void* abortAny = nullptr;
+    std::atomic<uint32_t> watchdogMs{0};
+    std::mutex wdMutex;
+    std::condition_variable wdCv;
+    std::thread wdThread;
+    bool wdStop = false, wdArmed = false;
+    std::chrono::steady_clock::time_point wdDeadline;
+    uint32_t lastIterBefore = 0, lastStepBefore = 0, lastEmaBefore = 0;
+    bool lastSynced = true; // the last render call waited for its frame
 
     std::vector<DevBuf*> owned()
     {
@@ -353,6 +373,22 @@ STK_API stk_status stk_create(int hipDevice, stk_ctx** out)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return STK_E_DEVICE;
     for (auto& ev : c->events)
         if (hipEventCreate(&ev) != hipSuccess) return STK_E_DEVICE;
+    void* h = nullptr;
+    if (hipHostMalloc(&h, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) return STK_E_DEVICE;
+    std::memset(h, 0, 64);
+    c->abortHost = static_cast<uint32_t*>(h);
+    void* hd = nullptr;
+    if (hipHostGetDevicePointer(&hd, h, 0) != hipSuccess || hipMalloc(&c->abortAny, 256) != hipSuccess ||
+        hipMemset(c->abortAny, 0, 256) != hipSuccess) {
+        if (c->abortAny) (void)hipFree(c->abortAny);
+        (void)hipHostFree(h);
+        return STK_E_DEVICE;
+    }
+    c->abortHostDev = static_cast<uint32_t*>(hd);
+    for (stk::PathState* ps : {&c->ps, &c->xPs[0], &c->xPs[1], &c->xPs[2], &c->xPs[3]}) {
+        ps->abortHost = c->abortHostDev;
+        ps->abortAny = static_cast<uint32_t*>(c->abortAny);
+    }
     setupScratch(hipDevice);
     *out = c.release();
     return STK_OK;
@@ -361,6 +397,14 @@ STK_API stk_status stk_create(int hipDevice, stk_ctx** out)
 STK_API void stk_destroy(stk_ctx* c)
 {
     if (!c) return;
+    if (c->wdThread.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(c->wdMutex);
+            c->wdStop = true;
+        }
+        c->wdCv.notify_all();
+        c->wdThread.join();
+    }
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipStream_t s : c->xStream)
@@ -382,6 +426,8 @@ STK_API void stk_destroy(stk_ctx* c)
     for (hipStream_t s : c->xStream)
         if (s) (void)hipStreamDestroy(s);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->abortAny) (void)hipFree(c->abortAny);
+    if (c->abortHost) (void)hipHostFree(c->abortHost);
     delete c;
 }
 
@@ -869,7 +915,7 @@ static stk_status ensureImage(stk_ctx* c, uint64_t key, size_t npix, size_t nsam
 
 // Renders p->pt.spp samples of every view in `descs` (one view for
 // stk_render, many for stk_batch_render) over the forest in `base`.
-static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vector<stk::SceneDesc>& descs,
+static stk_status renderForestRun(stk_ctx* c, const stk::SceneView& base, std::vector<stk::SceneDesc>& descs,
                                const StkRenderParams* p, float* outRgba, int outIsDevice, bool allowCache,
                                uint64_t imageKey)
 {
@@ -1112,7 +1158,8 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
                                              outIsDevice && outRgba ? reinterpret_cast<float4*>(outRgba)
                                                                     : static_cast<float4*>(c->out.ptr),
                                              (uint32_t)npix, c->iteration, acc, p->enableTonemap != 0,
-                                             p->tonemapperType, c->stream));
+                                             p->tonemapperType, static_cast<const uint32_t*>(c->abortAny),
+                                             c->stream));
     if (acc) ++c->iteration;
     if (c->accumFree[lane]) {
         STK_HIP(c, hipEventRecord(c->accumFree[lane], c->stream));
@@ -1122,6 +1169,139 @@ static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vect
         STK_HIP(c, hipMemcpyAsync(outRgba, c->out.ptr, npix * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
         STK_HIP(c, hipStreamSynchronize(c->stream));
     }
+    return STK_OK;
+}
+
+// ---- watchdog -------------------------------------------------------------------
+
+static void watchdogLoop(stk_ctx* c)
+{
+    std::unique_lock<std::mutex> lk(c->wdMutex);
+    while (!c->wdStop) {
+        if (!c->wdArmed) {
+            c->wdCv.wait(lk);
+            continue;
+        }
+        const auto deadline = c->wdDeadline;
+        c->wdCv.wait_until(lk, deadline);
+        if (c->wdArmed && !c->wdStop && c->wdDeadline == deadline && std::chrono::steady_clock::now() >= deadline) {
+            __atomic_store_n(c->abortHost, 1u, __ATOMIC_RELEASE);
+            c->wdArmed = false;
+        }
+    }
+}
+
+static void armWatchdog(stk_ctx* c, uint32_t ms)
+{
+    std::lock_guard<std::mutex> lk(c->wdMutex);
+    c->wdDeadline = std::chrono::steady_clock::now() + std::chrono::milliseconds(ms);
+    c->wdArmed = true;
+    c->wdCv.notify_all();
+}
+
+static void disarmWatchdog(stk_ctx* c)
+{
+    std::lock_guard<std::mutex> lk(c->wdMutex);
+    c->wdArmed = false;
+    c->wdCv.notify_all();
+}
+
+// After every stream of the frame has finished: if the abort word was set,
+// learns whether a pass was abandoned (abortAny), clears both words and, for
+// an abandoned frame, puts back the iteration count and the cache's step
+// counters of before the frame and rebuilds the cache's derived copies (its
+// optimiser steps returned at entry; their gradients are dropped).
+static stk_status settleAbort(stk_ctx* c, const char* what, bool restart = false)
+{
+    if (__atomic_load_n(c->abortHost, __ATOMIC_ACQUIRE) == 0u) return STK_OK;
+    uint32_t any = 0;
+    STK_HIP(c, hipStreamSynchronize(c->stream));
+    for (hipStream_t s : c->xStream)
+        if (s) STK_HIP(c, hipStreamSynchronize(s));
+    STK_HIP(c, hipMemcpyAsync(&any, c->abortAny, 4, hipMemcpyDeviceToHost, c->stream));
+    STK_HIP(c, hipMemsetAsync(c->abortAny, 0, 4, c->stream));
+    STK_HIP(c, hipStreamSynchronize(c->stream));
+    __atomic_store_n(c->abortHost, 0u, __ATOMIC_RELEASE);
+    if (!any) return STK_OK;
+    // restart: the abandoned passes may include an earlier call's (several
+    // pass streams overlap consecutive calls), so the image restarts
+    c->iteration = restart ? 0u : c->lastIterBefore;
+    if (c->haveCache) {
+        // the optimiser steps that ran to completion stay (they used the step
+        // numbers they were enqueued with, consecutive from lastStepBefore)
+        uint32_t done = c->lastStepBefore;
+        STK_HIP(c, hipMemcpyAsync(&done, c->cache.stepsDone, 4, hipMemcpyDeviceToHost, c->stream));
+        STK_HIP(c, hipStreamSynchronize(c->stream));
+        const uint32_t kept = done >= c->lastStepBefore ? done - c->lastStepBefore : 0u;
+        c->cache.step = c->lastStepBefore + kept;
+        c->cache.emaSteps = c->cache.emaDecay > 0.0f ? c->lastEmaBefore + kept : c->lastEmaBefore;
+        STK_HIP(c, stk::cacheRestore(c->cache, c->stream));
+        STK_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return fail(c, STK_E_TIMEOUT, what);
+}
+
+// renderForestRun under the watchdog: with a deadline armed the call waits
+// for its frame, and a frame abandoned by the deadline or by stk_abort
+// returns STK_E_TIMEOUT with the accumulation as it was before the call.
+static stk_status renderForest(stk_ctx* c, const stk::SceneView& base, std::vector<stk::SceneDesc>& descs,
+                               const StkRenderParams* p, float* outRgba, int outIsDevice, bool allowCache,
+                               uint64_t imageKey)
+{
+    stk_status st;
+    if (!c->lastSynced && __atomic_load_n(c->abortHost, __ATOMIC_ACQUIRE) != 0u) {
+        // an abort that reached the previous call's frame after that call
+        // had returned: its accumulation step is undone here
+        c->lastSynced = true;
+        if ((st = settleAbort(c, "render: the previous call's frame was abandoned after the call returned; its "
+                                 "accumulation step was undone")) != STK_OK)
+            return st;
+    }
+    const bool prevSynced = c->lastSynced;
+    c->lastIterBefore = c->iteration;
+    c->lastStepBefore = c->cache.step;
+    c->lastEmaBefore = c->cache.emaSteps;
+    const uint32_t ms = c->watchdogMs.load();
+    if (ms) armWatchdog(c, ms);
+    st = renderForestRun(c, base, descs, p, outRgba, outIsDevice, allowCache, imageKey);
+    if (ms) {
+        // the deadline covers the frame on the GPU, not only the enqueueing
+        hipError_t e = hipStreamSynchronize(c->stream);
+        disarmWatchdog(c);
+        if (e != hipSuccess && st == STK_OK) st = hipFail(c, e, "render: waiting for the frame");
+    }
+    c->lastSynced = ms != 0 || (outRgba && !outIsDevice);
+    const bool requested = __atomic_load_n(c->abortHost, __ATOMIC_ACQUIRE) != 0u;
+    if (requested && (c->lastSynced || st != STK_OK)) {
+        c->lastSynced = true;
+        const stk_status a = settleAbort(c,
+                                         ms ? "render: frame abandoned by the watchdog deadline"
+                                            : "render: frame abandoned by stk_abort",
+                                         !prevSynced && c->lastStreams > 1);
+        if (st == STK_OK) st = a;
+    }
+    return st;
+}
+
+STK_API stk_status stk_set_watchdog(stk_ctx* c, uint32_t milliseconds)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    c->watchdogMs.store(milliseconds);
+    if (milliseconds && !c->wdThread.joinable()) {
+        try {
+            c->wdThread = std::thread(watchdogLoop, c);
+        } catch (const std::system_error&) {
+            c->watchdogMs.store(0);
+            return fail(c, STK_E_STATE, "stk_set_watchdog: cannot start the watchdog thread");
+        }
+    }
+    return STK_OK;
+}
+
+STK_API stk_status stk_abort(stk_ctx* c)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    __atomic_store_n(c->abortHost, 1u, __ATOMIC_RELEASE);
     return STK_OK;
 }
 
@@ -1554,6 +1734,277 @@ STK_API stk_status stk_get_accumulation(stk_ctx* c, float* out, size_t count)
     return STK_OK;
 }
 
+// ---- checkpoint / resume ------------------------------------------------------
+//
+// The progressive state the reference keeps in GPU images (the accumulation
+// image and mPtIteration, src/render/ptrender.cpp:714-726,839) and the
+// radiance cache's training state, written to one file and read back into a
+// context so a long convergence run or a training run continues where it
+// stopped.  Layout (little-endian, no padding between records):
+//   CkptHeader; [CkptAccum, float4 history[pixels]] if section bit 0;
+//   [CkptCache, grid P, M, V, mlp P, M, V, and when emaDecay > 0 grid E, EW,
+//   mlp E] if bit 1; uint64 FNV-1a hash of every byte before it.
+
+namespace {
+
+constexpr char kCkptMagic[8] = {'S', 'T', 'K', 'C', 'K', 'P', 'T', '1'};
+constexpr uint32_t kCkptVersion = 1;
+
+struct CkptHeader {
+    char magic[8];
+    uint32_t version;
+    uint32_t sections; // bit 0 accumulation, bit 1 cache
+    uint64_t userFrame;
+};
+struct CkptAccum {
+    uint64_t imageKey;
+    uint64_t pixels;
+    uint32_t iteration;
+    uint32_t pad;
+};
+struct CkptCache {
+    StkCacheConfig cfg;
+    stk::GridDesc grid;
+    uint32_t step, emaSteps;
+    uint64_t gridParams, mlpParams;
+};
+
+uint64_t fnv1a(uint64_t h, const void* p, size_t n)
+{
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= b[i];
+        h *= 0x100000001B3ull;
+    }
+    return h;
+}
+
+struct CkptWriter {
+    FILE* f = nullptr;
+    uint64_t hash = 0xCBF29CE484222325ull;
+    bool ok = true;
+    void put(const void* p, size_t n)
+    {
+        if (!ok || n == 0) return;
+        hash = fnv1a(hash, p, n);
+        ok = std::fwrite(p, 1, n, f) == n;
+    }
+};
+
+struct CkptReader {
+    const std::vector<unsigned char>& buf;
+    size_t pos = 0;
+    bool take(void* dst, size_t n)
+    {
+        if (buf.size() - pos < n) return false;
+        std::memcpy(dst, buf.data() + pos, n);
+        pos += n;
+        return true;
+    }
+    const unsigned char* view(size_t n)
+    {
+        if (buf.size() - pos < n) return nullptr;
+        const unsigned char* p = buf.data() + pos;
+        pos += n;
+        return p;
+    }
+};
+
+// Waits for the context's stream and every pass stream.
+stk_status syncAll(stk_ctx* c)
+{
+    STK_HIP(c, hipStreamSynchronize(c->stream));
+    for (hipStream_t s : c->xStream)
+        if (s) STK_HIP(c, hipStreamSynchronize(s));
+    return STK_OK;
+}
+
+stk_status download(stk_ctx* c, std::vector<float>& dst, const void* src, size_t floats)
+{
+    dst.resize(floats);
+    if (floats) STK_HIP(c, hipMemcpyAsync(dst.data(), src, floats * 4, hipMemcpyDeviceToHost, c->stream));
+    return STK_OK;
+}
+
+} // namespace
+
+STK_API stk_status stk_checkpoint_save(stk_ctx* c, const char* path, uint64_t userFrame)
+{
+    if (!validCtx(c) || !path) return STK_E_INVALID;
+    (void)hipSetDevice(c->device);
+    stk_status st;
+    if ((st = syncAll(c)) != STK_OK) return st;
+    const bool accum = c->history.ptr && c->imagePixels > 0;
+    const bool cache = c->haveCache;
+    std::vector<float> history, gP, gM, gV, mP, mM, mV, gE, gEW, mE;
+    try {
+        if (accum && (st = download(c, history, c->history.ptr, c->imagePixels * 4)) != STK_OK) return st;
+        if (cache) {
+            const stk::CacheBuffers& cb = c->cache;
+            const size_t gp = cb.gridParams, mp = stk::kCacheMlpParams;
+            if ((st = download(c, gP, cb.gridP, gp)) != STK_OK || (st = download(c, gM, cb.gridM, gp)) != STK_OK ||
+                (st = download(c, gV, cb.gridV, gp)) != STK_OK || (st = download(c, mP, cb.mlpP, mp)) != STK_OK ||
+                (st = download(c, mM, cb.mlpM, mp)) != STK_OK || (st = download(c, mV, cb.mlpV, mp)) != STK_OK)
+                return st;
+            if (cb.emaDecay > 0.0f &&
+                ((st = download(c, gE, cb.gridE, gp)) != STK_OK || (st = download(c, gEW, cb.gridEW, gp)) != STK_OK ||
+                 (st = download(c, mE, cb.mlpE, mp)) != STK_OK))
+                return st;
+        }
+    } catch (const std::bad_alloc&) {
+        return fail(c, STK_E_OOM, "stk_checkpoint_save: host allocation failed");
+    }
+    STK_HIP(c, hipStreamSynchronize(c->stream));
+
+    // written next to the target and renamed over it, so an interrupted save
+    // leaves the previous checkpoint intact
+    const std::string tmp = std::string(path) + ".part";
+    CkptWriter w;
+    w.f = std::fopen(tmp.c_str(), "wb");
+    if (!w.f) return fail(c, STK_E_INVALID, std::string("stk_checkpoint_save: cannot open ") + tmp);
+    CkptHeader h;
+    std::memcpy(h.magic, kCkptMagic, 8);
+    h.version = kCkptVersion;
+    h.sections = (accum ? 1u : 0u) | (cache ? 2u : 0u);
+    h.userFrame = userFrame;
+    w.put(&h, sizeof(h));
+    if (accum) {
+        CkptAccum a{c->imageKey, (uint64_t)c->imagePixels, c->iteration, 0};
+        w.put(&a, sizeof(a));
+        w.put(history.data(), history.size() * 4);
+    }
+    if (cache) {
+        CkptCache k;
+        std::memset(&k, 0, sizeof(k));
+        k.cfg = c->cacheCfg;
+        k.grid = c->cache.grid;
+        k.step = c->cache.step;
+        k.emaSteps = c->cache.emaSteps;
+        k.gridParams = c->cache.gridParams;
+        k.mlpParams = stk::kCacheMlpParams;
+        w.put(&k, sizeof(k));
+        for (const std::vector<float>* v : {&gP, &gM, &gV, &mP, &mM, &mV, &gE, &gEW, &mE})
+            w.put(v->data(), v->size() * 4);
+    }
+    const uint64_t hash = w.hash;
+    w.put(&hash, sizeof(hash));
+    const bool closed = std::fclose(w.f) == 0;
+    if (!w.ok || !closed || std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        return fail(c, STK_E_INVALID, std::string("stk_checkpoint_save: write failed: ") + path);
+    }
+    return STK_OK;
+}
+
+STK_API stk_status stk_checkpoint_load(stk_ctx* c, const char* path, StkCheckpointInfo* info)
+{
+    if (!validCtx(c) || !path) return STK_E_INVALID;
+    std::vector<unsigned char> buf;
+    {
+        FILE* f = std::fopen(path, "rb");
+        if (!f) return fail(c, STK_E_INVALID, std::string("stk_checkpoint_load: cannot open ") + path);
+        std::fseek(f, 0, SEEK_END);
+        const long size = std::ftell(f);
+        std::fseek(f, 0, SEEK_SET);
+        try {
+            buf.resize(size > 0 ? (size_t)size : 0);
+        } catch (const std::bad_alloc&) {
+            std::fclose(f);
+            return fail(c, STK_E_OOM, "stk_checkpoint_load: host allocation failed");
+        }
+        const bool ok = size > 0 && std::fread(buf.data(), 1, buf.size(), f) == buf.size();
+        std::fclose(f);
+        if (!ok) return fail(c, STK_E_INVALID, "stk_checkpoint_load: read failed");
+    }
+    if (buf.size() < sizeof(CkptHeader) + 8) return fail(c, STK_E_INVALID, "stk_checkpoint_load: truncated file");
+    uint64_t stored;
+    std::memcpy(&stored, buf.data() + buf.size() - 8, 8);
+    if (fnv1a(0xCBF29CE484222325ull, buf.data(), buf.size() - 8) != stored)
+        return fail(c, STK_E_INVALID, "stk_checkpoint_load: checksum mismatch (truncated or damaged file)");
+    CkptReader r{buf};
+    CkptHeader h;
+    r.take(&h, sizeof(h));
+    if (std::memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != kCkptVersion || (h.sections & ~3u))
+        return fail(c, STK_E_INVALID, "stk_checkpoint_load: not a checkpoint of this version");
+
+    // everything is validated before any device state changes
+    CkptAccum a{};
+    const unsigned char* history = nullptr;
+    if (h.sections & 1u) {
+        if (!r.take(&a, sizeof(a)) || a.pixels == 0 || a.pixels > (1ull << 28) ||
+            !(history = r.view((size_t)a.pixels * 16)))
+            return fail(c, STK_E_INVALID, "stk_checkpoint_load: bad accumulation section");
+    }
+    CkptCache k{};
+    const unsigned char* cacheData = nullptr;
+    size_t gp = 0, mp = 0;
+    bool ema = false;
+    if (h.sections & 2u) {
+        if (!r.take(&k, sizeof(k))) return fail(c, STK_E_INVALID, "stk_checkpoint_load: bad cache section");
+        if (!c->haveCache)
+            return fail(c, STK_E_STATE, "stk_checkpoint_load: the checkpoint holds a cache; create one with its "
+                                        "configuration first (stk_cache_create)");
+        if (std::memcmp(&k.cfg, &c->cacheCfg, sizeof(k.cfg)) != 0 ||
+            std::memcmp(&k.grid, &c->cache.grid, sizeof(k.grid)) != 0 || k.gridParams != c->cache.gridParams ||
+            k.mlpParams != stk::kCacheMlpParams)
+            return fail(c, STK_E_STATE, "stk_checkpoint_load: the context's cache has another configuration or "
+                                        "scene bounds than the checkpoint's");
+        gp = (size_t)k.gridParams;
+        mp = (size_t)k.mlpParams;
+        ema = k.cfg.emaDecay > 0.0f;
+        if (!(cacheData = r.view((3 * gp + 3 * mp + (ema ? 2 * gp + mp : 0)) * 4)))
+            return fail(c, STK_E_INVALID, "stk_checkpoint_load: bad cache section");
+    }
+    if (r.pos != buf.size() - 8) return fail(c, STK_E_INVALID, "stk_checkpoint_load: trailing bytes");
+
+    (void)hipSetDevice(c->device);
+    stk_status st;
+    if ((st = syncAll(c)) != STK_OK) return st;
+    if (h.sections & 1u) {
+        const void* before = c->history.ptr;
+        if ((st = devReserve(c, c->history, (size_t)a.pixels * 16)) != STK_OK) return st;
+        if (c->history.ptr != before) invalidateGraph(c);
+        STK_HIP(c, hipMemcpyAsync(c->history.ptr, history, (size_t)a.pixels * 16, hipMemcpyHostToDevice, c->stream));
+        c->imageKey = a.imageKey;
+        c->imagePixels = (size_t)a.pixels;
+        c->iteration = a.iteration;
+    }
+    if (h.sections & 2u) {
+        stk::CacheBuffers& cb = c->cache;
+        const unsigned char* p = cacheData;
+        auto up = [&](void* dst, size_t floats) -> hipError_t {
+            hipError_t e = hipMemcpyAsync(dst, p, floats * 4, hipMemcpyHostToDevice, c->stream);
+            p += floats * 4;
+            return e;
+        };
+        STK_HIP(c, up(cb.gridP, gp));
+        STK_HIP(c, up(cb.gridM, gp));
+        STK_HIP(c, up(cb.gridV, gp));
+        STK_HIP(c, up(cb.mlpP, mp));
+        STK_HIP(c, up(cb.mlpM, mp));
+        STK_HIP(c, up(cb.mlpV, mp));
+        if (ema) {
+            STK_HIP(c, up(cb.gridE, gp));
+            STK_HIP(c, up(cb.gridEW, gp));
+            STK_HIP(c, up(cb.mlpE, mp));
+        }
+        cb.step = k.step;
+        cb.emaSteps = k.emaSteps;
+        STK_HIP(c, hipMemcpyAsync(cb.stepsDone, &k.step, 4, hipMemcpyHostToDevice, c->stream));
+        STK_HIP(c, stk::cacheRestore(cb, c->stream));
+    }
+    STK_HIP(c, hipStreamSynchronize(c->stream));
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->sections = h.sections;
+        info->iteration = a.iteration;
+        info->pixels = a.pixels;
+        info->userFrame = h.userFrame;
+        info->cacheStep = k.step;
+    }
+    return STK_OK;
+}
+
 // ---- radiance cache -----------------------------------------------------------
 
 static StkCacheConfig defaultCacheConfig()
@@ -1700,6 +2151,8 @@ STK_API stk_status stk_cache_create(stk_ctx* c, const StkCacheConfig* cfgIn)
     cb.xT = static_cast<__half*>(c->cXT.ptr);
     cb.dyT = static_cast<__half*>(c->cDyT.ptr);
     cb.loss = static_cast<float*>(c->cLoss.ptr);
+    cb.stepsDone = static_cast<uint32_t*>(c->cLoss.ptr) + 8;
+    STK_HIP(c, hipMemsetAsync(c->cLoss.ptr, 0, 64, c->stream));
     // initialisation: grid U(-1e-4, 1e-4); hidden weights He-uniform
     // U(-sqrt(6/64), sqrt(6/64)); the output layer zero, so the cache starts
     // by predicting no radiance instead of an arbitrary value it must unlearn
@@ -1770,7 +2223,7 @@ STK_API stk_status stk_cache_train_step(stk_ctx* c, const StkCacheRecord* recs, 
     STK_HIP(c, stk::cacheBackward(c->cache, reinterpret_cast<const stk::CacheRecord*>(recs), n, nullptr, c->stream));
     float loss = 0.0f;
     STK_HIP(c, hipMemcpyAsync(&loss, c->cache.loss, 4, hipMemcpyDeviceToHost, c->stream));
-    STK_HIP(c, stk::cacheAdam(c->cache, 1.0f / (float)n, nullptr, n, c->stream));
+    STK_HIP(c, stk::cacheAdam(c->cache, 1.0f / (float)n, nullptr, n, nullptr, c->stream));
     STK_HIP(c, hipStreamSynchronize(c->stream));
     if (lossOut) *lossOut = loss / (float)n;
     return STK_OK;

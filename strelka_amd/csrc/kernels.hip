@@ -205,10 +205,11 @@ __device__ __forceinline__ F3 tonemap(F3 c, uint32_t type)
 __global__ __launch_bounds__(kBlock) void k_accumulate_tonemap(const float4* __restrict__ cur,
                                                                float4* __restrict__ history, float4* __restrict__ out,
                                                                uint32_t npix, uint32_t iteration, int accumulate,
-                                                               int doTonemap, uint32_t type)
+                                                               int doTonemap, uint32_t type,
+                                                               const uint32_t* __restrict__ abortAny)
 {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= npix) return;
+    if (i >= npix || (abortAny && *abortAny)) return;
     F3 res = f3(cur[i]);
     if (accumulate) {
         // accPt (shaders/accumulation.hlsl:86-97)
@@ -323,11 +324,11 @@ hipError_t launchZero(float4* p, size_t n, hipStream_t s)
 
 hipError_t launchAccumulateTonemap(const float4* cur, float4* history, float4* out, uint32_t npix,
                                    uint32_t iteration, int accumulate, int doTonemap, uint32_t tonemapType,
-                                   hipStream_t s)
+                                   const uint32_t* abortAny, hipStream_t s)
 {
     if (npix == 0) return hipSuccess;
     hipLaunchKernelGGL(k_accumulate_tonemap, dim3(blocks(npix)), dim3(kBlock), 0, s, cur, history, out, npix,
-                       iteration, accumulate, doTonemap, tonemapType);
+                       iteration, accumulate, doTonemap, tonemapType, abortAny);
     return hipGetLastError();
 }
 

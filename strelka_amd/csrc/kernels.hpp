@@ -88,9 +88,19 @@ struct PathState {
     float4* shCd = nullptr; // NEE contribution, diffuse part
     float4* shCg = nullptr; // NEE contribution, glossy part
     uint32_t* counters = nullptr; // [0],[1] queue totals, [2] shadow total, [3] queries, [4] train records,
+                                  // [5] abort word of this pass (1: the pass's kernels return at entry),
                                   // [8..11] two 64-bit ray totals of the render call (closest, shadow)
     PassState* pass = nullptr;
+    // Watchdog: a host word in pinned memory (nonzero: abandon the frame),
+    // read by the one-wave bookkeeping kernels into counters[5] and, when
+    // set, into abortAny (a device word of the context read by the
+    // accumulation step after every pass stream has joined).
+    const uint32_t* abortHost = nullptr;
+    uint32_t* abortAny = nullptr;
 };
+
+// Index of the pass's abort word in PathState::counters.
+constexpr int kAbortCounter = 5;
 
 // Radiance-cache hooks of one pass.  Training paths are the slots with
 // slot % trainStride == trainOffset: they run to maxDepth and record every
@@ -176,8 +186,10 @@ hipError_t launchPackImage(const float4* src, uint32_t npix, uint32_t format, vo
 hipError_t launchZero(float4* p, size_t n, hipStream_t s);
 // Running-mean accumulation (shaders/accumulation.hlsl:86-97) + tonemap
 // (shaders/tonemap.hlsl:66-88) in one pass.
+// abortAny (may be null): when it holds a nonzero word the step returns
+// without touching the history or the output (a frame the watchdog abandoned).
 hipError_t launchAccumulateTonemap(const float4* cur, float4* history, float4* out, uint32_t npix,
                                    uint32_t iteration, int accumulate, int tonemap, uint32_t tonemapType,
-                                   hipStream_t s);
+                                   const uint32_t* abortAny, hipStream_t s);
 
 } // namespace stk

@@ -73,6 +73,22 @@ __global__ void k_pass_set(PassState* pass, PassState v)
 }
 
 // Clears the pass counters and every stripe counter (one 64-thread block).
+// The watchdog's host word into the pass's abort word (and the context's
+// abortAny when set); one lane, from a one-wave bookkeeping kernel.
+__device__ __forceinline__ void poll_abort(const PathState& ps)
+{
+    if (!ps.abortHost) return;
+    if (__hip_atomic_load(ps.abortHost, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) {
+        ps.counters[kAbortCounter] = 1u;
+        *ps.abortAny = 1u;
+    }
+}
+
+// Whether this pass was abandoned.  Only this pass stream's one-wave kernels
+// and its persistent extend write the word, never while another kernel of the
+// stream runs, so every thread of a launch reads the same value.
+__device__ __forceinline__ bool pass_aborted(const PathState& ps) { return ps.counters[kAbortCounter] != 0u; }
+
 __global__ void k_pass_reset(PathState ps)
 {
     const uint32_t t = threadIdx.x;
@@ -81,6 +97,7 @@ __global__ void k_pass_reset(PathState ps)
         for (int k = 0; k < kQueueKinds; ++k) ps.stripes[k * kStripes + t] = 0;
         ps.work[t] = 0;
     }
+    if (t == 0) poll_abort(ps);
 }
 
 // Stripe-major consumer grids of B-thread blocks: block b reads segment
@@ -102,6 +119,7 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 // a pass covers global slots [slotBegin, slotBegin + slotCount).
 __global__ __launch_bounds__(kBlock) void k_pt_raygen(SceneView sc, PtLaunch lp, PathState ps)
 {
+    if (pass_aborted(ps)) return;
     const uint32_t local = blockIdx.x * kBlock + threadIdx.x;
     const PassState pass = *ps.pass;
     bool valid = false;
@@ -165,6 +183,7 @@ __global__ void k_pt_setup(PathState ps, int depth)
         tot[1] += ns; // shadow rays of the previous depth (0 at depth 0)
         ps.counters[cur] = nq;
         if (depth > 0) ps.counters[2] = ns;
+        poll_abort(ps);
     }
 }
 
@@ -216,6 +235,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) v
                                                      int binary)
 {
     __shared__ uint32_t stack[(TL ? 2 * kWideStack : kStackDepth) * BLOCK];
+    if (pass_aborted(ps)) return;
     const int cur = depth & 1;
     const uint32_t* counts = ps.stripes + cur * kStripes;
     if (!persistent) {
@@ -229,10 +249,22 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) v
     }
     const uint32_t lane = threadIdx.x % 64;
     const uint32_t wave = (blockIdx.x * BLOCK + threadIdx.x) / 64;
+    uint32_t fetched = 0;
     for (uint32_t k = 0; k < kStripes; ++k) {
         const uint32_t sg = (wave + k) % kStripes;
         const uint32_t cnt = counts[sg];
         for (;;) {
+            // drain: every 16th fetch lane 0 polls the watchdog's host word;
+            // once it is set the wave stops taking entries (the pass's later
+            // kernels return at entry)
+            uint32_t stop = 0;
+            if (lane == 0 && (++fetched & 15u) == 0u && ps.abortHost &&
+                __hip_atomic_load(ps.abortHost, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) {
+                ps.counters[kAbortCounter] = 1u;
+                *ps.abortAny = 1u;
+                stop = 1;
+            }
+            if (__shfl((int)stop, 0)) return;
             uint32_t base = cnt;
             if (lane == 0 && __hip_atomic_load(&ps.work[sg], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < cnt)
                 base = atomicAdd(&ps.work[sg], 64u);
@@ -484,6 +516,7 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
 
 __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, PathState ps, int depth)
 {
+    if (pass_aborted(ps)) return;
     const uint32_t stripe = blockIdx.x % kStripes, cnt = ps.stripes[(depth & 1) * kStripes + stripe];
     for (uint32_t base = stripe_first(kBlock); base < cnt; base += stripe_step(kBlock)) {
         const uint32_t local = base + threadIdx.x;
@@ -538,6 +571,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
                                                                                          int refCompat)
 {
     __shared__ uint32_t stack[2 * kWideStack * kBlock];
+    if (pass_aborted(ps)) return;
     const uint32_t stripe = blockIdx.x % kStripes, cnt = ps.stripes[2 * kStripes + stripe];
     for (uint32_t base = stripe_first(kBlock); base < cnt; base += stripe_step(kBlock)) {
         const uint32_t local = base + threadIdx.x;
@@ -551,7 +585,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) 
 __global__ __launch_bounds__(kBlock) void k_cache_apply(PtLaunch lp, PathState ps)
 {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= ps.counters[3]) return;
+    if (pass_aborted(ps) || q >= ps.counters[3]) return;
     const CacheRecord& rec = lp.cache.queryRecs[q];
     const uint32_t slot = float_to_bits(rec.normal.w);
     const float4 o = lp.cache.queryOut[q];
@@ -564,7 +598,7 @@ __global__ __launch_bounds__(kBlock) void k_pt_finalize(PtLaunch lp, PathState p
                                                         float* __restrict__ samples)
 {
     const uint32_t local = blockIdx.x * kBlock + threadIdx.x;
-    if (local >= ps.pass->slotCount) return;
+    if (pass_aborted(ps) || local >= ps.pass->slotCount) return;
     const uint32_t pixel = ps.pixel[local];
     if (pixel == STK_INVALID_INDEX) return;
     const float4 c = ps.col[local];
@@ -589,6 +623,7 @@ __global__ __launch_bounds__(kBlock) void k_pt_finalize(PtLaunch lp, PathState p
 // estimator, of what a query at d stands in for.
 __global__ __launch_bounds__(kBlock) void k_cache_targets(PtLaunch lp, PathState ps)
 {
+    if (pass_aborted(ps)) return;
     const PtCache& pc = lp.cache;
     const PassState pass = *ps.pass;
     const uint32_t maxDepth = lp.maxDepth;
@@ -723,7 +758,7 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
         for (uint32_t k = 0; k < pc.trainSteps; ++k) {
             hipError_t e = cacheBackward(*cb, pc.trainBatch, pc.trainCap, ps.counters + 4, s);
             if (e != hipSuccess) return e;
-            e = cacheAdam(*cb, 1.0f, ps.counters + 4, pc.trainCap, s);
+            e = cacheAdam(*cb, 1.0f, ps.counters + 4, pc.trainCap, ps.counters + kAbortCounter, s);
             if (e != hipSuccess) return e;
         }
     }

@@ -64,6 +64,9 @@ def test_null_arguments_are_rejected_without_a_device():
     assert lib.stk_last_error(None) == b"null context"
     assert lib.stk_scene_generate(99, 0, 0, C.byref(C.c_void_p())) == 1
     assert lib.stk_bvh_build_host(None, 0, None, 0, None, 0, None, None) == 1
+    assert lib.stk_checkpoint_save(None, b"/tmp/x.stkckpt", 0) == 1
+    assert lib.stk_checkpoint_load(None, b"/tmp/x.stkckpt", None) == 1
+    assert lib.stk_set_scene(None, None, None) == 1
 
 
 def test_scene_generators_are_deterministic():
@@ -96,7 +99,7 @@ HEADER_STRUCTS = {
     "StkCacheInfo": "CacheInfo", "StkBvhOptions": "BvhOptions", "StkBvhHeader": "BvhHeader",
     "StkLaunchConfig": "LaunchConfig", "StkDeviceInfo": "DeviceInfo", "StkBatchScene": "BatchScene",
     "StkBatchViewInfo": "BatchViewInfo", "StkImageDesc": "ImageDesc", "StkBvhInfo": "BvhInfo",
-    "StkLightDesc": "LightDesc", "StkSceneArrays": "SceneArrays",
+    "StkLightDesc": "LightDesc", "StkSceneArrays": "SceneArrays", "StkCheckpointInfo": "CheckpointInfo",
 }
 
 
