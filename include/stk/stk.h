@@ -87,6 +87,10 @@ typedef struct StkBvhHeader {
 
 /* ---- library / context ---------------------------------------------------- */
 STK_API const char* stk_version(void);
+/* "source-sha256 <hex>; offload-arch <arch>": a SHA-256 over the sources the
+ * library was built from and their compile flags, so a caller can check that a
+ * binary matches a source tree (strelka_amd/build.py: source_hash). */
+STK_API const char* stk_build_info(void);
 STK_API stk_status stk_create(int hipDevice, stk_ctx** out);
 STK_API void stk_destroy(stk_ctx* ctx);
 STK_API const char* stk_last_error(const stk_ctx* ctx);

@@ -158,6 +158,7 @@ P = C.c_void_p
 U32 = C.c_uint32
 PROTOTYPES = [
     ("stk_version", C.c_char_p, []),
+    ("stk_build_info", C.c_char_p, []),
     ("stk_create", C.c_int, [C.c_int, C.POINTER(P)]),
     ("stk_destroy", None, [P]),
     ("stk_last_error", C.c_char_p, [P]),

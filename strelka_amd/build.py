@@ -7,6 +7,7 @@ device arithmetic agree bit for bit.  Objects go to strelka_amd/_build/.
 from __future__ import annotations
 
 import concurrent.futures as cf
+import hashlib
 import os
 import shutil
 import subprocess
@@ -19,13 +20,31 @@ OBJ = PKG / "_build"
 LIB = PKG / "libstrelka_amd.so"
 ARCH = os.environ.get("STK_OFFLOAD_ARCH", "gfx950")
 
-HOST_SOURCES = ["bvh.cpp", "bvh_api.cpp", "files.cpp", "scene.cpp", "scenegen.cpp"]
+HOST_SOURCES = ["bvh.cpp", "bvh_api.cpp", "files.cpp", "scene.cpp", "scenegen.cpp", "buildinfo.cpp"]
 HIP_SOURCES = ["kernels.hip", "pathtrace.hip", "context.hip", "cache.hip", "gpubuild.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fvisibility=hidden",
           "-Wall", "-Wno-unused-function"]
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-fhip-fp32-correctly-rounded-divide-sqrt", "-Wno-unused-command-line-argument"]
+
+
+def source_hash() -> str:
+    """SHA-256 over every source the library is built from (strelka_amd/csrc
+    and include/stk, each file's path relative to the repository and its
+    bytes, in path order) and the compile flags.  The library carries it
+    (stk_build_info), so a test can tell whether the loaded binary was built
+    from the sources in the tree."""
+    root = PKG.parent
+    files = sorted(p for d in (CSRC, root / "include" / "stk") for p in d.rglob("*")
+                   if p.is_file() and p.suffix in (".hip", ".cpp", ".hpp", ".h"))
+    h = hashlib.sha256()
+    for f in files:
+        h.update(f.relative_to(root).as_posix().encode() + b"\0")
+        h.update(f.read_bytes())
+        h.update(b"\0")
+    h.update(" ".join(COMMON + HIP_FLAGS + [ARCH]).encode())
+    return h.hexdigest()
 
 
 def _hipcc() -> str:
@@ -46,8 +65,10 @@ def build(verbose: bool = False) -> Path:
     OBJ.mkdir(exist_ok=True)
     hipcc = _hipcc()
     jobs = []
+    ident = [f'-DSTK_SOURCE_HASH="{source_hash()}"', f'-DSTK_OFFLOAD_ARCH="{ARCH}"']
     for src in HOST_SOURCES:
-        jobs.append(["g++", *COMMON, "-pthread", "-c", str(CSRC / src), "-o", str(OBJ / (src + ".o"))])
+        extra = ident if src == "buildinfo.cpp" else []
+        jobs.append(["g++", *COMMON, *extra, "-pthread", "-c", str(CSRC / src), "-o", str(OBJ / (src + ".o"))])
     for src in HIP_SOURCES:
         if not (CSRC / src).exists():
             continue
