@@ -130,10 +130,12 @@ def pt_frame_bench(ctx, scene, spp: int, frames: int, d_out: int) -> dict:
 
 def with_graph_state(shape: dict) -> dict:
     """The launch shape plus whether a captured graph is replayed: the library
-    replays one only when every pass runs on one stream (pass_streams 1) and the
-    radiance cache is off."""
+    replays one only when every pass runs on one stream (pass_streams 1), the
+    frame is not one persistent launch (frame_blocks 0) and the radiance cache
+    is off."""
     out = dict(shape)
-    out["graph_replayed"] = bool(shape.get("graph")) and int(shape.get("pass_streams", 1)) <= 1
+    out["graph_replayed"] = (bool(shape.get("graph")) and int(shape.get("pass_streams", 1)) <= 1
+                             and not shape.get("frame_blocks"))
     return out
 
 
@@ -186,8 +188,42 @@ def batch_bench(args) -> dict:
     ctx.sync()
     ms = ctx.elapsed_ms(8, 9) / args.steps
     rays = ctx.ray_totals()
-    ctx.free(d_out)
     dev = ctx.device_info()
+
+    # the fastest per-stage shape against the fastest persistent frame launch
+    # (one launch per frame, workgroups pulling 256-slot chunks), each timed
+    # on the same 16-spp frames
+    def timed(shape):
+        ctx.set_launch_config(**shape)
+        ctx.batch_render_device(p, d_out)
+        ctx.sync()
+        ctx.record(8)
+        for f in range(args.steps):
+            p.pt.frameNumber = 2 + f
+            ctx.batch_render_device(p, d_out)
+        ctx.record(9)
+        ctx.sync()
+        return round(ctx.elapsed_ms(8, 9) / args.steps, 3)
+
+    table = [r["shape"] for r in tuned["table"]]
+    per_stage = next((t for t in table if not t.get("frame_blocks")), dict(tuned["best"], frame_blocks=0))
+    # the frame launch's own knobs (workgroups, slots per chunk), swept on the
+    # autotuner's 4-spp frame
+    from strelka_amd import autotune
+    p4 = sa.render_params(scenes[0].camera, spp=4, max_depth=8, frame=0, sampling=1)
+    frame_sweep = []
+    for bs, per_cu in ((64, 8), (64, 16), (256, 2), (256, 4)):
+        for chunk in (256, 1024, 4096):
+            shape = dict(per_stage, block_size=bs, frame_blocks=per_cu * dev["compute_units"], frame_chunk=chunk)
+            ms4 = autotune.measure(ctx, lambda: ctx.batch_render_device(p4, d_out), shape)
+            frame_sweep.append({"shape": shape, "ms_4spp": round(ms4, 3)})
+            print(f"frame launch {shape}: {ms4:.3f} ms at 4 spp", file=sys.stderr, flush=True)
+    frame = min(frame_sweep, key=lambda r: r["ms_4spp"])["shape"]
+    comparison = {"per_stage": {"shape": per_stage, "ms": timed(per_stage)},
+                  "frame_launch": {"shape": frame, "ms": timed(frame)}, "frame_sweep_4spp": frame_sweep}
+    print(f"launch comparison: {comparison}", file=sys.stderr, flush=True)
+    ctx.set_launch_config(**tuned["best"])
+    ctx.free(d_out)
     ctx.close()
     samples = npix * spp
     return {"metric": "samples/s, batched path tracing of 32 uneven scenes", "value": round(samples / (ms * 1e-3), 1),
@@ -200,6 +236,7 @@ def batch_bench(args) -> dict:
                        "scene_gen_s": round(gen_s, 2)},
             "mrays_per_s": round((rays[0] + rays[1]) / (ms * 1e-3) / 1e6, 1),
             "rays_per_frame": {"closest": rays[0], "shadow": rays[1]},
+            "launch_comparison_16spp": comparison,
             "autotune": {"best": with_graph_state(tuned["best"]), "best_4spp_ms": tuned["ms"], "default_4spp_ms": tuned["default_ms"],
                          "table": tuned["table"],
                          "device": dev["name"], "compute_units": dev["compute_units"]}}

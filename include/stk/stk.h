@@ -315,6 +315,19 @@ typedef struct StkLaunchConfig {
      * image in sample order; with the radiance cache a pass's inference and
      * training also wait for the previous pass's (no graph replay then) */
     uint32_t passStreams;
+    /* n > 0: plain rendering runs each frame as one persistent launch of n
+     * workgroups of 256 threads over every view and sample: a workgroup takes
+     * chunks of frameChunk slots from a counter and runs all of a chunk's
+     * samples (raygen, bounces, shadow rays, reduction) with queues of its own,
+     * stages separated by workgroup barriers (no grid-wide wait); the context
+     * stream only (passStreams and useGraph do not apply).  0: one launch per
+     * stage and bounce.  Renders with the radiance cache always use the
+     * per-stage launches.  stk_get_counters is not kept in this mode. */
+    uint32_t frameBlocks;
+    /* slots per chunk of the frame launch: a multiple of 256 up to 8192 (0:
+     * 256); a chunk larger than the workgroup keeps its lanes busy in the
+     * late bounces */
+    uint32_t frameChunk;
 } StkLaunchConfig;
 STK_API stk_status stk_set_launch_config(stk_ctx* ctx, const StkLaunchConfig* lc);
 STK_API stk_status stk_get_launch_config(stk_ctx* ctx, StkLaunchConfig* lc);

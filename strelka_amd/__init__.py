@@ -651,9 +651,10 @@ class Context:
     # -- launch shape --------------------------------------------------------------
     def set_launch_config(self, block_size: int = 256, tile: int = 8, persistent_blocks: int = 0,
                           graph: bool = False, paths_in_flight: int = 0, binary_walk: bool = False,
-                          stripe_threads: int = 0, pass_streams: int = 1):
+                          stripe_threads: int = 0, pass_streams: int = 1, frame_blocks: int = 0,
+                          frame_chunk: int = 0):
         lc = LaunchConfig(block_size, tile, persistent_blocks, 1 if graph else 0, paths_in_flight,
-                          1 if binary_walk else 0, stripe_threads, pass_streams)
+                          1 if binary_walk else 0, stripe_threads, pass_streams, frame_blocks, frame_chunk)
         self._check(_LIB.stk_set_launch_config(self._h, C.byref(lc)), "stk_set_launch_config")
 
     def launch_config(self) -> dict:
@@ -662,7 +663,8 @@ class Context:
         return {"block_size": lc.blockSize, "tile": lc.tile, "persistent_blocks": lc.persistentBlocks,
                 "graph": bool(lc.useGraph), "paths_in_flight": lc.pathsInFlight,
                 "binary_walk": bool(lc.binaryWalk), "stripe_threads": lc.stripeThreads,
-                "pass_streams": lc.passStreams}
+                "pass_streams": lc.passStreams, "frame_blocks": lc.frameBlocks,
+                "frame_chunk": lc.frameChunk}
 
     def device_info(self) -> dict:
         d = DeviceInfo()

@@ -100,7 +100,8 @@ class BvhInfo(C.Structure):
 class LaunchConfig(C.Structure):
     _fields_ = [("blockSize", C.c_uint32), ("tile", C.c_uint32), ("persistentBlocks", C.c_uint32),
                 ("useGraph", C.c_uint32), ("pathsInFlight", C.c_uint32), ("binaryWalk", C.c_uint32),
-                ("stripeThreads", C.c_uint32), ("passStreams", C.c_uint32)]
+                ("stripeThreads", C.c_uint32), ("passStreams", C.c_uint32), ("frameBlocks", C.c_uint32),
+                ("frameChunk", C.c_uint32)]
 
 
 class DeviceInfo(C.Structure):

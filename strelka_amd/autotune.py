@@ -8,8 +8,10 @@ for the extend kernel, the traversal (compressed 8-wide or packed binary
 BVH), a cap on the threads per queue stripe of the consumer grids (fewer
 blocks that loop, for short late-bounce queues), the number of paths traced
 per pass (so a pass's path state fits the last-level cache), the number of
-pass streams (overlapping a pass's last bounces with the next passes) and
-graph replay of the pass.  None of them
+pass streams (overlapping a pass's last bounces with the next passes), graph
+replay of the pass, and the persistent frame launch (one launch per frame
+whose resident workgroups pull 256-slot chunks and run all of a chunk's
+samples; its size in workgroups per CU).  None of them
 changes a single output bit (the launch-shape tests check that), so the tuner
 only measures time: it runs the caller's workload under each candidate with
 device events on the context stream and keeps the fastest.
@@ -29,9 +31,12 @@ TILES = (1, 4, 8, 16)
 PERSISTENT_PER_CU = (0, 4, 8, 16)
 PATHS_IN_FLIGHT = (0, 1 << 20, 1 << 22)
 STRIPE_THREADS_PER_CU = (0, 8, 16, 32)  # x CUs threads per queue stripe (0: one per entry)
+FRAME_BLOCKS_PER_CU = (0, 4, 8, 16)  # x CUs workgroups of the persistent frame launch (0: per-stage launches)
+FRAME_CHUNKS = (256, 1024, 4096)     # slots per chunk of that launch
 
 DEFAULT = {"block_size": 256, "tile": 8, "persistent_blocks": 0, "graph": True, "paths_in_flight": 0,
-           "binary_walk": False, "stripe_threads": 0, "pass_streams": 1}
+           "binary_walk": False, "stripe_threads": 0, "pass_streams": 1, "frame_blocks": 0,
+           "frame_chunk": 256}
 
 
 def _key(shape: dict) -> tuple:
@@ -66,6 +71,8 @@ def tune(ctx, run: Callable[[], None], total_paths: int, repeats: int = 2, sweep
         "binary_walk": [False, True],
         "stripe_threads": [k * cus for k in STRIPE_THREADS_PER_CU],
         "pass_streams": [1, 2, 3, 4],
+        "frame_blocks": [k * cus for k in FRAME_BLOCKS_PER_CU],
+        "frame_chunk": list(FRAME_CHUNKS),
     }
     seen: dict = {}
 

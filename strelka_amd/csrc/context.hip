@@ -944,9 +944,13 @@ static stk_status renderForestRun(stk_ctx* c, const stk::SceneView& base, std::v
     size_t Q = slots;
     if (c->launch.pathsInFlight && p->cacheMode == 0)
         Q = std::min(slots, ((size_t)c->launch.pathsInFlight + 63) / 64 * 64);
+    // one persistent launch per frame (plain rendering): its workgroups hold
+    // 256 slots each of the path state, on the context stream
+    const bool frameKernel = c->launch.frameBlocks != 0 && p->cacheMode == 0;
+    if (frameKernel) Q = (size_t)c->launch.frameBlocks * (c->launch.frameChunk ? c->launch.frameChunk : 256u);
     // several pass streams (with the cache, a pass's bounces overlap the
     // previous pass's inference and training, which its own wait for)
-    const uint32_t nstreams = std::max<uint32_t>(1u, c->launch.passStreams);
+    const uint32_t nstreams = frameKernel ? 1u : std::max<uint32_t>(1u, c->launch.passStreams);
     const bool multi = nstreams > 1;
     if (!multi && (st = ensurePathState(c, c->ps, c->psBufs, Q)) != STK_OK) return st;
     for (uint32_t i = 0; multi && i < nstreams; ++i) {
@@ -1053,7 +1057,7 @@ static stk_status renderForestRun(stk_ctx* c, const stk::SceneView& base, std::v
     // One sample pass is captured once as a graph and replayed per sample (the
     // pass number and frame come from device memory).  Training passes of the
     // cache keep host-side Adam state and run eagerly.
-    const bool graph = c->useGraph && pc.mode == 0 && !multi;
+    const bool graph = c->useGraph && pc.mode == 0 && !multi && !frameKernel;
     if (graph) {
         // everything the captured kernels take by value
         std::vector<uint64_t> key = {(uint64_t)sv.bvh.packed, (uint64_t)sv.bvh.tris, (uint64_t)sv.bvh.refNodes,
@@ -1085,7 +1089,10 @@ static stk_status renderForestRun(stk_ctx* c, const stk::SceneView& base, std::v
     }
     uint32_t k = 0;              // pass number of this call
     hipError_t issued = hipSuccess; // first failure of the multi-stream passes
-    for (int s = 0; s < p->pt.spp && issued == hipSuccess; ++s) {
+    if (frameKernel)
+        STK_HIP(c, stk::launchPathTraceFrame(sv, lp, c->launch, c->ps, accum, samples, p->pt.frameNumber,
+                                             (uint32_t)slots, c->stream));
+    for (int s = 0; !frameKernel && s < p->pt.spp && issued == hipSuccess; ++s) {
         for (size_t begin = 0; begin < slots && issued == hipSuccess; begin += Q, ++k) {
             stk::PassState pass{};
             pass.frameNumber = p->pt.frameNumber;
@@ -1640,6 +1647,12 @@ STK_API stk_status stk_set_launch_config(stk_ctx* c, const StkLaunchConfig* lc)
         return fail(c, STK_E_INVALID, "stk_set_launch_config: passStreams must be 0..4");
     if (lc->pathsInFlight != 0 && lc->pathsInFlight < 4096)
         return fail(c, STK_E_INVALID, "stk_set_launch_config: pathsInFlight must be 0 or at least 4096");
+    if (lc->frameBlocks > (1u << 16))
+        return fail(c, STK_E_INVALID, "stk_set_launch_config: frameBlocks must be at most 65536");
+    if (lc->frameChunk % 256 != 0 || lc->frameChunk > 8192)
+        return fail(c, STK_E_INVALID, "stk_set_launch_config: frameChunk must be a multiple of 256, at most 8192");
+    if ((uint64_t)lc->frameBlocks * std::max<uint32_t>(256u, lc->frameChunk) > (1ull << 26))
+        return fail(c, STK_E_INVALID, "stk_set_launch_config: frameBlocks x frameChunk too large");
     c->launch.blockSize = lc->blockSize;
     c->launch.tile = lc->tile;
     c->launch.persistentBlocks = lc->persistentBlocks;
@@ -1647,6 +1660,8 @@ STK_API stk_status stk_set_launch_config(stk_ctx* c, const StkLaunchConfig* lc)
     c->launch.binaryWalk = lc->binaryWalk ? 1u : 0u;
     c->launch.stripeThreads = lc->stripeThreads;
     c->launch.passStreams = std::max<uint32_t>(1u, lc->passStreams);
+    c->launch.frameBlocks = lc->frameBlocks;
+    c->launch.frameChunk = lc->frameChunk;
     c->useGraph = lc->useGraph != 0;
     invalidateGraph(c);
     return STK_OK;
@@ -1681,6 +1696,8 @@ STK_API stk_status stk_get_launch_config(stk_ctx* c, StkLaunchConfig* lc)
     lc->binaryWalk = c->launch.binaryWalk;
     lc->stripeThreads = c->launch.stripeThreads;
     lc->passStreams = c->launch.passStreams;
+    lc->frameBlocks = c->launch.frameBlocks;
+    lc->frameChunk = c->launch.frameChunk;
     return STK_OK;
 }
 

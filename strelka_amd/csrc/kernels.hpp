@@ -144,6 +144,8 @@ struct LaunchConfig {
     uint32_t binaryWalk = 0;       // 1: packed BVH2 walks instead of the compressed 8-wide ones
     uint32_t stripeThreads = 0;    // threads per stripe of the consumer grids (0: one per entry)
     uint32_t passStreams = 1;      // n > 1: pass k runs on stream k % n
+    uint32_t frameBlocks = 0;      // n > 0: one persistent launch of n workgroups per frame (plain rendering)
+    uint32_t frameChunk = 0;       // slots per chunk of that launch (a multiple of 256; 0: 256)
 };
 
 // Stack entries per lane of the wide closest-hit kernel a flat tree takes (11
@@ -177,6 +179,15 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
                                float4* accum, float* samples, CacheBuffers* cb, hipStream_t s,
                                hipEvent_t finalizeAfter = nullptr, hipEvent_t finalizeDone = nullptr,
                                hipEvent_t cacheAfter = nullptr, hipEvent_t cacheDone = nullptr);
+// A whole frame (every view, lp.spp samples) in one persistent launch of
+// lc.frameBlocks workgroups (after a one-wave reset of the chunk counter):
+// each workgroup takes chunks of lc.frameChunk slots and runs all their
+// samples with queues private to it, adding into accum in sample order.  ps
+// must hold frameBlocks x frameChunk slots and queue entries; plain rendering only (cache
+// mode 0).  The image equals the per-stage passes' bit for bit.
+hipError_t launchPathTraceFrame(const SceneView& sc, const PtLaunch& lp, const LaunchConfig& lc, PathState& ps,
+                                float4* accum, float* samples, uint32_t frameNumber, uint32_t totalSlots,
+                                hipStream_t s);
 // Plain 16-B-per-lane copy kernel (bytes must be a multiple of 16).
 hipError_t launchCopy(void* dst, const void* src, size_t bytes, hipStream_t s);
 // RGBA32F pixels to a tightly packed RGBA32F (format 0) or RGBA8 (format 1)

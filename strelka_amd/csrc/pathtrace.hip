@@ -115,6 +115,42 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
     return v;
 }
 
+// One slot's camera ray: the pass's sample of the slot's pixel, or an empty
+// path for a padding slot.  Per-slot arrays are indexed by `local`; `slot` is
+// the global slot (which view and pixel).  Returns whether the path starts.
+__device__ __forceinline__ bool raygen_one(const SceneView& sc, const PtLaunch& lp, PathState& ps,
+                                           const PassState& pass, uint32_t local, uint32_t slot)
+{
+    const uint32_t sid = scene_of_slot(sc, slot);
+    const SceneDesc& d = sc.scenes[sid];
+    const uint32_t T = sc.tile, inView = slot - d.slotBase;
+    const uint32_t tileIdx = inView / (T * T), within = inView % (T * T);
+    const uint32_t px = (tileIdx % d.tilesX) * T + within % T;
+    const uint32_t py = (tileIdx / d.tilesX) * T + within / T;
+    bool valid = px < d.width && py < d.height;
+    // mode 4 traces the training paths only (cache warm-up)
+    if (lp.cache.mode == 4 && !is_train(lp.cache, pass, local)) valid = false;
+    ps.scene[local] = sid;
+    ps.pixel[local] = valid ? d.pixelBase + py * d.width + px : STK_INVALID_INDEX;
+    if (valid) {
+        const uint32_t frame = pass.frameNumber * lp.spp + pass.sampleNum;
+        uint32_t rng = init_rng(px, py, d.width, frame);
+        float4 o, dir;
+        camera_ray(d.cam, px, py, rng, frame, o, dir);
+        ps.rayO[local] = o;
+        ps.rayD[local] = dir;
+        ps.thr[local] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        ps.col[local] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        ps.rng[local] = rng;
+    } else {
+        ps.col[local] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    // padding slots too: the target kernel reads every training slot's count
+    // (the cache runs with one pass over all slots, so local == slot there)
+    if (is_train(lp.cache, pass, local)) lp.cache.trainCount[local / lp.cache.trainStride] = 0;
+    return valid;
+}
+
 // Path state arrays are indexed by the slot's position in the pass (`local`);
 // a pass covers global slots [slotBegin, slotBegin + slotCount).
 __global__ __launch_bounds__(kBlock) void k_pt_raygen(SceneView sc, PtLaunch lp, PathState ps)
@@ -123,36 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_pt_raygen(SceneView sc, PtLaunch lp,
     const uint32_t local = blockIdx.x * kBlock + threadIdx.x;
     const PassState pass = *ps.pass;
     bool valid = false;
-    if (local < pass.slotCount) {
-        const uint32_t slot = pass.slotBegin + local;
-        const uint32_t sid = scene_of_slot(sc, slot);
-        const SceneDesc& d = sc.scenes[sid];
-        const uint32_t T = sc.tile, inView = slot - d.slotBase;
-        const uint32_t tileIdx = inView / (T * T), within = inView % (T * T);
-        const uint32_t px = (tileIdx % d.tilesX) * T + within % T;
-        const uint32_t py = (tileIdx / d.tilesX) * T + within / T;
-        valid = px < d.width && py < d.height;
-        // mode 4 traces the training paths only (cache warm-up)
-        if (lp.cache.mode == 4 && !is_train(lp.cache, pass, local)) valid = false;
-        ps.scene[local] = sid;
-        ps.pixel[local] = valid ? d.pixelBase + py * d.width + px : STK_INVALID_INDEX;
-        if (valid) {
-            const uint32_t frame = pass.frameNumber * lp.spp + pass.sampleNum;
-            uint32_t rng = init_rng(px, py, d.width, frame);
-            float4 o, dir;
-            camera_ray(d.cam, px, py, rng, frame, o, dir);
-            ps.rayO[local] = o;
-            ps.rayD[local] = dir;
-            ps.thr[local] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-            ps.col[local] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            ps.rng[local] = rng;
-        } else {
-            ps.col[local] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
-        // padding slots too: the target kernel reads every training slot's count
-        // (the cache runs with one pass over all slots, so local == slot there)
-        if (is_train(lp.cache, pass, local)) lp.cache.trainCount[local / lp.cache.trainStride] = 0;
-    }
+    if (local < pass.slotCount) valid = raygen_one(sc, lp, ps, pass, local, pass.slotBegin + local);
     const uint32_t stripe = blockIdx.x % kStripes;
     const uint32_t qi = wave_push(&ps.stripes[stripe], valid);
     if (valid) ps.queue[0][stripe * ps.segCap + qi] = local;
@@ -278,8 +285,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) v
 
 // One queue entry of the shade step (active: q is a valid entry; every lane of
 // the block calls it, for the wave-wide appends at the end).
+// The continuing path goes to the next path queue at nextBase + (count
+// appended through nextCnt), a shadow ray to the shadow queue likewise.
 __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& lp, PathState& ps, int depth, bool active,
-                                          uint32_t q)
+                                          uint32_t q, uint32_t* nextCnt, uint32_t nextBase, uint32_t* shadowCnt,
+                                          uint32_t shadowBase)
 {
     const int cur = depth & 1;
     bool cont = false, shadow = false, cacheQuery = false;
@@ -496,12 +506,11 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
             ps.rayD[slot] = make_float4(nD.x, nD.y, nD.z, nPdf);
         }
     }
-    const uint32_t stripe = blockIdx.x % kStripes;
-    const uint32_t qi = wave_push(&ps.stripes[(1 - cur) * kStripes + stripe], cont);
-    if (cont) ps.queue[1 - cur][stripe * ps.segCap + qi] = slot;
-    const uint32_t si = wave_push(&ps.stripes[2 * kStripes + stripe], shadow);
+    const uint32_t qi = wave_push(nextCnt, cont);
+    if (cont) ps.queue[1 - cur][nextBase + qi] = slot;
+    const uint32_t si = wave_push(shadowCnt, shadow);
     if (shadow) {
-        const uint32_t se = stripe * ps.segCap + si;
+        const uint32_t se = shadowBase + si;
         ps.shadowQ[se] = slot;
         ps.shO[se] = make_float4(sO.x, sO.y, sO.z, sT);
         ps.shD[se] = make_float4(sD.x, sD.y, sD.z, 0.0f);
@@ -517,10 +526,14 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
 __global__ __launch_bounds__(kBlock) void k_pt_shade(SceneView sc, PtLaunch lp, PathState ps, int depth)
 {
     if (pass_aborted(ps)) return;
-    const uint32_t stripe = blockIdx.x % kStripes, cnt = ps.stripes[(depth & 1) * kStripes + stripe];
+    const int cur = depth & 1;
+    const uint32_t stripe = blockIdx.x % kStripes, cnt = ps.stripes[cur * kStripes + stripe];
+    uint32_t* nextCnt = &ps.stripes[(1 - cur) * kStripes + stripe];
+    uint32_t* shadowCnt = &ps.stripes[2 * kStripes + stripe];
+    const uint32_t segBase = stripe * ps.segCap;
     for (uint32_t base = stripe_first(kBlock); base < cnt; base += stripe_step(kBlock)) {
         const uint32_t local = base + threadIdx.x;
-        shade_one(sc, lp, ps, depth, local < cnt, stripe * ps.segCap + local);
+        shade_one(sc, lp, ps, depth, local < cnt, segBase + local, nextCnt, segBase, shadowCnt, segBase);
     }
 }
 
@@ -532,7 +545,7 @@ constexpr uint32_t kMaxEmitters = 64;
 // refCompat: the reference's rule (the first hit leaf in preorder decides,
 // and a light hit counts as visible, Q5); otherwise any non-emitter triangle
 // before tmax occludes (the emitter walks with no emitter triangles given).
-template <bool TL>
+template <bool TL, int BLOCK = kBlock>
 __device__ __forceinline__ void connect_one(const SceneView& sc, PathState& ps, uint32_t s, uint32_t* stack, int binary,
                                             int refCompat)
 {
@@ -543,10 +556,10 @@ __device__ __forceinline__ void connect_one(const SceneView& sc, PathState& ps, 
     bool vis;
     if (TL) {
         vis = visible_two_level(sc.bvh, sc.bvh.emitters + 3 * (size_t)desc.emitBase, emitCount, r,
-                                stack + threadIdx.x, stack + kWideStack * kBlock + threadIdx.x, kBlock);
+                                stack + threadIdx.x, stack + kWideStack * BLOCK + threadIdx.x, BLOCK);
     } else if (!binary && desc.wideRoot != STK_INVALID_INDEX && desc.emitCount <= kMaxEmitters) {
         vis = visible_wide(sc.bvh, desc.wideRoot, sc.bvh.emitters + 3 * (size_t)desc.emitBase, emitCount, r,
-                           stack + threadIdx.x, stack + kWideStack * kBlock + threadIdx.x, kBlock);
+                           stack + threadIdx.x, stack + kWideStack * BLOCK + threadIdx.x, BLOCK);
     } else if (!refCompat) {
         vis = !occluded_ref(sc.bvh, desc.hdr, desc.refRoot, r);
     } else {
@@ -594,27 +607,34 @@ __global__ __launch_bounds__(kBlock) void k_cache_apply(PtLaunch lp, PathState p
     ps.col[slot] = f4(col);
 }
 
-__global__ __launch_bounds__(kBlock) void k_pt_finalize(PtLaunch lp, PathState ps, float4* __restrict__ accum,
-                                                        float* __restrict__ samples)
+// One slot's sample into its pixel (reduction.hlsl:11-32: color += invSpp *
+// sample, in sample order) and, when kept, into the samples buffer.
+__device__ __forceinline__ void finalize_one(const PtLaunch& lp, const PathState& ps, float4* __restrict__ accum,
+                                             float* __restrict__ samples, uint32_t local, uint32_t sampleNum)
 {
-    const uint32_t local = blockIdx.x * kBlock + threadIdx.x;
-    if (pass_aborted(ps) || local >= ps.pass->slotCount) return;
     const uint32_t pixel = ps.pixel[local];
     if (pixel == STK_INVALID_INDEX) return;
     const float4 c = ps.col[local];
     float4 a = accum[pixel];
-    // reduction.hlsl:11-32: color += invSpp * sample, in sample order
     a.x = a.x + lp.invSpp * c.x;
     a.y = a.y + lp.invSpp * c.y;
     a.z = a.z + lp.invSpp * c.z;
     a.w = 1.0f;
     accum[pixel] = a;
     if (samples) {
-        float* o = samples + ((size_t)pixel * lp.spp + ps.pass->sampleNum) * 3;
+        float* o = samples + ((size_t)pixel * lp.spp + sampleNum) * 3;
         o[0] = c.x;
         o[1] = c.y;
         o[2] = c.z;
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pt_finalize(PtLaunch lp, PathState ps, float4* __restrict__ accum,
+                                                        float* __restrict__ samples)
+{
+    const uint32_t local = blockIdx.x * kBlock + threadIdx.x;
+    if (pass_aborted(ps) || local >= ps.pass->slotCount) return;
+    finalize_one(lp, ps, accum, samples, local, ps.pass->sampleNum);
 }
 
 // Training targets: the radiance a training path gathered after leaving vertex
@@ -656,6 +676,111 @@ __global__ __launch_bounds__(kBlock) void k_cache_targets(PtLaunch lp, PathState
     if (emit && i < pc.trainCap) pc.trainBatch[i] = rec;
 }
 
+// One launch per frame over every view and every sample.  Resident workgroups
+// of BLOCK threads (one wave or four) take chunks of `chunk` consecutive slots
+// (a multiple of 256; 256 slots are 4 raygen tiles of 8x8 at the default tile) from a
+// counter and run all of a chunk's samples, in sample order, through raygen,
+// the bounces (closest hit, shading, shadow rays) and the reduction, with the
+// path state, queues and hits in a region private to the workgroup (entries
+// base .. base + chunk - 1) and the queue counters in LDS; a stage loops over
+// its queue BLOCK entries at a time, so a chunk larger than the workgroup
+// keeps its lanes busy in the late bounces.  Stages are separated by
+// workgroup barriers only: no wave ever waits for another workgroup, and
+// every workgroup leaves when the counter passes the last chunk (or at the
+// next sample once the watchdog's word is set).  A chunk's slots are its
+// pixels alone, so every pixel adds its samples in sample order and each path
+// runs the same per-path functions as the per-stage launches: the image is
+// theirs bit for bit.  Plain rendering only (the radiance cache trains on
+// whole frames).
+template <int BLOCK, bool TL>
+__global__ __launch_bounds__(BLOCK) void k_pt_frame(SceneView sc, PtLaunch lp, PathState ps, uint32_t frameNumber,
+                                                     uint32_t totalSlots, uint32_t chunk, int extendBinary,
+                                                     int connectBinary, float4* __restrict__ accum,
+                                                     float* __restrict__ samples)
+{
+    __shared__ uint32_t stack[kStackDepth * BLOCK];
+    __shared__ uint32_t counts[3]; // path queues 0 and 1, shadow queue
+    __shared__ uint32_t taken, stopped;
+    const uint32_t t = threadIdx.x;
+    const uint32_t base = blockIdx.x * chunk; // this workgroup's private slots and queue entries
+    const uint32_t numChunks = (totalSlots + chunk - 1) / chunk;
+    unsigned long long closest = 0, shadows = 0; // rays this workgroup traced (thread 0)
+    bool stop = false;
+    while (!stop) {
+        if (t == 0) taken = pass_aborted(ps) ? numChunks : atomicAdd(&ps.work[0], 1u);
+        __syncthreads();
+        const uint32_t c = taken;
+        if (c >= numChunks) break;
+        const uint32_t first = c * chunk; // the chunk's first global slot
+        PassState pass{};
+        pass.frameNumber = frameNumber;
+        pass.slotBegin = first;
+        pass.slotCount = min(chunk, totalSlots - first);
+        for (uint32_t smp = 0; smp < lp.spp; ++smp) {
+            pass.sampleNum = smp;
+            if (t == 0) {
+                counts[0] = counts[1] = counts[2] = 0u;
+                uint32_t a = 0u;
+                if (ps.abortHost &&
+                    __hip_atomic_load(ps.abortHost, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u) {
+                    ps.counters[kAbortCounter] = 1u;
+                    *ps.abortAny = 1u;
+                    a = 1u;
+                }
+                stopped = a;
+            }
+            __syncthreads();
+            if (stopped) {
+                stop = true;
+                break;
+            }
+            for (uint32_t i = t; i < chunk; i += BLOCK) { // the loop bound is uniform: whole waves append
+                const uint32_t ws = base + i, slot = first + i;
+                bool valid = false;
+                if (slot < totalSlots)
+                    valid = raygen_one(sc, lp, ps, pass, ws, slot);
+                else
+                    ps.pixel[ws] = STK_INVALID_INDEX;
+                const uint32_t qi = wave_push(&counts[0], valid);
+                if (valid) ps.queue[0][base + qi] = ws;
+            }
+            __syncthreads();
+            for (int depth = 0; depth < (int)lp.maxDepth; ++depth) {
+                const int cur = depth & 1;
+                const uint32_t nq = counts[cur];
+                if (nq == 0u) break; // every thread read the same count after the barrier
+                for (uint32_t e = t; e < nq; e += BLOCK)
+                    extend_one<BLOCK, TL>(sc, ps, cur, base + e, stack + t, extendBinary);
+                if (t == 0) {
+                    counts[1 - cur] = 0u;
+                    counts[2] = 0u;
+                    closest += nq;
+                }
+                __syncthreads();
+                const uint32_t rounds = (nq + BLOCK - 1) / BLOCK;
+                for (uint32_t r = 0; r < rounds; ++r) {
+                    const uint32_t e = r * BLOCK + t;
+                    shade_one(sc, lp, ps, depth, e < nq, base + e, &counts[1 - cur], base, &counts[2], base);
+                }
+                __syncthreads();
+                const uint32_t ns = counts[2];
+                if (t == 0) shadows += ns;
+                for (uint32_t e = t; e < ns; e += BLOCK)
+                    connect_one<TL, BLOCK>(sc, ps, base + e, stack, connectBinary, (int)lp.refCompat);
+                __syncthreads();
+            }
+            for (uint32_t i = t; i < chunk; i += BLOCK)
+                if (first + i < totalSlots) finalize_one(lp, ps, accum, samples, base + i, smp);
+            __syncthreads();
+        }
+    }
+    if (t == 0) {
+        unsigned long long* tot = ray_totals(ps);
+        atomicAdd(&tot[0], closest);
+        atomicAdd(&tot[1], shadows);
+    }
+}
+
 inline uint32_t blocks(size_t n, uint32_t b = kBlock) { return (uint32_t)((n + b - 1) / b); }
 
 // Stripe-major consumer grid of B-thread blocks (segCap is a multiple of 256).
@@ -694,6 +819,36 @@ void launchExtend(const SceneView& sc, PathState& ps, int depth, const LaunchCon
 hipError_t launchSetPass(PathState& ps, const PassState& v, hipStream_t s)
 {
     hipLaunchKernelGGL(k_pass_set, dim3(1), dim3(64), 0, s, ps.pass, v);
+    return hipGetLastError();
+}
+
+hipError_t launchPathTraceFrame(const SceneView& sc, const PtLaunch& lp, const LaunchConfig& lc, PathState& ps,
+                                float4* accum, float* samples, uint32_t frameNumber, uint32_t totalSlots,
+                                hipStream_t s)
+{
+    const uint32_t chunk = lc.frameChunk ? lc.frameChunk : (uint32_t)kBlock;
+    const size_t region = (size_t)lc.frameBlocks * chunk;
+    if (lc.frameBlocks == 0 || chunk % 256 != 0 || region > ps.capacity || region > ps.queueCapacity ||
+        lp.cache.mode != 0 || totalSlots == 0)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pass_reset, dim3(1), dim3(64), 0, s, ps); // counters, chunk counter, abort word
+    const int eb = (lc.binaryWalk && sc.bvh.packedOk) ? 1 : 0, cb = lc.binaryWalk ? 1 : 0;
+    // one-wave workgroups for blockSize 64 (a finished wave waits for no
+    // other wave at the stage barriers), else four waves
+    if (lc.blockSize == 64) {
+        if (sc.bvh.tlas)
+            hipLaunchKernelGGL((k_pt_frame<64, true>), dim3(lc.frameBlocks), dim3(64), 0, s, sc, lp, ps, frameNumber,
+                               totalSlots, chunk, eb, 0, accum, samples);
+        else
+            hipLaunchKernelGGL((k_pt_frame<64, false>), dim3(lc.frameBlocks), dim3(64), 0, s, sc, lp, ps,
+                               frameNumber, totalSlots, chunk, eb, cb, accum, samples);
+    } else if (sc.bvh.tlas) {
+        hipLaunchKernelGGL((k_pt_frame<kBlock, true>), dim3(lc.frameBlocks), dim3(kBlock), 0, s, sc, lp, ps,
+                           frameNumber, totalSlots, chunk, eb, 0, accum, samples);
+    } else {
+        hipLaunchKernelGGL((k_pt_frame<kBlock, false>), dim3(lc.frameBlocks), dim3(kBlock), 0, s, sc, lp, ps,
+                           frameNumber, totalSlots, chunk, eb, cb, accum, samples);
+    }
     return hipGetLastError();
 }
 

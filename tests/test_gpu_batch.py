@@ -106,7 +106,11 @@ SHAPES = [dict(block_size=64, tile=1), dict(block_size=128, tile=16), dict(block
           dict(block_size=128, tile=8, pass_streams=2, paths_in_flight=20000, persistent_blocks=1024),
           dict(block_size=64, tile=4, pass_streams=3, paths_in_flight=8192),
           dict(block_size=256, tile=8, pass_streams=4, paths_in_flight=50000, stripe_threads=4096),
-          dict(block_size=64, tile=8, pass_streams=1, graph=True)]
+          dict(block_size=64, tile=8, pass_streams=1, graph=True),
+          # one persistent launch per frame: resident workgroups pull chunks of slots
+          dict(block_size=64, tile=8, frame_blocks=768), dict(block_size=256, tile=4, frame_blocks=3, frame_chunk=2048),
+          dict(block_size=64, tile=8, frame_blocks=512, frame_chunk=1024),
+          dict(block_size=128, tile=8, frame_blocks=256, binary_walk=True, pass_streams=4, graph=True)]
 
 
 def test_launch_shapes_give_identical_images(scenes):
