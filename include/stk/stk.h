@@ -65,6 +65,13 @@ typedef struct StkBvhOptions {
     uint32_t numBins;    /* SAH bins per axis (0 = 32) */
     float traversalCost; /* SAH costs, reference uses 1 and 1 */
     float intersectionCost;
+    /* > 0: spatial splits (the reference asks Embree for them,
+     * src/render/bvh.cpp:80-105,154): where an object split's children
+     * overlap by more than this fraction of the root box's area, a binned
+     * spatial split may clip straddling triangles into both children (a
+     * triangle then sits in several leaves; references grow by at most 30%).
+     * 0 = object splits only.  Host builds of flat trees only. */
+    float spatialSplitAlpha;
 } StkBvhOptions;
 
 /* Header of the packed BVH: root box and root reference. */

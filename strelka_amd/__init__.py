@@ -250,8 +250,8 @@ class HostBvh:
         self.num_emitters = int(_LIB.stk_bvh_num_emitters(handle))
 
     @classmethod
-    def build(cls, scene: SceneData, max_depth: int = 32, bins: int = 32) -> "HostBvh":
-        opts = BvhOptions(max_depth, bins, 1.0, 1.0)
+    def build(cls, scene: SceneData, max_depth: int = 32, bins: int = 32, spatial_split_alpha: float = 0.0) -> "HostBvh":
+        opts = BvhOptions(max_depth, bins, 1.0, 1.0, spatial_split_alpha)
         h = C.c_void_p()
         st = _LIB.stk_bvh_build_host(_ptr(scene.vb), scene.vb.size, _ptr(scene.ib), scene.ib.size, _ptr(scene.inst),
                                      scene.inst.size, C.byref(opts), C.byref(h))
@@ -309,7 +309,7 @@ class HostTwoLevel:
 
     @classmethod
     def build(cls, scene: SceneData, max_depth: int = 32, bins: int = 32) -> "HostTwoLevel":
-        opts = BvhOptions(max_depth, bins, 1.0, 1.0)
+        opts = BvhOptions(max_depth, bins, 1.0, 1.0, 0.0)
         h = C.c_void_p()
         st = _LIB.stk_tlas_build_host(_ptr(scene.vb), scene.vb.size, _ptr(scene.ib), scene.ib.size,
                                       _ptr(scene.inst), scene.inst.size, C.byref(opts), C.byref(h))
@@ -453,10 +453,10 @@ class Context:
         if s.textures is not None:
             self.set_textures(s.textures, s.texels)
 
-    def load_scene(self, s, max_depth: int = 32, bins: int = 32):
+    def load_scene(self, s, max_depth: int = 32, bins: int = 32, spatial_split_alpha: float = 0.0):
         """stk_set_scene: the whole container (a Scene, or a generated
         SceneData) goes to the device and the flat BVH is built over it."""
-        opts = BvhOptions(max_depth, bins, 1.0, 1.0)
+        opts = BvhOptions(max_depth, bins, 1.0, 1.0, spatial_split_alpha)
         self._check(_LIB.stk_set_scene(self._h, s._handle, C.byref(opts)), "stk_set_scene")
 
     def set_textures(self, textures: np.ndarray, texels: np.ndarray):
@@ -466,8 +466,12 @@ class Context:
         self._check(_LIB.stk_set_textures(self._h, _ptr(textures), textures.size, _ptr(texels), texels.shape[0]),
                     "stk_set_textures")
 
-    def build_bvh(self, max_depth: int = 32, bins: int = 32):
-        opts = BvhOptions(max_depth, bins, 1.0, 1.0)
+    def build_bvh(self, max_depth: int = 32, bins: int = 32, spatial_split_alpha: float = 0.0):
+        """stk_build_bvh: the flat tree by binned SAH on the host;
+        spatial_split_alpha > 0 allows spatial splits (1e-5 is the usual
+        threshold: children overlapping by more than that fraction of the
+        root's area)."""
+        opts = BvhOptions(max_depth, bins, 1.0, 1.0, spatial_split_alpha)
         self._check(_LIB.stk_build_bvh(self._h, C.byref(opts)), "stk_build_bvh")
 
     def build_bvh_gpu(self, method: str = "lbvh"):
@@ -478,7 +482,7 @@ class Context:
 
     def build_two_level(self, max_depth: int = 32, bins: int = 32):
         """stk_build_two_level: per-mesh trees under a tree of instances."""
-        opts = BvhOptions(max_depth, bins, 1.0, 1.0)
+        opts = BvhOptions(max_depth, bins, 1.0, 1.0, 0.0)
         self._check(_LIB.stk_build_two_level(self._h, C.byref(opts)), "stk_build_two_level")
 
     def set_bvh(self, nodes: np.ndarray):
@@ -691,8 +695,8 @@ class Context:
         self._check(_LIB.stk_batch_add_scene(self._h, C.byref(b), C.byref(idx)), "stk_batch_add_scene")
         return int(idx.value)
 
-    def batch_commit(self, max_depth: int = 32, bins: int = 32):
-        opts = BvhOptions(max_depth, bins, 1.0, 1.0)
+    def batch_commit(self, max_depth: int = 32, bins: int = 32, spatial_split_alpha: float = 0.0):
+        opts = BvhOptions(max_depth, bins, 1.0, 1.0, spatial_split_alpha)
         self._check(_LIB.stk_batch_commit(self._h, C.byref(opts)), "stk_batch_commit")
         self._batch_views = []
         i = 0

@@ -84,7 +84,7 @@ assert CACHE_RECORD.itemsize == 64
 
 class BvhOptions(C.Structure):
     _fields_ = [("maxDepth", C.c_uint32), ("numBins", C.c_uint32), ("traversalCost", C.c_float),
-                ("intersectionCost", C.c_float)]
+                ("intersectionCost", C.c_float), ("spatialSplitAlpha", C.c_float)]
 
 
 class BvhHeader(C.Structure):

@@ -11,6 +11,7 @@
 #include "bvh.hpp"
 
 #include <algorithm>
+#include <memory>
 #include <array>
 #include <atomic>
 #include <map>
@@ -26,6 +27,7 @@ StkBvhOptions defaultBvhOptions()
     o.numBins = 32;
     o.traversalCost = 1.0f;
     o.intersectionCost = 1.0f;
+    o.spatialSplitAlpha = 0.0f;
     return o;
 }
 
@@ -268,6 +270,431 @@ private:
     std::atomic<uint32_t> maxDepthSeen_{0};
 };
 
+// ---- spatial splits ----------------------------------------------------------
+//
+// Binned SAH with spatial splits (Stich, Friedrich and Dietrich, "Spatial
+// Splits in Bounding Volume Hierarchies", HPG 2009): a node's references are
+// split by the best object partition of their centroids, or - when the two
+// children of that partition overlap by more than alpha of the root's area -
+// by the best of nbins planes across the node box, the triangles straddling
+// the plane clipped into both children (each side keeps the bounds of its part
+// of the triangle within the reference's box).  Leaves still hold one
+// reference of one triangle, so the tree has 2R - 1 nodes for R references
+// (a triangle may sit in several leaves); references grow by at most the
+// split budget.  The tree is built with per-node reference lists and then
+// laid out in the same preorder with skip links as the in-place builder.
+
+struct SNode {
+    float lo[3], hi[3];
+    std::unique_ptr<SNode> left, right;
+    uint32_t tri = STK_INVALID_INDEX; // leaf: the input triangle
+    uint32_t leaves = 1;
+};
+
+// The bounds of triangle v (double) clipped to the box [lo, hi], rounded
+// outwards to float; false when the clipped part is empty.
+bool clippedBounds(const double v[3][3], const double lo[3], const double hi[3], float olo[3], float ohi[3])
+{
+    double poly[2][12][3];
+    int n = 3, cur = 0;
+    for (int k = 0; k < 3; ++k)
+        for (int a = 0; a < 3; ++a) poly[0][k][a] = v[k][a];
+    for (int a = 0; a < 3 && n > 0; ++a)
+        for (int side = 0; side < 2 && n > 0; ++side) {
+            const double plane = side ? hi[a] : lo[a];
+            int m = 0;
+            for (int k = 0; k < n; ++k) {
+                const double* p = poly[cur][k];
+                const double* q = poly[cur][(k + 1) % n];
+                const bool pin = side ? p[a] <= plane : p[a] >= plane;
+                const bool qin = side ? q[a] <= plane : q[a] >= plane;
+                if (pin) {
+                    for (int c = 0; c < 3; ++c) poly[1 - cur][m][c] = p[c];
+                    ++m;
+                }
+                if (pin != qin) {
+                    const double t = (plane - p[a]) / (q[a] - p[a]);
+                    for (int c = 0; c < 3; ++c) poly[1 - cur][m][c] = p[c] + (q[c] - p[c]) * t;
+                    poly[1 - cur][m][a] = plane;
+                    ++m;
+                }
+            }
+            n = m;
+            cur = 1 - cur;
+        }
+    if (n == 0) return false;
+    double blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < n; ++k)
+        for (int a = 0; a < 3; ++a) {
+            blo[a] = std::min(blo[a], poly[cur][k][a]);
+            bhi[a] = std::max(bhi[a], poly[cur][k][a]);
+        }
+    for (int a = 0; a < 3; ++a) {
+        // a margin of 1e-6 relative outwards, then rounded outwards to float:
+        // the parts' boxes overlap across the plane
+        const double m = 1e-6 * (1.0 + std::max(std::fabs(blo[a]), std::fabs(bhi[a])));
+        olo[a] = std::nextafter((float)(blo[a] - m), -INFINITY);
+        ohi[a] = std::nextafter((float)(bhi[a] + m), INFINITY);
+    }
+    return true;
+}
+
+class SpatialBuilder {
+public:
+    SpatialBuilder(const std::vector<BvhInputTri>& tris, const StkBvhOptions& o, std::vector<StkBvhNode>& nodes)
+        : tris_(tris), opts_(o), nodes_(nodes)
+    {
+    }
+
+    void run()
+    {
+        const uint32_t n = (uint32_t)tris_.size();
+        std::vector<PrimRef> refs(n);
+        Box all;
+        for (uint32_t i = 0; i < n; ++i) {
+            const BvhInputTri& t = tris_[i];
+            PrimRef& r = refs[i];
+            const float* v0 = &t.v[0].x;
+            const float* v1 = &t.v[1].x;
+            const float* v2 = &t.v[2].x;
+            for (int a = 0; a < 3; ++a) {
+                r.lo[a] = std::min(v0[a], std::min(v1[a], v2[a]));
+                r.hi[a] = std::max(v0[a], std::max(v1[a], v2[a])) + 1e-6f;
+                r.c[a] = (r.lo[a] + r.hi[a]) * 0.5f;
+            }
+            r.id = i;
+            all.grow(r.lo, r.hi);
+        }
+        rootArea_ = all.halfArea();
+        std::unique_ptr<SNode> root = build(std::move(refs), 0, (uint64_t)((double)n * 0.3));
+        nodes_.assign(2 * (size_t)root->leaves - 1, StkBvhNode{});
+        emit(*root, 0, STK_INVALID_INDEX, 0);
+    }
+
+    uint32_t depth() const { return maxDepth_; }
+    uint32_t references() const { return (uint32_t)((nodes_.size() + 1) / 2); }
+
+private:
+    struct Choice {
+        float cost = INFINITY;
+        int axis = -1;
+        uint32_t bin = 0;
+        bool spatial = false;
+    };
+
+    // budget: the references this subtree may add by splitting; a split hands
+    // what is left to its children in proportion to their references, so the
+    // tree does not depend on the threads' timing
+    std::unique_ptr<SNode> build(std::vector<PrimRef> refs, uint32_t depth, uint64_t budget)
+    {
+        std::unique_ptr<SNode> node(new SNode());
+        const uint32_t n = (uint32_t)refs.size();
+        Box b;
+        for (const PrimRef& r : refs) b.grow(r.lo, r.hi);
+        for (int a = 0; a < 3; ++a) {
+            node->lo[a] = b.lo[a];
+            node->hi[a] = b.hi[a];
+        }
+        if (n == 1) {
+            node->tri = refs[0].id;
+            return node;
+        }
+        std::vector<PrimRef> L, R;
+        const uint64_t used = partition(refs, b, depth, budget, L, R);
+        refs.clear();
+        refs.shrink_to_fit();
+        const uint64_t rest = budget - used;
+        const uint64_t bl = (uint64_t)((double)rest * (double)L.size() / (double)(L.size() + R.size()));
+        const uint64_t br = rest - bl;
+        if (n > 65536) {
+            std::unique_ptr<SNode> lc;
+            std::thread th([&] { lc = build(std::move(L), depth + 1, bl); });
+            node->right = build(std::move(R), depth + 1, br);
+            th.join();
+            node->left = std::move(lc);
+        } else {
+            node->left = build(std::move(L), depth + 1, bl);
+            node->right = build(std::move(R), depth + 1, br);
+        }
+        node->leaves = node->left->leaves + node->right->leaves;
+        return node;
+    }
+
+    // Object split by binned SAH over the centroids (as the in-place builder),
+    // then the spatial split where the children would overlap.  Returns the
+    // references the split added.
+    uint64_t partition(std::vector<PrimRef>& refs, const Box& nodeBox, uint32_t depth, uint64_t budget,
+                       std::vector<PrimRef>& L, std::vector<PrimRef>& R)
+    {
+        const uint32_t n = (uint32_t)refs.size();
+        const uint32_t nbins = std::min<uint32_t>(opts_.numBins ? opts_.numBins : 32, 64);
+        const uint32_t maxD = opts_.maxDepth;
+        const float parentArea = nodeBox.halfArea() > 0.0f ? nodeBox.halfArea() : 1.0f;
+        Box cb;
+        for (const PrimRef& r : refs) cb.grow(r.c, r.c);
+        Choice obj;
+        Box objL, objR;
+        for (int a = 0; a < 3; ++a) {
+            const float ext = cb.hi[a] - cb.lo[a];
+            if (!(ext > 0.0f)) continue;
+            Box bins[64];
+            uint32_t cnt[64] = {};
+            const float scale = (float)nbins / ext * (1.0f - 1e-6f);
+            for (const PrimRef& r : refs) {
+                uint32_t bi = (uint32_t)((r.c[a] - cb.lo[a]) * scale);
+                if (bi >= nbins) bi = nbins - 1;
+                bins[bi].grow(r.lo, r.hi);
+                cnt[bi]++;
+            }
+            Box racc[64];
+            uint32_t rn[64];
+            Box acc;
+            uint32_t c = 0;
+            for (uint32_t i = nbins - 1; i > 0; --i) {
+                acc.grow(bins[i].lo, bins[i].hi);
+                c += cnt[i];
+                racc[i] = acc;
+                rn[i] = c;
+            }
+            Box lacc;
+            uint32_t lc = 0;
+            for (uint32_t i = 0; i + 1 < nbins; ++i) {
+                lacc.grow(bins[i].lo, bins[i].hi);
+                lc += cnt[i];
+                if (lc == 0 || rn[i + 1] == 0) continue;
+                if (depth + 1 + ceilLog2(std::max(lc, rn[i + 1])) > maxD) continue;
+                const float cost = opts_.traversalCost +
+                                   opts_.intersectionCost *
+                                       (lacc.halfArea() * (float)lc + racc[i + 1].halfArea() * (float)rn[i + 1]) /
+                                       parentArea;
+                if (cost < obj.cost) {
+                    obj.cost = cost;
+                    obj.axis = a;
+                    obj.bin = i;
+                    objL = lacc;
+                    objR = racc[i + 1];
+                }
+            }
+        }
+        // spatial split where the object split's children overlap
+        Choice sp;
+        if (obj.axis >= 0 && budget > 0) {
+            Box ov;
+            for (int a = 0; a < 3; ++a) {
+                ov.lo[a] = std::max(objL.lo[a], objR.lo[a]);
+                ov.hi[a] = std::min(objL.hi[a], objR.hi[a]);
+            }
+            const bool overlap = ov.hi[0] > ov.lo[0] && ov.hi[1] > ov.lo[1] && ov.hi[2] > ov.lo[2];
+            if (overlap && ov.halfArea() > opts_.spatialSplitAlpha * rootArea_) sp = spatialChoice(refs, nodeBox, depth);
+        }
+        uint64_t dup = 0;
+        if (sp.axis >= 0 && sp.cost < obj.cost && spatialSplit(refs, nodeBox, sp, budget, L, R, dup)) return dup;
+        L.clear();
+        R.clear();
+        if (obj.axis >= 0) {
+            const int a = obj.axis;
+            const float ext = cb.hi[a] - cb.lo[a];
+            const float scale = (float)nbins / ext * (1.0f - 1e-6f);
+            for (const PrimRef& r : refs) {
+                uint32_t bi = (uint32_t)((r.c[a] - cb.lo[a]) * scale);
+                if (bi >= nbins) bi = nbins - 1;
+                (bi <= obj.bin ? L : R).push_back(r);
+            }
+            if (!L.empty() && !R.empty()) return 0;
+            L.clear();
+            R.clear();
+        }
+        // median by centroid on the widest axis (index order on ties)
+        int axis = 0;
+        for (int a = 1; a < 3; ++a)
+            if (cb.hi[a] - cb.lo[a] > cb.hi[axis] - cb.lo[axis]) axis = a;
+        std::sort(refs.begin(), refs.end(), [axis](const PrimRef& x, const PrimRef& y) {
+            return x.c[axis] < y.c[axis] || (x.c[axis] == y.c[axis] && x.id < y.id);
+        });
+        L.assign(refs.begin(), refs.begin() + n / 2);
+        R.assign(refs.begin() + n / 2, refs.end());
+        return 0;
+    }
+
+    double planeOf(const Box& nodeBox, int a, uint32_t nbins, uint32_t bin) const
+    {
+        const double w = ((double)nodeBox.hi[a] - nodeBox.lo[a]) / nbins;
+        return (double)nodeBox.lo[a] + w * (bin + 1);
+    }
+
+    void triangle(uint32_t id, double v[3][3]) const
+    {
+        const BvhInputTri& t = tris_[id];
+        for (int k = 0; k < 3; ++k) {
+            const float* p = &t.v[k].x;
+            for (int a = 0; a < 3; ++a) v[k][a] = p[a];
+        }
+    }
+
+    // Binned spatial split: each reference chopped into the bins it spans.
+    Choice spatialChoice(const std::vector<PrimRef>& refs, const Box& nodeBox, uint32_t depth) const
+    {
+        Choice best;
+        const uint32_t n = (uint32_t)refs.size();
+        const uint32_t nbins = std::min<uint32_t>(opts_.numBins ? opts_.numBins : 32, 64);
+        const float parentArea = nodeBox.halfArea() > 0.0f ? nodeBox.halfArea() : 1.0f;
+        for (int a = 0; a < 3; ++a) {
+            const double lo = nodeBox.lo[a], w = ((double)nodeBox.hi[a] - lo) / nbins;
+            if (!(w > 0.0)) continue;
+            Box bins[64];
+            uint32_t enter[64] = {}, exit[64] = {};
+            for (const PrimRef& r : refs) {
+                int b0 = (int)std::floor((r.lo[a] - lo) / w), b1 = (int)std::floor((r.hi[a] - lo) / w);
+                b0 = std::clamp(b0, 0, (int)nbins - 1);
+                b1 = std::clamp(b1, b0, (int)nbins - 1);
+                if (b0 == b1) {
+                    bins[b0].grow(r.lo, r.hi);
+                } else {
+                    double v[3][3];
+                    triangle(r.id, v);
+                    for (int b = b0; b <= b1; ++b) {
+                        double blo[3], bhi[3];
+                        for (int c = 0; c < 3; ++c) {
+                            blo[c] = r.lo[c];
+                            bhi[c] = r.hi[c];
+                        }
+                        blo[a] = std::max(blo[a], lo + w * b);
+                        bhi[a] = std::min(bhi[a], lo + w * (b + 1));
+                        float clo[3], chi[3];
+                        if (clippedBounds(v, blo, bhi, clo, chi)) bins[b].grow(clo, chi);
+                    }
+                }
+                enter[b0]++;
+                exit[b1]++;
+            }
+            Box racc[64];
+            uint32_t rn[64];
+            Box acc;
+            uint32_t c = 0;
+            for (uint32_t i = nbins - 1; i > 0; --i) {
+                acc.grow(bins[i].lo, bins[i].hi);
+                c += exit[i];
+                racc[i] = acc;
+                rn[i] = c;
+            }
+            Box lacc;
+            uint32_t lc = 0;
+            for (uint32_t i = 0; i + 1 < nbins; ++i) {
+                lacc.grow(bins[i].lo, bins[i].hi);
+                lc += enter[i];
+                const uint32_t rc = rn[i + 1];
+                if (lc == 0 || rc == 0 || lc >= n || rc >= n) continue;
+                if (depth + 1 + ceilLog2(std::max(lc, rc)) > opts_.maxDepth) continue;
+                const float cost =
+                    opts_.traversalCost +
+                    opts_.intersectionCost * (lacc.halfArea() * (float)lc + racc[i + 1].halfArea() * (float)rc) /
+                        parentArea;
+                if (cost < best.cost) {
+                    best.cost = cost;
+                    best.axis = a;
+                    best.bin = i;
+                    best.spatial = true;
+                }
+            }
+        }
+        return best;
+    }
+
+    // Splits the references at the chosen plane; false (nothing changed) when
+    // a side would be empty or keep every reference.
+    bool spatialSplit(const std::vector<PrimRef>& refs, const Box& nodeBox, const Choice& ch, uint64_t budget,
+                      std::vector<PrimRef>& L, std::vector<PrimRef>& R, uint64_t& dupOut)
+    {
+        const uint32_t nbins = std::min<uint32_t>(opts_.numBins ? opts_.numBins : 32, 64);
+        const int a = ch.axis;
+        const double plane = planeOf(nodeBox, a, nbins, ch.bin);
+        L.clear();
+        R.clear();
+        uint64_t dup = 0;
+        for (const PrimRef& r : refs) {
+            if ((double)r.hi[a] <= plane) {
+                L.push_back(r);
+                continue;
+            }
+            if ((double)r.lo[a] >= plane) {
+                R.push_back(r);
+                continue;
+            }
+            double v[3][3], blo[3], bhi[3];
+            triangle(r.id, v);
+            for (int c = 0; c < 3; ++c) {
+                blo[c] = r.lo[c];
+                bhi[c] = r.hi[c];
+            }
+            PrimRef pl = r, pr = r;
+            bool hasL, hasR;
+            {
+                double h2[3] = {bhi[0], bhi[1], bhi[2]};
+                h2[a] = plane;
+                hasL = clippedBounds(v, blo, h2, pl.lo, pl.hi);
+            }
+            {
+                double l2[3] = {blo[0], blo[1], blo[2]};
+                l2[a] = plane;
+                hasR = clippedBounds(v, l2, bhi, pr.lo, pr.hi);
+            }
+            // the parts stay inside the reference's box
+            for (int c = 0; c < 3; ++c) {
+                pl.lo[c] = std::max(pl.lo[c], r.lo[c]);
+                pl.hi[c] = std::min(pl.hi[c], r.hi[c]);
+                pr.lo[c] = std::max(pr.lo[c], r.lo[c]);
+                pr.hi[c] = std::min(pr.hi[c], r.hi[c]);
+            }
+            if (!hasL && !hasR) {
+                (r.c[a] < plane ? L : R).push_back(r);
+                continue;
+            }
+            for (PrimRef* q : {&pl, &pr})
+                for (int c = 0; c < 3; ++c) q->c[c] = (q->lo[c] + q->hi[c]) * 0.5f;
+            if (hasL) L.push_back(pl);
+            if (hasR) R.push_back(pr);
+            if (hasL && hasR) ++dup;
+        }
+        const size_t n = refs.size();
+        if (L.empty() || R.empty() || L.size() >= n || R.size() >= n || dup > budget) {
+            L.clear();
+            R.clear();
+            return false;
+        }
+        dupOut = dup;
+        return true;
+    }
+
+    void emit(const SNode& s, uint32_t idx, uint32_t skip, uint32_t depth)
+    {
+        StkBvhNode& node = nodes_[idx];
+        node.nodeOffset = (int32_t)skip;
+        maxDepth_ = std::max(maxDepth_, depth);
+        if (!s.left) {
+            const BvhInputTri& t = tris_[s.tri];
+            node.minBounds[0] = bits_to_float(t.primId);
+            node.minBounds[1] = node.minBounds[2] = 0.0f;
+            node.maxBounds[0] = node.maxBounds[1] = node.maxBounds[2] = 0.0f;
+            node.instId = (int32_t)t.instId;
+            return;
+        }
+        for (int a = 0; a < 3; ++a) {
+            node.minBounds[a] = s.lo[a];
+            node.maxBounds[a] = s.hi[a];
+        }
+        node.instId = (int32_t)STK_INVALID_INDEX;
+        const uint32_t left = idx + 1, right = idx + 2 * s.left->leaves;
+        emit(*s.left, left, right, depth + 1);
+        emit(*s.right, right, skip, depth + 1);
+    }
+
+    const std::vector<BvhInputTri>& tris_;
+    StkBvhOptions opts_;
+    std::vector<StkBvhNode>& nodes_;
+    float rootArea_ = 0.0f;
+    uint32_t maxDepth_ = 0;
+};
+
 } // namespace
 
 bool buildBvh(const std::vector<BvhInputTri>& tris, const StkBvhOptions& optsIn, Bvh& out, std::string& err)
@@ -286,6 +713,16 @@ bool buildBvh(const std::vector<BvhInputTri>& tris, const StkBvhOptions& optsIn,
     if (ceilLog2((uint32_t)tris.size()) > opts.maxDepth) {
         err = "too many triangles for the requested maxDepth";
         return false;
+    }
+    if (opts.spatialSplitAlpha > 0.0f) {
+        SpatialBuilder b(tris, opts, out.nodes);
+        b.run();
+        out.depth = b.depth();
+        if (out.depth > opts.maxDepth) {
+            err = "spatial-split build exceeded maxDepth";
+            return false;
+        }
+        return true;
     }
     Builder b(tris, opts, out.nodes);
     b.run();
@@ -720,6 +1157,28 @@ bool buildWide(const StkInstance* inst, uint32_t ninst, Bvh& bvh, std::string& e
     bvh.wideDepth = 0;
     for (const StkTriangle& t : bvh.tris)
         if (t.instId < ninst && inst[t.instId].lightId != -1) bvh.emitters.push_back(t);
+    // A tree with spatial splits holds a triangle in several leaves, each for
+    // the part of it inside its parent's box: there a leaf's box is the
+    // triangle's box cut to the parent's box widened by the same margin (it
+    // still holds the leaf's part; hits outside it belong to the triangle's
+    // other leaves).
+    const uint32_t nn = (uint32_t)bvh.nodes.size();
+    std::vector<uint32_t> parent;
+    {
+        std::vector<uint64_t> keys;
+        keys.reserve(bvh.tris.size());
+        for (const StkTriangle& t : bvh.tris) keys.push_back(((uint64_t)t.instId << 32) | t.primId);
+        std::sort(keys.begin(), keys.end());
+        if (std::adjacent_find(keys.begin(), keys.end()) != keys.end()) {
+            parent.assign(nn, STK_INVALID_INDEX);
+            for (uint32_t i = 0; i < nn; ++i)
+                if ((uint32_t)bvh.nodes[i].instId == STK_INVALID_INDEX && i + 1 < nn) {
+                    parent[i + 1] = i;
+                    const uint32_t r = (uint32_t)bvh.nodes[i + 1].nodeOffset;
+                    if (r < nn) parent[r] = i;
+                }
+        }
+    }
     // a leaf's box: its triangle's box, the vertices as the hit test forms them
     auto leafBox = [&](uint32_t idx, float* lo, float* hi) {
         const StkTriangle& t = bvh.tris[bvh.leafToTri[idx]];
@@ -729,6 +1188,22 @@ bool buildWide(const StkInstance* inst, uint32_t ninst, Bvh& bvh, std::string& e
             v2[a] = t.v0[a] + t.e1[a];
         }
         paddedTriangleBox(t.v0, v1, v2, lo, hi);
+        if (!parent.empty() && parent[idx] != STK_INVALID_INDEX) {
+            const StkBvhNode& pn = bvh.nodes[parent[idx]];
+            float plo[3], phi[3];
+            float c0[3], c1[3], c2[3];
+            for (int a = 0; a < 3; ++a) {
+                c0[a] = pn.minBounds[a];
+                c1[a] = pn.maxBounds[a];
+                c2[a] = pn.maxBounds[a];
+            }
+            paddedTriangleBox(c0, c1, c2, plo, phi); // the parent's box with the leaf margin
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::max(lo[a], plo[a]);
+                hi[a] = std::min(hi[a], phi[a]);
+                if (lo[a] > hi[a]) lo[a] = hi[a] = 0.5f * (lo[a] + hi[a]);
+            }
+        }
     };
     auto emitLeaf = [&](uint32_t idx) { bvh.wideTris.push_back(bvh.tris[bvh.leafToTri[idx]]); };
     if (!collapseWide(bvh.nodes, leafBox, emitLeaf, bvh.wide, bvh.wideDepth, err)) return false;
@@ -770,8 +1245,11 @@ void transformBox(const float* m, const float* lo, const float* hi, float* olo, 
 } // namespace
 
 bool buildTwoLevel(const StkVertex* vb, uint32_t nv, const uint32_t* ib, uint32_t ni, const StkInstance* inst,
-                   uint32_t ninst, const StkBvhOptions& opts, TwoLevel& out, std::string& err)
+                   uint32_t ninst, const StkBvhOptions& optsIn, TwoLevel& out, std::string& err)
 {
+    // spatial splits serve flat trees: the instance tree's records are boxes
+    StkBvhOptions opts = optsIn;
+    opts.spatialSplitAlpha = 0.0f;
     out = TwoLevel{};
     out.instMesh.assign(ninst, STK_INVALID_INDEX);
     out.instKey.assign(ninst, STK_INVALID_INDEX);

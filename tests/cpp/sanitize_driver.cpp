@@ -1,5 +1,6 @@
 // sanitize_driver.cpp - drives the host half of the library (scene container
-// and its building calls, workload generators, BVH builder and packers, two-level build, file
+// and its building calls, workload generators, BVH builders (with and without
+// spatial splits) and packers, two-level build, file
 // formats, host C-ABI entry points) and the CPU oracle, for a build under
 // -fsanitize=address,undefined (tests/test_sanitizers.py).  No device code:
 // the HIP sources are not part of this build.  Exits 0 when every check
@@ -148,6 +149,42 @@ void flatScene(uint32_t config, uint32_t variant, bool pathTrace)
     stk_bvh_free(b);
 }
 
+// The spatial-split builder: every walk agrees with the exhaustive test on
+// the closed rooms' camera rays (a sample of them).
+void spatialSplits(uint32_t config)
+{
+    Scene sc(config);
+    const StkSceneArrays& a = sc.a;
+    StkBvhOptions o;
+    std::memset(&o, 0, sizeof(o));
+    o.maxDepth = 32;
+    o.numBins = 32;
+    o.traversalCost = o.intersectionCost = 1.0f;
+    o.spatialSplitAlpha = 1e-5f;
+    stk_bvh* b = nullptr;
+    check(stk_bvh_build_host(a.vb, a.numVertices, a.ib, a.numIndices, a.inst, a.numInstances, &o, &b) == STK_OK,
+          "spatial-split build");
+    if (!b) return;
+    uint32_t tris = 0;
+    for (uint32_t i = 0; i < a.numInstances; ++i) tris += (uint32_t)a.inst[i].indexCount / 3;
+    check(stk_bvh_num_triangles(b) > tris && stk_bvh_num_triangles(b) <= tris + tris * 3 / 10 + 2,
+          "split references within the budget");
+    const uint32_t w = (uint32_t)a.camera.dimension[0], h = (uint32_t)a.camera.dimension[1];
+    std::vector<StkRay> rays((size_t)w * h);
+    check(stk_scene_jittered_rays(sc.h, 7, rays.data(), (uint32_t)rays.size()) == STK_OK, "rays");
+    std::vector<StkRay> sub;
+    for (size_t i = 0; i < rays.size(); i += 17) sub.push_back(rays[i]);
+    OrcScene os = oracleScene(a, b);
+    std::vector<StkHit> ref(sub.size()), other(sub.size());
+    OrcStats st;
+    orc_trace_batch(&os, sub.data(), (uint32_t)sub.size(), ref.data(), ORC_MODE_REF, 4, &st);
+    for (int mode : {ORC_MODE_PRUNED, ORC_MODE_WIDE, ORC_MODE_BRUTE}) {
+        orc_trace_batch(&os, sub.data(), (uint32_t)sub.size(), other.data(), mode, 4, &st);
+        check(std::memcmp(ref.data(), other.data(), ref.size() * sizeof(StkHit)) == 0, "split-tree walks agree");
+    }
+    stk_bvh_free(b);
+}
+
 void twoLevel()
 {
     Scene sc(5, 60);
@@ -293,6 +330,8 @@ int main()
     flatScene(6, 0, true);
     flatScene(7, 0, true);
     flatScene(9, 0, true);
+    spatialSplits(1);
+    spatialSplits(7);
     twoLevel();
     sceneApi();
     images();

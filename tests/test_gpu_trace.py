@@ -242,3 +242,41 @@ def test_failed_tree_updates_keep_the_previous_tree(ctx):
     # a valid replacement still takes effect
     ctx.set_bvh(nodes)
     assert same(ctx.trace_closest(rays), before)
+
+
+def test_spatial_split_tree_walks_match_the_oracle():
+    """A tree with spatial splits (a triangle in several leaves): the GPU's
+    wide, binary and reference-order walks return the oracle's restatements'
+    hits bit for bit on a config-2 subsample, and a path-traced config-6
+    frame over such a tree keeps every sample within 1e-4 of the oracle."""
+    from oracle.pyoracle import MODE_PRUNED, MODE_REF, MODE_WIDE
+    from test_gpu_pathtrace import compare_samples
+
+    print("case: config 2 built with spatial splits (alpha 1e-5), rays[::53]", flush=True)
+    s = sa.SceneData.generate(2)
+    ctx = sa.Context(0)
+    ctx.set_scene(s)
+    ctx.build_bvh(spatial_split_alpha=1e-5)
+    nodes = ctx.get_bvh()
+    b = sa.HostBvh.from_nodes(s, nodes)
+    o = Oracle.from_scene(s, b.nodes, b.wide, b.wide_tris)
+    rays = s.jittered_rays(0x5354524B00000001)[::53].copy()
+    assert ctx.trace_closest(rays).tobytes() == o.trace(rays, MODE_WIDE)[0].tobytes()
+    assert ctx.trace_closest(rays, binary=True).tobytes() == o.trace(rays, MODE_PRUNED)[0].tobytes()
+    assert ctx.trace_closest(rays, reference_order=True).tobytes() == o.trace(rays, MODE_REF)[0].tobytes()
+    ctx.close()
+
+    print("case: config 6 path samples over a spatial-split tree vs oracle", flush=True)
+    s6 = sa.SceneData.generate(6)
+    ctx = sa.Context(0)
+    ctx.set_scene(s6)
+    ctx.build_bvh(spatial_split_alpha=1e-5)
+    o6 = Oracle.from_scene(s6, ctx.get_bvh())
+    p = sa.render_params(s6.camera, spp=1, max_depth=6, frame=0, sampling=1, sky=s6.sky, keep_samples=True)
+    w, h = p.pt.dimension[0], p.pt.dimension[1]
+    ctx.render(p)
+    g = ctx.samples(w * h * 3)
+    ctx.close()
+    frac, err = compare_samples(g, o6.pt_frame(p, 0, 0, w, h))
+    print(f"divergent samples {frac}, max error {err}")
+    assert frac == 0
