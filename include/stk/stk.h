@@ -17,7 +17,8 @@
  *                                frame 0, :656-667)
  *   stk_set_lights               createLightsBuffer (src/render/ptrender.cpp:436-458)
  *   stk_set_materials            createMdlBuffers (src/render/ptrender.cpp:603-640);
- *                                Lambert/GGX records instead of MDL arg blocks
+ *                                Lambert / GGX / mirror / glass records with
+ *                                emission instead of MDL arg blocks
  *   stk_build_bvh / stk_set_bvh  createBvhBuffer + BvhBuilder::build
  *                                (src/render/ptrender.cpp:460-532,
  *                                 src/render/bvh.cpp:237-254)
@@ -109,13 +110,15 @@ STK_API void* stk_stream(stk_ctx* ctx);
 STK_API stk_status stk_set_geometry(stk_ctx* ctx, const StkVertex* vb, uint32_t numVertices, const uint32_t* ib,
                                     uint32_t numIndices, const StkInstance* inst, uint32_t numInstances);
 STK_API stk_status stk_set_lights(stk_ctx* ctx, const StkLight* lights, uint32_t numLights);
+/* STK_E_INVALID for a record of unknown type or with a negative (or NaN)
+ * index of refraction. */
 STK_API stk_status stk_set_materials(stk_ctx* ctx, const StkMaterial* mats, uint32_t numMaterials);
-/* Textures the materials' albedoTexture fields refer to (no counterpart in
- * the non-MDL reference path; the reference's MDL materials sample textures
- * through shaders/mdl_runtime.hlsl).  texels: numTexels RGBA32F values (4
- * floats each); every texture must lie inside them, and be at most
- * 16384 x 16384.  Single-scene renders only (a batch refuses textured
- * materials). */
+/* Textures the materials' albedo, roughness, emission and normal texture
+ * fields refer to (the reference's MDL materials sample theirs through
+ * shaders/mdl_runtime.hlsl:173-207).  texels: numTexels RGBA32F values (4
+ * floats each); every texture must lie inside them, be at most 16384 x 16384
+ * and have a known wrap (STK_TEX_*).  Single-scene renders only (a batch
+ * refuses textured materials). */
 STK_API stk_status stk_set_textures(stk_ctx* ctx, const StkTexture* textures, uint32_t numTextures,
                                     const float* texels, uint64_t numTexels);
 /* Builds the BVH on the host (binned SAH, one triangle per leaf, preorder plus

@@ -111,33 +111,60 @@ typedef struct StkPathTracerParam {
     float pad0[4];
 } StkPathTracerParam;
 
-/* Material record of this library, 32 B.  The reference binds MDL argument
- * blocks here (shaders/pathtracerMdl.hlsl:276); this library evaluates the
- * non-MDL Lambert and GGX models instead, with optional textures looked up at
- * the hit's texture coordinates (the vertices' packed uv, shaders/pack.h:15-23,
- * interpolated as shaders/pathtracerMdl.hlsl:261-265): albedo = albedo[0..2] x
- * the albedo texture's rgb, roughness = roughness x the roughness texture's
- * red channel. */
-enum { STK_MAT_LAMBERT = 0, STK_MAT_GGX = 1 };
+/* Material record of this library, 64 B.  The reference binds MDL argument
+ * blocks here (shaders/pathtracerMdl.hlsl:276) and calls the MDL-compiled
+ * scattering and emission functions (:299-372); this library evaluates
+ * closed-form models in their place:
+ *   STK_MAT_LAMBERT  diffuse reflection (BSDF event diffuse reflection);
+ *   STK_MAT_GGX      GGX microfacet reflection, Schlick Fresnel with f0 =
+ *                    albedo (glossy reflection);
+ *   STK_MAT_MIRROR   ideal specular reflection about the shading normal,
+ *                    weight Schlick(albedo, cos) (specular reflection);
+ *   STK_MAT_GLASS    smooth dielectric of index `ior` (0 selects 1.5): the
+ *                    exact unpolarised Fresnel reflectance F chooses between
+ *                    specular reflection (weight 1) and refraction by
+ *                    Snell's law (weight albedo, the tint); the side is that
+ *                    of the geometric normal (entering where the ray meets
+ *                    its front face).
+ * Specular events carry no pdf: next-event estimation adds nothing at such a
+ * vertex and an emitter the next ray meets counts in full, as after the
+ * reference's specularBounce (:229-234, :364).  emission (x the emission
+ * texture's rgb) is added, times the path throughput, wherever a ray meets
+ * the front face of the surface (the EDF term, :299-304).  Textures are
+ * looked up at the hit's texture coordinates (the vertices' packed uv,
+ * shaders/pack.h:15-23, interpolated as :261-265): albedo x the albedo
+ * texture's rgb, roughness x the roughness texture's red channel, and the
+ * normal texture a tangent-space normal map applied as the reference's
+ * CalcBumpedNormal (shaders/helper.h:101-115) with the interpolated vertex
+ * tangent. */
+enum { STK_MAT_LAMBERT = 0, STK_MAT_GGX = 1, STK_MAT_MIRROR = 2, STK_MAT_GLASS = 3 };
 typedef struct StkMaterial {
     uint32_t type;
     float roughness;           /* GGX alpha = roughness^2 */
     uint32_t albedoTexture;    /* 0: none; k: texture k - 1 of stk_set_textures */
     uint32_t roughnessTexture; /* 0: none; k: roughness x texture k - 1's red channel */
-    float albedo[4];
+    float albedo[4];           /* rgb; [3] unused */
+    float emission[3];         /* emitted radiance (front face) */
+    float ior;                 /* glass: index of refraction (0: 1.5) */
+    uint32_t normalTexture;    /* 0: none; k: tangent-space normal map k - 1 (rgb = (n + 1) / 2) */
+    uint32_t emissionTexture;  /* 0: none; k: emission x texture k - 1's rgb */
+    uint32_t pad[2];
 } StkMaterial;
 
 /* A texture of stk_set_textures: width x height RGBA32F texels, row-major
  * (row 0 at v = 0), from texel `offset` of the texel array.  Lookups follow
- * the reference's tex_lookup_float4_2d (shaders/mdl_runtime.hlsl:124-207):
- * wrap (repeat, or clamp to the edge texels' centres), the quintic
- * "smootherstep" of the fractional texel position, then a bilinear blend. */
-enum { STK_TEX_REPEAT = 0, STK_TEX_CLAMP = 1 };
+ * the reference's tex_lookup_float4_2d (shaders/mdl_runtime.hlsl:173-207):
+ * the wrap of apply_wrap_and_crop (:93-120; repeat, clamp to the edge
+ * texels' centres, mirrored repeat then that clamp, or clip: zero outside
+ * [0, 1), clamped inside), the quintic "smootherstep" of the fractional texel
+ * position (:125-140), then a bilinear blend at level 0 (the reference's
+ * SampleLevel(..., 0), :203-206: no mip levels are read). */
+enum { STK_TEX_REPEAT = 0, STK_TEX_CLAMP = 1, STK_TEX_MIRRORED_REPEAT = 2, STK_TEX_CLIP = 3 };
 typedef struct StkTexture {
     uint32_t width;
     uint32_t height;
     uint32_t offset; /* first texel (RGBA32F, 16 B) of the texture */
-    uint32_t wrap;   /* STK_TEX_REPEAT or STK_TEX_CLAMP, both axes */
+    uint32_t wrap;   /* STK_TEX_*, both axes */
 } StkTexture;
 
 /* Pre-transformed world-space triangle, 48 B: {v0, e0, e1} exactly as
@@ -284,9 +311,11 @@ STK_STATIC_ASSERT(sizeof(StkPathTracerParam) == 336, "PathTracerParam must be 33
 STK_STATIC_ASSERT(offsetof(StkPathTracerParam, dimension) == 272, "PathTracerParam.dimension offset");
 STK_STATIC_ASSERT(offsetof(StkPathTracerParam, frameNumber) == 288, "PathTracerParam.frameNumber offset");
 STK_STATIC_ASSERT(offsetof(StkPathTracerParam, len) == 304, "PathTracerParam.len offset");
-STK_STATIC_ASSERT(sizeof(StkMaterial) == 32, "Material must be 32 B");
+STK_STATIC_ASSERT(sizeof(StkMaterial) == 64, "Material must be 64 B");
 STK_STATIC_ASSERT(offsetof(StkMaterial, albedoTexture) == 8, "Material.albedoTexture offset");
 STK_STATIC_ASSERT(offsetof(StkMaterial, roughnessTexture) == 12, "Material.roughnessTexture offset");
+STK_STATIC_ASSERT(offsetof(StkMaterial, emission) == 32, "Material.emission offset");
+STK_STATIC_ASSERT(offsetof(StkMaterial, normalTexture) == 48, "Material.normalTexture offset");
 STK_STATIC_ASSERT(sizeof(StkTexture) == 16, "Texture must be 16 B");
 STK_STATIC_ASSERT(sizeof(StkTriangle) == 48, "Triangle must be 48 B");
 STK_STATIC_ASSERT(sizeof(StkPackedNode) == 64, "PackedNode must be 64 B");

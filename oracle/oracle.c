@@ -1281,9 +1281,51 @@ void orc_bsdf_sample(const StkMaterial* m, const float Np[3], const float Ngp[3]
                      OrcBsdfSample* out)
 {
     const f3 N = ld(Np), Ng = ld(Ngp), k1 = ld(k1p), albedo = ld(m->albedo);
+    memset(out, 0, sizeof(*out));
+    if (m->type == STK_MAT_MIRROR) {
+        /* ideal reflection about the shading normal, Schlick(albedo, cos) */
+        const float c1 = dot(N, k1);
+        const f3 k2 = sub(scl(N, 2.0f * c1), k1);
+        if (c1 <= 0.0f || dot(k2, Ng) <= 0.0f) return;
+        st3(out->k2, k2);
+        st3(out->bsdf_over_pdf, schlick(albedo, c1));
+        out->event = ORC_EVENT_SPECULAR_REFLECTION;
+        return;
+    }
+    if (m->type == STK_MAT_GLASS) {
+        /* smooth dielectric; the side is the geometric normal's */
+        const float ior = m->ior > 0.0f ? m->ior : 1.5f;
+        const int entering = dot(k1, Ng) > 0.0f;
+        const f3 n = entering ? N : neg(N), ng = entering ? Ng : neg(Ng);
+        const float eta = entering ? 1.0f / ior : ior; /* n1 / n2 */
+        const float c1 = dot(n, k1);
+        if (c1 <= 0.0f) return;
+        const float s2 = eta * eta * (1.0f - c1 * c1);
+        float F = 1.0f, c2 = 0.0f;
+        if (s2 < 1.0f) {
+            /* Fresnel's equations, unpolarised */
+            c2 = sqrtf(1.0f - s2);
+            const float rs = (eta * c1 - c2) / (eta * c1 + c2);
+            const float rp = (c1 - eta * c2) / (c1 + eta * c2);
+            F = 0.5f * (rs * rs + rp * rp);
+        }
+        if (xi[2] < F) {
+            const f3 k2 = sub(scl(n, 2.0f * c1), k1);
+            if (dot(k2, ng) <= 0.0f) return;
+            st3(out->k2, k2);
+            st3(out->bsdf_over_pdf, mk(1.0f, 1.0f, 1.0f));
+            out->event = ORC_EVENT_SPECULAR_REFLECTION;
+        } else {
+            const f3 k2 = sub(scl(n, eta * c1 - c2), scl(k1, eta)); /* Snell's law */
+            if (dot(k2, ng) >= 0.0f) return;
+            st3(out->k2, k2);
+            st3(out->bsdf_over_pdf, albedo);
+            out->event = ORC_EVENT_SPECULAR_TRANSMISSION;
+        }
+        return;
+    }
     f3 T, B;
     tangent_space(N, &T, &B);
-    memset(out, 0, sizeof(*out));
     if (m->type == STK_MAT_GGX) {
         const float alpha = ggx_alpha(m);
         const float a2 = alpha * alpha;
@@ -1326,6 +1368,7 @@ void orc_bsdf_eval(const StkMaterial* m, const float Np[3], const float Ngp[3], 
     st3(glossy, zero);
     *pdf = 0.0f;
     const float nk2 = dot(N, k2);
+    if (m->type == STK_MAT_MIRROR || m->type == STK_MAT_GLASS) return; /* zero off the specular direction */
     if (nk2 <= 0.0f || dot(k2, Ng) <= 0.0f) return;
     if (m->type == STK_MAT_GGX) {
         const float nk1 = dot(N, k1);
@@ -1371,7 +1414,8 @@ static inline f3 interp(f3 a, f3 b, f3 c, float u, float v)
 /* Path tracer: pathTraceCameraRays (shaders/pathtracerMdl.hlsl:204-404)      */
 /* with estimateDirectLighting/sampleLights (:31-92).                         */
 /* ------------------------------------------------------------------------ */
-static const StkMaterial k_default_material = {STK_MAT_LAMBERT, 1.0f, 0u, 0u, {0.5f, 0.5f, 0.5f, 1.0f}};
+static const StkMaterial k_default_material = {STK_MAT_LAMBERT, 1.0f, 0u, 0u, {0.5f, 0.5f, 0.5f, 1.0f},
+                                               {0.0f, 0.0f, 0.0f}, 0.0f, 0u, 0u, {0u, 0u}};
 
 /* unpackUV (shaders/pack.h:15-23) */
 static inline void unpack_uv(uint32_t p, float* s, float* t)
@@ -1381,13 +1425,19 @@ static inline void unpack_uv(uint32_t p, float* s, float* t)
 }
 
 /* One axis of tex_lookup_float4_2d (shaders/mdl_runtime.hlsl:173-207): the
- * clamp of apply_wrap_and_crop (:93-120) for a clamped texture, the quintic
- * smoothing of apply_smootherstep_filter (:125-140), then the bilinear
- * filter's texel coordinate x * n - 0.5. */
-static float tex_axis(float c, uint32_t n, int clampEdge)
+ * wrap of apply_wrap_and_crop (:93-120: repeat leaves the coordinate to the
+ * fetch; mirrored repeat folds it into [0, 1]; every wrap but repeat clamps
+ * it to the edge texels' centres), the quintic smoothing of
+ * apply_smootherstep_filter (:125-140), then the bilinear filter's texel
+ * coordinate x * n - 0.5. */
+static float tex_axis(float c, uint32_t n, uint32_t wrap)
 {
     const float fn = (float)n;
-    if (clampEdge) {
+    if (wrap != STK_TEX_REPEAT) {
+        if (wrap == STK_TEX_MIRRORED_REPEAT) {
+            const float fl = floorf(c);
+            c = ((int)fl & 1) != 0 ? 1.0f - (c - fl) : c - fl;
+        }
         const float h = 0.5f / fn;
         c = fminf(fmaxf(c, h), 1.0f - h);
     }
@@ -1408,8 +1458,13 @@ static int tex_wrap(int i, int n, int clampEdge)
 
 void orc_texture_lookup(const StkTexture* tx, const float* texels, float s, float t, float out[3])
 {
-    const int ce = tx->wrap == STK_TEX_CLAMP;
-    const float x = tex_axis(s, tx->width, ce), y = tex_axis(t, tx->height, ce);
+    if (tx->wrap == STK_TEX_CLIP && (s < 0.0f || s >= 1.0f || t < 0.0f || t >= 1.0f)) {
+        /* clip: zero outside the texture (:186-189) */
+        out[0] = out[1] = out[2] = 0.0f;
+        return;
+    }
+    const int ce = tx->wrap != STK_TEX_REPEAT;
+    const float x = tex_axis(s, tx->width, tx->wrap), y = tex_axis(t, tx->height, tx->wrap);
     const float x0 = floorf(x), y0 = floorf(y);
     const float fx = x - x0, fy = y - y0;
     const int w = (int)tx->width, h = (int)tx->height;
@@ -1421,6 +1476,20 @@ void orc_texture_lookup(const StkTexture* tx, const float* texels, float s, floa
     const float* d = texels + 4 * ((size_t)tx->offset + (size_t)iy1 * tx->width + ix1);
     const float gx = 1.0f - fx, gy = 1.0f - fy;
     for (int k = 0; k < 3; ++k) out[k] = (a[k] * gx + b[k] * fx) * gy + (c[k] * gx + d[k] * fx) * fy;
+}
+
+/* CalcBumpedNormal (shaders/helper.h:101-115): the tangent negated and made
+ * orthogonal to the normal (Gram-Schmidt), the bitangent N x T, and the
+ * map's texel (rgb = (n + 1) / 2) taken from that frame to world space:
+ * normalize(mul(transpose(float3x3(T, B, N)), texel * 2 - 1)). */
+void orc_bumped_normal(const float normal[3], const float tangent[3], const float texel[3], float out[3])
+{
+    const f3 n = nrm(ld(normal));
+    f3 t = neg(nrm(ld(tangent)));
+    t = nrm(sub(t, scl(n, dot(t, n))));
+    const f3 b = cross(n, t);
+    const f3 m = mk(texel[0] * 2.0f - 1.0f, texel[1] * 2.0f - 1.0f, texel[2] * 2.0f - 1.0f);
+    st3(out, nrm(to_world(t, b, n, m)));
 }
 
 void orc_pt_sample(const OrcScene* s, const StkRenderParams* rp, uint32_t px, uint32_t py, uint32_t sampleNum,
@@ -1454,11 +1523,11 @@ void orc_pt_sample(const OrcScene* s, const StkRenderParams* rp, uint32_t px, ui
                  * material -1 there). */
                 if (depth == 0 || specularBounce) color = add(color, mul(thr, ld(l->color)));
             } else if (dot(ld(ray.d), rect_light_normal(l)) < 0.0f) {
-                /* corrected: the front face emits; after a bounce the hit is
-                 * weighted against the previous vertex's light sample (balance
-                 * heuristic, both pdfs in solid angle) */
+                /* corrected: the front face emits; after a diffuse or glossy
+                 * bounce the hit is weighted against the previous vertex's light
+                 * sample (balance heuristic, both pdfs in solid angle) */
                 float w = 1.0f;
-                if (depth > 0) {
+                if (depth > 0 && !specularBounce) {
                     const SphQuad q = sphquad_init(l, ld(ray.o));
                     const float lpdf = q.S > 0.0f ? (1.0f / (float)s->nlights) / q.S : 0.0f;
                     w = prevPdf / (prevPdf + lpdf);
@@ -1478,12 +1547,15 @@ void orc_pt_sample(const OrcScene* s, const StkRenderParams* rp, uint32_t px, ui
         const f3 n1 = xform3(ic->normalMatrix, unpack_normal(s->vb[i1].normal));
         const f3 n2 = xform3(ic->normalMatrix, unpack_normal(s->vb[i2].normal));
         const f3 P = interp(p0, p1, p2, hit.bary[0], hit.bary[1]);
-        const f3 N = nrm(interp(n0, n1, n2, hit.bary[0], hit.bary[1]));
         StkMaterial matv = (ic->materialId >= 0 && (uint32_t)ic->materialId < s->nmats) ? s->mats[ic->materialId]
                                                                                          : k_default_material;
+        f3 N = nrm(interp(n0, n1, n2, hit.bary[0], hit.bary[1]));
+        f3 emit = ld(matv.emission);
         const int albTex = matv.albedoTexture != 0 && matv.albedoTexture <= s->ntextures;
         const int roughTex = matv.roughnessTexture != 0 && matv.roughnessTexture <= s->ntextures;
-        if (albTex || roughTex) {
+        const int normTex = matv.normalTexture != 0 && matv.normalTexture <= s->ntextures;
+        const int emitTex = matv.emissionTexture != 0 && matv.emissionTexture <= s->ntextures;
+        if (albTex || roughTex || normTex || emitTex) {
             /* texture coordinates as shaders/pathtracerMdl.hlsl:261-265 */
             float s0, t0, s1, t1, s2, t2, tc[3];
             unpack_uv(s->vb[i0].uv, &s0, &t0);
@@ -1500,9 +1572,27 @@ void orc_pt_sample(const OrcScene* s, const StkRenderParams* rp, uint32_t px, ui
                 orc_texture_lookup(&s->textures[matv.roughnessTexture - 1], s->texels, su, tv, tc);
                 matv.roughness = matv.roughness * tc[0];
             }
+            if (emitTex) {
+                orc_texture_lookup(&s->textures[matv.emissionTexture - 1], s->texels, su, tv, tc);
+                emit = mul(emit, ld(tc));
+            }
+            if (normTex) {
+                /* the world tangent as the reference forms it (:247-257) */
+                const f3 ta = xform3(ic->normalMatrix, unpack_normal(s->vb[i0].tangent));
+                const f3 tb = xform3(ic->normalMatrix, unpack_normal(s->vb[i1].tangent));
+                const f3 tcn = xform3(ic->normalMatrix, unpack_normal(s->vb[i2].tangent));
+                const f3 T = nrm(interp(ta, tb, tcn, hit.bary[0], hit.bary[1]));
+                float bn[3];
+                orc_texture_lookup(&s->textures[matv.normalTexture - 1], s->texels, su, tv, tc);
+                orc_bumped_normal((const float*)&N, (const float*)&T, tc, bn);
+                /* a tangent along the normal leaves no frame: keep the normal */
+                if (!(isnan(bn[0]) || isnan(bn[1]) || isnan(bn[2]))) N = ld(bn);
+            }
         }
         const StkMaterial* mat = &matv;
         const f3 k1 = neg(ld(ray.d));
+        /* the EDF: the front face emits (:299-304) */
+        if ((emit.x != 0.0f || emit.y != 0.0f || emit.z != 0.0f) && dot(k1, Ng) > 0.0f) color = add(color, mul(thr, emit));
 
         /* sampleLights + estimateDirectLighting */
         f3 toLight = mk(0.0f, 0.0f, 0.0f);
@@ -1570,8 +1660,9 @@ void orc_pt_sample(const OrcScene* s, const StkRenderParams* rp, uint32_t px, ui
         OrcBsdfSample bs;
         orc_bsdf_sample(mat, (const float*)&N, (const float*)&Ng, (const float*)&k1, xi, &bs);
         if (bs.event == 0) break;
-        specularBounce = 0;
-        const f3 nextO = offset_ray(P, Ng);
+        specularBounce = bs.event >= ORC_EVENT_SPECULAR_REFLECTION;
+        /* offset to the side the new ray leaves by (:368-370) */
+        const f3 nextO = offset_ray(P, dot(ld(bs.k2), Ng) > 0.0f ? Ng : neg(Ng));
         thr = mul(thr, ld(bs.bsdf_over_pdf));
         if (depth > (int)rp->rrStartDepth) {
             if (ref) {

@@ -164,18 +164,22 @@ typedef struct OrcBsdfSample {
     float k2[3];
     float pdf;
     float bsdf_over_pdf[3];
-    int event; /* 0 absorb, 1 diffuse reflection, 2 glossy reflection */
+    int event; /* 0 absorb, 1 diffuse reflection, 2 glossy reflection, 3 specular reflection, 4 specular
+                * transmission (specular samples report pdf 0) */
 } OrcBsdfSample;
+enum { ORC_EVENT_SPECULAR_REFLECTION = 3, ORC_EVENT_SPECULAR_TRANSMISSION = 4 };
 void orc_bsdf_sample(const StkMaterial* m, const float N[3], const float Ng[3], const float k1[3], const float xi[4],
                      OrcBsdfSample* out);
 /* diffuse/glossy outputs are bsdf * dot(N, k2), pdf is the solid-angle pdf. */
 void orc_bsdf_eval(const StkMaterial* m, const float N[3], const float Ng[3], const float k1[3], const float k2[3],
                    float diffuse[3], float glossy[3], float* pdf);
 
-/* Albedo texture lookup (rgb) at texture coordinates (s, t): the reference's
+/* Texture lookup (rgb) at texture coordinates (s, t): the reference's
  * tex_lookup_float4_2d (shaders/mdl_runtime.hlsl:173-207) with its
  * smootherstep filter, the bilinear blend restated in binary32. */
 void orc_texture_lookup(const StkTexture* tx, const float* texels, float s, float t, float out[3]);
+/* The normal map's shading normal: CalcBumpedNormal (shaders/helper.h:101-115). */
+void orc_bumped_normal(const float normal[3], const float tangent[3], const float texel[3], float out[3]);
 
 /* ---- path tracer (shaders/pathtracerMdl.hlsl:204-423) -------------------- */
 /* One (pixel, sample) path; seeds the RNG with frameNumber*spp + sampleNum. */

@@ -277,6 +277,7 @@ static d3 eval64(const StkMaterial* m, d3 N, d3 Ng, d3 k1, d3 k2, double* pdf)
 {
     *pdf = 0.0;
     const double nk2 = ddot(N, k2);
+    if (m->type == STK_MAT_MIRROR || m->type == STK_MAT_GLASS) return D(0, 0, 0); /* specular: a delta */
     if (nk2 <= 0.0 || ddot(k2, Ng) <= 0.0) return D(0, 0, 0);
     const d3 alb = dl(m->albedo);
     if (m->type == STK_MAT_GGX) {
@@ -294,11 +295,47 @@ static d3 eval64(const StkMaterial* m, d3 N, d3 Ng, d3 k1, d3 k2, double* pdf)
     return dscl(alb, nk2 / kPi);
 }
 
-static int sample64(const StkMaterial* m, d3 N, d3 Ng, d3 k1, double x0, double x1, d3* k2, d3* weight, double* pdf)
+/* Returns the event: 0 absorb, 1 diffuse, 2 glossy, 3 specular reflection,
+ * 4 specular transmission (mirror and glass as the binary32 oracle's models). */
+static int sample64(const StkMaterial* m, d3 N, d3 Ng, d3 k1, double x0, double x1, double x2, d3* k2, d3* weight,
+                    double* pdf)
 {
+    const d3 alb = dl(m->albedo);
+    *pdf = 0.0;
+    if (m->type == STK_MAT_MIRROR) {
+        const double c1 = ddot(N, k1);
+        *k2 = dsub(dscl(N, 2.0 * c1), k1);
+        if (c1 <= 0.0 || ddot(*k2, Ng) <= 0.0) return 0;
+        *weight = schlick64(alb, c1);
+        return 3;
+    }
+    if (m->type == STK_MAT_GLASS) {
+        const double ior = m->ior > 0.0f ? (double)m->ior : 1.5;
+        const int entering = ddot(k1, Ng) > 0.0;
+        const d3 n = entering ? N : dscl(N, -1.0), ng = entering ? Ng : dscl(Ng, -1.0);
+        const double eta = entering ? 1.0 / ior : ior;
+        const double c1 = ddot(n, k1);
+        if (c1 <= 0.0) return 0;
+        const double s2 = eta * eta * (1.0 - c1 * c1);
+        double F = 1.0, c2 = 0.0;
+        if (s2 < 1.0) {
+            c2 = sqrt(1.0 - s2);
+            const double rs = (eta * c1 - c2) / (eta * c1 + c2), rp = (c1 - eta * c2) / (c1 + eta * c2);
+            F = 0.5 * (rs * rs + rp * rp);
+        }
+        if (x2 < F) {
+            *k2 = dsub(dscl(n, 2.0 * c1), k1);
+            if (ddot(*k2, ng) <= 0.0) return 0;
+            *weight = D(1, 1, 1);
+            return 3;
+        }
+        *k2 = dsub(dscl(n, eta * c1 - c2), dscl(k1, eta));
+        if (ddot(*k2, ng) >= 0.0) return 0;
+        *weight = alb;
+        return 4;
+    }
     d3 T, B;
     tangent64(N, &T, &B);
-    const d3 alb = dl(m->albedo);
     const double phi = 2.0 * kPi * x1;
     if (m->type == STK_MAT_GGX) {
         const double a = alpha64(m), a2 = a * a;
@@ -353,6 +390,7 @@ void orc_pt_sample_fp64(const OrcScene* s, const StkRenderParams* rp, uint32_t p
     d3 color = D(0, 0, 0), thr = D(1, 1, 1);
     const d3 sky = dl(rp->skyColor);
     double prevPdf = 0.0;
+    int specularBounce = 0;
     for (int depth = 0; depth < (int)p->maxDepth; ++depth) {
         Hit64 hit;
         if (!walk64(s, &ray, 0, &hit)) {
@@ -363,10 +401,10 @@ void orc_pt_sample_fp64(const OrcScene* s, const StkRenderParams* rp, uint32_t p
         if (ic->lightId != -1) {
             const StkLight* l = &s->lights[ic->lightId];
             if (ref) {
-                if (depth == 0) color = dadd(color, dmul(thr, dl(l->color)));
+                if (depth == 0 || specularBounce) color = dadd(color, dmul(thr, dl(l->color)));
             } else if (ddot(ray.d, light_normal64(l)) < 0.0) {
                 double w = 1.0;
-                if (depth > 0) {
+                if (depth > 0 && !specularBounce) {
                     const Quad64 q = quad64(l, ray.o);
                     const double lpdf = q.S > 0.0 ? (1.0 / (double)s->nlights) / q.S : 0.0;
                     w = prevPdf / (prevPdf + lpdf);
@@ -396,6 +434,8 @@ void orc_pt_sample_fp64(const OrcScene* s, const StkRenderParams* rp, uint32_t p
             mat.albedo[0] = mat.albedo[1] = mat.albedo[2] = 0.5f;
         }
         const d3 k1 = dscl(ray.d, -1.0);
+        /* the EDF: the front face emits (:299-304) */
+        if (ddot(k1, Ng) > 0.0) color = dadd(color, dmul(thr, dl(mat.emission)));
         /* sampleLights / estimateDirectLighting (:31-92) */
         d3 toLight = D(0, 0, 0), radiance = D(0, 0, 0);
         double lightPdf = 0.0;
@@ -439,12 +479,13 @@ void orc_pt_sample_fp64(const OrcScene* s, const StkRenderParams* rp, uint32_t p
                 color = dadd(color, dmul(dscl(dmul(thr, radiance), mis / lightPdf), f));
             }
         }
-        const double x0 = orc_rand(&rng), x1 = orc_rand(&rng);
-        (void)orc_rand(&rng);
+        const double x0 = orc_rand(&rng), x1 = orc_rand(&rng), x2 = orc_rand(&rng);
         (void)orc_rand(&rng);
         d3 k2, weight;
         double bpdf;
-        if (!sample64(&mat, N, Ng, k1, x0, x1, &k2, &weight, &bpdf)) break;
+        const int event = sample64(&mat, N, Ng, k1, x0, x1, x2, &k2, &weight, &bpdf);
+        if (!event) break;
+        specularBounce = event >= 3;
         thr = dmul(thr, weight);
         if (depth > (int)rp->rrStartDepth) {
             double pr = thr.x > thr.y ? thr.x : thr.y;
@@ -459,7 +500,7 @@ void orc_pt_sample_fp64(const OrcScene* s, const StkRenderParams* rp, uint32_t p
             }
         }
         prevPdf = bpdf;
-        ray.o = offset64(P, Ng);
+        ray.o = offset64(P, ddot(k2, Ng) > 0.0 ? Ng : dscl(Ng, -1.0));
         ray.d = k2;
         ray.tmax = 1e9;
     }

@@ -94,6 +94,7 @@ def _bind(lib: C.CDLL) -> C.CDLL:
         "orc_pt_sample": (None, [P, P, U32, U32, U32, FP]),
         "orc_pt_frame": (None, [P, P, U32, U32, U32, U32, FP, C.c_int]),
         "orc_texture_lookup": (None, [P, FP, C.c_float, C.c_float, FP]),
+        "orc_bumped_normal": (None, [FP, FP, FP, FP]),
         "orc_reduce": (None, [FP, U32, U32, FP]),
         "orc_accumulate": (None, [FP, FP, U32, U32, FP]),
         "orc_tonemap": (None, [FP, U32, U32, FP]),
@@ -126,6 +127,15 @@ def texture_lookup(texture: np.ndarray, texels: np.ndarray, s: float, t: float) 
     tx = np.ascontiguousarray(texels, dtype=np.float32)
     out = np.zeros(3, np.float32)
     load().orc_texture_lookup(rec.ctypes.data, fptr(tx), C.c_float(s), C.c_float(t), fptr(out))
+    return out
+
+
+def bumped_normal(normal, tangent, texel) -> np.ndarray:
+    """orc_bumped_normal: the normal map's shading normal for a world normal,
+    a world tangent and a map texel (rgb)."""
+    n, t, x = (np.ascontiguousarray(v, dtype=np.float32)[:3] for v in (normal, tangent, texel))
+    out = np.zeros(3, np.float32)
+    load().orc_bumped_normal(fptr(n), fptr(t), fptr(x), fptr(out))
     return out
 
 
@@ -163,7 +173,7 @@ class Oracle:
             s.lights, s.nlights = self.lights.ctypes.data, self.lights.size
         if self.mats is not None and self.mats.size:
             s.mats, s.nmats = self.mats.ctypes.data, self.mats.size
-        # albedo textures (StkTexture records, RGBA32F texels), kept alive here
+        # material textures (StkTexture records, RGBA32F texels), kept alive here
         self.textures = None if textures is None else np.ascontiguousarray(textures)
         self.texels = None if texels is None else np.ascontiguousarray(texels, dtype=np.float32)
         if self.textures is not None and self.textures.size:
@@ -226,7 +236,13 @@ class Oracle:
 
     def pt_frame_fp64(self, params, x0: int, y0: int, w: int, h: int, threads: int | None = None):
         """Per-pixel mean and variance of the mean (h x w x 3, float64) of the
-        binary64 twin's params.pt.spp samples per pixel."""
+        binary64 twin's params.pt.spp samples per pixel.  The twin models the
+        materials' constant parameters only: a scene whose materials read
+        textures is refused."""
+        if self.mats is not None and self.mats.size and any(
+                np.any(self.mats[f] != 0) for f in ("albedoTexture", "roughnessTexture", "normalTexture",
+                                                   "emissionTexture")):
+            raise ValueError("the binary64 twin does not model material textures")
         n = params.pt.spp
         s1 = np.zeros(w * h * 3, np.float64)
         s2 = np.zeros(w * h * 3, np.float64)

@@ -15,14 +15,15 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _abi
-from ._abi import (BLAS_TRIANGLE, BVH_NODE, CACHE_RECORD, HIT, INSTANCE, INVALID, LIGHT, MAT_GGX, MAT_LAMBERT,
-                   MATERIAL, PACKED_NODE, RAY, TEXTURE, TLAS_LEAF, TRIANGLE, WIDE_NODE,
+from ._abi import (BLAS_TRIANGLE, BVH_NODE, CACHE_RECORD, HIT, INSTANCE, INVALID, LIGHT, MAT_GGX, MAT_GLASS,
+                   MAT_LAMBERT, MAT_MIRROR, MATERIAL, TEX_CLAMP, TEX_CLIP, TEX_MIRRORED_REPEAT, TEX_REPEAT, PACKED_NODE, RAY, TEXTURE, TLAS_LEAF, TRIANGLE, WIDE_NODE,
                    VERTEX, BatchScene, BatchViewInfo, BvhHeader, BvhInfo, BvhOptions, CacheConfig, CacheInfo, DeviceInfo, ImageDesc, LaunchConfig,
                    PathTracerParam, RenderParams, SceneArrays)
 
 __all__ = ["lib", "StkError", "SceneData", "Scene", "look_at", "pack_normal", "pack_uv", "HostBvh", "HostTwoLevel", "Context", "RenderParams", "PathTracerParam", "VERTEX",
            "BVH_NODE", "INSTANCE", "LIGHT", "RAY", "HIT", "MATERIAL", "TRIANGLE", "PACKED_NODE", "INVALID",
-           "render_params", "CACHE_RECORD", "MAT_LAMBERT", "MAT_GGX", "WIDE_NODE"]
+           "render_params", "CACHE_RECORD", "MAT_LAMBERT", "MAT_GGX", "MAT_MIRROR", "MAT_GLASS", "TEXTURE",
+           "TEX_REPEAT", "TEX_CLAMP", "TEX_MIRRORED_REPEAT", "TEX_CLIP", "WIDE_NODE"]
 
 _LIB = _abi.load()
 
@@ -60,7 +61,7 @@ class SceneData:
     camera: PathTracerParam
     sky: tuple
     _handle: C.c_void_p | None = None
-    textures: np.ndarray | None = None  # TEXTURE records (albedo textures of the materials)
+    textures: np.ndarray | None = None  # TEXTURE records (the materials' textures)
     texels: np.ndarray | None = None    # RGBA32F texels, shape (n, 4)
 
     @property
@@ -449,9 +450,15 @@ class Context:
         self._check(_LIB.stk_set_geometry(self._h, _ptr(s.vb), s.vb.size, _ptr(s.ib), s.ib.size, _ptr(s.inst),
                                           s.inst.size), "stk_set_geometry")
         self._check(_LIB.stk_set_lights(self._h, _ptr(s.lights), s.lights.size), "stk_set_lights")
-        self._check(_LIB.stk_set_materials(self._h, _ptr(s.mats), s.mats.size), "stk_set_materials")
+        self.set_materials(s.mats)
         if s.textures is not None:
             self.set_textures(s.textures, s.texels)
+
+    def set_materials(self, mats: np.ndarray):
+        """stk_set_materials: MATERIAL records (an unknown type or a negative
+        index of refraction is refused)."""
+        mats = np.ascontiguousarray(mats, dtype=_abi.MATERIAL)
+        self._check(_LIB.stk_set_materials(self._h, _ptr(mats), mats.size), "stk_set_materials")
 
     def load_scene(self, s, max_depth: int = 32, bins: int = 32, spatial_split_alpha: float = 0.0):
         """stk_set_scene: the whole container (a Scene, or a generated

@@ -23,11 +23,13 @@ LIGHT = np.dtype([("points", "<f4", (4, 4)), ("color", "<f4", 4), ("normal", "<f
                   ("pad", "<f4", 3)])
 RAY = np.dtype([("o", "<f4", 4), ("d", "<f4", 4)])
 HIT = np.dtype([("bary", "<f4", 2), ("t", "<f4"), ("instId", "<u4"), ("primId", "<u4")])
-MAT_LAMBERT, MAT_GGX = 0, 1  # StkMaterial.type
+MAT_LAMBERT, MAT_GGX, MAT_MIRROR, MAT_GLASS = 0, 1, 2, 3  # StkMaterial.type
 MATERIAL = np.dtype([("type", "<u4"), ("roughness", "<f4"), ("albedoTexture", "<u4"), ("roughnessTexture", "<u4"),
-                     ("albedo", "<f4", 4)])
+                     ("albedo", "<f4", 4), ("emission", "<f4", 3), ("ior", "<f4"), ("normalTexture", "<u4"),
+                     ("emissionTexture", "<u4"), ("pad", "<u4", 2)])
+TEX_REPEAT, TEX_CLAMP, TEX_MIRRORED_REPEAT, TEX_CLIP = 0, 1, 2, 3  # StkTexture.wrap
 TEXTURE = np.dtype([("width", "<u4"), ("height", "<u4"), ("offset", "<u4"), ("wrap", "<u4")])
-assert MATERIAL.itemsize == 32 and TEXTURE.itemsize == 16
+assert MATERIAL.itemsize == 64 and TEXTURE.itemsize == 16
 TRIANGLE = np.dtype([("v0", "<f4", 3), ("leaf", "<u4"), ("e0", "<f4", 3), ("instId", "<u4"), ("e1", "<f4", 3),
                      ("primId", "<u4")])
 PACKED_NODE = np.dtype([("x", "<f4", 4), ("y", "<f4", 4), ("z", "<f4", 4), ("child", "<i4", 2), ("pad", "<u4", 2)])
@@ -40,7 +42,7 @@ TLAS_LEAF = np.dtype([("lo", "<f4", 3), ("instId", "<u4"), ("hi", "<f4", 3), ("b
                       ("pad", "<u4", 3)])
 
 for _dt, _sz in ((VERTEX, 32), (BVH_NODE, 32), (INSTANCE, 208), (LIGHT, 112), (RAY, 32), (HIT, 20),
-                 (MATERIAL, 32), (TRIANGLE, 48), (PACKED_NODE, 64), (WIDE_NODE, 80), (BLAS_TRIANGLE, 48),
+                 (MATERIAL, 64), (TRIANGLE, 48), (PACKED_NODE, 64), (WIDE_NODE, 80), (BLAS_TRIANGLE, 48),
                  (TLAS_LEAF, 48)):
     assert _dt.itemsize == _sz, (_dt, _sz)
 

@@ -80,9 +80,21 @@ StkMaterial materialFromParams(const std::vector<oka::MaterialManager::Param>& p
         } else if (p.type == T::eFloat && p.value.size() >= 4 && p.name == "metallic") {
             float metallic = 0.0f;
             std::memcpy(&metallic, p.value.data(), 4);
-            if (metallic > 0.5f) m.type = STK_MAT_GGX;
+            if (metallic > 0.5f && m.type != STK_MAT_GLASS) m.type = STK_MAT_GGX;
+        } else if (p.type == T::eFloat3 && p.value.size() >= 12 &&
+                   (p.name == "emissive_color" || p.name == "emission_color")) {
+            std::memcpy(m.emission, p.value.data(), 12);
+        } else if (p.type == T::eFloat && p.value.size() >= 4 && p.name == "ior") {
+            std::memcpy(&m.ior, p.value.data(), 4);
+        } else if (p.type == T::eFloat && p.value.size() >= 4 &&
+                   (p.name == "transmission" || p.name == "transmission_weight")) {
+            float t = 0.0f;
+            std::memcpy(&t, p.value.data(), 4);
+            if (t > 0.5f) m.type = STK_MAT_GLASS;
         }
     }
+    // a GGX lobe of (near) zero roughness is the mirror's delta
+    if (m.type == STK_MAT_GGX && m.roughness < 0.02f) m.type = STK_MAT_MIRROR;
     return m;
 }
 

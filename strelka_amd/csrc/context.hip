@@ -490,6 +490,10 @@ STK_API stk_status stk_set_materials(stk_ctx* c, const StkMaterial* m, uint32_t 
 {
     if (!validCtx(c)) return STK_E_INVALID;
     if (n && !m) return fail(c, STK_E_INVALID, "stk_set_materials: null buffer");
+    for (uint32_t i = 0; i < n; ++i)
+        if (m[i].type > STK_MAT_GLASS || !(m[i].ior >= 0.0f))
+            return fail(c, STK_E_INVALID, "stk_set_materials: material " + std::to_string(i) +
+                                              " has an unknown type or a negative index of refraction");
     (void)hipSetDevice(c->device);
     stk_status st = devUpload(c, c->mats, m, (size_t)n * sizeof(StkMaterial));
     if (st == STK_OK) c->numMats = n;
@@ -504,7 +508,7 @@ STK_API stk_status stk_set_textures(stk_ctx* c, const StkTexture* t, uint32_t n,
     if (numTexels > (1ull << 31)) return fail(c, STK_E_INVALID, "stk_set_textures: too many texels");
     for (uint32_t i = 0; i < n; ++i) {
         const StkTexture& x = t[i];
-        if (x.width == 0 || x.height == 0 || x.width > 16384 || x.height > 16384 || x.wrap > STK_TEX_CLAMP ||
+        if (x.width == 0 || x.height == 0 || x.width > 16384 || x.height > 16384 || x.wrap > STK_TEX_CLIP ||
             (uint64_t)x.offset + (uint64_t)x.width * x.height > numTexels)
             return fail(c, STK_E_INVALID, "stk_set_textures: texture " + std::to_string(i) +
                                               " empty, too large, with an unknown wrap or outside the texels");
@@ -1375,9 +1379,13 @@ STK_API stk_status stk_batch_add_scene(stk_ctx* c, const StkBatchScene* s, uint3
         (uint64_t)s->camera.dimension[0] * (uint64_t)s->camera.dimension[1] > (1ull << 26))
         return fail(c, STK_E_INVALID, "stk_batch_add_scene: bad view size");
     if (c->batch.size() >= 4096) return fail(c, STK_E_INVALID, "stk_batch_add_scene: at most 4096 scenes");
-    for (uint32_t i = 0; i < s->numMaterials; ++i)
-        if (s->mats[i].albedoTexture != 0 || s->mats[i].roughnessTexture != 0)
+    for (uint32_t i = 0; i < s->numMaterials; ++i) {
+        const StkMaterial& m = s->mats[i];
+        if (m.albedoTexture != 0 || m.roughnessTexture != 0 || m.normalTexture != 0 || m.emissionTexture != 0)
             return fail(c, STK_E_INVALID, "stk_batch_add_scene: textured materials serve single-scene renders");
+        if (m.type > STK_MAT_GLASS || !(m.ior >= 0.0f))
+            return fail(c, STK_E_INVALID, "stk_batch_add_scene: unknown material type or negative index of refraction");
+    }
     for (uint32_t i = 0; i < s->numInstances; ++i) {
         const StkInstance& ic = s->inst[i];
         if (ic.indexOffset < 0 || ic.indexCount < 0 || ic.indexCount % 3 ||

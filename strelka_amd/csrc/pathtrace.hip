@@ -323,21 +323,24 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
             const StkInstance& ic = sc.inst[inst];
             if (ic.lightId != -1) {
                 const bool own = (uint32_t)ic.lightId < desc.lightBase + desc.lightCount;
+                // rayD.w < 0: the ray left a specular vertex (specularBounce)
+                const bool afterSpecular = depth > 0 && ps.rayD[slot].w < 0.0f;
                 if (lp.refCompat) {
-                    // emitter: counted on camera rays (shaders/pathtracerMdl.hlsl:229-234);
-                    // a BSDF-sampled emitter hit adds nothing and ends the path
-                    if (depth == 0 && own) {
+                    // emitter: counted on camera rays and after specular bounces
+                    // (shaders/pathtracerMdl.hlsl:229-234); a BSDF-sampled emitter
+                    // hit after a diffuse or glossy bounce adds nothing and ends the path
+                    if ((depth == 0 || afterSpecular) && own) {
                         col = col + thr * ld3(sc.lights[ic.lightId].color);
                         ps.col[slot] = f4(col);
                     }
                 } else if (own) {
-                    // corrected: the front face emits; after a bounce the hit is
-                    // weighted against the light sample of the previous vertex
-                    // (balance heuristic, both pdfs in solid angle)
+                    // corrected: the front face emits; after a diffuse or glossy
+                    // bounce the hit is weighted against the light sample of the
+                    // previous vertex (balance heuristic, both pdfs in solid angle)
                     const StkLight& l = sc.lights[ic.lightId];
                     if (dot(f3(ps.rayD[slot]), rect_light_normal(l)) < 0.0f) {
                         float w = 1.0f;
-                        if (depth > 0) {
+                        if (depth > 0 && !afterSpecular) {
                             const float bpdf = ps.rayD[slot].w;
                             const float S = sphquad_init(l, f3(ps.rayO[slot])).S;
                             const float lpdf = S > 0.0f ? (1.0f / (float)desc.lightCount) / S : 0.0f;
@@ -360,20 +363,25 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
                 const F3 n1 = xform3(ic.normalMatrix, unpack_normal(v1.normal));
                 const F3 n2 = xform3(ic.normalMatrix, unpack_normal(v2.normal));
                 const F3 P = interp(p0, p1, p2, h.x, h.y);
-                const F3 N = nrm(interp(n0, n1, n2, h.x, h.y));
                 StkMaterial mat;
                 if (ic.materialId >= 0 && (uint32_t)ic.materialId < sc.numMats) {
                     mat = sc.mats[ic.materialId];
                 } else {
                     mat.type = STK_MAT_LAMBERT;
                     mat.roughness = 1.0f;
-                    mat.albedoTexture = mat.roughnessTexture = 0;
+                    mat.albedoTexture = mat.roughnessTexture = mat.normalTexture = mat.emissionTexture = 0;
                     mat.albedo[0] = mat.albedo[1] = mat.albedo[2] = 0.5f;
                     mat.albedo[3] = 1.0f;
+                    mat.emission[0] = mat.emission[1] = mat.emission[2] = 0.0f;
+                    mat.ior = 0.0f;
                 }
+                F3 N = nrm(interp(n0, n1, n2, h.x, h.y));
+                F3 emit = ld3(mat.emission);
                 const bool albTex = mat.albedoTexture != 0 && mat.albedoTexture <= sc.numTextures;
                 const bool roughTex = mat.roughnessTexture != 0 && mat.roughnessTexture <= sc.numTextures;
-                if (albTex || roughTex) {
+                const bool normTex = mat.normalTexture != 0 && mat.normalTexture <= sc.numTextures;
+                const bool emitTex = mat.emissionTexture != 0 && mat.emissionTexture <= sc.numTextures;
+                if (albTex || roughTex || normTex || emitTex) {
                     // texture coordinates as shaders/pathtracerMdl.hlsl:261-265
                     float s0, t0, s1, t1, s2, t2;
                     unpack_uv(v0.uv, s0, t0);
@@ -389,15 +397,32 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
                     }
                     if (roughTex)
                         mat.roughness = mat.roughness * tex_lookup(sc.textures[mat.roughnessTexture - 1], sc.texels, s, t).x;
+                    if (emitTex) emit = emit * tex_lookup(sc.textures[mat.emissionTexture - 1], sc.texels, s, t);
+                    if (normTex) {
+                        // the world tangent as the reference forms it (:247-257)
+                        const F3 ta = xform3(ic.normalMatrix, unpack_normal(v0.tangent));
+                        const F3 tb = xform3(ic.normalMatrix, unpack_normal(v1.tangent));
+                        const F3 tcn = xform3(ic.normalMatrix, unpack_normal(v2.tangent));
+                        const F3 T = nrm(interp(ta, tb, tcn, h.x, h.y));
+                        const F3 bn = bumped_normal(N, T, tex_lookup(sc.textures[mat.normalTexture - 1], sc.texels, s, t));
+                        // a tangent along the normal leaves no frame: keep the normal
+                        if (!(isnan(bn.x) || isnan(bn.y) || isnan(bn.z))) N = bn;
+                    }
                 }
                 const F3 k1 = -f3(ps.rayD[slot]);
-                const float rough = mat.type == STK_MAT_GGX ? mat.roughness : 1.0f;
+                // the EDF: the front face emits (shaders/pathtracerMdl.hlsl:299-304)
+                if ((emit.x != 0.0f || emit.y != 0.0f || emit.z != 0.0f) && dot(k1, Ng) > 0.0f) {
+                    col = col + thr * emit;
+                    ps.col[slot] = f4(col);
+                }
+                const bool specular = is_specular(mat);
+                const float rough = mat.type == STK_MAT_GGX ? mat.roughness : (specular ? 0.0f : 1.0f);
                 // the cache learns radiance divided by the vertex's albedo (a
                 // Lambert vertex's outgoing radiance is albedo times a smooth
                 // irradiance term), so albedo detail need not be learned
                 const F3 alb = mk3(fmaxf(mat.albedo[0], kMinAlbedo), fmaxf(mat.albedo[1], kMinAlbedo),
                                    fmaxf(mat.albedo[2], kMinAlbedo));
-                if ((pc.mode == 1 || pc.mode == 2) && !isTrain && depth >= (int)pc.queryDepth &&
+                if ((pc.mode == 1 || pc.mode == 2) && !isTrain && !specular && depth >= (int)pc.queryDepth &&
                     rough >= pc.queryMinRough) {
                     // the path ends here after this vertex's direct light: the
                     // radiance scattered from the BSDF-sampled direction on
@@ -467,9 +492,9 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
                 if (alive && !cacheQuery) {
                     const float x0 = rnd(rng);
                     const float x1 = rnd(rng);
+                    const float x2 = rnd(rng);
                     (void)rnd(rng);
-                    (void)rnd(rng);
-                    const BsdfSample bs = bsdf_sample(mat, N, Ng, k1, x0, x1);
+                    const BsdfSample bs = bsdf_sample(mat, N, Ng, k1, x0, x1, x2);
                     if (bs.event != 0) {
                         thr = thr * bs.weight;
                         bool survive = true;
@@ -491,9 +516,10 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
                         }
                         if (survive) {
                             cont = true;
-                            nO = offset_ray(P, Ng);
+                            // offset to the side the new ray leaves by (:368-370)
+                            nO = offset_ray(P, dot(bs.k2, Ng) > 0.0f ? Ng : -Ng);
                             nD = bs.k2;
-                            nPdf = lp.refCompat ? 0.0f : bs.pdf;
+                            nPdf = bs.event >= kEventSpecularReflection ? -1.0f : (lp.refCompat ? 0.0f : bs.pdf);
                         }
                     }
                 }

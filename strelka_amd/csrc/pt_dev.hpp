@@ -271,14 +271,20 @@ __device__ __forceinline__ void unpack_uv(uint32_t p, float& s, float& t)
     s = (float)(p & 0x0000ffffu) / 16383.99999f * 20.0f - 10.0f;
 }
 
-// One axis of the lookup: the clamp of a clamped texture
-// (apply_wrap_and_crop, shaders/mdl_runtime.hlsl:93-120), then the quintic
-// smoothing of the fractional texel position (apply_smootherstep_filter,
-// :125-140).  Returns the filter's texel coordinate (x * n - 0.5).
-__device__ __forceinline__ float tex_axis(float c, uint32_t n, bool clampEdge)
+// One axis of the lookup: the wrap of apply_wrap_and_crop
+// (shaders/mdl_runtime.hlsl:93-120; repeat leaves the coordinate to the
+// fetch, mirrored repeat folds it into [0, 1], and every wrap but repeat then
+// clamps it to the edge texels' centres), then the quintic smoothing of the
+// fractional texel position (apply_smootherstep_filter, :125-140).  Returns
+// the filter's texel coordinate (x * n - 0.5).
+__device__ __forceinline__ float tex_axis(float c, uint32_t n, uint32_t wrap)
 {
     const float fn = (float)n;
-    if (clampEdge) {
+    if (wrap != STK_TEX_REPEAT) {
+        if (wrap == STK_TEX_MIRRORED_REPEAT) {
+            const float fl = floorf(c);
+            c = ((int)fl & 1) != 0 ? 1.0f - (c - fl) : c - fl;
+        }
         const float h = 0.5f / fn;
         c = fminf(fmaxf(c, h), 1.0f - h);
     }
@@ -297,11 +303,13 @@ __device__ __forceinline__ int tex_wrap(int i, int n, bool clampEdge)
     return m < 0 ? m + n : m;
 }
 
-// Bilinear RGB of texture tx at (s, t).
+// Bilinear RGB of texture tx at (s, t); a clip texture reads zero outside
+// [0, 1) on either axis (tex_lookup_float4_2d, :186-189).
 __device__ __forceinline__ F3 tex_lookup(const StkTexture& tx, const float4* __restrict__ texels, float s, float t)
 {
-    const bool ce = tx.wrap == STK_TEX_CLAMP;
-    const float x = tex_axis(s, tx.width, ce), y = tex_axis(t, tx.height, ce);
+    if (tx.wrap == STK_TEX_CLIP && (s < 0.0f || s >= 1.0f || t < 0.0f || t >= 1.0f)) return mk3(0.0f, 0.0f, 0.0f);
+    const bool ce = tx.wrap != STK_TEX_REPEAT;
+    const float x = tex_axis(s, tx.width, tx.wrap), y = tex_axis(t, tx.height, tx.wrap);
     const float x0 = floorf(x), y0 = floorf(y);
     const float fx = x - x0, fy = y - y0;
     const int w = (int)tx.width, h = (int)tx.height;
@@ -316,13 +324,37 @@ __device__ __forceinline__ F3 tex_lookup(const StkTexture& tx, const float4* __r
                (a.z * gx + b.z * fx) * gy + (c.z * gx + d.z * fx) * fy);
 }
 
+// CalcBumpedNormal (shaders/helper.h:101-115): the tangent negated and
+// made orthogonal to the normal, the bitangent N x T, and the map's texel
+// (rgb = (n + 1) / 2) taken from that tangent frame to world space.
+__device__ __forceinline__ F3 bumped_normal(F3 normal, F3 tangent, F3 texel)
+{
+    const F3 n = nrm(normal);
+    F3 t = -nrm(tangent);
+    t = nrm(t - n * dot(t, n));
+    const F3 b = cross(n, t);
+    const F3 m = mk3(texel.x * 2.0f - 1.0f, texel.y * 2.0f - 1.0f, texel.z * 2.0f - 1.0f);
+    return nrm((t * m.x + b * m.y) + n * m.z);
+}
+
 // ---- BSDFs --------------------------------------------------------------------
+// Events of a BSDF sample: 0 absorb, 1 diffuse reflection, 2 glossy
+// reflection, 3 specular reflection, 4 specular transmission (the reference's
+// BSDF_EVENT_* classes, shaders/mdl_types.hlsl:235-248).  Specular samples
+// report pdf 0.
+constexpr int kEventSpecularReflection = 3;
+constexpr int kEventSpecularTransmission = 4;
 struct BsdfSample {
     F3 k2;
     float pdf;
     F3 weight; // bsdf * cos / pdf
-    int event; // 0 absorb, 1 diffuse reflection, 2 glossy reflection
+    int event;
 };
+
+__device__ __forceinline__ bool is_specular(const StkMaterial& m)
+{
+    return m.type == STK_MAT_MIRROR || m.type == STK_MAT_GLASS;
+}
 
 __device__ __forceinline__ void tangent_space(F3 n, F3& t, F3& b)
 {
@@ -359,16 +391,58 @@ __device__ __forceinline__ F3 schlick(F3 f0, float c)
     return mk3(f0.x + (1.0f - f0.x) * w, f0.y + (1.0f - f0.y) * w, f0.z + (1.0f - f0.z) * w);
 }
 
-__device__ __forceinline__ BsdfSample bsdf_sample(const StkMaterial& m, F3 N, F3 Ng, F3 k1, float x0, float x1)
+__device__ __forceinline__ BsdfSample bsdf_sample(const StkMaterial& m, F3 N, F3 Ng, F3 k1, float x0, float x1,
+                                                  float x2)
 {
     BsdfSample s;
     s.k2 = mk3(0, 0, 0);
     s.pdf = 0.0f;
     s.weight = mk3(0, 0, 0);
     s.event = 0;
+    const F3 albedo = ld3(m.albedo);
+    if (m.type == STK_MAT_MIRROR) {
+        const float c1 = dot(N, k1);
+        const F3 k2 = N * (2.0f * c1) - k1;
+        if (c1 <= 0.0f || dot(k2, Ng) <= 0.0f) return s;
+        s.k2 = k2;
+        s.weight = schlick(albedo, c1);
+        s.event = kEventSpecularReflection;
+        return s;
+    }
+    if (m.type == STK_MAT_GLASS) {
+        // the side is the geometric normal's: entering through the front face
+        const float ior = m.ior > 0.0f ? m.ior : 1.5f;
+        const bool entering = dot(k1, Ng) > 0.0f;
+        const F3 n = entering ? N : -N, ng = entering ? Ng : -Ng;
+        const float eta = entering ? 1.0f / ior : ior; // n1 / n2
+        const float c1 = dot(n, k1);
+        if (c1 <= 0.0f) return s;
+        const float s2 = eta * eta * (1.0f - c1 * c1); // sin^2 of the refracted angle
+        float F = 1.0f, c2 = 0.0f;
+        if (s2 < 1.0f) {
+            // Fresnel's equations for unpolarised light
+            c2 = sqrtf(1.0f - s2);
+            const float rs = (eta * c1 - c2) / (eta * c1 + c2);
+            const float rp = (c1 - eta * c2) / (c1 + eta * c2);
+            F = 0.5f * (rs * rs + rp * rp);
+        }
+        if (x2 < F) {
+            const F3 k2 = n * (2.0f * c1) - k1;
+            if (dot(k2, ng) <= 0.0f) return s;
+            s.k2 = k2;
+            s.weight = mk3(1.0f, 1.0f, 1.0f);
+            s.event = kEventSpecularReflection;
+        } else {
+            const F3 k2 = n * (eta * c1 - c2) - k1 * eta; // Snell's law
+            if (dot(k2, ng) >= 0.0f) return s;
+            s.k2 = k2;
+            s.weight = albedo;
+            s.event = kEventSpecularTransmission;
+        }
+        return s;
+    }
     F3 T, B;
     tangent_space(N, T, B);
-    const F3 albedo = ld3(m.albedo);
     if (m.type == STK_MAT_GGX) {
         const float alpha = ggx_alpha(m.roughness);
         const float a2 = alpha * alpha;
@@ -404,7 +478,8 @@ __device__ __forceinline__ BsdfSample bsdf_sample(const StkMaterial& m, F3 N, F3
     return s;
 }
 
-// Returns bsdf*cos split into diffuse and glossy parts and the solid-angle pdf.
+// Returns bsdf*cos split into diffuse and glossy parts and the solid-angle pdf
+// (zero for the specular models).
 __device__ __forceinline__ void bsdf_eval(const StkMaterial& m, F3 N, F3 Ng, F3 k1, F3 k2, F3& diffuse, F3& glossy,
                                           float& pdf)
 {
@@ -412,7 +487,7 @@ __device__ __forceinline__ void bsdf_eval(const StkMaterial& m, F3 N, F3 Ng, F3 
     glossy = mk3(0, 0, 0);
     pdf = 0.0f;
     const float nk2 = dot(N, k2);
-    if (nk2 <= 0.0f || dot(k2, Ng) <= 0.0f) return;
+    if (is_specular(m) || nk2 <= 0.0f || dot(k2, Ng) <= 0.0f) return; // a specular BSDF is zero off its direction
     const F3 albedo = ld3(m.albedo);
     if (m.type == STK_MAT_GGX) {
         const float nk1 = dot(N, k1);

@@ -584,7 +584,7 @@ extern "C" STK_API stk_status stk_scene_update_instance_transform(stk_scene* s, 
 
 extern "C" STK_API stk_status stk_scene_add_material(stk_scene* s, const StkMaterial* m, uint32_t* materialId)
 {
-    if (!s || !m || !materialId) return STK_E_INVALID;
+    if (!s || !m || !materialId || m->type > STK_MAT_GLASS || !(m->ior >= 0.0f)) return STK_E_INVALID;
     try {
         *materialId = s->scene.addMaterial(*m);
     } catch (const std::bad_alloc&) {

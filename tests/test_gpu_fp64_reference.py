@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 import strelka_amd as sa
+from material_scenes import showcase
 from oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -53,6 +54,48 @@ def test_converged_render_matches_fp64_reference(ref_compat):
     o = Oracle.from_scene(s, nodes)
     p = sa.render_params(cam, spp=FP64_SPP, max_depth=8, frame=100000, sampling=1, sky=s.sky, ref_compat=ref_compat)
     m64, v64 = o.pt_frame_fp64(p, 0, 0, W, H)
+    var = vg + v64
+    ok = var > 0
+    z = (mg - m64)[ok] / np.sqrt(var[ok])
+    chi = float(np.mean(z * z))
+    tail = float(np.mean(np.abs(z) > 4.5))
+    rmse = float(np.sqrt(np.mean((mg - m64) ** 2)))
+    noise = float(np.sqrt(np.mean(var)))
+    print(f"mean pixel {m64.mean():.4f}; RMSE of GPU vs FP64 means {rmse:.5f} (RMS standard error {noise:.5f}); "
+          f"mean z^2 {chi:.3f}; |z| > 4.5 on {tail:.5f}")
+    assert 0.7 < chi < 1.4 and tail <= 0.002
+    assert rmse < 1.4 * noise
+
+
+@pytest.mark.parametrize("ref_compat", [True, False])
+def test_specular_and_emissive_materials_converge_to_fp64_reference(ref_compat):
+    """The untextured showcase scene (mirror, clear and tinted glass, an
+    emitting sphere, GGX and Lambert) at 64x48: 8,192 spp on the GPU against
+    8,192 spp of the binary64 twin, with the same statistical criteria."""
+    spp = 8192
+    print(f"case: untextured showcase scene, GPU {spp} spp vs FP64 CPU {spp} spp, ref_compat={ref_compat}", flush=True)
+    s = showcase(textured=False)
+    w, h = s.camera.dimension[0], s.camera.dimension[1]
+    ctx = sa.Context(0)
+    ctx.set_scene(s)
+    ctx.build_bvh()
+    nodes = ctx.get_bvh()
+    s1 = np.zeros((h, w, 3), np.float64)
+    s2 = np.zeros((h, w, 3), np.float64)
+    per = 1024
+    for f in range(spp // per):
+        p = sa.render_params(s.camera, spp=per, max_depth=8, frame=f, sampling=1, sky=s.sky, keep_samples=True,
+                             accumulate=False, enable_tonemap=False, ref_compat=ref_compat)
+        ctx.render(p)
+        smp = ctx.samples(w * h * per * 3).reshape(h, w, per, 3).astype(np.float64)
+        s1 += smp.sum(axis=2)
+        s2 += (smp * smp).sum(axis=2)
+    ctx.close()
+    mg = s1 / spp
+    vg = np.maximum(s2 / spp - mg * mg, 0.0) * spp / (spp - 1) / spp
+    o = Oracle.from_scene(s, nodes)
+    p = sa.render_params(s.camera, spp=spp, max_depth=8, frame=100000, sampling=1, sky=s.sky, ref_compat=ref_compat)
+    m64, v64 = o.pt_frame_fp64(p, 0, 0, w, h)
     var = vg + v64
     ok = var > 0
     z = (mg - m64)[ok] / np.sqrt(var[ok])

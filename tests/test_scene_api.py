@@ -181,3 +181,24 @@ def test_packing_matches_the_vertex_records():
     assert lib.stk_scene_camera_count(None) == 0
     assert lib.stk_scene_create(None) != 0
     assert lib.stk_scene_create_mesh(None, None, 0, None, 0, C.byref(C.c_uint32())) != 0
+
+
+def test_add_material_refuses_unknown_records():
+    """The container refuses a material of an unknown type or a negative
+    index of refraction, and keeps the specular and emissive ones."""
+    s = sa.Scene()
+    m = np.zeros(1, dtype=sa.MATERIAL)
+    for bad in (dict(type=9), dict(type=sa.MAT_GLASS, ior=-0.5)):
+        r = m.copy()
+        for k, v in bad.items():
+            r[k] = v
+        with pytest.raises(sa.StkError):
+            s.add_material(r)
+    for kind in (sa.MAT_MIRROR, sa.MAT_GLASS):
+        r = m.copy()
+        r["type"] = kind
+        r["ior"] = 1.4
+        r["emission"] = (1.0, 2.0, 3.0)
+        assert s.add_material(r) == kind - sa.MAT_MIRROR
+    got = s.data().mats
+    assert got["type"].tolist() == [sa.MAT_MIRROR, sa.MAT_GLASS] and got["emission"][1].tolist() == [1.0, 2.0, 3.0]
