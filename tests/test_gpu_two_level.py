@@ -2,13 +2,16 @@
 oracle: closest and any hits bit-exact against the reference walks over the
 expanded tree (and against the oracle's restatement of the two-level walk),
 per-sample path-traced radiance against the oracle path tracer on the expanded
-tree, and at full size (10,000 instances, 18.8M triangles) the same closest t
-as the flat tree on every camera ray."""
+tree, and at full size (10,000 instances, 18.8M triangles, 3840x2160) the same
+closest t as the flat tree on every camera ray, per-sample radiance of the
+path tracer on three crops of the view, and the cached leg (a 2^21-entry grid
+trained by the renderer) against the cache oracle."""
 import numpy as np
 import pytest
 
 import strelka_amd as sa
 from oracle import Oracle
+from oracle.cache_ref import CacheOracle
 from oracle.pyoracle import MODE_ANY, MODE_REF, MODE_TWO_LEVEL
 
 from test_gpu_pathtrace import MAX_DIVERGENT, compare_samples
@@ -137,3 +140,79 @@ def test_cache_over_two_level_scene(ctx, small):
     cnt = ctx.counters()
     assert 0 < cnt[3] <= w * h and cnt[4] > 0
     assert np.all(np.isfinite(img))
+
+
+def test_full_instanced_scene_path_traced_samples(ctx):
+    print("case: 10,000 instances (18.8M triangles), 3840x2160, 2 spp, 6 bounces over the two-level structure; "
+          "three crops vs the oracle", flush=True)
+    s = sa.SceneData.generate(5)
+    ctx.set_scene(s)
+    ctx.build_two_level()
+    p = sa.render_params(s.camera, spp=2, max_depth=6, frame=1, sampling=1, sky=s.sky, keep_samples=True)
+    w, h = p.pt.dimension[0], p.pt.dimension[1]
+    assert (w, h) == (3840, 2160)
+    ctx.reset_accumulation()
+    ctx.render(p)
+    gpu = ctx.samples(w * h * 2 * 3).reshape(h, w, 2, 3)
+    o = Oracle.from_two_level(s, sa.HostTwoLevel.build(s))
+    for x0, y0, cw, ch in ((1872, 1053, 96, 54), (200, 300, 64, 36), (3500, 1900, 64, 36)):
+        cpu = o.pt_frame(p, x0, y0, cw, ch)
+        frac, _ = compare_samples(gpu[y0:y0 + ch, x0:x0 + cw].reshape(-1), cpu)
+        print(f"crop at {x0},{y0}: divergent samples {frac:.5f}, mean radiance {cpu.mean():.4f}")
+        assert frac <= MAX_DIVERGENT
+        assert cpu.mean() > 1e-3
+
+
+def test_full_instanced_scene_cached_leg(ctx):
+    """The cached leg of the instanced workload: a 2^21-entry grid trained
+    by the renderer over 12 cached 4K frames (queries after one bounce,
+    65,536 training paths); on the queries and training records of the last
+    frame, GPU inference and gradients equal the FP64 cache oracle's (FP16
+    rounding emulated), and the training has fallen below the best constant
+    predictor of the fresh records."""
+    print("case: 10,000 instances, 3840x2160, two-level, 2^21 grid, 12 cached frames, parity on renderer records",
+          flush=True)
+    s = sa.SceneData.generate(5)
+    ctx.set_scene(s)
+    ctx.build_two_level()
+    ctx.cache_create(log2_table=21)
+    info = ctx.cache_info()
+    assert info.log2TableSize == 21
+    w, h = s.camera.dimension[0], s.camera.dimension[1]
+    ratio = []
+    for f in range(12):
+        p = sa.render_params(s.camera, spp=1, max_depth=6, frame=f, sampling=1, sky=s.sky, cache_mode=1,
+                             cache_query_depth=1, cache_train_paths=65536)
+        img = ctx.render(p)
+        recs = ctx.cache_last_records("train")
+        sub = recs[:: max(1, recs.size // 16384)][:16384].copy()
+        t = sub["target"]
+        const = float(np.mean((t - t.mean(axis=0)) ** 2))
+        ratio.append(float(np.mean((ctx.cache_infer(sub)[:, :3] - t) ** 2)) / const)
+    print("squared error over the best constant's, per frame: " + " ".join(f"{r:.3f}" for r in ratio))
+    assert np.all(np.isfinite(img))
+    assert np.mean(ratio[-4:]) < 1.0 and np.mean(ratio[-4:]) < np.mean(ratio[:2])
+    cnt = ctx.counters()
+    queries = ctx.cache_last_records("query")
+    train = ctx.cache_last_records("train")
+    assert 0 < queries.size == cnt[3] <= w * h and train.size > 65536
+    grid, mlp = ctx.cache_get_params()
+    ctx.cache_set_params(grid, mlp)
+    o = CacheOracle(info, grid, mlp, round16=True)
+    q = queries[:: max(1, queries.size // 4096)][:4096].copy()
+    gpu = ctx.cache_infer(q)[:, :3]
+    ora = o.predict(q)
+    scale = np.abs(ora).max()
+    err = np.abs(gpu - ora).max() / scale
+    print(f"inference on renderer queries: max relative error {err:.2e} (scale {scale:.3f})")
+    assert scale > 0.01 and err < 5e-3
+    tr = train[:: max(1, train.size // 2048)][:2048].copy()
+    g_grid, g_mlp, loss = ctx.cache_gradients(tr)
+    og, dW, oloss = o.gradients(tr)
+    om = o.flat_mlp_grad(dW)
+    rel_mlp = np.linalg.norm(g_mlp - om) / np.linalg.norm(om)
+    rel_grid = np.linalg.norm(g_grid - og.ravel()) / np.linalg.norm(og)
+    print(f"gradients on renderer training records: mlp {rel_mlp:.2e}, grid {rel_grid:.2e}, "
+          f"loss {loss:.4f} vs {oloss:.4f}")
+    assert loss == pytest.approx(oloss, rel=1e-2)
+    assert rel_mlp < 2e-2 and rel_grid < 2e-2
