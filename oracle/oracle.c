@@ -468,6 +468,55 @@ static inline uint32_t xor_permute8(uint32_t m, uint32_t oct)
 
 static inline uint32_t popc8(uint32_t m) { return (uint32_t)__builtin_popcount(m & 0xFFu); }
 
+/* The kernel's slot tests (trace_dev.hpp wide_axis / wide_slot): per axis
+ * A = scale x inverse direction and the near / far planes' offsets
+ * B = (plane origin - ray origin) x inverse direction moved out by
+ * e = (765 scale + 4 |origin| + 3 |ray origin|) |inv| 2^-23, twice a bound on
+ * the rounding of t = q A + B and of the builder's decode, so no box the
+ * exact slab test of the decoded box accepts is rejected; per slot the near
+ * and far distances are one fused multiply-add each, reduced by maxNum /
+ * minNum (NaN distances of a zero direction component drop out), with 0 and
+ * min(tmax, lim) as bounds.  pad (may be NULL): the plane origins moved out
+ * by pad[a] (a mesh tree's widened boxes).  tch[s] gets slot s's entry
+ * distance; returns the entered mask. */
+static uint32_t wide_slots(const StkWideNode* w, const StkRay* r, const float inv[3], float lim, const float pad[3],
+                           float tch[8])
+{
+    float A[3], BN[3], BF[3];
+    int neg[3];
+    for (int a = 0; a < 3; ++a) {
+        const float scale = wide_scale(w->exps[a]), origin = w->origin[a], ro = r->o[a];
+        neg[a] = signbit(inv[a]) != 0;
+        A[a] = scale * inv[a];
+        const float mag = pad ? fabsf(origin) + pad[a] : fabsf(origin);
+        const float e = fmaf(765.0f, scale, fmaf(4.0f, mag, 3.0f * fabsf(ro))) * (fabsf(inv[a]) * (1.0f / 8388608.0f));
+        if (pad) {
+            BN[a] = ((neg[a] ? origin + pad[a] : origin - pad[a]) - ro) * inv[a] - e;
+            BF[a] = ((neg[a] ? origin - pad[a] : origin + pad[a]) - ro) * inv[a] + e;
+        } else {
+            const float b = (origin - ro) * inv[a];
+            BN[a] = b - e;
+            BF[a] = b + e;
+        }
+    }
+    const float L = fminf(r->o[3], lim);
+    uint32_t hits = 0;
+    for (int sl = 0; sl < 8; ++sl) {
+        float tn[3], tf[3];
+        for (int a = 0; a < 3; ++a) {
+            const float qn = (float)(neg[a] ? w->qhi[a][sl] : w->qlo[a][sl]);
+            const float qf = (float)(neg[a] ? w->qlo[a][sl] : w->qhi[a][sl]);
+            tn[a] = fmaf(qn, A[a], BN[a]);
+            tf[a] = fmaf(qf, A[a], BF[a]);
+        }
+        const float t0 = fmaxf(fmaxf(tn[0], tn[1]), fmaxf(tn[2], 0.0f));
+        const float t1 = fminf(fminf(tf[0], tf[1]), fminf(tf[2], L));
+        tch[sl] = t0;
+        if (t0 <= t1) hits |= 1u << sl;
+    }
+    return hits & w->vmask & 0xFFu;
+}
+
 int orc_closest_hit_wide(const OrcScene* s, const StkRay* r, StkHit* hit, OrcStats* st)
 {
     miss_hit(hit);
@@ -487,23 +536,9 @@ int orc_closest_hit_wide(const OrcScene* s, const StkRay* r, StkHit* hit, OrcSta
         const StkWideNode* w = &s->wide[node];
         ++nvisit;
         const float lim = (found ? best : 1e9f) * ORC_PRUNE_SLACK;
-        float sc[3], tch[8];
-        uint32_t hits = 0;
-        for (int a = 0; a < 3; ++a) sc[a] = wide_scale(w->exps[a]);
-        for (int sl = 0; sl < 8; ++sl) {
-            float lo[3], hi[3], t0;
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = fmaf((float)w->qlo[a][sl], sc[a], w->origin[a]);
-                hi[a] = fmaf((float)w->qhi[a][sl], sc[a], w->origin[a]);
-            }
-            tch[sl] = 0.0f;
-            if (orc_intersect_ray_box(r, inv, lo, hi, &t0) && t0 <= tmax && !(t0 > lim)) {
-                hits |= 1u << sl;
-                tch[sl] = t0;
-            }
-        }
+        float tch[8];
+        const uint32_t hits = wide_slots(w, r, inv, lim, NULL, tch);
         const uint32_t vmask = w->vmask & 0xFFu, imask = w->imask, leafMask = vmask & ~imask;
-        hits &= vmask;
         for (int sl = 0; sl < 8; ++sl) {
             if (!((hits & leafMask) >> sl & 1u)) continue;
             const StkTriangle* tr = &s->wideTris[w->triBase + popc8(leafMask & ((1u << sl) - 1u))];
@@ -647,35 +682,6 @@ uint32_t orc_expand_two_level(const StkBvhNode* top, uint32_t ntop, const StkBvh
     if (n > capacity) return 0;
     expand_emit(&c, 0, 0, STK_INVALID_INDEX);
     return c.overflow ? 0 : n;
-}
-
-/* Slab tests of the 8 slots of a wide node (decoded as the kernel decodes them,
- * widened by pad[a]: the origin moved out by pad before the decode); tch[s]
- * gets the entry distance.  Returns the entered mask (t0 <= tmax and not
- * beyond lim). */
-static uint32_t wide_slots(const StkWideNode* w, const StkRay* r, const float inv[3], float lim, const float pad[3],
-                           float tch[8])
-{
-    float sc[3], olo[3], ohi[3];
-    uint32_t hits = 0;
-    for (int a = 0; a < 3; ++a) {
-        sc[a] = wide_scale(w->exps[a]);
-        olo[a] = pad ? w->origin[a] - pad[a] : w->origin[a];
-        ohi[a] = pad ? w->origin[a] + pad[a] : w->origin[a];
-    }
-    for (int sl = 0; sl < 8; ++sl) {
-        float lo[3], hi[3], t0;
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = fmaf((float)w->qlo[a][sl], sc[a], olo[a]);
-            hi[a] = fmaf((float)w->qhi[a][sl], sc[a], ohi[a]);
-        }
-        tch[sl] = 0.0f;
-        if (orc_intersect_ray_box(r, inv, lo, hi, &t0) && t0 <= r->o[3] && !(t0 > lim)) {
-            hits |= 1u << sl;
-            tch[sl] = t0;
-        }
-    }
-    return hits & w->vmask & 0xFFu;
 }
 
 /* The kernel's object-space ray and box widening (trace_dev.hpp object_ray). */

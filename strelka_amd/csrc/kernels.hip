@@ -53,8 +53,8 @@ static_assert(2 * kWideStack <= kStackDepth, "the wide walk's two stacks share t
 // at most D - 1 entries: a tree of at most S + 1 levels fits.  Fewer entries
 // mean less LDS per wave and more waves per CU.
 template <int S>
-__global__ __launch_bounds__(kWalkBlock) void k_trace_closest(BvhView bvh, const StkRay* __restrict__ rays,
-                                                              uint32_t n, StkHit* __restrict__ hits)
+__global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(S <= 11 ? 7 : 5, 8))) void k_trace_closest(
+    BvhView bvh, const StkRay* __restrict__ rays, uint32_t n, StkHit* __restrict__ hits)
 {
     __shared__ uint32_t stack[2 * S * kWalkBlock];
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kWalkBlock + threadIdx.x;

@@ -399,36 +399,70 @@ struct WideVisit {
     float t[8];                   // entry distance of slot s
 };
 
-template <int S>
-__device__ __forceinline__ float qbyte(uint32_t lo4, uint32_t hi4)
+// Byte K of w as a float (one v_cvt_f32_ubyteK).
+template <int K>
+__device__ __forceinline__ float ubyte(uint32_t w)
 {
-    const uint32_t w = S < 4 ? lo4 : hi4;
-    return (float)((w >> (8 * (S & 3))) & 0xFFu); // one v_cvt_f32_ubyteN
+    return (float)((w >> (8 * K)) & 0xFFu);
 }
 
+// The slab test of slot S: per axis the near and far planes' distances as
+// one fused multiply-add each, t = q * A + B, with the node's per-axis terms
+// A and B (near and far) of wide_axis.  nearW[2a] / nearW[2a + 1] hold axis
+// a's near-plane bytes of slots 0-3 / 4-7, farW its far-plane bytes.
 template <int S>
-__device__ __forceinline__ void wide_slot(const uint4& q0, const uint4& q1, const uint4& q2, float sx, float sy,
-                                          float sz, const float (&lo)[3], const float (&hi)[3], const RayF& r,
-                                          float lim, uint32_t& hits, float& t)
+__device__ __forceinline__ void wide_slot(const uint32_t (&nearW)[6], const uint32_t (&farW)[6], const float (&A)[3],
+                                          const float (&BN)[3], const float (&BF)[3], float L, uint32_t& hits,
+                                          float& t)
 {
-    // origin + q * scale with q * scale exact (q < 2^8, scale a power of two),
-    // so the fused form rounds once, exactly like the builder's check; lo / hi
-    // are the node origin, or the origin moved out by a widening
-    const float lox = fmaf(qbyte<S>(q0.x, q0.y), sx, lo[0]);
-    const float loy = fmaf(qbyte<S>(q0.z, q0.w), sy, lo[1]);
-    const float loz = fmaf(qbyte<S>(q1.x, q1.y), sz, lo[2]);
-    const float hix = fmaf(qbyte<S>(q1.z, q1.w), sx, hi[0]);
-    const float hiy = fmaf(qbyte<S>(q2.x, q2.y), sy, hi[1]);
-    const float hiz = fmaf(qbyte<S>(q2.z, q2.w), sz, hi[2]);
-    float t0;
-    const bool in = slab(r, lox, hix, loy, hiy, loz, hiz, t0) && t0 <= r.tmax && !(t0 > lim);
-    hits |= in ? (1u << S) : 0u;
+    const int w = S < 4 ? 0 : 1;
+    const float nx = fmaf(ubyte<S & 3>(nearW[0 + w]), A[0], BN[0]);
+    const float ny = fmaf(ubyte<S & 3>(nearW[2 + w]), A[1], BN[1]);
+    const float nz = fmaf(ubyte<S & 3>(nearW[4 + w]), A[2], BN[2]);
+    const float fx = fmaf(ubyte<S & 3>(farW[0 + w]), A[0], BF[0]);
+    const float fy = fmaf(ubyte<S & 3>(farW[2 + w]), A[1], BF[1]);
+    const float fz = fmaf(ubyte<S & 3>(farW[4 + w]), A[2], BF[2]);
+    const float t0 = fmaxf(fmaxf(nx, ny), fmaxf(nz, 0.0f));
+    const float t1 = fminf(fminf(fx, fy), fminf(fz, L));
+    hits |= (t0 <= t1) ? (1u << S) : 0u;
     t = t0;
 }
 
-// PAD: every child box widened by pad[a] per axis (the origin moved out by
-// pad before the decode: one more rounding, far inside the widening).
-// pad = 0 decodes exactly like the unwidened form.
+// Per-axis terms of a visited node for the ray: A = scale x inverse
+// direction, and the near and far planes' offsets B = (plane origin - ray
+// origin) x inverse direction moved out by a bound on the rounding of the
+// fused form and of the decode the builder checks its boxes with (u = 2^-24):
+//   |t - exact| <= u (|origin| + 255 scale) |inv|   the decode's rounding
+//                + u 255 |A|                        A's rounding, times q
+//                + 2u |B| + u |t|                   B's roundings, the fma's
+//              <= u (765 scale + 4 |origin| + 3 |ray origin|) |inv|,
+// taken twice; so a slot's test never rejects a box the exact slab test of
+// its decoded box accepts.  A zero direction component (infinite inverse)
+// gives NaN distances on that axis, which the maxNum / minNum reductions
+// ignore: the axis then accepts (the walk stays conservative).  The near plane
+// is the low one unless the inverse direction is negative.  PAD: the plane
+// origins moved out by pad (the widened boxes of a mesh tree).
+template <bool PAD>
+__device__ __forceinline__ void wide_axis(float origin, float pad, float scale, float ro, float inv, float& A,
+                                          float& BN, float& BF)
+{
+    A = scale * inv;
+    const float mag = PAD ? fabsf(origin) + pad : fabsf(origin);
+    const float e = fmaf(765.0f, scale, fmaf(4.0f, mag, 3.0f * fabsf(ro))) * (fabsf(inv) * (1.0f / 8388608.0f));
+    if (PAD) {
+        const bool neg = signbit(inv);
+        BN = ((neg ? origin + pad : origin - pad) - ro) * inv - e;
+        BF = ((neg ? origin - pad : origin + pad) - ro) * inv + e;
+    } else {
+        const float b = (origin - ro) * inv;
+        BN = b - e;
+        BF = b + e;
+    }
+}
+
+// PAD: every child box widened by pad[a] per axis (the plane origins moved
+// out by pad before the decode).  pad = 0 decodes exactly like the unwidened
+// form.
 template <bool PAD>
 __device__ __forceinline__ void wide_visit_t(const float4* node, const RayF& r, float lim, const float (&pad)[3],
                                              WideVisit& v)
@@ -447,23 +481,25 @@ __device__ __forceinline__ void wide_visit_t(const float4* node, const RayF& r, 
     v.triBase = float_to_bits(w1.y);
     const uint32_t vmask = float_to_bits(w1.z) & 0xFFu;
     v.leafMask = vmask & ~v.imask;
-    float lo[3] = {w0.x, w0.y, w0.z}, hi[3] = {w0.x, w0.y, w0.z};
-    if (PAD) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = lo[a] - pad[a];
-            hi[a] = hi[a] + pad[a];
-        }
-    }
+    float A[3], BN[3], BF[3];
+    wide_axis<PAD>(w0.x, pad[0], sx, r.ox, r.ix, A[0], BN[0], BF[0]);
+    wide_axis<PAD>(w0.y, pad[1], sy, r.oy, r.iy, A[1], BN[1], BF[1]);
+    wide_axis<PAD>(w0.z, pad[2], sz, r.oz, r.iz, A[2], BN[2], BF[2]);
+    const bool nx = signbit(r.ix), ny = signbit(r.iy), nz = signbit(r.iz);
+    const uint32_t nearW[6] = {nx ? q1.z : q0.x, nx ? q1.w : q0.y, ny ? q2.x : q0.z,
+                               ny ? q2.y : q0.w, nz ? q2.z : q1.x, nz ? q2.w : q1.y};
+    const uint32_t farW[6] = {nx ? q0.x : q1.z, nx ? q0.y : q1.w, ny ? q0.z : q2.x,
+                              ny ? q0.w : q2.y, nz ? q1.x : q2.z, nz ? q1.y : q2.w};
+    const float L = fminf(r.tmax, lim);
     uint32_t hits = 0;
-    wide_slot<0>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[0]);
-    wide_slot<1>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[1]);
-    wide_slot<2>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[2]);
-    wide_slot<3>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[3]);
-    wide_slot<4>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[4]);
-    wide_slot<5>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[5]);
-    wide_slot<6>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[6]);
-    wide_slot<7>(q0, q1, q2, sx, sy, sz, lo, hi, r, lim, hits, v.t[7]);
+    wide_slot<0>(nearW, farW, A, BN, BF, L, hits, v.t[0]);
+    wide_slot<1>(nearW, farW, A, BN, BF, L, hits, v.t[1]);
+    wide_slot<2>(nearW, farW, A, BN, BF, L, hits, v.t[2]);
+    wide_slot<3>(nearW, farW, A, BN, BF, L, hits, v.t[3]);
+    wide_slot<4>(nearW, farW, A, BN, BF, L, hits, v.t[4]);
+    wide_slot<5>(nearW, farW, A, BN, BF, L, hits, v.t[5]);
+    wide_slot<6>(nearW, farW, A, BN, BF, L, hits, v.t[6]);
+    wide_slot<7>(nearW, farW, A, BN, BF, L, hits, v.t[7]);
     hits &= vmask;
     v.innerHits = hits & v.imask;
     v.leafHits = hits & v.leafMask;
