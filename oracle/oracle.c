@@ -473,12 +473,14 @@ static inline uint32_t popc8(uint32_t m) { return (uint32_t)__builtin_popcount(m
  * B = (plane origin - ray origin) x inverse direction moved out by
  * e = (765 scale + 4 |origin| + 3 |ray origin|) |inv| 2^-23, twice a bound on
  * the rounding of t = q A + B and of the builder's decode, so no box the
- * exact slab test of the decoded box accepts is rejected; per slot the near
- * and far distances are one fused multiply-add each, reduced by maxNum /
- * minNum (NaN distances of a zero direction component drop out), with 0 and
- * min(tmax, lim) as bounds.  pad (may be NULL): the plane origins moved out
+ * exact slab test of the decoded box accepts is rejected (the inverse
+ * direction clamped to +-2^80, so a zero component gives distances beyond
+ * any hit instead of undefined ones); per slot the near and far distances
+ * are one fused multiply-add each, reduced with 0 and min(tmax, lim) as
+ * bounds.  pad (may be NULL): the plane origins moved out
  * by pad[a] (a mesh tree's widened boxes).  tch[s] gets slot s's entry
  * distance; returns the entered mask. */
+#define ORC_INV_CLAMP 1208925819614629174706176.0f /* 2^80 */
 static uint32_t wide_slots(const StkWideNode* w, const StkRay* r, const float inv[3], float lim, const float pad[3],
                            float tch[8])
 {
@@ -486,15 +488,16 @@ static uint32_t wide_slots(const StkWideNode* w, const StkRay* r, const float in
     int neg[3];
     for (int a = 0; a < 3; ++a) {
         const float scale = wide_scale(w->exps[a]), origin = w->origin[a], ro = r->o[a];
-        neg[a] = signbit(inv[a]) != 0;
-        A[a] = scale * inv[a];
+        const float ia = fminf(fmaxf(inv[a], -ORC_INV_CLAMP), ORC_INV_CLAMP);
+        neg[a] = signbit(ia) != 0;
+        A[a] = scale * ia;
         const float mag = pad ? fabsf(origin) + pad[a] : fabsf(origin);
-        const float e = fmaf(765.0f, scale, fmaf(4.0f, mag, 3.0f * fabsf(ro))) * (fabsf(inv[a]) * (1.0f / 8388608.0f));
+        const float e = fmaf(765.0f, scale, fmaf(4.0f, mag, 3.0f * fabsf(ro))) * (fabsf(ia) * (1.0f / 8388608.0f));
         if (pad) {
-            BN[a] = ((neg[a] ? origin + pad[a] : origin - pad[a]) - ro) * inv[a] - e;
-            BF[a] = ((neg[a] ? origin - pad[a] : origin + pad[a]) - ro) * inv[a] + e;
+            BN[a] = ((neg[a] ? origin + pad[a] : origin - pad[a]) - ro) * ia - e;
+            BF[a] = ((neg[a] ? origin - pad[a] : origin + pad[a]) - ro) * ia + e;
         } else {
-            const float b = (origin - ro) * inv[a];
+            const float b = (origin - ro) * ia;
             BN[a] = b - e;
             BF[a] = b + e;
         }

@@ -437,11 +437,15 @@ __device__ __forceinline__ void wide_slot(const uint32_t (&nearW)[6], const uint
 //                + 2u |B| + u |t|                   B's roundings, the fma's
 //              <= u (765 scale + 4 |origin| + 3 |ray origin|) |inv|,
 // taken twice; so a slot's test never rejects a box the exact slab test of
-// its decoded box accepts.  A zero direction component (infinite inverse)
-// gives NaN distances on that axis, which the maxNum / minNum reductions
-// ignore: the axis then accepts (the walk stays conservative).  The near plane
-// is the low one unless the inverse direction is negative.  PAD: the plane
-// origins moved out by pad (the widened boxes of a mesh tree).
+// its decoded box accepts.  The inverse direction is clamped to +-2^80 (a
+// zero component's infinite inverse would make q A + B undefined): for a
+// ray parallel to an axis's planes every distance on that axis is then
+// beyond 2^30 (further than any hit) unless the origin lies within the
+// margin of a plane, so the axis still accepts exactly the slabs that hold
+// the origin, and nothing overflows.  The near plane is the low one unless
+// the inverse direction is negative.  PAD: the plane origins moved out by
+// pad (the widened boxes of a mesh tree).
+constexpr float kInvClamp = 1208925819614629174706176.0f; // 2^80
 template <bool PAD>
 __device__ __forceinline__ void wide_axis(float origin, float pad, float scale, float ro, float inv, float& A,
                                           float& BN, float& BF)
@@ -482,10 +486,13 @@ __device__ __forceinline__ void wide_visit_t(const float4* node, const RayF& r, 
     const uint32_t vmask = float_to_bits(w1.z) & 0xFFu;
     v.leafMask = vmask & ~v.imask;
     float A[3], BN[3], BF[3];
-    wide_axis<PAD>(w0.x, pad[0], sx, r.ox, r.ix, A[0], BN[0], BF[0]);
-    wide_axis<PAD>(w0.y, pad[1], sy, r.oy, r.iy, A[1], BN[1], BF[1]);
-    wide_axis<PAD>(w0.z, pad[2], sz, r.oz, r.iz, A[2], BN[2], BF[2]);
-    const bool nx = signbit(r.ix), ny = signbit(r.iy), nz = signbit(r.iz);
+    const float ix = fminf(fmaxf(r.ix, -kInvClamp), kInvClamp);
+    const float iy = fminf(fmaxf(r.iy, -kInvClamp), kInvClamp);
+    const float iz = fminf(fmaxf(r.iz, -kInvClamp), kInvClamp);
+    wide_axis<PAD>(w0.x, pad[0], sx, r.ox, ix, A[0], BN[0], BF[0]);
+    wide_axis<PAD>(w0.y, pad[1], sy, r.oy, iy, A[1], BN[1], BF[1]);
+    wide_axis<PAD>(w0.z, pad[2], sz, r.oz, iz, A[2], BN[2], BF[2]);
+    const bool nx = signbit(ix), ny = signbit(iy), nz = signbit(iz);
     const uint32_t nearW[6] = {nx ? q1.z : q0.x, nx ? q1.w : q0.y, ny ? q2.x : q0.z,
                                ny ? q2.y : q0.w, nz ? q2.z : q1.x, nz ? q2.w : q1.y};
     const uint32_t farW[6] = {nx ? q0.x : q1.z, nx ? q0.y : q1.w, ny ? q0.z : q2.x,

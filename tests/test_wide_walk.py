@@ -78,3 +78,45 @@ def test_config2_full_frame_fp_edge_count():
     pick = np.arange(0, rays.size, rays.size // 192)[:192]
     brute, _ = o.trace(rays[pick].copy(), MODE_BRUTE)
     assert same(ref[pick], brute)
+
+
+def test_axis_parallel_rays_cost_what_oblique_rays_cost():
+    """Rays parallel to two axes' planes (infinite inverse components): the
+    slot test clamps the inverse, so such an axis still rejects the slabs
+    that do not hold the origin.  Were the axis left unconstrained, these
+    rays would visit a large part of the tree; here they cost about what
+    oblique rays do, on the flat tree of the 1M-triangle scene and on the
+    two-level structure of the instanced scene (mesh trees in object space)."""
+    from oracle.pyoracle import MODE_TWO_LEVEL
+    s, b, o = scene_oracle(2)
+    rng = np.random.default_rng(9)
+    n = 400
+    lo, hi = np.array(b.header.lo), np.array(b.header.hi)
+
+    def rays_with(dirs):
+        r = np.zeros(n, sa.RAY)
+        r["o"][:, :3] = rng.uniform(lo, hi, (n, 3))
+        r["o"][:, 3] = 1e9
+        r["d"][:, :3] = dirs
+        return r
+
+    d = rng.normal(size=(n, 3))
+    oblique = rays_with(d / np.linalg.norm(d, axis=1, keepdims=True))
+    axis = rays_with(np.array([[0.0, -1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, -1.0], [0.0, 1.0, 0.0]] * (n // 4)))
+    _, so = o.trace(oblique, MODE_WIDE)
+    ref, _ = o.trace(axis, MODE_REF)
+    wide, sa_ = o.trace(axis, MODE_WIDE)
+    assert same(ref, wide)
+    print(f"flat tree: {so.inner / n:.1f} visits per oblique ray, {sa_.inner / n:.1f} per axis-parallel ray")
+    assert sa_.inner < 3 * so.inner
+    s5 = sa.SceneData.generate(5, 60)
+    tl = sa.HostTwoLevel.build(s5)
+    o5 = Oracle.from_two_level(s5, tl)
+    lo, hi = np.min(tl.top_leaves["lo"], axis=0), np.max(tl.top_leaves["hi"], axis=0)
+    oblique, axis = rays_with(d / np.linalg.norm(d, axis=1, keepdims=True)), rays_with(axis["d"][:, :3])
+    _, so = o5.trace(oblique, MODE_TWO_LEVEL)
+    ref, _ = o5.trace(axis, MODE_REF)
+    two, st = o5.trace(axis, MODE_TWO_LEVEL)
+    assert same(ref, two)
+    print(f"two-level: {so.inner / n:.1f} visits per oblique ray, {st.inner / n:.1f} per axis-parallel ray")
+    assert st.inner < 3 * so.inner
