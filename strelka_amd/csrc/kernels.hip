@@ -46,7 +46,8 @@ static_assert(2 * kWideStack <= kStackDepth, "the wide walk's two stacks share t
 
 // Closest hit over the compressed 8-wide tree (two LDS node-group stacks per
 // lane).  The binary walk has a kernel of its own (56 registers instead of
-// the wide walk's 67).
+// the wide walk's 72).  The 11-entry form is held to 72 registers, so 7
+// waves per SIMD fit beside its 5.5 KiB of stacks per wave.
 // S: stack entries per lane.  At a node of level L (the root is level 1) the
 // stack holds at most one group of remaining siblings per level 2..L, and only
 // a node with inner children pushes, so a walk over a tree of D levels holds
