@@ -392,19 +392,21 @@ def cache_study(args) -> dict:
         ctx.build_bvh_gpu()
     else:
         ctx.build_bvh()
-    # two pass streams: plain frames overlap each other (cached frames run on
-    # one stream: each trains on the previous frame's network)
-    ctx.set_launch_config(block_size=64, tile=8, pass_streams=2)
+    # pass streams: a frame's bounces overlap the previous frame's tail (and,
+    # cached, its inference and training, which every cached frame waits for
+    # before its own inference)
+    streams = getattr(args, "study_streams", 2)
+    ctx.set_launch_config(block_size=64, tile=8, pass_streams=streams)
     npix = W * H
     default_paths = max(1024, int(65536 * npix / (1920 * 1080)))
     d_out = ctx.alloc(npix * 16)
     out = np.zeros((H, W, 4), np.float32)
 
-    def params(frame, mode, qd=2, paths=default_paths, min_rough=0.0, steps=1):
+    def params(frame, mode, qd=2, paths=default_paths, min_rough=0.0, steps=1, suffix=0):
         return sa.render_params(cam, spp=args.study_spp, max_depth=args.study_depth, frame=frame, sampling=1,
                                 sky=scene.sky, tonemap=1,
                                 cache_mode=mode, cache_query_depth=qd, cache_train_paths=paths,
-                                cache_min_roughness=min_rough, cache_train_steps=steps)
+                                cache_min_roughness=min_rough, cache_train_steps=steps, cache_train_suffix=suffix)
 
     ctx.reset_accumulation()
     t0 = time.perf_counter()
@@ -425,7 +427,7 @@ def cache_study(args) -> dict:
     # up to 50%)
     checkpoints = sorted({1, 2, 4, 8, 12, 16, 24} | set(range(32, args.study_frames + 1, 8)) | {args.study_frames})
 
-    def run(mode, qd=2, warmup=0, paths=default_paths, min_rough=0.0, steps=1, label=""):
+    def run(mode, qd=2, warmup=0, paths=default_paths, min_rough=0.0, steps=1, label="", suffix=0, every=1):
         # frames are issued back to back and timed between checkpoints (the
         # image read-back at a checkpoint is not timed)
         t_ms = 0.0
@@ -441,7 +443,9 @@ def cache_study(args) -> dict:
         ctx.sync()
         ctx.record(0)
         for f in range(args.study_frames):
-            ctx.render_device(params(f, mode, qd, paths, min_rough, steps), d_out)
+            # every: a query-and-train frame every `every` frames, query-only frames between
+            m = 2 if (mode == 1 and every > 1 and f % every) else mode
+            ctx.render_device(params(f, m, qd, paths, min_rough, steps, suffix), d_out)
             if f + 1 in checkpoints:
                 ctx.record(1)
                 t_ms += ctx.elapsed_ms(0, 1)
@@ -477,7 +481,8 @@ def cache_study(args) -> dict:
                          lr_half_life=v.get("lr_half_life", 0.0), min_lr=v.get("min_lr", 0.0))
         curve = run(2 if v.get("freeze") else 1, v.get("query_depth", 2), v.get("warmup", 0),
                     v.get("train_paths", default_paths),
-                    v.get("min_rough", 0.0), v.get("train_steps", 1), label=f"cached {v}")
+                    v.get("min_rough", 0.0), v.get("train_steps", 1), label=f"cached {v}",
+                    suffix=v.get("train_suffix", 0), every=v.get("train_every", 1))
         tc = time_to(curve, args.study_target)
         results.append({"variant": v, "time_to_target_s": tc, "speedup": (tp / tc) if (tp and tc) else None,
                         "curve": curve})
@@ -485,7 +490,7 @@ def cache_study(args) -> dict:
     ctx.close()
     best = min((r for r in results if r["time_to_target_s"]), key=lambda r: r["time_to_target_s"], default=None)
     return {"study": "cache", "scene": args.study_scene, "structure": args.study_structure,
-            "spp_per_frame": args.study_spp, "max_depth": args.study_depth, "resolution": [W, H], "target_rmse_tonemapped": args.study_target,
+            "pass_streams": streams, "spp_per_frame": args.study_spp, "max_depth": args.study_depth, "resolution": [W, H], "target_rmse_tonemapped": args.study_target,
             "reference_spp": args.study_ref_spp, "reference_s": round(ref_s, 2),
             "default_train_paths_per_frame": default_paths,
             "time_to_target_s": {"plain": tp, "cached_best": best["time_to_target_s"] if best else None},
@@ -493,10 +498,11 @@ def cache_study(args) -> dict:
 
 
 # The cached estimator the default run times against plain path tracing: the
-# cache answers at the second path vertex (query depth 1) after 16 training-only
-# frames, one optimiser step per frame over 16,384 training paths, learning
-# rate 1e-2 halving every 128 steps down to 2e-4.
-TIME_TO_RMSE_VARIANT = {"query_depth": 1, "warmup": 16, "train_steps": 1, "train_paths": 16384,
+# cache answers at the second path vertex (query depth 1) after 8 training-only
+# frames, one optimiser step per frame over 8,192 training paths that end one
+# vertex past the query depth with a query of their own (self-training),
+# learning rate 1e-2 halving every 128 steps down to 2e-4.
+TIME_TO_RMSE_VARIANT = {"query_depth": 1, "warmup": 8, "train_steps": 1, "train_paths": 8192, "train_suffix": 1,
                         "lr_half_life": 128, "min_lr": 0.0002}
 
 
@@ -556,6 +562,7 @@ def main() -> int:
     ap.add_argument("--study-frames", type=int, default=256)
     ap.add_argument("--study-spp", type=int, default=1, help="samples per pixel per frame of the study")
     ap.add_argument("--study-depth", type=int, default=8, help="bounces of the study's paths")
+    ap.add_argument("--study-streams", type=int, default=2, help="pass streams of both estimators in the study")
     ap.add_argument("--study-structure", choices=("flat", "two_level", "device"), default="flat",
                     help="acceleration structure of the study's scene")
     ap.add_argument("--study-ref-spp", type=int, default=2048)
@@ -570,8 +577,8 @@ def main() -> int:
                             '"lr_half_life": 128, "min_lr": 0.0002}]',
                     help="JSON list of cached-run settings: query_depth, min_rough, train_steps, warmup, freeze (no "
                          "training after the warm-up), lr, lr_half_life (Adam steps), min_lr, ema, train_paths, "
-                         "log2_table, "
-                         "max_res")
+                         "train_suffix (training paths end with a query that many vertices past the query depth), "
+                         "train_every (query-only frames between training frames), log2_table, max_res")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)

@@ -477,13 +477,18 @@ STK_API stk_status stk_event_elapsed(stk_ctx* ctx, int slotBegin, int slotEnd, f
  * No counterpart in the reference (its BVH lives in memory, frames go out as
  * PNG through HioImage, src/hdRunner/main.cpp:339-356).  BVH file: a 16-byte
  * header {"STKBVH01", node count, 0} and the skip-link StkBvhNode records that
- * stk_set_bvh accepts.  PFM: RGB float32, little-endian, bottom row first. */
+ * stk_set_bvh accepts.  PFM: RGB float32, little-endian, bottom row first.
+ * OpenEXR: uncompressed float scanlines. */
 STK_API stk_status stk_bvh_save_nodes(const StkBvhNode* nodes, uint32_t numNodes, const char* path);
 /* nodes may be NULL to query numNodes; the file must hold exactly the header
  * and numNodes records. */
 STK_API stk_status stk_bvh_load_nodes(const char* path, StkBvhNode* nodes, uint32_t capacity, uint32_t* numNodes);
 /* rgba: width x height RGBA32F, top row first (as stk_render writes it). */
 STK_API stk_status stk_image_write_pfm(const char* path, const float* rgba, uint32_t width, uint32_t height);
+/* OpenEXR 2.0 scanline file, uncompressed, channels B, G, R as 32-bit floats
+ * (alpha dropped), top row first: the lossless format image tools read for
+ * converged references.  rgba as for stk_image_write_pfm. */
+STK_API stk_status stk_image_write_exr(const char* path, const float* rgba, uint32_t width, uint32_t height);
 /* Reads an RGB PFM into RGBA32F rows (top row first, alpha 1); rgba may be
  * NULL to query the size. */
 STK_API stk_status stk_image_read_pfm(const char* path, float* rgba, size_t capacityFloats, uint32_t* width,

@@ -280,10 +280,17 @@ typedef struct StkRenderParams {
      * vertex's direct light is sampled as usual, and the radiance scattered
      * from the BSDF-sampled direction on comes from the cache. */
     uint32_t cacheQueryDepth;
-    uint32_t cacheTrainPaths; /* training paths per frame (extended to maxDepth) */
+    uint32_t cacheTrainPaths; /* training paths per frame (extended to maxDepth, or see cacheTrainSuffix) */
     float cacheMinRoughness;  /* in [0, 1]; 0 queries at cacheQueryDepth */
     uint32_t cacheTrainSteps; /* optimiser steps per frame over the frame's training records (0 = 1, <= 64) */
-    uint32_t pad[1];
+    /* 0: training paths run to maxDepth.  n > 0 (with cacheMode 1): a training
+     * path ends at the first vertex of depth >= cacheQueryDepth + n where the
+     * other paths would query, with a query of its own (self-training, Mueller
+     * et al. 2021, "Real-time Neural Radiance Caching", section 5): that vertex
+     * samples its direct light, the cache stands in for the rest, and the
+     * vertex itself is not a training record.  Shorter training paths, a
+     * smoother target. */
+    uint32_t cacheTrainSuffix;
 } StkRenderParams;
 
 #ifdef __cplusplus

@@ -355,6 +355,17 @@ def write_pfm(path, rgba: np.ndarray) -> None:
         raise StkError(st, f"stk_image_write_pfm({path})")
 
 
+def write_exr(path, rgba: np.ndarray) -> None:
+    """stk_image_write_exr: an (h, w, 4) float32 image, top row first, as an
+    uncompressed OpenEXR file of float B, G, R channels."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.float32)
+    h, w = rgba.shape[0], rgba.shape[1]
+    assert rgba.shape == (h, w, 4)
+    st = _LIB.stk_image_write_exr(os.fsencode(path), _ptr(rgba), w, h)
+    if st:
+        raise StkError(st, f"stk_image_write_exr({path})")
+
+
 def read_pfm(path) -> np.ndarray:
     """stk_image_read_pfm: an (h, w, 4) float32 image, top row first."""
     w, h = C.c_uint32(), C.c_uint32()
@@ -384,7 +395,7 @@ def render_params(camera: PathTracerParam, spp: int = 1, max_depth: int = 8, fra
                   keep_samples: bool = False, rr_start: int = 3, enable_tonemap: bool = True,
                   cache_mode: int = 0, cache_query_depth: int = 2, cache_train_paths: int = 65536,
                   cache_min_roughness: float = 0.0, cache_train_steps: int = 1,
-                  ref_compat: bool = True) -> RenderParams:
+                  ref_compat: bool = True, cache_train_suffix: int = 0) -> RenderParams:
     """StkRenderParams for a render call.  ref_compat: reproduce the
     reference estimator exactly (True) or run the corrected one (False: the
     light sample's solid-angle pdf, only non-emitters occlude, BSDF-sampled
@@ -406,6 +417,7 @@ def render_params(camera: PathTracerParam, spp: int = 1, max_depth: int = 8, fra
     p.cacheTrainPaths = cache_train_paths
     p.cacheMinRoughness = cache_min_roughness
     p.cacheTrainSteps = cache_train_steps
+    p.cacheTrainSuffix = cache_train_suffix
     for i in range(3):
         p.skyColor[i] = float(sky[i])
     p.skyColor[3] = 0.0

@@ -63,7 +63,7 @@ class RenderParams(C.Structure):
                 ("tonemapperType", C.c_uint32), ("enableAccumulation", C.c_uint32), ("skyColor", C.c_float * 4),
                 ("enableTonemap", C.c_uint32), ("keepSamples", C.c_uint32), ("cacheMode", C.c_uint32),
                 ("cacheQueryDepth", C.c_uint32), ("cacheTrainPaths", C.c_uint32),
-                ("cacheMinRoughness", C.c_float), ("cacheTrainSteps", C.c_uint32), ("pad", C.c_uint32 * 1)]
+                ("cacheMinRoughness", C.c_float), ("cacheTrainSteps", C.c_uint32), ("cacheTrainSuffix", C.c_uint32)]
 
 
 class CacheConfig(C.Structure):
@@ -236,6 +236,7 @@ PROTOTYPES = [
     ("stk_bvh_save_nodes", C.c_int, [P, U32, C.c_char_p]),
     ("stk_bvh_load_nodes", C.c_int, [C.c_char_p, P, U32, C.POINTER(C.c_uint32)]),
     ("stk_image_write_pfm", C.c_int, [C.c_char_p, P, U32, U32]),
+    ("stk_image_write_exr", C.c_int, [C.c_char_p, P, U32, U32]),
     ("stk_image_read_pfm", C.c_int, [C.c_char_p, P, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("stk_tlas_build_host", C.c_int, [P, U32, P, U32, P, U32, C.POINTER(BvhOptions), C.POINTER(P)]),
     ("stk_tlas_free", None, [P]),

@@ -1013,6 +1013,8 @@ static stk_status renderForestRun(stk_ctx* c, const stk::SceneView& base, std::v
         pc.queryMinRough = p->cacheMinRoughness;
         if (p->cacheTrainSteps > 64) return fail(c, STK_E_INVALID, "render: cacheTrainSteps must be at most 64");
         pc.trainSteps = p->cacheTrainSteps ? p->cacheTrainSteps : 1u;
+        if (p->cacheTrainSuffix > 64) return fail(c, STK_E_INVALID, "render: cacheTrainSuffix must be at most 64");
+        pc.trainSuffix = p->cacheMode == 1 ? p->cacheTrainSuffix : 0u;
         const uint32_t paths = p->cacheTrainPaths ? p->cacheTrainPaths : 65536u;
         pc.trainStride = std::max<uint32_t>(1u, (uint32_t)(slots / paths));
         pc.maxTrainPaths = (uint32_t)((slots + pc.trainStride - 1) / pc.trainStride);

@@ -114,6 +114,7 @@ struct PtCache {
     uint32_t queryDepth = 2;
     float queryMinRough = 0.0f;
     uint32_t trainSteps = 1; // optimiser steps per frame over the frame's records
+    uint32_t trainSuffix = 0; // > 0: training paths end with a query at depth >= queryDepth + trainSuffix
     uint32_t trainStride = 1;
     uint32_t trainCap = 0;
     uint32_t maxTrainPaths = 0;

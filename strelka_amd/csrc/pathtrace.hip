@@ -422,11 +422,15 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
                 // irradiance term), so albedo detail need not be learned
                 const F3 alb = mk3(fmaxf(mat.albedo[0], kMinAlbedo), fmaxf(mat.albedo[1], kMinAlbedo),
                                    fmaxf(mat.albedo[2], kMinAlbedo));
-                if ((pc.mode == 1 || pc.mode == 2) && !isTrain && !specular && depth >= (int)pc.queryDepth &&
-                    rough >= pc.queryMinRough) {
+                // a training path past its suffix queries like the others (self-training)
+                const bool selfQuery =
+                    isTrain && pc.trainSuffix != 0u && depth >= (int)(pc.queryDepth + pc.trainSuffix);
+                if ((pc.mode == 1 || pc.mode == 2) && (!isTrain || selfQuery) && !specular &&
+                    depth >= (int)pc.queryDepth && rough >= pc.queryMinRough) {
                     // the path ends here after this vertex's direct light: the
                     // radiance scattered from the BSDF-sampled direction on
-                    // comes from the cache
+                    // comes from the cache (a training path's earlier vertices
+                    // then learn from that answer; this vertex is no record)
                     cacheQuery = true;
                     qrec.posRough = make_float4(P.x, P.y, P.z, rough);
                     qrec.normal = make_float4(N.x, N.y, N.z, bits_to_float(slot));

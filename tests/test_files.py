@@ -73,3 +73,59 @@ def test_rmse_tool():
     expected = np.sqrt((15 * 3 * 0.25 + 3 * 1.0) / 48)
     assert sa.rmse(a, b) == pytest.approx(expected)
     assert sa.rmse(a, b, tonemapped=False) > expected
+
+
+def parse_exr(raw: bytes):
+    """Independent reading of an uncompressed single-part scanline OpenEXR
+    file: magic and version, the attribute list up to its empty name, the
+    scanline offset table, then per scanline {y, byte count, channel rows in
+    the channel list's order}."""
+    import struct
+    assert struct.unpack_from("<I", raw, 0)[0] == 20000630
+    version = struct.unpack_from("<I", raw, 4)[0]
+    assert version & 0xFF == 2 and not version & 0x200  # scanline, single part
+    pos, attrs = 8, {}
+    while raw[pos] != 0:
+        end = raw.index(b"\0", pos)
+        name = raw[pos:end].decode()
+        tend = raw.index(b"\0", end + 1)
+        typ = raw[end + 1:tend].decode()
+        size = struct.unpack_from("<i", raw, tend + 1)[0]
+        attrs[name] = (typ, raw[tend + 5:tend + 5 + size])
+        pos = tend + 5 + size
+    pos += 1
+    chans, cl = [], attrs["channels"][1]
+    p = 0
+    while cl[p] != 0:
+        end = cl.index(b"\0", p)
+        ptype, _lin, xs, ys = struct.unpack_from("<iB3xii", cl, end + 1)
+        chans.append((cl[p:end].decode(), ptype, xs, ys))
+        p = end + 1 + 16
+    assert attrs["compression"] == ("compression", b"\0")
+    x0, y0, x1, y1 = struct.unpack("<4i", attrs["dataWindow"][1])
+    w, h = x1 - x0 + 1, y1 - y0 + 1
+    offsets = struct.unpack_from(f"<{h}Q", raw, pos)
+    out = {c[0]: np.zeros((h, w), np.float32) for c in chans}
+    for off in offsets:
+        y, nbytes = struct.unpack_from("<ii", raw, off)
+        assert nbytes == 4 * w * len(chans)
+        data = np.frombuffer(raw, "<f4", w * len(chans), off + 8).reshape(len(chans), w)
+        for k, c in enumerate(chans):
+            out[c[0]][y - y0] = data[k]
+    return attrs, chans, out
+
+
+def test_exr_write(tmp_path):
+    rng = np.random.default_rng(5)
+    img = rng.uniform(-1, 8, (29, 41, 4)).astype(np.float32)
+    img[3, 7, :3] = [np.inf, -0.0, 1e-40]
+    path = tmp_path / "frame.exr"
+    sa.write_exr(path, img)
+    attrs, chans, planes = parse_exr(path.read_bytes())
+    assert [c[0] for c in chans] == ["B", "G", "R"] and all(c[1:] == (2, 1, 1) for c in chans)
+    for k in ("displayWindow", "lineOrder", "pixelAspectRatio", "screenWindowCenter", "screenWindowWidth"):
+        assert k in attrs
+    for k, name in enumerate("RGB"):
+        assert planes[name].tobytes() == np.ascontiguousarray(img[..., k]).tobytes()
+    with pytest.raises(sa.StkError):
+        sa.write_exr(tmp_path / "missing_dir" / "x.exr", img)

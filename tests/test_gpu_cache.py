@@ -288,3 +288,54 @@ def test_cached_frames_on_pass_streams_equal_one_stream():
     rel = np.abs(two - one).mean() / one.mean()
     print(f"query + train: mean relative difference between one and two streams {rel:.2e}")
     assert rel < 0.02
+
+
+def test_self_training_suffix_ends_training_paths_with_a_query():
+    """cacheTrainSuffix n: a training path ends with a query of its own at the
+    first vertex of depth >= query depth + n where the other paths query
+    (self-training).  A fresh cache answers 0 everywhere (its output layer
+    starts at zero), so the frame's samples are plain path-traced samples cut
+    after a vertex's direct light: a path that does not train equals the plain
+    render with max_depth = query depth + 1, a training path the plain render
+    with max_depth = query depth + n + 1, bit for bit.  Training records: at
+    most query depth + n per training path, fewer than full-depth training
+    paths leave."""
+    print("case: config 7 (Lambert room), query depth 1, suffix 2, fresh cache", flush=True)
+    s = sa.SceneData.generate(7)
+    assert np.all(s.mats["type"] == 0)  # Lambert only: every vertex may query
+    w, h = s.camera.dimension[0], s.camera.dimension[1]
+    n = w * h
+    ctx = sa.Context(0)
+    try:
+        ctx.set_scene(s)
+        ctx.build_bvh()
+        qd, suffix, paths = 1, 2, 2048
+
+        def plain(depth):
+            p = sa.render_params(s.camera, spp=1, max_depth=depth, frame=5, sampling=1, sky=(0.0, 0.0, 0.0),
+                                 keep_samples=True, accumulate=False)
+            ctx.render(p)
+            return ctx.samples(n * 3).reshape(n, 3).copy()
+
+        short, long = plain(qd + 1), plain(qd + suffix + 1)
+        counts = {}
+        for sfx in (suffix, 0):
+            ctx.cache_create(log2_table=14, seed=11)
+            p = sa.render_params(s.camera, spp=1, max_depth=8, frame=5, sampling=1, sky=(0.0, 0.0, 0.0),
+                                 keep_samples=True, accumulate=False, cache_mode=1, cache_query_depth=qd,
+                                 cache_train_paths=paths, cache_train_suffix=sfx)
+            ctx.render(p)
+            got = ctx.samples(n * 3).reshape(n, 3).copy()
+            counts[sfx] = ctx.cache_last_records("train").size
+            if sfx == suffix:
+                eq_short = np.all(got.view(np.uint32) == short.view(np.uint32), axis=1)
+                eq_long = np.all(got.view(np.uint32) == long.view(np.uint32), axis=1)
+                assert np.all(eq_short | eq_long), f"{np.sum(~(eq_short | eq_long))} samples are neither"
+                trained = int(np.sum(eq_long & ~eq_short))
+                print(f"training samples told apart: {trained} of about {paths}")
+                assert 0.5 * paths <= trained <= 1.01 * paths
+        print(f"training records: suffix {suffix}: {counts[suffix]}, full depth: {counts[0]}")
+        assert 0 < counts[suffix] <= paths * (qd + suffix) * 1.01
+        assert counts[suffix] < counts[0]
+    finally:
+        ctx.close()

@@ -280,3 +280,29 @@ def test_spatial_split_tree_walks_match_the_oracle():
     frac, err = compare_samples(g, o6.pt_frame(p, 0, 0, w, h))
     print(f"divergent samples {frac}, max error {err}")
     assert frac == 0
+
+
+def test_tree_loaded_from_a_file_traces_like_the_built_tree(ctx, tmp_path):
+    """A tree written to a BVH file (stk_bvh_save_nodes) and read back
+    (stk_bvh_load_nodes) into stk_set_bvh: every config-1 ray's hit equals the
+    oracle's on the built tree, and a config-6 path-traced frame over the
+    loaded tree equals the frame over the built tree bit for bit."""
+    print("case: config 1 rays and a config-6 frame over trees loaded from BVH files", flush=True)
+    s, o = setup(ctx, 1)
+    path = tmp_path / "config1.stkbvh"
+    sa.save_bvh(path, ctx.get_bvh())
+    rays = s.jittered_rays(0x5354524B00000001)
+    ref, _ = o.trace(rays, MODE_REF)
+    ctx.build_bvh(bins=4)  # a different tree in between
+    ctx.set_bvh(sa.load_bvh(path))
+    assert same(ctx.trace_closest(rays), ref)
+    s6 = sa.SceneData.generate(6)
+    ctx.set_scene(s6)
+    ctx.build_bvh()
+    p = sa.render_params(s6.camera, spp=4, max_depth=8, frame=2, sampling=1, sky=s6.sky, accumulate=False)
+    built = ctx.render(p).copy()
+    path6 = tmp_path / "config6.stkbvh"
+    sa.save_bvh(path6, ctx.get_bvh())
+    ctx.set_scene(s6)
+    ctx.set_bvh(sa.load_bvh(path6))
+    assert np.array_equal(ctx.render(p), built)
