@@ -180,33 +180,38 @@ __global__ __launch_bounds__(256) void k_cache_keys(GridDesc gd, const CacheReco
     vals[i] = i;
 }
 
-// Grid levels of at most this many cells per axis merge their gradient
-// contributions within the wave before the atomics.
-constexpr uint32_t kMergeRes = 160;
-
 // Adds (v0, v1) to tab[2e], tab[2e + 1] for every active lane, after summing
-// the values of the wave's active lanes that hold the same key (a key names
-// one entry of one table; lanes of one key hold the same tab and e): one
-// atomic per distinct key.  Every lane of the wave must call it.
-__device__ __forceinline__ void wave_add_by_key(float* tab, uint32_t key, uint32_t e, bool active, float v0, float v1)
+// the values of consecutive active lanes of the same 16-lane row that hold the
+// same entry e (a row holds one grid level's 16 samples, which are in Morton
+// order, so lanes that share a cell are mostly adjacent): one atomic pair per
+// run.  A segmented inclusive scan over the row - the index of the run's first
+// lane by a max-scan, then Hillis-Steele steps that add only within the run -
+// leaves the run's sum in its last lane.  Every lane of the row must call it
+// (the shuffles read the row's other lanes).
+__device__ __forceinline__ void row_add_runs(float* tab, uint32_t e, bool active, float v0, float v1)
 {
-    const uint32_t lane = __lane_id();
-    uint64_t todo = __ballot(active);
-    while (todo) {
-        const int leader = __ffsll((unsigned long long)todo) - 1;
-        const uint32_t lk = (uint32_t)__shfl((int)key, leader);
-        const bool mine = ((todo >> lane) & 1ull) && key == lk;
-        const uint64_t m = __ballot(mine);
-        float s0 = mine ? v0 : 0.0f, s1 = mine ? v1 : 0.0f;
-        for (int off = 32; off > 0; off >>= 1) {
-            s0 += __shfl_xor(s0, off);
-            s1 += __shfl_xor(s1, off);
+    const int r = (int)(__lane_id() & 15u);
+    const uint32_t key = active ? e : 0xFFFFFFFFu;
+    const uint32_t prev = (uint32_t)__shfl_up((int)key, 1, 16);
+    int head = (r == 0 || prev != key) ? r : 0;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+        const int h = __shfl_up(head, off, 16);
+        if (r >= off) head = max(head, h);
+    }
+    float s0 = active ? v0 : 0.0f, s1 = active ? v1 : 0.0f;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+        const float a0 = __shfl_up(s0, off, 16), a1 = __shfl_up(s1, off, 16);
+        if (r - off >= head) {
+            s0 += a0;
+            s1 += a1;
         }
-        if ((int)lane == leader) {
-            atomicAdd(tab + 2 * e, s0);
-            atomicAdd(tab + 2 * e + 1, s1);
-        }
-        todo &= ~m;
+    }
+    const uint32_t next = (uint32_t)__shfl_down((int)key, 1, 16);
+    if (active && (r == 15 || next != key)) {
+        atomicAdd(tab + 2 * e, s0);
+        atomicAdd(tab + 2 * e + 1, s1);
     }
 }
 
@@ -273,7 +278,7 @@ __global__ __launch_bounds__(256) void k_cache_train(GridDesc gd, const __half* 
         for (int t = 0; t < 4; ++t) dcur[t] = dnext[t];
     }
     // gradient of the grid features: rows 0..31 of W0 * dY0 (every lane takes
-    // part: the coarse levels merge across the wave)
+    // part: contributions merge along each row)
     float px = 0.0f, py = 0.0f, pz = 0.0f;
     if (valid) normalised(gd, rec.posRough, px, py, pz);
     for (int t = 0; t < 2; ++t) {
@@ -283,24 +288,11 @@ __global__ __launch_bounds__(256) void k_cache_train(GridDesc gd, const __half* 
             const int lvl = 8 * t + 2 * g + k;
             const float d0 = a[2 * k], d1 = a[2 * k + 1];
             const bool act = valid && !(d0 == 0.0f && d1 == 0.0f);
-            const bool merge = gd.res[lvl] <= kMergeRes;
             float* tab = gd.dense[lvl]
                              ? gridGDense + ((size_t)(blockIdx.x % kDenseCopies) * gd.denseEntries + gd.denseBase[lvl]) * 2
                              : gridG + ((size_t)lvl << gd.log2Table) * 2;
-            if (__ballot(merge) != 0) {
-                for_corners(gd, lvl, px, py, pz, [&](uint32_t e, float w) {
-                    wave_add_by_key(tab, e * 4u + (uint32_t)g, e, act && merge, w * d0, w * d1);
-                    if (act && !merge) {
-                        atomicAdd(tab + 2 * e, w * d0);
-                        atomicAdd(tab + 2 * e + 1, w * d1);
-                    }
-                });
-            } else if (act) {
-                for_corners(gd, lvl, px, py, pz, [&](uint32_t e, float w) {
-                    atomicAdd(tab + 2 * e, w * d0);
-                    atomicAdd(tab + 2 * e + 1, w * d1);
-                });
-            }
+            for_corners(gd, lvl, px, py, pz,
+                        [&](uint32_t e, float w) { row_add_runs(tab, e, act, w * d0, w * d1); });
         }
     }
 }
