@@ -21,10 +21,6 @@ constexpr int kBlock = 256;
 // One wave per block for the walks with LDS stacks: the stacks bound the
 // occupancy whatever the block size, and single-wave blocks retire one by one.
 constexpr int kWalkBlock = 64;
-// Stack entries of the wide closest-hit walk over trees of at most this many
-// levels: 5.5 KiB of LDS per wave instead of 8, so the 67-register walk runs 7
-// waves per SIMD instead of 5.
-constexpr int kWideStackShallow = 11;
 
 __device__ __forceinline__ void store_hit(StkHit* hits, uint32_t i, const HitF& h)
 {

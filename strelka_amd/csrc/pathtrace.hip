@@ -29,6 +29,7 @@ namespace {
 using namespace dev;
 
 constexpr int kBlock = 256;
+constexpr int kWaveBlock = 64; // one-wave blocks of the shadow-ray walks
 constexpr float kMinAlbedo = 1e-3f; // floor of the albedo the cache's targets are divided by
 constexpr int kTrainWords = 6;      // float4s recorded per training vertex
 
@@ -210,13 +211,18 @@ __global__ void k_totals_clear(PathState ps)
 
 static_assert(2 * kWideStack <= kStackDepth, "the wide walk's two stacks share the packed walk's LDS");
 
-// `stack` is this lane's column of kStackDepth words (BLOCK apart): the packed
-// walk's stack, or the wide walks' two node-group stacks of kWideStack words.
-// TL: the view's scene is a two-level structure.
-template <int BLOCK, bool TL>
+// `stack` is this lane's column of 2 S words (BLOCK apart): the wide walks'
+// two node-group stacks of S entries, or, with S = kWideStack, the packed
+// walk's stack of kStackDepth words.  With S = kWideStackShallow (trees of at
+// most S + 1 wide levels) the packed walk's stack does not fit, so a view
+// without a wide tree takes the stackless skip-link walk (same hits).
+// TL: the view's scene is a two-level structure (stacks of kWideStack).
+template <int BLOCK, bool TL, int S = kWideStack>
 __device__ __forceinline__ void extend_one(const SceneView& sc, PathState& ps, int cur, uint32_t e, uint32_t* stack,
                                            int binary)
 {
+    static_assert(S == kWideStack || S == kWideStackShallow, "wide stack sizes");
+    constexpr bool kPackedFits = 2 * S >= kStackDepth;
     const uint32_t slot = ps.queue[cur][e];
     const RayF r = make_ray(ps.rayO[slot], ps.rayD[slot]);
     const SceneDesc& d = sc.scenes[ps.scene[slot]];
@@ -224,10 +230,12 @@ __device__ __forceinline__ void extend_one(const SceneView& sc, PathState& ps, i
     uint32_t ik;
     if (TL)
         closest_two_level(sc.bvh, r, stack, stack + kWideStack * BLOCK, BLOCK, h, ik);
-    else if (!binary && d.wideRoot != STK_INVALID_INDEX)
-        closest_wide(sc.bvh, d.wideRoot, r, stack, stack + kWideStack * BLOCK, BLOCK, h);
-    else
+    else if ((!kPackedFits || !binary) && d.wideRoot != STK_INVALID_INDEX)
+        closest_wide(sc.bvh, d.wideRoot, r, stack, stack + S * BLOCK, BLOCK, h);
+    else if (kPackedFits)
         closest_packed(sc.bvh, d.hdr, r, stack, BLOCK, h);
+    else
+        closest_ref(sc.bvh, d.hdr, d.refRoot, r, h);
     ps.hit[e] = make_float4(h.u, h.v, h.t, bits_to_float(h.inst));
     ps.hitPrim[e] = h.prim;
 }
@@ -237,20 +245,21 @@ __device__ __forceinline__ void extend_one(const SceneView& sc, PathState& ps, i
 // takes 64 entries at a time from segment (w + k) % kStripes, k = 0, 1, ...,
 // moving on when a segment is exhausted (a plain load checks before the
 // atomic), so every wave ends after at most kStripes segments.
-template <int BLOCK, bool TL>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pt_extend(SceneView sc, PathState ps, int depth, int persistent,
+template <int BLOCK, bool TL, int S>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(S == kWideStackShallow ? 7 : 5, 8))) void k_pt_extend(SceneView sc, PathState ps, int depth, int persistent,
                                                      int binary)
 {
-    __shared__ uint32_t stack[(TL ? 2 * kWideStack : kStackDepth) * BLOCK];
+    __shared__ uint32_t stack[2 * (TL ? kWideStack : S) * BLOCK];
     if (pass_aborted(ps)) return;
     const int cur = depth & 1;
     const uint32_t* counts = ps.stripes + cur * kStripes;
-    if (!persistent) {
+    // the shallow-stack form is launched with stripe grids only
+    if (S == kWideStackShallow || !persistent) {
         const uint32_t stripe = blockIdx.x % kStripes, cnt = counts[stripe];
         for (uint32_t base = stripe_first(BLOCK); base < cnt; base += stripe_step(BLOCK)) {
             const uint32_t local = base + threadIdx.x;
             if (local < cnt)
-                extend_one<BLOCK, TL>(sc, ps, cur, stripe * ps.segCap + local, stack + threadIdx.x, binary);
+                extend_one<BLOCK, TL, S>(sc, ps, cur, stripe * ps.segCap + local, stack + threadIdx.x, binary);
         }
         return;
     }
@@ -278,7 +287,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5, 8))) v
             base = (uint32_t)__shfl((int)base, 0);
             if (base >= cnt) break;
             const uint32_t q = base + lane;
-            if (q < cnt) extend_one<BLOCK, TL>(sc, ps, cur, sg * ps.segCap + q, stack + threadIdx.x, binary);
+            if (q < cnt) extend_one<BLOCK, TL, S>(sc, ps, cur, sg * ps.segCap + q, stack + threadIdx.x, binary);
         }
     }
 }
@@ -575,7 +584,7 @@ constexpr uint32_t kMaxEmitters = 64;
 // refCompat: the reference's rule (the first hit leaf in preorder decides,
 // and a light hit counts as visible, Q5); otherwise any non-emitter triangle
 // before tmax occludes (the emitter walks with no emitter triangles given).
-template <bool TL, int BLOCK = kBlock>
+template <bool TL, int BLOCK, int S = kWideStack>
 __device__ __forceinline__ void connect_one(const SceneView& sc, PathState& ps, uint32_t s, uint32_t* stack, int binary,
                                             int refCompat)
 {
@@ -589,7 +598,7 @@ __device__ __forceinline__ void connect_one(const SceneView& sc, PathState& ps, 
                                 stack + threadIdx.x, stack + kWideStack * BLOCK + threadIdx.x, BLOCK);
     } else if (!binary && desc.wideRoot != STK_INVALID_INDEX && desc.emitCount <= kMaxEmitters) {
         vis = visible_wide(sc.bvh, desc.wideRoot, sc.bvh.emitters + 3 * (size_t)desc.emitBase, emitCount, r,
-                           stack + threadIdx.x, stack + kWideStack * BLOCK + threadIdx.x, BLOCK);
+                           stack + threadIdx.x, stack + S * BLOCK + threadIdx.x, BLOCK);
     } else if (!refCompat) {
         vis = !occluded_ref(sc.bvh, desc.hdr, desc.refRoot, r);
     } else {
@@ -608,17 +617,19 @@ __device__ __forceinline__ void connect_one(const SceneView& sc, PathState& ps, 
 // Shadow rays: the wide visibility walk (the reference's first-in-preorder
 // any-hit rule, decided in any order), or the stackless skip-link walk when
 // the scene has no usable wide tree or many emitter triangles; TL: the
-// two-level visibility walk.
-template <bool TL>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_pt_connect(SceneView sc, PathState ps, int binary,
+// two-level visibility walk.  One-wave blocks (a finished wave frees its
+// stacks at once); S wide stack entries per lane.
+template <bool TL, int S>
+__global__ __launch_bounds__(kWaveBlock) __attribute__((amdgpu_waves_per_eu(S == kWideStackShallow ? 7 : 5, 8))) void k_pt_connect(SceneView sc, PathState ps, int binary,
                                                                                          int refCompat)
 {
-    __shared__ uint32_t stack[2 * kWideStack * kBlock];
+    __shared__ uint32_t stack[2 * (TL ? kWideStack : S) * kWaveBlock];
     if (pass_aborted(ps)) return;
     const uint32_t stripe = blockIdx.x % kStripes, cnt = ps.stripes[2 * kStripes + stripe];
-    for (uint32_t base = stripe_first(kBlock); base < cnt; base += stripe_step(kBlock)) {
+    for (uint32_t base = stripe_first(kWaveBlock); base < cnt; base += stripe_step(kWaveBlock)) {
         const uint32_t local = base + threadIdx.x;
-        if (local < cnt) connect_one<TL>(sc, ps, stripe * ps.segCap + local, stack, binary, refCompat);
+        if (local < cnt)
+            connect_one<TL, kWaveBlock, S>(sc, ps, stripe * ps.segCap + local, stack, binary, refCompat);
     }
 }
 
@@ -821,27 +832,36 @@ inline dim3 stripeGrid(const PathState& ps, uint32_t B, uint32_t stripeThreads =
     return dim3(kStripes * perStripe);
 }
 
-template <int BLOCK, bool TL>
-void launchExtendWalk(const SceneView& sc, PathState& ps, int depth, const LaunchConfig& lc, hipStream_t s)
+template <int BLOCK, bool TL, int S>
+void launchExtendWalk(const SceneView& sc, PathState& ps, int depth, const LaunchConfig& lc, int binary, hipStream_t s)
 {
-    // the binary walk only where its stack holds the tree
-    const int binary = (lc.binaryWalk && sc.bvh.packedOk) ? 1 : 0;
     if (lc.persistentBlocks)
-        hipLaunchKernelGGL((k_pt_extend<BLOCK, TL>), dim3(lc.persistentBlocks), dim3(BLOCK), 0, s, sc, ps, depth, 1,
-                           binary);
+        hipLaunchKernelGGL((k_pt_extend<BLOCK, TL, S>), dim3(lc.persistentBlocks), dim3(BLOCK), 0, s, sc, ps, depth,
+                           1, binary);
     else
-        hipLaunchKernelGGL((k_pt_extend<BLOCK, TL>), stripeGrid(ps, BLOCK, lc.stripeThreads), dim3(BLOCK), 0, s, sc,
-                           ps, depth, 0,
-                           binary);
+        hipLaunchKernelGGL((k_pt_extend<BLOCK, TL, S>), stripeGrid(ps, BLOCK, lc.stripeThreads), dim3(BLOCK), 0, s,
+                           sc, ps, depth, 0, binary);
+}
+
+// Whether every view's walks fit stacks of kWideStackShallow entries (the
+// view records of a batch do not carry their trees' depths: a batch says
+// kWideStack).
+inline bool shallowWalks(const SceneView& sc)
+{
+    return !sc.bvh.tlas && closestWalkStack(sc.bvh) == (uint32_t)kWideStackShallow;
 }
 
 template <int BLOCK>
 void launchExtend(const SceneView& sc, PathState& ps, int depth, const LaunchConfig& lc, hipStream_t s)
 {
+    // the binary walk only where its stack holds the tree
+    const int binary = (lc.binaryWalk && sc.bvh.packedOk) ? 1 : 0;
     if (sc.bvh.tlas)
-        launchExtendWalk<BLOCK, true>(sc, ps, depth, lc, s);
+        launchExtendWalk<BLOCK, true, kWideStack>(sc, ps, depth, lc, binary, s);
+    else if (!binary && !lc.persistentBlocks && shallowWalks(sc))
+        launchExtendWalk<BLOCK, false, kWideStackShallow>(sc, ps, depth, lc, 0, s);
     else
-        launchExtendWalk<BLOCK, false>(sc, ps, depth, lc, s);
+        launchExtendWalk<BLOCK, false, kWideStack>(sc, ps, depth, lc, binary, s);
 }
 
 } // namespace
@@ -911,11 +931,14 @@ hipError_t launchPathTracePass(const SceneView& sc, const PtLaunch& lp, const La
         }
         hipLaunchKernelGGL(k_pt_shade, stripeGrid(ps, kBlock, lc.stripeThreads), b, 0, s, sc, lp, ps, depth);
         const int rc = lp.refCompat ? 1 : 0;
+        const dim3 cg = stripeGrid(ps, kWaveBlock, lc.stripeThreads), cb(kWaveBlock);
         if (sc.bvh.tlas)
-            hipLaunchKernelGGL(k_pt_connect<true>, stripeGrid(ps, kBlock, lc.stripeThreads), b, 0, s, sc, ps, 0, rc);
+            hipLaunchKernelGGL((k_pt_connect<true, kWideStack>), cg, cb, 0, s, sc, ps, 0, rc);
+        else if (shallowWalks(sc))
+            hipLaunchKernelGGL((k_pt_connect<false, kWideStackShallow>), cg, cb, 0, s, sc, ps, lc.binaryWalk ? 1 : 0,
+                               rc);
         else
-            hipLaunchKernelGGL(k_pt_connect<false>, stripeGrid(ps, kBlock, lc.stripeThreads), b, 0, s, sc, ps,
-                               lc.binaryWalk ? 1 : 0, rc);
+            hipLaunchKernelGGL((k_pt_connect<false, kWideStack>), cg, cb, 0, s, sc, ps, lc.binaryWalk ? 1 : 0, rc);
     }
     hipLaunchKernelGGL(k_pt_tally, dim3(1), dim3(64), 0, s, ps);
     if (cacheAfter && cb && (query || train)) {

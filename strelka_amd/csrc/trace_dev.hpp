@@ -52,6 +52,10 @@ struct BvhView {
 };
 
 constexpr int kWideStack = 16; // node-group entries; the builder's wide depth must not exceed it
+// Stack entries of the wide walks over trees of at most kWideStackShallow + 1
+// levels (a walk holds at most one entry per level below the root): 5.5 KiB
+// of LDS per wave instead of 8, so a 72-register walk runs 7 waves per SIMD.
+constexpr int kWideStackShallow = 11;
 
 struct RayF {
     float ox, oy, oz, tmax;
