@@ -1630,7 +1630,9 @@ STK_API stk_status stk_batch_render(stk_ctx* c, const StkRenderParams* p, float*
     sv.bvh.wideTris = static_cast<const float4*>(c->fWideTris.ptr);
     sv.bvh.emitters = static_cast<const float4*>(c->fEmit.ptr);
     sv.bvh.wideRoot = STK_INVALID_INDEX; // per scene: SceneDesc.wideRoot
-    sv.bvh.wideDepth = (uint32_t)stk::dev::kWideStack;
+    // the deepest view's wide tree sizes the walks' stacks
+    sv.bvh.wideDepth = 0;
+    for (const BatchScene& b : c->batch) sv.bvh.wideDepth = std::max(sv.bvh.wideDepth, b.bvh.wideDepth);
     sv.bvh.packedOk = 1;                  // host-built trees fit the packed walk's stack
     sv.vb = static_cast<const StkVertex*>(c->fVb.ptr);
     sv.ib = static_cast<const uint32_t*>(c->fIb.ptr);

@@ -843,12 +843,11 @@ void launchExtendWalk(const SceneView& sc, PathState& ps, int depth, const Launc
                            sc, ps, depth, 0, binary);
 }
 
-// Whether every view's walks fit stacks of kWideStackShallow entries (the
-// view records of a batch do not carry their trees' depths: a batch says
-// kWideStack).
+// Whether every view's wide walks fit stacks of kWideStackShallow entries:
+// wideDepth is the deepest view's wide tree (0: none).
 inline bool shallowWalks(const SceneView& sc)
 {
-    return !sc.bvh.tlas && closestWalkStack(sc.bvh) == (uint32_t)kWideStackShallow;
+    return !sc.bvh.tlas && sc.bvh.wideDepth > 0 && sc.bvh.wideDepth <= (uint32_t)kWideStackShallow + 1;
 }
 
 template <int BLOCK>
