@@ -217,6 +217,18 @@ void images()
     check(stk_image_read_pfm(path.c_str(), back.data(), back.size(), &rw, &rh) == STK_OK, "pfm read");
     check(back == rgba, "pfm round trip");
     check(stk_image_read_pfm(path.c_str(), back.data(), 3, &rw, &rh) != STK_OK, "short image buffer refused");
+    const std::string exr = path.substr(0, path.size() - 4) + ".exr";
+    check(stk_image_write_exr(exr.c_str(), rgba.data(), w, h) == STK_OK, "exr write");
+    FILE* f = std::fopen(exr.c_str(), "rb");
+    check(f != nullptr, "exr exists");
+    if (f) {
+        std::fseek(f, 0, SEEK_END);
+        const long size = std::ftell(f);
+        std::fclose(f);
+        // header + one 8-byte offset per row + per row {y, byte count, 3 float rows}
+        check(size > (long)(h * (8 + 8 + 12 * w)), "exr size");
+    }
+    check(stk_image_write_exr(exr.c_str(), rgba.data(), 0, h) != STK_OK, "empty exr refused");
 }
 
 // The scene-building calls: a generated workload re-entered from its own
