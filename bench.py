@@ -268,8 +268,11 @@ def instanced_bench(args) -> dict:
             ctx.build_bvh_gpu(method)  # first call loads the build kernels
             ctx.sync()
         t0 = time.perf_counter()
-        if structure == "two_level":
+        if structure in ("two_level", "two_level_world"):
             ctx.build_two_level()
+            if structure == "two_level_world":
+                ctx.two_level_world_triangles(True)
+                ctx.sync()
         elif method:
             ctx.build_bvh_gpu(method)
             ctx.sync()
@@ -309,10 +312,11 @@ def instanced_bench(args) -> dict:
 
     want = set(args.structures.split(","))
     two = measure("two_level") if "two_level" in want else None
+    two_world = measure("two_level_world") if "two_level_world" in want else None
     flat = measure("flat") if "flat" in want else None
     flat_dev = measure("flat_device") if "flat_device" in want else None
     flat_ploc = measure("flat_device_ploc") if "flat_device_ploc" in want else None
-    head = next(r for r in (two, flat, flat_dev, flat_ploc) if r is not None)
+    head = next(r for r in (two, two_world, flat, flat_dev, flat_ploc) if r is not None)
     for b in (d_out, d_rays, d_hits):
         ctx.free(b)
     ctx.close()
@@ -322,7 +326,8 @@ def instanced_bench(args) -> dict:
             "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded PCG meshes and transforms)",
             "config": {"workload": "10,000 instances of 20 seeded meshes under a tree of instances, 3840x2160, "
                                    "4 spp, 6 bounces", "triangles": ntri, "build_s": head["build_s"]},
-            "two_level": two, "flat": flat, "flat_device_built": flat_dev, "flat_device_clustered": flat_ploc}
+            "two_level": two, "two_level_world_triangles": two_world, "flat": flat, "flat_device_built": flat_dev,
+            "flat_device_clustered": flat_ploc}
 
 
 def load_traffic(kernel: str):
@@ -544,7 +549,7 @@ def main() -> int:
     ap.add_argument("--workload", choices=("trace", "batch", "instanced"), default="trace",
                     help="trace: closest-hit on the 1M-triangle scene (+ its 64-spp path-traced frame); "
                          "batch: the 32-scene batched path tracer; instanced: the 19M-triangle instanced scene")
-    ap.add_argument("--structures", default="two_level,flat,flat_device,flat_device_ploc",
+    ap.add_argument("--structures", default="two_level,two_level_world,flat,flat_device,flat_device_ploc",
                     help="instanced workload: the structures to measure (the first one gives value)")
     ap.add_argument("--no-time-to-rmse", action="store_true",
                     help="skip the time-to-RMSE leg (cached against plain path tracing) of the default run")

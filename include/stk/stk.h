@@ -162,6 +162,16 @@ STK_API stk_status stk_build_bvh_gpu_method(stk_ctx* ctx, uint32_t method);
  * world triangle getTriangle forms.  The reference-order and binary trace
  * flags and stk_get_bvh need the flat BVH (STK_E_STATE). */
 STK_API stk_status stk_build_two_level(stk_ctx* ctx, const StkBvhOptions* opts);
+/* No counterpart in the reference (it keeps only flattened world triangles,
+ * src/render/ptrender.cpp:460-532).  enable != 0, after stk_build_two_level:
+ * writes every instance's world triangles - the ones the two-level walks form
+ * per triangle test from the object-space mesh and objectToWorld
+ * (getTriangle, shaders/raytracing.h:134-153) - into device memory once, 48 B
+ * per instanced triangle (18.8 M triangles: 905 MB), and the walks read them
+ * instead; hits are bit-identical either way.  enable == 0 frees them.  A new
+ * stk_build_two_level drops them.  STK_E_STATE without a two-level structure,
+ * STK_E_INVALID beyond 2^31 instanced triangles. */
+STK_API stk_status stk_two_level_world_triangles(stk_ctx* ctx, int enable);
 /* Uses a caller-built skip-link BVH (e.g. the reference builder's output). */
 STK_API stk_status stk_set_bvh(stk_ctx* ctx, const StkBvhNode* nodes, uint32_t numNodes);
 /* Copies the skip-link BVH of the context out (numNodes receives the count;

@@ -504,6 +504,12 @@ class Context:
         opts = BvhOptions(max_depth, bins, 1.0, 1.0, 0.0)
         self._check(_LIB.stk_build_two_level(self._h, C.byref(opts)), "stk_build_two_level")
 
+    def two_level_world_triangles(self, enable: bool = True):
+        """stk_two_level_world_triangles: keep (or free) every instance's world
+        triangles in device memory for the two-level walks."""
+        self._check(_LIB.stk_two_level_world_triangles(self._h, 1 if enable else 0),
+                    "stk_two_level_world_triangles")
+
     def set_bvh(self, nodes: np.ndarray):
         nodes = np.ascontiguousarray(nodes, dtype=BVH_NODE)
         self._check(_LIB.stk_set_bvh(self._h, _ptr(nodes), nodes.size), "stk_set_bvh")

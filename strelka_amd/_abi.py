@@ -231,6 +231,7 @@ PROTOTYPES = [
     ("stk_bvh_wide_depth", U32, [P]),
     ("stk_bvh_num_emitters", U32, [P]),
     ("stk_build_two_level", C.c_int, [P, C.POINTER(BvhOptions)]),
+    ("stk_two_level_world_triangles", C.c_int, [P, C.c_int]),
     ("stk_build_bvh_gpu", C.c_int, [P]),
     ("stk_build_bvh_gpu_method", C.c_int, [P, C.c_uint32]),
     ("stk_bvh_save_nodes", C.c_int, [P, U32, C.c_char_p]),

@@ -1284,6 +1284,8 @@ bool buildTwoLevel(const StkVertex* vb, uint32_t nv, const uint32_t* ib, uint32_
     // one binary tree and one wide tree per mesh, in object space
     out.blasNodes.resize(nmesh);
     out.meshRoot.resize(nmesh);
+    out.meshTriBase.assign(nmesh, 0u);
+    out.meshTriCount.assign(nmesh, 0u);
     std::vector<std::array<float, 6>> meshBox(nmesh);
     for (uint32_t m = 0; m < nmesh; ++m) {
         const uint32_t ntri = out.meshCount[m] / 3;
@@ -1330,6 +1332,8 @@ bool buildTwoLevel(const StkVertex* vb, uint32_t nv, const uint32_t* ib, uint32_
             w.triBase += triBase;
         }
         out.meshRoot[m] = nodeBase;
+        out.meshTriBase[m] = triBase;
+        out.meshTriCount[m] = (uint32_t)leaves.size();
         out.blasWide.insert(out.blasWide.end(), wide.begin(), wide.end());
         out.blasTris.insert(out.blasTris.end(), leaves.begin(), leaves.end());
         out.blasDepth = std::max(out.blasDepth, depth);

@@ -57,6 +57,7 @@ struct TwoLevel {
     std::vector<StkWideNode> blasWide;                // every mesh's wide tree, indices rebased
     std::vector<StkBlasTriangle> blasTris;            // leaves of blasWide, slot order
     std::vector<uint32_t> meshRoot;                   // wide root of each mesh in blasWide
+    std::vector<uint32_t> meshTriBase, meshTriCount;  // each mesh's leaves in blasTris
     uint32_t blasDepth = 0;                           // deepest mesh tree (wide levels)
     std::vector<StkBvhNode> tlasNodes;                // binary skip-link tree over instance boxes
     std::vector<StkWideNode> tlasWide;

@@ -119,6 +119,9 @@ struct stk_ctx {
     DevBuf vb, ib, inst, lights, mats, packed, tris, refNodes, wideNodes, wideTris, emitters, textures, texels;
     uint32_t numTextures = 0;
     DevBuf tlasWide, tlasLeaves, blasWide, blasTris, instKey, instW, tlEmit;
+    // the two-level structure's world triangles (stk_two_level_world_triangles)
+    DevBuf tlWorldTris, tlTriOff;
+    bool tlWorld = false;
     DevBuf rayScratch, hitScratch;
 
     // render state.  One pass stream (passStreams = 1): every pass runs on
@@ -195,7 +198,8 @@ struct stk_ctx {
     std::vector<DevBuf*> owned()
     {
         return {&vb, &ib, &inst, &lights, &mats, &textures, &texels, &packed, &tris, &refNodes, &wideNodes, &wideTris, &emitters,
-                &tlasWide, &tlasLeaves, &blasWide, &blasTris, &instKey, &instW, &tlEmit, &rayScratch, &hitScratch,
+                &tlasWide, &tlasLeaves, &blasWide, &blasTris, &instKey, &instW, &tlEmit, &tlWorldTris, &tlTriOff,
+                &rayScratch, &hitScratch,
                 &accum, &accum2, &history, &out, &samples, &descs, &staging, &fVb, &fIb, &fInst, &fLights, &fMats, &fPacked,
                 &fTris, &fRefNodes, &fWide, &fWideTris, &fEmit, &cGridH, &cGridP, &cGridM, &cGridV, &cGridG, &cMlpP,
                 &cMlpM, &cMlpV, &cMlpG, &cFwd, &cBwd, &cXT, &cDyT, &cLoss, &cGridE, &cGridEW, &cGridHE, &cMlpE,
@@ -288,8 +292,8 @@ stk::BvhView bvhView(stk_ctx* c)
     v.emitCount = c->numEmit;
     v.packedOk = c->bvh.depth <= (uint32_t)stk::dev::kStackDepth ? 1u : 0u;
     v.wideDepth = c->bvh.wideDepth;
-    v.tlas = v.tlasLeaves = v.blas = v.blasTris = v.instW = nullptr;
-    v.instKey = nullptr;
+    v.tlas = v.tlasLeaves = v.blas = v.blasTris = v.instW = v.worldTris = nullptr;
+    v.instKey = v.instTriOff = nullptr;
     if (c->twoLevel) {
         v.tlas = static_cast<const float4*>(c->tlasWide.ptr);
         v.tlasLeaves = static_cast<const float4*>(c->tlasLeaves.ptr);
@@ -300,6 +304,10 @@ stk::BvhView bvhView(stk_ctx* c)
         v.emitters = static_cast<const float4*>(c->tlEmit.ptr);
         v.emitCount = (uint32_t)c->tl.emitters.size();
         v.wideRoot = STK_INVALID_INDEX;
+        if (c->tlWorld) {
+            v.worldTris = static_cast<const float4*>(c->tlWorldTris.ptr);
+            v.instTriOff = static_cast<const uint32_t*>(c->tlTriOff.ptr);
+        }
     }
     return v;
 }
@@ -595,6 +603,7 @@ STK_API stk_status stk_build_two_level(stk_ctx* c, const StkBvhOptions* opts)
     // from here on the device arrays change: without a tree until all are written
     c->haveBvh = false;
     c->twoLevel = false;
+    c->tlWorld = false; // the world triangles were the previous structure's
     c->numWide = 0;
     stk_status st;
     if ((st = devUpload(c, c->tlasWide, t.tlasWide.data(), t.tlasWide.size() * sizeof(StkWideNode))) != STK_OK)
@@ -621,6 +630,62 @@ STK_API stk_status stk_build_two_level(stk_ctx* c, const StkBvhOptions* opts)
     c->twoLevel = true;
     c->haveBvh = true;
     invalidateGraph(c);
+    return STK_OK;
+}
+
+STK_API stk_status stk_two_level_world_triangles(stk_ctx* c, int enable)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    (void)hipSetDevice(c->device);
+    c->tlWorld = false;
+    invalidateGraph(c);
+    stk_status st;
+    if (!enable) {
+        if ((st = devRelease(c, c->tlWorldTris)) != STK_OK) return st;
+        return devRelease(c, c->tlTriOff);
+    }
+    if (!c->twoLevel) return fail(c, STK_E_STATE, "stk_two_level_world_triangles: no two-level structure");
+    const stk::TwoLevel& t = c->tl;
+    const uint32_t ninst = (uint32_t)t.instMesh.size();
+    std::vector<uint32_t> meshBase(ninst, 0u), count(ninst, 0u), off(ninst, 0u);
+    std::vector<uint64_t> worldBase(ninst, 0u);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < ninst; ++i) {
+        const uint32_t m = t.instMesh[i];
+        if (m == STK_INVALID_INDEX) continue;
+        meshBase[i] = t.meshTriBase[m];
+        count[i] = t.meshTriCount[m];
+        worldBase[i] = total;
+        off[i] = (uint32_t)(total - meshBase[i]); // modulo 2^32: the walk adds it to a blasTris index
+        total += count[i];
+    }
+    // the walk indexes the array with 32-bit triangle numbers
+    if (total >= (1ull << 31))
+        return fail(c, STK_E_INVALID, "stk_two_level_world_triangles: more than 2^31 instanced triangles");
+    DevBuf dMeshBase, dCount, dWorldBase;
+    auto release = [&]() {
+        (void)devRelease(c, dMeshBase);
+        (void)devRelease(c, dCount);
+        (void)devRelease(c, dWorldBase);
+    };
+    if ((st = devReserve(c, c->tlWorldTris, (size_t)total * sizeof(StkTriangle))) != STK_OK ||
+        (st = devUpload(c, c->tlTriOff, off.data(), off.size() * sizeof(uint32_t))) != STK_OK ||
+        (st = devUpload(c, dMeshBase, meshBase.data(), meshBase.size() * sizeof(uint32_t))) != STK_OK ||
+        (st = devUpload(c, dCount, count.data(), count.size() * sizeof(uint32_t))) != STK_OK ||
+        (st = devUpload(c, dWorldBase, worldBase.data(), worldBase.size() * sizeof(uint64_t))) != STK_OK) {
+        release();
+        return st;
+    }
+    hipError_t e = stk::launchWorldTriangles(static_cast<const StkInstance*>(c->inst.ptr),
+                                             static_cast<const float4*>(c->blasTris.ptr),
+                                             static_cast<const uint32_t*>(dMeshBase.ptr),
+                                             static_cast<const uint32_t*>(dCount.ptr),
+                                             static_cast<const uint64_t*>(dWorldBase.ptr), ninst,
+                                             static_cast<float4*>(c->tlWorldTris.ptr), c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    release();
+    if (e != hipSuccess) return hipFail(c, e, "stk_two_level_world_triangles");
+    c->tlWorld = true;
     return STK_OK;
 }
 

@@ -241,6 +241,28 @@ __global__ __launch_bounds__(kBlock) void k_pack_image(const float4* __restrict_
 inline uint32_t blocks(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
 inline uint32_t walkBlocks(size_t n) { return (uint32_t)((n + kWalkBlock - 1) / kWalkBlock); }
 
+// Every instance's world triangles (world_triangle, the walk's own form):
+// block i writes instance i's count[i] triangles, mesh leaves from
+// meshBase[i] on, to worldBase[i] on.
+__global__ __launch_bounds__(256) void k_world_triangles(const StkInstance* __restrict__ inst,
+                                                         const float4* __restrict__ blasTris,
+                                                         const uint32_t* __restrict__ meshBase,
+                                                         const uint32_t* __restrict__ count,
+                                                         const uint64_t* __restrict__ worldBase,
+                                                         float4* __restrict__ worldTris)
+{
+    const uint32_t i = blockIdx.x, n = count[i];
+    const float* M = inst[i].objectToWorld;
+    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) {
+        float4 a, b, c;
+        world_triangle(M, blasTris + 3 * (size_t)(meshBase[i] + t), a, b, c);
+        float4* o = worldTris + 3 * (size_t)(worldBase[i] + t);
+        o[0] = a;
+        o[1] = b;
+        o[2] = c;
+    }
+}
+
 } // namespace
 
 hipError_t launchPackImage(const float4* src, uint32_t npix, uint32_t format, void* dst, hipStream_t s)
@@ -271,6 +293,16 @@ hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n
             hipLaunchKernelGGL(k_trace_closest<kWideStack>, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh, rays, n,
                                hits);
     }
+    return hipGetLastError();
+}
+
+hipError_t launchWorldTriangles(const StkInstance* inst, const float4* blasTris, const uint32_t* meshBase,
+                                const uint32_t* count, const uint64_t* worldBase, uint32_t numInstances,
+                                float4* worldTris, hipStream_t s)
+{
+    if (numInstances == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_world_triangles, dim3(numInstances), dim3(256), 0, s, inst, blasTris, meshBase, count,
+                       worldBase, worldTris);
     return hipGetLastError();
 }
 

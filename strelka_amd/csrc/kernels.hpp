@@ -153,6 +153,12 @@ struct LaunchConfig {
 // for trees of at most 12 wide levels, else 16); 0 when the tree has no wide
 // walk (two-level structures, or no wide tree).
 uint32_t closestWalkStack(const BvhView& bvh);
+// The two-level structure's world triangles (BvhView::worldTris): instance i's
+// count[i] mesh leaves from blasTris[meshBase[i]] on, through its
+// objectToWorld, to worldTris[worldBase[i]] on.
+hipError_t launchWorldTriangles(const StkInstance* inst, const float4* blasTris, const uint32_t* meshBase,
+                                const uint32_t* count, const uint64_t* worldBase, uint32_t numInstances,
+                                float4* worldTris, hipStream_t s);
 
 // flags: STK_TRACE_REFERENCE_ORDER (skip-link walk), STK_TRACE_BVH2 (packed
 // binary walk instead of the wide one; closest hit only).

@@ -49,6 +49,11 @@ struct BvhView {
     const uint32_t* instKey;      // TLAS preorder leaf index of each instance
     const float4* instW;          // 4 x float4 per instance: the rows of the inverse of
                                   // objectToWorld (linear part | translation), then {pad, 0, 0, 0}
+    // optional: every instance's world triangles as world_triangle forms them
+    // (3 x float4 each), mesh-tree leaf t of instance i at t + instTriOff[i]
+    // (modulo 2^32; t indexes blasTris); nullptr: formed per test
+    const float4* worldTris;
+    const uint32_t* instTriOff;
 };
 
 constexpr int kWideStack = 16; // node-group entries; the builder's wide depth must not exceed it
@@ -759,7 +764,7 @@ __device__ __forceinline__ bool two_level_walk(const BvhView& bvh, const RayF& r
     const uint32_t octW = ray_octant(r);
     RayF c = r;
     float pad[3] = {0.0f, 0.0f, 0.0f};
-    uint32_t oct = octW, inst = 0, ik = 0;
+    uint32_t oct = octW, inst = 0, ik = 0, triOff = 0;
     bool inMesh = false;
     int sp = 0;
     uint32_t node = 0;
@@ -775,6 +780,7 @@ __device__ __forceinline__ bool two_level_walk(const BvhView& bvh, const RayF& r
         if (MODE == kTlAny && found && nk > hIk) return false;
         inst = ni;
         ik = nk;
+        if (bvh.worldTris) triOff = bvh.instTriOff[ni];
         InstRay o;
         object_ray(bvh.instW + 4 * (size_t)ni, r, exit_bound(r, L0, L1), o);
         c = o.r;
@@ -796,7 +802,15 @@ __device__ __forceinline__ bool two_level_walk(const BvhView& bvh, const RayF& r
             for (uint32_t lm = v.leafHits; lm; lm &= lm - 1u) {
                 const uint32_t ls = (uint32_t)(__ffs(lm) - 1);
                 float4 a, b, cc;
-                world_triangle(M, bvh.blasTris + 3 * (size_t)(v.triBase + rank_below(v.leafMask, ls)), a, b, cc);
+                const uint32_t gt = v.triBase + rank_below(v.leafMask, ls);
+                if (bvh.worldTris) {
+                    const float4* q = bvh.worldTris + 3 * (size_t)(gt + triOff);
+                    a = q[0];
+                    b = q[1];
+                    cc = q[2];
+                } else {
+                    world_triangle(M, bvh.blasTris + 3 * (size_t)gt, a, b, cc);
+                }
                 const uint32_t leaf = float_to_bits(b.w);
                 if (MODE == kTlVisible && ik == lightIk && leaf >= lightLeaf) continue;
                 if (MODE == kTlAny && found && !key_less(ik, leaf, hIk, h.leaf)) continue;

@@ -88,6 +88,46 @@ def test_per_sample_radiance(ctx, small):
     assert cpu.mean() > 1e-3
 
 
+def test_world_triangles_give_the_same_hits_and_samples(ctx, small):
+    """stk_two_level_world_triangles: the walks read every instance's world
+    triangles from device memory instead of forming them per test; closest
+    and any hits, and per-sample radiance, are bit-identical to the walks
+    that form them (and the closest hits to the oracle's reference walk);
+    turning it off, or a new build, returns to forming them."""
+    print("case: 60-instance scene, world triangles kept vs formed per test", flush=True)
+    s, tl, o = small
+    ctx.set_scene(s)
+    ctx.build_bvh()
+    with pytest.raises(sa.StkError) as e:
+        ctx.two_level_world_triangles(True)  # a flat tree has none
+    assert e.value.status == 4
+    ctx.build_two_level()
+    lo = np.min(tl.top_leaves["lo"], axis=0)
+    hi = np.max(tl.top_leaves["hi"], axis=0)
+    rays = scene_rays(s, lo, hi, 100000, 31)
+    short = scene_rays(s, lo, hi, 50000, 32, tmax=30.0)
+    p = sa.render_params(s.camera, spp=2, max_depth=6, frame=3, sampling=1, sky=s.sky, keep_samples=True,
+                         accumulate=False)
+    w, h = p.pt.dimension[0], p.pt.dimension[1]
+
+    def run():
+        ctx.render(p)
+        return ctx.trace_closest(rays), ctx.trace_any(short), ctx.samples(w * h * 2 * 3).copy()
+
+    formed = run()
+    ctx.two_level_world_triangles(True)
+    kept = run()
+    assert same(kept[0], formed[0]) and same(kept[0], o.trace(rays, MODE_REF)[0])
+    assert same(kept[1], formed[1])
+    assert np.array_equal(kept[2].view(np.uint32), formed[2].view(np.uint32))
+    ctx.two_level_world_triangles(False)
+    assert same(run()[0], formed[0])
+    ctx.two_level_world_triangles(True)
+    ctx.build_two_level()  # drops them
+    again = run()
+    assert same(again[0], formed[0]) and np.array_equal(again[2].view(np.uint32), formed[2].view(np.uint32))
+
+
 def test_full_instanced_scene_matches_flat_tree(ctx):
     print("case: 10,000 instances (18.8M triangles), 3840x2160 camera rays, two-level vs flat", flush=True)
     s = sa.SceneData.generate(5)
