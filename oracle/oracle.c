@@ -687,7 +687,10 @@ uint32_t orc_expand_two_level(const StkBvhNode* top, uint32_t ntop, const StkBvh
     return c.overflow ? 0 : n;
 }
 
-/* The kernel's object-space ray and box widening (trace_dev.hpp object_ray). */
+/* The kernel's object-space ray and box widening (trace_dev.hpp object_ray).
+ * The inverse direction is the IEEE quotient here; the kernel takes the
+ * hardware reciprocal (within 1 ulp), which the widening covers: it can only
+ * change which mesh-tree boxes a walk visits, never the hits. */
 static void object_ray(const float* W, const StkRay* r, float T, StkRay* o, float inv[3], float pad[3])
 {
     const float* w0 = W;
