@@ -407,11 +407,12 @@ def cache_study(args) -> dict:
     d_out = ctx.alloc(npix * 16)
     out = np.zeros((H, W, 4), np.float32)
 
-    def params(frame, mode, qd=2, paths=default_paths, min_rough=0.0, steps=1, suffix=0):
+    def params(frame, mode, qd=2, paths=default_paths, min_rough=0.0, steps=1, suffix=0, direct=False):
         return sa.render_params(cam, spp=args.study_spp, max_depth=args.study_depth, frame=frame, sampling=1,
                                 sky=scene.sky, tonemap=1,
                                 cache_mode=mode, cache_query_depth=qd, cache_train_paths=paths,
-                                cache_min_roughness=min_rough, cache_train_steps=steps, cache_train_suffix=suffix)
+                                cache_min_roughness=min_rough, cache_train_steps=steps, cache_train_suffix=suffix,
+                                cache_query_direct=direct)
 
     ctx.reset_accumulation()
     t0 = time.perf_counter()
@@ -432,7 +433,8 @@ def cache_study(args) -> dict:
     # up to 50%)
     checkpoints = sorted({1, 2, 4, 8, 12, 16, 24} | set(range(32, args.study_frames + 1, 8)) | {args.study_frames})
 
-    def run(mode, qd=2, warmup=0, paths=default_paths, min_rough=0.0, steps=1, label="", suffix=0, every=1):
+    def run(mode, qd=2, warmup=0, paths=default_paths, min_rough=0.0, steps=1, label="", suffix=0, every=1,
+            direct=False):
         # frames are issued back to back and timed between checkpoints (the
         # image read-back at a checkpoint is not timed)
         t_ms = 0.0
@@ -440,7 +442,7 @@ def cache_study(args) -> dict:
             ctx.sync()
             ctx.record(0)
             for f in range(warmup):
-                ctx.render_device(params(500_000 + f, 4, qd, paths, min_rough, steps), d_out)
+                ctx.render_device(params(500_000 + f, 4, qd, paths, min_rough, steps, 0, direct), d_out)
             ctx.record(1)
             t_ms += ctx.elapsed_ms(0, 1)
         ctx.reset_accumulation()
@@ -450,7 +452,7 @@ def cache_study(args) -> dict:
         for f in range(args.study_frames):
             # every: a query-and-train frame every `every` frames, query-only frames between
             m = 2 if (mode == 1 and every > 1 and f % every) else mode
-            ctx.render_device(params(f, m, qd, paths, min_rough, steps, suffix), d_out)
+            ctx.render_device(params(f, m, qd, paths, min_rough, steps, suffix, direct), d_out)
             if f + 1 in checkpoints:
                 ctx.record(1)
                 t_ms += ctx.elapsed_ms(0, 1)
@@ -487,7 +489,8 @@ def cache_study(args) -> dict:
         curve = run(2 if v.get("freeze") else 1, v.get("query_depth", 2), v.get("warmup", 0),
                     v.get("train_paths", default_paths),
                     v.get("min_rough", 0.0), v.get("train_steps", 1), label=f"cached {v}",
-                    suffix=v.get("train_suffix", 0), every=v.get("train_every", 1))
+                    suffix=v.get("train_suffix", 0), every=v.get("train_every", 1),
+                    direct=bool(v.get("query_direct", 0)))
         tc = time_to(curve, args.study_target)
         results.append({"variant": v, "time_to_target_s": tc, "speedup": (tp / tc) if (tp and tc) else None,
                         "curve": curve})
@@ -586,7 +589,8 @@ def main() -> int:
                     help="JSON list of cached-run settings: query_depth, min_rough, train_steps, warmup, freeze (no "
                          "training after the warm-up), lr, lr_half_life (Adam steps), min_lr, ema, train_paths, "
                          "train_suffix (training paths end with a query that many vertices past the query depth), "
-                         "train_every (query-only frames between training frames), log2_table, max_res")
+                         "train_every (query-only frames between training frames), query_direct (queries "
+                         "cover their vertex's direct light), log2_table, max_res")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)

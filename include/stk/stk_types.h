@@ -277,8 +277,9 @@ typedef struct StkRenderParams {
     /* Paths other than the training paths query the cache at the first
      * surface vertex of depth >= cacheQueryDepth (0 = the camera ray's hit)
      * whose roughness is >= cacheMinRoughness (Lambert counts as 1): the
-     * vertex's direct light is sampled as usual, and the radiance scattered
-     * from the BSDF-sampled direction on comes from the cache. */
+     * vertex's direct light is sampled as usual (unless cacheQueryDirect),
+     * and the radiance scattered from the BSDF-sampled direction on comes
+     * from the cache. */
     uint32_t cacheQueryDepth;
     uint32_t cacheTrainPaths; /* training paths per frame (extended to maxDepth, or see cacheTrainSuffix) */
     float cacheMinRoughness;  /* in [0, 1]; 0 queries at cacheQueryDepth */
@@ -291,6 +292,13 @@ typedef struct StkRenderParams {
      * vertex itself is not a training record.  Shorter training paths, a
      * smoother target. */
     uint32_t cacheTrainSuffix;
+    /* 0: a query vertex samples its direct light and the cache answers for
+     * the rest.  1: the cache answers for all the radiance the vertex
+     * reflects, its direct light included (no shadow ray there; the training
+     * targets then include each vertex's own light sample), as neural
+     * radiance caching queries its cache (Mueller et al. 2021, section 3). */
+    uint32_t cacheQueryDirect;
+    uint32_t pad[1];
 } StkRenderParams;
 
 #ifdef __cplusplus
@@ -331,6 +339,6 @@ STK_STATIC_ASSERT(offsetof(StkWideNode, qlo) == 32, "WideNode.qlo offset");
 STK_STATIC_ASSERT(sizeof(StkCacheRecord) == 64, "CacheRecord must be 64 B");
 STK_STATIC_ASSERT(sizeof(StkBlasTriangle) == 48, "BlasTriangle must be 48 B");
 STK_STATIC_ASSERT(sizeof(StkTlasLeaf) == 48, "TlasLeaf must be 48 B");
-STK_STATIC_ASSERT(sizeof(StkRenderParams) == 336 + 64, "RenderParams must be 400 B");
+STK_STATIC_ASSERT(sizeof(StkRenderParams) == 336 + 72, "RenderParams must be 408 B");
 
 #endif /* STK_TYPES_H */

@@ -395,7 +395,8 @@ def render_params(camera: PathTracerParam, spp: int = 1, max_depth: int = 8, fra
                   keep_samples: bool = False, rr_start: int = 3, enable_tonemap: bool = True,
                   cache_mode: int = 0, cache_query_depth: int = 2, cache_train_paths: int = 65536,
                   cache_min_roughness: float = 0.0, cache_train_steps: int = 1,
-                  ref_compat: bool = True, cache_train_suffix: int = 0) -> RenderParams:
+                  ref_compat: bool = True, cache_train_suffix: int = 0,
+                  cache_query_direct: bool = False) -> RenderParams:
     """StkRenderParams for a render call.  ref_compat: reproduce the
     reference estimator exactly (True) or run the corrected one (False: the
     light sample's solid-angle pdf, only non-emitters occlude, BSDF-sampled
@@ -418,6 +419,7 @@ def render_params(camera: PathTracerParam, spp: int = 1, max_depth: int = 8, fra
     p.cacheMinRoughness = cache_min_roughness
     p.cacheTrainSteps = cache_train_steps
     p.cacheTrainSuffix = cache_train_suffix
+    p.cacheQueryDirect = 1 if cache_query_direct else 0
     for i in range(3):
         p.skyColor[i] = float(sky[i])
     p.skyColor[3] = 0.0

@@ -63,7 +63,8 @@ class RenderParams(C.Structure):
                 ("tonemapperType", C.c_uint32), ("enableAccumulation", C.c_uint32), ("skyColor", C.c_float * 4),
                 ("enableTonemap", C.c_uint32), ("keepSamples", C.c_uint32), ("cacheMode", C.c_uint32),
                 ("cacheQueryDepth", C.c_uint32), ("cacheTrainPaths", C.c_uint32),
-                ("cacheMinRoughness", C.c_float), ("cacheTrainSteps", C.c_uint32), ("cacheTrainSuffix", C.c_uint32)]
+                ("cacheMinRoughness", C.c_float), ("cacheTrainSteps", C.c_uint32), ("cacheTrainSuffix", C.c_uint32),
+                ("cacheQueryDirect", C.c_uint32), ("pad", C.c_uint32 * 1)]
 
 
 class CacheConfig(C.Structure):
@@ -150,7 +151,7 @@ class LightDesc(C.Structure):
 
 assert C.sizeof(LightDesc) == 96
 assert C.sizeof(PathTracerParam) == 336
-assert C.sizeof(RenderParams) == 336 + 64
+assert C.sizeof(RenderParams) == 336 + 72
 assert C.sizeof(CacheConfig) == 48
 assert C.sizeof(BvhHeader) == 32
 

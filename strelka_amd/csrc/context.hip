@@ -1080,6 +1080,8 @@ static stk_status renderForestRun(stk_ctx* c, const stk::SceneView& base, std::v
         pc.trainSteps = p->cacheTrainSteps ? p->cacheTrainSteps : 1u;
         if (p->cacheTrainSuffix > 64) return fail(c, STK_E_INVALID, "render: cacheTrainSuffix must be at most 64");
         pc.trainSuffix = p->cacheMode == 1 ? p->cacheTrainSuffix : 0u;
+        if (p->cacheQueryDirect > 1) return fail(c, STK_E_INVALID, "render: cacheQueryDirect must be 0 or 1");
+        pc.queryDirect = p->cacheQueryDirect;
         const uint32_t paths = p->cacheTrainPaths ? p->cacheTrainPaths : 65536u;
         pc.trainStride = std::max<uint32_t>(1u, (uint32_t)(slots / paths));
         pc.maxTrainPaths = (uint32_t)((slots + pc.trainStride - 1) / pc.trainStride);

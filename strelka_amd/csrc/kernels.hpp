@@ -115,6 +115,7 @@ struct PtCache {
     float queryMinRough = 0.0f;
     uint32_t trainSteps = 1; // optimiser steps per frame over the frame's records
     uint32_t trainSuffix = 0; // > 0: training paths end with a query at depth >= queryDepth + trainSuffix
+    uint32_t queryDirect = 0; // 1: a query covers its vertex's direct light (no shadow ray there)
     uint32_t trainStride = 1;
     uint32_t trainCap = 0;
     uint32_t maxTrainPaths = 0;

@@ -484,7 +484,8 @@ __device__ __forceinline__ void shade_one(const SceneView& sc, const PtLaunch& l
                     lightPdf *= selPdf;
                 }
                 bool alive = !(isnan(radiance.x) || isnan(radiance.y) || isnan(radiance.z) || isnan(lightPdf));
-                if (alive && (dot(toLight, Ng) > 0.0f) && lightPdf != 0.0f) {
+                // a query that covers the vertex's direct light sends no shadow ray
+                if (alive && (dot(toLight, Ng) > 0.0f) && lightPdf != 0.0f && !(cacheQuery && pc.queryDirect)) {
                     const F3 rop = mk3(radiance.x / lightPdf, radiance.y / lightPdf, radiance.z / lightPdf);
                     F3 bd, bg;
                     float bpdf;
@@ -679,7 +680,8 @@ __global__ __launch_bounds__(kBlock) void k_pt_finalize(PtLaunch lp, PathState p
 }
 
 // Training targets: the radiance a training path gathered after leaving vertex
-// d (everything but d's own direct-light sample), divided by the throughput at
+// d (everything but d's own direct-light sample, which counts too when
+// queries cover their vertex's direct light), divided by the throughput at
 // d and by d's albedo - a one-sample estimate, under the plain path tracer's
 // estimator, of what a query at d stands in for.
 __global__ __launch_bounds__(kBlock) void k_cache_targets(PtLaunch lp, PathState ps)
@@ -696,8 +698,11 @@ __global__ __launch_bounds__(kBlock) void k_cache_targets(PtLaunch lp, PathState
         const uint32_t slot = tidx * pc.trainStride + pass.trainOffset;
         const float4* v = pc.trainVerts + ((size_t)tidx * maxDepth + d) * kTrainWords;
         const float4 thr = v[4], fin = ps.col[slot];
-        // a path that ended at d gathered nothing after it
-        const float4 before = v[5].w != 0.0f ? v[5] : fin;
+        // queries that cover their vertex's direct light learn it too: the
+        // target starts before d's light sample (v[3], after d's emission);
+        // otherwise after it (v[5]; a path that ended at d gathered nothing
+        // after it)
+        const float4 before = pc.queryDirect ? v[3] : (v[5].w != 0.0f ? v[5] : fin);
         const float ax = v[1].w, ay = v[2].w, az = v[3].w; // the vertex's albedo
         if (thr.x > 1e-8f && thr.y > 1e-8f && thr.z > 1e-8f) {
             // one-sample targets are clamped to [0, kTargetMax]: Russian-roulette

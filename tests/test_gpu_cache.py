@@ -339,3 +339,52 @@ def test_self_training_suffix_ends_training_paths_with_a_query():
         assert counts[suffix] < counts[0]
     finally:
         ctx.close()
+
+
+def test_queries_covering_direct_light_send_no_shadow_ray():
+    """cacheQueryDirect: a query answers for all the radiance its vertex
+    reflects, direct light included, so the vertex sends no shadow ray.  With
+    a fresh cache (answers 0) in the closed Lambert room (black sky; a light
+    hit after a diffuse bounce adds nothing in the reference estimator), a
+    frame's paths that do not train equal the plain render with max_depth =
+    query depth, the training paths (suffix 2) the plain render with max_depth
+    = query depth + 2, bit for bit; the training targets then include each
+    vertex's light sample, so they exceed those of the default mode."""
+    print("case: config 7, query depth 1, suffix 2, fresh cache, queries covering direct light", flush=True)
+    s = sa.SceneData.generate(7)
+    assert np.all(s.mats["type"] == 0)
+    w, h = s.camera.dimension[0], s.camera.dimension[1]
+    n = w * h
+    ctx = sa.Context(0)
+    try:
+        ctx.set_scene(s)
+        ctx.build_bvh()
+        qd, suffix, paths = 1, 2, 2048
+
+        def plain(depth):
+            p = sa.render_params(s.camera, spp=1, max_depth=depth, frame=9, sampling=1, sky=(0.0, 0.0, 0.0),
+                                 keep_samples=True, accumulate=False)
+            ctx.render(p)
+            return ctx.samples(n * 3).reshape(n, 3).copy()
+
+        short, long = plain(qd), plain(qd + suffix)
+        targets = {}
+        for direct in (True, False):
+            ctx.cache_create(log2_table=14, seed=11)
+            p = sa.render_params(s.camera, spp=1, max_depth=8, frame=9, sampling=1, sky=(0.0, 0.0, 0.0),
+                                 keep_samples=True, accumulate=False, cache_mode=1, cache_query_depth=qd,
+                                 cache_train_paths=paths, cache_train_suffix=suffix, cache_query_direct=direct)
+            ctx.render(p)
+            got = ctx.samples(n * 3).reshape(n, 3).copy()
+            targets[direct] = float(ctx.cache_last_records("train")["target"].mean())
+            if direct:
+                eq_short = np.all(got.view(np.uint32) == short.view(np.uint32), axis=1)
+                eq_long = np.all(got.view(np.uint32) == long.view(np.uint32), axis=1)
+                assert np.all(eq_short | eq_long), f"{np.sum(~(eq_short | eq_long))} samples are neither"
+                trained = int(np.sum(eq_long & ~eq_short))
+                print(f"training samples told apart: {trained} of about {paths}")
+                assert 0.5 * paths <= trained <= 1.01 * paths
+        print(f"mean training target: direct light included {targets[True]:.4f}, excluded {targets[False]:.4f}")
+        assert targets[True] > targets[False]
+    finally:
+        ctx.close()
