@@ -507,13 +507,13 @@ def cache_study(args) -> dict:
 
 # The cached estimator the default run times against plain path tracing: the
 # cache answers at the second path vertex (query depth 1) after 8 training-only
-# frames, one optimiser step per frame over 8,192 training paths that end one
+# frames, one optimiser step per frame over 4,096 training paths that end one
 # vertex past the query depth with a query of their own (self-training),
 # learning rate 1e-2 halving every 128 steps down to 2e-4, a hash grid of 2^16
 # entries per level up to 256 cells across (a cache that stands in only after
 # the second vertex needs no finer one, and a smaller table makes inference
 # and the optimiser step cheaper).
-TIME_TO_RMSE_VARIANT = {"query_depth": 1, "warmup": 8, "train_steps": 1, "train_paths": 8192, "train_suffix": 1,
+TIME_TO_RMSE_VARIANT = {"query_depth": 1, "warmup": 8, "train_steps": 1, "train_paths": 4096, "train_suffix": 1,
                         "lr_half_life": 128, "min_lr": 0.0002, "log2_table": 16, "max_res": 256}
 
 
