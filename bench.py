@@ -559,6 +559,9 @@ def main() -> int:
                     help="instanced workload: the structures to measure (the first one gives value)")
     ap.add_argument("--no-time-to-rmse", action="store_true",
                     help="skip the time-to-RMSE leg (cached against plain path tracing) of the default run")
+    ap.add_argument("--ray-order", choices=("tiles", "linear"), default="tiles",
+                    help="headline closest-hit launches: the image entry point (one wave per 8x8 tile of "
+                         "the row-major frame) or the flat entry point (64 consecutive rays per wave)")
     ap.add_argument("--trace-only", action="store_true",
                     help="only the timed closest-hit launches (for counter profiles of that kernel)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -648,6 +651,20 @@ def main() -> int:
     d_hits = ctx.alloc(n * 20)
     ctx.upload(d_rays, rays)
 
+    # the rays are the frame's, row-major: the image entry point walks them
+    # one 8x8 tile per wave; the flat one 64 consecutive rays (one row
+    # segment) per wave.  Both write every hit at its ray's index, identical.
+    width, height = scene.camera.dimension[0], scene.camera.dimension[1]
+    assert width * height == n
+
+    def trace_tiles():
+        ctx.trace_closest_image_device(d_rays, width, height, d_hits)
+
+    def trace_linear():
+        ctx.trace_closest_device(d_rays, n, d_hits)
+
+    trace = trace_tiles if args.ray_order == "tiles" else trace_linear
+
     # settle: the GPU reaches its sustained clocks only after some tens of ms
     # of load, so the launches run untimed for settle_ms before the warm-up
     # steps (the first launches after a few warm-ups ran 2-5% slower)
@@ -655,16 +672,16 @@ def main() -> int:
     t_settle = time.perf_counter()
     while (time.perf_counter() - t_settle) * 1e3 < args.settle_ms:
         for _ in range(8):
-            ctx.trace_closest_device(d_rays, n, d_hits)
+            trace()
         settle_launches += 8
         ctx.sync()
     for _ in range(args.warmup):
-        ctx.trace_closest_device(d_rays, n, d_hits)
+        trace()
     ctx.sync()
     ctx.record(0)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ctx.trace_closest_device(d_rays, n, d_hits)
+        trace()
     ctx.record(1)
     ctx.sync()
     wall = time.perf_counter() - t0
@@ -672,9 +689,19 @@ def main() -> int:
     mrays = n * args.steps / wall / 1e6
     if args.trace_only:
         print(json.dumps({"metric": METRIC, "value": round(mrays, 2), "unit": "Mrays/s", "kernel_ms": round(kernel_ms, 4),
-                          "steps": args.steps, "rays": n, "walk_stack": ctx.bvh_info().walkStack}))
+                          "steps": args.steps, "rays": n, "ray_order": args.ray_order,
+                          "walk_stack": ctx.bvh_info().walkStack}))
         ctx.close()
         return 0
+
+    # the other entry point on the same rays (same hits)
+    other = trace_linear if args.ray_order == "tiles" else trace_tiles
+    other()
+    ctx.record(2)
+    for _ in range(max(1, args.steps // 2)):
+        other()
+    ctx.record(3)
+    other_kernel_ms = ctx.elapsed_ms(2, 3) / max(1, args.steps // 2)
 
     # reference-order (unpruned skip-link) GPU kernel on the same rays
     ctx.record(2)
@@ -745,7 +772,7 @@ def main() -> int:
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "limiter": limiter(load_counters("trace_closest_config2"), traffic, kernel_ms),
-                "kernel": "k_trace_closest (compressed 8-wide walk)", "kernel_ms": round(kernel_ms, 4),
+                "kernel": "k_trace_closest (compressed 8-wide walk, " + args.ray_order + " order)", "kernel_ms": round(kernel_ms, 4),
                 "bytes_per_ray": round(bytes_per_ray, 1),
                 "wide_node_visits_per_ray": round(wd["inner_per_ray"], 2),
                 "triangle_tests_per_ray": round(wd["leaf_per_ray"], 2),
@@ -784,10 +811,15 @@ def main() -> int:
         "config": {"workload": "1920x1080 jittered primary rays, closest hit, 1.16M-triangle heightfield + "
                                "2,000-sphere scene",
                    "bvh": f"host binned SAH, spatial splits alpha {SPATIAL_SPLIT_ALPHA}",
+                   "ray_order": "8x8 tiles of the row-major frame, one per wave (stk_trace_closest_image_device)"
+                                if args.ray_order == "tiles" else
+                                "64 consecutive rays per wave (stk_trace_closest_device)",
                    "rays": n, "triangles": ntri, "bvh_nodes": int(nodes.size), "bvh_build_s": round(build_s, 3)},
         "roofline": roof,
         "cpu_baseline": cpu,
         "extra": {"settle": {"ms": args.settle_ms, "launches": settle_launches},
+                  ("linear_order_gpu_mrays_per_s" if args.ray_order == "tiles" else "tile_order_gpu_mrays_per_s"):
+                      round(n / (other_kernel_ms * 1e-3) / 1e6, 2),
                   "reference_order_gpu_mrays_per_s": round(n / (ref_kernel_ms * 1e-3) / 1e6, 2),
                   "binary_bvh_gpu_mrays_per_s": round(n / (bin_kernel_ms * 1e-3) / 1e6, 2),
                   "device_built_tree": device_build,

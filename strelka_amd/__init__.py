@@ -430,6 +430,7 @@ class Context:
     """One device context (stk_ctx): owns device buffers and one HIP stream."""
 
     def __init__(self, device: int = 0):
+        self._h = None  # set before stk_create so that __del__ of a failed create is a no-op
         h = C.c_void_p()
         st = _LIB.stk_create(device, C.byref(h))
         if st:
@@ -580,6 +581,12 @@ class Context:
         flags = (1 if reference_order else 0) | (2 if binary else 0)
         self._check(_LIB.stk_trace_closest_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits), flags),
                     "stk_trace_closest_device")
+
+    def trace_closest_image_device(self, d_rays: int, width: int, height: int, d_hits: int):
+        """stk_trace_closest_image_device: an image's rays (row-major) traced
+        in 8x8 tiles; the hits of trace_closest_device."""
+        self._check(_LIB.stk_trace_closest_image_device(self._h, C.c_void_p(d_rays), width, height,
+                                                        C.c_void_p(d_hits), 0), "stk_trace_closest_image_device")
 
     def trace_any_device(self, d_rays: int, n: int, d_hits: int, reference_order: bool = False):
         self._check(_LIB.stk_trace_any_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits),

@@ -180,6 +180,7 @@ PROTOTYPES = [
     ("stk_trace_closest_noalloc", C.c_int, [P, P, U32, P, U32]),
     ("stk_trace_any", C.c_int, [P, P, U32, P, U32]),
     ("stk_trace_closest_device", C.c_int, [P, P, U32, P, U32]),
+    ("stk_trace_closest_image_device", C.c_int, [P, P, U32, U32, P, U32]),
     ("stk_trace_any_device", C.c_int, [P, P, U32, P, U32]),
     ("stk_generate_camera_rays_device", C.c_int, [P, C.POINTER(PathTracerParam), U32, P]),
     ("stk_render", C.c_int, [P, C.POINTER(RenderParams), P, C.c_int]),

@@ -843,6 +843,20 @@ STK_API stk_status stk_trace_closest_device(stk_ctx* c, const StkRay* rays, uint
     return STK_OK;
 }
 
+STK_API stk_status stk_trace_closest_image_device(stk_ctx* c, const StkRay* rays, uint32_t width, uint32_t height,
+                                                  StkHit* hits, uint32_t flags)
+{
+    if (!validCtx(c)) return STK_E_INVALID;
+    if (!c->haveBvh) return fail(c, STK_E_STATE, "stk_trace_closest_image: no BVH");
+    const uint64_t n = (uint64_t)width * height;
+    if (n > 0xFFFFFFFFull) return fail(c, STK_E_INVALID, "stk_trace_closest_image: image too large");
+    if (n && (!rays || !hits)) return fail(c, STK_E_INVALID, "stk_trace_closest_image: null buffer");
+    if (flatOnlyFlags(c, flags) != STK_OK) return STK_E_STATE;
+    (void)hipSetDevice(c->device);
+    STK_HIP(c, stk::launchTraceClosest(bvhView(c), rays, (uint32_t)n, hits, flags, c->stream, width));
+    return STK_OK;
+}
+
 STK_API stk_status stk_trace_any_device(stk_ctx* c, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags)
 {
     if (!validCtx(c)) return STK_E_INVALID;

@@ -49,12 +49,24 @@ static_assert(2 * kWideStack <= kStackDepth, "the wide walk's two stacks share t
 // a node with inner children pushes, so a walk over a tree of D levels holds
 // at most D - 1 entries: a tree of at most S + 1 levels fits.  Fewer entries
 // mean less LDS per wave and more waves per CU.
+// width > 0: the rays are an image's, width per row in row-major order, and
+// each wave takes an 8x8 tile of it (camera rays of a tile stay close in the
+// scene; a wave of one row's 64 would span 64 pixels); hits go to the rays'
+// own indices either way.
 template <int S>
 __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(S <= 11 ? 7 : 5, 8))) void k_trace_closest(
-    BvhView bvh, const StkRay* __restrict__ rays, uint32_t n, StkHit* __restrict__ hits)
+    BvhView bvh, const StkRay* __restrict__ rays, uint32_t n, StkHit* __restrict__ hits, uint32_t width)
 {
+    static_assert(kWalkBlock == 64, "one 8x8 tile per wave");
     __shared__ uint32_t stack[2 * S * kWalkBlock];
-    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kWalkBlock + threadIdx.x;
+    uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kWalkBlock + threadIdx.x;
+    if (width) {
+        const uint32_t tilesX = (width + 7) / 8, tile = i / 64, within = i % 64;
+        const uint32_t px = (tile % tilesX) * 8 + within % 8, py = (tile / tilesX) * 8 + within / 8;
+        const uint64_t j = (uint64_t)py * width + px;  // the padded rows can pass 2^32 - 1
+        if (px >= width || j >= n) return;
+        i = (uint32_t)j;
+    }
     if (i >= n) return;
     const RayF r = load_ray(rays, i);
     HitF h;
@@ -273,7 +285,7 @@ hipError_t launchPackImage(const float4* src, uint32_t npix, uint32_t format, vo
 }
 
 hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags,
-                              hipStream_t s)
+                              hipStream_t s, uint32_t width)
 {
     if (n == 0) return hipSuccess;
     if (bvh.tlas) {
@@ -286,12 +298,15 @@ hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n
     } else if ((flags & STK_TRACE_BVH2) || bvh.wideRoot == STK_INVALID_INDEX) {
         hipLaunchKernelGGL(k_trace_closest_packed, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh, rays, n, hits);
     } else {
+        // with an image width, one wave per 8x8 tile (the last tile row and column padded)
+        const uint32_t grid =
+            width ? ((width + 7) / 8) * (uint32_t)(((n + width - 1) / width + 7) / 8) : walkBlocks(n);
         if (closestWalkStack(bvh) == (uint32_t)kWideStackShallow)
-            hipLaunchKernelGGL(k_trace_closest<kWideStackShallow>, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh,
-                               rays, n, hits);
+            hipLaunchKernelGGL(k_trace_closest<kWideStackShallow>, dim3(grid), dim3(kWalkBlock), 0, s, bvh, rays, n,
+                               hits, width);
         else
-            hipLaunchKernelGGL(k_trace_closest<kWideStack>, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh, rays, n,
-                               hits);
+            hipLaunchKernelGGL(k_trace_closest<kWideStack>, dim3(grid), dim3(kWalkBlock), 0, s, bvh, rays, n, hits,
+                               width);
     }
     return hipGetLastError();
 }

@@ -164,7 +164,7 @@ hipError_t launchWorldTriangles(const StkInstance* inst, const float4* blasTris,
 // flags: STK_TRACE_REFERENCE_ORDER (skip-link walk), STK_TRACE_BVH2 (packed
 // binary walk instead of the wide one; closest hit only).
 hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags,
-                              hipStream_t s);
+                              hipStream_t s, uint32_t width = 0);
 hipError_t launchTraceAny(const BvhView& bvh, const StkRay* rays, uint32_t n, StkHit* hits, uint32_t flags,
                           hipStream_t s);
 hipError_t launchCameraRays(const StkPathTracerParam& p, uint32_t sampleNum, StkRay* rays, hipStream_t s);

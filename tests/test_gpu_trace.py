@@ -94,6 +94,52 @@ def test_config2_subsample_bit_exact_and_full_frame_orders_agree(ctx):
     assert np.mean(full["instId"] != INVALID) > 0.9999
 
 
+def trace_image(ctx, rays, width, height):
+    n = width * height
+    d_rays = ctx.alloc(max(1, n) * 32)
+    d_hits = ctx.alloc(max(1, n) * 20)
+    try:
+        if n:
+            ctx.upload(d_rays, rays[:n].copy())
+        ctx.trace_closest_image_device(d_rays, width, height, d_hits)
+        ctx.sync()
+        return ctx.download(d_hits, np.zeros(n, dtype=sa.HIT))
+    finally:
+        ctx.free(d_rays)
+        ctx.free(d_hits)
+
+
+def test_image_rays_traced_in_tiles_give_the_same_hits(ctx):
+    """stk_trace_closest_image_device walks an image's rays one 8x8 tile per
+    wave; every hit must equal the oracle's and the linear entry point's, at
+    its own index, also where the image is not a whole number of tiles."""
+    print("case: image entry point, config 1 256x256", flush=True)
+    s, o = setup(ctx, 1)
+    rays = s.jittered_rays(0x5354524B00000001)
+    ref, _ = o.trace(rays, MODE_REF)
+    assert same(trace_image(ctx, rays, 256, 256), ref)
+    print("case: image entry point, config 1 ragged sizes", flush=True)
+    for w, h in ((13, 7), (1, 1), (1000, 65), (7, 9000), (0, 5), (5, 0)):
+        n = w * h
+        assert same(trace_image(ctx, rays, w, h), ref[:n].copy()), f"{w}x{h}"
+    print("case: image entry point, config 2 1920x1080", flush=True)
+    s, o = setup(ctx, 2)
+    rays = s.jittered_rays(0x5354524B00000002)
+    tiled = trace_image(ctx, rays, 1920, 1080)
+    assert same(tiled, ctx.trace_closest(rays))
+    sub = np.arange(0, rays.size, 41)
+    ref, _ = o.trace(rays[sub].copy(), MODE_PRUNED)
+    assert same(tiled[sub].copy(), ref)
+    d = ctx.alloc(32)
+    try:
+        with pytest.raises(sa.StkError):
+            ctx.trace_closest_image_device(d, 65536, 65536, d)
+        with pytest.raises(sa.StkError):
+            ctx.trace_closest_image_device(0, 4, 4, d)
+    finally:
+        ctx.free(d)
+
+
 @pytest.mark.parametrize("sampling", [0, 1, 2, 3])
 def test_device_entry_points_and_camera_rays(ctx, sampling):
     """Every stratifiedSamplingType (0 centre, the reference's default in
