@@ -289,6 +289,15 @@ def instanced_bench(args) -> dict:
         ctx.record(7)
         ctx.sync()
         trace_ms = ctx.elapsed_ms(6, 7) / 5
+        # the same rays, one wave per 8x8 tile of the frame
+        assert cam_rays.size == w * h
+        ctx.trace_closest_image_device(d_rays, w, h, d_hits)
+        ctx.record(6)
+        for _ in range(5):
+            ctx.trace_closest_image_device(d_rays, w, h, d_hits)
+        ctx.record(7)
+        ctx.sync()
+        tiles_ms = ctx.elapsed_ms(6, 7) / 5
         p1 = sa.render_params(scene.camera, spp=4, max_depth=6, frame=0, sampling=1, sky=scene.sky)
         tuned = choose_launch(ctx, args.launch, lambda: ctx.render_device(p1, d_out), w * h)
         p = sa.render_params(scene.camera, spp=spp, max_depth=6, frame=1, sampling=1, sky=scene.sky)
@@ -307,7 +316,8 @@ def instanced_bench(args) -> dict:
                 "mrays_per_s": round((rays[0] + rays[1]) / (ms * 1e-3) / 1e6, 1),
                 "rays_per_frame": {"closest": rays[0], "shadow": rays[1]},
                 "primary_closest_mrays_per_s": round(cam_rays.size / (trace_ms * 1e-3) / 1e6, 1),
-                "primary_kernel_ms": round(trace_ms, 4), "launch": with_graph_state(tuned["best"]),
+                "primary_kernel_ms": round(trace_ms, 4),
+                "primary_tiles_mrays_per_s": round(cam_rays.size / (tiles_ms * 1e-3) / 1e6, 1), "launch": with_graph_state(tuned["best"]),
                 "autotune_4spp_ms": {"default": tuned.get("default_ms"), "best": tuned.get("ms")}}
 
     want = set(args.structures.split(","))

@@ -218,7 +218,9 @@ STK_API stk_status stk_trace_closest_device(stk_ctx* ctx, const StkRay* dRays, u
  * thread, shaders/pathtracerMdl.hlsl:406-423): the same closest hits for an
  * image's rays, width x height in row-major order, traced with one wave per
  * 8x8 tile of pixels, so that a wave's rays stay close in the scene; hits are
- * written at the rays' own indices, identical to stk_trace_closest_device's. */
+ * written at the rays' own indices, identical to stk_trace_closest_device's.
+ * flags as there: the wide, binary (STK_TRACE_BVH2) and two-level walks take
+ * the tiles; the reference-order walk takes the rays in order. */
 STK_API stk_status stk_trace_closest_image_device(stk_ctx* ctx, const StkRay* rays, uint32_t width, uint32_t height,
                                                   StkHit* hits, uint32_t flags);
 STK_API stk_status stk_trace_any_device(stk_ctx* ctx, const StkRay* dRays, uint32_t n, StkHit* dHits,

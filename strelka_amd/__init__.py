@@ -582,11 +582,12 @@ class Context:
         self._check(_LIB.stk_trace_closest_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits), flags),
                     "stk_trace_closest_device")
 
-    def trace_closest_image_device(self, d_rays: int, width: int, height: int, d_hits: int):
+    def trace_closest_image_device(self, d_rays: int, width: int, height: int, d_hits: int, binary: bool = False):
         """stk_trace_closest_image_device: an image's rays (row-major) traced
         in 8x8 tiles; the hits of trace_closest_device."""
         self._check(_LIB.stk_trace_closest_image_device(self._h, C.c_void_p(d_rays), width, height,
-                                                        C.c_void_p(d_hits), 0), "stk_trace_closest_image_device")
+                                                        C.c_void_p(d_hits), 2 if binary else 0),
+                    "stk_trace_closest_image_device")
 
     def trace_any_device(self, d_rays: int, n: int, d_hits: int, reference_order: bool = False):
         self._check(_LIB.stk_trace_any_device(self._h, C.c_void_p(d_rays), n, C.c_void_p(d_hits),

@@ -40,6 +40,26 @@ __device__ __forceinline__ RayF load_ray(const StkRay* rays, uint32_t i)
 
 static_assert(2 * kWideStack <= kStackDepth, "the wide walk's two stacks share the packed walk's LDS");
 
+// The ray a lane of a one-wave block takes.  width == 0: the block's 64
+// consecutive rays.  width > 0: the rays are an image's, width per row in
+// row-major order, and block b takes the 8x8 tile b of it (camera rays of a
+// tile stay close in the scene; 64 rays of one row span 64 pixels); lanes of
+// the padding tiles past the right and bottom edges get no ray.  Hits go to
+// the rays' own indices either way.
+__device__ __forceinline__ bool wave_ray(uint32_t width, uint32_t n, uint32_t& i)
+{
+    static_assert(kWalkBlock == 64, "one 8x8 tile per wave");
+    i = xcd_block(blockIdx.x, gridDim.x) * kWalkBlock + threadIdx.x;
+    if (width) {
+        const uint32_t tilesX = (width + 7) / 8, tile = i / 64, within = i % 64;
+        const uint32_t px = (tile % tilesX) * 8 + within % 8, py = (tile / tilesX) * 8 + within / 8;
+        const uint64_t j = (uint64_t)py * width + px;  // the padded rows can pass 2^32 - 1
+        if (px >= width || j >= n) return false;
+        i = (uint32_t)j;
+    }
+    return i < n;
+}
+
 // Closest hit over the compressed 8-wide tree (two LDS node-group stacks per
 // lane).  The binary walk has a kernel of its own (56 registers instead of
 // the wide walk's 72).  The 11-entry form is held to 72 registers, so 7
@@ -49,25 +69,13 @@ static_assert(2 * kWideStack <= kStackDepth, "the wide walk's two stacks share t
 // a node with inner children pushes, so a walk over a tree of D levels holds
 // at most D - 1 entries: a tree of at most S + 1 levels fits.  Fewer entries
 // mean less LDS per wave and more waves per CU.
-// width > 0: the rays are an image's, width per row in row-major order, and
-// each wave takes an 8x8 tile of it (camera rays of a tile stay close in the
-// scene; a wave of one row's 64 would span 64 pixels); hits go to the rays'
-// own indices either way.
 template <int S>
 __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(S <= 11 ? 7 : 5, 8))) void k_trace_closest(
     BvhView bvh, const StkRay* __restrict__ rays, uint32_t n, StkHit* __restrict__ hits, uint32_t width)
 {
-    static_assert(kWalkBlock == 64, "one 8x8 tile per wave");
     __shared__ uint32_t stack[2 * S * kWalkBlock];
-    uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kWalkBlock + threadIdx.x;
-    if (width) {
-        const uint32_t tilesX = (width + 7) / 8, tile = i / 64, within = i % 64;
-        const uint32_t px = (tile % tilesX) * 8 + within % 8, py = (tile / tilesX) * 8 + within / 8;
-        const uint64_t j = (uint64_t)py * width + px;  // the padded rows can pass 2^32 - 1
-        if (px >= width || j >= n) return;
-        i = (uint32_t)j;
-    }
-    if (i >= n) return;
+    uint32_t i;
+    if (!wave_ray(width, n, i)) return;
     const RayF r = load_ray(rays, i);
     HitF h;
     closest_wide(bvh, bvh.wideRoot, r, stack + threadIdx.x, stack + S * kWalkBlock + threadIdx.x, kWalkBlock, h);
@@ -76,11 +84,11 @@ __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(S <=
 
 // Closest hit over the packed binary tree.
 __global__ __launch_bounds__(kWalkBlock) void k_trace_closest_packed(BvhView bvh, const StkRay* __restrict__ rays,
-                                                                 uint32_t n, StkHit* __restrict__ hits)
+                                                                 uint32_t n, StkHit* __restrict__ hits, uint32_t width)
 {
     __shared__ uint32_t stack[kStackDepth * kWalkBlock];
-    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kWalkBlock + threadIdx.x;
-    if (i >= n) return;
+    uint32_t i;
+    if (!wave_ray(width, n, i)) return;
     const RayF r = load_ray(rays, i);
     HitF h;
     closest_packed(bvh, bvh.hdr, r, stack + threadIdx.x, kWalkBlock, h);
@@ -90,11 +98,11 @@ __global__ __launch_bounds__(kWalkBlock) void k_trace_closest_packed(BvhView bvh
 // Closest hit and any hit over a two-level structure.
 // five waves per SIMD: the two-level walk fits 96 registers without spills
 __global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_trace_closest_tl(BvhView bvh, const StkRay* __restrict__ rays, uint32_t n,
-                                                             StkHit* __restrict__ hits)
+                                                             StkHit* __restrict__ hits, uint32_t width)
 {
     __shared__ uint32_t stack[2 * kWideStack * kWalkBlock];
-    const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kWalkBlock + threadIdx.x;
-    if (i >= n) return;
+    uint32_t i;
+    if (!wave_ray(width, n, i)) return;
     const RayF r = load_ray(rays, i);
     HitF h;
     uint32_t ik;
@@ -288,19 +296,18 @@ hipError_t launchTraceClosest(const BvhView& bvh, const StkRay* rays, uint32_t n
                               hipStream_t s, uint32_t width)
 {
     if (n == 0) return hipSuccess;
+    // with an image width, one wave per 8x8 tile (the last tile row and column padded)
+    const uint32_t grid = width ? ((width + 7) / 8) * (uint32_t)(((n + width - 1) / width + 7) / 8) : walkBlocks(n);
     if (bvh.tlas) {
         if (flags & (STK_TRACE_REFERENCE_ORDER | STK_TRACE_BVH2)) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_trace_closest_tl, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh, rays, n, hits);
+        hipLaunchKernelGGL(k_trace_closest_tl, dim3(grid), dim3(kWalkBlock), 0, s, bvh, rays, n, hits, width);
     } else if (!bvh.packedOk && (flags & STK_TRACE_BVH2)) {
         return hipErrorInvalidValue; // deeper than the packed walk's stack
     } else if ((flags & STK_TRACE_REFERENCE_ORDER) || (!bvh.packedOk && bvh.wideRoot == STK_INVALID_INDEX)) {
         hipLaunchKernelGGL(k_trace_closest_ref, dim3(blocks(n)), dim3(kBlock), 0, s, bvh, rays, n, hits);
     } else if ((flags & STK_TRACE_BVH2) || bvh.wideRoot == STK_INVALID_INDEX) {
-        hipLaunchKernelGGL(k_trace_closest_packed, dim3(walkBlocks(n)), dim3(kWalkBlock), 0, s, bvh, rays, n, hits);
+        hipLaunchKernelGGL(k_trace_closest_packed, dim3(grid), dim3(kWalkBlock), 0, s, bvh, rays, n, hits, width);
     } else {
-        // with an image width, one wave per 8x8 tile (the last tile row and column padded)
-        const uint32_t grid =
-            width ? ((width + 7) / 8) * (uint32_t)(((n + width - 1) / width + 7) / 8) : walkBlocks(n);
         if (closestWalkStack(bvh) == (uint32_t)kWideStackShallow)
             hipLaunchKernelGGL(k_trace_closest<kWideStackShallow>, dim3(grid), dim3(kWalkBlock), 0, s, bvh, rays, n,
                                hits, width);

@@ -94,14 +94,14 @@ def test_config2_subsample_bit_exact_and_full_frame_orders_agree(ctx):
     assert np.mean(full["instId"] != INVALID) > 0.9999
 
 
-def trace_image(ctx, rays, width, height):
+def trace_image(ctx, rays, width, height, binary=False):
     n = width * height
     d_rays = ctx.alloc(max(1, n) * 32)
     d_hits = ctx.alloc(max(1, n) * 20)
     try:
         if n:
             ctx.upload(d_rays, rays[:n].copy())
-        ctx.trace_closest_image_device(d_rays, width, height, d_hits)
+        ctx.trace_closest_image_device(d_rays, width, height, d_hits, binary=binary)
         ctx.sync()
         return ctx.download(d_hits, np.zeros(n, dtype=sa.HIT))
     finally:
@@ -118,6 +118,8 @@ def test_image_rays_traced_in_tiles_give_the_same_hits(ctx):
     rays = s.jittered_rays(0x5354524B00000001)
     ref, _ = o.trace(rays, MODE_REF)
     assert same(trace_image(ctx, rays, 256, 256), ref)
+    assert same(trace_image(ctx, rays, 256, 256, binary=True), ref)  # the packed binary walk
+    assert same(trace_image(ctx, rays, 250, 262, binary=True), ref[:250 * 262].copy())
     print("case: image entry point, config 1 ragged sizes", flush=True)
     for w, h in ((13, 7), (1, 1), (1000, 65), (7, 9000), (0, 5), (5, 0)):
         n = w * h

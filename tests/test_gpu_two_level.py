@@ -55,7 +55,20 @@ def test_closest_and_any_hits(ctx, small):
     assert same(gpu, two)
     # camera rays of the scene's own view
     cam = s.jittered_rays(7)
-    assert same(ctx.trace_closest(cam), o.trace(cam, MODE_REF)[0])
+    cam_ref = o.trace(cam, MODE_REF)[0]
+    assert same(ctx.trace_closest(cam), cam_ref)
+    # the same rays walked one 8x8 tile of the image per wave
+    w, h = s.camera.dimension[0], s.camera.dimension[1]
+    for ww, hh in ((w, h), (w - 3, h - 5)):
+        d_rays, d_hits = ctx.alloc(ww * hh * 32), ctx.alloc(ww * hh * 20)
+        try:
+            ctx.upload(d_rays, cam[:ww * hh].copy())
+            ctx.trace_closest_image_device(d_rays, ww, hh, d_hits)
+            ctx.sync()
+            assert same(ctx.download(d_hits, np.zeros(ww * hh, dtype=sa.HIT)), cam_ref[:ww * hh].copy())
+        finally:
+            ctx.free(d_rays)
+            ctx.free(d_hits)
     # any hit (shadow-length rays): the first hit triangle in expanded preorder
     short = scene_rays(s, lo, hi, 50000, 22, tmax=30.0)
     g = ctx.trace_any(short)
